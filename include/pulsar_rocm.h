@@ -1,0 +1,155 @@
+/*
+ * pulsar_rocm.h — C-ABI of libpulsar_rocm.so, the MI355X (gfx950) GRAPE inner loop.
+ *
+ * One evaluation computes, for every problem of a plan, the ensemble-weighted
+ * fidelity F and its gradient G[n_ctrl x n_t] with respect to the
+ * piecewise-constant controls:
+ *
+ *   X_k  = c * dt_k * (A0_m + sum_j s_mj u_jk A_j)    c = -i (closed), 1 (Lindblad)
+ *   U_k  = exp(X_k)                                   one per member m and slice k
+ *   P_k  = U_{k-1} ... U_1,  R_k = U_n ... U_{k+1}   forward / backward chains
+ *   F    = sum_m w_m F_m
+ *   G_jk = sum_m w_m dF_m/du_jk
+ *
+ * The gradient follows the reference's first-order GRAPE formula
+ * (grad_mode 0) or the exact Frechet derivative of exp (grad_mode 1).
+ *
+ * Replaces, on the :rocm device, the reference's
+ *   compute_fidelity(sys, ctrl, tgt)          src/Physics/Objectives.jl:632-639
+ *   compute_grape_gradient(sys, ctrl, tgt)    src/Physics/Gradients.jl:80-127
+ *   _gate_gradient_into! / _state_gradient_into!  src/Physics/Gradients.jl:254-291, 357-399
+ *   ensemble_value_and_grad batched_eval      src/Optimization/Ensemble/EnsembleObjective.jl:272-307
+ *   grape_state_kernel (:cpu/:cuda branches)  src/Application/MR/GRAPEState.jl:73-85, 136-250
+ *   grape_lindblad_kernel                     src/Application/MR/GRAPELindblad.jl:83-92, 96-215
+ *
+ * Conventions (identical to Julia's in-memory layout, so a ccall passes
+ * arrays without copies):
+ *   - complex matrices are ComplexF64, column-major, interleaved (re, im):
+ *     element (r, c) of a d x d matrix is at doubles [2*(r + c*d)], [2*(r + c*d)+1];
+ *   - controls and gradients are Float64 [n_ctrl x n_t] column-major:
+ *     u_jk at [j + k*n_ctrl]  (the reference's ControlSequence.controls view,
+ *     src/Types/ControlSequence.jl:99-110);
+ *   - all entry points return PR_OK (0) or a negative PR_ERR_* code; the
+ *     message of the last failure on the calling thread is read with
+ *     pr_last_error.
+ *
+ * The library never keeps a caller's host pointer beyond the call that
+ * receives it. A plan owns its device memory; a plan is not re-entrant (one
+ * caller at a time per plan).
+ */
+#ifndef PULSAR_ROCM_H
+#define PULSAR_ROCM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PR_ABI_VERSION 1
+
+/* status codes */
+#define PR_OK 0
+#define PR_ERR_ARG (-1)         /* invalid argument / shape / non-Hermitian input */
+#define PR_ERR_DEVICE (-2)      /* no usable gfx950 device, or a HIP runtime error */
+#define PR_ERR_NOMEM (-3)       /* device allocation failed */
+#define PR_ERR_LAUNCH (-4)      /* kernel launch failed */
+#define PR_ERR_UNSUPPORTED (-5) /* valid input outside what this build supports */
+#define PR_ERR_INJECTED (-6)    /* failure forced by PULSAR_ROCM_FAULT (tests only) */
+
+/* dynamics */
+#define PR_CLOSED 0   /* Hilbert space, generator -i*H (H Hermitian) */
+#define PR_LINDBLAD 1 /* Liouville space, generator L (any complex matrix, dim = N^2) */
+
+/* objectives */
+#define PR_GATE 0         /* F_m = |Tr(U_t^dag U_total)/dim|^2 (Objectives.jl:239-246) */
+#define PR_PAIRS_SQUARE 1 /* F_m = sum_s |<t_s|U_total|i_s>|^2 (Objectives.jl:221) */
+#define PR_PAIRS_REAL 2   /* F_m = sum_s Re <t_s|U_total|i_s>  (Objectives.jl:220) */
+
+/* precision of the device path */
+#define PR_FP64 0  /* every product in FP64 (FP64 MFMA) */
+#define PR_MIXED 1 /* exp polynomial beyond first order in split-BF16 MFMA, FP64 chains */
+
+/* gradient formula */
+#define PR_GRAD_REFERENCE 0 /* first-order GRAPE formula of Gradients.jl:274-289 */
+#define PR_GRAD_EXACT 1     /* exact derivative (Frechet derivative of exp) */
+
+/* One GRAPE problem: a system (drift + controls), an ensemble of members
+ * and a target. Mirrors QuantumSystem + ControlSequence + QuantumTarget
+ * (src/Types/QuantumSystem.jl:23-130, ControlSequence.jl:37-59,
+ * Targets.jl:37-144) and the ensemble closures of
+ * build_ensemble_from_systems (src/Optimization/Ensemble/SystemBuilder.jl:49-87). */
+typedef struct {
+  int dim;                  /* d (closed) or N^2 (Lindblad); 1 <= dim <= 64 */
+  int n_ctrl;               /* number of control channels, >= 1 */
+  int n_t;                  /* number of time slices, >= 1 */
+  double dt;                /* uniform slice length (used when dt_slices == NULL) */
+  const double* dt_slices;  /* [n_t] per-slice lengths (MR pulse_dt) or NULL */
+  int dynamics;             /* PR_CLOSED or PR_LINDBLAD */
+  int objective;            /* PR_GATE, PR_PAIRS_SQUARE or PR_PAIRS_REAL */
+  int n_members;            /* ensemble members, >= 1 */
+  const double* H0;         /* drift A0: [dim*dim] complex, or [n_members][dim*dim] */
+  int H0_per_member;        /* 1: H0 holds one drift per member */
+  const double* Hc;         /* controls A_j: [n_ctrl][dim*dim] complex */
+  const double* ctrl_scale; /* [n_members][n_ctrl] scale s_mj, or NULL (all 1) */
+  const double* weight;     /* [n_members] w_m, or NULL: 1/n_members (gate),
+                               1/(n_members*n_pairs) (pairs) */
+  const double* U_target;   /* PR_GATE: [dim*dim] complex */
+  int n_pairs;              /* pair objectives: number of (init, target) pairs */
+  const double* psi_init;   /* [n_pairs][dim] complex */
+  const double* psi_targ;   /* [n_pairs][dim] complex */
+} pr_problem;
+
+typedef struct {
+  int precision;   /* PR_FP64 or PR_MIXED */
+  int grad_mode;   /* PR_GRAD_REFERENCE or PR_GRAD_EXACT */
+  int segment;     /* slices per time segment of the parallel chain scan; 0 = auto */
+  int store_props; /* 1: keep U_k in HBM between passes, 0: recompute, -1: auto */
+} pr_options;
+
+/* Per-problem diagnostics filled by pr_eval when diag != NULL:
+ * diag[4*p + 0] = max over members of ||U_total^dag U_total - I||_max (closed)
+ *                 or 0 (Lindblad: the channel is not unitary)
+ * diag[4*p + 1] = largest scaling exponent used by exp over the problem
+ * diag[4*p + 2] = Taylor degree used by exp
+ * diag[4*p + 3] = number of time segments of the chain scan */
+#define PR_DIAG_PER_PROBLEM 4
+
+int pr_abi_version(void);
+/* Number of visible devices; 0 when none. Never fails. */
+int pr_device_count(void);
+/* Select and initialise a device (gfx950 required). */
+int pr_init(int device);
+/* Copies the last error message of the calling thread (NUL-terminated). */
+int pr_last_error(char* buf, size_t n);
+
+/* Build a plan for n_probs problems. Validates every problem (Hermiticity of
+ * H0/Hc to 1e-10 for closed dynamics as QuantumSystem does,
+ * src/Types/QuantumSystem.jl:60-66; unitarity of U_target to 1e-8 as
+ * unitary_target does, src/Types/Targets.jl:138-142), copies the operators to
+ * the device and allocates all scratch. The host arrays may be freed after. */
+int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, void** plan);
+
+/* Host-in / host-out evaluation. controls[p] is [n_ctrl x n_t] of problem p;
+ * F[p] receives its fidelity; G[p] (may be NULL to skip the gradient for all
+ * problems) receives [n_ctrl x n_t]; diag may be NULL. */
+int pr_eval(void* plan, const double* const* controls, double* F, double* const* G, double* diag);
+
+/* Device-resident evaluation on `stream` (a hipStream_t, or NULL for the
+ * default stream): d_controls holds every problem's controls back to back in
+ * problem order, d_F receives n_probs fidelities and d_G (or NULL) the
+ * gradients back to back in the same order. Asynchronous: returns after
+ * enqueueing; all pointers are device pointers. */
+int pr_eval_device(void* plan, const double* d_controls, double* d_F, double* d_G, void* stream);
+
+/* Total number of control values of the plan (sum of n_ctrl*n_t), i.e. the
+ * length of d_controls / d_G for pr_eval_device. */
+long long pr_plan_n_params(void* plan);
+
+int pr_plan_destroy(void* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PULSAR_ROCM_H */

@@ -1,0 +1,445 @@
+// grape_kernels.hpp — device kernels of the GRAPE inner loop (FP64 MFMA).
+//
+// Per problem, members m and slices k are cut into time segments s of L
+// slices. One evaluation runs four kernels:
+//
+//   k_segment  (member, segment)  X_k = c dt_k (A0 + sum_j s_mj u_jk A_j),
+//                                 U_k = exp(X_k) by a degree-9 Taylor polynomial
+//                                 (Paterson-Stockmeyer, 4 products) with
+//                                 scaling and squaring chosen per slice from an
+//                                 FP64 1-norm bound; stores U_k^T and the segment
+//                                 product T_s = U_{k1-1} ... U_{k0}.
+//   k_combine  (member)           prefix products E_s = T_{s-1} ... T_0, the
+//                                 overlap(s) with the target, F_m, and the
+//                                 adjoint seed: Omega = Psi0 diag(conj w) PsiT^dag U
+//                                 (unitary) or backward boundaries (Lindblad).
+//   k_gradient (member, segment)  walks the segment again and contracts
+//                                 M_k = P_k Lambda_k^dag with every control:
+//                                 G_m[j,k] = Re(c dt_k s_mj Tr(A_j M_k)).
+//                                 Unitary dynamics use M_k = C_k U_k^dag with
+//                                 C_{k+1} = U_k M_k, C = E_s Omega E_s^dag
+//                                 (no stored forward chain); Lindblad dynamics
+//                                 store P_k per segment and walk Lambda back.
+//   k_reduce_*                    ensemble-weighted sums F = sum_m w_m F_m,
+//                                 G = sum_m w_m G_m (member order, deterministic).
+//
+// The reference computes the same quantities one member and one slice at a
+// time on the CPU: build_total_hamiltonian!/compute_propagators!
+// (src/Computation/Propagators.jl:166-184, 520-544), the forward/backward
+// chains (:309-325, :390-406), _gate_gradient_into! / _state_gradient_into!
+// (src/Physics/Gradients.jl:254-291, 357-399), the MR kernels
+// (src/Application/MR/GRAPEState.jl:186-231, GRAPELindblad.jl:146-196) and
+// the ensemble mean (src/Optimization/Ensemble/EnsembleObjective.jl:169-217).
+#pragma once
+#include "tiles.hpp"
+
+namespace pr {
+
+struct ProbDev {
+  int d, B, nc, nt, M, dyn, obj, npairs, nseg, seglen, h0pm, unitary, grad_exact;
+  double cre, cim;            // generator factor c: (0,-1) closed, (1,0) Lindblad
+  const double2* H0;          // [M or 1][NB][256] tile images of the drift
+  const double2* Hc;          // [nc][NB][256] controls A_j
+  const double2* HcT;         // [nc][NB][256] A_j^T (trace operand)
+  const double* n0;           // [M or 1] ||A0||_1
+  const double* ncn;          // [nc] ||A_j||_1
+  const double* scale;        // [M][nc]
+  const double* weight;       // [M]
+  const double* dts;          // [nt]
+  const double2* Ut;          // gate target image
+  const double2* Psi0;        // pairs: columns = initial states
+  const double2* PsiT;        // pairs: columns = target states
+  long long coff;             // offset into the controls / gradient vectors
+  double2* Ust;               // [M][nt][NB][256] U_k^T images
+  double2* Tseg;              // [M][nseg][NB][256]
+  double2* Eseg;              // [M][nseg+1][NB][256]
+  double2* Om;                // [M][NB][256]
+  double2* Bb;                // [M][nseg][NB][256] backward boundaries (Lindblad)
+  double2* Pscr;              // [M*nseg][seglen][NB][256] forward states (Lindblad)
+  double* Fm;                 // [M]
+  double* Gm;                 // [M][nc*nt]
+  double* udef;               // [M] ||U^dag U - I||_max (closed)
+  int* smax;                  // [M*nseg] largest scaling exponent per segment
+};
+
+struct WorkItem {
+  int prob, member, seg, k0, k1;
+};
+
+// exp truncation: degree-9 Taylor, ||X|| <= THETA9 keeps the remainder below
+// 2^-53 (||X||^10/10! <= 1.1e-16 for ||X|| <= 0.1147)
+constexpr double THETA9 = 0.11;
+constexpr int TAYLOR_DEG = 9;
+
+template <int B>
+struct Smem {
+  static constexpr int NB = B * B;
+  static constexpr int TILE = NB * 272;  // complex elements per LDS matrix tile
+  static constexpr size_t bytes = 2 * TILE * sizeof(double2) + NB * 64 * sizeof(double) + 128 * sizeof(double2);
+};
+
+template <int B>
+__device__ inline void smem_parts(double2*& LA, double2*& LB, double*& red, double2*& zb) {
+  extern __shared__ double2 smem_raw[];
+  LA = smem_raw;
+  LB = LA + Smem<B>::TILE;
+  red = reinterpret_cast<double*>(LB + Smem<B>::TILE);
+  zb = reinterpret_cast<double2*>(red + Smem<B>::NB * 64);
+}
+
+// workgroup sum of one complex value (every thread contributes its partial)
+template <int B>
+__device__ inline double2 block_sum2(double2 v, double* red, const Ln& x) {
+  double a[2] = {v.x, v.y};
+  double s = wave_reduce_spread<2>(a, x.lane);  // lane l: value index l >> 5
+  if ((x.lane & 31) == 0) red[x.w * 2 + (x.lane >> 5)] = s;
+  __syncthreads();
+  double2 out = make_double2(0, 0);
+  for (int w = 0; w < B * B; w++) {
+    out.x += red[2 * w];
+    out.y += red[2 * w + 1];
+  }
+  __syncthreads();
+  return out;
+}
+
+template <int B>
+__device__ inline double block_max(double v, double* red, const Ln& x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m));
+  if (x.lane == 0) red[x.w] = v;
+  __syncthreads();
+  double out = 0;
+  for (int w = 0; w < B * B; w++) out = fmax(out, red[w]);
+  __syncthreads();
+  return out;
+}
+
+// Generator of slice k, already divided by 2^s; returns s.
+template <int B>
+__device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __restrict__ u, int& s_out,
+                               const Ln& x) {
+  const int nc = P.nc;
+  const double* sc = P.scale + (size_t)m * nc;
+  const double* uk = u + P.coff + (size_t)k * nc;
+  const double dt = P.dts[k];
+  double nb = P.n0[P.h0pm ? m : 0];
+  Blk H = load_blk(P.H0 + (P.h0pm ? (size_t)m * B * B * 256 : 0), x);
+  for (int j = 0; j < nc; j++) {
+    const double coef = sc[j] * uk[j];
+    nb += fabs(coef) * P.ncn[j];
+    Blk A = load_blk(P.Hc + (size_t)j * B * B * 256, x);
+    H.re += coef * A.re;
+    H.im += coef * A.im;
+  }
+  nb *= fabs(dt);
+  int s = 0;
+  if (nb > THETA9) s = (int)ceil(log2(nb / THETA9));
+  s_out = s;
+  const double f = ldexp(dt, -s);
+  return cscale(P.cre * f, P.cim * f, H);
+}
+
+__device__ inline void add_diag(Blk& X, double v, const Ln& x) {
+  if (x.I == x.Jb) {
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (rowof(x.q, r) == x.c) X.re[r] += v;
+  }
+}
+
+// exp(X * 2^s) for X already scaled: degree-9 Taylor in Paterson-Stockmeyer form
+// p = B0 + X3 (B1 + X3 (B2 + c9 X3)), then s squarings.
+template <int B>
+__device__ inline Blk expm_blk(const Blk& X, int s, double2* LA, double2* LB, const Ln& x) {
+  const double c2 = 1.0 / 2, c3 = 1.0 / 6, c4 = 1.0 / 24, c5 = 1.0 / 120, c6 = 1.0 / 720,
+               c7 = 1.0 / 5040, c8 = 1.0 / 40320, c9 = 1.0 / 362880;
+  Blk X2 = mm<B, N, N>(X, X, LA, LB, x);
+  Blk X3 = mm<B, N, N>(X, X2, LA, LB, x);
+  Blk Tm;
+  Tm.re = c9 * X3.re + c8 * X2.re + c7 * X.re;
+  Tm.im = c9 * X3.im + c8 * X2.im + c7 * X.im;
+  add_diag(Tm, c6, x);
+  Blk R = mm<B, N, N>(X3, Tm, LA, LB, x);
+  R.re += c5 * X2.re + c4 * X.re;
+  R.im += c5 * X2.im + c4 * X.im;
+  add_diag(R, c3, x);
+  Blk U = mm<B, N, N>(X3, R, LA, LB, x);
+  U.re += c2 * X2.re + X.re;
+  U.im += c2 * X2.im + X.im;
+  add_diag(U, 1.0, x);
+  for (int i = 0; i < s; i++) U = mm<B, N, N>(U, U, LA, LB, x);
+  return U;
+}
+
+// ---------------------------------------------------------------- k_segment
+template <int B>
+__global__ __launch_bounds__(64 * B * B) void k_segment(const ProbDev* __restrict__ probs,
+                                                         const WorkItem* __restrict__ items,
+                                                         const double* __restrict__ u) {
+  const WorkItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const Ln x = lane_info<B>();
+  double2 *LA, *LB, *zb;
+  double* red;
+  smem_parts<B>(LA, LB, red, zb);
+  constexpr int NB = B * B;
+  const int m = it.member;
+  Blk Tseg = eye_blk(x);
+  int smax = 0;
+  for (int k = it.k0; k < it.k1; k++) {
+    int s;
+    Blk X = assemble<B>(P, m, k, u, s, x);
+    smax = max(smax, s);
+    Blk U = expm_blk<B>(X, s, LA, LB, x);
+    // chain T <- U T, and U^T image from the same LDS tile
+    put<B>(LA, U, x);
+    if constexpr (B > 1) put<B>(LB, Tseg, x);
+    __syncthreads();
+    Blk acc = zero_blk();
+    gemm_acc<B, N, N>(acc, LA, U, LB, Tseg, x);
+    Blk Ut;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      double2 v = LA[lidx(x.Jb * B + x.I, x.c, rowof(x.q, r))];  // U[16J + c][16I + q + 4r]
+      Ut.re[r] = v.x;
+      Ut.im[r] = v.y;
+    }
+    __syncthreads();
+    Tseg = acc;
+    store_blk(P.Ust + ((size_t)m * P.nt + k) * NB * 256, Ut, x);
+  }
+  store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * NB * 256, Tseg, x);
+  if (threadIdx.x == 0) P.smax[(size_t)m * P.nseg + it.seg] = smax;
+}
+
+// ---------------------------------------------------------------- k_combine
+template <int B>
+__global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restrict__ probs,
+                                                         const int2* __restrict__ members) {
+  const int2 pm = members[blockIdx.x];
+  const ProbDev& P = probs[pm.x];
+  const int m = pm.y;
+  const Ln x = lane_info<B>();
+  double2 *LA, *LB, *zb;
+  double* red;
+  smem_parts<B>(LA, LB, red, zb);
+  constexpr int NB = B * B;
+  const size_t MB = (size_t)NB * 256;
+  Blk E = eye_blk(x);
+  store_blk(P.Eseg + ((size_t)m * (P.nseg + 1)) * MB, E, x);
+  for (int s = 0; s < P.nseg; s++) {
+    Blk Ts = load_blk(P.Tseg + ((size_t)m * P.nseg + s) * MB, x);
+    E = mm<B, N, N>(Ts, E, LA, LB, x);
+    store_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + s + 1) * MB, E, x);
+  }
+  const Blk& Ut = E;  // U_total
+  if (P.unitary) {
+    Blk D = mm<B, C, N>(Ut, Ut, LA, LB, x);
+    add_diag(D, -1.0, x);
+    double v = 0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) v = fmax(v, fmax(fabs(D.re[r]), fabs(D.im[r])));
+    v = block_max<B>(v, red, x);
+    if (threadIdx.x == 0) P.udef[m] = v;
+  }
+  if (P.obj == 0) {
+    // gate: Phi = Tr(U_t^dag U)/d, F = |Phi|^2, w = 2 Phi / d, Omega = conj(w) U_t^dag U
+    Blk Tg = load_blk(P.Ut, x);
+    double2 part = make_double2(0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      part.x += Tg.re[r] * Ut.re[r] + Tg.im[r] * Ut.im[r];
+      part.y += Tg.re[r] * Ut.im[r] - Tg.im[r] * Ut.re[r];
+    }
+    double2 tr = block_sum2<B>(part, red, x);
+    const double phr = tr.x / P.d, phi = tr.y / P.d;
+    if (threadIdx.x == 0) P.Fm[m] = phr * phr + phi * phi;
+    const double wr = 2.0 * phr / P.d, wi = 2.0 * phi / P.d;
+    Blk Y = mm<B, C, N>(Tg, Ut, LA, LB, x);
+    store_blk(P.Om + (size_t)m * MB, cscale(wr, -wi, Y), x);
+    return;
+  }
+  // pairs: Z = PsiT^dag U Psi0, z_s = Z[s][s]
+  Blk P0 = load_blk(P.Psi0, x), PT = load_blk(P.PsiT, x);
+  Blk Y = mm<B, N, N>(Ut, P0, LA, LB, x);
+  Blk Z = mm<B, C, N>(PT, Y, LA, LB, x);
+  if (x.I == x.Jb) {
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (rowof(x.q, r) == x.c) zb[16 * x.I + x.c] = make_double2(Z.re[r], Z.im[r]);
+  }
+  __syncthreads();
+  const bool square = (P.obj == 1);
+  if (threadIdx.x == 0) {
+    double F = 0;
+    for (int s = 0; s < P.npairs; s++) {
+      double2 z = zb[s];
+      F += square ? (z.x * z.x + z.y * z.y) : z.x;
+    }
+    P.Fm[m] = F;
+  }
+  // w_s = 2 z_s (square) or 1 (real); zero beyond the pairs
+  auto wgt = [&](int s) -> double2 {
+    if (s >= P.npairs) return make_double2(0, 0);
+    double2 z = zb[s];
+    return square ? make_double2(2 * z.x, 2 * z.y) : make_double2(1, 0);
+  };
+  if (P.unitary) {
+    // Omega = Psi0 diag(conj w) PsiT^dag U: scale row s of W = PsiT^dag U by conj(w_s)
+    Blk W = mm<B, C, N>(PT, Ut, LA, LB, x);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      double2 ws = wgt(16 * x.I + rowof(x.q, r));
+      double a = W.re[r], b = W.im[r];
+      W.re[r] = ws.x * a + ws.y * b;
+      W.im[r] = ws.x * b - ws.y * a;
+    }
+    Blk Om = mm<B, N, N>(P0, W, LA, LB, x);
+    store_blk(P.Om + (size_t)m * MB, Om, x);
+    __syncthreads();
+    return;
+  }
+  // Lindblad: Lambda_n = PsiT diag(w); B_{S-1} = Lambda_n, B_s = T_{s+1}^dag B_{s+1}
+  Blk Lm;
+  {
+    double2 ws = wgt(16 * x.Jb + x.c);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      double a = PT.re[r], b = PT.im[r];
+      Lm.re[r] = ws.x * a - ws.y * b;
+      Lm.im[r] = ws.x * b + ws.y * a;
+    }
+  }
+  store_blk(P.Bb + ((size_t)m * P.nseg + P.nseg - 1) * MB, Lm, x);
+  for (int s = P.nseg - 2; s >= 0; s--) {
+    Blk Ts = load_blk(P.Tseg + ((size_t)m * P.nseg + s + 1) * MB, x);
+    Lm = mm<B, C, N>(Ts, Lm, LA, LB, x);
+    store_blk(P.Bb + ((size_t)m * P.nseg + s) * MB, Lm, x);
+  }
+  __syncthreads();
+}
+
+// Gradient contraction of one slice: G_m[j,k] = Re(c dt s_mj Tr(A_j M)).
+template <int B, int V>
+__device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, double* red, const Ln& x) {
+  const int nc = P.nc;
+  const size_t MB = (size_t)B * B * 256;
+  const bool want_im = (P.cim != 0.0);  // closed: Re(-i z) = Im z
+  for (int j0 = 0; j0 < nc; j0 += V) {
+    double v[V];
+#pragma unroll
+    for (int t = 0; t < V; t++) {
+      v[t] = 0;
+      const int j = j0 + t;
+      if (j < nc) {
+        Blk A = load_blk(P.HcT + (size_t)j * MB, x);
+        double s = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          s += want_im ? (A.re[r] * Mk.im[r] + A.im[r] * Mk.re[r]) : (A.re[r] * Mk.re[r] - A.im[r] * Mk.im[r]);
+        }
+        v[t] = s;
+      }
+    }
+    double tot = wave_reduce_spread<V>(v, x.lane);
+    constexpr int H = (V >= 64) ? 6 : (V >= 32) ? 5 : (V >= 16) ? 4 : (V >= 8) ? 3 : (V >= 4) ? 2 : (V >= 2) ? 1 : 0;
+    const int idx = x.lane >> (6 - H);
+    const bool leader = (x.lane & ((64 >> H) - 1)) == 0;
+    if constexpr (B > 1) {
+      if (leader) red[x.w * V + idx] = tot;
+      __syncthreads();
+      if (x.w == 0 && leader) {
+        double s = 0;
+        for (int w = 0; w < B * B; w++) s += red[w * V + idx];
+        tot = s;
+      }
+      __syncthreads();
+    }
+    if (x.w == 0 && leader) {
+      const int j = j0 + idx;
+      if (j < nc) {
+        const double f = P.dts[k] * P.scale[(size_t)m * nc + j];
+        // Re(c * f * z): closed c = -i -> f Im z ; Lindblad c = 1 -> f Re z
+        P.Gm[(size_t)m * nc * P.nt + (size_t)k * nc + j] = want_im ? f * tot * (-P.cim) : f * tot * P.cre;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- k_gradient
+template <int B, int V>
+__global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restrict__ probs,
+                                                          const WorkItem* __restrict__ items) {
+  const WorkItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const Ln x = lane_info<B>();
+  double2 *LA, *LB, *zb;
+  double* red;
+  smem_parts<B>(LA, LB, red, zb);
+  constexpr int NB = B * B;
+  const size_t MB = (size_t)NB * 256;
+  const int m = it.member;
+  Blk Es = load_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + it.seg) * MB, x);
+  if (P.unitary) {
+    Blk Om = load_blk(P.Om + (size_t)m * MB, x);
+    Blk Y = mm<B, N, N>(Es, Om, LA, LB, x);
+    Blk Cm = mm<B, N, C>(Y, Es, LA, LB, x);
+    for (int k = it.k0; k < it.k1; k++) {
+      Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
+      Blk Ud = conj_blk(Ut);                      // U^dag, own block
+      Blk Mk = mm<B, N, N>(Cm, Ud, LA, LB, x);    // C_k U_k^dag
+      contract<B, V>(P, m, k, Mk, red, x);
+      Cm = mm<B, T, N>(Ut, Mk, LA, LB, x);        // U_k M_k
+    }
+    return;
+  }
+  // Lindblad: forward states of the segment, then Lambda backward
+  Blk P0 = load_blk(P.Psi0, x);
+  Blk Pk = mm<B, N, N>(Es, P0, LA, LB, x);
+  const size_t wi = (size_t)m * P.nseg + it.seg;
+  double2* scr = P.Pscr + wi * (size_t)P.seglen * MB;
+  for (int k = it.k0; k < it.k1; k++) {
+    store_blk(scr + (size_t)(k - it.k0) * MB, Pk, x);
+    Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
+    Pk = mm<B, T, N>(Ut, Pk, LA, LB, x);
+  }
+  Blk Lm = load_blk(P.Bb + wi * MB, x);
+  for (int k = it.k1 - 1; k >= it.k0; k--) {
+    Blk Pc = load_blk(scr + (size_t)(k - it.k0) * MB, x);
+    Blk Mk = mm<B, N, C>(Pc, Lm, LA, LB, x);      // P_k Lambda_k^dag
+    contract<B, V>(P, m, k, Mk, red, x);
+    Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
+    Lm = mm<B, J, N>(Ut, Lm, LA, LB, x);          // U_k^dag Lambda_k  (U^dag = conj(U^T))
+  }
+}
+
+// ---------------------------------------------------------------- reductions
+__global__ void k_reduce_G(const ProbDev* __restrict__ probs, int nprob, const long long* __restrict__ off,
+                           long long total, double* __restrict__ G) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {  // last p with off[p] <= t
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const ProbDev& P = probs[lo];
+  const long long e = t - off[lo];
+  const long long ng = (long long)P.nc * P.nt;
+  double s = 0;
+  for (int m = 0; m < P.M; m++) s += P.weight[m] * P.Gm[(size_t)m * ng + e];
+  G[t] = s;
+}
+
+__global__ void k_reduce_F(const ProbDev* __restrict__ probs, int nprob, double* __restrict__ F) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob) return;
+  const ProbDev& P = probs[p];
+  double s = 0;
+  for (int m = 0; m < P.M; m++) s += P.weight[m] * P.Fm[m];
+  F[p] = s;
+}
+
+}  // namespace pr

@@ -1,0 +1,558 @@
+// pulsar_rocm.hip — C-ABI of libpulsar_rocm.so (declared in include/pulsar_rocm.h).
+//
+// Host side: validation of the problems, conversion of column-major ComplexF64
+// matrices to 16x16 tile images, time segmentation, device allocation and the
+// launch sequence k_segment -> k_combine -> k_gradient -> k_reduce per
+// evaluation. Every HIP call is checked; failures come back as PR_ERR_* codes
+// with a message for pr_last_error (the reference's GPU paths instead warn and
+// rerun on the CPU, CUDABackend.jl:144-147; that fallback belongs to the
+// caller, which sees the status code).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pulsar_rocm.h"
+#include "grape_kernels.hpp"
+
+using cd = std::complex<double>;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail(PR_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+bool fault_is(const char* what) {
+  const char* f = getenv("PULSAR_ROCM_FAULT");
+  return f && strcmp(f, what) == 0;
+}
+
+int g_device = -1;
+
+int ensure_device() {
+  if (g_device >= 0) {
+    HIPCHK(hipSetDevice(g_device));
+    return PR_OK;
+  }
+  return pr_init(0);
+}
+
+// element (r, c) of a column-major interleaved complex matrix
+inline cd at(const double* a, int d, int r, int c) {
+  size_t i = (size_t)r + (size_t)c * d;
+  return cd(a[2 * i], a[2 * i + 1]);
+}
+
+// Tile image of a d x d column-major matrix padded to 16B x 16B (transpose:
+// image of M^T). Block (I, J) holds 256 complex in [r][lane] order; lane
+// l = c + 16q, register r is element (16I + q + 4r, 16J + c), the FP64 MFMA
+// result layout.
+void tile_image(const double* a, int d, int B, bool transpose, double2* out) {
+  for (int I = 0; I < B; I++)
+    for (int J = 0; J < B; J++) {
+      double2* blk = out + (size_t)(I * B + J) * 256;
+      for (int r = 0; r < 4; r++)
+        for (int l = 0; l < 64; l++) {
+          int row = 16 * I + (l >> 4) + 4 * r, col = 16 * J + (l & 15);
+          cd v(0, 0);
+          if (row < d && col < d) v = transpose ? at(a, d, col, row) : at(a, d, row, col);
+          blk[r * 64 + l] = make_double2(v.real(), v.imag());
+        }
+    }
+}
+
+// tile image of the matrix whose column s is vector s of vecs ([n][d] complex)
+void tile_columns(const double* vecs, int n, int d, int B, double2* out) {
+  for (int I = 0; I < B; I++)
+    for (int J = 0; J < B; J++) {
+      double2* blk = out + (size_t)(I * B + J) * 256;
+      for (int r = 0; r < 4; r++)
+        for (int l = 0; l < 64; l++) {
+          int row = 16 * I + (l >> 4) + 4 * r, col = 16 * J + (l & 15);
+          double2 v = make_double2(0, 0);
+          if (row < d && col < n) v = make_double2(vecs[2 * ((size_t)col * d + row)], vecs[2 * ((size_t)col * d + row) + 1]);
+          blk[r * 64 + l] = v;
+        }
+    }
+}
+
+double norm1(const double* a, int d) {
+  double best = 0;
+  for (int c = 0; c < d; c++) {
+    double s = 0;
+    for (int r = 0; r < d; r++) s += std::abs(at(a, d, r, c));
+    best = std::max(best, s);
+  }
+  return best;
+}
+
+double herm_dev(const double* a, int d) {
+  double dev = 0;
+  for (int c = 0; c < d; c++)
+    for (int r = 0; r < d; r++) dev = std::max(dev, std::abs(at(a, d, r, c) - std::conj(at(a, d, c, r))));
+  return dev;
+}
+
+double unitary_dev(const double* a, int d) {
+  double s = 0;
+  for (int i = 0; i < d; i++)
+    for (int j = 0; j < d; j++) {
+      cd acc(0, 0);
+      for (int k = 0; k < d; k++) acc += std::conj(at(a, d, k, i)) * at(a, d, k, j);
+      if (i == j) acc -= 1.0;
+      s += std::norm(acc);
+    }
+  return std::sqrt(s) / d;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct Plan {
+  int n_probs = 0;
+  pr_options opt{};
+  std::vector<pr::ProbDev> host_probs;
+  std::vector<int> ng;                // n_ctrl * n_t per problem
+  std::vector<long long> coff;        // controls offsets (n_probs + 1)
+  long long n_params = 0;
+  std::vector<void*> allocs;
+  pr::ProbDev* d_probs = nullptr;
+  long long* d_off = nullptr;
+  double* d_controls = nullptr;
+  double* d_F = nullptr;
+  double* d_G = nullptr;
+  struct Group {
+    int B = 0, V = 0;
+    pr::WorkItem* d_items = nullptr;
+    int n_items = 0;
+    int2* d_members = nullptr;
+    int n_members = 0;
+  };
+  std::vector<Group> groups;
+  std::vector<double> taylor_deg;
+  hipStream_t stream = nullptr;
+  ~Plan() {
+    for (void* p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  template <class T>
+  int alloc(T** p, size_t count) {
+    void* q = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return fail(PR_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return PR_OK;
+  }
+  template <class T>
+  int upload(T** p, const T* src, size_t count) {
+    int rc = alloc(p, count);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return PR_OK;
+  }
+};
+
+int validate(const pr_problem& p, int idx) {
+  if (p.dim < 1 || p.n_ctrl < 1 || p.n_t < 1 || p.n_members < 1)
+    return fail(PR_ERR_ARG, "problem %d: dim, n_ctrl, n_t and n_members must be >= 1", idx);
+  if (!p.H0 || !p.Hc) return fail(PR_ERR_ARG, "problem %d: H0 and Hc are required", idx);
+  if (p.dynamics != PR_CLOSED && p.dynamics != PR_LINDBLAD) return fail(PR_ERR_ARG, "problem %d: unknown dynamics", idx);
+  if (p.dim > 64) return fail(PR_ERR_UNSUPPORTED, "problem %d: dim %d > 64 is not supported by this build", idx, p.dim);
+  if (!p.dt_slices && !(p.dt >= 0.0)) return fail(PR_ERR_ARG, "problem %d: dt must be non-negative", idx);
+  if (p.objective == PR_GATE) {
+    if (!p.U_target) return fail(PR_ERR_ARG, "problem %d: gate objective needs U_target", idx);
+    if (p.dynamics != PR_CLOSED) return fail(PR_ERR_UNSUPPORTED, "problem %d: gate objective needs closed dynamics", idx);
+    double dev = unitary_dev(p.U_target, p.dim);
+    if (dev > 1e-8) return fail(PR_ERR_ARG, "problem %d: target is not unitary: ||U^dag U - I||/dim = %g > 1e-8", idx, dev);
+  } else if (p.objective == PR_PAIRS_SQUARE || p.objective == PR_PAIRS_REAL) {
+    if (p.n_pairs < 1 || !p.psi_init || !p.psi_targ) return fail(PR_ERR_ARG, "problem %d: pair objective needs states", idx);
+    if (p.n_pairs > 16 * ((p.dim + 15) / 16))
+      return fail(PR_ERR_UNSUPPORTED, "problem %d: n_pairs %d exceeds the padded dimension", idx, p.n_pairs);
+  } else {
+    return fail(PR_ERR_ARG, "problem %d: unknown objective %d", idx, p.objective);
+  }
+  if (p.dynamics == PR_CLOSED) {
+    size_t dd = (size_t)p.dim * p.dim;
+    int nh0 = p.H0_per_member ? p.n_members : 1;
+    for (int m = 0; m < nh0; m++) {
+      double dev = herm_dev(p.H0 + 2 * dd * m, p.dim);
+      if (dev > 1e-10) return fail(PR_ERR_ARG, "problem %d: H0 (member %d) is not Hermitian: max|H - H^dag| = %g", idx, m, dev);
+    }
+    for (int j = 0; j < p.n_ctrl; j++) {
+      double dev = herm_dev(p.Hc + 2 * dd * j, p.dim);
+      if (dev > 1e-10) return fail(PR_ERR_ARG, "problem %d: control %d is not Hermitian: max|H - H^dag| = %g", idx, j, dev);
+    }
+  }
+  if (p.dt_slices)
+    for (int k = 0; k < p.n_t; k++)
+      if (!(p.dt_slices[k] >= 0.0)) return fail(PR_ERR_ARG, "problem %d: dt_slices[%d] must be non-negative", idx, k);
+  return PR_OK;
+}
+
+template <int B>
+int set_smem_attr() {
+  size_t bytes = pr::Smem<B>::bytes;
+  if (bytes > 65536) {
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_segment<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_combine<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  }
+  return PR_OK;
+}
+
+
+int next_pow2_cap32(int n) {
+  int v = 1;
+  while (v < n && v < 32) v <<= 1;
+  return v;
+}
+
+template <int B>
+int launch_gradient(const Plan& pl, const Plan::Group& g, hipStream_t st) {
+  const size_t sm = pr::Smem<B>::bytes;
+  const dim3 blk(64 * B * B);
+  switch (g.V) {
+#define PR_GCASE(VV)                                                                                        \
+  case VV:                                                                                                  \
+    hipLaunchKernelGGL((pr::k_gradient<B, VV>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items); \
+    break;
+    PR_GCASE(1) PR_GCASE(2) PR_GCASE(4) PR_GCASE(8) PR_GCASE(16) PR_GCASE(32)
+#undef PR_GCASE
+    default:
+      return fail(PR_ERR_LAUNCH, "internal: bad reduction width %d", g.V);
+  }
+  HIPCHK(hipGetLastError());
+  return PR_OK;
+}
+
+template <int B>
+int launch_group(const Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hipStream_t st) {
+  const size_t sm = pr::Smem<B>::bytes;
+  const dim3 blk(64 * B * B);
+  hipLaunchKernelGGL((pr::k_segment<B>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL((pr::k_combine<B>), dim3(g.n_members), blk, sm, st, pl.d_probs, g.d_members);
+  HIPCHK(hipGetLastError());
+  if (grad) return launch_gradient<B>(pl, g, st);
+  return PR_OK;
+}
+
+int run(const Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
+  for (const auto& g : pl.groups) {
+    int rc = PR_OK;
+    switch (g.B) {
+      case 1: rc = launch_group<1>(pl, g, d_u, d_G != nullptr, st); break;
+      case 2: rc = launch_group<2>(pl, g, d_u, d_G != nullptr, st); break;
+      case 3: rc = launch_group<3>(pl, g, d_u, d_G != nullptr, st); break;
+      case 4: rc = launch_group<4>(pl, g, d_u, d_G != nullptr, st); break;
+      default: rc = fail(PR_ERR_LAUNCH, "internal: block count %d", g.B);
+    }
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(pr::k_reduce_F, dim3((pl.n_probs + 127) / 128), dim3(128), 0, st, pl.d_probs, pl.n_probs, d_F);
+  HIPCHK(hipGetLastError());
+  if (d_G) {
+    long long total = pl.n_params;
+    hipLaunchKernelGGL(pr::k_reduce_G, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, pl.d_probs, pl.n_probs,
+                       pl.d_off, total, d_G);
+    HIPCHK(hipGetLastError());
+  }
+  return PR_OK;
+}
+
+int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
+  pl.n_probs = n_probs;
+  pl.host_probs.resize(n_probs);
+  pl.coff.assign(n_probs + 1, 0);
+  std::vector<std::vector<pr::WorkItem>> items(5);
+  std::vector<std::vector<int2>> members(5);
+  std::vector<int> vmax(5, 1);
+  for (int p = 0; p < n_probs; p++) {
+    const pr_problem& q = probs[p];
+    pr::ProbDev& D = pl.host_probs[p];
+    memset(&D, 0, sizeof D);
+    const int d = q.dim, B = (d + 15) / 16, NB = B * B, M = q.n_members, nc = q.n_ctrl, nt = q.n_t;
+    const size_t MB = (size_t)NB * 256, dd = (size_t)d * d;
+    D.d = d; D.B = B; D.nc = nc; D.nt = nt; D.M = M; D.dyn = q.dynamics; D.obj = q.objective;
+    D.npairs = q.objective == PR_GATE ? 0 : q.n_pairs;
+    D.h0pm = q.H0_per_member ? 1 : 0;
+    D.unitary = q.dynamics == PR_CLOSED ? 1 : 0;
+    D.grad_exact = pl.opt.grad_mode == PR_GRAD_EXACT;
+    D.cre = q.dynamics == PR_CLOSED ? 0.0 : 1.0;
+    D.cim = q.dynamics == PR_CLOSED ? -1.0 : 0.0;
+    // time segmentation: about 8192 wavefronts of work per problem group
+    int L;
+    if (pl.opt.segment > 0) {
+      L = pl.opt.segment;
+    } else {
+      long long want = std::max(1LL, (long long)(8192 / NB));
+      long long S = (want + M - 1) / M;
+      S = std::min<long long>(std::max<long long>(S, 1), nt);
+      L = (int)((nt + S - 1) / S);
+      L = std::max(L, std::min(nt, 4));
+    }
+    L = std::min(L, nt);
+    const int S = (nt + L - 1) / L;
+    D.nseg = S; D.seglen = L;
+    // operators
+    const int nh0 = D.h0pm ? M : 1;
+    std::vector<double2> img(MB * std::max(nh0, nc));
+    std::vector<double> nrm(std::max(nh0, nc));
+    for (int m = 0; m < nh0; m++) {
+      tile_image(q.H0 + 2 * dd * m, d, B, false, img.data() + MB * m);
+      nrm[m] = norm1(q.H0 + 2 * dd * m, d);
+    }
+    int rc;
+    double2* dp;
+    double* dd_;
+    if ((rc = pl.upload(&dp, img.data(), MB * nh0))) return rc;
+    D.H0 = dp;
+    if ((rc = pl.upload(&dd_, nrm.data(), nh0))) return rc;
+    D.n0 = dd_;
+    for (int j = 0; j < nc; j++) {
+      tile_image(q.Hc + 2 * dd * j, d, B, false, img.data() + MB * j);
+      nrm[j] = norm1(q.Hc + 2 * dd * j, d);
+    }
+    if ((rc = pl.upload(&dp, img.data(), MB * nc))) return rc;
+    D.Hc = dp;
+    if ((rc = pl.upload(&dd_, nrm.data(), nc))) return rc;
+    D.ncn = dd_;
+    for (int j = 0; j < nc; j++) tile_image(q.Hc + 2 * dd * j, d, B, true, img.data() + MB * j);
+    if ((rc = pl.upload(&dp, img.data(), MB * nc))) return rc;
+    D.HcT = dp;
+    std::vector<double> sc((size_t)M * nc, 1.0);
+    if (q.ctrl_scale) std::copy(q.ctrl_scale, q.ctrl_scale + (size_t)M * nc, sc.begin());
+    if ((rc = pl.upload(&dd_, sc.data(), sc.size()))) return rc;
+    D.scale = dd_;
+    std::vector<double> wt(M, q.objective == PR_GATE ? 1.0 / M : 1.0 / ((double)M * q.n_pairs));
+    if (q.weight) std::copy(q.weight, q.weight + M, wt.begin());
+    if ((rc = pl.upload(&dd_, wt.data(), M))) return rc;
+    D.weight = dd_;
+    std::vector<double> dts(nt, q.dt);
+    if (q.dt_slices) std::copy(q.dt_slices, q.dt_slices + nt, dts.begin());
+    if ((rc = pl.upload(&dd_, dts.data(), nt))) return rc;
+    D.dts = dd_;
+    if (q.objective == PR_GATE) {
+      tile_image(q.U_target, d, B, false, img.data());
+      if ((rc = pl.upload(&dp, img.data(), MB))) return rc;
+      D.Ut = dp;
+    } else {
+      tile_columns(q.psi_init, q.n_pairs, d, B, img.data());
+      if ((rc = pl.upload(&dp, img.data(), MB))) return rc;
+      D.Psi0 = dp;
+      tile_columns(q.psi_targ, q.n_pairs, d, B, img.data());
+      if ((rc = pl.upload(&dp, img.data(), MB))) return rc;
+      D.PsiT = dp;
+    }
+    // scratch
+    if ((rc = pl.alloc(&D.Ust, (size_t)M * nt * MB))) return rc;
+    if ((rc = pl.alloc(&D.Tseg, (size_t)M * S * MB))) return rc;
+    if ((rc = pl.alloc(&D.Eseg, (size_t)M * (S + 1) * MB))) return rc;
+    if ((rc = pl.alloc(&D.Om, (size_t)M * MB))) return rc;
+    if (!D.unitary) {
+      if ((rc = pl.alloc(&D.Bb, (size_t)M * S * MB))) return rc;
+      if ((rc = pl.alloc(&D.Pscr, (size_t)M * S * L * MB))) return rc;
+    }
+    if ((rc = pl.alloc(&D.Fm, M))) return rc;
+    if ((rc = pl.alloc(&D.Gm, (size_t)M * nc * nt))) return rc;
+    if ((rc = pl.alloc(&D.udef, M))) return rc;
+    HIPCHK(hipMemset(D.udef, 0, sizeof(double) * M));
+    if ((rc = pl.alloc(&D.smax, (size_t)M * S))) return rc;
+    D.coff = pl.coff[p];
+    pl.coff[p + 1] = pl.coff[p] + (long long)nc * nt;
+    for (int m = 0; m < M; m++) {
+      members[B].push_back(make_int2(p, m));
+      for (int s = 0; s < S; s++) {
+        pr::WorkItem w;
+        w.prob = p; w.member = m; w.seg = s; w.k0 = s * L; w.k1 = std::min(nt, (s + 1) * L);
+        items[B].push_back(w);
+      }
+    }
+    vmax[B] = std::max(vmax[B], next_pow2_cap32(nc));
+  }
+  pl.n_params = pl.coff[n_probs];
+  int rc;
+  if ((rc = pl.upload(&pl.d_probs, pl.host_probs.data(), n_probs))) return rc;
+  if ((rc = pl.upload(&pl.d_off, pl.coff.data(), n_probs + 1))) return rc;
+  if ((rc = pl.alloc(&pl.d_controls, pl.n_params))) return rc;
+  if ((rc = pl.alloc(&pl.d_F, n_probs))) return rc;
+  if ((rc = pl.alloc(&pl.d_G, pl.n_params))) return rc;
+  for (int B = 1; B <= 4; B++) {
+    if (items[B].empty()) continue;
+    Plan::Group g;
+    g.B = B;
+    g.V = vmax[B];
+    g.n_items = (int)items[B].size();
+    g.n_members = (int)members[B].size();
+    if ((rc = pl.upload(&g.d_items, items[B].data(), items[B].size()))) return rc;
+    if ((rc = pl.upload(&g.d_members, members[B].data(), members[B].size()))) return rc;
+    pl.groups.push_back(g);
+    switch (B) {
+      case 1: rc = set_smem_attr<1>(); break;
+      case 2: rc = set_smem_attr<2>(); break;
+      case 3: rc = set_smem_attr<3>(); break;
+      case 4: rc = set_smem_attr<4>(); break;
+    }
+    if (rc) return rc;
+  }
+  return PR_OK;
+}
+
+int fill_diag(const Plan& pl, double* diag) {
+  for (int p = 0; p < pl.n_probs; p++) {
+    const pr::ProbDev& D = pl.host_probs[p];
+    std::vector<double> ud(D.M);
+    std::vector<int> sm((size_t)D.M * D.nseg);
+    HIPCHK(hipMemcpy(ud.data(), D.udef, sizeof(double) * D.M, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sm.data(), D.smax, sizeof(int) * sm.size(), hipMemcpyDeviceToHost));
+    double umax = 0;
+    for (double v : ud) umax = std::max(umax, v);
+    int smax = 0;
+    for (int v : sm) smax = std::max(smax, v);
+    diag[PR_DIAG_PER_PROBLEM * p + 0] = umax;
+    diag[PR_DIAG_PER_PROBLEM * p + 1] = smax;
+    diag[PR_DIAG_PER_PROBLEM * p + 2] = pr::TAYLOR_DEG;
+    diag[PR_DIAG_PER_PROBLEM * p + 3] = D.nseg;
+  }
+  return PR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pr_abi_version(void) { return PR_ABI_VERSION; }
+
+int pr_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int pr_init(int device) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(PR_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n) return fail(PR_ERR_DEVICE, "device %d out of range (%d visible)", device, n);
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(PR_ERR_DEVICE, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+  HIPCHK(hipSetDevice(device));
+  g_device = device;
+  return PR_OK;
+}
+
+int pr_last_error(char* buf, size_t n) {
+  if (!buf || n == 0) return PR_ERR_ARG;
+  snprintf(buf, n, "%s", g_err.c_str());
+  return PR_OK;
+}
+
+int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, void** plan) {
+  if (fault_is("plan")) return fail(PR_ERR_INJECTED, "failure injected by PULSAR_ROCM_FAULT=plan");
+  if (!probs || n_probs < 1 || !plan) return fail(PR_ERR_ARG, "pr_plan_create: need problems and a plan pointer");
+  *plan = nullptr;
+  pr_options o{PR_FP64, PR_GRAD_REFERENCE, 0, -1};
+  if (opt) o = *opt;
+  if (o.precision != PR_FP64) return fail(PR_ERR_UNSUPPORTED, "precision %d is not available in this build", o.precision);
+  if (o.grad_mode != PR_GRAD_REFERENCE) return fail(PR_ERR_UNSUPPORTED, "grad_mode %d is not available in this build", o.grad_mode);
+  if (o.segment < 0) return fail(PR_ERR_ARG, "segment must be >= 0");
+  for (int p = 0; p < n_probs; p++) {
+    int rc = validate(probs[p], p);
+    if (rc) return rc;
+  }
+  int rc = ensure_device();
+  if (rc) return rc;
+  Plan* pl = new Plan;
+  pl->opt = o;
+  hipError_t e = hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete pl;
+    return fail(PR_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  rc = build_plan(*pl, probs, n_probs);
+  if (rc) {
+    delete pl;
+    return rc;
+  }
+  *plan = pl;
+  return PR_OK;
+}
+
+int pr_eval(void* plan, const double* const* controls, double* F, double* const* G, double* diag) {
+  if (fault_is("eval")) return fail(PR_ERR_INJECTED, "failure injected by PULSAR_ROCM_FAULT=eval");
+  if (!plan || !controls || !F) return fail(PR_ERR_ARG, "pr_eval: plan, controls and F are required");
+  Plan& pl = *static_cast<Plan*>(plan);
+  if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
+  for (int p = 0; p < pl.n_probs; p++) {
+    if (!controls[p]) return fail(PR_ERR_ARG, "pr_eval: controls[%d] is NULL", p);
+    HIPCHK(hipMemcpyAsync(pl.d_controls + pl.coff[p], controls[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]),
+                          hipMemcpyHostToDevice, pl.stream));
+  }
+  int rc = run(pl, pl.d_controls, pl.d_F, G ? pl.d_G : nullptr, pl.stream);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(F, pl.d_F, sizeof(double) * pl.n_probs, hipMemcpyDeviceToHost, pl.stream));
+  if (G)
+    for (int p = 0; p < pl.n_probs; p++)
+      if (G[p])
+        HIPCHK(hipMemcpyAsync(G[p], pl.d_G + pl.coff[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]),
+                              hipMemcpyDeviceToHost, pl.stream));
+  HIPCHK(hipStreamSynchronize(pl.stream));
+  if (diag) return fill_diag(pl, diag);
+  return PR_OK;
+}
+
+int pr_eval_device(void* plan, const double* d_controls, double* d_F, double* d_G, void* stream) {
+  if (fault_is("eval")) return fail(PR_ERR_INJECTED, "failure injected by PULSAR_ROCM_FAULT=eval");
+  if (!plan || !d_controls || !d_F) return fail(PR_ERR_ARG, "pr_eval_device: plan, d_controls and d_F are required");
+  Plan& pl = *static_cast<Plan*>(plan);
+  if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
+  return run(pl, d_controls, d_F, d_G, static_cast<hipStream_t>(stream));
+}
+
+long long pr_plan_n_params(void* plan) {
+  if (!plan) return -1;
+  return static_cast<Plan*>(plan)->n_params;
+}
+
+int pr_plan_destroy(void* plan) {
+  if (!plan) return PR_OK;
+  delete static_cast<Plan*>(plan);
+  return PR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+// tiles.hpp — 16x16 complex FP64 register blocks and FP64-MFMA products on gfx950.
+//
+// A d x d complex matrix (d <= 16*B) is held by a workgroup of B*B wavefronts;
+// wavefront w = I*B + J owns block (I, J). Inside a wavefront, lane l = c + 16*q
+// holds in register r the element (row q + 4r, column c) of its block: this is
+// the result layout of v_mfma_f64_16x16x4_f64 on gfx950 (measured by an
+// in-bounds lane-layout check), so products chain without reshuffling: the
+// k-step kb of a product sums over inner index k = q + 4*kb, which makes the
+// B operand of step kb exactly result register kb. Operands that must be read
+// in another layout go through LDS as plain 16 x 17 (padded) complex tiles.
+//
+// Per output block, a product C = op(A) * op(B) runs, for each k-block K and
+// k-step kb, four MFMAs: Re += Ar*Br - Ai*Bi, Im += Ar*Bi + Ai*Br.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pr {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct Blk {
+  d4 re, im;
+};
+
+enum Op { N = 0, T = 1, C = 2, J = 3 };  // none, transpose, conjugate transpose, conjugate
+
+struct Ln {
+  int lane, q, c, w, I, Jb;
+};
+
+template <int B>
+__device__ inline Ln lane_info() {
+  Ln x;
+  x.lane = threadIdx.x & 63;
+  x.q = x.lane >> 4;
+  x.c = x.lane & 15;
+  x.w = threadIdx.x >> 6;
+  x.I = x.w / B;
+  x.Jb = x.w % B;
+  return x;
+}
+
+// row of the block held by lane-group q in register (or k-step) r
+__device__ constexpr int rowof(int q, int r) { return q + 4 * r; }
+
+__device__ inline d4 mfma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ inline Blk zero_blk() {
+  Blk b;
+  b.re = d4{0, 0, 0, 0};
+  b.im = d4{0, 0, 0, 0};
+  return b;
+}
+
+// identity restricted to the wavefront's block
+__device__ inline Blk eye_blk(const Ln& x) {
+  Blk b = zero_blk();
+  if (x.I == x.Jb) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) b.re[r] = (rowof(x.q, r) == x.c) ? 1.0 : 0.0;
+  }
+  return b;
+}
+
+// Global tile image: 256 complex per block, register-major: [r][lane].
+__device__ inline Blk load_blk(const double2* __restrict__ img, const Ln& x) {
+  Blk b;
+  const double2* p = img + (size_t)x.w * 256;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    double2 v = p[r * 64 + x.lane];
+    b.re[r] = v.x;
+    b.im[r] = v.y;
+  }
+  return b;
+}
+
+__device__ inline void store_blk(double2* __restrict__ img, const Blk& b, const Ln& x) {
+  double2* p = img + (size_t)x.w * 256;
+#pragma unroll
+  for (int r = 0; r < 4; r++) p[r * 64 + x.lane] = make_double2(b.re[r], b.im[r]);
+}
+
+// LDS tile: block blk, row, col -> padded index
+__device__ inline int lidx(int blk, int row, int col) { return blk * 272 + row * 17 + col; }
+
+template <int B>
+__device__ inline void put(double2* L, const Blk& b, const Ln& x) {
+#pragma unroll
+  for (int r = 0; r < 4; r++) L[lidx(x.w, rowof(x.q, r), x.c)] = make_double2(b.re[r], b.im[r]);
+}
+
+// A-operand element op(A)[16I + c][16K + q + 4kb] for the MFMA k-step kb.
+template <int B, int OP>
+__device__ inline double2 fetch_a(const double2* L, const Blk& own, int K, int kb, const Ln& x) {
+  if constexpr (B == 1 && (OP == T || OP == C)) {
+    double2 v = make_double2(own.re[kb], own.im[kb]);
+    if constexpr (OP == C) v.y = -v.y;
+    return v;
+  } else {
+    double2 v;
+    if constexpr (OP == N || OP == J)
+      v = L[lidx(x.I * B + K, x.c, rowof(x.q, kb))];
+    else
+      v = L[lidx(K * B + x.I, rowof(x.q, kb), x.c)];
+    if constexpr (OP == C || OP == J) v.y = -v.y;
+    return v;
+  }
+}
+
+// B-operand element op(B)[16K + q + 4kb][16J + c].
+template <int B, int OP>
+__device__ inline double2 fetch_b(const double2* L, const Blk& own, int K, int kb, const Ln& x) {
+  if constexpr (B == 1 && (OP == N || OP == J)) {
+    double2 v = make_double2(own.re[kb], own.im[kb]);
+    if constexpr (OP == J) v.y = -v.y;
+    return v;
+  } else {
+    double2 v;
+    if constexpr (OP == N || OP == J)
+      v = L[lidx(K * B + x.Jb, rowof(x.q, kb), x.c)];
+    else
+      v = L[lidx(x.Jb * B + K, x.c, rowof(x.q, kb))];
+    if constexpr (OP == C || OP == J) v.y = -v.y;
+    return v;
+  }
+}
+
+template <int B, int OP>
+__device__ constexpr bool a_in_lds() {
+  return !(B == 1 && (OP == T || OP == C));
+}
+template <int B, int OP>
+__device__ constexpr bool b_in_lds() {
+  return !(B == 1 && (OP == N || OP == J));
+}
+
+// Accumulating block product: acc += sum_K op(A)_{I,K} op(B)_{K,J}
+template <int B, int OA, int OB>
+__device__ inline void gemm_acc(Blk& acc, const double2* LA, const Blk& Aown, const double2* LB,
+                                const Blk& Bown, const Ln& x) {
+#pragma unroll
+  for (int K = 0; K < B; K++) {
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+      double2 a = fetch_a<B, OA>(LA, Aown, K, kb, x);
+      double2 b = fetch_b<B, OB>(LB, Bown, K, kb, x);
+      acc.re = mfma(a.x, b.x, acc.re);
+      acc.im = mfma(a.x, b.y, acc.im);
+      acc.re = mfma(-a.y, b.y, acc.re);
+      acc.im = mfma(a.y, b.x, acc.im);
+    }
+  }
+}
+
+// C = op(A) op(B). LA/LB are this workgroup's LDS scratch tiles.
+template <int B, int OA, int OB>
+__device__ inline Blk mm(const Blk& A, const Blk& Bm, double2* LA, double2* LB, const Ln& x) {
+  constexpr bool la = a_in_lds<B, OA>(), lb = b_in_lds<B, OB>();
+  if constexpr (la) put<B>(LA, A, x);
+  if constexpr (lb) put<B>(LB, Bm, x);
+  if constexpr (la || lb) __syncthreads();
+  Blk acc = zero_blk();
+  gemm_acc<B, OA, OB>(acc, LA, A, LB, Bm, x);
+  if constexpr (la || lb) __syncthreads();
+  return acc;
+}
+
+__device__ inline Blk axpby(double a, const Blk& X, double b, const Blk& Y) {
+  Blk r;
+  r.re = a * X.re + b * Y.re;
+  r.im = a * X.im + b * Y.im;
+  return r;
+}
+__device__ inline Blk conj_blk(const Blk& X) {
+  Blk r;
+  r.re = X.re;
+  r.im = -X.im;
+  return r;
+}
+// (ar + i ai) * X
+__device__ inline Blk cscale(double ar, double ai, const Blk& X) {
+  Blk r;
+  r.re = ar * X.re - ai * X.im;
+  r.im = ar * X.im + ai * X.re;
+  return r;
+}
+
+// Sum of V = 2^H values over the 64 lanes of a wavefront. On return lane l
+// holds the total of value index l >> (6 - H) (every lane of its group).
+template <int V>
+__device__ inline double wave_reduce_spread(double (&v)[V], int lane) {
+  constexpr int H = (V >= 64) ? 6 : (V >= 32) ? 5 : (V >= 16) ? 4 : (V >= 8) ? 3 : (V >= 4) ? 2 : (V >= 2) ? 1 : 0;
+  static_assert((1 << H) == V, "V must be a power of two <= 64");
+#pragma unroll
+  for (int t = 0; t < H; t++) {
+    const int m = 32 >> t;
+    const int half = V >> (t + 1);
+    const bool up = (lane & m) != 0;
+#pragma unroll
+    for (int i = 0; i < half; i++) {
+      double send = up ? v[i] : v[i + half];
+      double keep = up ? v[i + half] : v[i];
+      v[i] = keep + __shfl_xor(send, m);
+    }
+  }
+  double s = v[0];
+#pragma unroll
+  for (int m = (32 >> H); m >= 1; m >>= 1) s += __shfl_xor(s, m);
+  return s;
+}
+
+}  // namespace pr

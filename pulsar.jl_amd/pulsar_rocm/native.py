@@ -1,0 +1,150 @@
+"""ctypes binding of libpulsar_rocm.so (the C-ABI in include/pulsar_rocm.h).
+
+This is the device path. There is no CPU fallback here: when the library or
+the GPU is missing, every call raises `RocmError`; falling back to another
+device is the caller's decision (the host API does it only on request).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Sequence
+
+import numpy as np
+
+from .problem import COptions, Problem, controls_cm, grad_from_cm, to_c
+
+LIB_NAME = "libpulsar_rocm.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+PR_ERR = {
+    -1: "PR_ERR_ARG",
+    -2: "PR_ERR_DEVICE",
+    -3: "PR_ERR_NOMEM",
+    -4: "PR_ERR_LAUNCH",
+    -5: "PR_ERR_UNSUPPORTED",
+    -6: "PR_ERR_INJECTED",
+}
+
+EXPORTS = ("pr_abi_version", "pr_device_count", "pr_init", "pr_last_error", "pr_plan_create",
+           "pr_eval", "pr_eval_device", "pr_plan_n_params", "pr_plan_destroy")
+
+
+class RocmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{PR_ERR.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the shared library and declare every exported entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RocmError(-2, f"{path} is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.pr_abi_version.restype = ctypes.c_int
+    L.pr_device_count.restype = ctypes.c_int
+    L.pr_init.argtypes = [ctypes.c_int]
+    L.pr_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.pr_plan_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(COptions),
+                                 ctypes.POINTER(ctypes.c_void_p)]
+    L.pr_eval.argtypes = [ctypes.c_void_p, ctypes.POINTER(dp), dp, ctypes.POINTER(dp), dp]
+    L.pr_eval_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p]
+    L.pr_plan_n_params.argtypes = [ctypes.c_void_p]
+    L.pr_plan_n_params.restype = ctypes.c_longlong
+    L.pr_plan_destroy.argtypes = [ctypes.c_void_p]
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(512)
+    load().pr_last_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise RocmError(rc, last_error())
+
+
+def device_count() -> int:
+    return int(load().pr_device_count())
+
+
+def init(device: int = 0) -> None:
+    check(load().pr_init(int(device)))
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Plan:
+    """A device plan over one or more problems (pr_plan_create)."""
+
+    def __init__(self, problems: Sequence[Problem], precision: int = 0, grad_mode: int = 0,
+                 segment: int = 0, store_props: int = -1):
+        L = load()
+        self.problems = list(problems)
+        n = len(self.problems)
+        arr = (type(to_c(self.problems[0])) * n)()
+        for i, p in enumerate(self.problems):
+            arr[i] = to_c(p)
+        self._structs = arr
+        opt = COptions(int(precision), int(grad_mode), int(segment), int(store_props))
+        h = ctypes.c_void_p()
+        check(L.pr_plan_create(ctypes.cast(arr, ctypes.c_void_p), n, ctypes.byref(opt), ctypes.byref(h)))
+        self._h = h
+        self.n_params = int(L.pr_plan_n_params(h))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def eval(self, controls: Sequence[np.ndarray], grad: bool = True, diag: bool = False):
+        """F (array [n_probs]) and G (list of [n_ctrl, n_t]) for per-problem controls."""
+        L = load()
+        n = len(self.problems)
+        if len(controls) != n:
+            raise ValueError(f"expected {n} control arrays, got {len(controls)}")
+        bufs = []
+        for p, u in zip(self.problems, controls):
+            u = np.asarray(u, dtype=np.float64)
+            if u.shape != (p.n_ctrl, p.n_t):
+                raise ValueError(f"controls shape {u.shape} != ({p.n_ctrl}, {p.n_t})")
+            bufs.append(controls_cm(u))
+        cptrs = (ctypes.POINTER(ctypes.c_double) * n)(*[_dptr(b) for b in bufs])
+        F = np.zeros(n)
+        gbufs = [np.zeros(p.n_ctrl * p.n_t) for p in self.problems] if grad else []
+        gptrs = (ctypes.POINTER(ctypes.c_double) * n)(*[_dptr(g) for g in gbufs]) if grad else None
+        D = np.zeros(4 * n) if diag else None
+        check(L.pr_eval(self._h, cptrs, _dptr(F), gptrs, _dptr(D) if diag else None))
+        G = [grad_from_cm(g, p.n_ctrl, p.n_t) for g, p in zip(gbufs, self.problems)] if grad else None
+        if diag:
+            return F, G, D.reshape(n, 4)
+        return F, G
+
+    def eval_device(self, d_controls: int, d_F: int, d_G: int | None, stream: int | None = None) -> None:
+        """Enqueue an evaluation on device pointers (ints), e.g. from torch tensors."""
+        check(load().pr_eval_device(self._h, ctypes.c_void_p(d_controls), ctypes.c_void_p(d_F),
+                                    ctypes.c_void_p(d_G) if d_G else None,
+                                    ctypes.c_void_p(stream) if stream else None))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load().pr_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
