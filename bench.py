@@ -1,0 +1,159 @@
+"""GRAPE fidelity+gradient evaluations per second on one MI355X.
+
+Workload: 4-qubit (d=16) closed-system gate synthesis with a 256-member
+robustness ensemble (16 detunings x 16 control-amplitude scales), 8 x/y
+drives, 2000 slices of 1 us, Haar-random 16x16 target; synthetic seeded
+inputs. One step = one ensemble F and its gradient dF/du [8 x 2000] at one
+control point, i.e. one ensemble_value_and_grad of build_ensemble_from_systems
+(src/Optimization/Ensemble/EnsembleObjective.jl:272-307), with the controls
+already resident in HBM (pr_eval_device on the plan's buffers).
+
+Printed JSON line: the metric, the roofline of the dominant kernel (kernel
+time from HIP events recorded by the library on its launch stream), the
+accuracy of the device result against the FP64 CPU oracle on a member sample,
+and the oracle timed on this host's cores (the CPU baseline).
+
+    python bench.py [--gpus 1] [--steps 30] [--warmup 3] [--precision fp64|mixed]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pulsar.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+from pulsar_rocm import native, workloads  # noqa: E402
+from pulsar_rocm.problem import PR_FP64, PR_MIXED  # noqa: E402
+
+MFMA_PEAK_TFLOPS = {"fp64": 78.6}
+HBM_PEAK_GBS = 8000.0
+
+
+def flops_model(d, nc):
+    """Algorithmic flops per member-slice: 9 complex d x d products (nominal
+    6 for exp, forward, backward, gradient) + control traces and assembly,
+    split over the two main kernels."""
+    total = 72 * d**3 + 16 * nc * d**2
+    segment = 56 * d**3 + 8 * nc * d**2  # exp (6 products) + chain product + assembly
+    gradient = 16 * d**3 + 8 * nc * d**2  # adjoint product + contraction product + traces
+    assert segment + gradient == total
+    return total, segment, gradient
+
+
+def cpu_baseline(p, u, n_sample, threads):
+    """The FP64 oracle (reference-faithful: separate fidelity and gradient
+    passes, as the per-member closures of EnsembleObjective do) on a member
+    sample; returns (evals/s scaled to the full ensemble, seconds, F, G)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    q, uu = workloads.shrink(p, u, n_members=n_sample)
+    t0 = time.perf_counter()
+    F, G = oracle.evaluate(q, uu, separate_fg=True, n_threads=threads)
+    dt = time.perf_counter() - t0
+    return (n_sample / p.n_members) / dt, dt, F, G, q
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--cpu-sample", type=int, default=32, help="members timed for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        if int(os.environ.get("RANK", "0")) == 0:
+            print(f"bench.py: only the single-GPU measurement exists (asked for {max(args.gpus, world)} GPUs); "
+                  "the ensemble would shard members across GPUs with one all-reduce of (F, G) per step, "
+                  "which this build does not run.", file=sys.stderr)
+        sys.exit(2)
+
+    prec = PR_FP64 if args.precision == "fp64" else PR_MIXED
+    p, u = workloads.four_qubit_ensemble()
+    native.init(0)
+    plan = native.Plan([p], precision=prec)
+    plan.upload_controls([u])
+    d_u, d_F, d_G = plan.buffers()
+    for _ in range(args.warmup):
+        plan.eval_device(d_u, d_F, d_G)
+    plan.synchronize()
+
+    plan.set_timing(True)
+    plan.kernel_times()  # reset
+    plan.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.eval_device(d_u, d_F, d_G)
+    plan.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = plan.kernel_times()
+    plan.set_timing(False)
+
+    # the same timed loop without event records, for the headline number
+    plan.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.eval_device(d_u, d_F, d_G)
+    plan.synchronize()
+    elapsed_plain = time.perf_counter() - t0
+    elapsed = min(elapsed, elapsed_plain)
+
+    evals_s = args.steps / elapsed
+    M, nt, d, nc = p.n_members, p.n_t, p.dim, p.n_ctrl
+    units = M * nt
+    f_tot, f_seg, f_grad = flops_model(d, nc)
+    seg_ms = kt["segment"][0] / max(kt["segment"][1], 1)
+    grad_ms = kt["gradient"][0] / max(kt["gradient"][1], 1)
+    dominant, dom_ms, dom_flops = ("k_segment", seg_ms, f_seg) if seg_ms >= grad_ms else ("k_gradient", grad_ms, f_grad)
+    achieved = dom_flops * units / (dom_ms * 1e-3) / 1e12
+    peak = MFMA_PEAK_TFLOPS["fp64"]
+
+    # accuracy against the FP64 oracle on a member sample (same inputs)
+    out = {
+        "metric": "GRAPE fidelity+gradient evals/s (4q x 256 ens) at |dF|<=1e-9 vs FP64",
+        "value": round(evals_s, 3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == PR_FP64 else "f64+bf16",
+        "data": "synthetic (seeded numpy: random controls, Haar target)",
+        "config": {"workload": "four_qubit_ensemble: d=16, 256 members (16 detuning x 16 amplitude), "
+                               "8 controls, 2000 slices, dt=1us, gate target",
+                   "members": M, "slices": nt, "dim": d, "controls": nc, "precision": args.precision},
+        "roofline": {"bound": "mfma", "kernel": dominant, "achieved": round(achieved, 3), "peak": peak,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel_ms": round(dom_ms, 4),
+                     "algorithmic_flops_per_launch": dom_flops * units},
+        "eval_tflops_algorithmic": round(f_tot * units * evals_s / 1e12, 3),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+    }
+    if args.cpu_sample > 0:
+        cpu_rate, cpu_s, Fo, Go, q = cpu_baseline(p, u, args.cpu_sample, args.cpu_threads)
+        qplan = native.Plan([q], precision=prec)
+        Fd, Gd, D = qplan.eval([u], diag=True)
+        qplan.close()
+        threads = args.cpu_threads or os.cpu_count()
+        out["cpu_baseline"] = {"value": round(cpu_rate, 5), "unit": "evals/s", "cores": threads, "kind": "port",
+                               "sample": f"{args.cpu_sample} of {M} members, all {nt} slices, FP64 oracle with "
+                                         f"separate F and G passes, {threads} OpenMP threads, {cpu_s:.2f} s"}
+        out["accuracy"] = {"members": args.cpu_sample, "abs_dF": float(abs(Fd[0] - Fo)),
+                           "max_rel_dG": float(np.abs(Gd[0] - Go).max() / np.abs(Go).max()),
+                           "unitarity_defect": float(D[0, 0]), "F": float(Fd[0])}
+    plan.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

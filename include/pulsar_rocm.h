@@ -136,15 +136,36 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
 int pr_eval(void* plan, const double* const* controls, double* F, double* const* G, double* diag);
 
 /* Device-resident evaluation on `stream` (a hipStream_t, or NULL for the
- * default stream): d_controls holds every problem's controls back to back in
- * problem order, d_F receives n_probs fidelities and d_G (or NULL) the
+ * plan's own stream): d_controls holds every problem's controls back to back
+ * in problem order, d_F receives n_probs fidelities and d_G (or NULL) the
  * gradients back to back in the same order. Asynchronous: returns after
- * enqueueing; all pointers are device pointers. */
+ * enqueueing; all pointers are device pointers. No counterpart in the
+ * reference (its GPU paths copy host arrays on every call); it serves callers
+ * that keep controls resident in HBM across optimizer iterations. */
 int pr_eval_device(void* plan, const double* d_controls, double* d_F, double* d_G, void* stream);
+
+/* The plan's own device buffers used by pr_eval (controls and gradient:
+ * pr_plan_n_params doubles each; F: n_probs doubles). */
+int pr_plan_buffers(void* plan, double** d_controls, double** d_F, double** d_G);
+/* Copy host controls (as for pr_eval) into the plan's device control buffer. */
+int pr_plan_upload_controls(void* plan, const double* const* controls);
+/* Wait for all work enqueued on the plan's own stream. */
+int pr_plan_synchronize(void* plan);
 
 /* Total number of control values of the plan (sum of n_ctrl*n_t), i.e. the
  * length of d_controls / d_G for pr_eval_device. */
 long long pr_plan_n_params(void* plan);
+
+/* Per-kernel timing (no counterpart in the reference, which has no device
+ * profiling): when enabled, every evaluation records a HIP event pair on its
+ * stream around each kernel launch. pr_plan_kernel_times waits for the
+ * recorded events, returns the accumulated milliseconds and launch counts per
+ * kernel class since the previous call, and resets them. Classes:
+ * 0 segment exps + chain products, 1 prefix combine + overlaps,
+ * 2 gradient walk + contraction, 3 ensemble reductions. */
+#define PR_KERNEL_CLASSES 4
+int pr_plan_set_timing(void* plan, int enable);
+int pr_plan_kernel_times(void* plan, double* ms, long long* launches);
 
 int pr_plan_destroy(void* plan);
 

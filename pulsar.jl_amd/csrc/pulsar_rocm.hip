@@ -157,7 +157,24 @@ struct Plan {
   std::vector<Group> groups;
   std::vector<double> taylor_deg;
   hipStream_t stream = nullptr;
+  // optional per-kernel timing with HIP events on the launch stream
+  bool timing = false;
+  std::vector<hipEvent_t> ev_free;
+  struct Rec {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  hipEvent_t ev_open = nullptr;
+  double acc_ms[PR_KERNEL_CLASSES] = {0};
+  long long acc_n[PR_KERNEL_CLASSES] = {0};
   ~Plan() {
+    for (auto& r : pending) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : ev_free) (void)hipEventDestroy(e);
+    if (ev_open) (void)hipEventDestroy(ev_open);
     for (void* p : allocs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -240,8 +257,37 @@ int next_pow2_cap32(int n) {
   return v;
 }
 
+int take_event(Plan& pl, hipEvent_t* e) {
+  if (!pl.ev_free.empty()) {
+    *e = pl.ev_free.back();
+    pl.ev_free.pop_back();
+    return PR_OK;
+  }
+  HIPCHK(hipEventCreate(e));
+  return PR_OK;
+}
+
+// event pair around one kernel launch of class cls (no-op unless timing is on)
+int tbegin(Plan& pl, hipStream_t st) {
+  if (!pl.timing) return PR_OK;
+  int rc = take_event(pl, &pl.ev_open);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(pl.ev_open, st));
+  return PR_OK;
+}
+int tend(Plan& pl, int cls, hipStream_t st) {
+  if (!pl.timing) return PR_OK;
+  hipEvent_t b;
+  int rc = take_event(pl, &b);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(b, st));
+  pl.pending.push_back({cls, pl.ev_open, b});
+  pl.ev_open = nullptr;
+  return PR_OK;
+}
+
 template <int B>
-int launch_gradient(const Plan& pl, const Plan::Group& g, hipStream_t st) {
+int launch_gradient(Plan& pl, const Plan::Group& g, hipStream_t st) {
   const size_t sm = pr::Smem<B>::bytes;
   const dim3 blk(64 * B * B);
   switch (g.V) {
@@ -259,18 +305,27 @@ int launch_gradient(const Plan& pl, const Plan::Group& g, hipStream_t st) {
 }
 
 template <int B>
-int launch_group(const Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hipStream_t st) {
+int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hipStream_t st) {
   const size_t sm = pr::Smem<B>::bytes;
   const dim3 blk(64 * B * B);
+  int rc;
+  if ((rc = tbegin(pl, st))) return rc;
   hipLaunchKernelGGL((pr::k_segment<B>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u);
   HIPCHK(hipGetLastError());
+  if ((rc = tend(pl, 0, st))) return rc;
+  if ((rc = tbegin(pl, st))) return rc;
   hipLaunchKernelGGL((pr::k_combine<B>), dim3(g.n_members), blk, sm, st, pl.d_probs, g.d_members);
   HIPCHK(hipGetLastError());
-  if (grad) return launch_gradient<B>(pl, g, st);
+  if ((rc = tend(pl, 1, st))) return rc;
+  if (grad) {
+    if ((rc = tbegin(pl, st))) return rc;
+    if ((rc = launch_gradient<B>(pl, g, st))) return rc;
+    if ((rc = tend(pl, 2, st))) return rc;
+  }
   return PR_OK;
 }
 
-int run(const Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
+int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
   for (const auto& g : pl.groups) {
     int rc = PR_OK;
     switch (g.B) {
@@ -282,6 +337,8 @@ int run(const Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t
     }
     if (rc) return rc;
   }
+  int rc;
+  if ((rc = tbegin(pl, st))) return rc;
   hipLaunchKernelGGL(pr::k_reduce_F, dim3((pl.n_probs + 127) / 128), dim3(128), 0, st, pl.d_probs, pl.n_probs, d_F);
   HIPCHK(hipGetLastError());
   if (d_G) {
@@ -290,6 +347,7 @@ int run(const Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t
                        pl.d_off, total, d_G);
     HIPCHK(hipGetLastError());
   }
+  if ((rc = tend(pl, 3, st))) return rc;
   return PR_OK;
 }
 
@@ -541,7 +599,64 @@ int pr_eval_device(void* plan, const double* d_controls, double* d_F, double* d_
   if (!plan || !d_controls || !d_F) return fail(PR_ERR_ARG, "pr_eval_device: plan, d_controls and d_F are required");
   Plan& pl = *static_cast<Plan*>(plan);
   if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
-  return run(pl, d_controls, d_F, d_G, static_cast<hipStream_t>(stream));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : pl.stream;
+  return run(pl, d_controls, d_F, d_G, st);
+}
+
+int pr_plan_buffers(void* plan, double** d_controls, double** d_F, double** d_G) {
+  if (!plan || !d_controls || !d_F || !d_G) return fail(PR_ERR_ARG, "pr_plan_buffers: NULL argument");
+  Plan& pl = *static_cast<Plan*>(plan);
+  *d_controls = pl.d_controls;
+  *d_F = pl.d_F;
+  *d_G = pl.d_G;
+  return PR_OK;
+}
+
+int pr_plan_upload_controls(void* plan, const double* const* controls) {
+  if (!plan || !controls) return fail(PR_ERR_ARG, "pr_plan_upload_controls: NULL argument");
+  Plan& pl = *static_cast<Plan*>(plan);
+  if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
+  for (int p = 0; p < pl.n_probs; p++) {
+    if (!controls[p]) return fail(PR_ERR_ARG, "pr_plan_upload_controls: controls[%d] is NULL", p);
+    HIPCHK(hipMemcpyAsync(pl.d_controls + pl.coff[p], controls[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]),
+                          hipMemcpyHostToDevice, pl.stream));
+  }
+  HIPCHK(hipStreamSynchronize(pl.stream));
+  return PR_OK;
+}
+
+int pr_plan_synchronize(void* plan) {
+  if (!plan) return fail(PR_ERR_ARG, "pr_plan_synchronize: plan is NULL");
+  HIPCHK(hipStreamSynchronize(static_cast<Plan*>(plan)->stream));
+  return PR_OK;
+}
+
+int pr_plan_set_timing(void* plan, int enable) {
+  if (!plan) return fail(PR_ERR_ARG, "pr_plan_set_timing: plan is NULL");
+  static_cast<Plan*>(plan)->timing = enable != 0;
+  return PR_OK;
+}
+
+int pr_plan_kernel_times(void* plan, double* ms, long long* launches) {
+  if (!plan || !ms || !launches) return fail(PR_ERR_ARG, "pr_plan_kernel_times: NULL argument");
+  Plan& pl = *static_cast<Plan*>(plan);
+  for (auto& r : pl.pending) {
+    HIPCHK(hipEventSynchronize(r.b));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+    pl.acc_ms[r.cls] += t;
+    pl.acc_n[r.cls] += 1;
+    pl.ev_free.push_back(r.a);
+    pl.ev_free.push_back(r.b);
+  }
+  pl.pending.clear();
+  for (int c = 0; c < PR_KERNEL_CLASSES; c++) {
+    ms[c] = pl.acc_ms[c];
+    launches[c] = pl.acc_n[c];
+    pl.acc_ms[c] = 0;
+    pl.acc_n[c] = 0;
+  }
+  return PR_OK;
 }
 
 long long pr_plan_n_params(void* plan) {

@@ -1,0 +1,164 @@
+"""Batched MR kernels on the :rocm device.
+
+  MRControl / grape_state_kernel          src/Application/MR/OptControl.jl:137-259,
+                                          src/Application/MR/GRAPEState.jl:73-250
+  LindbladMRControl / grape_lindblad_kernel
+                                          OptControl.jl:733-885, GRAPELindblad.jl:83-215
+  build_drift_liouvillian, build_control_liouvillian, pure_state_to_vec_rho
+                                          src/Physics/Lindblad.jl:89-197
+
+Ensemble members are the (drift, power level) pairs in the reference's order
+(`for H in drifts for p in pwr_levels`, GRAPEState.jl:146); every member
+propagates every (rho_init[s], rho_targ[s]) pair; F and G are the mean over
+N_ens = n_drifts * n_pwr * n_pairs (GRAPEState.jl:239-244). Penalties other
+than :none are not part of this path.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import native
+from .problem import PR_CLOSED, PR_GRAD_REFERENCE, PR_LINDBLAD, PR_PAIRS_REAL, PR_PAIRS_SQUARE, Problem
+
+
+def build_drift_liouvillian(H_drift, jump_ops=(), decay_rates=()):
+    """-i(I (x) H - H^T (x) I) + sum_k g_k (conj(L) (x) L - 1/2 I (x) L^dag L - 1/2 conj(L^dag L) (x) I)
+    on column-stacked vec(rho) (Lindblad.jl:126-157)."""
+    H = np.asarray(H_drift, dtype=np.complex128)
+    N = H.shape[0]
+    I = np.eye(N)
+    L = -1j * np.kron(I, H) + 1j * np.kron(H.T, I)
+    for Lk, g in zip(jump_ops, decay_rates):
+        if g == 0.0 or not np.isfinite(g):
+            continue
+        if g < 0:
+            raise ValueError(f"Lindblad decay rate must be non-negative; got {g}")
+        Lk = np.asarray(Lk, dtype=np.complex128)
+        A = Lk.conj().T @ Lk
+        L = L + g * (np.kron(Lk.conj(), Lk) - 0.5 * np.kron(I, A) - 0.5 * np.kron(A.conj(), I))
+    return L
+
+
+def build_control_liouvillian(H_ctrl):
+    """-i(I (x) H - H^T (x) I) (Lindblad.jl:189-197)."""
+    H = np.asarray(H_ctrl, dtype=np.complex128)
+    I = np.eye(H.shape[0])
+    return -1j * np.kron(I, H) + 1j * np.kron(H.T, I)
+
+
+def vec_rho(rho):
+    return np.asarray(rho, dtype=np.complex128).reshape(-1, order="F")
+
+
+def pure_state_to_vec_rho(psi):
+    c = np.asarray(psi, dtype=np.complex128).reshape(-1)
+    return vec_rho(np.outer(c, c.conj()))
+
+
+def _fid_code(fidelity: str) -> int:
+    f = fidelity.lstrip(":")
+    if f == "square":
+        return PR_PAIRS_SQUARE
+    if f == "real":
+        return PR_PAIRS_REAL
+    raise ValueError(f"fidelity :{f} is not supported on the :rocm device (use :square or :real)")
+
+
+@dataclass
+class MRControl:
+    """The fields of MRControl (OptControl.jl:137-259) that the state kernel reads."""
+
+    drifts: List[np.ndarray]
+    operators: List[np.ndarray]
+    rho_init: List[np.ndarray]
+    rho_targ: List[np.ndarray]
+    pulse_dt: np.ndarray
+    pwr_levels: List[float] = field(default_factory=lambda: [1.0])
+    fidelity: str = "square"
+    backend: str = "rocm"
+    precision: str = "f64"
+
+    def problem(self) -> Problem:
+        drifts = [np.asarray(h, np.complex128) for h in self.drifts]
+        ops = np.stack([np.asarray(o, np.complex128) for o in self.operators])
+        M = len(drifts) * len(self.pwr_levels)
+        H0 = np.stack([h for h in drifts for _ in self.pwr_levels])
+        scale = np.array([[p] * len(ops) for _ in drifts for p in self.pwr_levels], dtype=np.float64)
+        psi0 = np.stack([np.asarray(r, np.complex128).reshape(-1) for r in self.rho_init])
+        psit = np.stack([np.asarray(r, np.complex128).reshape(-1) for r in self.rho_targ])
+        dts = np.asarray(self.pulse_dt, dtype=np.float64)
+        return Problem(dim=ops.shape[1], n_ctrl=len(ops), n_t=len(dts), H0=H0, Hc=ops, dt_slices=dts,
+                       dynamics=PR_CLOSED, objective=_fid_code(self.fidelity), n_members=M, ctrl_scale=scale,
+                       psi_init=psi0, psi_targ=psit)
+
+
+@dataclass
+class LindbladMRControl:
+    """The fields of LindbladMRControl (OptControl.jl:733-885) read by the Lindblad kernel;
+    the Liouvillians are built as the reference precomputes _L_drifts/_L_controls."""
+
+    drifts: List[np.ndarray]
+    operators: List[np.ndarray]
+    rho_init: List[np.ndarray]
+    rho_targ: List[np.ndarray]
+    pulse_dt: np.ndarray
+    jump_ops: List[np.ndarray] = field(default_factory=list)
+    decay_rates: List[float] = field(default_factory=list)
+    pwr_levels: List[float] = field(default_factory=lambda: [1.0])
+    fidelity: str = "square"
+    backend: str = "rocm"
+
+    def problem(self) -> Problem:
+        Ld = [build_drift_liouvillian(h, self.jump_ops, self.decay_rates) for h in self.drifts]
+        Lc = np.stack([build_control_liouvillian(o) for o in self.operators])
+
+        def as_vec(r):
+            r = np.asarray(r, np.complex128)
+            return vec_rho(r) if r.ndim == 2 else pure_state_to_vec_rho(r) if r.size ** 2 == Lc.shape[1] else r
+
+        s0 = np.stack([as_vec(r) for r in self.rho_init])
+        st = np.stack([as_vec(r) for r in self.rho_targ])
+        M = len(Ld) * len(self.pwr_levels)
+        L0 = np.stack([L for L in Ld for _ in self.pwr_levels])
+        scale = np.array([[p] * len(Lc) for _ in Ld for p in self.pwr_levels], dtype=np.float64)
+        dts = np.asarray(self.pulse_dt, dtype=np.float64)
+        return Problem(dim=Lc.shape[1], n_ctrl=len(Lc), n_t=len(dts), H0=L0, Hc=Lc, dt_slices=dts,
+                       dynamics=PR_LINDBLAD, objective=_fid_code(self.fidelity), n_members=M, ctrl_scale=scale,
+                       psi_init=s0, psi_targ=st)
+
+
+class KernelPlan:
+    """A reusable device plan for repeated kernel calls with one control context."""
+
+    def __init__(self, ctrl, grad_mode: int = PR_GRAD_REFERENCE):
+        self.problem = ctrl.problem()
+        self.plan = native.Plan([self.problem], grad_mode=grad_mode)
+
+    def __call__(self, waveform: np.ndarray):
+        w = np.asarray(waveform, dtype=np.float64)
+        F, G = self.plan.eval([w], grad=True)
+        return float(F[0]), G[0]
+
+    def close(self):
+        self.plan.close()
+
+
+def grape_state_kernel(waveform: np.ndarray, ctrl: MRControl):
+    """(fidelity, grad[n_ctrl x n_t]) of GRAPEState.jl:73-85 on the device."""
+    kp = KernelPlan(ctrl)
+    try:
+        return kp(waveform)
+    finally:
+        kp.close()
+
+
+def grape_lindblad_kernel(waveform: np.ndarray, ctrl: LindbladMRControl):
+    """(fidelity, grad[n_ctrl x n_t]) of GRAPELindblad.jl:83-92 on the device."""
+    kp = KernelPlan(ctrl)
+    try:
+        return kp(waveform)
+    finally:
+        kp.close()

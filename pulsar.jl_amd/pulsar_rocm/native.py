@@ -27,7 +27,10 @@ PR_ERR = {
 }
 
 EXPORTS = ("pr_abi_version", "pr_device_count", "pr_init", "pr_last_error", "pr_plan_create",
-           "pr_eval", "pr_eval_device", "pr_plan_n_params", "pr_plan_destroy")
+           "pr_eval", "pr_eval_device", "pr_plan_n_params", "pr_plan_set_timing",
+           "pr_plan_kernel_times", "pr_plan_buffers", "pr_plan_upload_controls",
+           "pr_plan_synchronize", "pr_plan_destroy")
+KERNEL_CLASSES = ("segment", "combine", "gradient", "reduce")
 
 
 class RocmError(RuntimeError):
@@ -59,6 +62,11 @@ def load(path: str = LIB_PATH):
                                  ctypes.c_void_p]
     L.pr_plan_n_params.argtypes = [ctypes.c_void_p]
     L.pr_plan_n_params.restype = ctypes.c_longlong
+    L.pr_plan_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.pr_plan_kernel_times.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(ctypes.c_longlong)]
+    L.pr_plan_buffers.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_void_p)] * 3
+    L.pr_plan_upload_controls.argtypes = [ctypes.c_void_p, ctypes.POINTER(dp)]
+    L.pr_plan_synchronize.argtypes = [ctypes.c_void_p]
     L.pr_plan_destroy.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
@@ -137,6 +145,33 @@ class Plan:
         check(load().pr_eval_device(self._h, ctypes.c_void_p(d_controls), ctypes.c_void_p(d_F),
                                     ctypes.c_void_p(d_G) if d_G else None,
                                     ctypes.c_void_p(stream) if stream else None))
+
+    def buffers(self):
+        """(d_controls, d_F, d_G) device addresses of the plan-owned buffers."""
+        a, b, c = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(load().pr_plan_buffers(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def upload_controls(self, controls: Sequence[np.ndarray]) -> None:
+        bufs = [controls_cm(np.asarray(u, np.float64)) for u in controls]
+        for p, u in zip(self.problems, controls):
+            if np.asarray(u).shape != (p.n_ctrl, p.n_t):
+                raise ValueError(f"controls shape {np.asarray(u).shape} != ({p.n_ctrl}, {p.n_t})")
+        cptrs = (ctypes.POINTER(ctypes.c_double) * len(bufs))(*[_dptr(b) for b in bufs])
+        check(load().pr_plan_upload_controls(self._h, cptrs))
+
+    def synchronize(self) -> None:
+        check(load().pr_plan_synchronize(self._h))
+
+    def set_timing(self, enable: bool) -> None:
+        check(load().pr_plan_set_timing(self._h, 1 if enable else 0))
+
+    def kernel_times(self) -> dict:
+        """{class: (total_ms, launches)} since the previous call (HIP events)."""
+        ms = np.zeros(len(KERNEL_CLASSES))
+        n = (ctypes.c_longlong * len(KERNEL_CLASSES))()
+        check(load().pr_plan_kernel_times(self._h, _dptr(ms), n))
+        return {c: (float(ms[i]), int(n[i])) for i, c in enumerate(KERNEL_CLASSES)}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -1,0 +1,100 @@
+"""The C-ABI library loads, exports every declared entry point and reports
+errors correctly, without a GPU (no compute calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from pulsar_rocm import native, workloads
+from pulsar_rocm.problem import COptions, to_c
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pulsar_rocm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pr_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declarations_match_binding():
+    decl = declared_functions()
+    assert decl, "no pr_* declarations found"
+    assert sorted(native.EXPORTS) == decl
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(native.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_abi_version_and_no_device_here():
+    L = native.load()
+    assert L.pr_abi_version() == 1
+    if not os.path.exists("/dev/kfd"):
+        assert native.device_count() == 0
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-GPU error path")
+def test_plan_without_device_fails_cleanly():
+    p, u = workloads.four_qubit_ensemble(n_t=4, n_det=1, n_amp=2)
+    with pytest.raises(native.RocmError) as e:
+        native.Plan([p])
+    assert e.value.code == -2  # PR_ERR_DEVICE
+
+
+def test_validation_runs_before_device_use():
+    # a non-Hermitian drift is rejected as PR_ERR_ARG (QuantumSystem.jl Hermiticity check)
+    p, u = workloads.four_qubit_ensemble(n_t=4, n_det=1, n_amp=2)
+    H0 = np.array(p.H0)
+    H0[0, 0, 1] += 1e-6
+    p.H0 = H0
+    with pytest.raises(native.RocmError) as e:
+        native.Plan([p])
+    assert e.value.code == -1 and "Hermitian" in str(e.value)
+
+
+def test_non_unitary_target_rejected():
+    p, u = workloads.four_qubit_ensemble(n_t=4, n_det=1, n_amp=2)
+    p.U_target = 1.01 * np.asarray(p.U_target)
+    with pytest.raises(native.RocmError) as e:
+        native.Plan([p])
+    assert e.value.code == -1 and "unitary" in str(e.value)
+
+
+def test_injected_plan_failure(monkeypatch):
+    monkeypatch.setenv("PULSAR_ROCM_FAULT", "plan")
+    p, u = workloads.four_qubit_ensemble(n_t=4, n_det=1, n_amp=2)
+    with pytest.raises(native.RocmError) as e:
+        native.Plan([p])
+    assert e.value.code == -6
+
+
+def test_injected_eval_failure(monkeypatch):
+    monkeypatch.setenv("PULSAR_ROCM_FAULT", "eval")
+    L = native.load()
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = L.pr_eval(ctypes.c_void_p(1), None, None, None, None)
+    assert rc == -6
+    assert "PULSAR_ROCM_FAULT" in native.last_error()
+
+
+def test_null_arguments_rejected():
+    L = native.load()
+    h = ctypes.c_void_p()
+    assert L.pr_plan_create(None, 1, None, ctypes.byref(h)) == -1
+    assert L.pr_eval(None, None, None, None, None) == -1
+    assert L.pr_plan_destroy(None) == 0
+    assert L.pr_plan_n_params(None) == -1
+
+
+def test_oversized_dim_unsupported():
+    rng = np.random.default_rng(0)
+    p, u = workloads.spin_chain_128(n_t=2, n_members=1)
+    with pytest.raises(native.RocmError) as e:
+        native.Plan([p])
+    assert e.value.code == -5

@@ -1,0 +1,200 @@
+"""Device (libpulsar_rocm.so, gfx950) vs the FP64 oracle on identical inputs.
+
+FP64 device path tolerances: |dF| <= 1e-12 and max|dG| / max|G| <= 1e-10
+(the device uses a degree-9 Taylor exp and a unitarity-based adjoint chain,
+the oracle LAPACK-style eigen-exps and stored P/Q chains; both are exact to
+rounding, ~1e-13 relative after 2000 slices).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from pulsar_rocm import native, workloads as W
+from pulsar_rocm.problem import (PR_GATE, PR_LINDBLAD, PR_PAIRS_REAL, PR_PAIRS_SQUARE, Problem)
+
+pytestmark = pytest.mark.gpu
+
+TOL_F = 1e-12
+TOL_G = 1e-10
+FIX = np.load(os.path.join(os.path.dirname(__file__), "golden", "fixtures.npz"))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    native.init(0)
+
+
+def run(p, u, **kw):
+    plan = native.Plan([p], **kw)
+    try:
+        F, G, D = plan.eval([u], diag=True)
+    finally:
+        plan.close()
+    return F[0], G[0], D[0]
+
+
+def check(p, u, tol_f=TOL_F, tol_g=TOL_G, **kw):
+    print(f"case d={p.dim} M={p.n_members} n_t={p.n_t} obj={p.objective} dyn={p.dynamics} {kw}", flush=True)
+    F0, G0 = O.evaluate(p, u)
+    F, G, D = run(p, u, **kw)
+    dF = abs(F - F0)
+    rel = np.abs(G - G0).max() / max(np.abs(G0).max(), 1e-300)
+    assert dF <= tol_f, (F, F0)
+    assert rel <= tol_g, rel
+    return F, G, D
+
+
+def test_four_qubit_ensemble_small():
+    p, u = W.four_qubit_ensemble(n_t=64)
+    p, u = W.shrink(p, u, n_members=4)
+    F, G, D = check(p, u)
+    assert D[0] < 1e-12  # unitarity defect of U_total
+
+
+@pytest.mark.parametrize("seg", [1, 7, 64])
+def test_segment_length_does_not_change_results(seg):
+    p, u = W.four_qubit_ensemble(n_t=64)
+    p, u = W.shrink(p, u, n_members=3)
+    check(p, u, segment=seg)
+
+
+def test_four_qubit_ensemble_full_length_member_sample():
+    # all 2000 slices, 16 of the 256 members
+    p, u = W.four_qubit_ensemble()
+    p, u = W.shrink(p, u, n_members=16)
+    check(p, u)
+
+
+@pytest.mark.parametrize("d", [2, 3, 4, 8, 16, 20, 32, 48, 64])
+def test_random_gate_problems(d):
+    rng = np.random.default_rng(100 + d)
+    H0 = W.random_hermitian(d, rng, 3.0)
+    Hc = np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(3)])
+    u = rng.uniform(-1, 1, (3, 37))
+    p = Problem(dim=d, n_ctrl=3, n_t=37, H0=H0, Hc=Hc, dt=0.05, objective=PR_GATE, n_members=2,
+                ctrl_scale=rng.uniform(0.9, 1.1, (2, 3)), U_target=W.haar_unitary(d, rng))
+    check(p, u)
+
+
+def test_large_norm_uses_scaling_and_squaring():
+    rng = np.random.default_rng(7)
+    d = 8
+    p = Problem(dim=d, n_ctrl=2, n_t=12, H0=W.random_hermitian(d, rng, 40.0),
+                Hc=np.stack([W.random_hermitian(d, rng, 10.0) for _ in range(2)]), dt=0.1,
+                objective=PR_GATE, U_target=W.haar_unitary(d, rng))
+    u = rng.uniform(-3, 3, (2, 12))
+    F, G, D = check(p, u)
+    assert D[1] >= 3  # several squarings were needed
+
+
+def test_gate_fixture_on_device():
+    p = Problem(dim=4, n_ctrl=3, n_t=5, H0=FIX["gate_H0"], Hc=FIX["gate_Hc"], dt=float(FIX["gate_dt"]),
+                objective=PR_GATE, n_members=2, ctrl_scale=FIX["gate_scale"], U_target=FIX["gate_Ut"])
+    F, G, _ = run(p, FIX["gate_u"])
+    assert abs(F - float(FIX["gate_F"])) < 1e-13
+    assert np.abs(G - FIX["gate_G"]).max() / np.abs(FIX["gate_G"]).max() < 1e-11
+
+
+def test_pairs_fixture_on_device():
+    p = Problem(dim=2, n_ctrl=2, n_t=4, H0=FIX["pair_H0"], Hc=FIX["pair_Hc"], dt_slices=FIX["pair_dts"],
+                objective=PR_PAIRS_SQUARE, n_members=1, ctrl_scale=np.full((1, 2), float(FIX["pair_pwr"])),
+                psi_init=FIX["pair_psi0"], psi_targ=FIX["pair_psit"], weight=np.ones(1))
+    F, G, _ = run(p, FIX["pair_u"])
+    assert abs(F - float(FIX["pair_F"])) < 1e-13
+    assert np.abs(G - FIX["pair_G"]).max() < 1e-12
+
+
+def test_lindblad_fixture_on_device():
+    p = Problem(dim=4, n_ctrl=2, n_t=4, H0=FIX["lind_L0"], Hc=FIX["lind_Lc"], dt_slices=FIX["lind_dts"],
+                dynamics=PR_LINDBLAD, objective=PR_PAIRS_SQUARE, n_members=1,
+                ctrl_scale=np.full((1, 2), float(FIX["lind_pwr"])), psi_init=FIX["lind_s0"],
+                psi_targ=FIX["lind_st"], weight=np.ones(1))
+    F, G, _ = run(p, FIX["lind_u"])
+    assert abs(F - float(FIX["lind_F"])) < 1e-13
+    assert np.abs(G - FIX["lind_G"]).max() < 1e-12
+
+
+@pytest.mark.parametrize("psi_init", [None, [1, 0]])
+def test_single_qubit_state(psi_init):
+    p, u = W.single_qubit_state(psi_init=psi_init)
+    check(p, u)
+
+
+@pytest.mark.parametrize("obj", [PR_PAIRS_SQUARE, PR_PAIRS_REAL])
+def test_state_pairs_per_slice_dt(obj):
+    rng = np.random.default_rng(11)
+    d, npairs = 6, 3
+    p = Problem(dim=d, n_ctrl=2, n_t=25, H0=np.stack([W.random_hermitian(d, rng, 2.0) for _ in range(2)]),
+                Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)]), dt_slices=rng.uniform(0.01, 0.1, 25),
+                objective=obj, n_members=2, ctrl_scale=np.array([[1.0, 1.0], [1.1, 1.1]]),
+                psi_init=rng.standard_normal((npairs, d)) + 1j * rng.standard_normal((npairs, d)),
+                psi_targ=rng.standard_normal((npairs, d)) + 1j * rng.standard_normal((npairs, d)))
+    check(p, rng.uniform(-1, 1, (2, 25)))
+
+
+@pytest.mark.parametrize("obj", [PR_PAIRS_REAL, PR_PAIRS_SQUARE])
+def test_three_qubit_lindblad_small(obj):
+    p, u = W.three_qubit_lindblad(n_t=40, n_members=2, objective=obj)
+    check(p, u)
+
+
+def test_three_qubit_lindblad_full_length_one_member():
+    p, u = W.three_qubit_lindblad(n_members=1)
+    check(p, u)
+
+
+def test_uneven_batch_in_one_plan():
+    batch = W.uneven_batch(n_problems=24, seed=21, n_min=20, n_max=60)
+    probs = [b[0] for b in batch]
+    us = [b[1] for b in batch]
+    plan = native.Plan(probs)
+    try:
+        F, G = plan.eval(us)
+    finally:
+        plan.close()
+    for i, (p, u) in enumerate(batch):
+        F0, G0 = O.evaluate(p, u)
+        assert abs(F[i] - F0) <= TOL_F, (i, p.dim)
+        assert np.abs(G[i] - G0).max() / np.abs(G0).max() <= TOL_G, (i, p.dim)
+
+
+def test_edge_cases_single_slice_zero_dt_zero_controls():
+    rng = np.random.default_rng(3)
+    d = 5
+    base = dict(dim=d, n_ctrl=2, H0=W.random_hermitian(d, rng, 1.0),
+                Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)]), objective=PR_GATE,
+                U_target=W.haar_unitary(d, rng))
+    check(Problem(n_t=1, dt=0.3, **base), rng.uniform(-1, 1, (2, 1)))
+    check(Problem(n_t=9, dt=0.2, **base), np.zeros((2, 9)))
+    # dt = 0: every propagator is the identity
+    F, G, _ = run(Problem(n_t=3, dt=0.0, **base), rng.uniform(-1, 1, (2, 3)))
+    assert abs(F - abs(np.trace(base["U_target"]) / d) ** 2) < 1e-14
+    assert np.all(G == 0)
+
+
+def test_device_resident_path_matches_host_path():
+    p, u = W.four_qubit_ensemble(n_t=50)
+    p, u = W.shrink(p, u, n_members=5)
+    plan = native.Plan([p])
+    try:
+        F, G = plan.eval([u])
+        plan.upload_controls([u])
+        dc, dF, dG = plan.buffers()
+        plan.eval_device(dc, dF, dG)
+        plan.synchronize()
+        F2, G2 = plan.eval([u])
+        assert F2[0] == F[0] and np.array_equal(G2[0], G[0])  # deterministic
+    finally:
+        plan.close()
+
+
+def test_ensemble_weights_and_linearity():
+    p, u = W.four_qubit_ensemble(n_t=30)
+    p, u = W.shrink(p, u, n_members=6)
+    F, G, _ = run(p, u)
+    p2, _ = W.shrink(p, u)
+    p2.weight = np.full(6, 2.0 / 6)
+    F2, G2, _ = run(p2, u)
+    assert abs(F2 - 2 * F) < 1e-15 and np.abs(G2 - 2 * G).max() <= 1e-15 * np.abs(G).max() * 4
