@@ -15,7 +15,9 @@
 //                                 (unitary) or backward boundaries (Lindblad).
 //   k_gradient (member, segment)  walks the segment again and contracts
 //                                 M_k = P_k Lambda_k^dag with every control:
-//                                 G_m[j,k] = Re(c dt_k s_mj Tr(A_j M_k)).
+//                                 G_m[j,k] = Re(c dt_k s_mj Tr(A_j M_k)),
+//                                 or Tr(A_j L(X_k, M_k)) with the Frechet
+//                                 derivative of exp for the exact gradient.
 //                                 Unitary dynamics use M_k = C_k U_k^dag with
 //                                 C_{k+1} = U_k M_k, C = E_s Omega E_s^dag
 //                                 (no stored forward chain); Lindblad dynamics
@@ -170,6 +172,68 @@ __device__ inline Blk expm_blk(const Blk& X, int s, double2* LA, double2* LB, co
   add_diag(U, 1.0, x);
   for (int i = 0; i < s; i++) U = mm<B, N, N>(U, U, LA, LB, x);
   return U;
+}
+
+// Frechet derivative L(X 2^s, M) of exp for X already scaled by 2^-s: the
+// degree-9 Paterson-Stockmeyer polynomial evaluated on the block matrix
+// [[X, N], [0, X]] with N = M 2^-s (each block product [[A,B],[0,A]][[C,D],[0,C]]
+// = [[AC, AD + BC], [0, AC]]), then s squarings E <- E E, L <- E L + L E.
+template <int B>
+__device__ inline Blk frechet_blk(const Blk& X, int s, const Blk& M, double2* LA, double2* LB, const Ln& x) {
+  const double c2 = 1.0 / 2, c3 = 1.0 / 6, c4 = 1.0 / 24, c5 = 1.0 / 120, c6 = 1.0 / 720,
+               c7 = 1.0 / 5040, c8 = 1.0 / 40320, c9 = 1.0 / 362880;
+  const double sc = ldexp(1.0, -s);
+  Blk Nm;
+  Nm.re = sc * M.re;
+  Nm.im = sc * M.im;
+  Blk X2 = mm<B, N, N>(X, X, LA, LB, x);
+  Blk D2 = mm<B, N, N>(X, Nm, LA, LB, x);
+  {
+    Blk t = mm<B, N, N>(Nm, X, LA, LB, x);
+    D2.re += t.re;
+    D2.im += t.im;
+  }
+  Blk X3 = mm<B, N, N>(X, X2, LA, LB, x);
+  Blk D3 = mm<B, N, N>(X, D2, LA, LB, x);
+  {
+    Blk t = mm<B, N, N>(Nm, X2, LA, LB, x);
+    D3.re += t.re;
+    D3.im += t.im;
+  }
+  Blk Tm, Td;
+  Tm.re = c9 * X3.re + c8 * X2.re + c7 * X.re;
+  Tm.im = c9 * X3.im + c8 * X2.im + c7 * X.im;
+  add_diag(Tm, c6, x);
+  Td.re = c9 * D3.re + c8 * D2.re + c7 * Nm.re;
+  Td.im = c9 * D3.im + c8 * D2.im + c7 * Nm.im;
+  Blk R = mm<B, N, N>(X3, Tm, LA, LB, x);
+  Blk Rd = mm<B, N, N>(X3, Td, LA, LB, x);
+  {
+    Blk t = mm<B, N, N>(D3, Tm, LA, LB, x);
+    Rd.re += t.re + c5 * D2.re + c4 * Nm.re;
+    Rd.im += t.im + c5 * D2.im + c4 * Nm.im;
+  }
+  R.re += c5 * X2.re + c4 * X.re;
+  R.im += c5 * X2.im + c4 * X.im;
+  add_diag(R, c3, x);
+  Blk E = mm<B, N, N>(X3, R, LA, LB, x);
+  Blk L = mm<B, N, N>(X3, Rd, LA, LB, x);
+  {
+    Blk t = mm<B, N, N>(D3, R, LA, LB, x);
+    L.re += t.re + c2 * D2.re + Nm.re;
+    L.im += t.im + c2 * D2.im + Nm.im;
+  }
+  E.re += c2 * X2.re + X.re;
+  E.im += c2 * X2.im + X.im;
+  add_diag(E, 1.0, x);
+  for (int i = 0; i < s; i++) {
+    Blk a = mm<B, N, N>(E, L, LA, LB, x);
+    Blk b = mm<B, N, N>(L, E, LA, LB, x);
+    L.re = a.re + b.re;
+    L.im = a.im + b.im;
+    E = mm<B, N, N>(E, E, LA, LB, x);
+  }
+  return L;
 }
 
 // ---------------------------------------------------------------- k_segment
@@ -368,9 +432,10 @@ __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, d
 }
 
 // ---------------------------------------------------------------- k_gradient
-template <int B, int V>
+template <int B, int V, bool EXACT>
 __global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restrict__ probs,
-                                                          const WorkItem* __restrict__ items) {
+                                                          const WorkItem* __restrict__ items,
+                                                          const double* __restrict__ u) {
   const WorkItem it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
   const Ln x = lane_info<B>();
@@ -389,7 +454,13 @@ __global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restri
       Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
       Blk Ud = conj_blk(Ut);                      // U^dag, own block
       Blk Mk = mm<B, N, N>(Cm, Ud, LA, LB, x);    // C_k U_k^dag
-      contract<B, V>(P, m, k, Mk, red, x);
+      if constexpr (EXACT) {
+        int s;
+        Blk X = assemble<B>(P, m, k, u, s, x);
+        contract<B, V>(P, m, k, frechet_blk<B>(X, s, Mk, LA, LB, x), red, x);
+      } else {
+        contract<B, V>(P, m, k, Mk, red, x);
+      }
       Cm = mm<B, T, N>(Ut, Mk, LA, LB, x);        // U_k M_k
     }
     return;
@@ -408,7 +479,13 @@ __global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restri
   for (int k = it.k1 - 1; k >= it.k0; k--) {
     Blk Pc = load_blk(scr + (size_t)(k - it.k0) * MB, x);
     Blk Mk = mm<B, N, C>(Pc, Lm, LA, LB, x);      // P_k Lambda_k^dag
-    contract<B, V>(P, m, k, Mk, red, x);
+    if constexpr (EXACT) {
+      int s;
+      Blk X = assemble<B>(P, m, k, u, s, x);
+      contract<B, V>(P, m, k, frechet_blk<B>(X, s, Mk, LA, LB, x), red, x);
+    } else {
+      contract<B, V>(P, m, k, Mk, red, x);
+    }
     Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
     Lm = mm<B, J, N>(Ut, Lm, LA, LB, x);          // U_k^dag Lambda_k  (U^dag = conj(U^T))
   }

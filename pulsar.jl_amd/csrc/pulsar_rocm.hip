@@ -234,25 +234,29 @@ int validate(const pr_problem& p, int idx) {
   return PR_OK;
 }
 
+template <int B, int V>
+int set_grad_attr(size_t bytes) {
+  HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, V, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, V, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return PR_OK;
+}
+
 template <int B>
 int set_smem_attr() {
   size_t bytes = pr::Smem<B>::bytes;
   if (bytes > 65536) {
     HIPCHK(hipFuncSetAttribute((const void*)pr::k_segment<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     HIPCHK(hipFuncSetAttribute((const void*)pr::k_combine<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    int rc;
+    if ((rc = set_grad_attr<B, 2>(bytes)) || (rc = set_grad_attr<B, 4>(bytes)) || (rc = set_grad_attr<B, 8>(bytes)) ||
+        (rc = set_grad_attr<B, 16>(bytes)) || (rc = set_grad_attr<B, 32>(bytes)))
+      return rc;
   }
   return PR_OK;
 }
 
-
 int next_pow2_cap32(int n) {
-  int v = 1;
+  int v = 2;
   while (v < n && v < 32) v <<= 1;
   return v;
 }
@@ -287,15 +291,18 @@ int tend(Plan& pl, int cls, hipStream_t st) {
 }
 
 template <int B>
-int launch_gradient(Plan& pl, const Plan::Group& g, hipStream_t st) {
+int launch_gradient(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t st) {
   const size_t sm = pr::Smem<B>::bytes;
   const dim3 blk(64 * B * B);
   switch (g.V) {
 #define PR_GCASE(VV)                                                                                        \
   case VV:                                                                                                  \
-    hipLaunchKernelGGL((pr::k_gradient<B, VV>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items); \
+    if (pl.opt.grad_mode == PR_GRAD_EXACT)                                                                  \
+      hipLaunchKernelGGL((pr::k_gradient<B, VV, true>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); \
+    else                                                                                                    \
+      hipLaunchKernelGGL((pr::k_gradient<B, VV, false>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); \
     break;
-    PR_GCASE(1) PR_GCASE(2) PR_GCASE(4) PR_GCASE(8) PR_GCASE(16) PR_GCASE(32)
+    PR_GCASE(2) PR_GCASE(4) PR_GCASE(8) PR_GCASE(16) PR_GCASE(32)
 #undef PR_GCASE
     default:
       return fail(PR_ERR_LAUNCH, "internal: bad reduction width %d", g.V);
@@ -319,7 +326,7 @@ int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
   if ((rc = tend(pl, 1, st))) return rc;
   if (grad) {
     if ((rc = tbegin(pl, st))) return rc;
-    if ((rc = launch_gradient<B>(pl, g, st))) return rc;
+    if ((rc = launch_gradient<B>(pl, g, d_u, st))) return rc;
     if ((rc = tend(pl, 2, st))) return rc;
   }
   return PR_OK;
@@ -547,7 +554,8 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
   pr_options o{PR_FP64, PR_GRAD_REFERENCE, 0, -1};
   if (opt) o = *opt;
   if (o.precision != PR_FP64) return fail(PR_ERR_UNSUPPORTED, "precision %d is not available in this build", o.precision);
-  if (o.grad_mode != PR_GRAD_REFERENCE) return fail(PR_ERR_UNSUPPORTED, "grad_mode %d is not available in this build", o.grad_mode);
+  if (o.grad_mode != PR_GRAD_REFERENCE && o.grad_mode != PR_GRAD_EXACT)
+    return fail(PR_ERR_ARG, "unknown grad_mode %d", o.grad_mode);
   if (o.segment < 0) return fail(PR_ERR_ARG, "segment must be >= 0");
   for (int p = 0; p < n_probs; p++) {
     int rc = validate(probs[p], p);

@@ -198,3 +198,61 @@ def test_ensemble_weights_and_linearity():
     p2.weight = np.full(6, 2.0 / 6)
     F2, G2, _ = run(p2, u)
     assert abs(F2 - 2 * F) < 1e-15 and np.abs(G2 - 2 * G).max() <= 1e-15 * np.abs(G).max() * 4
+
+
+@pytest.mark.parametrize("case", ["gate4", "gate16", "gate32", "pairs", "lindblad"])
+def test_exact_gradient_matches_oracle(case):
+    rng = np.random.default_rng(31)
+    if case.startswith("gate"):
+        d = int(case[4:])
+        p = Problem(dim=d, n_ctrl=2, n_t=9, H0=W.random_hermitian(d, rng, 3.0),
+                    Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)]), dt=0.08,
+                    objective=PR_GATE, n_members=2, ctrl_scale=rng.uniform(0.9, 1.1, (2, 2)),
+                    U_target=W.haar_unitary(d, rng))
+        u = rng.uniform(-2, 2, (2, 9))
+    elif case == "pairs":
+        d = 5
+        p = Problem(dim=d, n_ctrl=2, n_t=11, H0=W.random_hermitian(d, rng, 3.0),
+                    Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)]), dt=0.1,
+                    objective=PR_PAIRS_SQUARE, psi_init=rng.standard_normal((2, d)) + 0j,
+                    psi_targ=rng.standard_normal((2, d)) + 1j * rng.standard_normal((2, d)))
+        u = rng.uniform(-2, 2, (2, 11))
+    else:
+        p, u = W.three_qubit_lindblad(n_t=12, n_members=2, objective=PR_PAIRS_SQUARE)
+    F0, G0 = O.evaluate(p, u, grad_mode=1)
+    F, G, _ = run(p, u, grad_mode=1)
+    assert abs(F - F0) <= TOL_F
+    assert np.abs(G - G0).max() / np.abs(G0).max() <= TOL_G
+
+
+def test_exact_gradient_finite_differences_uneven_subset():
+    # 32 problems of the uneven batch, central differences (step 1e-6) on 8
+    # sampled controls per problem, all problems in one plan per evaluation
+    batch = W.uneven_batch(n_problems=32, seed=5)
+    probs = [b[0] for b in batch]
+    us = [b[1] for b in batch]
+    plan_x = native.Plan(probs, grad_mode=1)
+    plan_f = native.Plan(probs)
+    try:
+        _, Gx = plan_x.eval(us)
+        rng = np.random.default_rng(0)
+        picks = [[(int(rng.integers(p.n_ctrl)), int(rng.integers(p.n_t))) for _ in range(8)] for p in probs]
+        eps = 1e-6
+        fd = np.zeros((32, 8))
+        for t in range(8):
+            up = [u.copy() for u in us]
+            um = [u.copy() for u in us]
+            for i in range(32):
+                j, k = picks[i][t]
+                up[i][j, k] += eps
+                um[i][j, k] -= eps
+            Fp, _ = plan_f.eval(up, grad=False)
+            Fm, _ = plan_f.eval(um, grad=False)
+            fd[:, t] = (Fp - Fm) / (2 * eps)
+        for i in range(32):
+            gx = np.array([Gx[i][j, k] for j, k in picks[i]])
+            scale = np.abs(Gx[i]).max()
+            assert np.abs(gx - fd[i]).max() <= 1e-6 * scale + 1e-12, (i, probs[i].dim)
+    finally:
+        plan_x.close()
+        plan_f.close()
