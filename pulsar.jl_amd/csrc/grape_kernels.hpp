@@ -41,8 +41,16 @@ struct ProbDev {
   int d, B, nc, nt, M, dyn, obj, npairs, nseg, seglen, h0pm, unitary, grad_exact;
   double cre, cim;            // generator factor c: (0,-1) closed, (1,0) Lindblad
   const double2* H0;          // [M or 1][NB][256] tile images of the drift
-  const double2* Hc;          // [nc][NB][256] controls A_j
-  const double2* HcT;         // [nc][NB][256] A_j^T (trace operand)
+  // controls A_j in register-layout sparse form: for block w, register r and
+  // lane l the nonzero entries (j, value) at that position are
+  // [ptr[(w*4 + r)*64 + l], ptr[... + 1]) of (ent_j, ent_v); cs_* holds A_j
+  // (assembly), ct_* holds A_j^T (trace operand)
+  const int* cs_ptr;
+  const int* cs_j;
+  const double2* cs_v;
+  const int* ct_ptr;
+  const int* ct_j;
+  const double2* ct_v;
   const double* n0;           // [M or 1] ||A0||_1
   const double* ncn;          // [nc] ||A_j||_1
   const double* scale;        // [M][nc]
@@ -117,22 +125,30 @@ __device__ inline double block_max(double v, double* red, const Ln& x) {
   return out;
 }
 
-// Generator of slice k, already divided by 2^s; returns s.
+// Generator of slice k, c dt_k (A0 + sum_j s_mj u_jk A_j), divided by 2^s
+// where s is the smallest scaling exponent with ||X||_1 bound <= THETA9.
+// H0 is the wavefront's block of the member's drift (kept in registers).
 template <int B>
-__device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __restrict__ u, int& s_out,
-                               const Ln& x) {
+__device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __restrict__ u, const Blk& H0,
+                               int& s_out, const Ln& x) {
   const int nc = P.nc;
   const double* sc = P.scale + (size_t)m * nc;
   const double* uk = u + P.coff + (size_t)k * nc;
   const double dt = P.dts[k];
   double nb = P.n0[P.h0pm ? m : 0];
-  Blk H = load_blk(P.H0 + (P.h0pm ? (size_t)m * B * B * 256 : 0), x);
-  for (int j = 0; j < nc; j++) {
-    const double coef = sc[j] * uk[j];
-    nb += fabs(coef) * P.ncn[j];
-    Blk A = load_blk(P.Hc + (size_t)j * B * B * 256, x);
-    H.re += coef * A.re;
-    H.im += coef * A.im;
+  for (int j = 0; j < nc; j++) nb += fabs(sc[j] * uk[j]) * P.ncn[j];
+  Blk H = H0;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int pos = (x.w * 4 + r) * 64 + x.lane;
+    const int e1 = P.cs_ptr[pos + 1];
+    for (int e = P.cs_ptr[pos]; e < e1; e++) {
+      const int j = P.cs_j[e];
+      const double2 a = P.cs_v[e];
+      const double coef = sc[j] * uk[j];
+      H.re[r] += coef * a.x;
+      H.im[r] += coef * a.y;
+    }
   }
   nb *= fabs(dt);
   int s = 0;
@@ -140,6 +156,11 @@ __device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __r
   s_out = s;
   const double f = ldexp(dt, -s);
   return cscale(P.cre * f, P.cim * f, H);
+}
+
+template <int B>
+__device__ inline Blk load_drift(const ProbDev& P, int m, const Ln& x) {
+  return load_blk(P.H0 + (P.h0pm ? (size_t)m * B * B * 256 : 0), x);
 }
 
 __device__ inline void add_diag(Blk& X, double v, const Ln& x) {
@@ -151,22 +172,41 @@ __device__ inline void add_diag(Blk& X, double v, const Ln& x) {
 }
 
 // exp(X * 2^s) for X already scaled: degree-9 Taylor in Paterson-Stockmeyer form
-// p = B0 + X3 (B1 + X3 (B2 + c9 X3)), then s squarings.
+// p = B0 + X3 (B1 + X3 (B2 + c9 X3)) (4 products), then s squarings. The A
+// operands X and X3 are staged in LDS once each.
 template <int B>
 __device__ inline Blk expm_blk(const Blk& X, int s, double2* LA, double2* LB, const Ln& x) {
   const double c2 = 1.0 / 2, c3 = 1.0 / 6, c4 = 1.0 / 24, c5 = 1.0 / 120, c6 = 1.0 / 720,
                c7 = 1.0 / 5040, c8 = 1.0 / 40320, c9 = 1.0 / 362880;
-  Blk X2 = mm<B, N, N>(X, X, LA, LB, x);
-  Blk X3 = mm<B, N, N>(X, X2, LA, LB, x);
+  put<B>(LA, X, x);
+  if constexpr (B > 1) put<B>(LB, X, x);
+  bsync<B>();
+  Blk X2 = gemm_prod<B, N, N>(LA, X, LB, X, x);
+  if constexpr (B > 1) {
+    bsync<B>();
+    put<B>(LB, X2, x);
+    bsync<B>();
+  }
+  Blk X3 = gemm_prod<B, N, N>(LA, X, LB, X2, x);
   Blk Tm;
   Tm.re = c9 * X3.re + c8 * X2.re + c7 * X.re;
   Tm.im = c9 * X3.im + c8 * X2.im + c7 * X.im;
   add_diag(Tm, c6, x);
-  Blk R = mm<B, N, N>(X3, Tm, LA, LB, x);
+  bsync<B>();
+  put<B>(LA, X3, x);
+  if constexpr (B > 1) put<B>(LB, Tm, x);
+  bsync<B>();
+  Blk R = gemm_prod<B, N, N>(LA, X3, LB, Tm, x);
   R.re += c5 * X2.re + c4 * X.re;
   R.im += c5 * X2.im + c4 * X.im;
   add_diag(R, c3, x);
-  Blk U = mm<B, N, N>(X3, R, LA, LB, x);
+  if constexpr (B > 1) {
+    bsync<B>();
+    put<B>(LB, R, x);
+    bsync<B>();
+  }
+  Blk U = gemm_prod<B, N, N>(LA, X3, LB, R, x);
+  bsync<B>();
   U.re += c2 * X2.re + X.re;
   U.im += c2 * X2.im + X.im;
   add_diag(U, 1.0, x);
@@ -250,18 +290,18 @@ __global__ __launch_bounds__(64 * B * B) void k_segment(const ProbDev* __restric
   constexpr int NB = B * B;
   const int m = it.member;
   Blk Tseg = eye_blk(x);
+  const Blk H0 = load_drift<B>(P, m, x);
   int smax = 0;
   for (int k = it.k0; k < it.k1; k++) {
     int s;
-    Blk X = assemble<B>(P, m, k, u, s, x);
+    Blk X = assemble<B>(P, m, k, u, H0, s, x);
     smax = max(smax, s);
     Blk U = expm_blk<B>(X, s, LA, LB, x);
     // chain T <- U T, and U^T image from the same LDS tile
     put<B>(LA, U, x);
     if constexpr (B > 1) put<B>(LB, Tseg, x);
-    __syncthreads();
-    Blk acc = zero_blk();
-    gemm_acc<B, N, N>(acc, LA, U, LB, Tseg, x);
+    bsync<B>();
+    Blk acc = gemm_prod<B, N, N>(LA, U, LB, Tseg, x);
     Blk Ut;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -269,7 +309,7 @@ __global__ __launch_bounds__(64 * B * B) void k_segment(const ProbDev* __restric
       Ut.re[r] = v.x;
       Ut.im[r] = v.y;
     }
-    __syncthreads();
+    bsync<B>();
     Tseg = acc;
     store_blk(P.Ust + ((size_t)m * P.nt + k) * NB * 256, Ut, x);
   }
@@ -384,26 +424,27 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
   __syncthreads();
 }
 
-// Gradient contraction of one slice: G_m[j,k] = Re(c dt s_mj Tr(A_j M)).
+// Gradient contraction of one slice: G_m[j,k] = Re(c dt s_mj Tr(A_j M)),
+// Tr(A_j M) = sum over M's elements (b, a) of A_j[a][b] M[b][a]: each lane walks
+// the nonzeros of A_j^T at its own register positions.
 template <int B, int V>
 __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, double* red, const Ln& x) {
   const int nc = P.nc;
-  const size_t MB = (size_t)B * B * 256;
   const bool want_im = (P.cim != 0.0);  // closed: Re(-i z) = Im z
   for (int j0 = 0; j0 < nc; j0 += V) {
     double v[V];
 #pragma unroll
-    for (int t = 0; t < V; t++) {
-      v[t] = 0;
-      const int j = j0 + t;
-      if (j < nc) {
-        Blk A = load_blk(P.HcT + (size_t)j * MB, x);
-        double s = 0;
+    for (int t = 0; t < V; t++) v[t] = 0;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          s += want_im ? (A.re[r] * Mk.im[r] + A.im[r] * Mk.re[r]) : (A.re[r] * Mk.re[r] - A.im[r] * Mk.im[r]);
-        }
-        v[t] = s;
+    for (int r = 0; r < 4; r++) {
+      const int pos = (x.w * 4 + r) * 64 + x.lane;
+      const int e1 = P.ct_ptr[pos + 1];
+      for (int e = P.ct_ptr[pos]; e < e1; e++) {
+        const int jj = P.ct_j[e] - j0;
+        const double2 a = P.ct_v[e];
+        const double sv = want_im ? (a.x * Mk.im[r] + a.y * Mk.re[r]) : (a.x * Mk.re[r] - a.y * Mk.im[r]);
+#pragma unroll
+        for (int t = 0; t < V; t++) v[t] += (jj == t) ? sv : 0.0;
       }
     }
     double tot = wave_reduce_spread<V>(v, x.lane);
@@ -414,9 +455,9 @@ __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, d
       if (leader) red[x.w * V + idx] = tot;
       __syncthreads();
       if (x.w == 0 && leader) {
-        double s = 0;
-        for (int w = 0; w < B * B; w++) s += red[w * V + idx];
-        tot = s;
+        double sum = 0;
+        for (int w = 0; w < B * B; w++) sum += red[w * V + idx];
+        tot = sum;
       }
       __syncthreads();
     }
@@ -446,6 +487,8 @@ __global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restri
   const size_t MB = (size_t)NB * 256;
   const int m = it.member;
   Blk Es = load_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + it.seg) * MB, x);
+  Blk H0;
+  if constexpr (EXACT) H0 = load_drift<B>(P, m, x);
   if (P.unitary) {
     Blk Om = load_blk(P.Om + (size_t)m * MB, x);
     Blk Y = mm<B, N, N>(Es, Om, LA, LB, x);
@@ -456,7 +499,7 @@ __global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restri
       Blk Mk = mm<B, N, N>(Cm, Ud, LA, LB, x);    // C_k U_k^dag
       if constexpr (EXACT) {
         int s;
-        Blk X = assemble<B>(P, m, k, u, s, x);
+        Blk X = assemble<B>(P, m, k, u, H0, s, x);
         contract<B, V>(P, m, k, frechet_blk<B>(X, s, Mk, LA, LB, x), red, x);
       } else {
         contract<B, V>(P, m, k, Mk, red, x);
@@ -481,7 +524,7 @@ __global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restri
     Blk Mk = mm<B, N, C>(Pc, Lm, LA, LB, x);      // P_k Lambda_k^dag
     if constexpr (EXACT) {
       int s;
-      Blk X = assemble<B>(P, m, k, u, s, x);
+      Blk X = assemble<B>(P, m, k, u, H0, s, x);
       contract<B, V>(P, m, k, frechet_blk<B>(X, s, Mk, LA, LB, x), red, x);
     } else {
       contract<B, V>(P, m, k, Mk, red, x);
@@ -492,21 +535,40 @@ __global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restri
 }
 
 // ---------------------------------------------------------------- reductions
-__global__ void k_reduce_G(const ProbDev* __restrict__ probs, int nprob, const long long* __restrict__ off,
-                           long long total, double* __restrict__ G) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
+// G = sum_m w_m G_m in two fixed-order stages: blockIdx.y = member chunk c sums
+// members [c*ch, (c+1)*ch) into part[c][t]; k_reduce_G2 adds the chunks.
+constexpr int RED_CHUNKS = 16;
+
+__device__ inline int prob_of(const long long* __restrict__ off, int nprob, long long t) {
   int lo = 0, hi = nprob - 1;
   while (lo < hi) {  // last p with off[p] <= t
     int mid = (lo + hi + 1) >> 1;
     if (off[mid] <= t) lo = mid;
     else hi = mid - 1;
   }
-  const ProbDev& P = probs[lo];
-  const long long e = t - off[lo];
+  return lo;
+}
+
+__global__ void k_reduce_G1(const ProbDev* __restrict__ probs, int nprob, const long long* __restrict__ off,
+                            long long total, double* __restrict__ part) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int p = prob_of(off, nprob, t);
+  const ProbDev& P = probs[p];
+  const long long e = t - off[p];
   const long long ng = (long long)P.nc * P.nt;
+  const int ch = (P.M + RED_CHUNKS - 1) / RED_CHUNKS;
+  const int m0 = blockIdx.y * ch, m1 = min(P.M, m0 + ch);
   double s = 0;
-  for (int m = 0; m < P.M; m++) s += P.weight[m] * P.Gm[(size_t)m * ng + e];
+  for (int m = m0; m < m1; m++) s += P.weight[m] * P.Gm[(size_t)m * ng + e];
+  part[(size_t)blockIdx.y * total + t] = s;
+}
+
+__global__ void k_reduce_G2(long long total, const double* __restrict__ part, double* __restrict__ G) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  double s = 0;
+  for (int c = 0; c < RED_CHUNKS; c++) s += part[(size_t)c * total + t];
   G[t] = s;
 }
 

@@ -98,6 +98,43 @@ void tile_columns(const double* vecs, int n, int d, int B, double2* out) {
     }
 }
 
+// Register-layout sparse form of the controls: for block (I, J), register r and
+// lane l (the element (16I + (l>>4) + 4r, 16J + (l&15))) the nonzero (j, A_j)
+// entries at that position, or of A_j^T when transpose is set.
+struct Csr {
+  std::vector<int> ptr, j;
+  std::vector<double2> v;
+};
+
+Csr build_csr(const double* Hc, int nc, int d, int B, bool transpose) {
+  Csr out;
+  const int NB = B * B;
+  const size_t dd = (size_t)d * d;
+  out.ptr.assign((size_t)NB * 256 + 1, 0);
+  for (int w = 0; w < NB; w++) {
+    const int I = w / B, J = w % B;
+    for (int r = 0; r < 4; r++)
+      for (int l = 0; l < 64; l++) {
+        const size_t pos = ((size_t)w * 4 + r) * 64 + l;
+        const int row = 16 * I + (l >> 4) + 4 * r, col = 16 * J + (l & 15);
+        if (row < d && col < d)
+          for (int j = 0; j < nc; j++) {
+            cd v = transpose ? at(Hc + 2 * dd * j, d, col, row) : at(Hc + 2 * dd * j, d, row, col);
+            if (v != cd(0, 0)) {
+              out.j.push_back(j);
+              out.v.push_back(make_double2(v.real(), v.imag()));
+            }
+          }
+        out.ptr[pos + 1] = (int)out.j.size();
+      }
+  }
+  if (out.j.empty()) {  // keep the device arrays non-empty
+    out.j.push_back(0);
+    out.v.push_back(make_double2(0, 0));
+  }
+  return out;
+}
+
 double norm1(const double* a, int d) {
   double best = 0;
   for (int c = 0; c < d; c++) {
@@ -147,6 +184,7 @@ struct Plan {
   double* d_controls = nullptr;
   double* d_F = nullptr;
   double* d_G = nullptr;
+  double* d_part = nullptr;  // [RED_CHUNKS][n_params] partial member sums
   struct Group {
     int B = 0, V = 0;
     pr::WorkItem* d_items = nullptr;
@@ -350,8 +388,11 @@ int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
   HIPCHK(hipGetLastError());
   if (d_G) {
     long long total = pl.n_params;
-    hipLaunchKernelGGL(pr::k_reduce_G, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, pl.d_probs, pl.n_probs,
-                       pl.d_off, total, d_G);
+    hipLaunchKernelGGL(pr::k_reduce_G1, dim3((unsigned)((total + 255) / 256), pr::RED_CHUNKS), dim3(256), 0, st,
+                       pl.d_probs, pl.n_probs, pl.d_off, total, pl.d_part);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(pr::k_reduce_G2, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, pl.d_part,
+                       d_G);
     HIPCHK(hipGetLastError());
   }
   if ((rc = tend(pl, 3, st))) return rc;
@@ -407,17 +448,22 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     D.H0 = dp;
     if ((rc = pl.upload(&dd_, nrm.data(), nh0))) return rc;
     D.n0 = dd_;
-    for (int j = 0; j < nc; j++) {
-      tile_image(q.Hc + 2 * dd * j, d, B, false, img.data() + MB * j);
-      nrm[j] = norm1(q.Hc + 2 * dd * j, d);
-    }
-    if ((rc = pl.upload(&dp, img.data(), MB * nc))) return rc;
-    D.Hc = dp;
+    for (int j = 0; j < nc; j++) nrm[j] = norm1(q.Hc + 2 * dd * j, d);
     if ((rc = pl.upload(&dd_, nrm.data(), nc))) return rc;
     D.ncn = dd_;
-    for (int j = 0; j < nc; j++) tile_image(q.Hc + 2 * dd * j, d, B, true, img.data() + MB * j);
-    if ((rc = pl.upload(&dp, img.data(), MB * nc))) return rc;
-    D.HcT = dp;
+    for (int t = 0; t < 2; t++) {
+      Csr cs = build_csr(q.Hc, nc, d, B, t == 1);
+      int *ip, *jp;
+      double2* vp;
+      if ((rc = pl.upload(&ip, cs.ptr.data(), cs.ptr.size()))) return rc;
+      if ((rc = pl.upload(&jp, cs.j.data(), cs.j.size()))) return rc;
+      if ((rc = pl.upload(&vp, cs.v.data(), cs.v.size()))) return rc;
+      if (t == 0) {
+        D.cs_ptr = ip; D.cs_j = jp; D.cs_v = vp;
+      } else {
+        D.ct_ptr = ip; D.ct_j = jp; D.ct_v = vp;
+      }
+    }
     std::vector<double> sc((size_t)M * nc, 1.0);
     if (q.ctrl_scale) std::copy(q.ctrl_scale, q.ctrl_scale + (size_t)M * nc, sc.begin());
     if ((rc = pl.upload(&dd_, sc.data(), sc.size()))) return rc;
@@ -475,6 +521,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   if ((rc = pl.alloc(&pl.d_controls, pl.n_params))) return rc;
   if ((rc = pl.alloc(&pl.d_F, n_probs))) return rc;
   if ((rc = pl.alloc(&pl.d_G, pl.n_params))) return rc;
+  if ((rc = pl.alloc(&pl.d_part, (size_t)pr::RED_CHUNKS * pl.n_params))) return rc;
   for (int B = 1; B <= 4; B++) {
     if (items[B].empty()) continue;
     Plan::Group g;

@@ -10,7 +10,7 @@
 // in another layout go through LDS as plain 16 x 17 (padded) complex tiles.
 //
 // Per output block, a product C = op(A) * op(B) runs, for each k-block K and
-// k-step kb, four MFMAs: Re += Ar*Br - Ai*Bi, Im += Ar*Bi + Ai*Br.
+// k-step kb, three real MFMAs (Gauss's complex product).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -137,22 +137,42 @@ __device__ constexpr bool b_in_lds() {
   return !(B == 1 && (OP == N || OP == J));
 }
 
-// Accumulating block product: acc += sum_K op(A)_{I,K} op(B)_{K,J}
+// Workgroup synchronisation of the LDS tiles. A single-wavefront workgroup
+// (B == 1) only needs the wave's own LDS accesses ordered, which LDS does in
+// issue order; the fences keep the compiler from moving them.
+template <int B>
+__device__ inline void bsync() {
+  if constexpr (B == 1) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// Block product op(A) op(B) for this wavefront's output block with three real
+// products per k-step (Gauss): t1 = Ar Br, t2 = Ai Bi, t3 = (Ar + Ai)(Br + Bi);
+// Re = t1 - t2, Im = t3 - t1 - t2. 12 MFMAs per k-block instead of 16.
 template <int B, int OA, int OB>
-__device__ inline void gemm_acc(Blk& acc, const double2* LA, const Blk& Aown, const double2* LB,
-                                const Blk& Bown, const Ln& x) {
+__device__ inline Blk gemm_prod(const double2* LA, const Blk& Aown, const double2* LB, const Blk& Bown,
+                                const Ln& x) {
+  d4 t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
 #pragma unroll
   for (int K = 0; K < B; K++) {
 #pragma unroll
     for (int kb = 0; kb < 4; kb++) {
       double2 a = fetch_a<B, OA>(LA, Aown, K, kb, x);
       double2 b = fetch_b<B, OB>(LB, Bown, K, kb, x);
-      acc.re = mfma(a.x, b.x, acc.re);
-      acc.im = mfma(a.x, b.y, acc.im);
-      acc.re = mfma(-a.y, b.y, acc.re);
-      acc.im = mfma(a.y, b.x, acc.im);
+      t1 = mfma(a.x, b.x, t1);
+      t2 = mfma(a.y, b.y, t2);
+      t3 = mfma(a.x + a.y, b.x + b.y, t3);
     }
   }
+  Blk r;
+  r.re = t1 - t2;
+  r.im = t3 - t1 - t2;
+  return r;
 }
 
 // C = op(A) op(B). LA/LB are this workgroup's LDS scratch tiles.
@@ -161,11 +181,10 @@ __device__ inline Blk mm(const Blk& A, const Blk& Bm, double2* LA, double2* LB, 
   constexpr bool la = a_in_lds<B, OA>(), lb = b_in_lds<B, OB>();
   if constexpr (la) put<B>(LA, A, x);
   if constexpr (lb) put<B>(LB, Bm, x);
-  if constexpr (la || lb) __syncthreads();
-  Blk acc = zero_blk();
-  gemm_acc<B, OA, OB>(acc, LA, A, LB, Bm, x);
-  if constexpr (la || lb) __syncthreads();
-  return acc;
+  if constexpr (la || lb) bsync<B>();
+  Blk r = gemm_prod<B, OA, OB>(LA, A, LB, Bm, x);
+  if constexpr (la || lb) bsync<B>();
+  return r;
 }
 
 __device__ inline Blk axpby(double a, const Blk& X, double b, const Blk& Y) {
