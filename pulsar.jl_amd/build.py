@@ -19,7 +19,7 @@ SOURCES = ["pulsar_rocm.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
-def build(debug: bool = False, verbose: bool = False) -> str:
+def build(debug: bool = False, verbose: bool = False, out: str = OUT, defines=()) -> str:
     flags = [
         "--offload-arch=gfx950",
         "-std=c++17",
@@ -28,10 +28,13 @@ def build(debug: bool = False, verbose: bool = False) -> str:
         "-O1" if debug else "-O3",
         # packed FP32 VALU instructions stay disabled for all device code
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        # MFMA accumulators in VGPRs (no v_accvgpr_read copies on gfx950)
+        "-mllvm", "-amdgpu-mfma-vgpr-form",
         "-Wall",
         "-Wno-unused-function",
+        *[f"-D{d}" for d in defines],
     ]
-    tmp = OUT + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, *flags, "-o", tmp, *[os.path.join(CSRC, s) for s in SOURCES]]
     r = subprocess.run(cmd, capture_output=True, text=True)
     noise = "is not a recognized feature for this target"
@@ -40,9 +43,12 @@ def build(debug: bool = False, verbose: bool = False) -> str:
         raise RuntimeError("hipcc failed:\n" + "\n".join(msgs))
     if verbose and msgs:
         print("\n".join(msgs))
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(debug="--debug" in sys.argv, verbose=True))
+    args = sys.argv[1:]
+    out = args[args.index("--out") + 1] if "--out" in args else OUT
+    defs = [a[2:] for a in args if a.startswith("-D")]
+    print(build(debug="--debug" in args, verbose=True, out=out, defines=defs))

@@ -48,9 +48,11 @@ struct ProbDev {
   const int* cs_ptr;
   const int* cs_j;
   const double2* cs_v;
-  const int* ct_ptr;
-  const int* ct_j;
-  const double2* ct_v;
+  // trace operand: for control j, the nonzeros A_j[a][b] as (LDS index of the
+  // element (b, a) of a tile, value), entries [tr_ptr[j], tr_ptr[j+1])
+  const int* tr_ptr;
+  const int* tr_idx;
+  const double2* tr_v;
   const double* n0;           // [M or 1] ||A0||_1
   const double* ncn;          // [nc] ||A_j||_1
   const double* scale;        // [M][nc]
@@ -76,16 +78,29 @@ struct WorkItem {
   int prob, member, seg, k0, k1;
 };
 
-// exp truncation: degree-9 Taylor, ||X|| <= THETA9 keeps the remainder below
-// 2^-53 (||X||^10/10! <= 1.1e-16 for ||X|| <= 0.1147)
+// exp truncation: the Taylor remainder of degree m is below 2^-53 relative
+// for ||X||_1 <= theta_m: ||X||^7/7! <= 1.1e-16 for ||X|| <= 0.01777 (m = 6),
+// ||X||^10/10! <= 1.1e-16 for ||X|| <= 0.1147 (m = 9). Each slice takes
+// degree 6 (3 products) when its norm allows, else degree 9 (4 products)
+// with scaling and squaring.
+constexpr double THETA6 = 0.0175;
 constexpr double THETA9 = 0.11;
 constexpr int TAYLOR_DEG = 9;
 
+// Dynamic LDS of a workgroup, in order: LA, LB (operand tiles), red (reduction
+// scratch), zb (pair overlaps, k_combine), LT (segment product, k_segment with
+// B == 1). Each kernel is launched with only the prefix it uses, so the
+// gradient walk keeps the smallest footprint.
 template <int B>
 struct Smem {
   static constexpr int NB = B * B;
   static constexpr int TILE = NB * 272;  // complex elements per LDS matrix tile
-  static constexpr size_t bytes = 2 * TILE * sizeof(double2) + NB * 64 * sizeof(double) + 128 * sizeof(double2);
+  static constexpr size_t base = 2 * TILE * sizeof(double2) + NB * 64 * sizeof(double);
+  static constexpr size_t zb = 128 * sizeof(double2);
+  static constexpr size_t gradient = base;
+  static constexpr size_t combine = base + zb;
+  static constexpr size_t segment = base + zb + (B == 1 ? TILE * sizeof(double2) : 0);
+  static constexpr size_t max = segment > combine ? segment : combine;
 };
 
 template <int B>
@@ -125,37 +140,64 @@ __device__ inline double block_max(double v, double* red, const Ln& x) {
   return out;
 }
 
-// Generator of slice k, c dt_k (A0 + sum_j s_mj u_jk A_j), divided by 2^s
-// where s is the smallest scaling exponent with ||X||_1 bound <= THETA9.
-// H0 is the wavefront's block of the member's drift (kept in registers).
+// Generator of slice k, X = c dt_k (A0 + sum_j s_mj u_jk A_j). H0 is the
+// wavefront's block of the member's drift.
 template <int B>
 __device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __restrict__ u, const Blk& H0,
-                               int& s_out, const Ln& x) {
+                               const Ln& x) {
   const int nc = P.nc;
   const double* sc = P.scale + (size_t)m * nc;
   const double* uk = u + P.coff + (size_t)k * nc;
   const double dt = P.dts[k];
-  double nb = P.n0[P.h0pm ? m : 0];
-  for (int j = 0; j < nc; j++) nb += fabs(sc[j] * uk[j]) * P.ncn[j];
   Blk H = H0;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int pos = (x.w * 4 + r) * 64 + x.lane;
-    const int e1 = P.cs_ptr[pos + 1];
-    for (int e = P.cs_ptr[pos]; e < e1; e++) {
-      const int j = P.cs_j[e];
-      const double2 a = P.cs_v[e];
-      const double coef = sc[j] * uk[j];
+    const int e1 = gptr(P.cs_ptr)[pos + 1];
+    for (int e = gptr(P.cs_ptr)[pos]; e < e1; e++) {
+      const int j = gptr(P.cs_j)[e];
+      const double2 a = gload2(P.cs_v, e);
+      const double coef = gptr(sc)[j] * gptr(uk)[j];
       H.re[r] += coef * a.x;
       H.im[r] += coef * a.y;
     }
   }
-  nb *= fabs(dt);
+  return cscale(P.cre * dt, P.cim * dt, H);
+}
+
+// Upper bound of ||X||_1: largest column sum of |Re| + |Im| over the workgroup.
+template <int B>
+__device__ inline double norm1_blk(const Blk& X, double* red, const Ln& x) {
+  double cs = 0;
+#pragma unroll
+  for (int r = 0; r < 4; r++) cs += fabs(X.re[r]) + fabs(X.im[r]);
+  cs += __shfl_xor(cs, 16);
+  cs += __shfl_xor(cs, 32);  // lane c + 16q: column c of the block, all rows
+  if constexpr (B > 1) {
+    if (x.q == 0) red[x.w * 16 + x.c] = cs;
+    __syncthreads();
+    double best = 0;
+    for (int col = 0; col < 16 * B; col++) {
+      double t = 0;
+      for (int I = 0; I < B; I++) t += red[(I * B + col / 16) * 16 + (col & 15)];
+      best = fmax(best, t);
+    }
+    __syncthreads();
+    return best;
+  } else {
+#pragma unroll
+    for (int msk = 1; msk < 16; msk <<= 1) cs = fmax(cs, __shfl_xor(cs, msk));
+    return cs;
+  }
+}
+
+// Taylor degree and scaling for a generator of 1-norm bound nb; returns s,
+// sets deg6 when the degree-6 polynomial suffices.
+__device__ inline int choose_scaling(double nb, bool& deg6) {
+  deg6 = nb <= THETA6;
   int s = 0;
   if (nb > THETA9) s = (int)ceil(log2(nb / THETA9));
-  s_out = s;
-  const double f = ldexp(dt, -s);
-  return cscale(P.cre * f, P.cim * f, H);
+  return s;
 }
 
 template <int B>
@@ -171,11 +213,12 @@ __device__ inline void add_diag(Blk& X, double v, const Ln& x) {
   }
 }
 
-// exp(X * 2^s) for X already scaled: degree-9 Taylor in Paterson-Stockmeyer form
-// p = B0 + X3 (B1 + X3 (B2 + c9 X3)) (4 products), then s squarings. The A
-// operands X and X3 are staged in LDS once each.
+// exp(X * 2^s) for X already scaled. Degree 6: p = B0 + X3 (B1 + c6 X3)
+// (3 products); degree 9: p = B0 + X3 (B1 + X3 (B2 + c9 X3)) (4 products)
+// (Paterson-Stockmeyer); then s squarings. The A operands X and X3 are staged
+// in LDS once each.
 template <int B>
-__device__ inline Blk expm_blk(const Blk& X, int s, double2* LA, double2* LB, const Ln& x) {
+__device__ inline Blk expm_blk(const Blk& X, int s, bool deg6, double2* LA, double2* LB, const Ln& x) {
   const double c2 = 1.0 / 2, c3 = 1.0 / 6, c4 = 1.0 / 24, c5 = 1.0 / 120, c6 = 1.0 / 720,
                c7 = 1.0 / 5040, c8 = 1.0 / 40320, c9 = 1.0 / 362880;
   put<B>(LA, X, x);
@@ -188,22 +231,33 @@ __device__ inline Blk expm_blk(const Blk& X, int s, double2* LA, double2* LB, co
     bsync<B>();
   }
   Blk X3 = gemm_prod<B, N, N>(LA, X, LB, X2, x);
-  Blk Tm;
-  Tm.re = c9 * X3.re + c8 * X2.re + c7 * X.re;
-  Tm.im = c9 * X3.im + c8 * X2.im + c7 * X.im;
-  add_diag(Tm, c6, x);
-  bsync<B>();
-  put<B>(LA, X3, x);
-  if constexpr (B > 1) put<B>(LB, Tm, x);
-  bsync<B>();
-  Blk R = gemm_prod<B, N, N>(LA, X3, LB, Tm, x);
-  R.re += c5 * X2.re + c4 * X.re;
-  R.im += c5 * X2.im + c4 * X.im;
-  add_diag(R, c3, x);
-  if constexpr (B > 1) {
+  Blk R;
+  if (deg6) {
+    R.re = c6 * X3.re + c5 * X2.re + c4 * X.re;
+    R.im = c6 * X3.im + c5 * X2.im + c4 * X.im;
+    add_diag(R, c3, x);
     bsync<B>();
-    put<B>(LB, R, x);
+    put<B>(LA, X3, x);
+    if constexpr (B > 1) put<B>(LB, R, x);
     bsync<B>();
+  } else {
+    Blk Tm;
+    Tm.re = c9 * X3.re + c8 * X2.re + c7 * X.re;
+    Tm.im = c9 * X3.im + c8 * X2.im + c7 * X.im;
+    add_diag(Tm, c6, x);
+    bsync<B>();
+    put<B>(LA, X3, x);
+    if constexpr (B > 1) put<B>(LB, Tm, x);
+    bsync<B>();
+    R = gemm_prod<B, N, N>(LA, X3, LB, Tm, x);
+    R.re += c5 * X2.re + c4 * X.re;
+    R.im += c5 * X2.im + c4 * X.im;
+    add_diag(R, c3, x);
+    if constexpr (B > 1) {
+      bsync<B>();
+      put<B>(LB, R, x);
+      bsync<B>();
+    }
   }
   Blk U = gemm_prod<B, N, N>(LA, X3, LB, R, x);
   bsync<B>();
@@ -276,6 +330,19 @@ __device__ inline Blk frechet_blk(const Blk& X, int s, const Blk& M, double2* LA
   return L;
 }
 
+// L(X_k, M) for the exact gradient (degree-9 Frechet polynomial, scaled).
+template <int B>
+__device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const double* __restrict__ u, const Blk& H0,
+                                      const Blk& M, double2* LA, double2* LB, double* red, const Ln& x) {
+  Blk X = assemble<B>(P, m, k, u, H0, x);
+  bool deg6;
+  const int s = choose_scaling(norm1_blk<B>(X, red, x), deg6);
+  const double f = ldexp(1.0, -s);
+  X.re *= f;
+  X.im *= f;
+  return frechet_blk<B>(X, s, M, LA, LB, x);
+}
+
 // ---------------------------------------------------------------- k_segment
 template <int B>
 __global__ __launch_bounds__(64 * B * B) void k_segment(const ProbDev* __restrict__ probs,
@@ -289,19 +356,33 @@ __global__ __launch_bounds__(64 * B * B) void k_segment(const ProbDev* __restric
   smem_parts<B>(LA, LB, red, zb);
   constexpr int NB = B * B;
   const int m = it.member;
+  // the running segment product lives in the second LDS tile (B == 1: the
+  // product reads it from there and writes the new product back) or in registers
   Blk Tseg = eye_blk(x);
-  const Blk H0 = load_drift<B>(P, m, x);
+  double2* LT = zb + 128;  // B == 1: the third tile
+  if constexpr (B == 1) {
+    put<B>(LT, Tseg, x);
+    bsync<B>();
+  }
   int smax = 0;
   for (int k = it.k0; k < it.k1; k++) {
-    int s;
-    Blk X = assemble<B>(P, m, k, u, H0, s, x);
+    Blk X = assemble<B>(P, m, k, u, load_drift<B>(P, m, x), x);
+    bool deg6;
+    const int s = choose_scaling(norm1_blk<B>(X, red, x), deg6);
+    const double f = ldexp(1.0, -s);
+    X.re *= f;
+    X.im *= f;
     smax = max(smax, s);
-    Blk U = expm_blk<B>(X, s, LA, LB, x);
+    Blk U = expm_blk<B>(X, s, deg6, LA, LB, x);
     // chain T <- U T, and U^T image from the same LDS tile
     put<B>(LA, U, x);
     if constexpr (B > 1) put<B>(LB, Tseg, x);
     bsync<B>();
-    Blk acc = gemm_prod<B, N, N>(LA, U, LB, Tseg, x);
+    Blk acc;
+    if constexpr (B == 1)
+      acc = gemm_prod<B, N, NL>(LA, U, LT, U, x);
+    else
+      acc = gemm_prod<B, N, N>(LA, U, LB, Tseg, x);
     Blk Ut;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -310,8 +391,21 @@ __global__ __launch_bounds__(64 * B * B) void k_segment(const ProbDev* __restric
       Ut.im[r] = v.y;
     }
     bsync<B>();
-    Tseg = acc;
+    if constexpr (B == 1) {
+      put<B>(LT, acc, x);
+    } else {
+      Tseg = acc;
+    }
     store_blk(P.Ust + ((size_t)m * P.nt + k) * NB * 256, Ut, x);
+  }
+  if constexpr (B == 1) {
+    bsync<B>();
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      double2 v = LT[lidx(0, rowof(x.q, r), x.c)];
+      Tseg.re[r] = v.x;
+      Tseg.im[r] = v.y;
+    }
   }
   store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * NB * 256, Tseg, x);
   if (threadIdx.x == 0) P.smax[(size_t)m * P.nseg + it.seg] = smax;
@@ -424,52 +518,41 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
   __syncthreads();
 }
 
-// Gradient contraction of one slice: G_m[j,k] = Re(c dt s_mj Tr(A_j M)),
-// Tr(A_j M) = sum over M's elements (b, a) of A_j[a][b] M[b][a]: each lane walks
-// the nonzeros of A_j^T at its own register positions.
+// Gradient contraction of one slice: G_m[j,k] = Re(c dt s_mj Tr(A_j M)) with
+// Tr(A_j M) = sum over the nonzeros A_j[a][b] of A_j[a][b] M[b][a]. M is staged
+// in the LDS tile LA; the workgroup's threads form V groups of G = 64 B^2 / V
+// lanes (at most one wavefront), group g walks the nonzeros of control j0 + g
+// and reduces them inside its lanes.
 template <int B, int V>
-__device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, double* red, const Ln& x) {
+__device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, double2* LA, const Ln& x) {
+  constexpr int THREADS = 64 * B * B;
+  constexpr int G = (THREADS / V) < 64 ? (THREADS / V) : 64;
   const int nc = P.nc;
   const bool want_im = (P.cim != 0.0);  // closed: Re(-i z) = Im z
+  put<B>(LA, Mk, x);
+  bsync<B>();
+  const int t = threadIdx.x;
+  const int grp = t / G, gl = t % G;
   for (int j0 = 0; j0 < nc; j0 += V) {
-    double v[V];
-#pragma unroll
-    for (int t = 0; t < V; t++) v[t] = 0;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int pos = (x.w * 4 + r) * 64 + x.lane;
-      const int e1 = P.ct_ptr[pos + 1];
-      for (int e = P.ct_ptr[pos]; e < e1; e++) {
-        const int jj = P.ct_j[e] - j0;
-        const double2 a = P.ct_v[e];
-        const double sv = want_im ? (a.x * Mk.im[r] + a.y * Mk.re[r]) : (a.x * Mk.re[r] - a.y * Mk.im[r]);
-#pragma unroll
-        for (int t = 0; t < V; t++) v[t] += (jj == t) ? sv : 0.0;
+    const int j = j0 + grp;
+    double sv = 0;
+    if (grp < V && j < nc) {
+      const int e1 = gptr(P.tr_ptr)[j + 1];
+      for (int e = gptr(P.tr_ptr)[j] + gl; e < e1; e += G) {
+        const double2 a = gload2(P.tr_v, e);
+        const double2 mv = LA[gptr(P.tr_idx)[e]];
+        sv += want_im ? (a.x * mv.y + a.y * mv.x) : (a.x * mv.x - a.y * mv.y);
       }
     }
-    double tot = wave_reduce_spread<V>(v, x.lane);
-    constexpr int H = (V >= 64) ? 6 : (V >= 32) ? 5 : (V >= 16) ? 4 : (V >= 8) ? 3 : (V >= 4) ? 2 : (V >= 2) ? 1 : 0;
-    const int idx = x.lane >> (6 - H);
-    const bool leader = (x.lane & ((64 >> H) - 1)) == 0;
-    if constexpr (B > 1) {
-      if (leader) red[x.w * V + idx] = tot;
-      __syncthreads();
-      if (x.w == 0 && leader) {
-        double sum = 0;
-        for (int w = 0; w < B * B; w++) sum += red[w * V + idx];
-        tot = sum;
-      }
-      __syncthreads();
-    }
-    if (x.w == 0 && leader) {
-      const int j = j0 + idx;
-      if (j < nc) {
-        const double f = P.dts[k] * P.scale[(size_t)m * nc + j];
-        // Re(c * f * z): closed c = -i -> f Im z ; Lindblad c = 1 -> f Re z
-        P.Gm[(size_t)m * nc * P.nt + (size_t)k * nc + j] = want_im ? f * tot * (-P.cim) : f * tot * P.cre;
-      }
+#pragma unroll
+    for (int msk = 1; msk < G; msk <<= 1) sv += __shfl_xor(sv, msk);
+    if (grp < V && gl == 0 && j < nc) {
+      const double f = P.dts[k] * P.scale[(size_t)m * nc + j];
+      // Re(c * f * z): closed c = -i -> f Im z ; Lindblad c = 1 -> f Re z
+      gptr_w(P.Gm)[(size_t)m * nc * P.nt + (size_t)k * nc + j] = want_im ? f * sv * (-P.cim) : f * sv * P.cre;
     }
   }
+  bsync<B>();
 }
 
 // ---------------------------------------------------------------- k_gradient
@@ -498,11 +581,9 @@ __global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restri
       Blk Ud = conj_blk(Ut);                      // U^dag, own block
       Blk Mk = mm<B, N, N>(Cm, Ud, LA, LB, x);    // C_k U_k^dag
       if constexpr (EXACT) {
-        int s;
-        Blk X = assemble<B>(P, m, k, u, H0, s, x);
-        contract<B, V>(P, m, k, frechet_blk<B>(X, s, Mk, LA, LB, x), red, x);
+        contract<B, V>(P, m, k, exact_direction<B>(P, m, k, u, H0, Mk, LA, LB, red, x), LA, x);
       } else {
-        contract<B, V>(P, m, k, Mk, red, x);
+        contract<B, V>(P, m, k, Mk, LA, x);
       }
       Cm = mm<B, T, N>(Ut, Mk, LA, LB, x);        // U_k M_k
     }
@@ -523,11 +604,9 @@ __global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restri
     Blk Pc = load_blk(scr + (size_t)(k - it.k0) * MB, x);
     Blk Mk = mm<B, N, C>(Pc, Lm, LA, LB, x);      // P_k Lambda_k^dag
     if constexpr (EXACT) {
-      int s;
-      Blk X = assemble<B>(P, m, k, u, H0, s, x);
-      contract<B, V>(P, m, k, frechet_blk<B>(X, s, Mk, LA, LB, x), red, x);
+      contract<B, V>(P, m, k, exact_direction<B>(P, m, k, u, H0, Mk, LA, LB, red, x), LA, x);
     } else {
-      contract<B, V>(P, m, k, Mk, red, x);
+      contract<B, V>(P, m, k, Mk, LA, x);
     }
     Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
     Lm = mm<B, J, N>(Ut, Lm, LA, LB, x);          // U_k^dag Lambda_k  (U^dag = conj(U^T))

@@ -135,6 +135,36 @@ Csr build_csr(const double* Hc, int nc, int d, int B, bool transpose) {
   return out;
 }
 
+// Trace operand of the gradient: per control j the nonzeros A_j[a][b] with the
+// LDS tile index of the element (b, a) of the matrix they multiply.
+struct Trace {
+  std::vector<int> ptr, idx;
+  std::vector<double2> v;
+};
+
+Trace build_trace(const double* Hc, int nc, int d) {
+  Trace out;
+  const int B = (d + 15) / 16;
+  const size_t dd = (size_t)d * d;
+  out.ptr.push_back(0);
+  for (int j = 0; j < nc; j++) {
+    for (int b = 0; b < d; b++)
+      for (int a = 0; a < d; a++) {
+        cd v = at(Hc + 2 * dd * j, d, a, b);
+        if (v == cd(0, 0)) continue;
+        const int blk = (b / 16) * B + (a / 16);
+        out.idx.push_back(blk * 272 + (b % 16) * 17 + (a % 16));
+        out.v.push_back(make_double2(v.real(), v.imag()));
+      }
+    out.ptr.push_back((int)out.idx.size());
+  }
+  if (out.idx.empty()) {
+    out.idx.push_back(0);
+    out.v.push_back(make_double2(0, 0));
+  }
+  return out;
+}
+
 double norm1(const double* a, int d) {
   double best = 0;
   for (int c = 0; c < d; c++) {
@@ -281,7 +311,7 @@ int set_grad_attr(size_t bytes) {
 
 template <int B>
 int set_smem_attr() {
-  size_t bytes = pr::Smem<B>::bytes;
+  size_t bytes = pr::Smem<B>::max;
   if (bytes > 65536) {
     HIPCHK(hipFuncSetAttribute((const void*)pr::k_segment<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     HIPCHK(hipFuncSetAttribute((const void*)pr::k_combine<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -330,7 +360,7 @@ int tend(Plan& pl, int cls, hipStream_t st) {
 
 template <int B>
 int launch_gradient(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t st) {
-  const size_t sm = pr::Smem<B>::bytes;
+  const size_t sm = pr::Smem<B>::gradient;
   const dim3 blk(64 * B * B);
   switch (g.V) {
 #define PR_GCASE(VV)                                                                                        \
@@ -351,15 +381,14 @@ int launch_gradient(Plan& pl, const Plan::Group& g, const double* d_u, hipStream
 
 template <int B>
 int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hipStream_t st) {
-  const size_t sm = pr::Smem<B>::bytes;
   const dim3 blk(64 * B * B);
   int rc;
   if ((rc = tbegin(pl, st))) return rc;
-  hipLaunchKernelGGL((pr::k_segment<B>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u);
+  hipLaunchKernelGGL((pr::k_segment<B>), dim3(g.n_items), blk, pr::Smem<B>::segment, st, pl.d_probs, g.d_items, d_u);
   HIPCHK(hipGetLastError());
   if ((rc = tend(pl, 0, st))) return rc;
   if ((rc = tbegin(pl, st))) return rc;
-  hipLaunchKernelGGL((pr::k_combine<B>), dim3(g.n_members), blk, sm, st, pl.d_probs, g.d_members);
+  hipLaunchKernelGGL((pr::k_combine<B>), dim3(g.n_members), blk, pr::Smem<B>::combine, st, pl.d_probs, g.d_members);
   HIPCHK(hipGetLastError());
   if ((rc = tend(pl, 1, st))) return rc;
   if (grad) {
@@ -451,18 +480,19 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     for (int j = 0; j < nc; j++) nrm[j] = norm1(q.Hc + 2 * dd * j, d);
     if ((rc = pl.upload(&dd_, nrm.data(), nc))) return rc;
     D.ncn = dd_;
-    for (int t = 0; t < 2; t++) {
-      Csr cs = build_csr(q.Hc, nc, d, B, t == 1);
+    {
+      Csr cs = build_csr(q.Hc, nc, d, B, false);
       int *ip, *jp;
       double2* vp;
       if ((rc = pl.upload(&ip, cs.ptr.data(), cs.ptr.size()))) return rc;
       if ((rc = pl.upload(&jp, cs.j.data(), cs.j.size()))) return rc;
       if ((rc = pl.upload(&vp, cs.v.data(), cs.v.size()))) return rc;
-      if (t == 0) {
-        D.cs_ptr = ip; D.cs_j = jp; D.cs_v = vp;
-      } else {
-        D.ct_ptr = ip; D.ct_j = jp; D.ct_v = vp;
-      }
+      D.cs_ptr = ip; D.cs_j = jp; D.cs_v = vp;
+      Trace tr = build_trace(q.Hc, nc, d);
+      if ((rc = pl.upload(&ip, tr.ptr.data(), tr.ptr.size()))) return rc;
+      if ((rc = pl.upload(&jp, tr.idx.data(), tr.idx.size()))) return rc;
+      if ((rc = pl.upload(&vp, tr.v.data(), tr.v.size()))) return rc;
+      D.tr_ptr = ip; D.tr_idx = jp; D.tr_v = vp;
     }
     std::vector<double> sc((size_t)M * nc, 1.0);
     if (q.ctrl_scale) std::copy(q.ctrl_scale, q.ctrl_scale + (size_t)M * nc, sc.begin());

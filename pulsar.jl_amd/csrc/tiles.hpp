@@ -14,6 +14,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#ifndef PR_MFMA_4M
+#define PR_MFMA_4M 0
+#endif
+
 namespace pr {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -22,7 +26,8 @@ struct Blk {
   d4 re, im;
 };
 
-enum Op { N = 0, T = 1, C = 2, J = 3 };  // none, transpose, conjugate transpose, conjugate
+// none, transpose, conjugate transpose, conjugate; NL: none, always read from LDS
+enum Op { N = 0, T = 1, C = 2, J = 3, NL = 4 };
 
 struct Ln {
   int lane, q, c, w, I, Jb;
@@ -64,13 +69,37 @@ __device__ inline Blk eye_blk(const Ln& x) {
   return b;
 }
 
+// Pointers held in structs are generic; these accessors let the compiler emit
+// global (not flat) memory instructions for the device buffers.
+#define PR_GLOBAL __attribute__((address_space(1)))
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+template <class T>
+__device__ inline const PR_GLOBAL T* gptr(const T* p) {
+  return (const PR_GLOBAL T*)p;
+}
+template <class T>
+__device__ inline PR_GLOBAL T* gptr_w(T* p) {
+  return (PR_GLOBAL T*)p;
+}
+__device__ inline double2 gload2(const double2* p, size_t i) {
+  dv2 v = ((const PR_GLOBAL dv2*)p)[i];
+  return make_double2(v.x, v.y);
+}
+__device__ inline void gstore2(double2* p, size_t i, double a, double b) {
+  dv2 v;
+  v.x = a;
+  v.y = b;
+  ((PR_GLOBAL dv2*)p)[i] = v;
+}
+
 // Global tile image: 256 complex per block, register-major: [r][lane].
 __device__ inline Blk load_blk(const double2* __restrict__ img, const Ln& x) {
   Blk b;
-  const double2* p = img + (size_t)x.w * 256;
+  const size_t base = (size_t)x.w * 256;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    double2 v = p[r * 64 + x.lane];
+    double2 v = gload2(img, base + r * 64 + x.lane);
     b.re[r] = v.x;
     b.im[r] = v.y;
   }
@@ -78,9 +107,9 @@ __device__ inline Blk load_blk(const double2* __restrict__ img, const Ln& x) {
 }
 
 __device__ inline void store_blk(double2* __restrict__ img, const Blk& b, const Ln& x) {
-  double2* p = img + (size_t)x.w * 256;
+  const size_t base = (size_t)x.w * 256;
 #pragma unroll
-  for (int r = 0; r < 4; r++) p[r * 64 + x.lane] = make_double2(b.re[r], b.im[r]);
+  for (int r = 0; r < 4; r++) gstore2(img, base + r * 64 + x.lane, b.re[r], b.im[r]);
 }
 
 // LDS tile: block blk, row, col -> padded index
@@ -119,7 +148,7 @@ __device__ inline double2 fetch_b(const double2* L, const Blk& own, int K, int k
     return v;
   } else {
     double2 v;
-    if constexpr (OP == N || OP == J)
+    if constexpr (OP == N || OP == J || OP == NL)
       v = L[lidx(K * B + x.Jb, rowof(x.q, kb), x.c)];
     else
       v = L[lidx(x.Jb * B + K, x.c, rowof(x.q, kb))];
@@ -157,6 +186,23 @@ __device__ inline void bsync() {
 template <int B, int OA, int OB>
 __device__ inline Blk gemm_prod(const double2* LA, const Blk& Aown, const double2* LB, const Blk& Bown,
                                 const Ln& x) {
+#if PR_MFMA_4M
+  // four real products per k-step: Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br
+  Blk r = zero_blk();
+#pragma unroll
+  for (int K = 0; K < B; K++) {
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+      double2 a = fetch_a<B, OA>(LA, Aown, K, kb, x);
+      double2 b = fetch_b<B, OB>(LB, Bown, K, kb, x);
+      r.re = mfma(a.x, b.x, r.re);
+      r.im = mfma(a.x, b.y, r.im);
+      r.re = mfma(-a.y, b.y, r.re);
+      r.im = mfma(a.y, b.x, r.im);
+    }
+  }
+  return r;
+#else
   d4 t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
 #pragma unroll
   for (int K = 0; K < B; K++) {
@@ -173,6 +219,7 @@ __device__ inline Blk gemm_prod(const double2* LA, const Blk& Aown, const double
   r.re = t1 - t2;
   r.im = t3 - t1 - t2;
   return r;
+#endif
 }
 
 // C = op(A) op(B). LA/LB are this workgroup's LDS scratch tiles.

@@ -15,7 +15,7 @@ import numpy as np
 from .problem import COptions, Problem, controls_cm, grad_from_cm, to_c
 
 LIB_NAME = "libpulsar_rocm.so"
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+LIB_PATH = os.environ.get("PULSAR_ROCM_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 PR_ERR = {
     -1: "PR_ERR_ARG",
