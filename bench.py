@@ -77,12 +77,56 @@ def calibrate_copy(mib):
     print(json.dumps({"calibrate_copy_bytes": n, "copies": 3}))
 
 
+def other_workload(args):
+    """Timing of the other GRAPE configurations (parity is covered by the tests):
+    evaluations per second of the whole batch, device-resident controls."""
+    if args.workload == "three_qubit_lindblad":
+        p, u = workloads.three_qubit_lindblad()
+        probs, us = [p], [u]
+        desc = "three_qubit_lindblad: N=8 (64x64 Liouvillian), 64 members, 6 controls, 1000 slices, dt=2ns, 64 pairs"
+        flops = 8 * 64 ** 3 * (6 + 2 + p.n_ctrl) * p.n_members * p.n_t
+    elif args.workload == "uneven_batch":
+        batch = workloads.uneven_batch()
+        probs, us = [b[0] for b in batch], [b[1] for b in batch]
+        desc = "uneven_batch: 1024 gate problems, d in {2..64} with p ~ 1/d, 200..2000 slices, 2 controls per qubit"
+        flops = sum((72 * q.dim ** 3 + 16 * q.n_ctrl * q.dim ** 2) * q.n_t * q.n_members for q in probs)
+    else:
+        p, u = workloads.single_qubit_state()
+        probs, us = [p], [u]
+        desc = "single_qubit_state: d=2, 100 slices, state transfer"
+        flops = (72 * 8 + 16 * 2 * 4) * 100
+    native.init(0)
+    plan = native.Plan(probs, precision=PR_FP64 if args.precision == "fp64" else PR_MIXED)
+    plan.upload_controls(us)
+    d_u, d_F, d_G = plan.buffers()
+    for _ in range(args.warmup):
+        plan.eval_device(d_u, d_F, d_G)
+    plan.synchronize()
+    plan.set_timing(True)
+    plan.kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.eval_device(d_u, d_F, d_G)
+    plan.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = plan.kernel_times()
+    plan.close()
+    rate = args.steps / elapsed
+    print(json.dumps({"workload": args.workload, "description": desc, "evals_per_s": round(rate, 3),
+                      "ms_per_eval": round(1e3 * elapsed / args.steps, 4), "steps": args.steps,
+                      "algorithmic_tflops": round(flops * rate / 1e12, 3),
+                      "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--workload", default="four_qubit_ensemble",
+                    choices=["four_qubit_ensemble", "three_qubit_lindblad", "uneven_batch", "single_qubit_state"],
+                    help="the headline metric is four_qubit_ensemble; the others print timing lines of their own")
     ap.add_argument("--cpu-sample", type=int, default=32, help="members timed for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -90,6 +134,9 @@ def main():
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.workload != "four_qubit_ensemble":
+        other_workload(args)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

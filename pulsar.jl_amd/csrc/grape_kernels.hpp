@@ -365,8 +365,11 @@ __global__ __launch_bounds__(64 * B * B) void k_segment(const ProbDev* __restric
     bsync<B>();
   }
   int smax = 0;
+  // software pipeline: the generator of slice k+1 is assembled while the
+  // chain product of slice k runs on the matrix cores
+  Blk Xn = assemble<B>(P, m, it.k0, u, load_drift<B>(P, m, x), x);
   for (int k = it.k0; k < it.k1; k++) {
-    Blk X = assemble<B>(P, m, k, u, load_drift<B>(P, m, x), x);
+    Blk X = Xn;
     bool deg6;
     const int s = choose_scaling(norm1_blk<B>(X, red, x), deg6);
     const double f = ldexp(1.0, -s);
@@ -390,13 +393,14 @@ __global__ __launch_bounds__(64 * B * B) void k_segment(const ProbDev* __restric
       Ut.re[r] = v.x;
       Ut.im[r] = v.y;
     }
+    if (k + 1 < it.k1) Xn = assemble<B>(P, m, k + 1, u, load_drift<B>(P, m, x), x);
+    store_blk(P.Ust + ((size_t)m * P.nt + k) * NB * 256, Ut, x);
     bsync<B>();
     if constexpr (B == 1) {
       put<B>(LT, acc, x);
     } else {
       Tseg = acc;
     }
-    store_blk(P.Ust + ((size_t)m * P.nt + k) * NB * 256, Ut, x);
   }
   if constexpr (B == 1) {
     bsync<B>();
@@ -576,16 +580,21 @@ __global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restri
     Blk Om = load_blk(P.Om + (size_t)m * MB, x);
     Blk Y = mm<B, N, N>(Es, Om, LA, LB, x);
     Blk Cm = mm<B, N, C>(Y, Es, LA, LB, x);
+    // U_{k+1} is fetched and the contraction of slice k runs while the
+    // product C_{k+1} = U_k M_k is on the matrix cores
+    Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + it.k0) * MB, x);
     for (int k = it.k0; k < it.k1; k++) {
-      Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
       Blk Ud = conj_blk(Ut);                      // U^dag, own block
       Blk Mk = mm<B, N, N>(Cm, Ud, LA, LB, x);    // C_k U_k^dag
+      Blk Un = Ut;
+      if (k + 1 < it.k1) Un = load_blk(P.Ust + ((size_t)m * P.nt + k + 1) * MB, x);
+      Cm = mm<B, T, N>(Ut, Mk, LA, LB, x);        // U_k M_k
       if constexpr (EXACT) {
         contract<B, V>(P, m, k, exact_direction<B>(P, m, k, u, H0, Mk, LA, LB, red, x), LA, x);
       } else {
         contract<B, V>(P, m, k, Mk, LA, x);
       }
-      Cm = mm<B, T, N>(Ut, Mk, LA, LB, x);        // U_k M_k
+      Ut = Un;
     }
     return;
   }

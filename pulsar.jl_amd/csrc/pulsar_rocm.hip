@@ -435,6 +435,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   std::vector<std::vector<pr::WorkItem>> items(5);
   std::vector<std::vector<int2>> members(5);
   std::vector<int> vmax(5, 1);
+  std::vector<long long> group_members(5, 0);
+  for (int p = 0; p < n_probs; p++) group_members[(probs[p].dim + 15) / 16] += probs[p].n_members;
   for (int p = 0; p < n_probs; p++) {
     const pr_problem& q = probs[p];
     pr::ProbDev& D = pl.host_probs[p];
@@ -448,13 +450,14 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     D.grad_exact = pl.opt.grad_mode == PR_GRAD_EXACT;
     D.cre = q.dynamics == PR_CLOSED ? 0.0 : 1.0;
     D.cim = q.dynamics == PR_CLOSED ? -1.0 : 0.0;
-    // time segmentation: about 8192 wavefronts of work per problem group
+    // time segmentation: about 8192 wavefronts of work over all problems of
+    // the same block count (members of the group share the segments)
     int L;
     if (pl.opt.segment > 0) {
       L = pl.opt.segment;
     } else {
       long long want = std::max(1LL, (long long)(8192 / NB));
-      long long S = (want + M - 1) / M;
+      long long S = (want + group_members[B] - 1) / group_members[B];
       S = std::min<long long>(std::max<long long>(S, 1), nt);
       L = (int)((nt + S - 1) / S);
       L = std::max(L, std::min(nt, 4));
