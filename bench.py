@@ -32,6 +32,14 @@ from pulsar_rocm.problem import PR_FP64, PR_MIXED  # noqa: E402
 MFMA_PEAK_TFLOPS = {"fp64": 78.6}
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the two main kernels on this workload, measured with
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by what the same counters
+# read for a 256 MiB device-to-device copy (FETCH x 2.0, WRITE x 1.0); the
+# commands and per-kernel figures are in profiles/grape_four_qubit_ensemble_hbm_traffic.md.
+# k_segment writes the U_k^T images (256 x 2000 x 4 KiB = 2.10 GB) and
+# k_gradient reads them back once.
+MEASURED_TRAFFIC_BYTES = {"k_segment": 0.011e9 + 2.131e9, "k_gradient": 2.140e9 + 0.040e9}
+
 
 def flops_model(d, nc):
     """Algorithmic flops per member-slice: 9 complex d x d products (nominal
@@ -204,7 +212,8 @@ def main():
                                "8 controls, 2000 slices, dt=1us, gate target",
                    "members": M, "slices": nt, "dim": d, "controls": nc, "precision": args.precision},
         "roofline": {"bound": "mfma", "kernel": dominant, "achieved": round(achieved, 3), "peak": peak,
-                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                     "traffic": MEASURED_TRAFFIC_BYTES[dominant] if prec == PR_FP64 else None,
                      "kernel_ms": round(dom_ms, 4),
                      "algorithmic_flops_per_launch": dom_flops * units},
         "eval_tflops_algorithmic": round(f_tot * units * evals_s / 1e12, 3),

@@ -88,19 +88,25 @@ constexpr double THETA9 = 0.11;
 constexpr int TAYLOR_DEG = 9;
 
 // Dynamic LDS of a workgroup, in order: LA, LB (operand tiles), red (reduction
-// scratch), zb (pair overlaps, k_combine), LT (segment product, k_segment with
-// B == 1). Each kernel is launched with only the prefix it uses, so the
-// gradient walk keeps the smallest footprint.
+// scratch), zb (pair overlaps, k_combine). Each kernel is launched with only
+// the prefix it uses, since LDS per workgroup bounds how many wavefronts a CU
+// holds. Single-wavefront workgroups (B == 1) read the B operand of N
+// products and the A operand of T/C products from registers:
+//   k_segment   LA (operand) + LB (the running segment product)
+//   k_gradient  LA only for unitary dynamics; LA + LB for Lindblad
+// Wider workgroups use LA, LB and red in every kernel.
 template <int B>
 struct Smem {
   static constexpr int NB = B * B;
   static constexpr int TILE = NB * 272;  // complex elements per LDS matrix tile
-  static constexpr size_t base = 2 * TILE * sizeof(double2) + NB * 64 * sizeof(double);
+  static constexpr size_t tile = TILE * sizeof(double2);
+  static constexpr size_t base = 2 * tile + NB * 64 * sizeof(double);
   static constexpr size_t zb = 128 * sizeof(double2);
-  static constexpr size_t gradient = base;
   static constexpr size_t combine = base + zb;
-  static constexpr size_t segment = base + zb + (B == 1 ? TILE * sizeof(double2) : 0);
-  static constexpr size_t max = segment > combine ? segment : combine;
+  static constexpr size_t segment = B == 1 ? 2 * tile : base;
+  static constexpr size_t gradient_unitary = B == 1 ? tile : base;
+  static constexpr size_t gradient_lindblad = B == 1 ? 2 * tile : base;
+  static constexpr size_t max = combine;
 };
 
 template <int B>
@@ -344,8 +350,19 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
 }
 
 // ---------------------------------------------------------------- k_segment
+// Minimum wavefronts per SIMD the compiler must leave room for (caps VGPRs
+// at 512 / waves); 0 leaves the register budget to the compiler. The d <= 16
+// segment walk runs faster at 4 waves (128 VGPRs, a few spilled values
+// outside the products) than at the compiler's 3; the gradient walk is not
+// occupancy-bound.
+#ifndef PR_SEGMENT_WAVES
+#define PR_SEGMENT_WAVES 4
+#endif
+#ifndef PR_GRADIENT_WAVES
+#define PR_GRADIENT_WAVES 0
+#endif
 template <int B>
-__global__ __launch_bounds__(64 * B * B) void k_segment(const ProbDev* __restrict__ probs,
+__global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B == 1 ? PR_SEGMENT_WAVES : 0))) void k_segment(const ProbDev* __restrict__ probs,
                                                          const WorkItem* __restrict__ items,
                                                          const double* __restrict__ u) {
   const WorkItem it = items[blockIdx.x];
@@ -359,7 +376,7 @@ __global__ __launch_bounds__(64 * B * B) void k_segment(const ProbDev* __restric
   // the running segment product lives in the second LDS tile (B == 1: the
   // product reads it from there and writes the new product back) or in registers
   Blk Tseg = eye_blk(x);
-  double2* LT = zb + 128;  // B == 1: the third tile
+  double2* LT = LB;
   if constexpr (B == 1) {
     put<B>(LT, Tseg, x);
     bsync<B>();
@@ -561,7 +578,8 @@ __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, d
 
 // ---------------------------------------------------------------- k_gradient
 template <int B, int V, bool EXACT>
-__global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restrict__ probs,
+__global__ __launch_bounds__(64 * B * B)
+__attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT ? PR_GRADIENT_WAVES : 0))) void k_gradient(const ProbDev* __restrict__ probs,
                                                           const WorkItem* __restrict__ items,
                                                           const double* __restrict__ u) {
   const WorkItem it = items[blockIdx.x];
@@ -579,7 +597,23 @@ __global__ __launch_bounds__(64 * B * B) void k_gradient(const ProbDev* __restri
   if (P.unitary) {
     Blk Om = load_blk(P.Om + (size_t)m * MB, x);
     Blk Y = mm<B, N, N>(Es, Om, LA, LB, x);
-    Blk Cm = mm<B, N, C>(Y, Es, LA, LB, x);
+    Blk Cm;
+    if constexpr (B == 1) {
+      // E_s^dag through the one LDS tile, then an N x N product (no second tile)
+      put<B>(LA, Es, x);
+      bsync<B>();
+      Blk Ed;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        double2 v = LA[lidx(0, x.c, rowof(x.q, r))];  // Es[c][q + 4r]
+        Ed.re[r] = v.x;
+        Ed.im[r] = -v.y;
+      }
+      bsync<B>();
+      Cm = mm<B, N, N>(Y, Ed, LA, LB, x);
+    } else {
+      Cm = mm<B, N, C>(Y, Es, LA, LB, x);
+    }
     // U_{k+1} is fetched and the contraction of slice k runs while the
     // product C_{k+1} = U_k M_k is on the matrix cores
     Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + it.k0) * MB, x);
@@ -660,13 +694,19 @@ __global__ void k_reduce_G2(long long total, const double* __restrict__ part, do
   G[t] = s;
 }
 
-__global__ void k_reduce_F(const ProbDev* __restrict__ probs, int nprob, double* __restrict__ F) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per problem: lanes take members m = lane, lane + 64, ...; the
+// lane partials are combined by a fixed butterfly, so the sum is reproducible
+__global__ void __launch_bounds__(64) k_reduce_F(const ProbDev* __restrict__ probs, int nprob,
+                                                 double* __restrict__ F) {
+  const int p = blockIdx.x;
   if (p >= nprob) return;
   const ProbDev& P = probs[p];
+  const int lane = threadIdx.x;
   double s = 0;
-  for (int m = 0; m < P.M; m++) s += P.weight[m] * P.Fm[m];
-  F[p] = s;
+  for (int m = lane; m < P.M; m += 64) s += P.weight[m] * P.Fm[m];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) F[p] = s;
 }
 
 }  // namespace pr

@@ -217,6 +217,7 @@ struct Plan {
   double* d_part = nullptr;  // [RED_CHUNKS][n_params] partial member sums
   struct Group {
     int B = 0, V = 0;
+    bool lindblad = false;  // some problem of the group has Lindblad dynamics
     pr::WorkItem* d_items = nullptr;
     int n_items = 0;
     int2* d_members = nullptr;
@@ -360,7 +361,7 @@ int tend(Plan& pl, int cls, hipStream_t st) {
 
 template <int B>
 int launch_gradient(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t st) {
-  const size_t sm = pr::Smem<B>::gradient;
+  const size_t sm = g.lindblad ? pr::Smem<B>::gradient_lindblad : pr::Smem<B>::gradient_unitary;
   const dim3 blk(64 * B * B);
   switch (g.V) {
 #define PR_GCASE(VV)                                                                                        \
@@ -413,7 +414,7 @@ int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
   }
   int rc;
   if ((rc = tbegin(pl, st))) return rc;
-  hipLaunchKernelGGL(pr::k_reduce_F, dim3((pl.n_probs + 127) / 128), dim3(128), 0, st, pl.d_probs, pl.n_probs, d_F);
+  hipLaunchKernelGGL(pr::k_reduce_F, dim3(pl.n_probs), dim3(64), 0, st, pl.d_probs, pl.n_probs, d_F);
   HIPCHK(hipGetLastError());
   if (d_G) {
     long long total = pl.n_params;
@@ -562,6 +563,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     g.V = vmax[B];
     g.n_items = (int)items[B].size();
     g.n_members = (int)members[B].size();
+    for (int p = 0; p < n_probs; p++)
+      if ((probs[p].dim + 15) / 16 == B && probs[p].dynamics == PR_LINDBLAD) g.lindblad = true;
     if ((rc = pl.upload(&g.d_items, items[B].data(), items[B].size()))) return rc;
     if ((rc = pl.upload(&g.d_members, members[B].data(), members[B].size()))) return rc;
     pl.groups.push_back(g);
