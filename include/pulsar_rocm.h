@@ -68,7 +68,10 @@ extern "C" {
 
 /* precision of the device path */
 #define PR_FP64 0  /* every product in FP64 (FP64 MFMA) */
-#define PR_MIXED 1 /* exp polynomial beyond first order in split-BF16 MFMA, FP64 chains */
+#define PR_MIXED 1 /* d <= 16 closed dynamics, reference gradient: the exp polynomial
+                      beyond first order and the gradient walk's increments on
+                      FP32 MFMA, accumulated in FP64; generators, forward chain,
+                      traces and reductions FP64. Other problems run as PR_FP64. */
 
 /* gradient formula */
 #define PR_GRAD_REFERENCE 0 /* first-order GRAPE formula of Gradients.jl:274-289 */

@@ -63,6 +63,10 @@ struct ProbDev {
   const double2* PsiT;        // pairs: columns = target states
   long long coff;             // offset into the controls / gradient vectors
   double2* Ust;               // [M][nt][NB][256] U_k^T images
+  float2* Udm;                // mixed precision (d <= 16): [M][nt][256] images of
+                              // D_k = U_k^dag - I in the FP32 own layout
+  unsigned char* big;         // mixed precision: [M][nt] 1 when slice k took the FP64
+                              // exp (its image is in Ust, not Udm)
   double2* Tseg;              // [M][nseg][NB][256]
   double2* Eseg;              // [M][nseg+1][NB][256]
   double2* Om;                // [M][NB][256]
@@ -274,6 +278,42 @@ __device__ inline Blk expm_blk(const Blk& X, int s, bool deg6, double2* LA, doub
   return U;
 }
 
+// exp(X * 2^s) for a single-wavefront anti-Hermitian X (closed dynamics: X =
+// -i dt H with H exactly Hermitian). Every polynomial product has an odd power
+// of X as its left factor, and X^k Y = -(X^k)^dag Y for odd k, so the A operand
+// is read from the factor's own registers (no LDS staging): the same degree-6
+// or degree-9 Paterson-Stockmeyer polynomial as expm_blk. Squarings (s > 0)
+// use the general product.
+__device__ inline Blk expm_ah1(const Blk& X, int s, bool deg6, double2* LA, double2* LB, const Ln& x) {
+  const double c2 = 1.0 / 2, c3 = 1.0 / 6, c4 = 1.0 / 24, c5 = 1.0 / 120, c6 = 1.0 / 720,
+               c7 = 1.0 / 5040, c8 = 1.0 / 40320, c9 = 1.0 / 362880;
+  const Blk X2 = gemm_prod<1, C, N>(LA, X, LB, X, x);   // -X X
+  const Blk X3 = gemm_prod<1, C, N>(LA, X, LB, X2, x);  // -X (-X X) = X^3
+  // X2 above holds -X^2
+  Blk R;
+  if (deg6) {
+    R.re = c6 * X3.re - c5 * X2.re + c4 * X.re;
+    R.im = c6 * X3.im - c5 * X2.im + c4 * X.im;
+    add_diag(R, c3, x);
+  } else {
+    Blk Tm;
+    Tm.re = c9 * X3.re - c8 * X2.re + c7 * X.re;
+    Tm.im = c9 * X3.im - c8 * X2.im + c7 * X.im;
+    add_diag(Tm, c6, x);
+    const Blk P = gemm_prod<1, C, N>(LA, X3, LB, Tm, x);  // -X^3 Tm
+    R.re = -P.re - c5 * X2.re + c4 * X.re;
+    R.im = -P.im - c5 * X2.im + c4 * X.im;
+    add_diag(R, c3, x);
+  }
+  const Blk P = gemm_prod<1, C, N>(LA, X3, LB, R, x);  // -X^3 R
+  Blk U;
+  U.re = -P.re - c2 * X2.re + X.re;
+  U.im = -P.im - c2 * X2.im + X.im;
+  add_diag(U, 1.0, x);
+  for (int i = 0; i < s; i++) U = mm<1, N, N>(U, U, LA, LB, x);
+  return U;
+}
+
 // Frechet derivative L(X 2^s, M) of exp for X already scaled by 2^-s: the
 // degree-9 Paterson-Stockmeyer polynomial evaluated on the block matrix
 // [[X, N], [0, X]] with N = M 2^-s (each block product [[A,B],[0,A]][[C,D],[0,C]]
@@ -393,7 +433,15 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
     X.re *= f;
     X.im *= f;
     smax = max(smax, s);
-    Blk U = expm_blk<B>(X, s, deg6, LA, LB, x);
+    Blk U;
+    if constexpr (B == 1) {
+      if (P.unitary)
+        U = expm_ah1(X, s, deg6, LA, LB, x);
+      else
+        U = expm_blk<1>(X, s, deg6, LA, LB, x);
+    } else {
+      U = expm_blk<B>(X, s, deg6, LA, LB, x);
+    }
     // chain T <- U T, and U^T image from the same LDS tile
     put<B>(LA, U, x);
     if constexpr (B > 1) put<B>(LB, Tseg, x);
@@ -429,6 +477,124 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
     }
   }
   store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * NB * 256, Tseg, x);
+  if (threadIdx.x == 0) P.smax[(size_t)m * P.nseg + it.seg] = smax;
+}
+
+// ------------------------------------------------------ mixed precision, d <= 16
+// exp(X) - I - X for a single-wavefront anti-Hermitian generator of 1-norm at
+// most THETA6, given in the FP32 own layout: the degree-6 Paterson-Stockmeyer
+// products on FP32 MFMA with every A operand from registers (X^k Y =
+// -(X^k)^dag Y for odd k, as in expm_ah1).
+__device__ inline Blk32 expm_rest32_ah(const Blk32& Xo, const Ln& x) {
+  const float c2 = 1.0f / 2, c3 = 1.0f / 6, c4 = 1.0f / 24, c5 = 1.0f / 120, c6 = 1.0f / 720;
+  const Blk32 X2 = prod32_adj(Xo, Xo);  // -X^2
+  const Blk32 X3 = prod32_adj(Xo, X2);  // X^3
+  Blk32 Q;
+  Q.re = c6 * X3.re - c5 * X2.re + c4 * Xo.re;
+  Q.im = c6 * X3.im - c5 * X2.im + c4 * Xo.im;
+  add_diag32(Q, c3, x);
+  const Blk32 P = prod32_adj(X3, Q);  // -X^3 Q
+  Blk32 R;
+  R.re = -P.re - c2 * X2.re;
+  R.im = -P.im - c2 * X2.im;
+  return R;
+}
+
+// k_segment for d <= 16 with the exp polynomial on FP32 MFMA: U_k = I + X_k +
+// R_k with X_k exact in FP64 and only R_k = O(||X||^2) rounded; the chain
+// product and the segment products stay FP64 (they set F). Stores the FP32
+// image of D_k = U_k^dag - I for the mixed gradient walk. A slice whose
+// generator exceeds the degree-6 bound (||X||_1 > THETA6, where the FP32
+// rounding of R_k and of the walk increments would no longer be small) takes
+// the FP64 exp and stores the FP64 image U_k^T instead, flagged in P.big.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SEGMENT_WAVES))) void k_segment_mx(
+    const ProbDev* __restrict__ probs, const WorkItem* __restrict__ items, const double* __restrict__ u) {
+  const WorkItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const Ln x = lane_info<1>();
+  double2 *LA, *LB, *zb;
+  double* red;
+  smem_parts<1>(LA, LB, red, zb);
+  const int m = it.member;
+  double2* LT = LB;
+  Blk Tseg = eye_blk(x);
+  put<1>(LT, Tseg, x);
+  bsync<1>();
+  int smax = 0;
+  Blk Xn = assemble<1>(P, m, it.k0, u, load_drift<1>(P, m, x), x);
+  for (int k = it.k0; k < it.k1; k++) {
+    Blk X = Xn;
+    bool deg6;
+    const int s = choose_scaling(norm1_blk<1>(X, red, x), deg6);
+    const double f = ldexp(1.0, -s);
+    X.re *= f;
+    X.im *= f;
+    smax = max(smax, s);
+    const bool small = deg6;  // degree 6 implies s == 0
+    if (small) {
+      // X into the own FP32 layout (kept FP64 for U = I + X + R)
+      put<1>(LA, X, x);
+      bsync<1>();
+      Blk Xo;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double2 v = LA[lidx(0, rowof32(x.q, r), x.c)];
+        Xo.re[r] = v.x;
+        Xo.im[r] = v.y;
+      }
+      Blk32 X32;
+      X32.re = __builtin_convertvector(Xo.re, f4);
+      X32.im = __builtin_convertvector(Xo.im, f4);
+      const Blk32 R = expm_rest32_ah(X32, x);
+      bsync<1>();
+      Blk Uo;
+      Uo.re = Xo.re + __builtin_convertvector(R.re, d4);
+      Uo.im = Xo.im + __builtin_convertvector(R.im, d4);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        if (rowof32(x.q, r) == x.c) Uo.re[r] += 1.0;
+        LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Uo.re[r], Uo.im[r]);
+      }
+    } else {
+      put<1>(LA, expm_ah1(X, s, false, LA, LB, x), x);
+    }
+    bsync<1>();
+    const Blk none = zero_blk();  // both chain operands are read from LDS
+    const Blk acc = gemm_prod<1, N, NL>(LA, none, LT, none, x);
+    const size_t mk = (size_t)m * P.nt + k;
+    if (small) {
+      Blk32 D;  // D[4q + r][c] = conj(U[c][4q + r]) - delta
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double2 v = LA[lidx(0, x.c, rowof32(x.q, r))];
+        const double dg = (rowof32(x.q, r) == x.c) ? 1.0 : 0.0;
+        D.re[r] = (float)(v.x - dg);
+        D.im[r] = (float)(-v.y);
+      }
+      store_blk32(P.Udm + mk * 256, D, x);
+    } else {
+      Blk Ut;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double2 v = LA[lidx(0, x.c, rowof(x.q, r))];  // U[c][q + 4r]
+        Ut.re[r] = v.x;
+        Ut.im[r] = v.y;
+      }
+      store_blk(P.Ust + mk * 256, Ut, x);
+    }
+    if (threadIdx.x == 0) P.big[mk] = small ? 0 : 1;
+    if (k + 1 < it.k1) Xn = assemble<1>(P, m, k + 1, u, load_drift<1>(P, m, x), x);
+    bsync<1>();
+    put<1>(LT, acc, x);
+  }
+  bsync<1>();
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const double2 v = LT[lidx(0, rowof(x.q, r), x.c)];
+    Tseg.re[r] = v.x;
+    Tseg.im[r] = v.y;
+  }
+  store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * 256, Tseg, x);
   if (threadIdx.x == 0) P.smax[(size_t)m * P.nseg + it.seg] = smax;
 }
 
@@ -544,13 +710,19 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
 // in the LDS tile LA; the workgroup's threads form V groups of G = 64 B^2 / V
 // lanes (at most one wavefront), group g walks the nonzeros of control j0 + g
 // and reduces them inside its lanes.
-template <int B, int V>
+template <int B, int V, bool OWN32 = false>
 __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, double2* LA, const Ln& x) {
   constexpr int THREADS = 64 * B * B;
   constexpr int G = (THREADS / V) < 64 ? (THREADS / V) : 64;
   const int nc = P.nc;
   const bool want_im = (P.cim != 0.0);  // closed: Re(-i z) = Im z
-  put<B>(LA, Mk, x);
+  if constexpr (OWN32) {
+    // M held in the FP32 own layout (rows 4q + r), same LDS positions
+#pragma unroll
+    for (int r = 0; r < 4; r++) LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Mk.re[r], Mk.im[r]);
+  } else {
+    put<B>(LA, Mk, x);
+  }
   bsync<B>();
   const int t = threadIdx.x;
   const int grp = t / G, gl = t % G;
@@ -653,6 +825,113 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT ? PR_GRADIENT_WAVES : 0))) v
     }
     Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
     Lm = mm<B, J, N>(Ut, Lm, LA, LB, x);          // U_k^dag Lambda_k  (U^dag = conj(U^T))
+  }
+}
+
+// k_gradient for d <= 16, unitary dynamics, mixed precision: the walk
+// M_k = C_k U_k^dag = C_k + C_k D_k and C_{k+1} = U_k M_k = M_k + D_k^dag M_k
+// forms the increments C D and D^dag M on FP32 MFMA and accumulates C and M in
+// FP64 (held in the FP32 own layout); the traces are FP64.
+template <int V>
+__global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ probs,
+                                                    const WorkItem* __restrict__ items,
+                                                    const double* __restrict__ u) {
+  const WorkItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const Ln x = lane_info<1>();
+  double2 *LA, *LB, *zb;
+  double* red;
+  smem_parts<1>(LA, LB, red, zb);
+  float2* L32 = reinterpret_cast<float2*>(LA);
+  const int m = it.member;
+  Blk Es = load_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + it.seg) * 256, x);
+  Blk Om = load_blk(P.Om + (size_t)m * 256, x);
+  Blk Y = mm<1, N, N>(Es, Om, LA, LB, x);
+  put<1>(LA, Es, x);
+  bsync<1>();
+  Blk Ed;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    double2 v = LA[lidx(0, x.c, rowof(x.q, r))];  // Es[c][q + 4r]
+    Ed.re[r] = v.x;
+    Ed.im[r] = -v.y;
+  }
+  bsync<1>();
+  const Blk Cs = mm<1, N, N>(Y, Ed, LA, LB, x);
+  // C into the own FP32 layout (kept in FP64)
+  put<1>(LA, Cs, x);
+  bsync<1>();
+  Blk Cm;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    double2 v = LA[lidx(0, rowof32(x.q, r), x.c)];
+    Cm.re[r] = v.x;
+    Cm.im[r] = v.y;
+  }
+  bsync<1>();
+  const unsigned char* big = P.big + (size_t)m * P.nt;
+  Blk32 D;
+  if (!gptr(big)[it.k0]) D = load_blk32(P.Udm + ((size_t)m * P.nt + it.k0) * 256, x);
+  for (int k = it.k0; k < it.k1; k++) {
+    const bool nsmall = k + 1 < it.k1 && !gptr(big)[k + 1];
+    Blk Mk;
+    if (!gptr(big)[k]) {
+      Blk32 C32;
+      C32.re = __builtin_convertvector(Cm.re, f4);
+      C32.im = __builtin_convertvector(Cm.im, f4);
+      put32(L32, C32, x);
+      bsync<1>();
+      const Blk32 P1 = prod32_lds(L32, D, x);  // C D
+      Blk32 Dn = D;
+      if (nsmall) Dn = load_blk32(P.Udm + ((size_t)m * P.nt + k + 1) * 256, x);
+      Mk.re = Cm.re + __builtin_convertvector(P1.re, d4);
+      Mk.im = Cm.im + __builtin_convertvector(P1.im, d4);
+      Blk32 M32;
+      M32.re = __builtin_convertvector(Mk.re, f4);
+      M32.im = __builtin_convertvector(Mk.im, f4);
+      const Blk32 P2 = prod32_adj(D, M32);  // D^dag M = (U - I) M
+      Cm.re = Mk.re + __builtin_convertvector(P2.re, d4);
+      Cm.im = Mk.im + __builtin_convertvector(P2.im, d4);
+      D = Dn;
+      bsync<1>();
+    } else {
+      // FP64 step from the FP64 image: C to the FP64 layout, M = C U^dag,
+      // C' = U M, both back to the own layout
+      const Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * 256, x);
+#pragma unroll
+      for (int r = 0; r < 4; r++) LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Cm.re[r], Cm.im[r]);
+      bsync<1>();
+      Blk Cs64;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double2 v = LA[lidx(0, rowof(x.q, r), x.c)];
+        Cs64.re[r] = v.x;
+        Cs64.im[r] = v.y;
+      }
+      bsync<1>();
+      const Blk M64 = mm<1, N, N>(Cs64, conj_blk(Ut), LA, LB, x);
+      const Blk C64 = mm<1, T, N>(Ut, M64, LA, LB, x);
+      put<1>(LA, M64, x);
+      bsync<1>();
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double2 v = LA[lidx(0, rowof32(x.q, r), x.c)];
+        Mk.re[r] = v.x;
+        Mk.im[r] = v.y;
+      }
+      bsync<1>();
+      put<1>(LA, C64, x);
+      bsync<1>();
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double2 v = LA[lidx(0, rowof32(x.q, r), x.c)];
+        Cm.re[r] = v.x;
+        Cm.im[r] = v.y;
+      }
+      bsync<1>();
+      if (nsmall) D = load_blk32(P.Udm + ((size_t)m * P.nt + k + 1) * 256, x);
+    }
+    contract<1, V, true>(P, m, k, Mk, LA, x);
   }
 }
 

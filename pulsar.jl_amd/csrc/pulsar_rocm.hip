@@ -165,6 +165,26 @@ Trace build_trace(const double* Hc, int nc, int d) {
   return out;
 }
 
+// Copy of n column-major complex d x d matrices made exactly Hermitian from
+// their upper triangles (row <= column), the part Julia's Hermitian(H) reads
+// (the reference exponentiates eigen(Hermitian(H)), Propagators.jl:663).
+std::vector<double> hermitian_upper(const double* a, int n, int d) {
+  const size_t dd = (size_t)d * d;
+  std::vector<double> out(a, a + 2 * dd * n);
+  for (int m = 0; m < n; m++) {
+    double* h = out.data() + 2 * dd * m;
+    for (int c = 0; c < d; c++) {
+      h[2 * ((size_t)c + (size_t)c * d) + 1] = 0.0;
+      for (int r = c + 1; r < d; r++) {  // lower entry (r, c) from upper (c, r)
+        const size_t lo = (size_t)r + (size_t)c * d, up = (size_t)c + (size_t)r * d;
+        h[2 * lo] = h[2 * up];
+        h[2 * lo + 1] = -h[2 * up + 1];
+      }
+    }
+  }
+  return out;
+}
+
 double norm1(const double* a, int d) {
   double best = 0;
   for (int c = 0; c < d; c++) {
@@ -218,6 +238,7 @@ struct Plan {
   struct Group {
     int B = 0, V = 0;
     bool lindblad = false;  // some problem of the group has Lindblad dynamics
+    bool mixed = false;     // FP32-MFMA exp and gradient walk (d <= 16 only)
     pr::WorkItem* d_items = nullptr;
     int n_items = 0;
     int2* d_members = nullptr;
@@ -359,8 +380,25 @@ int tend(Plan& pl, int cls, hipStream_t st) {
   return PR_OK;
 }
 
+int launch_gradient_mixed(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t st) {
+  const size_t sm = pr::Smem<1>::gradient_unitary;
+  switch (g.V) {
+#define PR_MCASE(VV)                                                                                    \
+  case VV:                                                                                              \
+    hipLaunchKernelGGL((pr::k_gradient_mx<VV>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u); \
+    break;
+    PR_MCASE(2) PR_MCASE(4) PR_MCASE(8) PR_MCASE(16) PR_MCASE(32)
+#undef PR_MCASE
+    default:
+      return fail(PR_ERR_LAUNCH, "internal: bad reduction width %d", g.V);
+  }
+  HIPCHK(hipGetLastError());
+  return PR_OK;
+}
+
 template <int B>
 int launch_gradient(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t st) {
+  if (B == 1 && g.mixed) return launch_gradient_mixed(pl, g, d_u, st);
   const size_t sm = g.lindblad ? pr::Smem<B>::gradient_lindblad : pr::Smem<B>::gradient_unitary;
   const dim3 blk(64 * B * B);
   switch (g.V) {
@@ -385,7 +423,10 @@ int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
   const dim3 blk(64 * B * B);
   int rc;
   if ((rc = tbegin(pl, st))) return rc;
-  hipLaunchKernelGGL((pr::k_segment<B>), dim3(g.n_items), blk, pr::Smem<B>::segment, st, pl.d_probs, g.d_items, d_u);
+  if (B == 1 && g.mixed)
+    hipLaunchKernelGGL(pr::k_segment_mx, dim3(g.n_items), blk, pr::Smem<1>::segment, st, pl.d_probs, g.d_items, d_u);
+  else
+    hipLaunchKernelGGL((pr::k_segment<B>), dim3(g.n_items), blk, pr::Smem<B>::segment, st, pl.d_probs, g.d_items, d_u);
   HIPCHK(hipGetLastError());
   if ((rc = tend(pl, 0, st))) return rc;
   if ((rc = tbegin(pl, st))) return rc;
@@ -438,6 +479,12 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   std::vector<int> vmax(5, 1);
   std::vector<long long> group_members(5, 0);
   for (int p = 0; p < n_probs; p++) group_members[(probs[p].dim + 15) / 16] += probs[p].n_members;
+  // the mixed-precision kernels cover single-wavefront (d <= 16) unitary
+  // problems under the reference gradient formula; any other problem of the
+  // plan, or a d <= 16 group holding a Lindblad problem, runs in FP64
+  bool mixed1 = pl.opt.precision == PR_MIXED && pl.opt.grad_mode == PR_GRAD_REFERENCE;
+  for (int p = 0; p < n_probs; p++)
+    if (probs[p].dim <= 16 && probs[p].dynamics != PR_CLOSED) mixed1 = false;
   for (int p = 0; p < n_probs; p++) {
     const pr_problem& q = probs[p];
     pr::ProbDev& D = pl.host_probs[p];
@@ -466,13 +513,23 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     L = std::min(L, nt);
     const int S = (nt + L - 1) / L;
     D.nseg = S; D.seglen = L;
-    // operators
+    // operators; closed dynamics use the Hermitian part of the upper triangle
+    // (validated above to 1e-10), so generators -i dt H are exactly anti-Hermitian
     const int nh0 = D.h0pm ? M : 1;
+    std::vector<double> h0h, hch;
+    const double* H0src = q.H0;
+    const double* Hcsrc = q.Hc;
+    if (q.dynamics == PR_CLOSED) {
+      h0h = hermitian_upper(q.H0, nh0, d);
+      hch = hermitian_upper(q.Hc, nc, d);
+      H0src = h0h.data();
+      Hcsrc = hch.data();
+    }
     std::vector<double2> img(MB * std::max(nh0, nc));
     std::vector<double> nrm(std::max(nh0, nc));
     for (int m = 0; m < nh0; m++) {
-      tile_image(q.H0 + 2 * dd * m, d, B, false, img.data() + MB * m);
-      nrm[m] = norm1(q.H0 + 2 * dd * m, d);
+      tile_image(H0src + 2 * dd * m, d, B, false, img.data() + MB * m);
+      nrm[m] = norm1(H0src + 2 * dd * m, d);
     }
     int rc;
     double2* dp;
@@ -481,17 +538,18 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     D.H0 = dp;
     if ((rc = pl.upload(&dd_, nrm.data(), nh0))) return rc;
     D.n0 = dd_;
-    for (int j = 0; j < nc; j++) nrm[j] = norm1(q.Hc + 2 * dd * j, d);
+    for (int j = 0; j < nc; j++) nrm[j] = norm1(Hcsrc + 2 * dd * j, d);
     if ((rc = pl.upload(&dd_, nrm.data(), nc))) return rc;
     D.ncn = dd_;
     {
-      Csr cs = build_csr(q.Hc, nc, d, B, false);
+      Csr cs = build_csr(Hcsrc, nc, d, B, false);
       int *ip, *jp;
       double2* vp;
       if ((rc = pl.upload(&ip, cs.ptr.data(), cs.ptr.size()))) return rc;
       if ((rc = pl.upload(&jp, cs.j.data(), cs.j.size()))) return rc;
       if ((rc = pl.upload(&vp, cs.v.data(), cs.v.size()))) return rc;
       D.cs_ptr = ip; D.cs_j = jp; D.cs_v = vp;
+      // the gradient contracts the operators as given (Gradients.jl:274-289)
       Trace tr = build_trace(q.Hc, nc, d);
       if ((rc = pl.upload(&ip, tr.ptr.data(), tr.ptr.size()))) return rc;
       if ((rc = pl.upload(&jp, tr.idx.data(), tr.idx.size()))) return rc;
@@ -524,6 +582,11 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     }
     // scratch
     if ((rc = pl.alloc(&D.Ust, (size_t)M * nt * MB))) return rc;
+    if (B == 1 && mixed1) {
+      // FP32 images of small-generator slices; the FP64 images above hold the others
+      if ((rc = pl.alloc(&D.Udm, (size_t)M * nt * 256))) return rc;
+      if ((rc = pl.alloc(&D.big, (size_t)M * nt))) return rc;
+    }
     if ((rc = pl.alloc(&D.Tseg, (size_t)M * S * MB))) return rc;
     if ((rc = pl.alloc(&D.Eseg, (size_t)M * (S + 1) * MB))) return rc;
     if ((rc = pl.alloc(&D.Om, (size_t)M * MB))) return rc;
@@ -565,6 +628,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     g.n_members = (int)members[B].size();
     for (int p = 0; p < n_probs; p++)
       if ((probs[p].dim + 15) / 16 == B && probs[p].dynamics == PR_LINDBLAD) g.lindblad = true;
+    g.mixed = B == 1 && mixed1;
     if ((rc = pl.upload(&g.d_items, items[B].data(), items[B].size()))) return rc;
     if ((rc = pl.upload(&g.d_members, members[B].data(), members[B].size()))) return rc;
     pl.groups.push_back(g);
@@ -636,7 +700,7 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
   *plan = nullptr;
   pr_options o{PR_FP64, PR_GRAD_REFERENCE, 0, -1};
   if (opt) o = *opt;
-  if (o.precision != PR_FP64) return fail(PR_ERR_UNSUPPORTED, "precision %d is not available in this build", o.precision);
+  if (o.precision != PR_FP64 && o.precision != PR_MIXED) return fail(PR_ERR_ARG, "unknown precision %d", o.precision);
   if (o.grad_mode != PR_GRAD_REFERENCE && o.grad_mode != PR_GRAD_EXACT)
     return fail(PR_ERR_ARG, "unknown grad_mode %d", o.grad_mode);
   if (o.segment < 0) return fail(PR_ERR_ARG, "segment must be >= 0");

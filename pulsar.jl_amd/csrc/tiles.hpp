@@ -279,3 +279,95 @@ __device__ inline double wave_reduce_spread(double (&v)[V], int lane) {
 }
 
 }  // namespace pr
+
+// ------------------------------------------------------------ FP32 products
+// Mixed-precision form for single-wavefront matrices (d <= 16). The result
+// layout of v_mfma_f32_16x16x4f32 is lane c + 16q, register r = element
+// (row 4q + r, column c) — the "own" layout of Blk32. Its k-step kb sums over
+// k = 4q + kb, so a Blk32 in registers is directly the B operand; the A
+// operand comes from an LDS tile of float2 with the same padded indexing as
+// the FP64 tiles (lidx), or from the registers of a matrix whose conjugate
+// transpose is wanted.
+namespace pr {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+struct Blk32 {
+  f4 re, im;
+};
+
+__device__ inline f4 mfma32(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// row of the own FP32 layout held by lane-group q in register r
+__device__ constexpr int rowof32(int q, int r) { return 4 * q + r; }
+
+typedef float fv2 __attribute__((ext_vector_type(2)));
+__device__ inline Blk32 load_blk32(const float2* __restrict__ img, const Ln& x) {
+  Blk32 b;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    fv2 v = ((const PR_GLOBAL fv2*)img)[r * 64 + x.lane];
+    b.re[r] = v.x;
+    b.im[r] = v.y;
+  }
+  return b;
+}
+__device__ inline void store_blk32(float2* __restrict__ img, const Blk32& b, const Ln& x) {
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    fv2 v;
+    v.x = b.re[r];
+    v.y = b.im[r];
+    ((PR_GLOBAL fv2*)img)[r * 64 + x.lane] = v;
+  }
+}
+
+// own-layout FP32 block into an FP32 tile
+__device__ inline void put32(float2* L, const Blk32& b, const Ln& x) {
+#pragma unroll
+  for (int r = 0; r < 4; r++) L[lidx(0, rowof32(x.q, r), x.c)] = make_float2(b.re[r], b.im[r]);
+}
+// A * Bown with A from the FP32 tile L (Gauss's three real products per k-step)
+__device__ inline Blk32 prod32_lds(const float2* L, const Blk32& Bo, const Ln& x) {
+  f4 t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) {
+    const float2 a = L[lidx(0, x.c, rowof32(x.q, kb))];  // A[c][4q + kb]
+    const float br = Bo.re[kb], bi = Bo.im[kb];
+    t1 = mfma32(a.x, br, t1);
+    t2 = mfma32(a.y, bi, t2);
+    t3 = mfma32(a.x + a.y, br + bi, t3);
+  }
+  Blk32 r;
+  r.re = t1 - t2;
+  r.im = t3 - t1 - t2;
+  return r;
+}
+
+// D^dag * Bown, D in the own layout (lane c + 16q, register kb holds
+// D[4q + kb][c] = conj(D^dag[c][4q + kb]): the A operand without LDS)
+__device__ inline Blk32 prod32_adj(const Blk32& D, const Blk32& Bo) {
+  f4 t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) {
+    const float ar = D.re[kb], ai = -D.im[kb];
+    const float br = Bo.re[kb], bi = Bo.im[kb];
+    t1 = mfma32(ar, br, t1);
+    t2 = mfma32(ai, bi, t2);
+    t3 = mfma32(ar + ai, br + bi, t3);
+  }
+  Blk32 r;
+  r.re = t1 - t2;
+  r.im = t3 - t1 - t2;
+  return r;
+}
+
+__device__ inline void add_diag32(Blk32& X, float v, const Ln& x) {
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+    if (rowof32(x.q, r) == x.c) X.re[r] += v;
+}
+
+}  // namespace pr

@@ -1,0 +1,137 @@
+"""Mixed-precision device path (PR_MIXED) against the FP64 oracle.
+
+Acceptance bar of the mixed form: |dF| <= 1e-9 and max|dG| / max|G| <= 1e-6
+against the FP64 oracle on the same inputs. The mixed kernels cover d <= 16
+closed-system problems under the reference gradient formula; everything else
+in a mixed plan runs the FP64 kernels and keeps the FP64 tolerances.
+"""
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+import oracle as O
+from pulsar_rocm import native, workloads as W
+from pulsar_rocm.problem import PR_FP64, PR_GATE, PR_LINDBLAD, PR_MIXED, PR_PAIRS_SQUARE, Problem
+
+pytestmark = pytest.mark.gpu
+
+TOL_F = 1e-9
+TOL_G = 1e-6
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    native.init(0)
+
+
+def run(p, u, **kw):
+    plan = native.Plan([p], **kw)
+    try:
+        F, G, D = plan.eval([u], diag=True)
+    finally:
+        plan.close()
+    return F[0], G[0], D[0]
+
+
+def errors(p, u, **kw):
+    print(f"case d={p.dim} M={p.n_members} n_t={p.n_t} obj={p.objective} {kw}", flush=True)
+    F0, G0 = O.evaluate(p, u)
+    F, G, D = run(p, u, precision=PR_MIXED, **kw)
+    return abs(F - F0), np.abs(G - G0).max() / np.abs(G0).max(), F0, D
+
+
+def test_four_qubit_ensemble_full_length_member_sample():
+    # the benchmark workload (random controls, Haar target): all 2000 slices, 16 members
+    p, u = W.four_qubit_ensemble()
+    p, u = W.shrink(p, u, n_members=16)
+    dF, dG, _, D = errors(p, u)
+    print("mixed C2 sample: dF", dF, "relG", dG, flush=True)
+    assert dF <= TOL_F and dG <= TOL_G
+    # ||U^dag U - I||: the FP32 rounding of R_k = exp(X_k) - I - X_k (~1e-11 per
+    # slice) accumulates over 2000 slices; the FP64 form reaches ~1e-13
+    print("unitarity defect", D[0], flush=True)
+    assert D[0] < 1e-8
+
+
+def test_near_optimum_target():
+    # a target 0.15 away (in generator norm) from member 0's U_total: F ~ 0.99
+    p, u = W.four_qubit_ensemble()
+    p, u = W.shrink(p, u, n_members=4)
+    rng = np.random.default_rng(5)
+    H = rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16))
+    H = (H + H.conj().T) / 2
+    H /= np.linalg.norm(H, 2)
+    p.U_target = O.total_propagator(p, u, member=0) @ expm(-1j * 0.15 * H)
+    dF, dG, F0, _ = errors(p, u)
+    print("mixed near optimum: F", F0, "dF", dF, "relG", dG, flush=True)
+    assert F0 > 0.5
+    assert dF <= TOL_F and dG <= TOL_G
+
+
+@pytest.mark.parametrize("d", [2, 4, 8, 16])
+def test_small_dimensions_and_segments(d):
+    rng = np.random.default_rng(300 + d)
+    nq = int(np.log2(d))
+    H0 = W.random_hermitian(d, rng, 2 * np.pi * 100)
+    Hc = np.stack([2 * np.pi * 50 * W.embed(s, nq, q) / 2 for q in range(nq) for s in (W.SX, W.SY)])
+    nc = Hc.shape[0]
+    u = rng.uniform(-0.5, 0.5, (nc, 500))
+    p = Problem(dim=d, n_ctrl=nc, n_t=500, H0=H0, Hc=Hc, dt=1e-3, objective=PR_GATE, n_members=3,
+                ctrl_scale=rng.uniform(0.9, 1.1, (3, nc)), U_target=W.haar_unitary(d, rng))
+    for seg in (0, 37):
+        dF, dG, _, _ = errors(p, u, segment=seg)
+        assert dF <= TOL_F and dG <= TOL_G, (seg, dF, dG)
+
+
+def test_scaling_and_squaring_in_mixed_mode():
+    # generators above the degree-6 bound take degree 9 plus squarings
+    rng = np.random.default_rng(9)
+    d = 8
+    p = Problem(dim=d, n_ctrl=2, n_t=40, H0=W.random_hermitian(d, rng, 4.0),
+                Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)]), dt=0.1,
+                objective=PR_GATE, U_target=W.haar_unitary(d, rng))
+    u = rng.uniform(-1, 1, (2, 40))
+    dF, dG, _, D = errors(p, u)
+    assert D[1] >= 1
+    assert dF <= TOL_F and dG <= TOL_G
+
+
+def test_state_pairs_in_mixed_mode():
+    p, u = W.single_qubit_state(psi_init=[1, 0])
+    dF, dG, _, _ = errors(p, u)
+    assert dF <= TOL_F and dG <= TOL_G
+
+
+def test_mixed_plan_runs_other_problems_in_fp64():
+    # a d=32 gate problem and a d=4 Lindblad problem in a mixed plan keep FP64 results
+    rng = np.random.default_rng(11)
+    d = 32
+    big = Problem(dim=d, n_ctrl=2, n_t=30, H0=W.random_hermitian(d, rng, 3.0),
+                  Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)]), dt=0.05,
+                  objective=PR_GATE, U_target=W.haar_unitary(d, rng))
+    ub = rng.uniform(-1, 1, (2, 30))
+    sm = np.array([[0, 1], [0, 0]], complex)
+    L0 = W.drift_liouvillian(2 * np.pi * 0.3 * W.SZ / 2, [sm], [0.2])
+    Lc = np.stack([W.control_liouvillian(W.SX / 2), W.control_liouvillian(W.SY / 2)])
+    lind = Problem(dim=4, n_ctrl=2, n_t=30, H0=L0, Hc=Lc, dt=0.05, dynamics=PR_LINDBLAD,
+                   objective=PR_PAIRS_SQUARE, n_members=2, ctrl_scale=np.array([[1.0, 1.0], [0.95, 0.95]]),
+                   psi_init=np.array([[1, 0, 0, 0]], complex), psi_targ=np.array([[0, 0, 0, 1]], complex))
+    ul = rng.uniform(-2, 2, (2, 30))
+    plan = native.Plan([big, lind], precision=PR_MIXED)
+    try:
+        F, G = plan.eval([ub, ul])
+    finally:
+        plan.close()
+    for q, uu, f, g in ((big, ub, F[0], G[0]), (lind, ul, F[1], G[1])):
+        F0, G0 = O.evaluate(q, uu)
+        assert abs(f - F0) <= 1e-12
+        assert np.abs(g - G0).max() <= 1e-10 * np.abs(G0).max()
+
+
+def test_mixed_matches_fp64_device_result_closely():
+    p, u = W.four_qubit_ensemble(n_t=200)
+    p, u = W.shrink(p, u, n_members=8)
+    F64, G64, _ = run(p, u, precision=PR_FP64)
+    Fm, Gm, _ = run(p, u, precision=PR_MIXED)
+    assert abs(Fm - F64) <= 1e-10
+    assert np.abs(Gm - G64).max() <= 1e-7 * np.abs(G64).max()
