@@ -401,7 +401,9 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
 #ifndef PR_GRADIENT_WAVES
 #define PR_GRADIENT_WAVES 0
 #endif
-template <int B>
+// CLOSED: every problem of the launch has closed dynamics (anti-Hermitian
+// generators), so the d <= 16 exp takes the register-only form unconditionally.
+template <int B, bool CLOSED>
 __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B == 1 ? PR_SEGMENT_WAVES : 0))) void k_segment(const ProbDev* __restrict__ probs,
                                                          const WorkItem* __restrict__ items,
                                                          const double* __restrict__ u) {
@@ -434,7 +436,9 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
     X.im *= f;
     smax = max(smax, s);
     Blk U;
-    if constexpr (B == 1) {
+    if constexpr (B == 1 && CLOSED) {
+      U = expm_ah1(X, s, deg6, LA, LB, x);
+    } else if constexpr (B == 1) {
       if (P.unitary)
         U = expm_ah1(X, s, deg6, LA, LB, x);
       else

@@ -335,7 +335,8 @@ template <int B>
 int set_smem_attr() {
   size_t bytes = pr::Smem<B>::max;
   if (bytes > 65536) {
-    HIPCHK(hipFuncSetAttribute((const void*)pr::k_segment<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_segment<B, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_segment<B, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     HIPCHK(hipFuncSetAttribute((const void*)pr::k_combine<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     int rc;
     if ((rc = set_grad_attr<B, 2>(bytes)) || (rc = set_grad_attr<B, 4>(bytes)) || (rc = set_grad_attr<B, 8>(bytes)) ||
@@ -425,8 +426,10 @@ int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
   if ((rc = tbegin(pl, st))) return rc;
   if (B == 1 && g.mixed)
     hipLaunchKernelGGL(pr::k_segment_mx, dim3(g.n_items), blk, pr::Smem<1>::segment, st, pl.d_probs, g.d_items, d_u);
+  else if (g.lindblad)
+    hipLaunchKernelGGL((pr::k_segment<B, false>), dim3(g.n_items), blk, pr::Smem<B>::segment, st, pl.d_probs, g.d_items, d_u);
   else
-    hipLaunchKernelGGL((pr::k_segment<B>), dim3(g.n_items), blk, pr::Smem<B>::segment, st, pl.d_probs, g.d_items, d_u);
+    hipLaunchKernelGGL((pr::k_segment<B, true>), dim3(g.n_items), blk, pr::Smem<B>::segment, st, pl.d_probs, g.d_items, d_u);
   HIPCHK(hipGetLastError());
   if ((rc = tend(pl, 0, st))) return rc;
   if ((rc = tbegin(pl, st))) return rc;

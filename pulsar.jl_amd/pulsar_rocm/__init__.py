@@ -9,8 +9,12 @@ from .api import (BatchedEnsemble, ControlSequence, EnsembleObjective, GRAPEConf
                   OptimizationResult, QuantumSystem, QuantumTarget, adapt_grape_step_size,
                   available_devices, build_ensemble_from_systems, check_grape_convergence,
                   compute_fidelity, compute_grape_gradient, ensemble_value, ensemble_value_and_grad,
-                  get_device, grape_optimize, grape_step, quantum_system, random_controls, set_device,
-                  state_target, unitary_target, with_device, zero_controls)
+                  evaluate_fidelity, get_device, grape_gradient, grape_optimize, grape_optimize_ensemble,
+                  grape_step, quantum_system, random_controls, set_device, state_target, unitary_target,
+                  with_device, zero_controls)
+from .verify import (AutoDiffConfig, compute_gradient_autodiff, finite_diff_gradient_ad,  # noqa: F401
+                     finite_difference_gradient, grape_gradient_check, select_autodiff_backend,
+                     verify_gradient, verify_gradient_autodiff)
 from .mr import (LindbladMRControl, MRControl, build_control_liouvillian,  # noqa: F401
                  build_drift_liouvillian, grape_lindblad_kernel, grape_state_kernel,
                  pure_state_to_vec_rho, vec_rho)

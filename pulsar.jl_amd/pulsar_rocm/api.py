@@ -9,6 +9,8 @@ for this path; every fidelity and gradient is computed by libpulsar_rocm.so:
   compute_fidelity(sys, ctrl, tgt)                  src/Physics/Objectives.jl:609-639
   compute_grape_gradient(sys, ctrl, tgt)            src/Physics/Gradients.jl:80-127
   GRAPEConfig, OptimizationResult, grape_optimize   src/Optimization/GRAPE.jl:60-211, 469-643
+  grape_optimize_ensemble (L-BFGS)                  src/Optimization/GRAPE.jl:1012-1230
+  grape_gradient, evaluate_fidelity                 src/Optimization/GRAPE.jl:1462-1490
   grape_step!, adapt_grape_step_size,
   check_grape_convergence                           src/Optimization/GRAPE.jl:261-402
   EnsembleObjective, ensemble_value(_and_grad),
@@ -496,6 +498,20 @@ def grape_optimize(system, target: QuantumTarget, controls_init: ControlSequence
                                "n_timesteps": n_t, "dt": dt, "total_time": dt * n_t})
 
 
+def grape_gradient(system, target: QuantumTarget, u, dt: float, parallel: bool = False) -> np.ndarray:
+    """grape_gradient (GRAPE.jl:1462-1472): compute_grape_gradient on a raw
+    [n_c x n_t] control matrix; `parallel` is accepted and ignored, as there."""
+    u = np.asarray(u, dtype=np.float64)
+    return compute_grape_gradient(system, ControlSequence(u.T, float(dt), u.shape[1]), target)
+
+
+def evaluate_fidelity(system, target: QuantumTarget, u, dt: float) -> float:
+    """evaluate_fidelity (GRAPE.jl:1474-1490): compute_fidelity of the total
+    propagator of a raw [n_c x n_t] control matrix."""
+    u = np.asarray(u, dtype=np.float64)
+    return compute_fidelity(system, ControlSequence(u.T, float(dt), u.shape[1]), target)
+
+
 # --------------------------------------------------------------------------- ensembles
 
 
@@ -555,7 +571,8 @@ def ensemble_value(obj: EnsembleObjective, theta) -> float:
     if obj.resample is not None:
         obj.resample(obj)
     if obj.batched_eval is not None:
-        return float(obj.batched_eval(theta)[0])
+        value = getattr(obj.batched_eval, "value", None)
+        return float(value(theta)) if value is not None else float(obj.batched_eval(theta)[0])
     fv = np.array([f(theta) for f in obj.f_samples])
     return _agg_value(fv, obj.aggregator, obj.cvar_alpha)
 
@@ -640,6 +657,12 @@ class BatchedEnsemble:
         F, G = self.plan.eval([u] * len(self.probs), grad=True)
         return float(F.sum()), np.sum(G, axis=0)
 
+    def value(self, theta) -> float:
+        """F only (the gradient walk is skipped), for line searches."""
+        u = np.asarray(theta, dtype=np.float64).reshape(self.n_c, self.n_t, order="F")
+        F, _ = self.plan.eval([u] * len(self.probs), grad=False)
+        return float(F.sum())
+
     def close(self):
         self.plan.close()
 
@@ -673,3 +696,138 @@ def build_ensemble_from_systems(systems, target: QuantumTarget, controls: Contro
     fs, gs = zip(*[mk(s) for s in systems])
     return EnsembleObjective(list(fs), grad_samples=list(gs), aggregator=aggregator, cvar_alpha=cvar_alpha,
                              n_samples=len(systems))
+
+
+# --------------------------------------------------------------------------- ensemble L-BFGS
+
+
+def _lbfgs_direction(g, s_list, y_list):
+    """L-BFGS two-loop recursion (Nocedal & Wright Alg. 7.4) returning an ascent
+    direction, with the safeguards of GRAPE.jl:1062-1080."""
+    k = len(s_list)
+    if k == 0:
+        return g.copy()
+    q = g.copy()
+    alphas = np.zeros(k)
+    for i in range(k - 1, -1, -1):
+        rho = 1.0 / max(float(np.dot(y_list[i], s_list[i])), 1e-14)
+        alphas[i] = rho * float(np.dot(s_list[i], q))
+        q -= alphas[i] * y_list[i]
+    gamma = float(np.dot(s_list[-1], y_list[-1])) / max(float(np.dot(y_list[-1], y_list[-1])), 1e-14)
+    q *= gamma
+    for i in range(k):
+        rho = 1.0 / max(float(np.dot(y_list[i], s_list[i])), 1e-14)
+        beta = rho * float(np.dot(y_list[i], q))
+        q += (alphas[i] - beta) * s_list[i]
+    return q
+
+
+def _ensemble_lbfgs(F_fn, G_fn, controls0, config, lbfgs_memory=10, amplitude_limit=np.inf):
+    """The loop of grape_optimize_ensemble (GRAPE.jl:1012-1230) over an
+    evaluator pair F_fn(u) -> float, G_fn(u) -> [n_c x n_t]: normalised L-BFGS
+    ascent direction, backtracking line search accepting F_new >= F0 (step
+    halved up to 20 times), curvature pairs kept when y.s > 1e-14 |s|^2,
+    amplitude clipping, and the reference's three convergence tests."""
+    controls = np.array(controls0, dtype=np.float64)
+    f_hist, g_hist = [], []
+    n_f = n_g = 0
+    s_list, y_list = [], []
+    clip = np.isfinite(amplitude_limit)
+
+    def line_search(u, d, F0, a0, max_bt=20):
+        a, n_ls = a0, 0
+        for _ in range(max_bt):
+            un = u + a * d
+            if clip:
+                np.clip(un, -amplitude_limit, amplitude_limit, out=un)
+            Fn = F_fn(un)
+            n_ls += 1
+            if Fn >= F0:
+                return a, un, Fn, n_ls
+            a *= 0.5
+        un = u + a * d
+        if clip:
+            np.clip(un, -amplitude_limit, amplitude_limit, out=un)
+        return a, un, F_fn(un), max_bt + 1
+
+    F = F_fn(controls)
+    n_f += 1
+    f_hist.append(F)
+    best_F, best = F, controls.copy()
+    converged, reason = False, "maximum iterations reached"
+    for it in range(1, config.max_iter + 1):
+        G = np.asarray(G_fn(controls), dtype=np.float64)
+        n_g += 1
+        g = G.reshape(-1, order="F")
+        gnorm = float(np.linalg.norm(g))
+        g_hist.append(gnorm)
+        d = _lbfgs_direction(g, s_list, y_list)
+        dmax = float(np.max(np.abs(d)))
+        if dmax < 1e-14:
+            converged, reason = True, "zero gradient direction"
+            break
+        d /= dmax
+        dm = d.reshape(controls.shape, order="F")
+        a0 = config.step_size * amplitude_limit if clip else config.step_size
+        a, cn, Fn, n_ls = line_search(controls, dm, F, a0)
+        n_f += n_ls
+        Gn = np.asarray(G_fn(cn), dtype=np.float64)
+        n_g += 1
+        sv = a * d
+        yv = Gn.reshape(-1, order="F") - g
+        if float(np.dot(yv, sv)) > 1e-14 * float(np.dot(sv, sv)):
+            s_list.append(sv)
+            y_list.append(yv)
+            if len(s_list) > lbfgs_memory:
+                s_list.pop(0)
+                y_list.pop(0)
+        controls, F = cn, Fn
+        f_hist.append(F)
+        if F > best_F:
+            best_F, best = F, controls.copy()
+        if config.verbose and (it % config.print_interval == 0 or it == 1):
+            print(f"[Ensemble L-BFGS-GRAPE] iter={it:4d}  F_bb={F:.8f}  a={a:.3e}  |dF|={gnorm:.3e}")
+        if config.callback is not None:
+            config.callback(it, F, gnorm, controls)
+        if gnorm < config.gradient_norm_tol:
+            converged, reason = True, f"gradient norm < {config.gradient_norm_tol}"
+            break
+        if F >= 1.0 - 1e-8:
+            converged, reason = True, "near-perfect fidelity"
+            break
+        if len(f_hist) >= 2 and abs(f_hist[-1] - f_hist[-2]) < config.convergence_tol:
+            converged, reason = True, f"fidelity change < {config.convergence_tol}"
+            break
+    return best, best_F, f_hist, g_hist, converged, reason, n_f, n_g
+
+
+def grape_optimize_ensemble(systems, target: QuantumTarget, controls_init: ControlSequence,
+                            config: Optional[GRAPEConfig] = None, lbfgs_memory: int = 10,
+                            amplitude_limit: float = np.inf) -> OptimizationResult:
+    """grape_optimize_ensemble (GRAPE.jl:1012-1230): broadband L-BFGS-GRAPE over
+    an ensemble of systems, maximising the ensemble-mean fidelity. Every F and
+    gradient is one batched device evaluation of the whole ensemble (line-search
+    steps evaluate F only)."""
+    config = config or GRAPEConfig()
+    if len(systems) == 0:
+        raise ValueError("systems must be non-empty")
+    if lbfgs_memory <= 0:
+        raise ValueError("lbfgs_memory must be positive")
+    t0 = time.time()
+    obj = build_ensemble_from_systems(systems, target, controls_init, aggregator="mean")
+    shape = controls_init.controls.shape
+
+    def F_fn(u):
+        return ensemble_value(obj, u.reshape(-1, order="F"))
+
+    def G_fn(u):
+        return ensemble_value_and_grad(obj, u.reshape(-1, order="F"))[1].reshape(shape, order="F")
+
+    try:
+        best, best_F, f_hist, g_hist, conv, reason, n_f, n_g = _ensemble_lbfgs(
+            F_fn, G_fn, controls_init.controls, config, lbfgs_memory, amplitude_limit)
+    finally:
+        obj.batched_eval.close()
+    return OptimizationResult(best, best_F, f_hist, g_hist, len(f_hist), conv, reason, time.time() - t0, n_f, n_g,
+                              {"algorithm": "Ensemble L-BFGS-GRAPE", "n_systems": len(systems),
+                               "lbfgs_memory": lbfgs_memory})
