@@ -53,6 +53,12 @@ struct ProbDev {
   const int* tr_ptr;
   const int* tr_idx;
   const double2* tr_v;
+  // mixed precision (d <= 16): drift images and control entries in the FP32
+  // own layout (register r of lane c + 16q = element (4q + r, c))
+  const double2* H0o;
+  const int* co_ptr;
+  const int* co_j;
+  const double2* co_v;
   const double* n0;           // [M or 1] ||A0||_1
   const double* ncn;          // [nc] ||A_j||_1
   const double* scale;        // [M][nc]
@@ -152,21 +158,24 @@ __device__ inline double block_max(double v, double* red, const Ln& x) {
 
 // Generator of slice k, X = c dt_k (A0 + sum_j s_mj u_jk A_j). H0 is the
 // wavefront's block of the member's drift.
-template <int B>
+template <int B, bool OWN32 = false>
 __device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __restrict__ u, const Blk& H0,
                                const Ln& x) {
   const int nc = P.nc;
   const double* sc = P.scale + (size_t)m * nc;
   const double* uk = u + P.coff + (size_t)k * nc;
   const double dt = P.dts[k];
+  const int* cptr = OWN32 ? P.co_ptr : P.cs_ptr;
+  const int* cj = OWN32 ? P.co_j : P.cs_j;
+  const double2* cv = OWN32 ? P.co_v : P.cs_v;
   Blk H = H0;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int pos = (x.w * 4 + r) * 64 + x.lane;
-    const int e1 = gptr(P.cs_ptr)[pos + 1];
-    for (int e = gptr(P.cs_ptr)[pos]; e < e1; e++) {
-      const int j = gptr(P.cs_j)[e];
-      const double2 a = gload2(P.cs_v, e);
+    const int e1 = gptr(cptr)[pos + 1];
+    for (int e = gptr(cptr)[pos]; e < e1; e++) {
+      const int j = gptr(cj)[e];
+      const double2 a = gload2(cv, e);
       const double coef = gptr(sc)[j] * gptr(uk)[j];
       H.re[r] += coef * a.x;
       H.im[r] += coef * a.y;
@@ -284,10 +293,11 @@ __device__ inline Blk expm_blk(const Blk& X, int s, bool deg6, double2* LA, doub
 // is read from the factor's own registers (no LDS staging): the same degree-6
 // or degree-9 Paterson-Stockmeyer polynomial as expm_blk. Squarings (s > 0)
 // use the general product.
-__device__ inline Blk expm_ah1(const Blk& X, int s, bool deg6, double2* LA, double2* LB, const Ln& x) {
+// The same polynomial given X2 = X^dag X = -X^2 (computed ahead by the caller).
+__device__ inline Blk expm_ah1_from2(const Blk& X, const Blk& X2, int s, bool deg6, double2* LA, double2* LB,
+                                     const Ln& x) {
   const double c2 = 1.0 / 2, c3 = 1.0 / 6, c4 = 1.0 / 24, c5 = 1.0 / 120, c6 = 1.0 / 720,
                c7 = 1.0 / 5040, c8 = 1.0 / 40320, c9 = 1.0 / 362880;
-  const Blk X2 = gemm_prod<1, C, N>(LA, X, LB, X, x);   // -X X
   const Blk X3 = gemm_prod<1, C, N>(LA, X, LB, X2, x);  // -X (-X X) = X^3
   // X2 above holds -X^2
   Blk R;
@@ -312,6 +322,10 @@ __device__ inline Blk expm_ah1(const Blk& X, int s, bool deg6, double2* LA, doub
   add_diag(U, 1.0, x);
   for (int i = 0; i < s; i++) U = mm<1, N, N>(U, U, LA, LB, x);
   return U;
+}
+
+__device__ inline Blk expm_ah1(const Blk& X, int s, bool deg6, double2* LA, double2* LB, const Ln& x) {
+  return expm_ah1_from2(X, gemm_prod<1, C, N>(LA, X, LB, X, x), s, deg6, LA, LB, x);
 }
 
 // Frechet derivative L(X 2^s, M) of exp for X already scaled by 2^-s: the
@@ -397,6 +411,11 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
 // occupancy-bound.
 #ifndef PR_SEGMENT_WAVES
 #define PR_SEGMENT_WAVES 4
+#endif
+// the mixed segment walk holds FP32 and FP64 operands of two slices; at a
+// 4-wave cap it spills (244 B per lane) and runs slower than at 3
+#ifndef PR_MIXED_SEGMENT_WAVES
+#define PR_MIXED_SEGMENT_WAVES 3
 #endif
 #ifndef PR_GRADIENT_WAVES
 #define PR_GRADIENT_WAVES 0
@@ -485,34 +504,19 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
 }
 
 // ------------------------------------------------------ mixed precision, d <= 16
-// exp(X) - I - X for a single-wavefront anti-Hermitian generator of 1-norm at
-// most THETA6, given in the FP32 own layout: the degree-6 Paterson-Stockmeyer
-// products on FP32 MFMA with every A operand from registers (X^k Y =
-// -(X^k)^dag Y for odd k, as in expm_ah1).
-__device__ inline Blk32 expm_rest32_ah(const Blk32& Xo, const Ln& x) {
-  const float c2 = 1.0f / 2, c3 = 1.0f / 6, c4 = 1.0f / 24, c5 = 1.0f / 120, c6 = 1.0f / 720;
-  const Blk32 X2 = prod32_adj(Xo, Xo);  // -X^2
-  const Blk32 X3 = prod32_adj(Xo, X2);  // X^3
-  Blk32 Q;
-  Q.re = c6 * X3.re - c5 * X2.re + c4 * Xo.re;
-  Q.im = c6 * X3.im - c5 * X2.im + c4 * Xo.im;
-  add_diag32(Q, c3, x);
-  const Blk32 P = prod32_adj(X3, Q);  // -X^3 Q
-  Blk32 R;
-  R.re = -P.re - c2 * X2.re;
-  R.im = -P.im - c2 * X2.im;
-  return R;
-}
-
 // k_segment for d <= 16 with the exp polynomial on FP32 MFMA: U_k = I + X_k +
 // R_k with X_k exact in FP64 and only R_k = O(||X||^2) rounded; the chain
-// product and the segment products stay FP64 (they set F). Stores the FP32
+// product and the segment products stay FP64 (they set F). Generators are
+// assembled directly in the FP32 own layout, and the loop is pipelined across
+// slices: the FP64 chain product of slice k runs on the matrix cores while
+// slice k+1 is assembled and its first FP32 product issued. Stores the FP32
 // image of D_k = U_k^dag - I for the mixed gradient walk. A slice whose
 // generator exceeds the degree-6 bound (||X||_1 > THETA6, where the FP32
 // rounding of R_k and of the walk increments would no longer be small) takes
 // the FP64 exp and stores the FP64 image U_k^T instead, flagged in P.big.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SEGMENT_WAVES))) void k_segment_mx(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEGMENT_WAVES))) void k_segment_mx(
     const ProbDev* __restrict__ probs, const WorkItem* __restrict__ items, const double* __restrict__ u) {
+  const float c2 = 1.0f / 2, c3 = 1.0f / 6, c4 = 1.0f / 24, c5 = 1.0f / 120, c6 = 1.0f / 720;
   const WorkItem it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
   const Ln x = lane_info<1>();
@@ -521,50 +525,59 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SEGMENT_W
   smem_parts<1>(LA, LB, red, zb);
   const int m = it.member;
   double2* LT = LB;
-  Blk Tseg = eye_blk(x);
-  put<1>(LT, Tseg, x);
+  put<1>(LT, eye_blk(x), x);
   bsync<1>();
+  const double2* H0o = P.H0o + (P.h0pm ? (size_t)m * 256 : 0);
   int smax = 0;
-  Blk Xn = assemble<1>(P, m, it.k0, u, load_drift<1>(P, m, x), x);
-  for (int k = it.k0; k < it.k1; k++) {
-    Blk X = Xn;
-    bool deg6;
-    const int s = choose_scaling(norm1_blk<1>(X, red, x), deg6);
+  // slice k0: generator (own layout), scaling, first FP32 product
+  Blk Xo = assemble<1, true>(P, m, it.k0, u, load_blk(H0o, x), x);
+  bool deg6;
+  int s = choose_scaling(norm1_blk<1>(Xo, red, x), deg6);
+  {
     const double f = ldexp(1.0, -s);
-    X.re *= f;
-    X.im *= f;
+    Xo.re *= f;
+    Xo.im *= f;
+  }
+  Blk32 X32, X2;
+  X32.re = __builtin_convertvector(Xo.re, f4);
+  X32.im = __builtin_convertvector(Xo.im, f4);
+  X2 = prod32_adj(X32, X32);  // -X^2
+  for (int k = it.k0; k < it.k1; k++) {
     smax = max(smax, s);
     const bool small = deg6;  // degree 6 implies s == 0
     if (small) {
-      // X into the own FP32 layout (kept FP64 for U = I + X + R)
-      put<1>(LA, X, x);
-      bsync<1>();
-      Blk Xo;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const double2 v = LA[lidx(0, rowof32(x.q, r), x.c)];
-        Xo.re[r] = v.x;
-        Xo.im[r] = v.y;
-      }
-      Blk32 X32;
-      X32.re = __builtin_convertvector(Xo.re, f4);
-      X32.im = __builtin_convertvector(Xo.im, f4);
-      const Blk32 R = expm_rest32_ah(X32, x);
-      bsync<1>();
+      const Blk32 X3 = prod32_adj(X32, X2);  // X^3
+      Blk32 Q;
+      Q.re = c6 * X3.re - c5 * X2.re + c4 * X32.re;
+      Q.im = c6 * X3.im - c5 * X2.im + c4 * X32.im;
+      add_diag32(Q, c3, x);
+      const Blk32 Pq = prod32_adj(X3, Q);  // -X^3 Q
       Blk Uo;
-      Uo.re = Xo.re + __builtin_convertvector(R.re, d4);
-      Uo.im = Xo.im + __builtin_convertvector(R.im, d4);
+      Uo.re = Xo.re - __builtin_convertvector(Pq.re + c2 * X2.re, d4);
+      Uo.im = Xo.im - __builtin_convertvector(Pq.im + c2 * X2.im, d4);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         if (rowof32(x.q, r) == x.c) Uo.re[r] += 1.0;
         LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Uo.re[r], Uo.im[r]);
       }
     } else {
+      // FP64 exp from the FP64 layout of X
+#pragma unroll
+      for (int r = 0; r < 4; r++) LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Xo.re[r], Xo.im[r]);
+      bsync<1>();
+      Blk X;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double2 v = LA[lidx(0, rowof(x.q, r), x.c)];
+        X.re[r] = v.x;
+        X.im[r] = v.y;
+      }
+      bsync<1>();
       put<1>(LA, expm_ah1(X, s, false, LA, LB, x), x);
     }
     bsync<1>();
-    const Blk none = zero_blk();  // both chain operands are read from LDS
-    const Blk acc = gemm_prod<1, N, NL>(LA, none, LT, none, x);
+    const Blk none = zero_blk();
+    const Blk acc = gemm_prod<1, N, NL>(LA, none, LT, none, x);  // U_k T (FP64)
     const size_t mk = (size_t)m * P.nt + k;
     if (small) {
       Blk32 D;  // D[4q + r][c] = conj(U[c][4q + r]) - delta
@@ -587,11 +600,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SEGMENT_W
       store_blk(P.Ust + mk * 256, Ut, x);
     }
     if (threadIdx.x == 0) P.big[mk] = small ? 0 : 1;
-    if (k + 1 < it.k1) Xn = assemble<1>(P, m, k + 1, u, load_drift<1>(P, m, x), x);
+    if (k + 1 < it.k1) {
+      Xo = assemble<1, true>(P, m, k + 1, u, load_blk(H0o, x), x);
+      s = choose_scaling(norm1_blk<1>(Xo, red, x), deg6);
+      const double f = ldexp(1.0, -s);
+      Xo.re *= f;
+      Xo.im *= f;
+      X32.re = __builtin_convertvector(Xo.re, f4);
+      X32.im = __builtin_convertvector(Xo.im, f4);
+      if (deg6) X2 = prod32_adj(X32, X32);
+    }
     bsync<1>();
     put<1>(LT, acc, x);
+    bsync<1>();
   }
-  bsync<1>();
+  Blk Tseg;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const double2 v = LT[lidx(0, rowof(x.q, r), x.c)];
@@ -836,7 +859,11 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT ? PR_GRADIENT_WAVES : 0))) v
 // M_k = C_k U_k^dag = C_k + C_k D_k and C_{k+1} = U_k M_k = M_k + D_k^dag M_k
 // forms the increments C D and D^dag M on FP32 MFMA and accumulates C and M in
 // FP64 (held in the FP32 own layout); the traces are FP64.
-template <int V>
+// Trace operand held in registers for the whole walk (REG): with n_c <= V each
+// lane group owns one control and each lane at most TR_REG of its nonzeros.
+constexpr int TR_REG = 4;
+
+template <int V, bool REG>
 __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ probs,
                                                     const WorkItem* __restrict__ items,
                                                     const double* __restrict__ u) {
@@ -873,6 +900,23 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
     Cm.im[r] = v.y;
   }
   bsync<1>();
+  constexpr int G = 64 / V;
+  const int grp = x.lane / G, gl = x.lane % G;
+  int ridx[TR_REG];
+  double2 rv[TR_REG];
+  if constexpr (REG) {
+    int e0 = 0, e1 = 0;
+    if (grp < P.nc) {
+      e0 = gptr(P.tr_ptr)[grp];
+      e1 = gptr(P.tr_ptr)[grp + 1];
+    }
+#pragma unroll
+    for (int i = 0; i < TR_REG; i++) {
+      const int e = e0 + gl + i * G;
+      ridx[i] = e < e1 ? gptr(P.tr_idx)[e] : 0;
+      rv[i] = e < e1 ? gload2(P.tr_v, e) : make_double2(0, 0);
+    }
+  }
   const unsigned char* big = P.big + (size_t)m * P.nt;
   Blk32 D;
   if (!gptr(big)[it.k0]) D = load_blk32(P.Udm + ((size_t)m * P.nt + it.k0) * 256, x);
@@ -935,7 +979,28 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
       bsync<1>();
       if (nsmall) D = load_blk32(P.Udm + ((size_t)m * P.nt + k + 1) * 256, x);
     }
-    contract<1, V, true>(P, m, k, Mk, LA, x);
+    if constexpr (REG) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Mk.re[r], Mk.im[r]);
+      bsync<1>();
+      const bool want_im = (P.cim != 0.0);
+      double sv = 0;
+#pragma unroll
+      for (int i = 0; i < TR_REG; i++) {
+        const double2 mv = LA[ridx[i]];
+        sv += want_im ? (rv[i].x * mv.y + rv[i].y * mv.x) : (rv[i].x * mv.x - rv[i].y * mv.y);
+      }
+#pragma unroll
+      for (int msk = 1; msk < G; msk <<= 1) sv += __shfl_xor(sv, msk);
+      if (gl == 0 && grp < P.nc) {
+        const double f = P.dts[k] * P.scale[(size_t)m * P.nc + grp];
+        gptr_w(P.Gm)[(size_t)m * P.nc * P.nt + (size_t)k * P.nc + grp] =
+            want_im ? f * sv * (-P.cim) : f * sv * P.cre;
+      }
+      bsync<1>();
+    } else {
+      contract<1, V, true>(P, m, k, Mk, LA, x);
+    }
   }
 }
 

@@ -69,13 +69,17 @@ inline cd at(const double* a, int d, int r, int c) {
 // image of M^T). Block (I, J) holds 256 complex in [r][lane] order; lane
 // l = c + 16q, register r is element (16I + q + 4r, 16J + c), the FP64 MFMA
 // result layout.
-void tile_image(const double* a, int d, int B, bool transpose, double2* out) {
+// own32: the FP32 MFMA result layout instead (d <= 16): register r of lane
+// c + 16q is element (4q + r, c).
+inline int lane_row(int l, int r, bool own32) { return own32 ? 4 * (l >> 4) + r : (l >> 4) + 4 * r; }
+
+void tile_image(const double* a, int d, int B, bool transpose, double2* out, bool own32 = false) {
   for (int I = 0; I < B; I++)
     for (int J = 0; J < B; J++) {
       double2* blk = out + (size_t)(I * B + J) * 256;
       for (int r = 0; r < 4; r++)
         for (int l = 0; l < 64; l++) {
-          int row = 16 * I + (l >> 4) + 4 * r, col = 16 * J + (l & 15);
+          int row = 16 * I + lane_row(l, r, own32), col = 16 * J + (l & 15);
           cd v(0, 0);
           if (row < d && col < d) v = transpose ? at(a, d, col, row) : at(a, d, row, col);
           blk[r * 64 + l] = make_double2(v.real(), v.imag());
@@ -106,7 +110,7 @@ struct Csr {
   std::vector<double2> v;
 };
 
-Csr build_csr(const double* Hc, int nc, int d, int B, bool transpose) {
+Csr build_csr(const double* Hc, int nc, int d, int B, bool transpose, bool own32 = false) {
   Csr out;
   const int NB = B * B;
   const size_t dd = (size_t)d * d;
@@ -116,7 +120,7 @@ Csr build_csr(const double* Hc, int nc, int d, int B, bool transpose) {
     for (int r = 0; r < 4; r++)
       for (int l = 0; l < 64; l++) {
         const size_t pos = ((size_t)w * 4 + r) * 64 + l;
-        const int row = 16 * I + (l >> 4) + 4 * r, col = 16 * J + (l & 15);
+        const int row = 16 * I + lane_row(l, r, own32), col = 16 * J + (l & 15);
         if (row < d && col < d)
           for (int j = 0; j < nc; j++) {
             cd v = transpose ? at(Hc + 2 * dd * j, d, col, row) : at(Hc + 2 * dd * j, d, row, col);
@@ -239,6 +243,7 @@ struct Plan {
     int B = 0, V = 0;
     bool lindblad = false;  // some problem of the group has Lindblad dynamics
     bool mixed = false;     // FP32-MFMA exp and gradient walk (d <= 16 only)
+    bool trace_regs = false;  // mixed walk keeps the trace operand in registers
     pr::WorkItem* d_items = nullptr;
     int n_items = 0;
     int2* d_members = nullptr;
@@ -386,7 +391,10 @@ int launch_gradient_mixed(Plan& pl, const Plan::Group& g, const double* d_u, hip
   switch (g.V) {
 #define PR_MCASE(VV)                                                                                    \
   case VV:                                                                                              \
-    hipLaunchKernelGGL((pr::k_gradient_mx<VV>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u); \
+    if (g.trace_regs)                                                                                   \
+      hipLaunchKernelGGL((pr::k_gradient_mx<VV, true>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u); \
+    else                                                                                                \
+      hipLaunchKernelGGL((pr::k_gradient_mx<VV, false>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u); \
     break;
     PR_MCASE(2) PR_MCASE(4) PR_MCASE(8) PR_MCASE(16) PR_MCASE(32)
 #undef PR_MCASE
@@ -481,6 +489,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   std::vector<std::vector<int2>> members(5);
   std::vector<int> vmax(5, 1);
   std::vector<long long> group_members(5, 0);
+  std::vector<int> tr_nnz_max(5, 0);  // most nonzeros of one control, per group
+  std::vector<int> nc_max(5, 0);
   for (int p = 0; p < n_probs; p++) group_members[(probs[p].dim + 15) / 16] += probs[p].n_members;
   // the mixed-precision kernels cover single-wavefront (d <= 16) unitary
   // problems under the reference gradient formula; any other problem of the
@@ -539,6 +549,18 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     double* dd_;
     if ((rc = pl.upload(&dp, img.data(), MB * nh0))) return rc;
     D.H0 = dp;
+    if (B == 1 && mixed1) {
+      for (int m = 0; m < nh0; m++) tile_image(H0src + 2 * dd * m, d, 1, false, img.data() + MB * m, true);
+      if ((rc = pl.upload(&dp, img.data(), MB * nh0))) return rc;
+      D.H0o = dp;
+      Csr co = build_csr(Hcsrc, nc, d, 1, false, true);
+      int *ip, *jp;
+      double2* vp;
+      if ((rc = pl.upload(&ip, co.ptr.data(), co.ptr.size()))) return rc;
+      if ((rc = pl.upload(&jp, co.j.data(), co.j.size()))) return rc;
+      if ((rc = pl.upload(&vp, co.v.data(), co.v.size()))) return rc;
+      D.co_ptr = ip; D.co_j = jp; D.co_v = vp;
+    }
     if ((rc = pl.upload(&dd_, nrm.data(), nh0))) return rc;
     D.n0 = dd_;
     for (int j = 0; j < nc; j++) nrm[j] = norm1(Hcsrc + 2 * dd * j, d);
@@ -558,6 +580,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       if ((rc = pl.upload(&jp, tr.idx.data(), tr.idx.size()))) return rc;
       if ((rc = pl.upload(&vp, tr.v.data(), tr.v.size()))) return rc;
       D.tr_ptr = ip; D.tr_idx = jp; D.tr_v = vp;
+      for (int j = 0; j < nc; j++) tr_nnz_max[B] = std::max(tr_nnz_max[B], tr.ptr[j + 1] - tr.ptr[j]);
     }
     std::vector<double> sc((size_t)M * nc, 1.0);
     if (q.ctrl_scale) std::copy(q.ctrl_scale, q.ctrl_scale + (size_t)M * nc, sc.begin());
@@ -613,6 +636,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       }
     }
     vmax[B] = std::max(vmax[B], next_pow2_cap32(nc));
+    nc_max[B] = std::max(nc_max[B], nc);
   }
   pl.n_params = pl.coff[n_probs];
   int rc;
@@ -632,6 +656,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     for (int p = 0; p < n_probs; p++)
       if ((probs[p].dim + 15) / 16 == B && probs[p].dynamics == PR_LINDBLAD) g.lindblad = true;
     g.mixed = B == 1 && mixed1;
+    // every control fits one lane group of 64 / V lanes with <= TR_REG nonzeros per lane
+    g.trace_regs = g.mixed && vmax[B] >= nc_max[B] && tr_nnz_max[B] <= pr::TR_REG * (64 / vmax[B]);
     if ((rc = pl.upload(&g.d_items, items[B].data(), items[B].size()))) return rc;
     if ((rc = pl.upload(&g.d_members, members[B].data(), members[B].size()))) return rc;
     pl.groups.push_back(g);
