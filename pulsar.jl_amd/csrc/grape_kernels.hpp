@@ -775,8 +775,63 @@ __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, d
   bsync<B>();
 }
 
+// Trace operand held in registers for a whole walk (single wavefront): with
+// n_c <= V each lane group of 64 / V lanes owns one control and each lane at
+// most TR_REG of its nonzeros (LDS index of the element of M, value).
+constexpr int TR_REG = 4;
+
+template <int V>
+struct TraceRegs {
+  int idx[TR_REG];
+  double2 v[TR_REG];
+};
+
+template <int V>
+__device__ inline void trace_regs_load(const ProbDev& P, TraceRegs<V>& t, const Ln& x) {
+  constexpr int G = 64 / V;
+  const int grp = x.lane / G, gl = x.lane % G;
+  int e0 = 0, e1 = 0;
+  if (grp < P.nc) {
+    e0 = gptr(P.tr_ptr)[grp];
+    e1 = gptr(P.tr_ptr)[grp + 1];
+  }
+#pragma unroll
+  for (int i = 0; i < TR_REG; i++) {
+    const int e = e0 + gl + i * G;
+    t.idx[i] = e < e1 ? gptr(P.tr_idx)[e] : 0;
+    t.v[i] = e < e1 ? gload2(P.tr_v, e) : make_double2(0, 0);
+  }
+}
+
+// contract() with the trace operand from registers; OWN32: M is held in the
+// FP32 own layout (rows 4q + r) instead of the FP64 one
+template <int V, bool OWN32>
+__device__ inline void contract_regs(const ProbDev& P, int m, int k, const Blk& Mk, double2* LA,
+                                     const TraceRegs<V>& t, const Ln& x) {
+  constexpr int G = 64 / V;
+  const int grp = x.lane / G, gl = x.lane % G;
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+    LA[lidx(0, OWN32 ? rowof32(x.q, r) : rowof(x.q, r), x.c)] = make_double2(Mk.re[r], Mk.im[r]);
+  bsync<1>();
+  const bool want_im = (P.cim != 0.0);
+  double sv = 0;
+#pragma unroll
+  for (int i = 0; i < TR_REG; i++) {
+    const double2 mv = LA[t.idx[i]];
+    sv += want_im ? (t.v[i].x * mv.y + t.v[i].y * mv.x) : (t.v[i].x * mv.x - t.v[i].y * mv.y);
+  }
+#pragma unroll
+  for (int msk = 1; msk < G; msk <<= 1) sv += __shfl_xor(sv, msk);
+  if (gl == 0 && grp < P.nc) {
+    const double f = P.dts[k] * P.scale[(size_t)m * P.nc + grp];
+    gptr_w(P.Gm)[(size_t)m * P.nc * P.nt + (size_t)k * P.nc + grp] = want_im ? f * sv * (-P.cim) : f * sv * P.cre;
+  }
+  bsync<1>();
+}
+
 // ---------------------------------------------------------------- k_gradient
-template <int B, int V, bool EXACT>
+template <int B, int V, bool EXACT, bool REG = false>
 __global__ __launch_bounds__(64 * B * B)
 __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT ? PR_GRADIENT_WAVES : 0))) void k_gradient(const ProbDev* __restrict__ probs,
                                                           const WorkItem* __restrict__ items,
@@ -815,6 +870,8 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT ? PR_GRADIENT_WAVES : 0))) v
     }
     // U_{k+1} is fetched and the contraction of slice k runs while the
     // product C_{k+1} = U_k M_k is on the matrix cores
+    TraceRegs<V> tregs;
+    if constexpr (REG) trace_regs_load<V>(P, tregs, x);
     Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + it.k0) * MB, x);
     for (int k = it.k0; k < it.k1; k++) {
       Blk Ud = conj_blk(Ut);                      // U^dag, own block
@@ -824,6 +881,8 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT ? PR_GRADIENT_WAVES : 0))) v
       Cm = mm<B, T, N>(Ut, Mk, LA, LB, x);        // U_k M_k
       if constexpr (EXACT) {
         contract<B, V>(P, m, k, exact_direction<B>(P, m, k, u, H0, Mk, LA, LB, red, x), LA, x);
+      } else if constexpr (REG && B == 1) {
+        contract_regs<V, false>(P, m, k, Mk, LA, tregs, x);
       } else {
         contract<B, V>(P, m, k, Mk, LA, x);
       }
@@ -859,10 +918,6 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT ? PR_GRADIENT_WAVES : 0))) v
 // M_k = C_k U_k^dag = C_k + C_k D_k and C_{k+1} = U_k M_k = M_k + D_k^dag M_k
 // forms the increments C D and D^dag M on FP32 MFMA and accumulates C and M in
 // FP64 (held in the FP32 own layout); the traces are FP64.
-// Trace operand held in registers for the whole walk (REG): with n_c <= V each
-// lane group owns one control and each lane at most TR_REG of its nonzeros.
-constexpr int TR_REG = 4;
-
 template <int V, bool REG>
 __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ probs,
                                                     const WorkItem* __restrict__ items,
@@ -900,23 +955,8 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
     Cm.im[r] = v.y;
   }
   bsync<1>();
-  constexpr int G = 64 / V;
-  const int grp = x.lane / G, gl = x.lane % G;
-  int ridx[TR_REG];
-  double2 rv[TR_REG];
-  if constexpr (REG) {
-    int e0 = 0, e1 = 0;
-    if (grp < P.nc) {
-      e0 = gptr(P.tr_ptr)[grp];
-      e1 = gptr(P.tr_ptr)[grp + 1];
-    }
-#pragma unroll
-    for (int i = 0; i < TR_REG; i++) {
-      const int e = e0 + gl + i * G;
-      ridx[i] = e < e1 ? gptr(P.tr_idx)[e] : 0;
-      rv[i] = e < e1 ? gload2(P.tr_v, e) : make_double2(0, 0);
-    }
-  }
+  TraceRegs<V> tregs;
+  if constexpr (REG) trace_regs_load<V>(P, tregs, x);
   const unsigned char* big = P.big + (size_t)m * P.nt;
   Blk32 D;
   if (!gptr(big)[it.k0]) D = load_blk32(P.Udm + ((size_t)m * P.nt + it.k0) * 256, x);
@@ -979,28 +1019,10 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
       bsync<1>();
       if (nsmall) D = load_blk32(P.Udm + ((size_t)m * P.nt + k + 1) * 256, x);
     }
-    if constexpr (REG) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Mk.re[r], Mk.im[r]);
-      bsync<1>();
-      const bool want_im = (P.cim != 0.0);
-      double sv = 0;
-#pragma unroll
-      for (int i = 0; i < TR_REG; i++) {
-        const double2 mv = LA[ridx[i]];
-        sv += want_im ? (rv[i].x * mv.y + rv[i].y * mv.x) : (rv[i].x * mv.x - rv[i].y * mv.y);
-      }
-#pragma unroll
-      for (int msk = 1; msk < G; msk <<= 1) sv += __shfl_xor(sv, msk);
-      if (gl == 0 && grp < P.nc) {
-        const double f = P.dts[k] * P.scale[(size_t)m * P.nc + grp];
-        gptr_w(P.Gm)[(size_t)m * P.nc * P.nt + (size_t)k * P.nc + grp] =
-            want_im ? f * sv * (-P.cim) : f * sv * P.cre;
-      }
-      bsync<1>();
-    } else {
+    if constexpr (REG)
+      contract_regs<V, true>(P, m, k, Mk, LA, tregs, x);
+    else
       contract<1, V, true>(P, m, k, Mk, LA, x);
-    }
   }
 }
 

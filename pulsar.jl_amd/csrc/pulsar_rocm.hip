@@ -415,7 +415,9 @@ int launch_gradient(Plan& pl, const Plan::Group& g, const double* d_u, hipStream
   case VV:                                                                                                  \
     if (pl.opt.grad_mode == PR_GRAD_EXACT)                                                                  \
       hipLaunchKernelGGL((pr::k_gradient<B, VV, true>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); \
-    else                                                                                                    \
+    else if (B == 1 && g.trace_regs)                                                                    \
+      hipLaunchKernelGGL((pr::k_gradient<B, VV, false, true>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); \
+    else                                                                                                \
       hipLaunchKernelGGL((pr::k_gradient<B, VV, false>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); \
     break;
     PR_GCASE(2) PR_GCASE(4) PR_GCASE(8) PR_GCASE(16) PR_GCASE(32)
@@ -657,7 +659,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       if ((probs[p].dim + 15) / 16 == B && probs[p].dynamics == PR_LINDBLAD) g.lindblad = true;
     g.mixed = B == 1 && mixed1;
     // every control fits one lane group of 64 / V lanes with <= TR_REG nonzeros per lane
-    g.trace_regs = g.mixed && vmax[B] >= nc_max[B] && tr_nnz_max[B] <= pr::TR_REG * (64 / vmax[B]);
+    g.trace_regs = B == 1 && vmax[B] >= nc_max[B] && tr_nnz_max[B] <= pr::TR_REG * (64 / vmax[B]);
     if ((rc = pl.upload(&g.d_items, items[B].data(), items[B].size()))) return rc;
     if ((rc = pl.upload(&g.d_members, members[B].data(), members[B].size()))) return rc;
     pl.groups.push_back(g);
