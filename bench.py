@@ -51,7 +51,62 @@ HBM_PEAK_GBS = 8000.0
 # commands and per-kernel figures are in profiles/grape_four_qubit_ensemble_hbm_traffic.md.
 # k_segment writes the U_k^T images (256 x 2000 x 4 KiB = 2.10 GB) and
 # k_gradient reads them back once.
-MEASURED_TRAFFIC_BYTES = {"k_segment": 0.011e9 + 2.131e9, "k_gradient": 2.140e9 + 0.040e9}
+MEASURED_TRAFFIC_BYTES = {"k_segment": 0.027e9 + 2.139e9, "k_gradient": 2.141e9 + 0.038e9}
+
+
+def probe_rates():
+    """Sustained TFLOP/s of dense FP64 MFMA, FP32 MFMA, FP64 VALU FMA, and FP64
+    MFMA interleaved with an equal issue time of FP64 VALU FMAs. If the two
+    overlapped, the interleaved loop would take as long as either alone; if
+    they serialise, it takes the sum (rate = about half the two rates' mean)."""
+    import ctypes
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "pulsar.jl_amd", "pulsar_rocm", "libpulsar_probe.so"))
+    lib.pr_probe_mfma.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    out = {}
+    for name, kind in (("mfma_f64", 0), ("mfma_f32", 1), ("valu_f64_fma", 2), ("mfma_f64_with_valu_f64", 3)):
+        v = ctypes.c_double(0)
+        rc = lib.pr_probe_mfma(kind, ctypes.byref(v))
+        if rc != 0:
+            raise RuntimeError(f"pr_probe_mfma({kind}) failed with {rc}")
+        out[name + "_tflops"] = round(v.value, 2)
+    # time of the interleaved loop relative to MFMA alone and VALU alone
+    flops_mfma = 2.0 * 16 * 16 * 4 * 4
+    flops_valu = 2.0 * 64 * 64
+    t_m = flops_mfma / out["mfma_f64_tflops"]
+    t_v = flops_valu / out["valu_f64_fma_tflops"]
+    t_b = (flops_mfma + flops_valu) / out["mfma_f64_with_valu_f64_tflops"]
+    out["interleaved_time_over_sum_of_times"] = round(t_b / (t_m + t_v), 3)
+    out["interleaved_time_over_max_of_times"] = round(t_b / max(t_m, t_v), 3)
+    return out
+
+
+def measured_mfma_peaks():
+    """Sustained dense MFMA TFLOP/s of this device (libpulsar_probe.so): FP64 and
+    FP32 16x16x4 loops with eight wavefronts per SIMD and no memory traffic."""
+    import ctypes
+
+    path = os.path.join(ROOT, "pulsar.jl_amd", "pulsar_rocm", "libpulsar_probe.so")
+    lib = ctypes.CDLL(path)
+    lib.pr_probe_mfma.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    out = {}
+    for name, kind in (("fp64", 0), ("fp32", 1)):
+        v = ctypes.c_double(0)
+        rc = lib.pr_probe_mfma(kind, ctypes.byref(v))
+        if rc != 0:
+            raise RuntimeError(f"pr_probe_mfma({name}) failed with {rc}")
+        out[name] = v.value
+    return out
+
+
+def executed_mfma_flops(kernel, precision, M, nt):
+    """MFMA flops the d=16 kernels issue per launch on this workload (every
+    slice takes the degree-6 polynomial, no squarings): 3 real 16x16x16
+    products per complex product (Gauss), each 4 MFMAs of 2048 flops;
+    k_segment: 3 exp products + 1 chain product, k_gradient: 2 walk products."""
+    per_product = 3 * 4 * 2048
+    products = 4 if kernel == "k_segment" else 2
+    return products * per_product * M * nt
 
 
 def flops_model(d, nc):
@@ -150,9 +205,14 @@ def main():
                     help="the headline metric is four_qubit_ensemble; the others print timing lines of their own")
     ap.add_argument("--cpu-sample", type=int, default=32, help="members timed for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    ap.add_argument("--probe", action="store_true",
+                    help="only print the device's sustained MFMA and FP64 VALU rates and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.probe:
+        print(json.dumps(probe_rates()))
+        return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
@@ -206,6 +266,11 @@ def main():
     dominant, dom_ms, dom_flops = ("k_segment", seg_ms, f_seg) if seg_ms >= grad_ms else ("k_gradient", grad_ms, f_grad)
     achieved = dom_flops * units / (dom_ms * 1e-3) / 1e12
     peak = round(mfma_peak(dominant, args.precision), 1)
+    probe = measured_mfma_peaks()
+    if args.precision == "fp64" or dominant != "k_segment":
+        mpeak = probe["fp64"] if args.precision == "fp64" else probe["fp32"]
+    else:
+        mpeak = 4.0 / (3.0 / probe["fp32"] + 1.0 / probe["fp64"])
 
     # accuracy against the FP64 oracle on a member sample (same inputs)
     out = {
@@ -228,7 +293,11 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                      "traffic": MEASURED_TRAFFIC_BYTES[dominant] if prec == PR_FP64 else None,
                      "kernel_ms": round(dom_ms, 4),
-                     "algorithmic_flops_per_launch": dom_flops * units},
+                     "algorithmic_flops_per_launch": dom_flops * units,
+                     "measured_dense_mfma_peak": round(mpeak, 1),
+                     "frac_of_measured_peak": round(achieved / mpeak, 4),
+                     "executed_mfma_tflops": round(executed_mfma_flops(dominant, args.precision, M, nt)
+                                                   / (dom_ms * 1e-3) / 1e12, 3)},
         "eval_tflops_algorithmic": round(f_tot * units * evals_s / 1e12, 3),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
     }

@@ -16,10 +16,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "pulsar_rocm", "libpulsar_rocm.so")
 SOURCES = ["pulsar_rocm.hip"]
+# the benchmark's MFMA throughput probe, a library of its own
+PROBE_OUT = os.path.join(HERE, "pulsar_rocm", "libpulsar_probe.so")
+PROBE_SOURCES = ["mfma_probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
-def build(debug: bool = False, verbose: bool = False, out: str = OUT, defines=()) -> str:
+def build(debug: bool = False, verbose: bool = False, out: str = OUT, defines=(), sources=None) -> str:
     flags = [
         "--offload-arch=gfx950",
         "-std=c++17",
@@ -35,7 +38,7 @@ def build(debug: bool = False, verbose: bool = False, out: str = OUT, defines=()
         *[f"-D{d}" for d in defines],
     ]
     tmp = out + ".tmp"
-    cmd = [HIPCC, *flags, "-o", tmp, *[os.path.join(CSRC, s) for s in SOURCES]]
+    cmd = [HIPCC, *flags, "-o", tmp, *[os.path.join(CSRC, s) for s in (sources or SOURCES)]]
     r = subprocess.run(cmd, capture_output=True, text=True)
     noise = "is not a recognized feature for this target"
     msgs = [ln for ln in (r.stdout + r.stderr).splitlines() if noise not in ln]
@@ -52,3 +55,5 @@ if __name__ == "__main__":
     out = args[args.index("--out") + 1] if "--out" in args else OUT
     defs = [a[2:] for a in args if a.startswith("-D")]
     print(build(debug="--debug" in args, verbose=True, out=out, defines=defs))
+    if "--out" not in args:
+        print(build(debug="--debug" in args, verbose=True, out=PROBE_OUT, sources=PROBE_SOURCES))

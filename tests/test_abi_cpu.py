@@ -14,8 +14,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "pulsar_rocm.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(pr_[a-z_0-9]+)\s*\(", text)))
 
@@ -30,6 +30,18 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(native.LIB_PATH)
     for name in declared_functions():
         assert hasattr(lib, name), name
+
+
+def test_probe_library_exports_its_header():
+    header = os.path.join(ROOT, "include", "pulsar_probe.h")
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(native.LIB_PATH), "libpulsar_probe.so"))
+    names = declared_functions(header)
+    assert names == ["pr_probe_mfma"]
+    for name in names:
+        assert hasattr(lib, name), name
+    # argument checks run before any device use
+    v = ctypes.c_double(0)
+    assert lib.pr_probe_mfma(7, ctypes.byref(v)) == -1
 
 
 def test_abi_version_and_no_device_here():
