@@ -39,6 +39,8 @@ namespace pr {
 
 struct ProbDev {
   int d, B, nc, nt, M, dyn, obj, npairs, nseg, seglen, h0pm, unitary, grad_exact;
+  int prescaled;              // 1: drift images and control entries already hold c dt A
+                              // (uniform dt), so assemble() needs no final scaling
   double cre, cim;            // generator factor c: (0,-1) closed, (1,0) Lindblad
   const double2* H0;          // [M or 1][NB][256] tile images of the drift
   // controls A_j in register-layout sparse form: for block w, register r and
@@ -171,16 +173,17 @@ __device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __r
   Blk H = H0;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const int pos = (x.w * 4 + r) * 64 + x.lane;
-    const int e1 = gptr(cptr)[pos + 1];
-    for (int e = gptr(cptr)[pos]; e < e1; e++) {
-      const int j = gptr(cj)[e];
+    const unsigned pos = (unsigned)((x.w * 4 + r) * 64 + x.lane);
+    const unsigned e1 = (unsigned)gptr(cptr)[pos + 1];
+    for (unsigned e = (unsigned)gptr(cptr)[pos]; e < e1; e++) {
+      const unsigned j = (unsigned)gptr(cj)[e];
       const double2 a = gload2(cv, e);
       const double coef = gptr(sc)[j] * gptr(uk)[j];
       H.re[r] += coef * a.x;
       H.im[r] += coef * a.y;
     }
   }
+  if (P.prescaled) return H;
   return cscale(P.cre * dt, P.cim * dt, H);
 }
 
@@ -397,9 +400,11 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
   Blk X = assemble<B>(P, m, k, u, H0, x);
   bool deg6;
   const int s = choose_scaling(norm1_blk<B>(X, red, x), deg6);
-  const double f = ldexp(1.0, -s);
-  X.re *= f;
-  X.im *= f;
+  if (s) {
+    const double f = ldexp(1.0, -s);
+    X.re *= f;
+    X.im *= f;
+  }
   return frechet_blk<B>(X, s, M, LA, LB, x);
 }
 
@@ -450,9 +455,11 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
     Blk X = Xn;
     bool deg6;
     const int s = choose_scaling(norm1_blk<B>(X, red, x), deg6);
-    const double f = ldexp(1.0, -s);
-    X.re *= f;
-    X.im *= f;
+    if (s) {
+      const double f = ldexp(1.0, -s);
+      X.re *= f;
+      X.im *= f;
+    }
     smax = max(smax, s);
     Blk U;
     if constexpr (B == 1 && CLOSED) {
@@ -533,7 +540,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
   Blk Xo = assemble<1, true>(P, m, it.k0, u, load_blk(H0o, x), x);
   bool deg6;
   int s = choose_scaling(norm1_blk<1>(Xo, red, x), deg6);
-  {
+  if (s) {
     const double f = ldexp(1.0, -s);
     Xo.re *= f;
     Xo.im *= f;
@@ -603,9 +610,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
     if (k + 1 < it.k1) {
       Xo = assemble<1, true>(P, m, k + 1, u, load_blk(H0o, x), x);
       s = choose_scaling(norm1_blk<1>(Xo, red, x), deg6);
-      const double f = ldexp(1.0, -s);
-      Xo.re *= f;
-      Xo.im *= f;
+      if (s) {
+        const double f = ldexp(1.0, -s);
+        Xo.re *= f;
+        Xo.im *= f;
+      }
       X32.re = __builtin_convertvector(Xo.re, f4);
       X32.im = __builtin_convertvector(Xo.im, f4);
       if (deg6) X2 = prod32_adj(X32, X32);

@@ -537,9 +537,24 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     if (q.dynamics == PR_CLOSED) {
       h0h = hermitian_upper(q.H0, nh0, d);
       hch = hermitian_upper(q.Hc, nc, d);
-      H0src = h0h.data();
-      Hcsrc = hch.data();
+    } else {
+      h0h.assign(q.H0, q.H0 + 2 * dd * nh0);
+      hch.assign(q.Hc, q.Hc + 2 * dd * nc);
     }
+    // with a uniform slice length the assembly operators carry the factor
+    // c dt (c = -i closed, 1 Lindblad): X = c dt A0 + sum_j s u_j (c dt A_j)
+    D.prescaled = q.dt_slices ? 0 : 1;
+    if (D.prescaled) {
+      const double fr = D.cre * q.dt, fi = D.cim * q.dt;
+      for (auto* v : {&h0h, &hch})
+        for (size_t i = 0; i < v->size(); i += 2) {
+          const double a = (*v)[i], b = (*v)[i + 1];
+          (*v)[i] = fr * a - fi * b;
+          (*v)[i + 1] = fr * b + fi * a;
+        }
+    }
+    H0src = h0h.data();
+    Hcsrc = hch.data();
     std::vector<double2> img(MB * std::max(nh0, nc));
     std::vector<double> nrm(std::max(nh0, nc));
     for (int m = 0; m < nh0; m++) {
