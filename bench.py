@@ -166,6 +166,11 @@ def other_workload(args):
         probs, us = [b[0] for b in batch], [b[1] for b in batch]
         desc = "uneven_batch: 1024 gate problems, d in {2..64} with p ~ 1/d, 200..2000 slices, 2 controls per qubit"
         flops = sum((72 * q.dim ** 3 + 16 * q.n_ctrl * q.dim ** 2) * q.n_t * q.n_members for q in probs)
+    elif args.workload == "spin_chain_128":
+        p, u = workloads.spin_chain_128()
+        probs, us = [p], [u]
+        desc = "spin_chain_128: d=128 Heisenberg chain, 64 members, 14 controls, 500 slices, gate target"
+        flops = (72 * 128 ** 3 + 16 * p.n_ctrl * 128 ** 2) * p.n_members * p.n_t
     else:
         p, u = workloads.single_qubit_state()
         probs, us = [p], [u]
@@ -201,7 +206,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--workload", default="four_qubit_ensemble",
-                    choices=["four_qubit_ensemble", "three_qubit_lindblad", "uneven_batch", "single_qubit_state"],
+                    choices=["four_qubit_ensemble", "three_qubit_lindblad", "uneven_batch", "single_qubit_state",
+                             "spin_chain_128"],
                     help="the headline metric is four_qubit_ensemble; the others print timing lines of their own")
     ap.add_argument("--cpu-sample", type=int, default=32, help="members timed for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))

@@ -21,6 +21,7 @@
 
 #include "../../include/pulsar_rocm.h"
 #include "grape_kernels.hpp"
+#include "big_kernels.hpp"
 
 using cd = std::complex<double>;
 
@@ -244,6 +245,7 @@ struct Plan {
     bool lindblad = false;  // some problem of the group has Lindblad dynamics
     bool mixed = false;     // FP32-MFMA exp and gradient walk (d <= 16 only)
     bool trace_regs = false;  // mixed walk keeps the trace operand in registers
+    double2* big_scratch = nullptr;  // d > 64: per work item matrices in HBM
     pr::WorkItem* d_items = nullptr;
     int n_items = 0;
     int2* d_members = nullptr;
@@ -297,7 +299,10 @@ int validate(const pr_problem& p, int idx) {
     return fail(PR_ERR_ARG, "problem %d: dim, n_ctrl, n_t and n_members must be >= 1", idx);
   if (!p.H0 || !p.Hc) return fail(PR_ERR_ARG, "problem %d: H0 and Hc are required", idx);
   if (p.dynamics != PR_CLOSED && p.dynamics != PR_LINDBLAD) return fail(PR_ERR_ARG, "problem %d: unknown dynamics", idx);
-  if (p.dim > 64) return fail(PR_ERR_UNSUPPORTED, "problem %d: dim %d > 64 is not supported by this build", idx, p.dim);
+  if (p.dim > 128) return fail(PR_ERR_UNSUPPORTED, "problem %d: dim %d > 128 is not supported by this build", idx, p.dim);
+  if (p.dim > 64 && (p.dynamics != PR_CLOSED || p.objective != PR_GATE))
+    return fail(PR_ERR_UNSUPPORTED, "problem %d: dim %d > 64 is supported for closed-dynamics gate problems only",
+                idx, p.dim);
   if (!p.dt_slices && !(p.dt >= 0.0)) return fail(PR_ERR_ARG, "problem %d: dt must be non-negative", idx);
   if (p.objective == PR_GATE) {
     if (!p.U_target) return fail(PR_ERR_ARG, "problem %d: gate objective needs U_target", idx);
@@ -454,6 +459,37 @@ int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
   return PR_OK;
 }
 
+template <int B>
+int set_big_attr() {
+  const int bytes = (int)pr::big::Cfg<B>::smem;
+  HIPCHK(hipFuncSetAttribute((const void*)pr::k_segment_big<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  HIPCHK(hipFuncSetAttribute((const void*)pr::k_combine_big<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient_big<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  return PR_OK;
+}
+
+template <int B>
+int launch_big(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hipStream_t st) {
+  const dim3 blk(64 * pr::big::W);
+  const size_t sm = pr::big::Cfg<B>::smem;
+  int rc;
+  if ((rc = tbegin(pl, st))) return rc;
+  hipLaunchKernelGGL((pr::k_segment_big<B>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u, g.big_scratch);
+  HIPCHK(hipGetLastError());
+  if ((rc = tend(pl, 0, st))) return rc;
+  if ((rc = tbegin(pl, st))) return rc;
+  hipLaunchKernelGGL((pr::k_combine_big<B>), dim3(g.n_members), blk, sm, st, pl.d_probs, g.d_members, g.big_scratch);
+  HIPCHK(hipGetLastError());
+  if ((rc = tend(pl, 1, st))) return rc;
+  if (grad) {
+    if ((rc = tbegin(pl, st))) return rc;
+    hipLaunchKernelGGL((pr::k_gradient_big<B>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, g.big_scratch);
+    HIPCHK(hipGetLastError());
+    if ((rc = tend(pl, 2, st))) return rc;
+  }
+  return PR_OK;
+}
+
 int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
   for (const auto& g : pl.groups) {
     int rc = PR_OK;
@@ -462,6 +498,10 @@ int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
       case 2: rc = launch_group<2>(pl, g, d_u, d_G != nullptr, st); break;
       case 3: rc = launch_group<3>(pl, g, d_u, d_G != nullptr, st); break;
       case 4: rc = launch_group<4>(pl, g, d_u, d_G != nullptr, st); break;
+      case 5: rc = launch_big<5>(pl, g, d_u, d_G != nullptr, st); break;
+      case 6: rc = launch_big<6>(pl, g, d_u, d_G != nullptr, st); break;
+      case 7: rc = launch_big<7>(pl, g, d_u, d_G != nullptr, st); break;
+      case 8: rc = launch_big<8>(pl, g, d_u, d_G != nullptr, st); break;
       default: rc = fail(PR_ERR_LAUNCH, "internal: block count %d", g.B);
     }
     if (rc) return rc;
@@ -487,12 +527,12 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   pl.n_probs = n_probs;
   pl.host_probs.resize(n_probs);
   pl.coff.assign(n_probs + 1, 0);
-  std::vector<std::vector<pr::WorkItem>> items(5);
-  std::vector<std::vector<int2>> members(5);
-  std::vector<int> vmax(5, 1);
-  std::vector<long long> group_members(5, 0);
-  std::vector<int> tr_nnz_max(5, 0);  // most nonzeros of one control, per group
-  std::vector<int> nc_max(5, 0);
+  std::vector<std::vector<pr::WorkItem>> items(9);
+  std::vector<std::vector<int2>> members(9);
+  std::vector<int> vmax(9, 1);
+  std::vector<long long> group_members(9, 0);
+  std::vector<int> tr_nnz_max(9, 0);  // most nonzeros of one control, per group
+  std::vector<int> nc_max(9, 0);
   for (int p = 0; p < n_probs; p++) group_members[(probs[p].dim + 15) / 16] += probs[p].n_members;
   // the mixed-precision kernels cover single-wavefront (d <= 16) unitary
   // problems under the reference gradient formula; any other problem of the
@@ -519,7 +559,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     if (pl.opt.segment > 0) {
       L = pl.opt.segment;
     } else {
-      long long want = std::max(1LL, (long long)(8192 / NB));
+      // (d > 64: workgroups of 16 wavefronts, about 512 of them)
+      long long want = B > 4 ? 512 : std::max(1LL, (long long)(8192 / NB));
       long long S = (want + group_members[B] - 1) / group_members[B];
       S = std::min<long long>(std::max<long long>(S, 1), nt);
       L = (int)((nt + S - 1) / S);
@@ -663,7 +704,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   if ((rc = pl.alloc(&pl.d_F, n_probs))) return rc;
   if ((rc = pl.alloc(&pl.d_G, pl.n_params))) return rc;
   if ((rc = pl.alloc(&pl.d_part, (size_t)pr::RED_CHUNKS * pl.n_params))) return rc;
-  for (int B = 1; B <= 4; B++) {
+  for (int B = 1; B <= 8; B++) {
     if (items[B].empty()) continue;
     Plan::Group g;
     g.B = B;
@@ -678,11 +719,19 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     if ((rc = pl.upload(&g.d_items, items[B].data(), items[B].size()))) return rc;
     if ((rc = pl.upload(&g.d_members, members[B].data(), members[B].size()))) return rc;
     pl.groups.push_back(g);
+    if (B > 4) {
+      const size_t n = (size_t)std::max(g.n_items, g.n_members) * pr::BIG_SCRATCH * (size_t)(B * B) * 256;
+      if ((rc = pl.alloc(&pl.groups.back().big_scratch, n))) return rc;
+    }
     switch (B) {
       case 1: rc = set_smem_attr<1>(); break;
       case 2: rc = set_smem_attr<2>(); break;
       case 3: rc = set_smem_attr<3>(); break;
       case 4: rc = set_smem_attr<4>(); break;
+      case 5: rc = set_big_attr<5>(); break;
+      case 6: rc = set_big_attr<6>(); break;
+      case 7: rc = set_big_attr<7>(); break;
+      case 8: rc = set_big_attr<8>(); break;
     }
     if (rc) return rc;
   }
@@ -751,6 +800,8 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
     return fail(PR_ERR_ARG, "unknown grad_mode %d", o.grad_mode);
   if (o.segment < 0) return fail(PR_ERR_ARG, "segment must be >= 0");
   for (int p = 0; p < n_probs; p++) {
+    if (probs[p].dim > 64 && o.grad_mode == PR_GRAD_EXACT)
+      return fail(PR_ERR_UNSUPPORTED, "problem %d: the exact gradient is available for dim <= 64", p);
     int rc = validate(probs[p], p);
     if (rc) return rc;
   }

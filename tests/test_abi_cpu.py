@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from pulsar_rocm import native, workloads
-from pulsar_rocm.problem import COptions, to_c
+from pulsar_rocm.problem import PR_GATE, PR_GRAD_EXACT, PR_PAIRS_SQUARE, COptions, Problem, to_c
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "pulsar_rocm.h")
@@ -105,8 +105,23 @@ def test_null_arguments_rejected():
 
 
 def test_oversized_dim_unsupported():
+    # d > 128 at all; d > 64 outside closed-dynamics gate problems; exact mode at d > 64
     rng = np.random.default_rng(0)
+    d = 129
+    H = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+    H = (H + H.conj().T) / 2
+    big = Problem(dim=d, n_ctrl=1, n_t=2, H0=H, Hc=H[None], dt=0.1, objective=PR_GATE,
+                  U_target=np.eye(d, dtype=complex))
+    with pytest.raises(native.RocmError) as e:
+        native.Plan([big])
+    assert e.value.code == -5
     p, u = workloads.spin_chain_128(n_t=2, n_members=1)
     with pytest.raises(native.RocmError) as e:
-        native.Plan([p])
+        native.Plan([p], grad_mode=PR_GRAD_EXACT)
+    assert e.value.code == -5
+    pairs = Problem(dim=80, n_ctrl=1, n_t=2, H0=np.eye(80, dtype=complex), Hc=np.eye(80, dtype=complex)[None],
+                    dt=0.1, objective=PR_PAIRS_SQUARE, psi_init=np.eye(80, dtype=complex)[:1],
+                    psi_targ=np.eye(80, dtype=complex)[:1])
+    with pytest.raises(native.RocmError) as e:
+        native.Plan([pairs])
     assert e.value.code == -5

@@ -256,3 +256,45 @@ def test_exact_gradient_finite_differences_uneven_subset():
     finally:
         plan_x.close()
         plan_f.close()
+
+
+# ------------------------------------------------------------------ 64 < d <= 128
+
+
+def test_spin_chain_128_small():
+    # the 7-qubit Heisenberg chain (d = 128), 2 members, 24 slices
+    p, u = W.spin_chain_128(n_t=24, n_members=2)
+    F, G, D = check(p, u)
+    assert D[0] < 1e-12
+    assert D[1] >= 1  # generators above the degree-9 bound: scaling and squaring
+
+
+@pytest.mark.parametrize("d,seg", [(65, 0), (80, 3), (100, 0), (128, 5)])
+def test_random_gate_problems_beyond_64(d, seg):
+    rng = np.random.default_rng(200 + d)
+    H0 = W.random_hermitian(d, rng, 3.0)
+    Hc = np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)])
+    u = rng.uniform(-1, 1, (2, 11))
+    p = Problem(dim=d, n_ctrl=2, n_t=11, H0=H0, Hc=Hc, dt=0.05, objective=PR_GATE, n_members=2,
+                ctrl_scale=rng.uniform(0.9, 1.1, (2, 2)), U_target=W.haar_unitary(d, rng))
+    check(p, u, segment=seg)
+
+
+def test_mixed_dimensions_up_to_128_in_one_plan():
+    rng = np.random.default_rng(5)
+    probs, us = [], []
+    for d in (16, 64, 96):
+        H0 = W.random_hermitian(d, rng, 3.0)
+        Hc = np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)])
+        probs.append(Problem(dim=d, n_ctrl=2, n_t=7, H0=H0, Hc=Hc, dt=0.05, objective=PR_GATE,
+                             U_target=W.haar_unitary(d, rng)))
+        us.append(rng.uniform(-1, 1, (2, 7)))
+    plan = native.Plan(probs)
+    try:
+        F, G = plan.eval(us)
+    finally:
+        plan.close()
+    for q, uu, f, g in zip(probs, us, F, G):
+        F0, G0 = O.evaluate(q, uu)
+        assert abs(f - F0) <= TOL_F
+        assert np.abs(g - G0).max() <= TOL_G * np.abs(G0).max()
