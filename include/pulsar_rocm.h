@@ -83,7 +83,9 @@ extern "C" {
  * Targets.jl:37-144) and the ensemble closures of
  * build_ensemble_from_systems (src/Optimization/Ensemble/SystemBuilder.jl:49-87). */
 typedef struct {
-  int dim;                  /* d (closed) or N^2 (Lindblad); 1 <= dim <= 64 */
+  int dim;                  /* d (closed) or N^2 (Lindblad); 1 <= dim <= 64, or
+                               up to 128 for closed-dynamics gate problems under
+                               PR_GRAD_REFERENCE (PR_ERR_UNSUPPORTED otherwise) */
   int n_ctrl;               /* number of control channels, >= 1 */
   int n_t;                  /* number of time slices, >= 1 */
   double dt;                /* uniform slice length (used when dt_slices == NULL) */
