@@ -183,14 +183,22 @@ def other_workload(args):
     for _ in range(args.warmup):
         plan.eval_device(d_u, d_F, d_G)
     plan.synchronize()
+    # kernel breakdown with per-launch events (direct launches) ...
     plan.set_timing(True)
     plan.kernel_times()
+    for _ in range(args.steps):
+        plan.eval_device(d_u, d_F, d_G)
+    plan.synchronize()
+    kt = plan.kernel_times()
+    plan.set_timing(False)
+    # ... and the throughput of replayed evaluation graphs
+    plan.eval_device(d_u, d_F, d_G)
+    plan.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.eval_device(d_u, d_F, d_G)
     plan.synchronize()
     elapsed = time.perf_counter() - t0
-    kt = plan.kernel_times()
     plan.close()
     rate = args.steps / elapsed
     print(json.dumps({"workload": args.workload, "description": desc, "evals_per_s": round(rate, 3),

@@ -265,7 +265,19 @@ struct Plan {
   hipEvent_t ev_open = nullptr;
   double acc_ms[PR_KERNEL_CLASSES] = {0};
   long long acc_n[PR_KERNEL_CLASSES] = {0};
+  // captured launch sequences of one evaluation, keyed by the buffers and
+  // stream they were recorded with (replayed with hipGraphLaunch)
+  struct Captured {
+    const double* u;
+    double* F;
+    double* G;
+    hipStream_t st;
+    hipGraphExec_t exec;
+  };
+  std::vector<Captured> graphs;
+  bool use_graphs = true;
   ~Plan() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     for (auto& r : pending) {
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);
@@ -520,6 +532,39 @@ int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
     HIPCHK(hipGetLastError());
   }
   if ((rc = tend(pl, 3, st))) return rc;
+  return PR_OK;
+}
+
+// One evaluation as a replayed HIP graph: the first call with a given set of
+// buffers and stream captures run()'s launches (no memset or copy nodes), later
+// calls replay them. Per-kernel timing records events between launches, so
+// timed evaluations launch directly.
+int run_graph(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
+  if (pl.timing || !pl.use_graphs) return run(pl, d_u, d_F, d_G, st);
+  for (const auto& g : pl.graphs)
+    if (g.u == d_u && g.F == d_F && g.G == d_G && g.st == st) {
+      HIPCHK(hipGraphLaunch(g.exec, st));
+      return PR_OK;
+    }
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int rc = run(pl, d_u, d_F, d_G, st);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(PR_ERR_LAUNCH, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(PR_ERR_LAUNCH, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+  if (pl.graphs.size() >= 8) {  // bounded cache: drop the oldest
+    (void)hipGraphExecDestroy(pl.graphs.front().exec);
+    pl.graphs.erase(pl.graphs.begin());
+  }
+  pl.graphs.push_back({d_u, d_F, d_G, st, exec});
+  HIPCHK(hipGraphLaunch(exec, st));
   return PR_OK;
 }
 
@@ -809,6 +854,8 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
   if (rc) return rc;
   Plan* pl = new Plan;
   pl->opt = o;
+  const char* ng = getenv("PULSAR_ROCM_NO_GRAPH");
+  pl->use_graphs = !(ng && ng[0] && ng[0] != '0');
   hipError_t e = hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete pl;
@@ -833,7 +880,7 @@ int pr_eval(void* plan, const double* const* controls, double* F, double* const*
     HIPCHK(hipMemcpyAsync(pl.d_controls + pl.coff[p], controls[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]),
                           hipMemcpyHostToDevice, pl.stream));
   }
-  int rc = run(pl, pl.d_controls, pl.d_F, G ? pl.d_G : nullptr, pl.stream);
+  int rc = run_graph(pl, pl.d_controls, pl.d_F, G ? pl.d_G : nullptr, pl.stream);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(F, pl.d_F, sizeof(double) * pl.n_probs, hipMemcpyDeviceToHost, pl.stream));
   if (G)
@@ -852,7 +899,7 @@ int pr_eval_device(void* plan, const double* d_controls, double* d_F, double* d_
   Plan& pl = *static_cast<Plan*>(plan);
   if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : pl.stream;
-  return run(pl, d_controls, d_F, d_G, st);
+  return run_graph(pl, d_controls, d_F, d_G, st);
 }
 
 int pr_plan_buffers(void* plan, double** d_controls, double** d_F, double** d_G) {

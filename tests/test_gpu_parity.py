@@ -190,6 +190,40 @@ def test_device_resident_path_matches_host_path():
         plan.close()
 
 
+def test_graph_replays_match_direct_launches():
+    # evaluations replay a captured HIP graph; with per-kernel timing on, the
+    # same evaluation launches its kernels directly. Several replays with new
+    # controls, device-path replays and direct launches in between.
+    p, u = W.four_qubit_ensemble(n_t=64)
+    p, u = W.shrink(p, u, n_members=4)
+    q, uq = W.three_qubit_lindblad(n_t=12, n_members=2)
+    plan = native.Plan([p, q])
+    rng = np.random.default_rng(8)
+    try:
+        dc, dF, dG = plan.buffers()
+        for rep in range(4):
+            us = [u * rng.uniform(0.5, 1.5), uq * rng.uniform(0.5, 1.5)]
+            plan.set_timing(True)
+            F0, G0 = plan.eval(us)
+            Fn, _ = plan.eval(us, grad=False)
+            plan.kernel_times()
+            plan.set_timing(False)
+            F1, G1 = plan.eval(us)
+            assert np.array_equal(F1, F0)
+            assert all(np.array_equal(a, b) for a, b in zip(G1, G0))
+            F2, _ = plan.eval(us, grad=False)
+            assert np.array_equal(F2, Fn)
+            plan.upload_controls(us)
+            plan.eval_device(dc, dF, dG)
+            plan.synchronize()
+        F3, G3 = plan.eval(us)
+        assert np.array_equal(F3, F0) and all(np.array_equal(a, b) for a, b in zip(G3, G0))
+        F4, G4 = O.evaluate(p, us[0])
+        assert abs(F3[0] - F4) < TOL_F
+    finally:
+        plan.close()
+
+
 def test_ensemble_weights_and_linearity():
     p, u = W.four_qubit_ensemble(n_t=30)
     p, u = W.shrink(p, u, n_members=6)
