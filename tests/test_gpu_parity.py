@@ -332,3 +332,31 @@ def test_mixed_dimensions_up_to_128_in_one_plan():
         F0, G0 = O.evaluate(q, uu)
         assert abs(f - F0) <= TOL_F
         assert np.abs(g - G0).max() <= TOL_G * np.abs(G0).max()
+
+
+@pytest.mark.parametrize("d,nc", [(4, 40), (8, 33), (16, 70), (24, 5)])
+def test_many_controls_general_contraction(d, nc):
+    # more controls than one lane group holds (32 groups), dense controls, and a
+    # two-block problem: the general trace path and its reduction widths
+    rng = np.random.default_rng(400 + d + nc)
+    H0 = W.random_hermitian(d, rng, 2.0)
+    Hc = np.stack([W.random_hermitian(d, rng, 0.3) for _ in range(nc)])
+    u = rng.uniform(-1, 1, (nc, 9))
+    p = Problem(dim=d, n_ctrl=nc, n_t=9, H0=H0, Hc=Hc, dt=0.05, objective=PR_GATE, n_members=2,
+                U_target=W.haar_unitary(d, rng))
+    check(p, u)
+    check(p, u, segment=1)
+
+
+def test_mixed_plan_with_large_generators_falls_back_per_slice():
+    # every slice above the degree-6 bound: the mixed kernels take their FP64
+    # fallback for each slice and keep FP64 accuracy
+    from pulsar_rocm.problem import PR_MIXED
+    rng = np.random.default_rng(77)
+    d = 12
+    p = Problem(dim=d, n_ctrl=3, n_t=30, H0=W.random_hermitian(d, rng, 20.0),
+                Hc=np.stack([W.random_hermitian(d, rng, 5.0) for _ in range(3)]), dt=0.02,
+                objective=PR_GATE, n_members=3, ctrl_scale=rng.uniform(0.9, 1.1, (3, 3)),
+                U_target=W.haar_unitary(d, rng))
+    u = rng.uniform(-1, 1, (3, 30))
+    check(p, u, precision=PR_MIXED)
