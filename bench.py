@@ -219,6 +219,8 @@ def main():
                     help="the headline metric is four_qubit_ensemble; the others print timing lines of their own")
     ap.add_argument("--cpu-sample", type=int, default=32, help="members timed for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    ap.add_argument("--segment", type=int, default=0,
+                    help="slices per time segment of the chain scan (0: the library's choice)")
     ap.add_argument("--probe", action="store_true",
                     help="only print the device's sustained MFMA and FP64 VALU rates and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -244,7 +246,7 @@ def main():
     prec = PR_FP64 if args.precision == "fp64" else PR_MIXED
     p, u = workloads.four_qubit_ensemble()
     native.init(0)
-    plan = native.Plan([p], precision=prec)
+    plan = native.Plan([p], precision=prec, segment=args.segment)
     plan.upload_controls([u])
     d_u, d_F, d_G = plan.buffers()
     for _ in range(args.warmup):

@@ -174,6 +174,48 @@ int pr_plan_kernel_times(void* plan, double* ms, long long* launches);
 
 int pr_plan_destroy(void* plan);
 
+/* ---- the GRAPE outer loop on the device (grape_optimize, GRAPE.jl:469-643) ----
+ * Gradient ascent with the reference's step adaptation (adapt_grape_step_size,
+ * GRAPE.jl:309-334: x1.05 when F rose, x0.5 when it fell by more than 1e-14,
+ * clamped), update (grape_step!, GRAPE.jl:261-268, u += step G) and
+ * convergence test (check_grape_convergence, GRAPE.jl:371-402), with the
+ * controls, gradient and best controls resident in HBM: per iteration one
+ * evaluation graph, the gradient norm and the update run on the device and
+ * only F and |G| come back. fid_plan (one problem) evaluates F; grad_plan (one
+ * problem, same controls layout) evaluates G, or NULL when fid_plan's
+ * gradient is the one wanted (gate targets). */
+typedef struct {
+  int max_iter;
+  double convergence_tol;
+  double step_size;
+  int adapt_step_size;
+  double min_step_size;
+  double max_step_size;
+  double gradient_norm_tol;
+} pr_grape_config;
+
+#define PR_STOP_MAX_ITER 0
+#define PR_STOP_PERFECT 1  /* F >= 1 - 1e-12 */
+#define PR_STOP_GRADIENT 2 /* |G| < gradient_norm_tol */
+#define PR_STOP_FIDELITY 3 /* |F_n - F_{n-1}| < convergence_tol (n >= 3 entries) */
+
+typedef struct {
+  int n_iterations;
+  int converged;
+  int reason;              /* PR_STOP_* */
+  int n_fidelity_history;  /* entries written to fidelity_history */
+  int n_fidelity_evaluations;
+  int n_gradient_evaluations;
+  double fidelity;         /* F at the best controls (re-evaluated, as the reference does) */
+  double step_size_final;
+} pr_grape_result;
+
+/* controls_init / controls_best: [n_ctrl x n_t] column-major host arrays.
+ * fidelity_history needs max_iter + 2 entries, gnorm_history max_iter. */
+int pr_grape_optimize(void* fid_plan, void* grad_plan, const double* controls_init, const pr_grape_config* cfg,
+                      double* controls_best, double* fidelity_history, double* gnorm_history,
+                      pr_grape_result* result);
+
 #ifdef __cplusplus
 }
 #endif

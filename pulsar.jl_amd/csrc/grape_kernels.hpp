@@ -1088,4 +1088,37 @@ __global__ void __launch_bounds__(64) k_reduce_F(const ProbDev* __restrict__ pro
   if (lane == 0) F[p] = s;
 }
 
+// ---------------------------------------------------------------- GRAPE step
+// u += step * g with the product and the sum rounded separately (as the
+// host update controls .+= step .* G does), n values.
+__global__ void k_grape_step(double* __restrict__ u, const double* __restrict__ g, double step, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[i] = __dadd_rn(u[i], __dmul_rn(step, g[i]));
+}
+
+// sum of g^2 in a fixed order: block b sums elements b*256 + t + k*NORM_BLOCKS*256
+// per thread, then a butterfly; k_norm2_final adds the block sums in order.
+constexpr int NORM_BLOCKS = 256;
+
+__global__ void __launch_bounds__(256) k_norm2_part(const double* __restrict__ g, long long n,
+                                                    double* __restrict__ part) {
+  __shared__ double ws[4];
+  double s = 0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)NORM_BLOCKS * 256)
+    s += g[i] * g[i];
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+}
+
+__global__ void __launch_bounds__(64) k_norm2_final(const double* __restrict__ part, double* __restrict__ out) {
+  double s = 0;
+  for (int i = threadIdx.x; i < NORM_BLOCKS; i += 64) s += part[i];
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
 }  // namespace pr

@@ -963,6 +963,131 @@ long long pr_plan_n_params(void* plan) {
   return static_cast<Plan*>(plan)->n_params;
 }
 
+int pr_grape_optimize(void* fid_plan, void* grad_plan, const double* controls_init, const pr_grape_config* cfg,
+                      double* controls_best, double* fidelity_history, double* gnorm_history,
+                      pr_grape_result* result) {
+  if (fault_is("eval")) return fail(PR_ERR_INJECTED, "failure injected by PULSAR_ROCM_FAULT=eval");
+  if (!fid_plan || !controls_init || !cfg || !controls_best || !fidelity_history || !gnorm_history || !result)
+    return fail(PR_ERR_ARG, "pr_grape_optimize: NULL argument");
+  Plan& fp = *static_cast<Plan*>(fid_plan);
+  Plan* gp = static_cast<Plan*>(grad_plan);
+  if (fp.n_probs != 1 || (gp && (gp->n_probs != 1 || gp->n_params != fp.n_params)))
+    return fail(PR_ERR_ARG, "pr_grape_optimize: plans must hold one problem each, with the same controls");
+  if (cfg->max_iter <= 0 || !(cfg->step_size > 0) || !(cfg->min_step_size > 0) ||
+      cfg->max_step_size < cfg->step_size || cfg->min_step_size > cfg->step_size)
+    return fail(PR_ERR_ARG, "pr_grape_optimize: invalid configuration");
+  if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
+  const long long n = fp.n_params;
+  hipStream_t st = fp.stream;
+  DevBuf best, part, nrm;
+  HIPCHK(hipMalloc(&best.p, sizeof(double) * n));
+  HIPCHK(hipMalloc(&part.p, sizeof(double) * pr::NORM_BLOCKS));
+  HIPCHK(hipMalloc(&nrm.p, sizeof(double)));
+  double* d_u = fp.d_controls;
+  double* d_best = static_cast<double*>(best.p);
+  HIPCHK(hipMemcpyAsync(d_u, controls_init, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  int rc;
+  // F at controls d (F only), read back
+  auto fid = [&](const double* d, double* F) -> int {
+    int r = run_graph(fp, d, fp.d_F, nullptr, st);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(F, fp.d_F, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PR_OK;
+  };
+  auto copy_best = [&]() -> int {
+    HIPCHK(hipMemcpyAsync(d_best, d_u, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    return PR_OK;
+  };
+  std::vector<double> fh;
+  int n_f = 0, n_g = 0, n_iter = 0, converged = 0, reason = PR_STOP_MAX_ITER;
+  double step = cfg->step_size, F = 0, best_F;
+  if ((rc = fid(d_u, &F))) return rc;
+  n_f++;
+  fh.push_back(F);
+  best_F = F;
+  if ((rc = copy_best())) return rc;
+  for (int it = 1; it <= cfg->max_iter; it++) {
+    n_iter = it;
+    double* d_G;
+    if (gp) {
+      if ((rc = run_graph(fp, d_u, fp.d_F, nullptr, st))) return rc;
+      if ((rc = run_graph(*gp, d_u, gp->d_F, gp->d_G, st))) return rc;
+      d_G = gp->d_G;
+    } else {
+      if ((rc = run_graph(fp, d_u, fp.d_F, fp.d_G, st))) return rc;
+      d_G = fp.d_G;
+    }
+    n_f++;
+    n_g++;
+    hipLaunchKernelGGL(pr::k_norm2_part, dim3(pr::NORM_BLOCKS), dim3(256), 0, st, d_G, n,
+                       static_cast<double*>(part.p));
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(pr::k_norm2_final, dim3(1), dim3(64), 0, st, static_cast<const double*>(part.p),
+                       static_cast<double*>(nrm.p));
+    HIPCHK(hipGetLastError());
+    double g2 = 0;
+    HIPCHK(hipMemcpyAsync(&F, fp.d_F, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&g2, nrm.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const double gnorm = std::sqrt(g2);
+    fh.push_back(F);
+    gnorm_history[it - 1] = gnorm;
+    if (F > best_F) {
+      best_F = F;
+      if ((rc = copy_best())) return rc;
+    }
+    const size_t nh = fh.size();
+    if (cfg->adapt_step_size && nh >= 2) {
+      const double delta = fh[nh - 1] - fh[nh - 2];
+      double s2 = step;
+      if (delta > 0.0)
+        s2 = step * 1.05;
+      else if (delta < -1e-14)
+        s2 = step * 0.5;
+      step = std::min(std::max(s2, cfg->min_step_size), cfg->max_step_size);
+    }
+    hipLaunchKernelGGL(pr::k_grape_step, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_u,
+                       static_cast<const double*>(d_G), step, n);
+    HIPCHK(hipGetLastError());
+    int why = -1;
+    if (F >= 1.0 - 1e-12)
+      why = PR_STOP_PERFECT;
+    else if (gnorm < cfg->gradient_norm_tol)
+      why = PR_STOP_GRADIENT;
+    else if (nh >= 3 && std::fabs(fh[nh - 1] - fh[nh - 2]) < cfg->convergence_tol)
+      why = PR_STOP_FIDELITY;
+    if (why >= 0) {
+      converged = 1;
+      reason = why;
+      double Fp = 0;
+      if ((rc = fid(d_u, &Fp))) return rc;
+      fh.push_back(Fp);
+      n_f++;
+      if (Fp > best_F) {
+        best_F = Fp;
+        if ((rc = copy_best())) return rc;
+      }
+      break;
+    }
+  }
+  double F_final = 0;
+  if ((rc = fid(d_best, &F_final))) return rc;
+  n_f++;
+  HIPCHK(hipMemcpyAsync(controls_best, d_best, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::copy(fh.begin(), fh.end(), fidelity_history);
+  result->n_iterations = n_iter;
+  result->converged = converged;
+  result->reason = reason;
+  result->n_fidelity_history = (int)fh.size();
+  result->n_fidelity_evaluations = n_f;
+  result->n_gradient_evaluations = n_g;
+  result->fidelity = F_final;
+  result->step_size_final = step;
+  return PR_OK;
+}
+
 int pr_plan_destroy(void* plan) {
   if (!plan) return PR_OK;
   delete static_cast<Plan*>(plan);

@@ -413,14 +413,63 @@ def check_grape_convergence(fidelity_history, gradient_norm, config: GRAPEConfig
 
 def grape_optimize(system, target: QuantumTarget, controls_init: ControlSequence,
                    config: Optional[GRAPEConfig] = None, max_iter: Optional[int] = None,
-                   step_size: Optional[float] = None) -> OptimizationResult:
+                   step_size: Optional[float] = None, on_device: bool = True) -> OptimizationResult:
     """grape_optimize (GRAPE.jl:469-643, with the positional-max_iter overload :665-675).
-    Each iteration is one device evaluation of F and G; the loop logic (history,
-    adaptation, convergence, best-controls tracking, final re-evaluation) follows
-    the reference line by line."""
+
+    The loop logic (history, adaptation, convergence, best-controls tracking,
+    final re-evaluation) follows the reference. With on_device (the default,
+    when no callback or printing is asked for) the whole loop runs in the
+    library (pr_grape_optimize): controls, gradient and best controls stay in
+    HBM and each iteration returns only F and |G|. Otherwise this host loop
+    drives one device evaluation per iteration."""
     if config is None:
         config = GRAPEConfig(**({"max_iter": max_iter} if max_iter else {}),
                              **({"step_size": step_size} if step_size else {}))
+    if on_device and config.callback is None and not config.verbose:
+        return _grape_optimize_device(system, target, controls_init, config)
+    return _grape_optimize_host(system, target, controls_init, config)
+
+
+def _reason(code: int, f_hist, g_hist, config: GRAPEConfig) -> str:
+    if code == 1:
+        return "perfect fidelity (F >= 1 - 1e-12)"
+    if code == 2:
+        return f"gradient norm {g_hist[-1]} < tol {config.gradient_norm_tol}"
+    if code == 3:
+        # the test ran before the post-convergence evaluation was appended
+        dF = abs(f_hist[-2] - f_hist[-3])
+        return f"fidelity change {dF} < tol {config.convergence_tol}"
+    return "maximum iterations reached"
+
+
+def _grape_optimize_device(system, target: QuantumTarget, controls_init: ControlSequence,
+                           config: GRAPEConfig) -> OptimizationResult:
+    t_start = time.time()
+    dt, n_t, n_c = controls_init.dt, controls_init.n_timesteps, system.n_controls
+    if controls_init.controls.shape != (n_c, n_t):
+        raise ValueError(f"controls_init.controls size {controls_init.controls.shape} does not match ({n_c}, {n_t})")
+    seq = ControlSequence(np.asarray(controls_init.controls).T, dt, n_t)
+    fplan = native.Plan([fidelity_problem(system, seq, target)])
+    gplan = None if target.type == "unitary" else native.Plan([gradient_problem(system, seq, target,
+                                                                                 in_optimizer=True)])
+    cfg = native.GrapeConfig(config.max_iter, config.convergence_tol, config.step_size,
+                             1 if config.adapt_step_size else 0, config.min_step_size, config.max_step_size,
+                             config.gradient_norm_tol)
+    try:
+        best, f_hist, g_hist, res = native.grape_optimize_device(fplan, gplan, controls_init.controls, cfg)
+    finally:
+        fplan.close()
+        if gplan is not None:
+            gplan.close()
+    return OptimizationResult(best, float(res.fidelity), f_hist, g_hist, int(res.n_iterations), bool(res.converged),
+                              _reason(res.reason, f_hist, g_hist, config), time.time() - t_start,
+                              int(res.n_fidelity_evaluations), int(res.n_gradient_evaluations),
+                              {"step_size_final": float(res.step_size_final), "algorithm": "GRAPE",
+                               "n_controls": n_c, "n_timesteps": n_t, "dt": dt, "total_time": dt * n_t})
+
+
+def _grape_optimize_host(system, target: QuantumTarget, controls_init: ControlSequence,
+                         config: GRAPEConfig) -> OptimizationResult:
     t_start = time.time()
     dt, n_t, n_c = controls_init.dt, controls_init.n_timesteps, system.n_controls
     if controls_init.controls.shape != (n_c, n_t):

@@ -29,8 +29,26 @@ PR_ERR = {
 EXPORTS = ("pr_abi_version", "pr_device_count", "pr_init", "pr_last_error", "pr_plan_create",
            "pr_eval", "pr_eval_device", "pr_plan_n_params", "pr_plan_set_timing",
            "pr_plan_kernel_times", "pr_plan_buffers", "pr_plan_upload_controls",
-           "pr_plan_synchronize", "pr_plan_destroy")
+           "pr_plan_synchronize", "pr_plan_destroy", "pr_grape_optimize")
 KERNEL_CLASSES = ("segment", "combine", "gradient", "reduce")
+
+
+class GrapeConfig(ctypes.Structure):
+    """pr_grape_config (GRAPEConfig's loop knobs, GRAPE.jl:60-127)."""
+
+    _fields_ = [("max_iter", ctypes.c_int), ("convergence_tol", ctypes.c_double), ("step_size", ctypes.c_double),
+                ("adapt_step_size", ctypes.c_int), ("min_step_size", ctypes.c_double),
+                ("max_step_size", ctypes.c_double), ("gradient_norm_tol", ctypes.c_double)]
+
+
+class GrapeResult(ctypes.Structure):
+    _fields_ = [("n_iterations", ctypes.c_int), ("converged", ctypes.c_int), ("reason", ctypes.c_int),
+                ("n_fidelity_history", ctypes.c_int), ("n_fidelity_evaluations", ctypes.c_int),
+                ("n_gradient_evaluations", ctypes.c_int), ("fidelity", ctypes.c_double),
+                ("step_size_final", ctypes.c_double)]
+
+
+PR_STOP = {0: "max_iter", 1: "perfect", 2: "gradient", 3: "fidelity"}
 
 
 class RocmError(RuntimeError):
@@ -68,6 +86,8 @@ def load(path: str = LIB_PATH):
     L.pr_plan_upload_controls.argtypes = [ctypes.c_void_p, ctypes.POINTER(dp)]
     L.pr_plan_synchronize.argtypes = [ctypes.c_void_p]
     L.pr_plan_destroy.argtypes = [ctypes.c_void_p]
+    L.pr_grape_optimize.argtypes = [ctypes.c_void_p, ctypes.c_void_p, dp, ctypes.POINTER(GrapeConfig), dp, dp, dp,
+                                    ctypes.POINTER(GrapeResult)]
     _lib = L
     return L
 
@@ -183,3 +203,19 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def grape_optimize_device(fid_plan: Plan, grad_plan, controls: np.ndarray, cfg: GrapeConfig):
+    """pr_grape_optimize: the gradient-ascent loop with controls resident on the
+    device. controls [n_ctrl, n_t]; returns (best controls, fidelity history,
+    gradient-norm history, GrapeResult)."""
+    u = np.asarray(controls, dtype=np.float64)
+    n_c, n_t = u.shape
+    u0 = controls_cm(u)
+    best = np.zeros_like(u0)
+    fh = np.zeros(cfg.max_iter + 2)
+    gh = np.zeros(cfg.max_iter)
+    res = GrapeResult()
+    check(load().pr_grape_optimize(fid_plan._h, grad_plan._h if grad_plan is not None else None, _dptr(u0),
+                                   ctypes.byref(cfg), _dptr(best), _dptr(fh), _dptr(gh), ctypes.byref(res)))
+    return (grad_from_cm(best, n_c, n_t), fh[:res.n_fidelity_history].tolist(), gh[:res.n_iterations].tolist(), res)

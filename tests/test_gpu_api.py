@@ -210,3 +210,31 @@ def test_grape_optimize_ensemble_matches_oracle_driven_loop():
     assert np.abs(np.array(res.fidelity_history) - np.array(ref[2])).max() < 1e-10
     assert res.fidelity >= res.fidelity_history[0]
     assert res.metadata["algorithm"] == "Ensemble L-BFGS-GRAPE"
+
+
+# ----------------------------------------------------------------- device GRAPE loop
+
+
+@pytest.mark.parametrize("case", ["state_fixed_point", "state_transfer", "gate"])
+def test_device_grape_loop_matches_host_loop(case):
+    if case == "gate":
+        rng = np.random.default_rng(3)
+        sys_ = P.quantum_system(0.5 * SZ, [0.5 * SX, 0.5 * SY])
+        tgt = P.unitary_target(SX)
+        ctrl = P.random_controls(sys_, np.pi, 50, amplitude=1.0, rng=rng)
+        cfg = P.GRAPEConfig(max_iter=150, step_size=0.05)
+    else:
+        sys_, tgt, ctrl = quickstart(psi_init=[1, 0] if case == "state_transfer" else None)
+        cfg = P.GRAPEConfig(max_iter=300, step_size=0.05)
+    dev = P.grape_optimize(sys_, tgt, ctrl, config=cfg)  # pr_grape_optimize
+    host = P.grape_optimize(sys_, tgt, ctrl, config=cfg, on_device=False)
+    assert dev.n_iterations == host.n_iterations
+    assert dev.converged == host.converged and dev.termination_reason.split()[0] == host.termination_reason.split()[0]
+    assert len(dev.fidelity_history) == len(host.fidelity_history)
+    assert np.abs(np.array(dev.fidelity_history) - np.array(host.fidelity_history)).max() <= 1e-13
+    assert np.allclose(dev.gradient_norm_history, host.gradient_norm_history, rtol=1e-12, atol=0)
+    assert np.abs(dev.controls - host.controls).max() <= 1e-12 * max(1.0, np.abs(host.controls).max())
+    assert abs(dev.fidelity - host.fidelity) <= 1e-13
+    assert dev.n_fidelity_evaluations == host.n_fidelity_evaluations
+    assert dev.n_gradient_evaluations == host.n_gradient_evaluations
+    assert dev.metadata["step_size_final"] == pytest.approx(host.metadata["step_size_final"], rel=1e-12)
