@@ -69,3 +69,65 @@ end
 # (H0_per_member = 1, ctrl_scale[m, :] .= p), pairs = ctrl.rho_init / ctrl.rho_targ,
 # dt_slices = ctrl.pulse_dt, objective = :square → PR_PAIRS_SQUARE, :real → PR_PAIRS_REAL;
 # the Lindblad branch passes ctrl._L_drifts / ctrl._L_controls with dynamics = PR_LINDBLAD.
+
+# grape_optimize(system, target, controls_init; config)  (src/Optimization/GRAPE.jl:469-643)
+# The whole gradient-ascent loop in the library (pr_grape_optimize): controls,
+# gradient and best controls stay in HBM; the step adaptation, update and
+# convergence rules are those of GRAPE.jl:261-402. Used when no callback or
+# printing is requested (those need the Julia loop).
+struct PRGrapeConfig
+    max_iter::Cint
+    convergence_tol::Cdouble
+    step_size::Cdouble
+    adapt_step_size::Cint
+    min_step_size::Cdouble
+    max_step_size::Cdouble
+    gradient_norm_tol::Cdouble
+end
+
+mutable struct PRGrapeResult
+    n_iterations::Cint
+    converged::Cint
+    reason::Cint
+    n_fidelity_history::Cint
+    n_fidelity_evaluations::Cint
+    n_gradient_evaluations::Cint
+    fidelity::Cdouble
+    step_size_final::Cdouble
+    PRGrapeResult() = new(0, 0, 0, 0, 0, 0, 0.0, 0.0)
+end
+
+function _rocm_grape_optimize(system::AbstractQuantumSystem, target::QuantumTarget,
+                              controls_init::ControlSequence, config::GRAPEConfig)::OptimizationResult
+    t0 = time()
+    n_c, n_t, dt = system.n_controls, controls_init.n_timesteps, controls_init.dt
+    fprob, froots = rocm_problem(system.H_drift, system.H_controls, n_t, dt, target; state_normalize = false)
+    fplan = ROCmPlan([fprob], froots)
+    gplan = nothing
+    if target.type != "unitary"   # the gradient uses target_state as init (Gradients.jl:156-157)
+        gprob, groots = rocm_problem(system.H_drift, system.H_controls, n_t, dt, target; fixed_point = true)
+        gplan = ROCmPlan([gprob], groots)
+    end
+    cfg = PRGrapeConfig(config.max_iter, config.convergence_tol, config.step_size, config.adapt_step_size,
+                        config.min_step_size, config.max_step_size, config.gradient_norm_tol)
+    u0 = Matrix{Float64}(controls_init.controls)
+    best = similar(u0)
+    fh = zeros(config.max_iter + 2)
+    gh = zeros(config.max_iter)
+    res = PRGrapeResult()
+    GC.@preserve u0 best fh gh begin
+        _rocm_check(ccall((:pr_grape_optimize, _ROCM_LIB[]), Cint,
+                          (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ref{PRGrapeConfig}, Ptr{Float64}, Ptr{Float64},
+                           Ptr{Float64}, Ref{PRGrapeResult}),
+                          fplan.handle, gplan === nothing ? C_NULL : gplan.handle, u0, cfg, best, fh, gh, res),
+                    "pr_grape_optimize")
+    end
+    reasons = ("maximum iterations reached", "perfect fidelity", "gradient norm below tolerance",
+               "fidelity change below tolerance")
+    return OptimizationResult(best, res.fidelity, fh[1:res.n_fidelity_history], gh[1:res.n_iterations],
+                              Int(res.n_iterations), res.converged != 0, reasons[res.reason + 1], time() - t0,
+                              Int(res.n_fidelity_evaluations), Int(res.n_gradient_evaluations),
+                              Dict{String, Any}("step_size_final" => res.step_size_final, "algorithm" => "GRAPE",
+                                                "n_controls" => n_c, "n_timesteps" => n_t, "dt" => dt,
+                                                "total_time" => dt * n_t))
+end
