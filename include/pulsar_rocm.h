@@ -71,7 +71,8 @@ extern "C" {
 #define PR_MIXED 1 /* d <= 16 closed dynamics, reference gradient: the exp polynomial
                       beyond first order and the gradient walk's increments on
                       FP32 MFMA, accumulated in FP64; generators, forward chain,
-                      traces and reductions FP64. Other problems run as PR_FP64. */
+                      traces and reductions FP64. Time segments with a generator
+                      above the degree-6 bound, and other problems, run as PR_FP64. */
 
 /* gradient formula */
 #define PR_GRAD_REFERENCE 0 /* first-order GRAPE formula of Gradients.jl:274-289 */

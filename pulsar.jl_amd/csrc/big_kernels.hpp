@@ -193,7 +193,7 @@ __device__ double assemble_norm(const ProbDev& P, int m, int k, const double* u,
     store_blk(X, Xb, x);
     double cs = 0;
 #pragma unroll
-    for (int r = 0; r < 4; r++) cs += fabs(Xb.re[r]) + fabs(Xb.im[r]);
+    for (int r = 0; r < 4; r++) cs += abs_bound(Xb.re[r], Xb.im[r]);
     cs += __shfl_xor(cs, 16);
     cs += __shfl_xor(cs, 32);
     if (x.q == 0) colsum[x.I * 16 * B + 16 * x.Jb + x.c] = cs;  // [row block][column]

@@ -75,6 +75,8 @@ struct ProbDev {
                               // D_k = U_k^dag - I in the FP32 own layout
   unsigned char* big;         // mixed precision: [M][nt] 1 when slice k took the FP64
                               // exp (its image is in Ust, not Udm)
+  int* seg_big;               // mixed precision: [M][nseg] 1 when the segment has a slice
+                              // above the degree-6 bound (redone in FP64)
   double2* Tseg;              // [M][nseg][NB][256]
   double2* Eseg;              // [M][nseg+1][NB][256]
   double2* Om;                // [M][NB][256]
@@ -95,7 +97,7 @@ struct WorkItem {
 // ||X||^10/10! <= 1.1e-16 for ||X|| <= 0.1147 (m = 9). Each slice takes
 // degree 6 (3 products) when its norm allows, else degree 9 (4 products)
 // with scaling and squaring.
-constexpr double THETA6 = 0.0175;
+constexpr double THETA6 = 0.01777;
 constexpr double THETA9 = 0.11;
 constexpr int TAYLOR_DEG = 9;
 
@@ -187,12 +189,20 @@ __device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __r
   return cscale(P.cre * dt, P.cim * dt, H);
 }
 
-// Upper bound of ||X||_1: largest column sum of |Re| + |Im| over the workgroup.
+// |z| <= max(|Re z|, |Im z|) + min(|Re z|, |Im z|) / 2 (at most 6% above |z|;
+// |Re| + |Im| can exceed it by 41% and push slices to the larger polynomial)
+__device__ inline double abs_bound(double a, double b) {
+  a = fabs(a);
+  b = fabs(b);
+  return fmax(a, b) + 0.5 * fmin(a, b);
+}
+
+// Upper bound of ||X||_1: largest column sum of the element bounds over the workgroup.
 template <int B>
 __device__ inline double norm1_blk(const Blk& X, double* red, const Ln& x) {
   double cs = 0;
 #pragma unroll
-  for (int r = 0; r < 4; r++) cs += fabs(X.re[r]) + fabs(X.im[r]);
+  for (int r = 0; r < 4; r++) cs += abs_bound(X.re[r], X.im[r]);
   cs += __shfl_xor(cs, 16);
   cs += __shfl_xor(cs, 32);  // lane c + 16q: column c of the block, all rows
   if constexpr (B > 1) {
@@ -417,22 +427,25 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
 #ifndef PR_SEGMENT_WAVES
 #define PR_SEGMENT_WAVES 4
 #endif
-// the mixed segment walk holds FP32 and FP64 operands of two slices; at a
-// 4-wave cap it spills (244 B per lane) and runs slower than at 3
 #ifndef PR_MIXED_SEGMENT_WAVES
-#define PR_MIXED_SEGMENT_WAVES 3
+#define PR_MIXED_SEGMENT_WAVES 4
 #endif
 #ifndef PR_GRADIENT_WAVES
 #define PR_GRADIENT_WAVES 0
 #endif
 // CLOSED: every problem of the launch has closed dynamics (anti-Hermitian
 // generators), so the d <= 16 exp takes the register-only form unconditionally.
-template <int B, bool CLOSED>
+// FIX (mixed plans): only work items that k_segment_mx flagged in P.seg_big
+// run, and they mark each slice as FP64 in P.big.
+template <int B, bool CLOSED, bool FIX = false>
 __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B == 1 ? PR_SEGMENT_WAVES : 0))) void k_segment(const ProbDev* __restrict__ probs,
                                                          const WorkItem* __restrict__ items,
                                                          const double* __restrict__ u) {
   const WorkItem it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
+  if constexpr (FIX) {
+    if (!P.seg_big[(size_t)it.member * P.nseg + it.seg]) return;
+  }
   const Ln x = lane_info<B>();
   double2 *LA, *LB, *zb;
   double* red;
@@ -490,6 +503,9 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
     }
     if (k + 1 < it.k1) Xn = assemble<B>(P, m, k + 1, u, load_drift<B>(P, m, x), x);
     store_blk(P.Ust + ((size_t)m * P.nt + k) * NB * 256, Ut, x);
+    if constexpr (FIX) {
+      if (threadIdx.x == 0) P.big[(size_t)m * P.nt + k] = 1;
+    }
     bsync<B>();
     if constexpr (B == 1) {
       put<B>(LT, acc, x);
@@ -517,10 +533,12 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
 // assembled directly in the FP32 own layout, and the loop is pipelined across
 // slices: the FP64 chain product of slice k runs on the matrix cores while
 // slice k+1 is assembled and its first FP32 product issued. Stores the FP32
-// image of D_k = U_k^dag - I for the mixed gradient walk. A slice whose
+// image of D_k = U_k^dag - I for the mixed gradient walk. When a slice's
 // generator exceeds the degree-6 bound (||X||_1 > THETA6, where the FP32
-// rounding of R_k and of the walk increments would no longer be small) takes
-// the FP64 exp and stores the FP64 image U_k^T instead, flagged in P.big.
+// rounding of R_k and of the walk increments would no longer be small), the
+// work item stops and flags itself in P.seg_big; k_segment<1, true, true>,
+// launched next, redoes flagged items in FP64 (FP64 images U_k^T, P.big set
+// for their slices) and skips the others.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEGMENT_WAVES))) void k_segment_mx(
     const ProbDev* __restrict__ probs, const WorkItem* __restrict__ items, const double* __restrict__ u) {
   const float c2 = 1.0f / 2, c3 = 1.0f / 6, c4 = 1.0f / 24, c5 = 1.0f / 120, c6 = 1.0f / 720;
@@ -535,6 +553,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
   put<1>(LT, eye_blk(x), x);
   bsync<1>();
   const double2* H0o = P.H0o + (P.h0pm ? (size_t)m * 256 : 0);
+  if (threadIdx.x == 0) P.seg_big[(size_t)m * P.nseg + it.seg] = 0;
   int smax = 0;
   // slice k0: generator (own layout), scaling, first FP32 product
   Blk Xo = assemble<1, true>(P, m, it.k0, u, load_blk(H0o, x), x);
@@ -551,8 +570,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
   X2 = prod32_adj(X32, X32);  // -X^2
   for (int k = it.k0; k < it.k1; k++) {
     smax = max(smax, s);
-    const bool small = deg6;  // degree 6 implies s == 0
-    if (small) {
+    if (!deg6) {
+      // a generator above the degree-6 bound: the FP64 segment kernel redoes
+      // this work item (launched right after, it reads the flag)
+      if (threadIdx.x == 0) P.seg_big[(size_t)m * P.nseg + it.seg] = 1;
+      return;
+    }
+    {
       const Blk32 X3 = prod32_adj(X32, X2);  // X^3
       Blk32 Q;
       Q.re = c6 * X3.re - c5 * X2.re + c4 * X32.re;
@@ -567,26 +591,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
         if (rowof32(x.q, r) == x.c) Uo.re[r] += 1.0;
         LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Uo.re[r], Uo.im[r]);
       }
-    } else {
-      // FP64 exp from the FP64 layout of X
-#pragma unroll
-      for (int r = 0; r < 4; r++) LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Xo.re[r], Xo.im[r]);
-      bsync<1>();
-      Blk X;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const double2 v = LA[lidx(0, rowof(x.q, r), x.c)];
-        X.re[r] = v.x;
-        X.im[r] = v.y;
-      }
-      bsync<1>();
-      put<1>(LA, expm_ah1(X, s, false, LA, LB, x), x);
     }
     bsync<1>();
     const Blk none = zero_blk();
     const Blk acc = gemm_prod<1, N, NL>(LA, none, LT, none, x);  // U_k T (FP64)
     const size_t mk = (size_t)m * P.nt + k;
-    if (small) {
+    {
       Blk32 D;  // D[4q + r][c] = conj(U[c][4q + r]) - delta
 #pragma unroll
       for (int r = 0; r < 4; r++) {
@@ -596,17 +606,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
         D.im[r] = (float)(-v.y);
       }
       store_blk32(P.Udm + mk * 256, D, x);
-    } else {
-      Blk Ut;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const double2 v = LA[lidx(0, x.c, rowof(x.q, r))];  // U[c][q + 4r]
-        Ut.re[r] = v.x;
-        Ut.im[r] = v.y;
-      }
-      store_blk(P.Ust + mk * 256, Ut, x);
     }
-    if (threadIdx.x == 0) P.big[mk] = small ? 0 : 1;
+    if (threadIdx.x == 0) P.big[mk] = 0;
     if (k + 1 < it.k1) {
       Xo = assemble<1, true>(P, m, k + 1, u, load_blk(H0o, x), x);
       s = choose_scaling(norm1_blk<1>(Xo, red, x), deg6);

@@ -451,9 +451,12 @@ int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
   const dim3 blk(64 * B * B);
   int rc;
   if ((rc = tbegin(pl, st))) return rc;
-  if (B == 1 && g.mixed)
+  if (B == 1 && g.mixed) {
     hipLaunchKernelGGL(pr::k_segment_mx, dim3(g.n_items), blk, pr::Smem<1>::segment, st, pl.d_probs, g.d_items, d_u);
-  else if (g.lindblad)
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL((pr::k_segment<1, true, true>), dim3(g.n_items), blk, pr::Smem<1>::segment, st, pl.d_probs,
+                       g.d_items, d_u);
+  } else if (g.lindblad)
     hipLaunchKernelGGL((pr::k_segment<B, false>), dim3(g.n_items), blk, pr::Smem<B>::segment, st, pl.d_probs, g.d_items, d_u);
   else
     hipLaunchKernelGGL((pr::k_segment<B, true>), dim3(g.n_items), blk, pr::Smem<B>::segment, st, pl.d_probs, g.d_items, d_u);
@@ -715,6 +718,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       // FP32 images of small-generator slices; the FP64 images above hold the others
       if ((rc = pl.alloc(&D.Udm, (size_t)M * nt * 256))) return rc;
       if ((rc = pl.alloc(&D.big, (size_t)M * nt))) return rc;
+      if ((rc = pl.alloc(&D.seg_big, (size_t)M * S))) return rc;
     }
     if ((rc = pl.alloc(&D.Tseg, (size_t)M * S * MB))) return rc;
     if ((rc = pl.alloc(&D.Eseg, (size_t)M * (S + 1) * MB))) return rc;

@@ -135,3 +135,16 @@ def test_mixed_matches_fp64_device_result_closely():
     Fm, Gm, _ = run(p, u, precision=PR_MIXED)
     assert abs(Fm - F64) <= 1e-10
     assert np.abs(Gm - G64).max() <= 1e-7 * np.abs(G64).max()
+
+
+def test_some_slices_above_the_bound_redo_their_segments_in_fp64():
+    # small generators except a few spikes: segments holding a spike are redone
+    # in FP64, the others stay mixed; both kinds meet the mixed tolerance
+    p, u = W.four_qubit_ensemble(n_t=400)
+    p, u = W.shrink(p, u, n_members=4)
+    u = u.copy()
+    u[:, 37] *= 40.0
+    u[:, 250] *= 40.0
+    for seg in (0, 16):
+        dF, dG, _, _ = errors(p, u, segment=seg)
+        assert dF <= TOL_F and dG <= TOL_G, (seg, dF, dG)
