@@ -61,6 +61,18 @@ struct ProbDev {
   const int* co_ptr;
   const int* co_j;
   const double2* co_v;
+  // slot form of the same entries for the assembly (nc <= 64; null otherwise):
+  // block w owns the passes [sl_pass[w], sl_pass[w + 1]); pass p holds, for
+  // register r and lane l, the p-th nonzero at that position as (4 j, A_j) in
+  // sl_idx[p * 64 + l][r] and sl_v[(p * 4 + r) * 64 + l], zero entries where a
+  // position has fewer. so_* is the same in the FP32 own layout (one block).
+  // The pass bounds sit in the struct so that they arrive with scalar loads.
+  const int4* sl_idx;
+  const double2* sl_v;
+  const int4* so_idx;
+  const double2* so_v;
+  int sl_pass[65];
+  int so_pass[2];
   const double* n0;           // [M or 1] ||A0||_1
   const double* ncn;          // [nc] ||A_j||_1
   const double* scale;        // [M][nc]
@@ -173,6 +185,31 @@ __device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __r
   const int* cj = OWN32 ? P.co_j : P.cs_j;
   const double2* cv = OWN32 ? P.co_v : P.cs_v;
   Blk H = H0;
+  const int4* sidx = OWN32 ? P.so_idx : P.sl_idx;
+  if (sidx) {
+    // lane j holds the weight s_mj u_jk; each slot entry fetches its control's
+    // weight from that lane, so the entry loads do not depend on one another
+    const double wl = x.lane < nc ? gptr(sc)[x.lane] * gptr(uk)[x.lane] : 0.0;
+    const int lo = __double2loint(wl), hi = __double2hiint(wl);
+    const double2* sv = OWN32 ? P.so_v : P.sl_v;
+    const int w = __builtin_amdgcn_readfirstlane(x.w);
+    const int p1 = OWN32 ? P.so_pass[1] : P.sl_pass[w + 1];
+    for (int p = OWN32 ? P.so_pass[0] : P.sl_pass[w]; p < p1; p++) {
+      typedef int iv4 __attribute__((ext_vector_type(4)));
+      const iv4 id = ((const PR_GLOBAL iv4*)sidx)[(unsigned)(p * 64 + x.lane)];
+      const int ids[4] = {id.x, id.y, id.z, id.w};
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double2 a = gload2(sv, (unsigned)((p * 4 + r) * 64 + x.lane));
+        const double wj = __hiloint2double(__builtin_amdgcn_ds_bpermute(ids[r], hi),
+                                           __builtin_amdgcn_ds_bpermute(ids[r], lo));
+        H.re[r] = fma(wj, a.x, H.re[r]);
+        H.im[r] = fma(wj, a.y, H.im[r]);
+      }
+    }
+    if (P.prescaled) return H;
+    return cscale(P.cre * dt, P.cim * dt, H);
+  }
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const unsigned pos = (unsigned)((x.w * 4 + r) * 64 + x.lane);

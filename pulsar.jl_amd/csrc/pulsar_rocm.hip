@@ -140,6 +140,49 @@ Csr build_csr(const double* Hc, int nc, int d, int B, bool transpose, bool own32
   return out;
 }
 
+// Slot form of a register-layout sparse form (nc <= 64, NB <= 64): block w gets as many
+// passes as its fullest position has entries; pass p holds the p-th entry of
+// every position of the block, zero where a position has fewer. The index is
+// 4 j, the ds_bpermute byte address of the lane that holds control j's weight.
+struct Slots {
+  std::vector<int> ptr;
+  std::vector<int4> idx;
+  std::vector<double2> v;
+};
+
+Slots build_slots(const Csr& c, int NB) {
+  Slots out;
+  out.ptr.assign(NB + 1, 0);
+  for (int w = 0; w < NB; w++) {
+    int passes = 0;
+    for (int pos = w * 256; pos < (w + 1) * 256; pos++) passes = std::max(passes, c.ptr[pos + 1] - c.ptr[pos]);
+    out.ptr[w + 1] = out.ptr[w] + passes;
+    for (int p = 0; p < passes; p++)
+      for (int l = 0; l < 64; l++) {
+        int id[4];
+        for (int r = 0; r < 4; r++) {
+          const int pos = (w * 4 + r) * 64 + l, e = c.ptr[pos] + p;
+          const bool has = e < c.ptr[pos + 1];
+          id[r] = has ? 4 * c.j[e] : 0;
+          out.v.push_back(has ? c.v[e] : make_double2(0, 0));
+        }
+        out.idx.push_back(make_int4(id[0], id[1], id[2], id[3]));
+      }
+  }
+  // v is [pass][lane][r] above; the device reads [pass][r][lane]
+  std::vector<double2> t(out.v.size());
+  const size_t np = out.idx.size() / 64;
+  for (size_t p = 0; p < np; p++)
+    for (int l = 0; l < 64; l++)
+      for (int r = 0; r < 4; r++) t[(p * 4 + r) * 64 + l] = out.v[(p * 64 + l) * 4 + r];
+  out.v.swap(t);
+  if (out.idx.empty()) {  // keep the device arrays non-empty
+    out.idx.push_back(make_int4(0, 0, 0, 0));
+    out.v.push_back(make_double2(0, 0));
+  }
+  return out;
+}
+
 // Trace operand of the gradient: per control j the nonzeros A_j[a][b] with the
 // LDS tile index of the element (b, a) of the matrix they multiply.
 struct Trace {
@@ -666,6 +709,14 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       if ((rc = pl.upload(&jp, co.j.data(), co.j.size()))) return rc;
       if ((rc = pl.upload(&vp, co.v.data(), co.v.size()))) return rc;
       D.co_ptr = ip; D.co_j = jp; D.co_v = vp;
+      if (nc <= 64) {
+        Slots so = build_slots(co, 1);
+        int4* sp;
+        if ((rc = pl.upload(&sp, so.idx.data(), so.idx.size()))) return rc;
+        if ((rc = pl.upload(&vp, so.v.data(), so.v.size()))) return rc;
+        D.so_idx = sp; D.so_v = vp;
+        std::copy(so.ptr.begin(), so.ptr.end(), D.so_pass);
+      }
     }
     if ((rc = pl.upload(&dd_, nrm.data(), nh0))) return rc;
     D.n0 = dd_;
@@ -680,6 +731,14 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       if ((rc = pl.upload(&jp, cs.j.data(), cs.j.size()))) return rc;
       if ((rc = pl.upload(&vp, cs.v.data(), cs.v.size()))) return rc;
       D.cs_ptr = ip; D.cs_j = jp; D.cs_v = vp;
+      if (nc <= 64) {
+        Slots sl = build_slots(cs, NB);
+        int4* sp;
+        if ((rc = pl.upload(&sp, sl.idx.data(), sl.idx.size()))) return rc;
+        if ((rc = pl.upload(&vp, sl.v.data(), sl.v.size()))) return rc;
+        D.sl_idx = sp; D.sl_v = vp;
+        std::copy(sl.ptr.begin(), sl.ptr.end(), D.sl_pass);
+      }
       // the gradient contracts the operators as given (Gradients.jl:274-289)
       Trace tr = build_trace(q.Hc, nc, d);
       if ((rc = pl.upload(&ip, tr.ptr.data(), tr.ptr.size()))) return rc;
