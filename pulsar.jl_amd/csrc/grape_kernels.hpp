@@ -172,6 +172,33 @@ __device__ inline double block_max(double v, double* red, const Ln& x) {
   return out;
 }
 
+// NP passes of the slot-form assembly starting at pass p: H += w_j A_j for the
+// entries of each position, in pass order. lo/hi: halves of the lane's weight.
+template <int NP>
+__device__ inline void slot_passes(Blk& H, const int4* sidx, const double2* sv, int p, int lo, int hi,
+                                   const Ln& x) {
+  typedef int iv4 __attribute__((ext_vector_type(4)));
+  iv4 id[NP];
+  double2 a[NP][4];
+#pragma unroll
+  for (int i = 0; i < NP; i++) {
+    id[i] = ((const PR_GLOBAL iv4*)sidx)[(unsigned)((p + i) * 64 + x.lane)];
+#pragma unroll
+    for (int r = 0; r < 4; r++) a[i][r] = gload2(sv, (unsigned)(((p + i) * 4 + r) * 64 + x.lane));
+  }
+#pragma unroll
+  for (int i = 0; i < NP; i++) {
+    const int ids[4] = {id[i].x, id[i].y, id[i].z, id[i].w};
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const double wj = __hiloint2double(__builtin_amdgcn_ds_bpermute(ids[r], hi),
+                                         __builtin_amdgcn_ds_bpermute(ids[r], lo));
+      H.re[r] = fma(wj, a[i][r].x, H.re[r]);
+      H.im[r] = fma(wj, a[i][r].y, H.im[r]);
+    }
+  }
+}
+
 // Generator of slice k, X = c dt_k (A0 + sum_j s_mj u_jk A_j). H0 is the
 // wavefront's block of the member's drift.
 template <int B, bool OWN32 = false>
@@ -194,19 +221,10 @@ __device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __r
     const double2* sv = OWN32 ? P.so_v : P.sl_v;
     const int w = __builtin_amdgcn_readfirstlane(x.w);
     const int p1 = OWN32 ? P.so_pass[1] : P.sl_pass[w + 1];
-    for (int p = OWN32 ? P.so_pass[0] : P.sl_pass[w]; p < p1; p++) {
-      typedef int iv4 __attribute__((ext_vector_type(4)));
-      const iv4 id = ((const PR_GLOBAL iv4*)sidx)[(unsigned)(p * 64 + x.lane)];
-      const int ids[4] = {id.x, id.y, id.z, id.w};
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const double2 a = gload2(sv, (unsigned)((p * 4 + r) * 64 + x.lane));
-        const double wj = __hiloint2double(__builtin_amdgcn_ds_bpermute(ids[r], hi),
-                                           __builtin_amdgcn_ds_bpermute(ids[r], lo));
-        H.re[r] = fma(wj, a.x, H.re[r]);
-        H.im[r] = fma(wj, a.y, H.im[r]);
-      }
-    }
+    int p = OWN32 ? P.so_pass[0] : P.sl_pass[w];
+    // one pass at a time (batching passes measured slower: the extra live
+    // registers spill at this kernel's 4-wavefront register budget)
+    for (; p < p1; p++) slot_passes<1>(H, sidx, sv, p, lo, hi, x);
     if (P.prescaled) return H;
     return cscale(P.cre * dt, P.cim * dt, H);
   }
@@ -234,15 +252,48 @@ __device__ inline double abs_bound(double a, double b) {
   return fmax(a, b) + 0.5 * fmin(a, b);
 }
 
+// Cross-lane steps on VALU (no LDS round trip). sum_rows_pairwise: every lane
+// gets (x_0 + x_1) + (x_2 + x_3) over the four 16-lane rows at its row offset,
+// the same sums as xor-16 then xor-32 exchanges. v_permlane16/32_swap on two
+// copies of v leave the lane's pair in the two outputs, whichever is which.
+__device__ inline double pair_sum16(double v) {
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+__device__ inline double pair_sum32(double v) {
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+__device__ inline double sum_rows_pairwise(double v) { return pair_sum32(pair_sum16(v)); }
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// maximum over the 16 lanes of each row: quad swaps, then half-row and row mirrors
+__device__ inline double max_in_row(double v) {
+  v = fmax(v, dpp_d<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = fmax(v, dpp_d<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = fmax(v, dpp_d<0x141>(v));  // row_half_mirror
+  return fmax(v, dpp_d<0x140>(v));  // row_mirror
+}
+
 // Upper bound of ||X||_1: largest column sum of the element bounds over the workgroup.
 template <int B>
 __device__ inline double norm1_blk(const Blk& X, double* red, const Ln& x) {
   double cs = 0;
 #pragma unroll
   for (int r = 0; r < 4; r++) cs += abs_bound(X.re[r], X.im[r]);
-  cs += __shfl_xor(cs, 16);
-  cs += __shfl_xor(cs, 32);  // lane c + 16q: column c of the block, all rows
-  if constexpr (B > 1) {
+  // lane c + 16q: column c of the block, all rows
+  cs = sum_rows_pairwise(cs);
+  if constexpr (B == 1) {
+    return max_in_row(cs);  // the largest column
+  } else {
     if (x.q == 0) red[x.w * 16 + x.c] = cs;
     __syncthreads();
     double best = 0;
@@ -253,10 +304,6 @@ __device__ inline double norm1_blk(const Blk& X, double* red, const Ln& x) {
     }
     __syncthreads();
     return best;
-  } else {
-#pragma unroll
-    for (int msk = 1; msk < 16; msk <<= 1) cs = fmax(cs, __shfl_xor(cs, msk));
-    return cs;
   }
 }
 
