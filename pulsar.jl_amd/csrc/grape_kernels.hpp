@@ -252,37 +252,6 @@ __device__ inline double abs_bound(double a, double b) {
   return fmax(a, b) + 0.5 * fmin(a, b);
 }
 
-// Cross-lane steps on VALU (no LDS round trip). sum_rows_pairwise: every lane
-// gets (x_0 + x_1) + (x_2 + x_3) over the four 16-lane rows at its row offset,
-// the same sums as xor-16 then xor-32 exchanges. v_permlane16/32_swap on two
-// copies of v leave the lane's pair in the two outputs, whichever is which.
-__device__ inline double pair_sum16(double v) {
-  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
-  const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
-  const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
-  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
-}
-__device__ inline double pair_sum32(double v) {
-  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
-  const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
-  const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
-  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
-}
-__device__ inline double sum_rows_pairwise(double v) { return pair_sum32(pair_sum16(v)); }
-template <int CTRL>
-__device__ inline double dpp_d(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-// maximum over the 16 lanes of each row: quad swaps, then half-row and row mirrors
-__device__ inline double max_in_row(double v) {
-  v = fmax(v, dpp_d<0xB1>(v));   // quad_perm [1,0,3,2]
-  v = fmax(v, dpp_d<0x4E>(v));   // quad_perm [2,3,0,1]
-  v = fmax(v, dpp_d<0x141>(v));  // row_half_mirror
-  return fmax(v, dpp_d<0x140>(v));  // row_mirror
-}
-
 // Upper bound of ||X||_1: largest column sum of the element bounds over the workgroup.
 template <int B>
 __device__ inline double norm1_blk(const Blk& X, double* red, const Ln& x) {
@@ -552,7 +521,8 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
     Blk X = Xn;
     bool deg6;
     const int s = choose_scaling(norm1_blk<B>(X, red, x), deg6);
-    if (s) {
+    {
+      // unconditional (exact for s = 0): a branch here costs register copies
       const double f = ldexp(1.0, -s);
       X.re *= f;
       X.im *= f;
@@ -642,12 +612,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
   // slice k0: generator (own layout), scaling, first FP32 product
   Blk Xo = assemble<1, true>(P, m, it.k0, u, load_blk(H0o, x), x);
   bool deg6;
+  // (s > 0 only above the degree-6 bound, where the work item exits unscaled)
   int s = choose_scaling(norm1_blk<1>(Xo, red, x), deg6);
-  if (s) {
-    const double f = ldexp(1.0, -s);
-    Xo.re *= f;
-    Xo.im *= f;
-  }
   Blk32 X32, X2;
   X32.re = __builtin_convertvector(Xo.re, f4);
   X32.im = __builtin_convertvector(Xo.im, f4);
@@ -695,11 +661,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
     if (k + 1 < it.k1) {
       Xo = assemble<1, true>(P, m, k + 1, u, load_blk(H0o, x), x);
       s = choose_scaling(norm1_blk<1>(Xo, red, x), deg6);
-      if (s) {
-        const double f = ldexp(1.0, -s);
-        Xo.re *= f;
-        Xo.im *= f;
-      }
       X32.re = __builtin_convertvector(Xo.re, f4);
       X32.im = __builtin_convertvector(Xo.im, f4);
       if (deg6) X2 = prod32_adj(X32, X32);
@@ -828,13 +789,15 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
 
 // Gradient contraction of one slice: G_m[j,k] = Re(c dt s_mj Tr(A_j M)) with
 // Tr(A_j M) = sum over the nonzeros A_j[a][b] of A_j[a][b] M[b][a]. M is staged
-// in the LDS tile LA; the workgroup's threads form V groups of G = 64 B^2 / V
-// lanes (at most one wavefront), group g walks the nonzeros of control j0 + g
-// and reduces them inside its lanes.
+// in the LDS tile LA; the workgroup's threads form V groups of G lanes, G the
+// largest power of two with G <= 64 and V G <= 64 B^2 (so a group never spans
+// two wavefronts), group g walks the nonzeros of control j0 + g and reduces
+// them inside its lanes.
+constexpr int floor_pow2(int n) { return n >= 64 ? 64 : n >= 32 ? 32 : n >= 16 ? 16 : n >= 8 ? 8 : n >= 4 ? 4 : n >= 2 ? 2 : 1; }
 template <int B, int V, bool OWN32 = false>
 __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, double2* LA, const Ln& x) {
   constexpr int THREADS = 64 * B * B;
-  constexpr int G = (THREADS / V) < 64 ? (THREADS / V) : 64;
+  constexpr int G = floor_pow2(THREADS / V);
   const int nc = P.nc;
   const bool want_im = (P.cim != 0.0);  // closed: Re(-i z) = Im z
   if constexpr (OWN32) {
@@ -858,8 +821,7 @@ __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, d
         sv += want_im ? (a.x * mv.y + a.y * mv.x) : (a.x * mv.x - a.y * mv.y);
       }
     }
-#pragma unroll
-    for (int msk = 1; msk < G; msk <<= 1) sv += __shfl_xor(sv, msk);
+    sv = group_sum<G>(sv);
     if (grp < V && gl == 0 && j < nc) {
       const double f = P.dts[k] * P.scale[(size_t)m * nc + j];
       // Re(c * f * z): closed c = -i -> f Im z ; Lindblad c = 1 -> f Re z
@@ -915,8 +877,7 @@ __device__ inline void contract_regs(const ProbDev& P, int m, int k, const Blk& 
     const double2 mv = LA[t.idx[i]];
     sv += want_im ? (t.v[i].x * mv.y + t.v[i].y * mv.x) : (t.v[i].x * mv.x - t.v[i].y * mv.y);
   }
-#pragma unroll
-  for (int msk = 1; msk < G; msk <<= 1) sv += __shfl_xor(sv, msk);
+  sv = group_sum<G>(sv);
   if (gl == 0 && grp < P.nc) {
     const double f = P.dts[k] * P.scale[(size_t)m * P.nc + grp];
     gptr_w(P.Gm)[(size_t)m * P.nc * P.nt + (size_t)k * P.nc + grp] = want_im ? f * sv * (-P.cim) : f * sv * P.cre;

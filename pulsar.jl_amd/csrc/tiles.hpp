@@ -254,6 +254,52 @@ __device__ inline Blk cscale(double ar, double ai, const Blk& X) {
   return r;
 }
 
+// Cross-lane steps on VALU (no LDS round trip). sum_rows_pairwise: every lane
+// gets (x_0 + x_1) + (x_2 + x_3) over the four 16-lane rows at its row offset,
+// the same sums as xor-16 then xor-32 exchanges. v_permlane16/32_swap on two
+// copies of v leave the lane's pair in the two outputs, whichever is which.
+__device__ inline double pair_sum16(double v) {
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+__device__ inline double pair_sum32(double v) {
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+__device__ inline double sum_rows_pairwise(double v) { return pair_sum32(pair_sum16(v)); }
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// maximum over the 16 lanes of each row: quad swaps, then half-row and row mirrors
+__device__ inline double max_in_row(double v) {
+  v = fmax(v, dpp_d<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = fmax(v, dpp_d<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = fmax(v, dpp_d<0x141>(v));  // row_half_mirror
+  return fmax(v, dpp_d<0x140>(v));  // row_mirror
+}
+
+// Sum over aligned groups of G lanes (G a power of two <= 64). Each step's
+// partner lane holds the same value as the xor-(1, 2, ..., G/2) partner, so
+// the sums equal those of xor exchanges bit for bit.
+template <int G>
+__device__ inline double group_sum(double v) {
+  static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "G must be a power of two <= 64");
+  if constexpr (G >= 2) v += dpp_d<0xB1>(v);   // lane ^ 1
+  if constexpr (G >= 4) v += dpp_d<0x4E>(v);   // lane ^ 2
+  if constexpr (G >= 8) v += dpp_d<0x141>(v);  // other quad of the 8-lane group
+  if constexpr (G >= 16) v += dpp_d<0x140>(v); // other half of the row
+  if constexpr (G >= 32) v = pair_sum16(v);
+  if constexpr (G >= 64) v = pair_sum32(v);
+  return v;
+}
+
 // Sum of V = 2^H values over the 64 lanes of a wavefront. On return lane l
 // holds the total of value index l >> (6 - H) (every lane of its group).
 template <int V>

@@ -334,10 +334,11 @@ def test_mixed_dimensions_up_to_128_in_one_plan():
         assert np.abs(g - G0).max() <= TOL_G * np.abs(G0).max()
 
 
-@pytest.mark.parametrize("d,nc", [(4, 40), (8, 33), (16, 70), (24, 5)])
+@pytest.mark.parametrize("d,nc", [(4, 40), (8, 33), (16, 70), (24, 5), (40, 12), (40, 20)])
 def test_many_controls_general_contraction(d, nc):
-    # more controls than one lane group holds (32 groups), dense controls, and a
-    # two-block problem: the general trace path and its reduction widths
+    # more controls than one lane group holds (32 groups), dense controls, and
+    # two- and three-block problems: the general trace path and its reduction
+    # widths (d = 40 with 16 or 32 lane groups: 576 threads, groups of 32 or 16)
     rng = np.random.default_rng(400 + d + nc)
     H0 = W.random_hermitian(d, rng, 2.0)
     Hc = np.stack([W.random_hermitian(d, rng, 0.3) for _ in range(nc)])
