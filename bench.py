@@ -11,9 +11,11 @@ already resident in HBM (pr_eval_device on the plan's buffers).
 Printed JSON line: the metric, the roofline of the dominant kernel (kernel
 time from HIP events recorded by the library on its launch stream), the
 accuracy of the device result against the FP64 CPU oracle on a member sample,
-and the oracle timed on this host's cores (the CPU baseline).
+and the oracle timed on this host's cores (the CPU baseline). The FP64 run
+(the default) also times the mixed-precision form on the same inputs and
+reports it, with its own accuracy, under "mixed_precision_form".
 
-    python bench.py [--gpus 1] [--steps 30] [--warmup 3] [--precision fp64|mixed]
+    python bench.py [--gpus 1] [--steps 30] [--warmup 3] [--precision fp64|mixed] [--fp64-only]
 """
 import argparse
 import json
@@ -207,6 +209,38 @@ def other_workload(args):
                       "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}}))
 
 
+def timed_form(p, u, prec, args):
+    """Warm-up, then K evaluations with the library's per-kernel events and K
+    without them (the headline time is the faster loop). Returns the open plan,
+    the elapsed seconds of K evaluations and the per-kernel times."""
+    plan = native.Plan([p], precision=prec, segment=args.segment)
+    plan.upload_controls([u])
+    d_u, d_F, d_G = plan.buffers()
+    for _ in range(args.warmup):
+        plan.eval_device(d_u, d_F, d_G)
+    plan.synchronize()
+
+    plan.set_timing(True)
+    plan.kernel_times()  # reset
+    plan.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.eval_device(d_u, d_F, d_G)
+    plan.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = plan.kernel_times()
+    plan.set_timing(False)
+
+    # the same timed loop without event records, for the headline number
+    plan.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.eval_device(d_u, d_F, d_G)
+    plan.synchronize()
+    elapsed_plain = time.perf_counter() - t0
+    return plan, min(elapsed, elapsed_plain), kt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +253,8 @@ def main():
                     help="the headline metric is four_qubit_ensemble; the others print timing lines of their own")
     ap.add_argument("--cpu-sample", type=int, default=32, help="members timed for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="skip the mixed-precision form that the FP64 run also times")
     ap.add_argument("--segment", type=int, default=0,
                     help="slices per time segment of the chain scan (0: the library's choice)")
     ap.add_argument("--probe", action="store_true",
@@ -246,33 +282,7 @@ def main():
     prec = PR_FP64 if args.precision == "fp64" else PR_MIXED
     p, u = workloads.four_qubit_ensemble()
     native.init(0)
-    plan = native.Plan([p], precision=prec, segment=args.segment)
-    plan.upload_controls([u])
-    d_u, d_F, d_G = plan.buffers()
-    for _ in range(args.warmup):
-        plan.eval_device(d_u, d_F, d_G)
-    plan.synchronize()
-
-    plan.set_timing(True)
-    plan.kernel_times()  # reset
-    plan.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.eval_device(d_u, d_F, d_G)
-    plan.synchronize()
-    elapsed = time.perf_counter() - t0
-    kt = plan.kernel_times()
-    plan.set_timing(False)
-
-    # the same timed loop without event records, for the headline number
-    plan.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.eval_device(d_u, d_F, d_G)
-    plan.synchronize()
-    elapsed_plain = time.perf_counter() - t0
-    elapsed = min(elapsed, elapsed_plain)
-
+    plan, elapsed, kt = timed_form(p, u, prec, args)
     evals_s = args.steps / elapsed
     M, nt, d, nc = p.n_members, p.n_t, p.dim, p.n_ctrl
     units = M * nt
@@ -330,6 +340,22 @@ def main():
                            "max_rel_dG": float(np.abs(Gd[0] - Go).max() / np.abs(Go).max()),
                            "unitarity_defect": float(D[0, 0]), "F": float(Fd[0])}
     plan.close()
+    if prec == PR_FP64 and not args.fp64_only:
+        # the mixed-precision form (FP32-MFMA exp polynomial and walk increments)
+        # on the same inputs, beside the FP64 value
+        mplan, m_el, m_kt = timed_form(p, u, PR_MIXED, args)
+        mplan.close()
+        mixed = {"value": round(args.steps / m_el, 3), "ms_per_step": round(1e3 * m_el / args.steps, 4),
+                 "vs_fp64": round(elapsed / m_el, 3), "dtype": "f64+f32",
+                 "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in m_kt.items()}}
+        if args.cpu_sample > 0:
+            qplan = native.Plan([q], precision=PR_MIXED)
+            Fm, Gm, Dm = qplan.eval([u], diag=True)
+            qplan.close()
+            mixed["accuracy"] = {"members": args.cpu_sample, "abs_dF": float(abs(Fm[0] - Fo)),
+                                 "max_rel_dG": float(np.abs(Gm[0] - Go).max() / np.abs(Go).max()),
+                                 "unitarity_defect": float(Dm[0, 0])}
+        out["mixed_precision_form"] = mixed
     print(json.dumps(out))
 
 

@@ -368,7 +368,7 @@ __global__ __launch_bounds__(1024) void k_gradient_big(const ProbDev* __restrict
 #pragma unroll
       for (int msk = 32; msk >= 1; msk >>= 1) sv += __shfl_xor(sv, msk);
       if (lane == 0) {
-        const double f = P.dts[k] * P.scale[(size_t)m * nc + j];
+        const double f = slice_dt(P, k) * P.scale[(size_t)m * nc + j];
         gptr_w(P.Gm)[(size_t)m * nc * P.nt + (size_t)k * nc + j] = f * sv;
       }
     }

@@ -42,6 +42,7 @@ struct ProbDev {
   int prescaled;              // 1: drift images and control entries already hold c dt A
                               // (uniform dt), so assemble() needs no final scaling
   double cre, cim;            // generator factor c: (0,-1) closed, (1,0) Lindblad
+  double dt0;                 // the slice length when prescaled (uniform dt)
   const double2* H0;          // [M or 1][NB][256] tile images of the drift
   // controls A_j in register-layout sparse form: for block w, register r and
   // lane l the nonzero entries (j, value) at that position are
@@ -103,6 +104,10 @@ struct ProbDev {
 struct WorkItem {
   int prob, member, seg, k0, k1;
 };
+
+// Length of slice k: the uniform dt from the problem record (a scalar load)
+// when the slices are uniform, else the per-slice array.
+__device__ inline double slice_dt(const ProbDev& P, int k) { return P.prescaled ? P.dt0 : P.dts[k]; }
 
 // exp truncation: the Taylor remainder of degree m is below 2^-53 relative
 // for ||X||_1 <= theta_m: ||X||^7/7! <= 1.1e-16 for ||X|| <= 0.01777 (m = 6),
@@ -207,7 +212,7 @@ __device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __r
   const int nc = P.nc;
   const double* sc = P.scale + (size_t)m * nc;
   const double* uk = u + P.coff + (size_t)k * nc;
-  const double dt = P.dts[k];
+  const double dt = slice_dt(P, k);
   const int* cptr = OWN32 ? P.co_ptr : P.cs_ptr;
   const int* cj = OWN32 ? P.co_j : P.cs_j;
   const double2* cv = OWN32 ? P.co_v : P.cs_v;
@@ -823,7 +828,7 @@ __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, d
     }
     sv = group_sum<G>(sv);
     if (grp < V && gl == 0 && j < nc) {
-      const double f = P.dts[k] * P.scale[(size_t)m * nc + j];
+      const double f = slice_dt(P, k) * P.scale[(size_t)m * nc + j];
       // Re(c * f * z): closed c = -i -> f Im z ; Lindblad c = 1 -> f Re z
       gptr_w(P.Gm)[(size_t)m * nc * P.nt + (size_t)k * nc + j] = want_im ? f * sv * (-P.cim) : f * sv * P.cre;
     }
@@ -840,16 +845,19 @@ template <int V>
 struct TraceRegs {
   int idx[TR_REG];
   double2 v[TR_REG];
+  double sc;  // s_mj of the lane group's control j (member m)
 };
 
 template <int V>
-__device__ inline void trace_regs_load(const ProbDev& P, TraceRegs<V>& t, const Ln& x) {
+__device__ inline void trace_regs_load(const ProbDev& P, int m, TraceRegs<V>& t, const Ln& x) {
   constexpr int G = 64 / V;
   const int grp = x.lane / G, gl = x.lane % G;
   int e0 = 0, e1 = 0;
+  t.sc = 0;
   if (grp < P.nc) {
     e0 = gptr(P.tr_ptr)[grp];
     e1 = gptr(P.tr_ptr)[grp + 1];
+    t.sc = gptr(P.scale)[(size_t)m * P.nc + grp];
   }
 #pragma unroll
   for (int i = 0; i < TR_REG; i++) {
@@ -860,9 +868,11 @@ __device__ inline void trace_regs_load(const ProbDev& P, TraceRegs<V>& t, const 
 }
 
 // contract() with the trace operand from registers; OWN32: M is held in the
-// FP32 own layout (rows 4q + r) instead of the FP64 one
+// FP32 own layout (rows 4q + r) instead of the FP64 one. dtk: the length of
+// slice k, fetched by the caller together with its prefetch of the next slice
+// (a load here would wait for that prefetch too).
 template <int V, bool OWN32>
-__device__ inline void contract_regs(const ProbDev& P, int m, int k, const Blk& Mk, double2* LA,
+__device__ inline void contract_regs(const ProbDev& P, int m, int k, double dtk, const Blk& Mk, double2* LA,
                                      const TraceRegs<V>& t, const Ln& x) {
   constexpr int G = 64 / V;
   const int grp = x.lane / G, gl = x.lane % G;
@@ -879,7 +889,7 @@ __device__ inline void contract_regs(const ProbDev& P, int m, int k, const Blk& 
   }
   sv = group_sum<G>(sv);
   if (gl == 0 && grp < P.nc) {
-    const double f = P.dts[k] * P.scale[(size_t)m * P.nc + grp];
+    const double f = dtk * t.sc;
     gptr_w(P.Gm)[(size_t)m * P.nc * P.nt + (size_t)k * P.nc + grp] = want_im ? f * sv * (-P.cim) : f * sv * P.cre;
   }
   bsync<1>();
@@ -926,22 +936,28 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT ? PR_GRADIENT_WAVES : 0))) v
     // U_{k+1} is fetched and the contraction of slice k runs while the
     // product C_{k+1} = U_k M_k is on the matrix cores
     TraceRegs<V> tregs;
-    if constexpr (REG) trace_regs_load<V>(P, tregs, x);
+    if constexpr (REG) trace_regs_load<V>(P, m, tregs, x);
+    double dtk = gptr(P.dts)[it.k0];
     Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + it.k0) * MB, x);
     for (int k = it.k0; k < it.k1; k++) {
       Blk Ud = conj_blk(Ut);                      // U^dag, own block
       Blk Mk = mm<B, N, N>(Cm, Ud, LA, LB, x);    // C_k U_k^dag
       Blk Un = Ut;
-      if (k + 1 < it.k1) Un = load_blk(P.Ust + ((size_t)m * P.nt + k + 1) * MB, x);
+      double dtn = dtk;
+      if (k + 1 < it.k1) {
+        dtn = gptr(P.dts)[k + 1];
+        Un = load_blk(P.Ust + ((size_t)m * P.nt + k + 1) * MB, x);
+      }
       Cm = mm<B, T, N>(Ut, Mk, LA, LB, x);        // U_k M_k
       if constexpr (EXACT) {
         contract<B, V>(P, m, k, exact_direction<B>(P, m, k, u, H0, Mk, LA, LB, red, x), LA, x);
       } else if constexpr (REG && B == 1) {
-        contract_regs<V, false>(P, m, k, Mk, LA, tregs, x);
+        contract_regs<V, false>(P, m, k, dtk, Mk, LA, tregs, x);
       } else {
         contract<B, V>(P, m, k, Mk, LA, x);
       }
       Ut = Un;
+      dtk = dtn;
     }
     return;
   }
@@ -1011,14 +1027,25 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
   }
   bsync<1>();
   TraceRegs<V> tregs;
-  if constexpr (REG) trace_regs_load<V>(P, tregs, x);
+  if constexpr (REG) trace_regs_load<V>(P, m, tregs, x);
   const unsigned char* big = P.big + (size_t)m * P.nt;
+  // the FP64-slice flags of 64 slices at a time as a wave-uniform bit mask
+  // (no per-slice load and wait); bit i: slice kb + i
+  int kb = it.k0;
+  unsigned long long bigmask = __ballot(kb + x.lane < it.k1 && gptr(big)[kb + x.lane] != 0);
+  double dtk = gptr(P.dts)[it.k0];
   Blk32 D;
-  if (!gptr(big)[it.k0]) D = load_blk32(P.Udm + ((size_t)m * P.nt + it.k0) * 256, x);
+  if (!(bigmask & 1ull)) D = load_blk32(P.Udm + ((size_t)m * P.nt + it.k0) * 256, x);
   for (int k = it.k0; k < it.k1; k++) {
-    const bool nsmall = k + 1 < it.k1 && !gptr(big)[k + 1];
+    const bool kbig = (bigmask >> (k - kb)) & 1ull;
+    if (k + 1 - kb == 64) {
+      kb = k + 1;
+      bigmask = __ballot(kb + x.lane < it.k1 && gptr(big)[kb + x.lane] != 0);
+    }
+    const bool nsmall = k + 1 < it.k1 && !((bigmask >> (k + 1 - kb)) & 1ull);
+    const double dtn = k + 1 < it.k1 ? gptr(P.dts)[k + 1] : dtk;
     Blk Mk;
-    if (!gptr(big)[k]) {
+    if (!kbig) {
       Blk32 C32;
       C32.re = __builtin_convertvector(Cm.re, f4);
       C32.im = __builtin_convertvector(Cm.im, f4);
@@ -1075,9 +1102,10 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
       if (nsmall) D = load_blk32(P.Udm + ((size_t)m * P.nt + k + 1) * 256, x);
     }
     if constexpr (REG)
-      contract_regs<V, true>(P, m, k, Mk, LA, tregs, x);
+      contract_regs<V, true>(P, m, k, dtk, Mk, LA, tregs, x);
     else
       contract<1, V, true>(P, m, k, Mk, LA, x);
+    dtk = dtn;
   }
 }
 

@@ -676,6 +676,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     // with a uniform slice length the assembly operators carry the factor
     // c dt (c = -i closed, 1 Lindblad): X = c dt A0 + sum_j s u_j (c dt A_j)
     D.prescaled = q.dt_slices ? 0 : 1;
+    D.dt0 = q.dt;
     if (D.prescaled) {
       const double fr = D.cre * q.dt, fi = D.cim * q.dt;
       for (auto* v : {&h0h, &hch})
