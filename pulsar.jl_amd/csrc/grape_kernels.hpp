@@ -41,6 +41,8 @@ struct ProbDev {
   int d, B, nc, nt, M, dyn, obj, npairs, nseg, seglen, h0pm, unitary, grad_exact;
   int prescaled;              // 1: drift images and control entries already hold c dt A
                               // (uniform dt), so assemble() needs no final scaling
+  int recompute;              // 1: no U_k images (Ust is null); the gradient walk
+                              // recomputes U_k (closed dynamics, d <= 64, FP64)
   double cre, cim;            // generator factor c: (0,-1) closed, (1,0) Lindblad
   double dt0;                 // the slice length when prescaled (uniform dt)
   const double2* H0;          // [M or 1][NB][256] tile images of the drift
@@ -561,7 +563,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
       Ut.im[r] = v.y;
     }
     if (k + 1 < it.k1) Xn = assemble<B>(P, m, k + 1, u, load_drift<B>(P, m, x), x);
-    store_blk(P.Ust + ((size_t)m * P.nt + k) * NB * 256, Ut, x);
+    if (!P.recompute) store_blk(P.Ust + ((size_t)m * P.nt + k) * NB * 256, Ut, x);
     if constexpr (FIX) {
       if (threadIdx.x == 0) P.big[(size_t)m * P.nt + k] = 1;
     }
@@ -895,10 +897,46 @@ __device__ inline void contract_regs(const ProbDev& P, int m, int k, double dtk,
   bsync<1>();
 }
 
+// U_k^T, this wavefront's block, recomputed for the gradient walk of a problem
+// without stored images (closed dynamics): the same assembly, norm bound,
+// scaling and exp calls as k_segment, so the result is the stored image bit
+// for bit; the transpose goes through the LDS tile LA.
+template <int B>
+__device__ inline Blk recompute_ut(const ProbDev& P, int m, int k, const double* __restrict__ u, double2* LA,
+                                   double2* LB, double* red, const Ln& x) {
+  Blk X = assemble<B>(P, m, k, u, load_drift<B>(P, m, x), x);
+  bool deg6;
+  const int s = choose_scaling(norm1_blk<B>(X, red, x), deg6);
+  {
+    const double f = ldexp(1.0, -s);
+    X.re *= f;
+    X.im *= f;
+  }
+  Blk U;
+  if constexpr (B == 1)
+    U = expm_ah1(X, s, deg6, LA, LB, x);
+  else
+    U = expm_blk<B>(X, s, deg6, LA, LB, x);
+  bsync<B>();
+  put<B>(LA, U, x);
+  bsync<B>();
+  Blk Ut;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const double2 v = LA[lidx(x.Jb * B + x.I, x.c, rowof(x.q, r))];  // U[16J + c][16I + q + 4r]
+    Ut.re[r] = v.x;
+    Ut.im[r] = v.y;
+  }
+  bsync<B>();
+  return Ut;
+}
+
 // ---------------------------------------------------------------- k_gradient
-template <int B, int V, bool EXACT, bool REG = false>
+// RECOMP: problems flagged P.recompute get U_k from recompute_ut instead of
+// their (absent) images.
+template <int B, int V, bool EXACT, bool REG = false, bool RECOMP = false>
 __global__ __launch_bounds__(64 * B * B)
-__attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT ? PR_GRADIENT_WAVES : 0))) void k_gradient(const ProbDev* __restrict__ probs,
+__attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAVES : 0))) void k_gradient(const ProbDev* __restrict__ probs,
                                                           const WorkItem* __restrict__ items,
                                                           const double* __restrict__ u) {
   const WorkItem it = items[blockIdx.x];
@@ -938,7 +976,11 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT ? PR_GRADIENT_WAVES : 0))) v
     TraceRegs<V> tregs;
     if constexpr (REG) trace_regs_load<V>(P, m, tregs, x);
     double dtk = gptr(P.dts)[it.k0];
-    Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + it.k0) * MB, x);
+    Blk Ut;
+    if (RECOMP && P.recompute)
+      Ut = recompute_ut<B>(P, m, it.k0, u, LA, LB, red, x);
+    else
+      Ut = load_blk(P.Ust + ((size_t)m * P.nt + it.k0) * MB, x);
     for (int k = it.k0; k < it.k1; k++) {
       Blk Ud = conj_blk(Ut);                      // U^dag, own block
       Blk Mk = mm<B, N, N>(Cm, Ud, LA, LB, x);    // C_k U_k^dag
@@ -946,7 +988,10 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT ? PR_GRADIENT_WAVES : 0))) v
       double dtn = dtk;
       if (k + 1 < it.k1) {
         dtn = gptr(P.dts)[k + 1];
-        Un = load_blk(P.Ust + ((size_t)m * P.nt + k + 1) * MB, x);
+        if (RECOMP && P.recompute)
+          Un = recompute_ut<B>(P, m, k + 1, u, LA, LB, red, x);
+        else
+          Un = load_blk(P.Ust + ((size_t)m * P.nt + k + 1) * MB, x);
       }
       Cm = mm<B, T, N>(Ut, Mk, LA, LB, x);        // U_k M_k
       if constexpr (EXACT) {

@@ -288,6 +288,7 @@ struct Plan {
     bool lindblad = false;  // some problem of the group has Lindblad dynamics
     bool mixed = false;     // FP32-MFMA exp and gradient walk (d <= 16 only)
     bool trace_regs = false;  // mixed walk keeps the trace operand in registers
+    bool recompute = false;   // some problem of the group recomputes U_k in the gradient walk
     double2* big_scratch = nullptr;  // d > 64: per work item matrices in HBM
     pr::WorkItem* d_items = nullptr;
     int n_items = 0;
@@ -393,6 +394,10 @@ template <int B, int V>
 int set_grad_attr(size_t bytes) {
   HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, V, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, V, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, V, false, false, true>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  HIPCHK(hipFuncSetAttribute((const void*)pr::k_gradient<B, V, true, false, true>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   return PR_OK;
 }
 
@@ -468,12 +473,18 @@ int launch_gradient_mixed(Plan& pl, const Plan::Group& g, const double* d_u, hip
 template <int B>
 int launch_gradient(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t st) {
   if (B == 1 && g.mixed) return launch_gradient_mixed(pl, g, d_u, st);
-  const size_t sm = g.lindblad ? pr::Smem<B>::gradient_lindblad : pr::Smem<B>::gradient_unitary;
+  size_t sm = g.lindblad ? pr::Smem<B>::gradient_lindblad : pr::Smem<B>::gradient_unitary;
+  // the recomputing walk runs the segment kernel's exp (two tiles at B == 1)
+  if (g.recompute) sm = std::max(sm, pr::Smem<B>::segment);
   const dim3 blk(64 * B * B);
   switch (g.V) {
 #define PR_GCASE(VV)                                                                                        \
   case VV:                                                                                                  \
-    if (pl.opt.grad_mode == PR_GRAD_EXACT)                                                                  \
+    if (g.recompute && pl.opt.grad_mode == PR_GRAD_EXACT)                                                   \
+      hipLaunchKernelGGL((pr::k_gradient<B, VV, true, false, true>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); \
+    else if (g.recompute)                                                                               \
+      hipLaunchKernelGGL((pr::k_gradient<B, VV, false, false, true>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); \
+    else if (pl.opt.grad_mode == PR_GRAD_EXACT)                                                             \
       hipLaunchKernelGGL((pr::k_gradient<B, VV, true>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); \
     else if (B == 1 && g.trace_regs)                                                                    \
       hipLaunchKernelGGL((pr::k_gradient<B, VV, false, true>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); \
@@ -631,6 +642,23 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   bool mixed1 = pl.opt.precision == PR_MIXED && pl.opt.grad_mode == PR_GRAD_REFERENCE;
   for (int p = 0; p < n_probs; p++)
     if (probs[p].dim <= 16 && probs[p].dynamics != PR_CLOSED) mixed1 = false;
+  // U_k images between the passes (store_props): closed-dynamics FP64 problems
+  // with d <= 64 can recompute U_k in the gradient walk instead. Auto (-1)
+  // recomputes when all images would take more than half of the free HBM.
+  auto can_recompute = [&](const pr_problem& q) {
+    return q.dynamics == PR_CLOSED && q.dim <= 64 && !(q.dim <= 16 && mixed1);
+  };
+  bool recompute = pl.opt.store_props == 0;
+  if (pl.opt.store_props < 0) {
+    double bytes = 0;
+    for (int p = 0; p < n_probs; p++) {
+      const double nb = (double)((probs[p].dim + 15) / 16) * ((probs[p].dim + 15) / 16);
+      bytes += (double)probs[p].n_members * probs[p].n_t * nb * 256 * sizeof(double2);
+    }
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    recompute = bytes > 0.5 * (double)free_b;
+  }
   for (int p = 0; p < n_probs; p++) {
     const pr_problem& q = probs[p];
     pr::ProbDev& D = pl.host_probs[p];
@@ -773,7 +801,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       D.PsiT = dp;
     }
     // scratch
-    if ((rc = pl.alloc(&D.Ust, (size_t)M * nt * MB))) return rc;
+    D.recompute = recompute && can_recompute(q) ? 1 : 0;
+    if (!D.recompute && (rc = pl.alloc(&D.Ust, (size_t)M * nt * MB))) return rc;
     if (B == 1 && mixed1) {
       // FP32 images of small-generator slices; the FP64 images above hold the others
       if ((rc = pl.alloc(&D.Udm, (size_t)M * nt * 256))) return rc;
@@ -823,6 +852,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     for (int p = 0; p < n_probs; p++)
       if ((probs[p].dim + 15) / 16 == B && probs[p].dynamics == PR_LINDBLAD) g.lindblad = true;
     g.mixed = B == 1 && mixed1;
+    for (int p = 0; p < n_probs; p++)
+      if ((probs[p].dim + 15) / 16 == B && pl.host_probs[p].recompute) g.recompute = true;
     // every control fits one lane group of 64 / V lanes with <= TR_REG nonzeros per lane
     g.trace_regs = B == 1 && vmax[B] >= nc_max[B] && tr_nnz_max[B] <= pr::TR_REG * (64 / vmax[B]);
     if ((rc = pl.upload(&g.d_items, items[B].data(), items[B].size()))) return rc;
@@ -907,6 +938,7 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
   if (o.precision != PR_FP64 && o.precision != PR_MIXED) return fail(PR_ERR_ARG, "unknown precision %d", o.precision);
   if (o.grad_mode != PR_GRAD_REFERENCE && o.grad_mode != PR_GRAD_EXACT)
     return fail(PR_ERR_ARG, "unknown grad_mode %d", o.grad_mode);
+  if (o.store_props < -1 || o.store_props > 1) return fail(PR_ERR_ARG, "store_props %d is not -1, 0 or 1", o.store_props);
   if (o.segment < 0) return fail(PR_ERR_ARG, "segment must be >= 0");
   for (int p = 0; p < n_probs; p++) {
     if (probs[p].dim > 64 && o.grad_mode == PR_GRAD_EXACT)

@@ -361,3 +361,68 @@ def test_mixed_plan_with_large_generators_falls_back_per_slice():
                 U_target=W.haar_unitary(d, rng))
     u = rng.uniform(-1, 1, (3, 30))
     check(p, u, precision=PR_MIXED)
+
+
+def _random_state(d, rng):
+    v = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+    return v / np.linalg.norm(v)
+
+
+def _random_gate(d, nc, n_t, M, seed, objective=PR_GATE, n_pairs=0):
+    rng = np.random.default_rng(seed)
+    if objective != PR_GATE:
+        kw = dict(psi_init=np.stack([_random_state(d, rng) for _ in range(n_pairs)]),
+                  psi_targ=np.stack([_random_state(d, rng) for _ in range(n_pairs)]))
+    else:
+        kw = dict(U_target=W.haar_unitary(d, rng))
+    p = Problem(dim=d, n_ctrl=nc, n_t=n_t, H0=W.random_hermitian(d, rng, 2.0),
+                Hc=np.stack([W.random_hermitian(d, rng, 0.5) for _ in range(nc)]), dt=0.05,
+                objective=objective, n_members=M, ctrl_scale=rng.uniform(0.9, 1.1, (M, nc)), **kw)
+    return p, rng.uniform(-1, 1, (nc, n_t))
+
+
+def _same(plans_args):
+    """F and G of the same problems under store_props = 1 and 0."""
+    out = []
+    for sp in (1, 0):
+        probs, us, kw = plans_args
+        plan = native.Plan(probs, store_props=sp, **kw)
+        try:
+            out.append(plan.eval(us))
+        finally:
+            plan.close()
+    (F1, G1), (F0, G0) = out
+    for a, b in zip(F0, F1):
+        assert a == b
+    for a, b in zip(G0, G1):
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("d,objective,grad_mode", [(16, PR_GATE, 0), (24, PR_PAIRS_SQUARE, 0),
+                                                    (40, PR_GATE, 1), (7, PR_PAIRS_REAL, 1)])
+def test_recomputed_propagators_match_stored(d, objective, grad_mode):
+    # store_props = 0: no U_k images; the gradient walk recomputes U_k with the
+    # segment kernel's own calls, so F and G equal the stored-image plan's and
+    # the oracle's
+    p, u = _random_gate(d, 3, 37, 3, 500 + d, objective=objective, n_pairs=2)
+    print(f"case d={d} objective={objective} grad_mode={grad_mode}", flush=True)
+    _same(([p], [u], dict(grad_mode=grad_mode)))
+    F0, G0, _ = run(p, u, store_props=0, grad_mode=grad_mode)
+    Fo, Go = O.evaluate(p, u, grad_mode=grad_mode)
+    assert abs(F0 - Fo) <= TOL_F
+    assert np.abs(G0 - Go).max() <= TOL_G * np.abs(Go).max()
+
+
+def test_recompute_keeps_images_where_it_does_not_apply():
+    # under store_props = 0 a Lindblad problem and the mixed-precision d <= 16
+    # problems keep their images; the closed d = 20 problem beside them
+    # recomputes. Results equal the stored-image plans.
+    from pulsar_rocm.problem import PR_MIXED
+    lind = Problem(dim=4, n_ctrl=2, n_t=4, H0=FIX["lind_L0"], Hc=FIX["lind_Lc"], dt_slices=FIX["lind_dts"],
+                   dynamics=PR_LINDBLAD, objective=PR_PAIRS_SQUARE, n_members=1,
+                   ctrl_scale=np.full((1, 2), float(FIX["lind_pwr"])), psi_init=FIX["lind_s0"],
+                   psi_targ=FIX["lind_st"], weight=np.ones(1))
+    pg, ug = _random_gate(20, 2, 25, 2, 92)
+    pm, um = _random_gate(8, 2, 25, 2, 93)
+    _same(([pg, lind], [ug, FIX["lind_u"]], dict()))
+    _same(([pg, pm], [ug, um], dict(precision=PR_MIXED)))
