@@ -229,8 +229,8 @@ __device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __r
     const int w = __builtin_amdgcn_readfirstlane(x.w);
     const int p1 = OWN32 ? P.so_pass[1] : P.sl_pass[w + 1];
     int p = OWN32 ? P.so_pass[0] : P.sl_pass[w];
-    // one pass at a time (batching passes measured slower: the extra live
-    // registers spill at this kernel's 4-wavefront register budget)
+    // one pass at a time (two at once measured slower: their live registers
+    // spill, or cost occupancy)
     for (; p < p1; p++) slot_passes<1>(H, sidx, sv, p, lo, hi, x);
     if (P.prescaled) return H;
     return cscale(P.cre * dt, P.cim * dt, H);
@@ -481,11 +481,11 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
 // ---------------------------------------------------------------- k_segment
 // Minimum wavefronts per SIMD the compiler must leave room for (caps VGPRs
 // at 512 / waves); 0 leaves the register budget to the compiler. The d <= 16
-// segment walk runs faster at 4 waves (128 VGPRs, a few spilled values
-// outside the products) than at the compiler's 3; the gradient walk is not
-// occupancy-bound.
+// FP64 segment walk runs faster at 3 waves (168 VGPRs, no spills) than at 4
+// (128 VGPRs with spills); the mixed walk is equal at 3 and 4 and keeps 4;
+// the gradient walk is not occupancy-bound.
 #ifndef PR_SEGMENT_WAVES
-#define PR_SEGMENT_WAVES 4
+#define PR_SEGMENT_WAVES 3
 #endif
 #ifndef PR_MIXED_SEGMENT_WAVES
 #define PR_MIXED_SEGMENT_WAVES 4
