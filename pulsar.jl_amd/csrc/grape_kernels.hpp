@@ -702,10 +702,15 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
   const size_t MB = (size_t)NB * 256;
   Blk E = eye_blk(x);
   store_blk(P.Eseg + ((size_t)m * (P.nseg + 1)) * MB, E, x);
+  // T_{s+1} is fetched while the product with T_s runs (the prefix is a
+  // sequential chain, so an unhidden load would cost a round trip per step)
+  Blk Ts = load_blk(P.Tseg + ((size_t)m * P.nseg) * MB, x);
   for (int s = 0; s < P.nseg; s++) {
-    Blk Ts = load_blk(P.Tseg + ((size_t)m * P.nseg + s) * MB, x);
+    Blk Tn = Ts;
+    if (s + 1 < P.nseg) Tn = load_blk(P.Tseg + ((size_t)m * P.nseg + s + 1) * MB, x);
     E = mm<B, N, N>(Ts, E, LA, LB, x);
     store_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + s + 1) * MB, E, x);
+    Ts = Tn;
   }
   const Blk& Ut = E;  // U_total
   if (P.unitary) {
