@@ -125,7 +125,8 @@ constexpr int TAYLOR_DEG = 9;
 // the prefix it uses, since LDS per workgroup bounds how many wavefronts a CU
 // holds. Single-wavefront workgroups (B == 1) read the B operand of N
 // products and the A operand of T/C products from registers:
-//   k_segment   LA (operand) + LB (the running segment product)
+//   k_segment   LA (operand); the running segment product stays in registers
+//               (two tiles are reserved, as for the recomputing gradient walk)
 //   k_gradient  LA only for unitary dynamics; LA + LB for Lindblad
 // Wider workgroups use LA, LB and red in every kernel.
 template <int B>
@@ -514,12 +515,9 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
   const int m = it.member;
   // the running segment product lives in the second LDS tile (B == 1: the
   // product reads it from there and writes the new product back) or in registers
+  // the running segment product stays in registers (the B operand of the
+  // chain product U_k T for B == 1; staged through LB for wider blocks)
   Blk Tseg = eye_blk(x);
-  double2* LT = LB;
-  if constexpr (B == 1) {
-    put<B>(LT, Tseg, x);
-    bsync<B>();
-  }
   int smax = 0;
   // software pipeline: the generator of slice k+1 is assembled while the
   // chain product of slice k runs on the matrix cores
@@ -551,10 +549,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
     if constexpr (B > 1) put<B>(LB, Tseg, x);
     bsync<B>();
     Blk acc;
-    if constexpr (B == 1)
-      acc = gemm_prod<B, N, NL>(LA, U, LT, U, x);
-    else
-      acc = gemm_prod<B, N, N>(LA, U, LB, Tseg, x);
+    acc = gemm_prod<B, N, N>(LA, U, LB, Tseg, x);
     Blk Ut;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -568,20 +563,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
       if (threadIdx.x == 0) P.big[(size_t)m * P.nt + k] = 1;
     }
     bsync<B>();
-    if constexpr (B == 1) {
-      put<B>(LT, acc, x);
-    } else {
-      Tseg = acc;
-    }
-  }
-  if constexpr (B == 1) {
-    bsync<B>();
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      double2 v = LT[lidx(0, rowof(x.q, r), x.c)];
-      Tseg.re[r] = v.x;
-      Tseg.im[r] = v.y;
-    }
+    Tseg = acc;
   }
   store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * NB * 256, Tseg, x);
   if (threadIdx.x == 0) P.smax[(size_t)m * P.nseg + it.seg] = smax;
@@ -610,9 +592,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
   double* red;
   smem_parts<1>(LA, LB, red, zb);
   const int m = it.member;
-  double2* LT = LB;
-  put<1>(LT, eye_blk(x), x);
-  bsync<1>();
+  Blk Tseg = eye_blk(x);  // running segment product, in registers
   const double2* H0o = P.H0o + (P.h0pm ? (size_t)m * 256 : 0);
   if (threadIdx.x == 0) P.seg_big[(size_t)m * P.nseg + it.seg] = 0;
   int smax = 0;
@@ -651,7 +631,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
     }
     bsync<1>();
     const Blk none = zero_blk();
-    const Blk acc = gemm_prod<1, N, NL>(LA, none, LT, none, x);  // U_k T (FP64)
+    const Blk acc = gemm_prod<1, N, N>(LA, none, LB, Tseg, x);  // U_k T (FP64)
     const size_t mk = (size_t)m * P.nt + k;
     {
       Blk32 D;  // D[4q + r][c] = conj(U[c][4q + r]) - delta
@@ -673,15 +653,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
       if (deg6) X2 = prod32_adj(X32, X32);
     }
     bsync<1>();
-    put<1>(LT, acc, x);
-    bsync<1>();
-  }
-  Blk Tseg;
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const double2 v = LT[lidx(0, rowof(x.q, r), x.c)];
-    Tseg.re[r] = v.x;
-    Tseg.im[r] = v.y;
+    Tseg = acc;
   }
   store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * 256, Tseg, x);
   if (threadIdx.x == 0) P.smax[(size_t)m * P.nseg + it.seg] = smax;
