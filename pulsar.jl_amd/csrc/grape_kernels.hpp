@@ -76,6 +76,15 @@ struct ProbDev {
   const double2* so_v;
   int sl_pass[65];
   int so_pass[2];
+  // d <= 16 closed dynamics: the transposed drift images and operator tables
+  // (X^T is assembled directly, so exp gives U^T in the register layout)
+  const double2* H0t;
+  const int* ct_ptr;
+  const int* ct_j;
+  const double2* ct_v;
+  const int4* slt_idx;
+  const double2* slt_v;
+  int slt_pass[2];
   const double* n0;           // [M or 1] ||A0||_1
   const double* ncn;          // [nc] ||A_j||_1
   const double* scale;        // [M][nc]
@@ -209,27 +218,29 @@ __device__ inline void slot_passes(Blk& H, const int4* sidx, const double2* sv, 
 
 // Generator of slice k, X = c dt_k (A0 + sum_j s_mj u_jk A_j). H0 is the
 // wavefront's block of the member's drift.
-template <int B, bool OWN32 = false>
+// TRANS (B == 1, closed dynamics): X^T from the transposed operator tables;
+// H0 is then the transposed drift image.
+template <int B, bool OWN32 = false, bool TRANS = false>
 __device__ inline Blk assemble(const ProbDev& P, int m, int k, const double* __restrict__ u, const Blk& H0,
                                const Ln& x) {
   const int nc = P.nc;
   const double* sc = P.scale + (size_t)m * nc;
   const double* uk = u + P.coff + (size_t)k * nc;
   const double dt = slice_dt(P, k);
-  const int* cptr = OWN32 ? P.co_ptr : P.cs_ptr;
-  const int* cj = OWN32 ? P.co_j : P.cs_j;
-  const double2* cv = OWN32 ? P.co_v : P.cs_v;
+  const int* cptr = OWN32 ? P.co_ptr : TRANS ? P.ct_ptr : P.cs_ptr;
+  const int* cj = OWN32 ? P.co_j : TRANS ? P.ct_j : P.cs_j;
+  const double2* cv = OWN32 ? P.co_v : TRANS ? P.ct_v : P.cs_v;
   Blk H = H0;
-  const int4* sidx = OWN32 ? P.so_idx : P.sl_idx;
+  const int4* sidx = OWN32 ? P.so_idx : TRANS ? P.slt_idx : P.sl_idx;
   if (sidx) {
     // lane j holds the weight s_mj u_jk; each slot entry fetches its control's
     // weight from that lane, so the entry loads do not depend on one another
     const double wl = x.lane < nc ? gptr(sc)[x.lane] * gptr(uk)[x.lane] : 0.0;
     const int lo = __double2loint(wl), hi = __double2hiint(wl);
-    const double2* sv = OWN32 ? P.so_v : P.sl_v;
+    const double2* sv = OWN32 ? P.so_v : TRANS ? P.slt_v : P.sl_v;
     const int w = __builtin_amdgcn_readfirstlane(x.w);
-    const int p1 = OWN32 ? P.so_pass[1] : P.sl_pass[w + 1];
-    int p = OWN32 ? P.so_pass[0] : P.sl_pass[w];
+    const int p1 = OWN32 ? P.so_pass[1] : TRANS ? P.slt_pass[1] : P.sl_pass[w + 1];
+    int p = OWN32 ? P.so_pass[0] : TRANS ? P.slt_pass[0] : P.sl_pass[w];
     // one pass at a time (two at once measured slower: their live registers
     // spill, or cost occupancy)
     for (; p < p1; p++) slot_passes<1>(H, sidx, sv, p, lo, hi, x);
@@ -296,6 +307,11 @@ __device__ inline int choose_scaling(double nb, bool& deg6) {
 template <int B>
 __device__ inline Blk load_drift(const ProbDev& P, int m, const Ln& x) {
   return load_blk(P.H0 + (P.h0pm ? (size_t)m * B * B * 256 : 0), x);
+}
+
+// transposed drift image (d <= 16, closed dynamics)
+__device__ inline Blk load_drift_t(const ProbDev& P, int m, const Ln& x) {
+  return load_blk(P.H0t + (P.h0pm ? (size_t)m * 256 : 0), x);
 }
 
 __device__ inline void add_diag(Blk& X, double v, const Ln& x) {
@@ -513,15 +529,18 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
   smem_parts<B>(LA, LB, red, zb);
   constexpr int NB = B * B;
   const int m = it.member;
-  // the running segment product lives in the second LDS tile (B == 1: the
-  // product reads it from there and writes the new product back) or in registers
   // the running segment product stays in registers (the B operand of the
   // chain product U_k T for B == 1; staged through LB for wider blocks)
   Blk Tseg = eye_blk(x);
   int smax = 0;
+  // B == 1 with closed dynamics: X^T is assembled from the transposed tables,
+  // so exp gives V = U^T in the register layout. The chain product U T takes
+  // V as a transposed A operand and the stored image is V itself: no LDS.
+  constexpr bool TR = B == 1 && CLOSED;
   // software pipeline: the generator of slice k+1 is assembled while the
   // chain product of slice k runs on the matrix cores
-  Blk Xn = assemble<B>(P, m, it.k0, u, load_drift<B>(P, m, x), x);
+  Blk Xn = TR ? assemble<1, false, true>(P, m, it.k0, u, load_drift_t(P, m, x), x)
+              : assemble<B>(P, m, it.k0, u, load_drift<B>(P, m, x), x);
   for (int k = it.k0; k < it.k1; k++) {
     Blk X = Xn;
     bool deg6;
@@ -533,10 +552,19 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
       X.im *= f;
     }
     smax = max(smax, s);
+    if constexpr (TR) {
+      const Blk V = expm_ah1(X, s, deg6, LA, LB, x);         // U^T
+      const Blk acc = gemm_prod<1, T, N>(LA, V, LB, Tseg, x);  // U T
+      if (k + 1 < it.k1) Xn = assemble<1, false, true>(P, m, k + 1, u, load_drift_t(P, m, x), x);
+      if (!P.recompute) store_blk(P.Ust + ((size_t)m * P.nt + k) * 256, V, x);
+      if constexpr (FIX) {
+        if (threadIdx.x == 0) P.big[(size_t)m * P.nt + k] = 1;
+      }
+      Tseg = acc;
+      continue;
+    }
     Blk U;
-    if constexpr (B == 1 && CLOSED) {
-      U = expm_ah1(X, s, deg6, LA, LB, x);
-    } else if constexpr (B == 1) {
+    if constexpr (B == 1) {
       if (P.unitary)
         U = expm_ah1(X, s, deg6, LA, LB, x);
       else
@@ -881,6 +909,16 @@ __device__ inline void contract_regs(const ProbDev& P, int m, int k, double dtk,
 template <int B>
 __device__ inline Blk recompute_ut(const ProbDev& P, int m, int k, const double* __restrict__ u, double2* LA,
                                    double2* LB, double* red, const Ln& x) {
+  if constexpr (B == 1) {
+    // as k_segment: exp of X^T is U^T itself
+    Blk X = assemble<1, false, true>(P, m, k, u, load_drift_t(P, m, x), x);
+    bool deg6;
+    const int s = choose_scaling(norm1_blk<1>(X, red, x), deg6);
+    const double f = ldexp(1.0, -s);
+    X.re *= f;
+    X.im *= f;
+    return expm_ah1(X, s, deg6, LA, LB, x);
+  }
   Blk X = assemble<B>(P, m, k, u, load_drift<B>(P, m, x), x);
   bool deg6;
   const int s = choose_scaling(norm1_blk<B>(X, red, x), deg6);

@@ -645,8 +645,14 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   // U_k images between the passes (store_props): closed-dynamics FP64 problems
   // with d <= 64 can recompute U_k in the gradient walk instead. Auto (-1)
   // recomputes when all images would take more than half of the free HBM.
+  // (a d <= 16 closed problem sharing its group with a Lindblad problem runs
+  // the general segment kernel, whose U_k image the recomputation would not
+  // reproduce bit for bit: it keeps its images)
+  bool lindblad1 = false;
+  for (int p = 0; p < n_probs; p++)
+    if (probs[p].dim <= 16 && probs[p].dynamics != PR_CLOSED) lindblad1 = true;
   auto can_recompute = [&](const pr_problem& q) {
-    return q.dynamics == PR_CLOSED && q.dim <= 64 && !(q.dim <= 16 && mixed1);
+    return q.dynamics == PR_CLOSED && q.dim <= 64 && !(q.dim <= 16 && (mixed1 || lindblad1));
   };
   bool recompute = pl.opt.store_props == 0;
   if (pl.opt.store_props < 0) {
@@ -745,6 +751,27 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
         if ((rc = pl.upload(&vp, so.v.data(), so.v.size()))) return rc;
         D.so_idx = sp; D.so_v = vp;
         std::copy(so.ptr.begin(), so.ptr.end(), D.so_pass);
+      }
+    }
+    if (B == 1 && q.dynamics == PR_CLOSED) {
+      // transposed drift images and operator tables: k_segment assembles X^T
+      for (int m = 0; m < nh0; m++) tile_image(H0src + 2 * dd * m, d, 1, true, img.data() + MB * m);
+      if ((rc = pl.upload(&dp, img.data(), MB * nh0))) return rc;
+      D.H0t = dp;
+      Csr ct = build_csr(Hcsrc, nc, d, 1, true);
+      int *ip, *jp;
+      double2* vp;
+      if ((rc = pl.upload(&ip, ct.ptr.data(), ct.ptr.size()))) return rc;
+      if ((rc = pl.upload(&jp, ct.j.data(), ct.j.size()))) return rc;
+      if ((rc = pl.upload(&vp, ct.v.data(), ct.v.size()))) return rc;
+      D.ct_ptr = ip; D.ct_j = jp; D.ct_v = vp;
+      if (nc <= 64) {
+        Slots st = build_slots(ct, 1);
+        int4* sp;
+        if ((rc = pl.upload(&sp, st.idx.data(), st.idx.size()))) return rc;
+        if ((rc = pl.upload(&vp, st.v.data(), st.v.size()))) return rc;
+        D.slt_idx = sp; D.slt_v = vp;
+        std::copy(st.ptr.begin(), st.ptr.end(), D.slt_pass);
       }
     }
     if ((rc = pl.upload(&dd_, nrm.data(), nh0))) return rc;
