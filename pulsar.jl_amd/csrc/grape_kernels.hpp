@@ -986,6 +986,19 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
     } else {
       Cm = mm<B, N, C>(Y, Es, LA, LB, x);
     }
+    if constexpr (B == 1) {
+      // B == 1 carries G = C^T: M = G^T U^dag and G' = M^T U^T = (U M)^T take
+      // both operands from registers (C as an A operand would need LDS)
+      put<1>(LA, Cm, x);
+      bsync<1>();
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double2 v = LA[lidx(0, x.c, rowof(x.q, r))];  // C[c][q + 4r]
+        Cm.re[r] = v.x;
+        Cm.im[r] = v.y;
+      }
+      bsync<1>();
+    }
     // U_{k+1} is fetched and the contraction of slice k runs while the
     // product C_{k+1} = U_k M_k is on the matrix cores
     TraceRegs<V> tregs;
@@ -998,7 +1011,11 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
       Ut = load_blk(P.Ust + ((size_t)m * P.nt + it.k0) * MB, x);
     for (int k = it.k0; k < it.k1; k++) {
       Blk Ud = conj_blk(Ut);                      // U^dag, own block
-      Blk Mk = mm<B, N, N>(Cm, Ud, LA, LB, x);    // C_k U_k^dag
+      Blk Mk;
+      if constexpr (B == 1)
+        Mk = mm<1, T, N>(Cm, Ud, LA, LB, x);      // G^T U^dag = C_k U_k^dag
+      else
+        Mk = mm<B, N, N>(Cm, Ud, LA, LB, x);      // C_k U_k^dag
       Blk Un = Ut;
       double dtn = dtk;
       if (k + 1 < it.k1) {
@@ -1008,7 +1025,10 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
         else
           Un = load_blk(P.Ust + ((size_t)m * P.nt + k + 1) * MB, x);
       }
-      Cm = mm<B, T, N>(Ut, Mk, LA, LB, x);        // U_k M_k
+      if constexpr (B == 1)
+        Cm = mm<1, T, N>(Mk, Ut, LA, LB, x);      // M^T U^T = (U_k M_k)^T
+      else
+        Cm = mm<B, T, N>(Ut, Mk, LA, LB, x);      // U_k M_k
       if constexpr (EXACT) {
         contract<B, V>(P, m, k, exact_direction<B>(P, m, k, u, H0, Mk, LA, LB, red, x), LA, x);
       } else if constexpr (REG && B == 1) {
