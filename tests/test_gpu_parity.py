@@ -426,3 +426,25 @@ def test_recompute_keeps_images_where_it_does_not_apply():
     pm, um = _random_gate(8, 2, 25, 2, 93)
     _same(([pg, lind], [ug, FIX["lind_u"]], dict()))
     _same(([pg, pm], [ug, um], dict(precision=PR_MIXED)))
+
+
+def test_closed_and_lindblad_problems_in_one_single_block_group():
+    # both d <= 16: the group runs the general segment kernel (Lindblad
+    # generators), where the closed problem takes the anti-Hermitian exp and
+    # the LDS transpose for its images; under store_props = 0 it keeps them
+    lind = Problem(dim=4, n_ctrl=2, n_t=4, H0=FIX["lind_L0"], Hc=FIX["lind_Lc"], dt_slices=FIX["lind_dts"],
+                   dynamics=PR_LINDBLAD, objective=PR_PAIRS_SQUARE, n_members=1,
+                   ctrl_scale=np.full((1, 2), float(FIX["lind_pwr"])), psi_init=FIX["lind_s0"],
+                   psi_targ=FIX["lind_st"], weight=np.ones(1))
+    pg, ug = _random_gate(8, 3, 29, 2, 94)
+    for sp in (1, 0):
+        plan = native.Plan([pg, lind], store_props=sp)
+        try:
+            F, G = plan.eval([ug, FIX["lind_u"]])
+        finally:
+            plan.close()
+        F0, G0 = O.evaluate(pg, ug)
+        assert abs(F[0] - F0) <= TOL_F
+        assert np.abs(G[0] - G0).max() <= TOL_G * np.abs(G0).max()
+        assert abs(F[1] - float(FIX["lind_F"])) < 1e-13
+        assert np.abs(G[1] - FIX["lind_G"]).max() < 1e-12
