@@ -304,6 +304,27 @@ __device__ inline int choose_scaling(double nb, bool& deg6) {
   return s;
 }
 
+// Scaling exponent and degree of the exp of X from its 1-norm bound. For
+// B == 1 the column sums are checked against the degree-6 bound with one
+// ballot; the maximum over the columns (a cross-lane reduction) is formed only
+// when some column exceeds it. The outcome equals choose_scaling(norm1_blk).
+template <int B>
+__device__ inline int scaling_of(const Blk& X, double* red, const Ln& x, bool& deg6) {
+  if constexpr (B == 1) {
+    double cs = 0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) cs += abs_bound(X.re[r], X.im[r]);
+    cs = sum_rows_pairwise(cs);  // lane c + 16q: column c
+    if (__ballot(!(cs <= THETA6)) == 0) {
+      deg6 = true;
+      return 0;
+    }
+    return choose_scaling(max_in_row(cs), deg6);
+  } else {
+    return choose_scaling(norm1_blk<B>(X, red, x), deg6);
+  }
+}
+
 template <int B>
 __device__ inline Blk load_drift(const ProbDev& P, int m, const Ln& x) {
   return load_blk(P.H0 + (P.h0pm ? (size_t)m * B * B * 256 : 0), x);
@@ -486,7 +507,7 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
                                       const Blk& M, double2* LA, double2* LB, double* red, const Ln& x) {
   Blk X = assemble<B>(P, m, k, u, H0, x);
   bool deg6;
-  const int s = choose_scaling(norm1_blk<B>(X, red, x), deg6);
+  const int s = scaling_of<B>(X, red, x, deg6);
   if (s) {
     const double f = ldexp(1.0, -s);
     X.re *= f;
@@ -544,7 +565,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
   for (int k = it.k0; k < it.k1; k++) {
     Blk X = Xn;
     bool deg6;
-    const int s = choose_scaling(norm1_blk<B>(X, red, x), deg6);
+    const int s = scaling_of<B>(X, red, x, deg6);
     {
       // unconditional (exact for s = 0): a branch here costs register copies
       const double f = ldexp(1.0, -s);
@@ -628,7 +649,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
   Blk Xo = assemble<1, true>(P, m, it.k0, u, load_blk(H0o, x), x);
   bool deg6;
   // (s > 0 only above the degree-6 bound, where the work item exits unscaled)
-  int s = choose_scaling(norm1_blk<1>(Xo, red, x), deg6);
+  int s = scaling_of<1>(Xo, red, x, deg6);
   Blk32 X32, X2;
   X32.re = __builtin_convertvector(Xo.re, f4);
   X32.im = __builtin_convertvector(Xo.im, f4);
@@ -675,7 +696,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
     if (threadIdx.x == 0) P.big[mk] = 0;
     if (k + 1 < it.k1) {
       Xo = assemble<1, true>(P, m, k + 1, u, load_blk(H0o, x), x);
-      s = choose_scaling(norm1_blk<1>(Xo, red, x), deg6);
+      s = scaling_of<1>(Xo, red, x, deg6);
       X32.re = __builtin_convertvector(Xo.re, f4);
       X32.im = __builtin_convertvector(Xo.im, f4);
       if (deg6) X2 = prod32_adj(X32, X32);
@@ -913,7 +934,7 @@ __device__ inline Blk recompute_ut(const ProbDev& P, int m, int k, const double*
     // as k_segment: exp of X^T is U^T itself
     Blk X = assemble<1, false, true>(P, m, k, u, load_drift_t(P, m, x), x);
     bool deg6;
-    const int s = choose_scaling(norm1_blk<1>(X, red, x), deg6);
+    const int s = scaling_of<1>(X, red, x, deg6);
     const double f = ldexp(1.0, -s);
     X.re *= f;
     X.im *= f;
@@ -921,7 +942,7 @@ __device__ inline Blk recompute_ut(const ProbDev& P, int m, int k, const double*
   }
   Blk X = assemble<B>(P, m, k, u, load_drift<B>(P, m, x), x);
   bool deg6;
-  const int s = choose_scaling(norm1_blk<B>(X, red, x), deg6);
+  const int s = scaling_of<B>(X, red, x, deg6);
   {
     const double f = ldexp(1.0, -s);
     X.re *= f;
