@@ -335,11 +335,14 @@ __device__ inline Blk load_drift_t(const ProbDev& P, int m, const Ln& x) {
   return load_blk(P.H0t + (P.h0pm ? (size_t)m * 256 : 0), x);
 }
 
+// X += v I on this wavefront's block. A 0/1 factor and an FMA per element
+// (the factor is loop-invariant, so it stays in a register) instead of a
+// conditional add, which costs an add and two selects per element.
 __device__ inline void add_diag(Blk& X, double v, const Ln& x) {
-  if (x.I == x.Jb) {
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-      if (rowof(x.q, r) == x.c) X.re[r] += v;
+  for (int r = 0; r < 4; r++) {
+    const double on = (x.I == x.Jb && rowof(x.q, r) == x.c) ? 1.0 : 0.0;
+    X.re[r] = fma(on, v, X.re[r]);
   }
 }
 

@@ -51,6 +51,10 @@ __device__ constexpr int rowof(int q, int r) { return q + 4 * r; }
 __device__ inline d4 mfma(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+// (-a) b + c (the instruction's negate modifier on the A input)
+__device__ inline d4 mfma_nega(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 1);
+}
 
 __device__ inline Blk zero_blk() {
   Blk b;
@@ -122,11 +126,13 @@ __device__ inline void put(double2* L, const Blk& b, const Ln& x) {
 }
 
 // A-operand element op(A)[16I + c][16K + q + 4kb] for the MFMA k-step kb.
-template <int B, int OP>
+// RAW: the element before conjugation (the caller applies the sign of the
+// imaginary part, e.g. as an MFMA negate modifier).
+template <int B, int OP, bool RAW = false>
 __device__ inline double2 fetch_a(const double2* L, const Blk& own, int K, int kb, const Ln& x) {
   if constexpr (B == 1 && (OP == T || OP == C)) {
     double2 v = make_double2(own.re[kb], own.im[kb]);
-    if constexpr (OP == C) v.y = -v.y;
+    if constexpr (OP == C && !RAW) v.y = -v.y;
     return v;
   } else {
     double2 v;
@@ -134,17 +140,17 @@ __device__ inline double2 fetch_a(const double2* L, const Blk& own, int K, int k
       v = L[lidx(x.I * B + K, x.c, rowof(x.q, kb))];
     else
       v = L[lidx(K * B + x.I, rowof(x.q, kb), x.c)];
-    if constexpr (OP == C || OP == J) v.y = -v.y;
+    if constexpr ((OP == C || OP == J) && !RAW) v.y = -v.y;
     return v;
   }
 }
 
 // B-operand element op(B)[16K + q + 4kb][16J + c].
-template <int B, int OP>
+template <int B, int OP, bool RAW = false>
 __device__ inline double2 fetch_b(const double2* L, const Blk& own, int K, int kb, const Ln& x) {
   if constexpr (B == 1 && (OP == N || OP == J)) {
     double2 v = make_double2(own.re[kb], own.im[kb]);
-    if constexpr (OP == J) v.y = -v.y;
+    if constexpr (OP == J && !RAW) v.y = -v.y;
     return v;
   } else {
     double2 v;
@@ -152,7 +158,7 @@ __device__ inline double2 fetch_b(const double2* L, const Blk& own, int K, int k
       v = L[lidx(K * B + x.Jb, rowof(x.q, kb), x.c)];
     else
       v = L[lidx(x.Jb * B + K, x.c, rowof(x.q, kb))];
-    if constexpr (OP == C || OP == J) v.y = -v.y;
+    if constexpr ((OP == C || OP == J) && !RAW) v.y = -v.y;
     return v;
   }
 }
@@ -203,16 +209,19 @@ __device__ inline Blk gemm_prod(const double2* LA, const Blk& Aown, const double
   }
   return r;
 #else
+  // conjugated operands: the imaginary sign goes into the MFMA negate
+  // modifier and the (free) source negation of the sums, not into VALU
+  constexpr bool na = OA == C || OA == J, nb = OB == C || OB == J;
   d4 t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
 #pragma unroll
   for (int K = 0; K < B; K++) {
 #pragma unroll
     for (int kb = 0; kb < 4; kb++) {
-      double2 a = fetch_a<B, OA>(LA, Aown, K, kb, x);
-      double2 b = fetch_b<B, OB>(LB, Bown, K, kb, x);
+      const double2 a = fetch_a<B, OA, true>(LA, Aown, K, kb, x);
+      const double2 b = fetch_b<B, OB, true>(LB, Bown, K, kb, x);
       t1 = mfma(a.x, b.x, t1);
-      t2 = mfma(a.y, b.y, t2);
-      t3 = mfma(a.x + a.y, b.x + b.y, t3);
+      t2 = (na != nb) ? mfma_nega(a.y, b.y, t2) : mfma(a.y, b.y, t2);
+      t3 = mfma(na ? a.x - a.y : a.x + a.y, nb ? b.x - b.y : b.x + b.y, t3);
     }
   }
   Blk r;
