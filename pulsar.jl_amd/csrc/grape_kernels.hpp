@@ -1034,12 +1034,11 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
     else
       Ut = load_blk(P.Ust + ((size_t)m * P.nt + it.k0) * MB, x);
     for (int k = it.k0; k < it.k1; k++) {
-      Blk Ud = conj_blk(Ut);                      // U^dag, own block
       Blk Mk;
       if constexpr (B == 1)
-        Mk = mm<1, T, N>(Cm, Ud, LA, LB, x);      // G^T U^dag = C_k U_k^dag
+        Mk = mm<1, T, J>(Cm, Ut, LA, LB, x);      // G^T conj(U^T) = C_k U_k^dag
       else
-        Mk = mm<B, N, N>(Cm, Ud, LA, LB, x);      // C_k U_k^dag
+        Mk = mm<B, N, J>(Cm, Ut, LA, LB, x);      // C_k conj(U^T) = C_k U_k^dag
       Blk Un = Ut;
       double dtn = dtk;
       if (k + 1 < it.k1) {
@@ -1183,7 +1182,7 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
         Cs64.im[r] = v.y;
       }
       bsync<1>();
-      const Blk M64 = mm<1, N, N>(Cs64, conj_blk(Ut), LA, LB, x);
+      const Blk M64 = mm<1, N, J>(Cs64, Ut, LA, LB, x);
       const Blk C64 = mm<1, T, N>(Ut, M64, LA, LB, x);
       put<1>(LA, M64, x);
       bsync<1>();
