@@ -677,7 +677,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
       Uo.im = Xo.im - __builtin_convertvector(Pq.im + c2 * X2.im, d4);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        if (rowof32(x.q, r) == x.c) Uo.re[r] += 1.0;
+        Uo.re[r] += rowof32(x.q, r) == x.c ? 1.0 : 0.0;
         LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Uo.re[r], Uo.im[r]);
       }
     }
