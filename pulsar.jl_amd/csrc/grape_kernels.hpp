@@ -335,15 +335,21 @@ __device__ inline Blk load_drift_t(const ProbDev& P, int m, const Ln& x) {
   return load_blk(P.H0t + (P.h0pm ? (size_t)m * 256 : 0), x);
 }
 
-// X += v I on this wavefront's block. A 0/1 factor and an FMA per element
-// (the factor is loop-invariant, so it stays in a register) instead of a
-// conditional add, which costs an add and two selects per element.
 __device__ inline void add_diag(Blk& X, double v, const Ln& x) {
+  if (x.I == x.Jb) {
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const double on = (x.I == x.Jb && rowof(x.q, r) == x.c) ? 1.0 : 0.0;
-    X.re[r] = fma(on, v, X.re[r]);
+    for (int r = 0; r < 4; r++)
+      if (rowof(x.q, r) == x.c) X.re[r] += v;
   }
+}
+
+// add_diag for a single-block wavefront (d <= 16): a 0/1 factor and an FMA per
+// element (the factor is loop-invariant) instead of a conditional add, which
+// costs an add and two selects per element. (In the wider kernels the factors'
+// registers cost more in spills than the selects.)
+__device__ inline void add_diag1(Blk& X, double v, const Ln& x) {
+#pragma unroll
+  for (int r = 0; r < 4; r++) X.re[r] = fma(rowof(x.q, r) == x.c ? 1.0 : 0.0, v, X.re[r]);
 }
 
 // exp(X * 2^s) for X already scaled. Degree 6: p = B0 + X3 (B1 + c6 X3)
@@ -418,22 +424,22 @@ __device__ inline Blk expm_ah1_from2(const Blk& X, const Blk& X2, int s, bool de
   if (deg6) {
     R.re = c6 * X3.re - c5 * X2.re + c4 * X.re;
     R.im = c6 * X3.im - c5 * X2.im + c4 * X.im;
-    add_diag(R, c3, x);
+    add_diag1(R, c3, x);
   } else {
     Blk Tm;
     Tm.re = c9 * X3.re - c8 * X2.re + c7 * X.re;
     Tm.im = c9 * X3.im - c8 * X2.im + c7 * X.im;
-    add_diag(Tm, c6, x);
+    add_diag1(Tm, c6, x);
     const Blk P = gemm_prod<1, C, N>(LA, X3, LB, Tm, x);  // -X^3 Tm
     R.re = -P.re - c5 * X2.re + c4 * X.re;
     R.im = -P.im - c5 * X2.im + c4 * X.im;
-    add_diag(R, c3, x);
+    add_diag1(R, c3, x);
   }
   const Blk P = gemm_prod<1, C, N>(LA, X3, LB, R, x);  // -X^3 R
   Blk U;
   U.re = -P.re - c2 * X2.re + X.re;
   U.im = -P.im - c2 * X2.im + X.im;
-  add_diag(U, 1.0, x);
+  add_diag1(U, 1.0, x);
   for (int i = 0; i < s; i++) U = mm<1, N, N>(U, U, LA, LB, x);
   return U;
 }
@@ -677,7 +683,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
       Uo.im = Xo.im - __builtin_convertvector(Pq.im + c2 * X2.im, d4);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        Uo.re[r] += rowof32(x.q, r) == x.c ? 1.0 : 0.0;
+        if (rowof32(x.q, r) == x.c) Uo.re[r] += 1.0;
         LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Uo.re[r], Uo.im[r]);
       }
     }

@@ -421,7 +421,8 @@ __device__ inline Blk32 prod32_adj(const Blk32& D, const Blk32& Bo) {
 
 __device__ inline void add_diag32(Blk32& X, float v, const Ln& x) {
 #pragma unroll
-  for (int r = 0; r < 4; r++) X.re[r] = __builtin_fmaf(rowof32(x.q, r) == x.c ? 1.0f : 0.0f, v, X.re[r]);
+  for (int r = 0; r < 4; r++)
+    if (rowof32(x.q, r) == x.c) X.re[r] += v;
 }
 
 }  // namespace pr
