@@ -53,7 +53,7 @@ HBM_PEAK_GBS = 8000.0
 # commands and per-kernel figures are in profiles/grape_four_qubit_ensemble_hbm_traffic.md.
 # k_segment writes the U_k^T images (256 x 2000 x 4 KiB = 2.10 GB) and
 # k_gradient reads them back once.
-MEASURED_TRAFFIC_BYTES = {"k_segment": 0.028e9 + 2.160e9, "k_gradient": 2.141e9 + 0.038e9}
+MEASURED_TRAFFIC_BYTES = {"k_segment": 0.022e9 + 2.131e9, "k_gradient": 2.141e9 + 0.037e9}
 
 
 def probe_rates():
