@@ -363,7 +363,7 @@ __global__ __launch_bounds__(1024) void k_gradient_big(const ProbDev* __restrict
         const int blk = idx / 272, rem = idx % 272, row = rem / 17, col = rem % 17;
         const double2 mv = gload2(Mk, (size_t)blk * 256 + (row >> 2) * 64 + col + 16 * (row & 3));
         const double2 a = gload2(P.tr_v, e);
-        sv += a.x * mv.y + a.y * mv.x;  // Im(a * M[b][a])
+        sv += a.x * mv.x + a.y * mv.y;  // Im(a * M[b][a]), a stored as (Im, Re)
       }
 #pragma unroll
       for (int msk = 32; msk >= 1; msk >>= 1) sv += __shfl_xor(sv, msk);

@@ -54,7 +54,9 @@ struct ProbDev {
   const int* cs_j;
   const double2* cs_v;
   // trace operand: for control j, the nonzeros A_j[a][b] as (LDS index of the
-  // element (b, a) of a tile, value), entries [tr_ptr[j], tr_ptr[j+1])
+  // element (b, a) of a tile, value), entries [tr_ptr[j], tr_ptr[j+1]); the
+  // value is stored as (Im a, Re a) for closed dynamics and (Re a, -Im a) for
+  // Lindblad, so that Re(c a m) = v.x Re m + v.y Im m in both cases
   const int* tr_ptr;
   const int* tr_idx;
   const double2* tr_v;
@@ -841,7 +843,6 @@ __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, d
   constexpr int THREADS = 64 * B * B;
   constexpr int G = floor_pow2(THREADS / V);
   const int nc = P.nc;
-  const bool want_im = (P.cim != 0.0);  // closed: Re(-i z) = Im z
   if constexpr (OWN32) {
     // M held in the FP32 own layout (rows 4q + r), same LDS positions
 #pragma unroll
@@ -860,14 +861,14 @@ __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, d
       for (int e = gptr(P.tr_ptr)[j] + gl; e < e1; e += G) {
         const double2 a = gload2(P.tr_v, e);
         const double2 mv = LA[gptr(P.tr_idx)[e]];
-        sv += want_im ? (a.x * mv.y + a.y * mv.x) : (a.x * mv.x - a.y * mv.y);
+        sv += a.x * mv.x + a.y * mv.y;
       }
     }
     sv = group_sum<G>(sv);
     if (grp < V && gl == 0 && j < nc) {
       const double f = slice_dt(P, k) * P.scale[(size_t)m * nc + j];
       // Re(c * f * z): closed c = -i -> f Im z ; Lindblad c = 1 -> f Re z
-      gptr_w(P.Gm)[(size_t)m * nc * P.nt + (size_t)k * nc + j] = want_im ? f * sv * (-P.cim) : f * sv * P.cre;
+      gptr_w(P.Gm)[(size_t)m * nc * P.nt + (size_t)k * nc + j] = f * sv;
     }
   }
   bsync<B>();
@@ -917,17 +918,16 @@ __device__ inline void contract_regs(const ProbDev& P, int m, int k, double dtk,
   for (int r = 0; r < 4; r++)
     LA[lidx(0, OWN32 ? rowof32(x.q, r) : rowof(x.q, r), x.c)] = make_double2(Mk.re[r], Mk.im[r]);
   bsync<1>();
-  const bool want_im = (P.cim != 0.0);
   double sv = 0;
 #pragma unroll
   for (int i = 0; i < TR_REG; i++) {
     const double2 mv = LA[t.idx[i]];
-    sv += want_im ? (t.v[i].x * mv.y + t.v[i].y * mv.x) : (t.v[i].x * mv.x - t.v[i].y * mv.y);
+    sv = fma(t.v[i].x, mv.x, fma(t.v[i].y, mv.y, sv));
   }
   sv = group_sum<G>(sv);
   if (gl == 0 && grp < P.nc) {
     const double f = dtk * t.sc;
-    gptr_w(P.Gm)[(size_t)m * P.nc * P.nt + (size_t)k * P.nc + grp] = want_im ? f * sv * (-P.cim) : f * sv * P.cre;
+    gptr_w(P.Gm)[(size_t)m * P.nc * P.nt + (size_t)k * P.nc + grp] = f * sv;
   }
   bsync<1>();
 }

@@ -797,6 +797,11 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       }
       // the gradient contracts the operators as given (Gradients.jl:274-289)
       Trace tr = build_trace(q.Hc, nc, d);
+      // the contraction needs Re(c Tr(A_j M)): Im z for closed dynamics
+      // (c = -i), Re z for Lindblad (c = 1). Each entry a = A_j[a][b] is stored
+      // as (Im a, Re a) or (Re a, -Im a), so that term = v.x Re m + v.y Im m.
+      for (auto& v : tr.v)
+        v = q.dynamics == PR_CLOSED ? make_double2(v.y, v.x) : make_double2(v.x, -v.y);
       if ((rc = pl.upload(&ip, tr.ptr.data(), tr.ptr.size()))) return rc;
       if ((rc = pl.upload(&jp, tr.idx.data(), tr.idx.size()))) return rc;
       if ((rc = pl.upload(&vp, tr.v.data(), tr.v.size()))) return rc;

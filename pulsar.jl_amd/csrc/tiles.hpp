@@ -282,8 +282,9 @@ __device__ inline double pair_sum32(double v) {
 __device__ inline double sum_rows_pairwise(double v) { return pair_sum32(pair_sum16(v)); }
 template <int CTRL>
 __device__ inline double dpp_d(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  // every pattern used reads a lane of the same row, so no lane keeps "old"
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 // maximum over the 16 lanes of each row: quad swaps, then half-row and row mirrors
