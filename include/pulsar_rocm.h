@@ -220,6 +220,33 @@ int pr_grape_optimize(void* fid_plan, void* grad_plan, const double* controls_in
                       double* controls_best, double* fidelity_history, double* gnorm_history,
                       pr_grape_result* result);
 
+/* Ensemble L-BFGS-GRAPE with the iteration on the device: replaces the loop of
+ * grape_optimize_ensemble (src/Optimization/GRAPE.jl:1012-1230), with the
+ * two-loop recursion and safeguards of :1062-1080 and the backtracking line
+ * search of :1082-1104 (F only, step halved up to 20 times, accepted at
+ * F_new >= F). The curvature pairs (S, Y), the controls, the gradient and the
+ * best controls stay in device memory; the host reads F, |g|, max|d| and the
+ * pair's curvature each iteration. plan: one problem (the ensemble, weights
+ * 1/M for the mean). Stops: PR_STOP_GRADIENT (|g| < gradient_norm_tol),
+ * PR_STOP_PERFECT (F >= 1 - 1e-8), PR_STOP_FIDELITY (|F_n - F_{n-1}| <
+ * convergence_tol), PR_STOP_ZERO_DIRECTION (max|d| < 1e-14). */
+typedef struct {
+  int max_iter;
+  double convergence_tol;
+  double step_size;          /* initial trial step (times amplitude_limit when it is > 0) */
+  double gradient_norm_tol;
+  int lbfgs_memory;          /* 1..64 curvature pairs */
+  double amplitude_limit;    /* > 0: controls clipped to [-limit, limit]; <= 0: no clipping */
+} pr_lbfgs_config;
+
+#define PR_STOP_ZERO_DIRECTION 4
+
+/* controls_init / controls_best: [n_ctrl x n_t] column-major host arrays;
+ * fidelity_history needs max_iter + 1 entries, gnorm_history max_iter.
+ * result->step_size_final is the last accepted line-search step. */
+int pr_lbfgs_optimize(void* plan, const double* controls_init, const pr_lbfgs_config* cfg, double* controls_best,
+                      double* fidelity_history, double* gnorm_history, pr_grape_result* result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1304,4 +1304,115 @@ __global__ void __launch_bounds__(64) k_norm2_final(const double* __restrict__ p
   if (threadIdx.x == 0) out[0] = s;
 }
 
+// ------------------------------------------------------- ensemble L-BFGS
+// The ascent direction of the L-BFGS two-loop recursion (Nocedal & Wright,
+// Algorithm 7.4) with the safeguards of the reference's ensemble optimizer
+// (GRAPE.jl:1062-1080): rho_i = 1 / max(y_i.s_i, 1e-14), initial scaling
+// gamma = s.y / max(y.y, 1e-14) from the newest pair. One workgroup walks the
+// whole vector; every dot product is a fixed-order sum (deterministic).
+// Pairs are rows of S and Y ([rows][n]); pair i (0 = oldest) is row
+// (first + i) % rows. out[0] = max |d_i|.
+constexpr int LBFGS_THREADS = 1024;
+
+__device__ inline double lbfgs_block_sum(double v, double* ws) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0;
+  for (int w = 0; w < LBFGS_THREADS / 64; w++) t += ws[w];
+  __syncthreads();
+  return t;
+}
+
+__global__ void __launch_bounds__(LBFGS_THREADS)
+    k_lbfgs_direction(const double* __restrict__ g, const double* __restrict__ S, const double* __restrict__ Y,
+                      int k, int first, int rows, long long n, double* __restrict__ d, double* __restrict__ out) {
+  __shared__ double ws[LBFGS_THREADS / 64];
+  __shared__ double alpha[64];
+  const int t = threadIdx.x;
+  for (long long i = t; i < n; i += LBFGS_THREADS) d[i] = g[i];
+  __syncthreads();
+  auto row = [&](const double* M, int i) { return M + (size_t)((first + i) % rows) * n; };
+  auto dot = [&](const double* a, const double* b) {
+    double v = 0;
+    for (long long i = t; i < n; i += LBFGS_THREADS) v += a[i] * b[i];
+    return lbfgs_block_sum(v, ws);
+  };
+  for (int i = k - 1; i >= 0; i--) {
+    const double* s = row(S, i);
+    const double* y = row(Y, i);
+    const double rho = 1.0 / fmax(dot(y, s), 1e-14);
+    const double a = rho * dot(s, d);
+    if (t == 0) alpha[i] = a;
+    for (long long j = t; j < n; j += LBFGS_THREADS) d[j] -= a * y[j];
+    __syncthreads();
+  }
+  if (k > 0) {
+    const double* s = row(S, k - 1);
+    const double* y = row(Y, k - 1);
+    const double gamma = dot(s, y) / fmax(dot(y, y), 1e-14);
+    for (long long j = t; j < n; j += LBFGS_THREADS) d[j] *= gamma;
+    __syncthreads();
+  }
+  for (int i = 0; i < k; i++) {
+    const double* s = row(S, i);
+    const double* y = row(Y, i);
+    const double rho = 1.0 / fmax(dot(y, s), 1e-14);
+    const double beta = rho * dot(y, d);
+    const double c = alpha[i] - beta;
+    for (long long j = t; j < n; j += LBFGS_THREADS) d[j] += c * s[j];
+    __syncthreads();
+  }
+  double m = 0;
+  for (long long i = t; i < n; i += LBFGS_THREADS) m = fmax(m, fabs(d[i]));
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((t & 63) == 0) ws[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    double r = 0;
+    for (int w = 0; w < LBFGS_THREADS / 64; w++) r = fmax(r, ws[w]);
+    out[0] = r;
+  }
+}
+
+// trial point un = u + a d (d already divided by max |d|), clipped to
+// [-limit, limit] when limit > 0
+__global__ void k_lbfgs_trial(const double* __restrict__ u, const double* __restrict__ d, double a, double limit,
+                              long long n, double* __restrict__ un) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = u[i] + a * d[i];
+  if (limit > 0) v = fmin(fmax(v, -limit), limit);
+  un[i] = v;
+}
+
+// d /= dmax (dmax read from device memory)
+__global__ void k_lbfgs_scale(double* __restrict__ d, const double* __restrict__ dmax, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] /= dmax[0];
+}
+
+// curvature pair in row `slot`: s = a d, y = gn - g; out[0] = y.s, out[1] = s.s
+__global__ void __launch_bounds__(LBFGS_THREADS)
+    k_lbfgs_pair(const double* __restrict__ d, double a, const double* __restrict__ gn, const double* __restrict__ g,
+                 long long n, double* __restrict__ s, double* __restrict__ y, double* __restrict__ out) {
+  __shared__ double ws[LBFGS_THREADS / 64];
+  double ys = 0, ss = 0;
+  for (long long i = threadIdx.x; i < n; i += LBFGS_THREADS) {
+    const double sv = a * d[i], yv = gn[i] - g[i];
+    s[i] = sv;
+    y[i] = yv;
+    ys += yv * sv;
+    ss += sv * sv;
+  }
+  ys = lbfgs_block_sum(ys, ws);
+  ss = lbfgs_block_sum(ss, ws);
+  if (threadIdx.x == 0) {
+    out[0] = ys;
+    out[1] = ss;
+  }
+}
+
 }  // namespace pr

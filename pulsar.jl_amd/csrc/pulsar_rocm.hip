@@ -1216,6 +1216,161 @@ int pr_grape_optimize(void* fid_plan, void* grad_plan, const double* controls_in
   return PR_OK;
 }
 
+int pr_lbfgs_optimize(void* plan, const double* controls_init, const pr_lbfgs_config* cfg, double* controls_best,
+                      double* fidelity_history, double* gnorm_history, pr_grape_result* result) {
+  if (fault_is("eval")) return fail(PR_ERR_INJECTED, "failure injected by PULSAR_ROCM_FAULT=eval");
+  if (!plan || !controls_init || !cfg || !controls_best || !fidelity_history || !gnorm_history || !result)
+    return fail(PR_ERR_ARG, "pr_lbfgs_optimize: NULL argument");
+  Plan& pl = *static_cast<Plan*>(plan);
+  if (pl.n_probs != 1) return fail(PR_ERR_ARG, "pr_lbfgs_optimize: the plan must hold one problem");
+  if (cfg->max_iter <= 0 || !(cfg->step_size > 0) || cfg->lbfgs_memory < 1 || cfg->lbfgs_memory > 64)
+    return fail(PR_ERR_ARG, "pr_lbfgs_optimize: invalid configuration");
+  if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
+  const long long n = pl.n_params;
+  const int mem = cfg->lbfgs_memory;
+  const bool clip = cfg->amplitude_limit > 0;
+  hipStream_t st = pl.stream;
+  // device vectors: u (plan controls), un, g, gn, d, best, S and Y rows, scalars
+  DevBuf vecs, hist, scal;
+  HIPCHK(hipMalloc(&vecs.p, sizeof(double) * (size_t)n * 5));
+  // mem + 1 rows: a new pair is written to the free row and joins the history
+  // only if its curvature is accepted
+  const int R = mem + 1;
+  HIPCHK(hipMalloc(&hist.p, sizeof(double) * (size_t)n * 2 * R));
+  HIPCHK(hipMalloc(&scal.p, sizeof(double) * (pr::NORM_BLOCKS + 4)));
+  double* d_u = pl.d_controls;
+  double* d_un = static_cast<double*>(vecs.p);
+  double* d_g = d_un + n;
+  double* d_gn = d_g + n;
+  double* d_d = d_gn + n;
+  double* d_best = d_d + n;
+  double* d_S = static_cast<double*>(hist.p);
+  double* d_Y = d_S + (size_t)n * R;
+  double* d_part = static_cast<double*>(scal.p);
+  double* d_sc = d_part + pr::NORM_BLOCKS;  // [0] |g|^2 or max|d|, [1..2] y.s, s.s
+  const dim3 eb((unsigned)((n + 255) / 256)), et(256);
+  HIPCHK(hipMemcpyAsync(d_u, controls_init, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  int rc;
+  auto read = [&](const double* src, double* dst, int cnt) -> int {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PR_OK;
+  };
+  auto fid = [&](const double* x, double* F) -> int {
+    int r = run_graph(pl, x, pl.d_F, nullptr, st);
+    return r ? r : read(pl.d_F, F, 1);
+  };
+  // F and the gradient at x into gout
+  auto grad = [&](const double* x, double* gout, double* F) -> int {
+    int r = run_graph(pl, x, pl.d_F, gout, st);
+    return r ? r : read(pl.d_F, F, 1);
+  };
+  auto trial = [&](double a) -> int {
+    hipLaunchKernelGGL(pr::k_lbfgs_trial, eb, et, 0, st, d_u, d_d, a, clip ? cfg->amplitude_limit : 0.0, n, d_un);
+    HIPCHK(hipGetLastError());
+    return PR_OK;
+  };
+  std::vector<double> fh;
+  int n_f = 0, n_g = 0, n_iter = 0, converged = 0, reason = PR_STOP_MAX_ITER, k = 0, first = 0;
+  double F = 0, best_F, a_last = 0;
+  if ((rc = fid(d_u, &F))) return rc;
+  n_f++;
+  fh.push_back(F);
+  best_F = F;
+  HIPCHK(hipMemcpyAsync(d_best, d_u, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  for (int it = 1; it <= cfg->max_iter; it++) {
+    n_iter = it;
+    double Fg = 0;
+    if ((rc = grad(d_u, d_g, &Fg))) return rc;
+    n_g++;
+    hipLaunchKernelGGL(pr::k_norm2_part, dim3(pr::NORM_BLOCKS), dim3(256), 0, st, d_g, n, d_part);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(pr::k_norm2_final, dim3(1), dim3(64), 0, st, d_part, d_sc);
+    HIPCHK(hipGetLastError());
+    double g2 = 0;
+    if ((rc = read(d_sc, &g2, 1))) return rc;
+    const double gnorm = std::sqrt(g2);
+    gnorm_history[it - 1] = gnorm;
+    hipLaunchKernelGGL(pr::k_lbfgs_direction, dim3(1), dim3(pr::LBFGS_THREADS), 0, st, d_g, d_S, d_Y, k, first, R,
+                       n, d_d, d_sc);
+    HIPCHK(hipGetLastError());
+    double dmax = 0;
+    if ((rc = read(d_sc, &dmax, 1))) return rc;
+    if (dmax < 1e-14) {
+      converged = 1;
+      reason = PR_STOP_ZERO_DIRECTION;
+      break;
+    }
+    hipLaunchKernelGGL(pr::k_lbfgs_scale, eb, et, 0, st, d_d, d_sc, n);
+    HIPCHK(hipGetLastError());
+    // backtracking line search on F (GRAPE.jl:1082-1104)
+    double a = clip ? cfg->step_size * cfg->amplitude_limit : cfg->step_size, Fn = 0;
+    bool ok = false;
+    for (int bt = 0; bt < 20; bt++) {
+      if ((rc = trial(a)) || (rc = fid(d_un, &Fn))) return rc;
+      n_f++;
+      if (Fn >= F) {
+        ok = true;
+        break;
+      }
+      a *= 0.5;
+    }
+    if (!ok) {
+      if ((rc = trial(a)) || (rc = fid(d_un, &Fn))) return rc;
+      n_f++;
+    }
+    a_last = a;
+    if ((rc = grad(d_un, d_gn, &Fg))) return rc;
+    n_g++;
+    // curvature pair into the free row, kept when y.s > 1e-14 |s|^2 (the
+    // oldest pair dropped beyond mem)
+    const int slot = (first + k) % R;
+    hipLaunchKernelGGL(pr::k_lbfgs_pair, dim3(1), dim3(pr::LBFGS_THREADS), 0, st, d_d, a, d_gn, d_g, n,
+                       d_S + (size_t)slot * n, d_Y + (size_t)slot * n, d_sc + 1);
+    HIPCHK(hipGetLastError());
+    double ysss[2];
+    if ((rc = read(d_sc + 1, ysss, 2))) return rc;
+    if (ysss[0] > 1e-14 * ysss[1]) {
+      if (k < mem)
+        k++;
+      else
+        first = (first + 1) % R;
+    }
+    HIPCHK(hipMemcpyAsync(d_u, d_un, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    F = Fn;
+    fh.push_back(F);
+    if (F > best_F) {
+      best_F = F;
+      HIPCHK(hipMemcpyAsync(d_best, d_u, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    }
+    const size_t nh = fh.size();
+    int why = -1;
+    if (gnorm < cfg->gradient_norm_tol)
+      why = PR_STOP_GRADIENT;
+    else if (F >= 1.0 - 1e-8)
+      why = PR_STOP_PERFECT;
+    else if (nh >= 2 && std::fabs(fh[nh - 1] - fh[nh - 2]) < cfg->convergence_tol)
+      why = PR_STOP_FIDELITY;
+    if (why >= 0) {
+      converged = 1;
+      reason = why;
+      break;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(controls_best, d_best, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::copy(fh.begin(), fh.end(), fidelity_history);
+  result->n_iterations = n_iter;
+  result->converged = converged;
+  result->reason = reason;
+  result->n_fidelity_history = (int)fh.size();
+  result->n_fidelity_evaluations = n_f;
+  result->n_gradient_evaluations = n_g;
+  result->fidelity = best_F;
+  result->step_size_final = a_last;
+  return PR_OK;
+}
+
 int pr_plan_destroy(void* plan) {
   if (!plan) return PR_OK;
   delete static_cast<Plan*>(plan);

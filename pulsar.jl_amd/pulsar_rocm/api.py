@@ -852,11 +852,14 @@ def _ensemble_lbfgs(F_fn, G_fn, controls0, config, lbfgs_memory=10, amplitude_li
 
 def grape_optimize_ensemble(systems, target: QuantumTarget, controls_init: ControlSequence,
                             config: Optional[GRAPEConfig] = None, lbfgs_memory: int = 10,
-                            amplitude_limit: float = np.inf) -> OptimizationResult:
+                            amplitude_limit: float = np.inf, on_device: bool = True) -> OptimizationResult:
     """grape_optimize_ensemble (GRAPE.jl:1012-1230): broadband L-BFGS-GRAPE over
     an ensemble of systems, maximising the ensemble-mean fidelity. Every F and
     gradient is one batched device evaluation of the whole ensemble (line-search
-    steps evaluate F only)."""
+    steps evaluate F only). With on_device (and no callback or printing, which
+    need the Python loop) the whole loop runs in the library
+    (pr_lbfgs_optimize): controls, gradients and curvature pairs stay on the
+    device."""
     config = config or GRAPEConfig()
     if len(systems) == 0:
         raise ValueError("systems must be non-empty")
@@ -865,6 +868,27 @@ def grape_optimize_ensemble(systems, target: QuantumTarget, controls_init: Contr
     t0 = time.time()
     obj = build_ensemble_from_systems(systems, target, controls_init, aggregator="mean")
     shape = controls_init.controls.shape
+    be = obj.batched_eval
+    if (on_device and config.callback is None and not config.verbose and len(be.probs) == 1
+            and lbfgs_memory <= 64):
+        try:
+            cfg = native.LbfgsConfig(int(config.max_iter), float(config.convergence_tol), float(config.step_size),
+                                     float(config.gradient_norm_tol), int(lbfgs_memory),
+                                     float(amplitude_limit) if np.isfinite(amplitude_limit) else 0.0)
+            best, f_hist, g_hist, res = native.lbfgs_optimize_device(be.plan, controls_init.controls, cfg)
+        finally:
+            be.close()
+        reason = {native.PR_STOP_CODES["gradient"]: f"gradient norm < {config.gradient_norm_tol}",
+                  native.PR_STOP_CODES["perfect"]: "near-perfect fidelity",
+                  native.PR_STOP_CODES["fidelity"]: f"fidelity change < {config.convergence_tol}",
+                  native.PR_STOP_CODES["zero_direction"]: "zero gradient direction"}.get(
+                      res.reason, "maximum iterations reached")
+        return OptimizationResult(best, float(res.fidelity), f_hist, g_hist, len(f_hist), bool(res.converged),
+                                  reason, time.time() - t0, int(res.n_fidelity_evaluations),
+                                  int(res.n_gradient_evaluations),
+                                  {"algorithm": "Ensemble L-BFGS-GRAPE", "n_systems": len(systems),
+                                   "lbfgs_memory": lbfgs_memory, "on_device": True,
+                                   "step_size_final": float(res.step_size_final)})
 
     def F_fn(u):
         return ensemble_value(obj, u.reshape(-1, order="F"))

@@ -29,7 +29,7 @@ PR_ERR = {
 EXPORTS = ("pr_abi_version", "pr_device_count", "pr_init", "pr_last_error", "pr_plan_create",
            "pr_eval", "pr_eval_device", "pr_plan_n_params", "pr_plan_set_timing",
            "pr_plan_kernel_times", "pr_plan_buffers", "pr_plan_upload_controls",
-           "pr_plan_synchronize", "pr_plan_destroy", "pr_grape_optimize")
+           "pr_plan_synchronize", "pr_plan_destroy", "pr_grape_optimize", "pr_lbfgs_optimize")
 KERNEL_CLASSES = ("segment", "combine", "gradient", "reduce")
 
 
@@ -48,7 +48,16 @@ class GrapeResult(ctypes.Structure):
                 ("step_size_final", ctypes.c_double)]
 
 
-PR_STOP = {0: "max_iter", 1: "perfect", 2: "gradient", 3: "fidelity"}
+class LbfgsConfig(ctypes.Structure):
+    """pr_lbfgs_config (the ensemble L-BFGS loop's knobs, GRAPE.jl:1012-1230)."""
+
+    _fields_ = [("max_iter", ctypes.c_int), ("convergence_tol", ctypes.c_double), ("step_size", ctypes.c_double),
+                ("gradient_norm_tol", ctypes.c_double), ("lbfgs_memory", ctypes.c_int),
+                ("amplitude_limit", ctypes.c_double)]
+
+
+PR_STOP = {0: "max_iter", 1: "perfect", 2: "gradient", 3: "fidelity", 4: "zero_direction"}
+PR_STOP_CODES = {v: k for k, v in PR_STOP.items()}
 
 
 class RocmError(RuntimeError):
@@ -87,6 +96,8 @@ def load(path: str = LIB_PATH):
     L.pr_plan_synchronize.argtypes = [ctypes.c_void_p]
     L.pr_plan_destroy.argtypes = [ctypes.c_void_p]
     L.pr_grape_optimize.argtypes = [ctypes.c_void_p, ctypes.c_void_p, dp, ctypes.POINTER(GrapeConfig), dp, dp, dp,
+                                    ctypes.POINTER(GrapeResult)]
+    L.pr_lbfgs_optimize.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(LbfgsConfig), dp, dp, dp,
                                     ctypes.POINTER(GrapeResult)]
     _lib = L
     return L
@@ -218,4 +229,21 @@ def grape_optimize_device(fid_plan: Plan, grad_plan, controls: np.ndarray, cfg: 
     res = GrapeResult()
     check(load().pr_grape_optimize(fid_plan._h, grad_plan._h if grad_plan is not None else None, _dptr(u0),
                                    ctypes.byref(cfg), _dptr(best), _dptr(fh), _dptr(gh), ctypes.byref(res)))
+    return (grad_from_cm(best, n_c, n_t), fh[:res.n_fidelity_history].tolist(), gh[:res.n_iterations].tolist(), res)
+
+
+def lbfgs_optimize_device(plan: Plan, controls: np.ndarray, cfg: LbfgsConfig):
+    """pr_lbfgs_optimize: the ensemble L-BFGS loop with the controls, gradient
+    and curvature pairs resident on the device. controls [n_ctrl, n_t];
+    returns (best controls, fidelity history, gradient-norm history,
+    GrapeResult)."""
+    u = np.asarray(controls, dtype=np.float64)
+    n_c, n_t = u.shape
+    u0 = controls_cm(u)
+    best = np.zeros_like(u0)
+    fh = np.zeros(cfg.max_iter + 1)
+    gh = np.zeros(cfg.max_iter)
+    res = GrapeResult()
+    check(load().pr_lbfgs_optimize(plan._h, _dptr(u0), ctypes.byref(cfg), _dptr(best), _dptr(fh), _dptr(gh),
+                                   ctypes.byref(res)))
     return (grad_from_cm(best, n_c, n_t), fh[:res.n_fidelity_history].tolist(), gh[:res.n_iterations].tolist(), res)

@@ -212,6 +212,33 @@ def test_grape_optimize_ensemble_matches_oracle_driven_loop():
     assert res.metadata["algorithm"] == "Ensemble L-BFGS-GRAPE"
 
 
+@pytest.mark.parametrize("memory,limit", [(3, 10_000.0), (10, np.inf)])
+def test_device_ensemble_lbfgs_matches_host_loop(memory, limit):
+    # pr_lbfgs_optimize against the Python loop over the same device
+    # evaluations: a short history (the oldest pairs dropped) with clipping, and
+    # the default history without it
+    Iz, Ix, Iy = 0.5 * SZ, 0.5 * SX, 0.5 * SY
+    systems = [P.quantum_system(2 * np.pi * df * Iz, [2 * np.pi * Ix, 2 * np.pi * Iy])
+               for df in np.linspace(-4000, 4000, 5)]
+    tgt = P.unitary_target(np.array([[0, -1j], [-1j, 0]]))
+    ctrl = P.random_controls(systems[2], 400e-6, 40, amplitude=800.0, rng=np.random.default_rng(12))
+    cfg = P.GRAPEConfig(max_iter=14, step_size=0.05, convergence_tol=1e-12)
+    print(f"case memory={memory} limit={limit}", flush=True)
+    dev = P.grape_optimize_ensemble(systems, tgt, ctrl, config=cfg, lbfgs_memory=memory, amplitude_limit=limit)
+    host = P.grape_optimize_ensemble(systems, tgt, ctrl, config=cfg, lbfgs_memory=memory, amplitude_limit=limit,
+                                     on_device=False)
+    assert dev.metadata.get("on_device") is True
+    assert dev.converged == host.converged and dev.termination_reason == host.termination_reason
+    assert len(dev.fidelity_history) == len(host.fidelity_history)
+    assert np.abs(np.array(dev.fidelity_history) - np.array(host.fidelity_history)).max() <= 1e-10
+    assert np.allclose(dev.gradient_norm_history, host.gradient_norm_history, rtol=1e-8, atol=0)
+    assert dev.n_fidelity_evaluations == host.n_fidelity_evaluations
+    assert dev.n_gradient_evaluations == host.n_gradient_evaluations
+    assert abs(dev.fidelity - host.fidelity) <= 1e-10
+    assert np.abs(dev.controls - host.controls).max() <= 1e-6 * np.abs(host.controls).max()
+    assert dev.fidelity > dev.fidelity_history[0]
+
+
 # ----------------------------------------------------------------- device GRAPE loop
 
 
