@@ -38,13 +38,13 @@ end
 #       ... existing CPU body ...
 #   end
 
-# build_ensemble_from_systems → batched_eval  (src/Optimization/Ensemble/SystemBuilder.jl:49-87)
-function _rocm_ensemble_objective(systems::Vector{<:AbstractQuantumSystem}, target::QuantumTarget,
-                                  controls::ControlSequence)
+# one device plan for the whole ensemble (members share the control operators
+# up to per-channel scale factors, _scale_controls)
+function _rocm_ensemble_plan(systems::Vector{<:AbstractQuantumSystem}, target::QuantumTarget,
+                             controls::ControlSequence)
     n_c = size(controls.controls, 1)
     n_t = controls.n_steps
     ref = systems[1].H_controls
-    # members share the control operators up to per-channel scale factors (_scale_controls)
     scales = zeros(length(systems), n_c)
     for (i, s) in enumerate(systems), j in 1:n_c
         a = ref[j]; b = s.H_controls[j]
@@ -55,7 +55,15 @@ function _rocm_ensemble_objective(systems::Vector{<:AbstractQuantumSystem}, targ
     end
     prob, roots = rocm_problem([s.H_drift for s in systems], ref, n_t, controls.dt, target;
                                scales = scales, fixed_point = true)
-    plan = ROCmPlan([prob], roots)
+    return ROCmPlan([prob], roots)
+end
+
+# build_ensemble_from_systems → batched_eval  (src/Optimization/Ensemble/SystemBuilder.jl:49-87)
+function _rocm_ensemble_objective(systems::Vector{<:AbstractQuantumSystem}, target::QuantumTarget,
+                                  controls::ControlSequence)
+    n_c = size(controls.controls, 1)
+    n_t = controls.n_steps
+    plan = _rocm_ensemble_plan(systems, target, controls)
     batched = θ -> begin
         F, G = rocm_eval(plan, [reshape(Vector{Float64}(θ), n_c, n_t)])
         (F[1], vec(G[1]))
@@ -131,3 +139,45 @@ function _rocm_grape_optimize(system::AbstractQuantumSystem, target::QuantumTarg
                                                 "n_controls" => n_c, "n_timesteps" => n_t, "dt" => dt,
                                                 "total_time" => dt * n_t))
 end
+
+# grape_optimize_ensemble(systems, target, controls_init; config, lbfgs_memory, amplitude_limit)
+# (src/Optimization/GRAPE.jl:1012-1230). The L-BFGS loop in the library
+# (pr_lbfgs_optimize): two-loop recursion and line search of GRAPE.jl:1062-1104
+# with controls, gradient and curvature pairs resident on the device. Used when
+# no callback or printing is requested (those need the Julia loop).
+struct PRLbfgsConfig
+    max_iter::Cint
+    convergence_tol::Cdouble
+    step_size::Cdouble
+    gradient_norm_tol::Cdouble
+    lbfgs_memory::Cint
+    amplitude_limit::Cdouble
+end
+
+function _rocm_grape_optimize_ensemble(systems::Vector{<:AbstractQuantumSystem}, target::QuantumTarget,
+                                       controls_init::ControlSequence, config::GRAPEConfig;
+                                       lbfgs_memory::Int = 10, amplitude_limit::Real = Inf)::OptimizationResult
+    t0 = time()
+    plan = _rocm_ensemble_plan(systems, target, controls_init)
+    cfg = PRLbfgsConfig(config.max_iter, config.convergence_tol, config.step_size, config.gradient_norm_tol,
+                        lbfgs_memory, isfinite(amplitude_limit) ? amplitude_limit : 0.0)
+    u0 = Matrix{Float64}(controls_init.controls)
+    best = similar(u0)
+    fh = zeros(config.max_iter + 1)
+    gh = zeros(config.max_iter)
+    res = PRGrapeResult()
+    GC.@preserve u0 best fh gh begin
+        _rocm_check(ccall((:pr_lbfgs_optimize, _ROCM_LIB[]), Cint,
+                          (Ptr{Cvoid}, Ptr{Float64}, Ref{PRLbfgsConfig}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                           Ref{PRGrapeResult}), plan.handle, u0, cfg, best, fh, gh, res),
+                    "pr_lbfgs_optimize")
+    end
+    reasons = ("maximum iterations reached", "near-perfect fidelity", "gradient norm below tolerance",
+               "fidelity change below tolerance", "zero gradient direction")
+    return OptimizationResult(best, res.fidelity, fh[1:res.n_fidelity_history], gh[1:res.n_iterations],
+                              Int(res.n_fidelity_history), res.converged != 0, reasons[res.reason + 1], time() - t0,
+                              Int(res.n_fidelity_evaluations), Int(res.n_gradient_evaluations),
+                              Dict{String, Any}("algorithm" => "Ensemble L-BFGS-GRAPE",
+                                                "n_systems" => length(systems), "lbfgs_memory" => lbfgs_memory))
+end
+
