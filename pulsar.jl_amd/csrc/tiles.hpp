@@ -26,8 +26,8 @@ struct Blk {
   d4 re, im;
 };
 
-// none, transpose, conjugate transpose, conjugate; NL: none, always read from LDS
-enum Op { N = 0, T = 1, C = 2, J = 3, NL = 4 };
+// none, transpose, conjugate transpose, conjugate
+enum Op { N = 0, T = 1, C = 2, J = 3 };
 
 struct Ln {
   int lane, q, c, w, I, Jb;
@@ -154,7 +154,7 @@ __device__ inline double2 fetch_b(const double2* L, const Blk& own, int K, int k
     return v;
   } else {
     double2 v;
-    if constexpr (OP == N || OP == J || OP == NL)
+    if constexpr (OP == N || OP == J)
       v = L[lidx(K * B + x.Jb, rowof(x.q, kb), x.c)];
     else
       v = L[lidx(x.Jb * B + K, x.c, rowof(x.q, kb))];
