@@ -85,17 +85,18 @@ __device__ void mm(double2* __restrict__ Out, const double2* A, const double2* B
         const Ln x = blk_info<B>(b);
 #pragma unroll
         for (int kb = 0; kb < 4; kb++) {
-          // op(A)(I, K)[c][q + 4kb]
-          double2 a = (OA == N) ? LP[I * 272 + x.c * 17 + rowof(x.q, kb)] : LP[I * 272 + rowof(x.q, kb) * 17 + x.c];
-          if (OA == C) a.y = -a.y;
-          // op(B)(K, J)[q + 4kb][c]
-          double2 bb = (OB == N) ? LP[(B + J) * 272 + rowof(x.q, kb) * 17 + x.c]
-                                 : LP[(B + J) * 272 + x.c * 17 + rowof(x.q, kb)];
-          if (OB == C) bb.y = -bb.y;
+          // op(A)(I, K)[c][q + 4kb] and op(B)(K, J)[q + 4kb][c] before
+          // conjugation: the signs of the imaginary parts go into the MFMA
+          // negate modifiers (Re += ar br - ai bi, Im += ar bi + ai br)
+          const double2 a = (OA == N) ? LP[I * 272 + x.c * 17 + rowof(x.q, kb)]
+                                      : LP[I * 272 + rowof(x.q, kb) * 17 + x.c];
+          const double2 bb = (OB == N) ? LP[(B + J) * 272 + rowof(x.q, kb) * 17 + x.c]
+                                       : LP[(B + J) * 272 + x.c * 17 + rowof(x.q, kb)];
+          constexpr bool ca = OA == C, cb = OB == C;
           acc[p].re = mfma(a.x, bb.x, acc[p].re);
-          acc[p].im = mfma(a.x, bb.y, acc[p].im);
-          acc[p].re = mfma(-a.y, bb.y, acc[p].re);
-          acc[p].im = mfma(a.y, bb.x, acc[p].im);
+          acc[p].im = cb ? mfma_nega(a.x, bb.y, acc[p].im) : mfma(a.x, bb.y, acc[p].im);
+          acc[p].re = (ca != cb) ? mfma(a.y, bb.y, acc[p].re) : mfma_nega(a.y, bb.y, acc[p].re);
+          acc[p].im = ca ? mfma_nega(a.y, bb.x, acc[p].im) : mfma(a.y, bb.x, acc[p].im);
         }
       }
     }
