@@ -21,6 +21,7 @@
  *   ensemble_value_and_grad batched_eval      src/Optimization/Ensemble/EnsembleObjective.jl:272-307
  *   grape_state_kernel (:cpu/:cuda branches)  src/Application/MR/GRAPEState.jl:73-85, 136-250
  *   grape_lindblad_kernel                     src/Application/MR/GRAPELindblad.jl:83-92, 96-215
+ *   grape_tracking_kernel                     src/Application/MR/GRAPETracking.jl:145-263
  *
  * Conventions (identical to Julia's in-memory layout, so a ccall passes
  * arrays without copies):
@@ -65,6 +66,13 @@ extern "C" {
 #define PR_GATE 0         /* F_m = |Tr(U_t^dag U_total)/dim|^2 (Objectives.jl:239-246) */
 #define PR_PAIRS_SQUARE 1 /* F_m = sum_s |<t_s|U_total|i_s>|^2 (Objectives.jl:221) */
 #define PR_PAIRS_REAL 2   /* F_m = sum_s Re <t_s|U_total|i_s>  (Objectives.jl:220) */
+#define PR_TRACKING 3     /* F_m = sum_s sum_c w_c Re Tr(rho_c P_{n_c} |i_s><i_s| P_{n_c}^dag),
+                             P_n = U_n ... U_1: density-matrix checkpoints of
+                             _grape_tracking_cpu_threaded (GRAPETracking.jl:162-263);
+                             its gradient is that kernel's co-state formula
+                             -dt Re(i Tr(Lambda^dag [s_mj A_j, rho_{n+1}])) (:238-248).
+                             Closed dynamics, dim <= 64, PR_GRAD_REFERENCE; runs FP64
+                             with stored U_k images whatever precision/store_props say */
 
 /* precision of the device path */
 #define PR_FP64 0  /* every product in FP64 (FP64 MFMA) */
@@ -103,7 +111,15 @@ typedef struct {
   const double* U_target;   /* PR_GATE: [dim*dim] complex */
   int n_pairs;              /* pair objectives: number of (init, target) pairs */
   const double* psi_init;   /* [n_pairs][dim] complex */
-  const double* psi_targ;   /* [n_pairs][dim] complex */
+  const double* psi_targ;   /* [n_pairs][dim] complex (not read by PR_TRACKING) */
+  /* PR_TRACKING checkpoints (TrackingPoint, GRAPETracking.jl:89-93); unused otherwise.
+   * Checkpoint c compares rho after slice track_step[c] (1-based, 1..n_t) with
+   * the density matrix track_target[c] at weight track_weight[c]; several
+   * checkpoints may share a step. psi_init holds the initial pure states. */
+  int n_track;                /* number of checkpoints, >= 1 for PR_TRACKING */
+  const int* track_step;      /* [n_track] */
+  const double* track_weight; /* [n_track] */
+  const double* track_target; /* [n_track][dim*dim] complex */
 } pr_problem;
 
 typedef struct {

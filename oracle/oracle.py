@@ -124,6 +124,78 @@ def evaluate(p: Problem, u: np.ndarray, grad: bool = True, grad_mode: int = PR_G
     return float(F[0]), out_G
 
 
+def tracking_evaluate(p: Problem, u: np.ndarray, grad: bool = True):
+    """F and G[n_ctrl, n_t] of the tracking objective (PR_TRACKING), restating
+    _grape_tracking_cpu_threaded (src/Application/MR/GRAPETracking.jl:162-263)
+    with density matrices, one member (drift x power level) at a time:
+
+      U_n = exp(-i H_n dt_n), H_n = H0_m + sum_j s_mj u_jn A_j        (:200-206)
+      rho[1] = psi psi^dag, rho[n+1] = U_n rho[n] U_n^dag            (:214-218)
+      F_m += w_c Re Tr(rho_c rho[step_c + 1])                         (:221-225)
+      backward n = n_t..1: Lambda += 2 w_c (rho_c - rho[n+1]) at a
+      checkpoint step (:231-235); G[j,n] += -dt_n Re(i Tr(Lambda^dag
+      [s_mj A_j, rho[n+1]])) (:237-243); Lambda = U_n^dag Lambda U_n (:246)
+
+    F = sum_m w_m F_m and G = sum_m w_m G_m with w_m = 1/(n_members n_pairs)
+    unless p.weight is given (the division by N_ens, :252-258). The
+    reference's power level pwr is s_mj = pwr for every control. Exps come
+    from or_expm_neg_i (eigen-decomposition of the Hermitian upper triangle,
+    as _expm_neg_i_into!, src/Computation/Propagators.jl:659-674). Pure
+    Python loops over slices: for small cases only.
+    """
+    d, nc, nt, M = p.dim, p.n_ctrl, p.n_t, p.n_members
+    u = np.asarray(u, np.float64)
+    H0 = np.asarray(p.H0, np.complex128)
+    H0s = H0 if H0.ndim == 3 else np.broadcast_to(H0, (M, d, d))
+    Hc = np.asarray(p.Hc, np.complex128)
+    sc = np.ones((M, nc)) if p.ctrl_scale is None else np.asarray(p.ctrl_scale, np.float64)
+    dts = np.full(nt, float(p.dt)) if p.dt_slices is None else np.asarray(p.dt_slices, np.float64)
+    npairs = p.n_pairs
+    w = np.full(M, 1.0 / (M * npairs)) if p.weight is None else np.asarray(p.weight, np.float64)
+    if p.n_track < 1:
+        raise ValueError("tracking objective without checkpoints")
+    ck = {}
+    for step, wt, tg in zip(np.asarray(p.track_step), np.asarray(p.track_weight), np.asarray(p.track_target)):
+        if not 1 <= int(step) <= nt:
+            raise ValueError(f"TrackingPoint.step={int(step)} out of range [1, {nt}]")
+        ck.setdefault(int(step), []).append((float(wt), np.asarray(tg, np.complex128)))
+    steps = sorted(ck)
+    F = 0.0
+    G = np.zeros((nc, nt))
+    for m in range(M):
+        Ps = []
+        for n in range(nt):
+            H = H0s[m].copy()
+            for j in range(nc):
+                H = H + sc[m, j] * u[j, n] * Hc[j]
+            Ps.append(expm_neg_i(H, dts[n]))
+        Hk = [sc[m, j] * Hc[j] for j in range(nc)]
+        fm = 0.0
+        gm = np.zeros((nc, nt))
+        for s in range(npairs):
+            psi = np.asarray(p.psi_init[s], np.complex128)
+            rho = [np.outer(psi, psi.conj())]
+            for n in range(nt):
+                rho.append(Ps[n] @ rho[n] @ Ps[n].conj().T)
+            for step in steps:
+                for wt, tg in ck[step]:
+                    fm += wt * np.real(np.trace(tg @ rho[step]))
+            if not grad:
+                continue
+            Lam = np.zeros((d, d), np.complex128)
+            for n in range(nt - 1, -1, -1):
+                for wt, tg in ck.get(n + 1, ()):
+                    Lam = Lam + 2.0 * wt * (tg - rho[n + 1])
+                r = rho[n + 1]
+                for j in range(nc):
+                    comm = Hk[j] @ r - r @ Hk[j]
+                    gm[j, n] += -dts[n] * np.real(1j * np.trace(Lam.conj().T @ comm))
+                Lam = Ps[n].conj().T @ Lam @ Ps[n]
+        F += w[m] * fm
+        G += w[m] * gm
+    return F, (G if grad else None)
+
+
 def total_propagator(p: Problem, u: np.ndarray, member: int = 0):
     c = to_c(p)
     ub = controls_cm(u)
@@ -135,4 +207,4 @@ def total_propagator(p: Problem, u: np.ndarray, member: int = 0):
 
 
 __all__ = ["build", "lib", "expm_neg_i", "expm", "expm_frechet", "eigh", "evaluate",
-           "total_propagator", "PR_GRAD_EXACT", "PR_GRAD_REFERENCE"]
+           "tracking_evaluate", "total_propagator", "PR_GRAD_EXACT", "PR_GRAD_REFERENCE"]

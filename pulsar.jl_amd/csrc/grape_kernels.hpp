@@ -112,6 +112,12 @@ struct ProbDev {
   double* Gm;                 // [M][nc*nt]
   double* udef;               // [M] ||U^dag U - I||_max (closed)
   int* smax;                  // [M*nseg] largest scaling exponent per segment
+  // tracking objective: ck_of[k] = index of the checkpoint matrix of slice k
+  // or -1; Ck holds the images of C_c = sum w Herm(rho_target) per checkpoint
+  // slice; Psit the forward state columns per member and slice
+  const int* ck_of;           // [nt]
+  const double2* Ck;          // [n_ck][NB][256]
+  double2* Psit;              // [M][nt][NB][256]
 };
 
 struct WorkItem {
@@ -1215,6 +1221,75 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
     else
       contract<1, V, true>(P, m, k, Mk, LA, x);
     dtk = dtn;
+  }
+}
+
+// ------------------------------------------------------------------ tracking
+// Density-matrix checkpoints (GRAPETracking.jl:162-263) for closed dynamics.
+// The initial states are pure and the evolution unitary, so every rho_s(n) =
+// psi_s(n) psi_s(n)^dag and the kernel carries the state columns Psi (d x
+// n_pairs, zero-padded to whole blocks) instead of d x d density matrices:
+//   forward   Psi_{k+1} = U_k Psi_k, stored per slice;
+//             F_m = sum_c Re Tr(C_c Psi Psi^dag) at each checkpoint slice,
+//             C_c = sum of w Herm(rho_target) over the checkpoints of that slice
+//             (Re Tr(rho_t rho) only sees the Hermitian part of rho_t);
+//   backward  Lambda Psi is the reference's co-state applied to the states:
+//             L += 2 C_c Psi_{k+1} at a checkpoint slice, then
+//             G_m[j,k] = -dt Re(i Tr(Lambda^dag [s A_j, rho])) summed over pairs
+//                      = 2 dt s Im Tr(A_j Psi_{k+1} L^dag)  (contract, M = 2 Psi L^dag),
+//             L <- U_k^dag L.
+// The reference's -2 w rho part of Lambda drops out: Tr(rho [A, rho]) = 0.
+// One workgroup per member walks all slices (the walk is sequential in k);
+// U_k^T comes from the images k_segment stored.
+template <int B, int V>
+__global__ __launch_bounds__(64 * B * B) void k_track(const ProbDev* __restrict__ probs,
+                                                      const int2* __restrict__ members) {
+  const int2 pm = members[blockIdx.x];
+  const ProbDev& P = probs[pm.x];
+  const int m = pm.y;
+  const Ln x = lane_info<B>();
+  double2 *LA, *LB, *zb;
+  double* red;
+  smem_parts<B>(LA, LB, red, zb);
+  constexpr size_t MB = (size_t)B * B * 256;
+  const int nt = P.nt;
+  const double2* Um = P.Ust + (size_t)m * nt * MB;
+  double2* Sm = P.Psit + (size_t)m * nt * MB;
+  const int* ck = P.ck_of;
+  Blk S = load_blk(P.Psi0, x);
+  double f = 0;
+  Blk Vn = load_blk(Um, x);
+  for (int k = 0; k < nt; k++) {
+    const Blk Uk = Vn;
+    if (k + 1 < nt) Vn = load_blk(Um + (size_t)(k + 1) * MB, x);
+    S = mm<B, T, N>(Uk, S, LA, LB, x);  // U_k Psi_k
+    store_blk(Sm + (size_t)k * MB, S, x);
+    const int c = gptr(ck)[k];
+    if (c >= 0) {
+      const Blk Y = mm<B, N, N>(load_blk(P.Ck + (size_t)c * MB, x), S, LA, LB, x);
+#pragma unroll
+      for (int r = 0; r < 4; r++) f += S.re[r] * Y.re[r] + S.im[r] * Y.im[r];
+    }
+  }
+  const double2 fs = block_sum2<B>(make_double2(f, 0.0), red, x);
+  if (threadIdx.x == 0) gptr_w(P.Fm)[m] = fs.x;
+  Blk L = zero_blk();
+  Blk Sn = load_blk(Sm + (size_t)(nt - 1) * MB, x);
+  Vn = load_blk(Um + (size_t)(nt - 1) * MB, x);
+  for (int k = nt - 1; k >= 0; k--) {
+    const Blk Sk = Sn, Vk = Vn;
+    if (k > 0) {
+      Sn = load_blk(Sm + (size_t)(k - 1) * MB, x);
+      Vn = load_blk(Um + (size_t)(k - 1) * MB, x);
+    }
+    const int c = gptr(ck)[k];
+    if (c >= 0) {
+      const Blk Y = mm<B, N, N>(load_blk(P.Ck + (size_t)c * MB, x), Sk, LA, LB, x);
+      L = axpby(1.0, L, 2.0, Y);
+    }
+    const Blk Mk = mm<B, N, C>(Sk, L, LA, LB, x);  // Psi L^dag
+    contract<B, V>(P, m, k, axpby(2.0, Mk, 0.0, Mk), LA, x);
+    L = mm<B, J, N>(Vk, L, LA, LB, x);  // U_k^dag L
   }
 }
 

@@ -289,6 +289,7 @@ struct Plan {
     bool mixed = false;     // FP32-MFMA exp and gradient walk (d <= 16 only)
     bool trace_regs = false;  // mixed walk keeps the trace operand in registers
     bool recompute = false;   // some problem of the group recomputes U_k in the gradient walk
+    bool track = false;       // tracking-objective problems (k_segment, then k_track)
     double2* big_scratch = nullptr;  // d > 64: per work item matrices in HBM
     pr::WorkItem* d_items = nullptr;
     int n_items = 0;
@@ -369,6 +370,20 @@ int validate(const pr_problem& p, int idx) {
     if (p.n_pairs < 1 || !p.psi_init || !p.psi_targ) return fail(PR_ERR_ARG, "problem %d: pair objective needs states", idx);
     if (p.n_pairs > 16 * ((p.dim + 15) / 16))
       return fail(PR_ERR_UNSUPPORTED, "problem %d: n_pairs %d exceeds the padded dimension", idx, p.n_pairs);
+  } else if (p.objective == PR_TRACKING) {
+    if (p.dynamics != PR_CLOSED)
+      return fail(PR_ERR_UNSUPPORTED, "problem %d: the tracking objective needs closed dynamics", idx);
+    if (p.n_pairs < 1 || !p.psi_init) return fail(PR_ERR_ARG, "problem %d: tracking objective needs initial states", idx);
+    if (p.n_pairs > 16 * ((p.dim + 15) / 16))
+      return fail(PR_ERR_UNSUPPORTED, "problem %d: n_pairs %d exceeds the padded dimension", idx, p.n_pairs);
+    // GRAPETracking.jl:146-148 (empty tracking) and :177-178 (step range)
+    if (p.n_track < 1 || !p.track_step || !p.track_weight || !p.track_target)
+      return fail(PR_ERR_ARG, "problem %d: tracking objective needs n_track >= 1 checkpoints", idx);
+    for (int c = 0; c < p.n_track; c++) {
+      if (p.track_step[c] < 1 || p.track_step[c] > p.n_t)
+        return fail(PR_ERR_ARG, "problem %d: TrackingPoint.step=%d out of range [1, %d]", idx, p.track_step[c], p.n_t);
+      if (!std::isfinite(p.track_weight[c])) return fail(PR_ERR_ARG, "problem %d: checkpoint %d weight is not finite", idx, c);
+    }
   } else {
     return fail(PR_ERR_ARG, "problem %d: unknown objective %d", idx, p.objective);
   }
@@ -412,6 +427,9 @@ int set_smem_attr() {
     if ((rc = set_grad_attr<B, 2>(bytes)) || (rc = set_grad_attr<B, 4>(bytes)) || (rc = set_grad_attr<B, 8>(bytes)) ||
         (rc = set_grad_attr<B, 16>(bytes)) || (rc = set_grad_attr<B, 32>(bytes)))
       return rc;
+    for (const void* f : {(const void*)pr::k_track<B, 2>, (const void*)pr::k_track<B, 4>, (const void*)pr::k_track<B, 8>,
+                          (const void*)pr::k_track<B, 16>, (const void*)pr::k_track<B, 32>})
+      HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   }
   return PR_OK;
 }
@@ -528,6 +546,32 @@ int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
   return PR_OK;
 }
 
+// Tracking group: the segment kernel stores every U_k^T image, then one
+// workgroup per member walks the slices forward and back.
+template <int B>
+int launch_track(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hipStream_t st) {
+  const dim3 blk(64 * B * B);
+  int rc;
+  if ((rc = tbegin(pl, st))) return rc;
+  hipLaunchKernelGGL((pr::k_segment<B, true>), dim3(g.n_items), blk, pr::Smem<B>::segment, st, pl.d_probs, g.d_items, d_u);
+  HIPCHK(hipGetLastError());
+  if ((rc = tend(pl, 0, st))) return rc;
+  if ((rc = tbegin(pl, st))) return rc;
+  switch (g.V) {
+#define PR_TCASE(VV)                                                                                         \
+  case VV:                                                                                                   \
+    hipLaunchKernelGGL((pr::k_track<B, VV>), dim3(g.n_members), blk, pr::Smem<B>::base, st, pl.d_probs, g.d_members); \
+    break;
+    PR_TCASE(2) PR_TCASE(4) PR_TCASE(8) PR_TCASE(16) PR_TCASE(32)
+#undef PR_TCASE
+    default:
+      return fail(PR_ERR_LAUNCH, "internal: bad reduction width %d", g.V);
+  }
+  HIPCHK(hipGetLastError());
+  (void)grad;  // the walk produces F and G together
+  return tend(pl, 2, st);
+}
+
 template <int B>
 int set_big_attr() {
   const int bytes = (int)pr::big::Cfg<B>::smem;
@@ -562,6 +606,17 @@ int launch_big(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hip
 int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
   for (const auto& g : pl.groups) {
     int rc = PR_OK;
+    if (g.track) {
+      switch (g.B) {
+        case 1: rc = launch_track<1>(pl, g, d_u, d_G != nullptr, st); break;
+        case 2: rc = launch_track<2>(pl, g, d_u, d_G != nullptr, st); break;
+        case 3: rc = launch_track<3>(pl, g, d_u, d_G != nullptr, st); break;
+        case 4: rc = launch_track<4>(pl, g, d_u, d_G != nullptr, st); break;
+        default: rc = fail(PR_ERR_LAUNCH, "internal: block count %d", g.B);
+      }
+      if (rc) return rc;
+      continue;
+    }
     switch (g.B) {
       case 1: rc = launch_group<1>(pl, g, d_u, d_G != nullptr, st); break;
       case 2: rc = launch_group<2>(pl, g, d_u, d_G != nullptr, st); break;
@@ -629,19 +684,23 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   pl.n_probs = n_probs;
   pl.host_probs.resize(n_probs);
   pl.coff.assign(n_probs + 1, 0);
-  std::vector<std::vector<pr::WorkItem>> items(9);
-  std::vector<std::vector<int2>> members(9);
-  std::vector<int> vmax(9, 1);
-  std::vector<long long> group_members(9, 0);
-  std::vector<int> tr_nnz_max(9, 0);  // most nonzeros of one control, per group
-  std::vector<int> nc_max(9, 0);
-  for (int p = 0; p < n_probs; p++) group_members[(probs[p].dim + 15) / 16] += probs[p].n_members;
+  // groups: index B (1..8) by block count; tracking problems, which run their
+  // own walk kernel, in groups 8 + B (B <= 4)
+  auto group_of = [&](const pr_problem& q) { return (q.objective == PR_TRACKING ? 8 : 0) + (q.dim + 15) / 16; };
+  std::vector<std::vector<pr::WorkItem>> items(13);
+  std::vector<std::vector<int2>> members(13);
+  std::vector<int> vmax(13, 1);
+  std::vector<long long> group_members(13, 0);
+  std::vector<int> tr_nnz_max(13, 0);  // most nonzeros of one control, per group
+  std::vector<int> nc_max(13, 0);
+  for (int p = 0; p < n_probs; p++) group_members[group_of(probs[p])] += probs[p].n_members;
   // the mixed-precision kernels cover single-wavefront (d <= 16) unitary
   // problems under the reference gradient formula; any other problem of the
   // plan, or a d <= 16 group holding a Lindblad problem, runs in FP64
   bool mixed1 = pl.opt.precision == PR_MIXED && pl.opt.grad_mode == PR_GRAD_REFERENCE;
   for (int p = 0; p < n_probs; p++)
     if (probs[p].dim <= 16 && probs[p].dynamics != PR_CLOSED) mixed1 = false;
+  // (tracking problems run FP64 in their own groups, with stored images)
   // U_k images between the passes (store_props): closed-dynamics FP64 problems
   // with d <= 64 can recompute U_k in the gradient walk instead. Auto (-1)
   // recomputes when all images would take more than half of the free HBM.
@@ -652,12 +711,13 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   for (int p = 0; p < n_probs; p++)
     if (probs[p].dim <= 16 && probs[p].dynamics != PR_CLOSED) lindblad1 = true;
   auto can_recompute = [&](const pr_problem& q) {
-    return q.dynamics == PR_CLOSED && q.dim <= 64 && !(q.dim <= 16 && (mixed1 || lindblad1));
+    return q.dynamics == PR_CLOSED && q.dim <= 64 && q.objective != PR_TRACKING && !(q.dim <= 16 && (mixed1 || lindblad1));
   };
   bool recompute = pl.opt.store_props == 0;
   if (pl.opt.store_props < 0) {
     double bytes = 0;
     for (int p = 0; p < n_probs; p++) {
+      if (probs[p].objective == PR_TRACKING) continue;
       const double nb = (double)((probs[p].dim + 15) / 16) * ((probs[p].dim + 15) / 16);
       bytes += (double)probs[p].n_members * probs[p].n_t * nb * 256 * sizeof(double2);
     }
@@ -670,6 +730,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     pr::ProbDev& D = pl.host_probs[p];
     memset(&D, 0, sizeof D);
     const int d = q.dim, B = (d + 15) / 16, NB = B * B, M = q.n_members, nc = q.n_ctrl, nt = q.n_t;
+    const int gi = group_of(q);
+    const bool track = q.objective == PR_TRACKING;
     const size_t MB = (size_t)NB * 256, dd = (size_t)d * d;
     D.d = d; D.B = B; D.nc = nc; D.nt = nt; D.M = M; D.dyn = q.dynamics; D.obj = q.objective;
     D.npairs = q.objective == PR_GATE ? 0 : q.n_pairs;
@@ -686,7 +748,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     } else {
       // (d > 64: workgroups of 16 wavefronts, about 512 of them)
       long long want = B > 4 ? 512 : std::max(1LL, (long long)(8192 / NB));
-      long long S = (want + group_members[B] - 1) / group_members[B];
+      long long S = (want + group_members[gi] - 1) / group_members[gi];
       S = std::min<long long>(std::max<long long>(S, 1), nt);
       L = (int)((nt + S - 1) / S);
       L = std::max(L, std::min(nt, 4));
@@ -733,7 +795,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     double* dd_;
     if ((rc = pl.upload(&dp, img.data(), MB * nh0))) return rc;
     D.H0 = dp;
-    if (B == 1 && mixed1) {
+    if (B == 1 && mixed1 && !track) {
       for (int m = 0; m < nh0; m++) tile_image(H0src + 2 * dd * m, d, 1, false, img.data() + MB * m, true);
       if ((rc = pl.upload(&dp, img.data(), MB * nh0))) return rc;
       D.H0o = dp;
@@ -806,7 +868,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       if ((rc = pl.upload(&jp, tr.idx.data(), tr.idx.size()))) return rc;
       if ((rc = pl.upload(&vp, tr.v.data(), tr.v.size()))) return rc;
       D.tr_ptr = ip; D.tr_idx = jp; D.tr_v = vp;
-      for (int j = 0; j < nc; j++) tr_nnz_max[B] = std::max(tr_nnz_max[B], tr.ptr[j + 1] - tr.ptr[j]);
+      for (int j = 0; j < nc; j++) tr_nnz_max[gi] = std::max(tr_nnz_max[gi], tr.ptr[j + 1] - tr.ptr[j]);
     }
     std::vector<double> sc((size_t)M * nc, 1.0);
     if (q.ctrl_scale) std::copy(q.ctrl_scale, q.ctrl_scale + (size_t)M * nc, sc.begin());
@@ -828,14 +890,43 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       tile_columns(q.psi_init, q.n_pairs, d, B, img.data());
       if ((rc = pl.upload(&dp, img.data(), MB))) return rc;
       D.Psi0 = dp;
-      tile_columns(q.psi_targ, q.n_pairs, d, B, img.data());
-      if ((rc = pl.upload(&dp, img.data(), MB))) return rc;
-      D.PsiT = dp;
+      if (!track) {
+        tile_columns(q.psi_targ, q.n_pairs, d, B, img.data());
+        if ((rc = pl.upload(&dp, img.data(), MB))) return rc;
+        D.PsiT = dp;
+      }
+    }
+    if (track) {
+      // one checkpoint matrix per distinct step: C = sum of w (rho + rho^dag) / 2
+      std::vector<int> ck(nt, -1);
+      std::vector<std::vector<cd>> cm;
+      for (int c = 0; c < q.n_track; c++) {
+        const int k = q.track_step[c] - 1;
+        if (ck[k] < 0) {
+          ck[k] = (int)cm.size();
+          cm.emplace_back(dd, cd(0, 0));
+        }
+        const double w = q.track_weight[c];
+        const double* t = q.track_target + 2 * dd * c;
+        auto& C = cm[ck[k]];
+        for (int col = 0; col < d; col++)
+          for (int row = 0; row < d; row++)
+            C[row + (size_t)col * d] += 0.5 * w * (at(t, d, row, col) + std::conj(at(t, d, col, row)));
+      }
+      std::vector<double2> cimg(MB * cm.size());
+      for (size_t c = 0; c < cm.size(); c++)
+        tile_image(reinterpret_cast<const double*>(cm[c].data()), d, B, false, cimg.data() + MB * c);
+      int* ip;
+      if ((rc = pl.upload(&ip, ck.data(), ck.size()))) return rc;
+      D.ck_of = ip;
+      if ((rc = pl.upload(&dp, cimg.data(), cimg.size()))) return rc;
+      D.Ck = dp;
+      if ((rc = pl.alloc(&D.Psit, (size_t)M * nt * MB))) return rc;
     }
     // scratch
     D.recompute = recompute && can_recompute(q) ? 1 : 0;
     if (!D.recompute && (rc = pl.alloc(&D.Ust, (size_t)M * nt * MB))) return rc;
-    if (B == 1 && mixed1) {
+    if (B == 1 && mixed1 && !track) {
       // FP32 images of small-generator slices; the FP64 images above hold the others
       if ((rc = pl.alloc(&D.Udm, (size_t)M * nt * 256))) return rc;
       if ((rc = pl.alloc(&D.big, (size_t)M * nt))) return rc;
@@ -856,15 +947,15 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     D.coff = pl.coff[p];
     pl.coff[p + 1] = pl.coff[p] + (long long)nc * nt;
     for (int m = 0; m < M; m++) {
-      members[B].push_back(make_int2(p, m));
+      members[gi].push_back(make_int2(p, m));
       for (int s = 0; s < S; s++) {
         pr::WorkItem w;
         w.prob = p; w.member = m; w.seg = s; w.k0 = s * L; w.k1 = std::min(nt, (s + 1) * L);
-        items[B].push_back(w);
+        items[gi].push_back(w);
       }
     }
-    vmax[B] = std::max(vmax[B], next_pow2_cap32(nc));
-    nc_max[B] = std::max(nc_max[B], nc);
+    vmax[gi] = std::max(vmax[gi], next_pow2_cap32(nc));
+    nc_max[gi] = std::max(nc_max[gi], nc);
   }
   pl.n_params = pl.coff[n_probs];
   int rc;
@@ -874,22 +965,24 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   if ((rc = pl.alloc(&pl.d_F, n_probs))) return rc;
   if ((rc = pl.alloc(&pl.d_G, pl.n_params))) return rc;
   if ((rc = pl.alloc(&pl.d_part, (size_t)pr::RED_CHUNKS * pl.n_params))) return rc;
-  for (int B = 1; B <= 8; B++) {
-    if (items[B].empty()) continue;
+  for (int gi = 1; gi <= 12; gi++) {
+    if (items[gi].empty()) continue;
+    const int B = gi > 8 ? gi - 8 : gi;
     Plan::Group g;
     g.B = B;
-    g.V = vmax[B];
-    g.n_items = (int)items[B].size();
-    g.n_members = (int)members[B].size();
+    g.track = gi > 8;
+    g.V = vmax[gi];
+    g.n_items = (int)items[gi].size();
+    g.n_members = (int)members[gi].size();
     for (int p = 0; p < n_probs; p++)
-      if ((probs[p].dim + 15) / 16 == B && probs[p].dynamics == PR_LINDBLAD) g.lindblad = true;
-    g.mixed = B == 1 && mixed1;
+      if (group_of(probs[p]) == gi && probs[p].dynamics == PR_LINDBLAD) g.lindblad = true;
+    g.mixed = B == 1 && mixed1 && !g.track;
     for (int p = 0; p < n_probs; p++)
-      if ((probs[p].dim + 15) / 16 == B && pl.host_probs[p].recompute) g.recompute = true;
+      if (group_of(probs[p]) == gi && pl.host_probs[p].recompute) g.recompute = true;
     // every control fits one lane group of 64 / V lanes with <= TR_REG nonzeros per lane
-    g.trace_regs = B == 1 && vmax[B] >= nc_max[B] && tr_nnz_max[B] <= pr::TR_REG * (64 / vmax[B]);
-    if ((rc = pl.upload(&g.d_items, items[B].data(), items[B].size()))) return rc;
-    if ((rc = pl.upload(&g.d_members, members[B].data(), members[B].size()))) return rc;
+    g.trace_regs = B == 1 && vmax[gi] >= nc_max[gi] && tr_nnz_max[gi] <= pr::TR_REG * (64 / vmax[gi]);
+    if ((rc = pl.upload(&g.d_items, items[gi].data(), items[gi].size()))) return rc;
+    if ((rc = pl.upload(&g.d_members, members[gi].data(), members[gi].size()))) return rc;
     pl.groups.push_back(g);
     if (B > 4) {
       const size_t n = (size_t)std::max(g.n_items, g.n_members) * pr::BIG_SCRATCH * (size_t)(B * B) * 256;
@@ -975,6 +1068,8 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
   for (int p = 0; p < n_probs; p++) {
     if (probs[p].dim > 64 && o.grad_mode == PR_GRAD_EXACT)
       return fail(PR_ERR_UNSUPPORTED, "problem %d: the exact gradient is available for dim <= 64", p);
+    if (probs[p].objective == PR_TRACKING && o.grad_mode == PR_GRAD_EXACT)
+      return fail(PR_ERR_UNSUPPORTED, "problem %d: the tracking objective has the reference gradient only", p);
     int rc = validate(probs[p], p);
     if (rc) return rc;
   }

@@ -6,6 +6,7 @@
                                           OptControl.jl:733-885, GRAPELindblad.jl:83-215
   build_drift_liouvillian, build_control_liouvillian, pure_state_to_vec_rho
                                           src/Physics/Lindblad.jl:89-197
+  TrackingPoint / grape_tracking_kernel   src/Application/MR/GRAPETracking.jl:89-157
 
 Ensemble members are the (drift, power level) pairs in the reference's order
 (`for H in drifts for p in pwr_levels`, GRAPEState.jl:146); every member
@@ -21,7 +22,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import native
-from .problem import PR_CLOSED, PR_GRAD_REFERENCE, PR_LINDBLAD, PR_PAIRS_REAL, PR_PAIRS_SQUARE, Problem
+from .problem import PR_CLOSED, PR_GRAD_REFERENCE, PR_LINDBLAD, PR_PAIRS_REAL, PR_PAIRS_SQUARE, PR_TRACKING, Problem
 
 
 def build_drift_liouvillian(H_drift, jump_ops=(), decay_rates=()):
@@ -68,8 +69,19 @@ def _fid_code(fidelity: str) -> int:
 
 
 @dataclass
+class TrackingPoint:
+    """TrackingPoint (GRAPETracking.jl:89-97): the density matrix after slice
+    `step` (1-based, 1..n_t) is compared with `target_dm` at weight `weight`."""
+
+    step: int
+    target_dm: np.ndarray
+    weight: float
+
+
+@dataclass
 class MRControl:
-    """The fields of MRControl (OptControl.jl:137-259) that the state kernel reads."""
+    """The fields of MRControl (OptControl.jl:137-259) that the state and
+    tracking kernels read."""
 
     drifts: List[np.ndarray]
     operators: List[np.ndarray]
@@ -80,6 +92,22 @@ class MRControl:
     fidelity: str = "square"
     backend: str = "rocm"
     precision: str = "f64"
+    tracking: List[TrackingPoint] = field(default_factory=list)
+
+    def tracking_problem(self) -> Problem:
+        """The tracking objective over the same ensemble: initial states
+        rho_init (pure), checkpoints from `tracking` (GRAPETracking.jl:162-186)."""
+        if not self.tracking:
+            raise ValueError("grape_tracking_kernel: ctrl.tracking is empty. "
+                             "Add TrackingPoint entries or call grape_state_kernel instead.")
+        base = self.problem()
+        d = base.dim
+        base.objective = PR_TRACKING
+        base.psi_targ = None
+        base.track_step = np.array([int(t.step) for t in self.tracking], dtype=np.int32)
+        base.track_weight = np.array([float(t.weight) for t in self.tracking], dtype=np.float64)
+        base.track_target = np.stack([np.asarray(t.target_dm, np.complex128).reshape(d, d) for t in self.tracking])
+        return base
 
     def problem(self) -> Problem:
         drifts = [np.asarray(h, np.complex128) for h in self.drifts]
@@ -131,10 +159,11 @@ class LindbladMRControl:
 
 
 class KernelPlan:
-    """A reusable device plan for repeated kernel calls with one control context."""
+    """A reusable device plan for repeated kernel calls with one control context
+    (tracking=True: the tracking objective of an MRControl)."""
 
-    def __init__(self, ctrl, grad_mode: int = PR_GRAD_REFERENCE):
-        self.problem = ctrl.problem()
+    def __init__(self, ctrl, grad_mode: int = PR_GRAD_REFERENCE, tracking: bool = False):
+        self.problem = ctrl.tracking_problem() if tracking else ctrl.problem()
         self.plan = native.Plan([self.problem], grad_mode=grad_mode)
 
     def __call__(self, waveform: np.ndarray):
@@ -158,6 +187,16 @@ def grape_state_kernel(waveform: np.ndarray, ctrl: MRControl):
 def grape_lindblad_kernel(waveform: np.ndarray, ctrl: LindbladMRControl):
     """(fidelity, grad[n_ctrl x n_t]) of GRAPELindblad.jl:83-92 on the device."""
     kp = KernelPlan(ctrl)
+    try:
+        return kp(waveform)
+    finally:
+        kp.close()
+
+
+def grape_tracking_kernel(waveform: np.ndarray, ctrl: MRControl):
+    """(fidelity, grad[n_ctrl x n_t]) of GRAPETracking.jl:145-157 on the device
+    (ctrl.tracking must be non-empty, as there)."""
+    kp = KernelPlan(ctrl, tracking=True)
     try:
         return kp(waveform)
     finally:

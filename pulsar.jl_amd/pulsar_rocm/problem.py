@@ -18,7 +18,7 @@ from typing import Optional
 import numpy as np
 
 PR_CLOSED, PR_LINDBLAD = 0, 1
-PR_GATE, PR_PAIRS_SQUARE, PR_PAIRS_REAL = 0, 1, 2
+PR_GATE, PR_PAIRS_SQUARE, PR_PAIRS_REAL, PR_TRACKING = 0, 1, 2, 3
 PR_FP64, PR_MIXED = 0, 1
 PR_GRAD_REFERENCE, PR_GRAD_EXACT = 0, 1
 
@@ -62,6 +62,10 @@ class Problem:
     U_target: Optional[np.ndarray] = None
     psi_init: Optional[np.ndarray] = None
     psi_targ: Optional[np.ndarray] = None
+    # PR_TRACKING checkpoints: 1-based steps, weights, (n_track, d, d) density matrices
+    track_step: Optional[np.ndarray] = None
+    track_weight: Optional[np.ndarray] = None
+    track_target: Optional[np.ndarray] = None
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -71,6 +75,10 @@ class Problem:
     @property
     def n_pairs(self) -> int:
         return 0 if self.psi_init is None else int(np.asarray(self.psi_init).shape[0])
+
+    @property
+    def n_track(self) -> int:
+        return 0 if self.track_step is None else int(np.asarray(self.track_step).shape[0])
 
     def validate_shapes(self) -> None:
         d = self.dim
@@ -88,6 +96,13 @@ class Problem:
         if self.objective == PR_GATE:
             if self.U_target is None or np.asarray(self.U_target).shape != (d, d):
                 raise ValueError("gate objective needs U_target (dim x dim)")
+        elif self.objective == PR_TRACKING:
+            if self.psi_init is None or np.asarray(self.psi_init).ndim != 2 or np.asarray(self.psi_init).shape[1] != d:
+                raise ValueError("the tracking objective needs psi_init [n_pairs, dim]")
+            n = self.n_track
+            if (self.track_weight is None or self.track_target is None or np.asarray(self.track_weight).shape != (n,)
+                    or np.asarray(self.track_target).shape != (n, d, d)):
+                raise ValueError("tracking needs track_step [n], track_weight [n] and track_target [n, dim, dim]")
         else:
             if self.psi_init is None or self.psi_targ is None:
                 raise ValueError("pair objectives need psi_init and psi_targ")
@@ -116,6 +131,10 @@ class CProblem(ctypes.Structure):
         ("n_pairs", ctypes.c_int),
         ("psi_init", ctypes.POINTER(ctypes.c_double)),
         ("psi_targ", ctypes.POINTER(ctypes.c_double)),
+        ("n_track", ctypes.c_int),
+        ("track_step", ctypes.POINTER(ctypes.c_int)),
+        ("track_weight", ctypes.POINTER(ctypes.c_double)),
+        ("track_target", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -157,6 +176,13 @@ def to_c(p: Problem) -> CProblem:
     c.n_pairs = p.n_pairs
     c.psi_init = _dptr(None if p.psi_init is None else np.asarray(p.psi_init, np.complex128), keep)
     c.psi_targ = _dptr(None if p.psi_targ is None else np.asarray(p.psi_targ, np.complex128), keep)
+    c.n_track = p.n_track
+    if p.track_step is not None:
+        steps = np.ascontiguousarray(p.track_step, dtype=np.int32)
+        keep.append(steps)
+        c.track_step = steps.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    c.track_weight = _dptr(None if p.track_weight is None else np.asarray(p.track_weight, np.float64), keep)
+    c.track_target = _dptr(None if p.track_target is None else cmat(p.track_target), keep)
     p._keep = keep
     return c
 

@@ -12,6 +12,9 @@ benchmark configurations:
                        Haar-unitary channel target on the 64 vec(E_ij) pairs
   uneven_batch         independent problems, dims {2..64} with p ~ 1/d
   spin_chain_128       d=128 Heisenberg chain, 14 drives, 500 slices, 64 members
+  four_qubit_tracking  d=16 tracking GRAPE (density-matrix checkpoints,
+                       GRAPETracking.jl): the four_qubit_ensemble system and
+                       ensemble, 4 initial states, 2000 slices, 21 checkpoints
 
 The Haar-random target follows random_unitary_problem
 (src/Utilities/ProblemLibrary.jl:742-748): QR of a complex Gaussian matrix,
@@ -24,7 +27,7 @@ from __future__ import annotations
 import numpy as np
 
 from .problem import (PR_CLOSED, PR_GATE, PR_LINDBLAD, PR_PAIRS_REAL, PR_PAIRS_SQUARE,
-                      Problem)
+                      PR_TRACKING, Problem)
 
 SX = np.array([[0, 1], [1, 0]], dtype=np.complex128)
 SY = np.array([[0, -1j], [1j, 0]], dtype=np.complex128)
@@ -225,6 +228,50 @@ def spin_chain_128(seed: int = 13, n_t: int = 500, n_members: int = 64, target_s
     return p, u
 
 
+def random_state(d: int, rng: np.random.Generator) -> np.ndarray:
+    v = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+    return v / np.linalg.norm(v)
+
+
+def random_density(d: int, rng: np.random.Generator, rank: int = 2) -> np.ndarray:
+    """A mixed state: sum of `rank` random pure states with random weights."""
+    p = rng.uniform(0.2, 1.0, rank)
+    p /= p.sum()
+    out = np.zeros((d, d), np.complex128)
+    for pi in p:
+        v = random_state(d, rng)
+        out += pi * np.outer(v, v.conj())
+    return out
+
+
+def tracking_checkpoints(d: int, n_t: int, n_checkpoints: int, rng: np.random.Generator):
+    """Checkpoints every n_t / n_checkpoints slices alternating between two
+    pure targets (the oscillation example of GRAPETracking.jl:69-80), plus a
+    mixed target at the last slice; weights sum to 1."""
+    a, b = random_state(d, rng), random_state(d, rng)
+    per = max(1, n_t // n_checkpoints)
+    steps = [min(n_t, k * per) for k in range(1, n_checkpoints + 1)]
+    targets = [np.outer(v, v.conj()) for v in (b if k % 2 else a for k in range(1, n_checkpoints + 1))]
+    steps.append(n_t)
+    targets.append(random_density(d, rng))
+    w = np.full(len(steps), 1.0 / len(steps))
+    return np.array(steps, np.int32), w, np.stack(targets)
+
+
+def four_qubit_tracking(seed: int = 23, n_t: int = 2000, n_det: int = 16, n_amp: int = 16, n_pairs: int = 4,
+                        n_checkpoints: int = 20):
+    """d=16 tracking GRAPE: the four_qubit_ensemble drifts, drives and
+    detuning x amplitude ensemble; n_pairs random pure initial states."""
+    p, u = four_qubit_ensemble(n_t=n_t, n_det=n_det, n_amp=n_amp)
+    rng = np.random.default_rng(seed)
+    steps, w, targets = tracking_checkpoints(16, n_t, n_checkpoints, rng)
+    p.objective = PR_TRACKING
+    p.U_target = None
+    p.psi_init = np.stack([random_state(16, rng) for _ in range(n_pairs)])
+    p.track_step, p.track_weight, p.track_target = steps, w, targets
+    return p, u
+
+
 def shrink(p: Problem, u: np.ndarray, n_members: int | None = None, n_t: int | None = None):
     """The same workload with fewer members and/or slices (for bounded checks)."""
     import copy
@@ -243,5 +290,10 @@ def shrink(p: Problem, u: np.ndarray, n_members: int | None = None, n_t: int | N
         uu = u[:, :n_t]
         if p.dt_slices is not None:
             q.dt_slices = np.asarray(p.dt_slices)[:n_t]
+        if p.track_step is not None:
+            keep = np.asarray(p.track_step) <= n_t
+            q.track_step = np.asarray(p.track_step)[keep]
+            q.track_weight = np.asarray(p.track_weight)[keep]
+            q.track_target = np.asarray(p.track_target)[keep]
     q._keep = []
     return q, np.ascontiguousarray(uu)

@@ -1,0 +1,139 @@
+"""Tracking objective on the device (k_segment images + k_track walk) against
+the tracking oracle (GRAPETracking.jl:162-263 restated with density matrices).
+
+Both sides are FP64; the device propagates state columns with Taylor exps, the
+oracle density matrices with eigen-exps: |dF| <= 1e-12 and
+max|dG| / max|G| <= 1e-10, as for the other objectives.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from pulsar_rocm import mr, native, workloads as W
+from pulsar_rocm.problem import PR_GATE, PR_MIXED, PR_TRACKING, Problem
+
+pytestmark = pytest.mark.gpu
+
+TOL_F = 1e-12
+TOL_G = 1e-10
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    native.init(0)
+
+
+def tracking_case(d, M, nc, nt, npairs, n_ck, seed, per_slice_dt=False, norm=3.0):
+    rng = np.random.default_rng(seed)
+    H0 = np.stack([W.random_hermitian(d, rng, norm) for _ in range(M)])
+    Hc = np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(nc)])
+    scale = rng.uniform(0.8, 1.2, (M, 1)) * np.ones((1, nc))
+    psi = np.stack([W.random_state(d, rng) for _ in range(npairs)])
+    steps, w, targets = W.tracking_checkpoints(d, nt, n_ck, rng)
+    # a checkpoint at the first slice and one sharing a step with another
+    steps = np.concatenate([steps, [1, steps[0]]]).astype(np.int32)
+    w = np.concatenate([w, [0.05, 0.07]])
+    targets = np.concatenate([targets, [W.random_density(d, rng), W.random_density(d, rng, rank=1)]])
+    dts = rng.uniform(0.05, 0.15, nt) if per_slice_dt else None
+    p = Problem(dim=d, n_ctrl=nc, n_t=nt, H0=H0, Hc=Hc, dt=0.1, dt_slices=dts, objective=PR_TRACKING,
+                n_members=M, ctrl_scale=scale, psi_init=psi, track_step=steps, track_weight=w,
+                track_target=targets)
+    u = rng.uniform(-2, 2, (nc, nt))
+    return p, u
+
+
+def run(p, u, **kw):
+    plan = native.Plan([p], **kw)
+    try:
+        F, G = plan.eval([u])
+    finally:
+        plan.close()
+    return F[0], G[0]
+
+
+def check(p, u, **kw):
+    print(f"tracking case d={p.dim} M={p.n_members} nc={p.n_ctrl} n_t={p.n_t} pairs={p.n_pairs} "
+          f"checkpoints={p.n_track} {kw}", flush=True)
+    F0, G0 = O.tracking_evaluate(p, u)
+    F, G = run(p, u, **kw)
+    assert abs(F - F0) <= TOL_F, (F, F0)
+    assert np.abs(G - G0).max() <= TOL_G * np.abs(G0).max(), np.abs(G - G0).max() / np.abs(G0).max()
+    return F, G
+
+
+@pytest.mark.parametrize("d,M,nc,nt,npairs", [
+    (2, 4, 2, 40, 1), (4, 3, 3, 30, 2), (16, 3, 8, 40, 4), (7, 2, 3, 25, 3),
+    (24, 2, 4, 20, 5), (40, 2, 3, 12, 6), (64, 1, 2, 8, 3)])
+def test_tracking_matches_oracle(d, M, nc, nt, npairs):
+    p, u = tracking_case(d, M, nc, nt, npairs, n_ck=5, seed=d + nt)
+    check(p, u)
+
+
+def test_tracking_per_slice_dt_and_full_pair_count():
+    p, u = tracking_case(16, 2, 4, 30, 16, n_ck=6, seed=3, per_slice_dt=True)
+    check(p, u)
+
+
+def test_tracking_every_slice_a_checkpoint():
+    p, u = tracking_case(8, 2, 3, 15, 2, n_ck=15, seed=4)
+    check(p, u)
+
+
+def test_tracking_ignores_precision_and_store_props_options():
+    # tracking runs FP64 with stored images whatever the plan options say
+    p, u = tracking_case(16, 2, 4, 24, 3, n_ck=4, seed=5)
+    F, G = run(p, u)
+    for kw in ({"precision": PR_MIXED}, {"store_props": 0}, {"segment": 5}):
+        F2, G2 = run(p, u, **kw)
+        assert abs(F2 - F) <= 1e-14 and np.abs(G2 - G).max() <= 1e-13 * np.abs(G).max(), kw
+
+
+def test_tracking_beside_a_gate_problem_in_one_plan():
+    pt, ut = tracking_case(16, 2, 4, 24, 3, n_ck=4, seed=6)
+    pg, ug = W.shrink(*W.four_qubit_ensemble(n_t=24, n_det=2, n_amp=1))
+    plan = native.Plan([pg, pt])
+    try:
+        F, G = plan.eval([ug, ut])
+    finally:
+        plan.close()
+    Fg, Gg = O.evaluate(pg, ug)
+    Ft, Gt = O.tracking_evaluate(pt, ut)
+    assert abs(F[0] - Fg) <= TOL_F and np.abs(G[0] - Gg).max() <= TOL_G * np.abs(Gg).max()
+    assert abs(F[1] - Ft) <= TOL_F and np.abs(G[1] - Gt).max() <= TOL_G * np.abs(Gt).max()
+
+
+def test_tracking_fidelity_only_evaluation_and_replays():
+    p, u = tracking_case(4, 2, 2, 20, 2, n_ck=4, seed=7)
+    plan = native.Plan([p])
+    try:
+        F, G = plan.eval([u])
+        F1, _ = plan.eval([u], grad=False)
+        F2, G2 = plan.eval([u])
+    finally:
+        plan.close()
+    assert F1[0] == F[0] and F2[0] == F[0] and np.array_equal(G2[0], G[0])
+
+
+def test_mr_tracking_kernel_single_spin_oscillation():
+    # the GRAPETracking.jl:69-80 example: one spin driven between +z and -z,
+    # checkpoints every 10 slices, two power levels and two offsets
+    Ix = W.SX / 2
+    Iy = W.SY / 2
+    Iz = W.SZ / 2
+    up, dn = np.array([1, 0], complex), np.array([0, 1], complex)
+    cps = [mr.TrackingPoint(10 * k, np.outer(dn, dn) if k % 2 else np.outer(up, up), 0.1) for k in range(1, 11)]
+    ctrl = mr.MRControl(drifts=[2 * np.pi * 50 * Iz, -2 * np.pi * 80 * Iz], operators=[Ix, Iy],
+                        rho_init=[up], rho_targ=[dn], pulse_dt=np.full(100, 2e-6), pwr_levels=[0.9, 1.1],
+                        tracking=cps)
+    w = np.random.default_rng(8).uniform(-2 * np.pi * 5e4, 2 * np.pi * 5e4, (2, 100))
+    F, G = mr.grape_tracking_kernel(w, ctrl)
+    F0, G0 = O.tracking_evaluate(ctrl.tracking_problem(), w)
+    assert abs(F - F0) <= TOL_F and np.abs(G - G0).max() <= TOL_G * np.abs(G0).max()
+    with pytest.raises(ValueError, match="tracking is empty"):
+        mr.grape_tracking_kernel(w, mr.MRControl(drifts=ctrl.drifts, operators=ctrl.operators, rho_init=[up],
+                                                 rho_targ=[dn], pulse_dt=ctrl.pulse_dt))
+
+
+def test_four_qubit_tracking_workload_members_sample():
+    p, u = W.shrink(*W.four_qubit_tracking(n_t=200), n_members=3)
+    check(p, u)
