@@ -173,6 +173,13 @@ def other_workload(args):
         probs, us = [p], [u]
         desc = "spin_chain_128: d=128 Heisenberg chain, 64 members, 14 controls, 500 slices, gate target"
         flops = (72 * 128 ** 3 + 16 * p.n_ctrl * 128 ** 2) * p.n_members * p.n_t
+    elif args.workload == "four_qubit_tracking":
+        p, u = workloads.four_qubit_tracking()
+        probs, us = [p], [u]
+        desc = (f"four_qubit_tracking: d=16, 256 members, 8 controls, 2000 slices, {p.n_pairs} initial states, "
+                f"{p.n_track} density-matrix checkpoints")
+        # exps + segment chain (5 complex products) and the walk (3 per slice)
+        flops = (64 * 16 ** 3 + 16 * p.n_ctrl * 16 ** 2) * p.n_members * p.n_t
     else:
         p, u = workloads.single_qubit_state()
         probs, us = [p], [u]
@@ -203,10 +210,23 @@ def other_workload(args):
     elapsed = time.perf_counter() - t0
     plan.close()
     rate = args.steps / elapsed
-    print(json.dumps({"workload": args.workload, "description": desc, "evals_per_s": round(rate, 3),
-                      "ms_per_eval": round(1e3 * elapsed / args.steps, 4), "steps": args.steps,
-                      "algorithmic_tflops": round(flops * rate / 1e12, 3),
-                      "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}}))
+    line = {"workload": args.workload, "description": desc, "evals_per_s": round(rate, 3),
+            "ms_per_eval": round(1e3 * elapsed / args.steps, 4), "steps": args.steps,
+            "algorithmic_tflops": round(flops * rate / 1e12, 3),
+            "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}}
+    if args.workload == "four_qubit_tracking" and args.cpu_sample > 0:
+        # the tracking oracle (density matrices, one thread) on a member sample
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        n = min(args.cpu_sample, 2)
+        q, uu = workloads.shrink(p, u, n_members=n)
+        t0 = time.perf_counter()
+        O.tracking_evaluate(q, uu)
+        cpu_s = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(n / (cpu_s * p.n_members), 6), "unit": "evals/s", "cores": 1,
+                                "kind": "port", "sample": f"{n} of {p.n_members} members, all {p.n_t} slices, "
+                                f"{cpu_s:.1f} s; rate scaled to the full ensemble"}
+    print(json.dumps(line))
 
 
 def timed_form(p, u, prec, args):
@@ -249,7 +269,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--workload", default="four_qubit_ensemble",
                     choices=["four_qubit_ensemble", "three_qubit_lindblad", "uneven_batch", "single_qubit_state",
-                             "spin_chain_128"],
+                             "spin_chain_128", "four_qubit_tracking"],
                     help="the headline metric is four_qubit_ensemble; the others print timing lines of their own")
     ap.add_argument("--cpu-sample", type=int, default=32, help="members timed for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))

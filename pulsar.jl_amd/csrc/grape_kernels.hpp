@@ -114,10 +114,14 @@ struct ProbDev {
   int* smax;                  // [M*nseg] largest scaling exponent per segment
   // tracking objective: ck_of[k] = index of the checkpoint matrix of slice k
   // or -1; Ck holds the images of C_c = sum w Herm(rho_target) per checkpoint
-  // slice; Psit the forward state columns per member and slice
+  // slice. Per member and segment: Zseg the checkpoint operator of the segment
+  // (then its local co-state), Pend the states after its last slice, Lend the
+  // co-state arriving at its end from later segments.
   const int* ck_of;           // [nt]
   const double2* Ck;          // [n_ck][NB][256]
-  double2* Psit;              // [M][nt][NB][256]
+  double2* Zseg;              // [M][nseg][NB][256]
+  double2* Pend;              // [M][nseg][NB][256]
+  double2* Lend;              // [M][nseg][NB][256]
 };
 
 struct WorkItem {
@@ -552,7 +556,20 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
 // generators), so the d <= 16 exp takes the register-only form unconditionally.
 // FIX (mixed plans): only work items that k_segment_mx flagged in P.seg_big
 // run, and they mark each slice as FP64 in P.big.
-template <int B, bool CLOSED, bool FIX = false>
+// TRK (tracking objective): also accumulates the segment's checkpoint operator
+// Z_s = sum over its checkpoint slices k of R_k^dag C R_k, R_k = U_k ... U_{k0}.
+template <int B>
+__device__ inline void track_accumulate(const ProbDev& P, int k, const Blk& R, Blk& Z, double2* LA, double2* LB,
+                                        const Ln& x) {
+  const int c = gptr(P.ck_of)[k];
+  if (c < 0) return;
+  const Blk Y = mm<B, N, N>(load_blk(P.Ck + (size_t)c * B * B * 256, x), R, LA, LB, x);  // C R
+  const Blk W = mm<B, C, N>(R, Y, LA, LB, x);                                          // R^dag C R
+  Z.re += W.re;
+  Z.im += W.im;
+}
+
+template <int B, bool CLOSED, bool FIX = false, bool TRK = false>
 __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B == 1 ? PR_SEGMENT_WAVES : 0))) void k_segment(const ProbDev* __restrict__ probs,
                                                          const WorkItem* __restrict__ items,
                                                          const double* __restrict__ u) {
@@ -570,6 +587,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
   // the running segment product stays in registers (the B operand of the
   // chain product U_k T for B == 1; staged through LB for wider blocks)
   Blk Tseg = eye_blk(x);
+  Blk Z = zero_blk();
   int smax = 0;
   // B == 1 with closed dynamics: X^T is assembled from the transposed tables,
   // so exp gives V = U^T in the register layout. The chain product U T takes
@@ -598,6 +616,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
       if constexpr (FIX) {
         if (threadIdx.x == 0) P.big[(size_t)m * P.nt + k] = 1;
       }
+      if constexpr (TRK) track_accumulate<1>(P, k, acc, Z, LA, LB, x);
       Tseg = acc;
       continue;
     }
@@ -629,9 +648,11 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
       if (threadIdx.x == 0) P.big[(size_t)m * P.nt + k] = 1;
     }
     bsync<B>();
+    if constexpr (TRK) track_accumulate<B>(P, k, acc, Z, LA, LB, x);
     Tseg = acc;
   }
   store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * NB * 256, Tseg, x);
+  if constexpr (TRK) store_blk(P.Zseg + ((size_t)m * P.nseg + it.seg) * NB * 256, Z, x);
   if (threadIdx.x == 0) P.smax[(size_t)m * P.nseg + it.seg] = smax;
 }
 
@@ -1227,23 +1248,32 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
 // ------------------------------------------------------------------ tracking
 // Density-matrix checkpoints (GRAPETracking.jl:162-263) for closed dynamics.
 // The initial states are pure and the evolution unitary, so every rho_s(n) =
-// psi_s(n) psi_s(n)^dag and the kernel carries the state columns Psi (d x
-// n_pairs, zero-padded to whole blocks) instead of d x d density matrices:
-//   forward   Psi_{k+1} = U_k Psi_k, stored per slice;
-//             F_m = sum_c Re Tr(C_c Psi Psi^dag) at each checkpoint slice,
-//             C_c = sum of w Herm(rho_target) over the checkpoints of that slice
-//             (Re Tr(rho_t rho) only sees the Hermitian part of rho_t);
-//   backward  Lambda Psi is the reference's co-state applied to the states:
-//             L += 2 C_c Psi_{k+1} at a checkpoint slice, then
-//             G_m[j,k] = -dt Re(i Tr(Lambda^dag [s A_j, rho])) summed over pairs
-//                      = 2 dt s Im Tr(A_j Psi_{k+1} L^dag)  (contract, M = 2 Psi L^dag),
-//             L <- U_k^dag L.
-// The reference's -2 w rho part of Lambda drops out: Tr(rho [A, rho]) = 0.
-// One workgroup per member walks all slices (the walk is sequential in k);
-// U_k^T comes from the images k_segment stored.
-template <int B, int V>
-__global__ __launch_bounds__(64 * B * B) void k_track(const ProbDev* __restrict__ probs,
-                                                      const int2* __restrict__ members) {
+// psi_s(n) psi_s(n)^dag: the kernels carry the state columns Psi (d x
+// n_pairs, zero-padded to whole blocks) instead of d x d density matrices.
+// With C_c = sum of w Herm(rho_target) over the checkpoints of slice c
+// (Re Tr(rho_t rho) only sees the Hermitian part of rho_t):
+//   F_m = sum_c Re Tr(C_c Psi_{c+1} Psi_{c+1}^dag)
+//   G_m[j,k] = -dt Re(i Tr(Lambda^dag [s A_j, rho_{k+1}])) summed over pairs
+//            = 2 dt s Im Tr(A_j Psi_{k+1} L_{k+1}^dag),
+// where L = Lambda Psi is the reference's co-state applied to the states:
+// L += 2 C_c Psi_{c+1} at a checkpoint slice, L <- U_k^dag L before slice k
+// (the reference's -2 w rho part of Lambda drops out: Tr(rho [A, rho]) = 0).
+//
+// Time-parallel over the same segments as the other objectives:
+//   k_segment<.., TRK>  per segment s: T_s and Z_s = sum_k R_k^dag C_k R_k
+//                       (so Psi_start^dag Z_s Psi_start sums its checkpoints);
+//   k_track_scan        per member, over the segments in order: Psi at each
+//                       segment end (T_s Psi), F_m = sum_s Re Tr(Z_s Psi Psi^dag),
+//                       the local co-states 2 Z_s Psi_start(s) and, backwards,
+//                       the co-state at each segment end:
+//                       Lend(s-1) = T_s^dag Lend(s) + 2 Z_s Psi_start(s);
+//   k_track_walk        per segment, from (Psi, L) at its end back to its
+//                       start: L += 2 C Psi at checkpoints, contract
+//                       M = 2 Psi L^dag with every control, then
+//                       L <- U_k^dag L and Psi <- U_k^dag Psi.
+template <int B>
+__global__ __launch_bounds__(64 * B * B) void k_track_scan(const ProbDev* __restrict__ probs,
+                                                           const int2* __restrict__ members) {
   const int2 pm = members[blockIdx.x];
   const ProbDev& P = probs[pm.x];
   const int m = pm.y;
@@ -1252,44 +1282,57 @@ __global__ __launch_bounds__(64 * B * B) void k_track(const ProbDev* __restrict_
   double* red;
   smem_parts<B>(LA, LB, red, zb);
   constexpr size_t MB = (size_t)B * B * 256;
-  const int nt = P.nt;
-  const double2* Um = P.Ust + (size_t)m * nt * MB;
-  double2* Sm = P.Psit + (size_t)m * nt * MB;
-  const int* ck = P.ck_of;
+  const size_t base = (size_t)m * P.nseg * MB;
   Blk S = load_blk(P.Psi0, x);
   double f = 0;
-  Blk Vn = load_blk(Um, x);
-  for (int k = 0; k < nt; k++) {
-    const Blk Uk = Vn;
-    if (k + 1 < nt) Vn = load_blk(Um + (size_t)(k + 1) * MB, x);
-    S = mm<B, T, N>(Uk, S, LA, LB, x);  // U_k Psi_k
-    store_blk(Sm + (size_t)k * MB, S, x);
-    const int c = gptr(ck)[k];
-    if (c >= 0) {
-      const Blk Y = mm<B, N, N>(load_blk(P.Ck + (size_t)c * MB, x), S, LA, LB, x);
+  for (int s = 0; s < P.nseg; s++) {
+    const Blk Y = mm<B, N, N>(load_blk(P.Zseg + base + s * MB, x), S, LA, LB, x);  // Z_s Psi
 #pragma unroll
-      for (int r = 0; r < 4; r++) f += S.re[r] * Y.re[r] + S.im[r] * Y.im[r];
-    }
+    for (int r = 0; r < 4; r++) f += S.re[r] * Y.re[r] + S.im[r] * Y.im[r];
+    store_blk(P.Zseg + base + s * MB, axpby(2.0, Y, 0.0, Y), x);  // local co-state at the segment start
+    S = mm<B, N, N>(load_blk(P.Tseg + base + s * MB, x), S, LA, LB, x);
+    store_blk(P.Pend + base + s * MB, S, x);
   }
   const double2 fs = block_sum2<B>(make_double2(f, 0.0), red, x);
   if (threadIdx.x == 0) gptr_w(P.Fm)[m] = fs.x;
   Blk L = zero_blk();
-  Blk Sn = load_blk(Sm + (size_t)(nt - 1) * MB, x);
-  Vn = load_blk(Um + (size_t)(nt - 1) * MB, x);
-  for (int k = nt - 1; k >= 0; k--) {
-    const Blk Sk = Sn, Vk = Vn;
-    if (k > 0) {
-      Sn = load_blk(Sm + (size_t)(k - 1) * MB, x);
-      Vn = load_blk(Um + (size_t)(k - 1) * MB, x);
-    }
-    const int c = gptr(ck)[k];
+  for (int s = P.nseg - 1; s >= 0; s--) {
+    store_blk(P.Lend + base + s * MB, L, x);
+    const Blk TL = mm<B, C, N>(load_blk(P.Tseg + base + s * MB, x), L, LA, LB, x);  // T_s^dag L
+    const Blk Ll = load_blk(P.Zseg + base + s * MB, x);
+    L.re = TL.re + Ll.re;
+    L.im = TL.im + Ll.im;
+  }
+}
+
+template <int B, int V>
+__global__ __launch_bounds__(64 * B * B) void k_track_walk(const ProbDev* __restrict__ probs,
+                                                           const WorkItem* __restrict__ items) {
+  const WorkItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const int m = it.member;
+  const Ln x = lane_info<B>();
+  double2 *LA, *LB, *zb;
+  double* red;
+  smem_parts<B>(LA, LB, red, zb);
+  constexpr size_t MB = (size_t)B * B * 256;
+  const size_t sb = ((size_t)m * P.nseg + it.seg) * MB;
+  const double2* Um = P.Ust + (size_t)m * P.nt * MB;
+  Blk S = load_blk(P.Pend + sb, x);
+  Blk L = load_blk(P.Lend + sb, x);
+  Blk Vn = load_blk(Um + (size_t)(it.k1 - 1) * MB, x);
+  for (int k = it.k1 - 1; k >= it.k0; k--) {
+    const Blk Uk = Vn;
+    if (k > it.k0) Vn = load_blk(Um + (size_t)(k - 1) * MB, x);
+    const int c = gptr(P.ck_of)[k];
     if (c >= 0) {
-      const Blk Y = mm<B, N, N>(load_blk(P.Ck + (size_t)c * MB, x), Sk, LA, LB, x);
+      const Blk Y = mm<B, N, N>(load_blk(P.Ck + (size_t)c * MB, x), S, LA, LB, x);
       L = axpby(1.0, L, 2.0, Y);
     }
-    const Blk Mk = mm<B, N, C>(Sk, L, LA, LB, x);  // Psi L^dag
+    const Blk Mk = mm<B, N, C>(S, L, LA, LB, x);  // Psi L^dag
     contract<B, V>(P, m, k, axpby(2.0, Mk, 0.0, Mk), LA, x);
-    L = mm<B, J, N>(Vk, L, LA, LB, x);  // U_k^dag L
+    L = mm<B, J, N>(Uk, L, LA, LB, x);  // U_k^dag L
+    S = mm<B, J, N>(Uk, S, LA, LB, x);  // U_k^dag Psi
   }
 }
 

@@ -427,8 +427,10 @@ int set_smem_attr() {
     if ((rc = set_grad_attr<B, 2>(bytes)) || (rc = set_grad_attr<B, 4>(bytes)) || (rc = set_grad_attr<B, 8>(bytes)) ||
         (rc = set_grad_attr<B, 16>(bytes)) || (rc = set_grad_attr<B, 32>(bytes)))
       return rc;
-    for (const void* f : {(const void*)pr::k_track<B, 2>, (const void*)pr::k_track<B, 4>, (const void*)pr::k_track<B, 8>,
-                          (const void*)pr::k_track<B, 16>, (const void*)pr::k_track<B, 32>})
+    for (const void* f : {(const void*)pr::k_segment<B, true, false, true>, (const void*)pr::k_track_scan<B>,
+                          (const void*)pr::k_track_walk<B, 2>, (const void*)pr::k_track_walk<B, 4>,
+                          (const void*)pr::k_track_walk<B, 8>, (const void*)pr::k_track_walk<B, 16>,
+                          (const void*)pr::k_track_walk<B, 32>})
       HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   }
   return PR_OK;
@@ -546,21 +548,27 @@ int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
   return PR_OK;
 }
 
-// Tracking group: the segment kernel stores every U_k^T image, then one
-// workgroup per member walks the slices forward and back.
+// Tracking group: the segment kernel stores every U_k^T image and each
+// segment's checkpoint operator, a per-member scan chains the segments, then
+// every segment walks back in parallel.
 template <int B>
-int launch_track(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hipStream_t st) {
+int launch_track(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t st) {
   const dim3 blk(64 * B * B);
   int rc;
   if ((rc = tbegin(pl, st))) return rc;
-  hipLaunchKernelGGL((pr::k_segment<B, true>), dim3(g.n_items), blk, pr::Smem<B>::segment, st, pl.d_probs, g.d_items, d_u);
+  hipLaunchKernelGGL((pr::k_segment<B, true, false, true>), dim3(g.n_items), blk, pr::Smem<B>::segment, st, pl.d_probs,
+                     g.d_items, d_u);
   HIPCHK(hipGetLastError());
   if ((rc = tend(pl, 0, st))) return rc;
   if ((rc = tbegin(pl, st))) return rc;
+  hipLaunchKernelGGL((pr::k_track_scan<B>), dim3(g.n_members), blk, pr::Smem<B>::base, st, pl.d_probs, g.d_members);
+  HIPCHK(hipGetLastError());
+  if ((rc = tend(pl, 1, st))) return rc;
+  if ((rc = tbegin(pl, st))) return rc;
   switch (g.V) {
-#define PR_TCASE(VV)                                                                                         \
-  case VV:                                                                                                   \
-    hipLaunchKernelGGL((pr::k_track<B, VV>), dim3(g.n_members), blk, pr::Smem<B>::base, st, pl.d_probs, g.d_members); \
+#define PR_TCASE(VV)                                                                                                 \
+  case VV:                                                                                                           \
+    hipLaunchKernelGGL((pr::k_track_walk<B, VV>), dim3(g.n_items), blk, pr::Smem<B>::base, st, pl.d_probs, g.d_items); \
     break;
     PR_TCASE(2) PR_TCASE(4) PR_TCASE(8) PR_TCASE(16) PR_TCASE(32)
 #undef PR_TCASE
@@ -568,7 +576,6 @@ int launch_track(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
       return fail(PR_ERR_LAUNCH, "internal: bad reduction width %d", g.V);
   }
   HIPCHK(hipGetLastError());
-  (void)grad;  // the walk produces F and G together
   return tend(pl, 2, st);
 }
 
@@ -608,10 +615,10 @@ int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
     int rc = PR_OK;
     if (g.track) {
       switch (g.B) {
-        case 1: rc = launch_track<1>(pl, g, d_u, d_G != nullptr, st); break;
-        case 2: rc = launch_track<2>(pl, g, d_u, d_G != nullptr, st); break;
-        case 3: rc = launch_track<3>(pl, g, d_u, d_G != nullptr, st); break;
-        case 4: rc = launch_track<4>(pl, g, d_u, d_G != nullptr, st); break;
+        case 1: rc = launch_track<1>(pl, g, d_u, st); break;
+        case 2: rc = launch_track<2>(pl, g, d_u, st); break;
+        case 3: rc = launch_track<3>(pl, g, d_u, st); break;
+        case 4: rc = launch_track<4>(pl, g, d_u, st); break;
         default: rc = fail(PR_ERR_LAUNCH, "internal: block count %d", g.B);
       }
       if (rc) return rc;
@@ -921,7 +928,6 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       D.ck_of = ip;
       if ((rc = pl.upload(&dp, cimg.data(), cimg.size()))) return rc;
       D.Ck = dp;
-      if ((rc = pl.alloc(&D.Psit, (size_t)M * nt * MB))) return rc;
     }
     // scratch
     D.recompute = recompute && can_recompute(q) ? 1 : 0;
@@ -944,6 +950,11 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     if ((rc = pl.alloc(&D.udef, M))) return rc;
     HIPCHK(hipMemset(D.udef, 0, sizeof(double) * M));
     if ((rc = pl.alloc(&D.smax, (size_t)M * S))) return rc;
+    if (track) {
+      if ((rc = pl.alloc(&D.Zseg, (size_t)M * S * MB))) return rc;
+      if ((rc = pl.alloc(&D.Pend, (size_t)M * S * MB))) return rc;
+      if ((rc = pl.alloc(&D.Lend, (size_t)M * S * MB))) return rc;
+    }
     D.coff = pl.coff[p];
     pl.coff[p + 1] = pl.coff[p] + (long long)nc * nt;
     for (int m = 0; m < M; m++) {
