@@ -1270,7 +1270,8 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
 //   k_track_walk        per segment, from (Psi, L) at its end back to its
 //                       start: L += 2 C Psi at checkpoints, contract
 //                       M = 2 Psi L^dag with every control, then
-//                       L <- U_k^dag L and Psi <- U_k^dag Psi.
+//                       L <- U_k^dag L and Psi <- U_k^dag Psi (recomputing
+//                       the states backwards: U_k is unitary).
 template <int B>
 __global__ __launch_bounds__(64 * B * B) void k_track_scan(const ProbDev* __restrict__ probs,
                                                            const int2* __restrict__ members) {
@@ -1305,7 +1306,48 @@ __global__ __launch_bounds__(64 * B * B) void k_track_scan(const ProbDev* __rest
   }
 }
 
-template <int B, int V>
+// Columns [sh, 2 sh) of the result hold columns [0, sh) of X, the others
+// are zero (lane shifts inside each 16-lane row).
+__device__ inline Blk shift_cols(const Blk& X, int sh, const Ln& x) {
+  const bool in = x.c >= sh && x.c < 2 * sh;
+  const int src = in ? x.lane - sh : x.lane;
+  Blk r;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const double re = __shfl(X.re[i], src), im = __shfl(X.im[i], src);
+    r.re[i] = in ? re : 0.0;
+    r.im[i] = in ? im : 0.0;
+  }
+  return r;
+}
+
+// M = Z[:, 0:np] Z[:, np:2np]^dag for one wavefront, Z in the LDS tile LZ:
+// the k-steps cover k < np only, and both operands of lane (c, q) come from
+// row c of Z (A = Z[c][k], B = conj(Z[c][np + k])).
+__device__ inline Blk outer_pairs(const double2* LZ, int np, const Ln& x) {
+  d4 t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
+  const int nkb = (np + 3) >> 2;
+  for (int kb = 0; kb < nkb; kb++) {
+    const int k = rowof(x.q, kb);
+    double2 a = make_double2(0, 0), b = make_double2(0, 0);
+    if (k < np) {
+      a = LZ[lidx(0, x.c, k)];
+      b = LZ[lidx(0, x.c, np + k)];
+    }
+    t1 = mfma(a.x, b.x, t1);
+    t2 = mfma_nega(a.y, b.y, t2);
+    t3 = mfma(a.x + a.y, b.x - b.y, t3);
+  }
+  Blk r;
+  r.re = t1 - t2;
+  r.im = t3 - t1 - t2;
+  return r;
+}
+
+// PACK (d <= 16, n_pairs <= 8): states and co-states share one block,
+// Z = [Psi | L], so U_k^dag Z is one product and M = Psi L^dag takes
+// ceil(n_pairs / 4) k-steps instead of a full product.
+template <int B, int V, bool PACK = false>
 __global__ __launch_bounds__(64 * B * B) void k_track_walk(const ProbDev* __restrict__ probs,
                                                            const WorkItem* __restrict__ items) {
   const WorkItem it = items[blockIdx.x];
@@ -1321,6 +1363,34 @@ __global__ __launch_bounds__(64 * B * B) void k_track_walk(const ProbDev* __rest
   Blk S = load_blk(P.Pend + sb, x);
   Blk L = load_blk(P.Lend + sb, x);
   Blk Vn = load_blk(Um + (size_t)(it.k1 - 1) * MB, x);
+  if constexpr (PACK) {
+    static_assert(B == 1, "the packed walk is single-wavefront");
+    const int np = P.npairs;
+    Blk Z = S;
+    {
+      const Blk Ls = shift_cols(L, np, x);
+      Z.re += Ls.re;
+      Z.im += Ls.im;
+    }
+    for (int k = it.k1 - 1; k >= it.k0; k--) {
+      const Blk Uk = Vn;
+      if (k > it.k0) Vn = load_blk(Um + (size_t)(k - 1) * MB, x);
+      const int c = gptr(P.ck_of)[k];
+      if (c >= 0) {
+        const Blk Y = shift_cols(mm<1, N, N>(load_blk(P.Ck + (size_t)c * MB, x), Z, LA, LB, x), np, x);
+        Z = axpby(1.0, Z, 2.0, Y);  // L += 2 C Psi
+      }
+      put<1>(LB, Z, x);
+      bsync<1>();
+      const Blk Mk = outer_pairs(LB, np, x);  // Psi L^dag
+      contract<1, V>(P, m, k, axpby(2.0, Mk, 0.0, Mk), LA, x);
+      put<1>(LA, Uk, x);
+      bsync<1>();
+      Z = gemm_prod<1, J, N>(LA, Uk, LB, Z, x);  // U_k^dag [Psi | L]
+      bsync<1>();
+    }
+    return;
+  }
   for (int k = it.k1 - 1; k >= it.k0; k--) {
     const Blk Uk = Vn;
     if (k > it.k0) Vn = load_blk(Um + (size_t)(k - 1) * MB, x);

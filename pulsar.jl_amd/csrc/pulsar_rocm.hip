@@ -289,7 +289,8 @@ struct Plan {
     bool mixed = false;     // FP32-MFMA exp and gradient walk (d <= 16 only)
     bool trace_regs = false;  // mixed walk keeps the trace operand in registers
     bool recompute = false;   // some problem of the group recomputes U_k in the gradient walk
-    bool track = false;       // tracking-objective problems (k_segment, then k_track)
+    bool track = false;       // tracking-objective problems (k_segment, k_track_scan, k_track_walk)
+    bool track_packed = false;  // d <= 16 tracking with n_pairs <= 8: states and co-states in one block
     double2* big_scratch = nullptr;  // d > 64: per work item matrices in HBM
     pr::WorkItem* d_items = nullptr;
     int n_items = 0;
@@ -568,7 +569,12 @@ int launch_track(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t 
   switch (g.V) {
 #define PR_TCASE(VV)                                                                                                 \
   case VV:                                                                                                           \
-    hipLaunchKernelGGL((pr::k_track_walk<B, VV>), dim3(g.n_items), blk, pr::Smem<B>::base, st, pl.d_probs, g.d_items); \
+    if (B == 1 && g.track_packed)                                                                                    \
+      hipLaunchKernelGGL((pr::k_track_walk<1, VV, true>), dim3(g.n_items), blk, pr::Smem<1>::base, st, pl.d_probs,   \
+                         g.d_items);                                                                                 \
+    else                                                                                                             \
+      hipLaunchKernelGGL((pr::k_track_walk<B, VV>), dim3(g.n_items), blk, pr::Smem<B>::base, st, pl.d_probs,         \
+                         g.d_items);                                                                                 \
     break;
     PR_TCASE(2) PR_TCASE(4) PR_TCASE(8) PR_TCASE(16) PR_TCASE(32)
 #undef PR_TCASE
@@ -982,6 +988,9 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     Plan::Group g;
     g.B = B;
     g.track = gi > 8;
+    g.track_packed = gi == 9;
+    for (int p = 0; p < n_probs; p++)
+      if (group_of(probs[p]) == gi && probs[p].n_pairs > 8) g.track_packed = false;
     g.V = vmax[gi];
     g.n_items = (int)items[gi].size();
     g.n_members = (int)members[gi].size();
