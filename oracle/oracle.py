@@ -3,6 +3,12 @@
 TEST INFRASTRUCTURE ONLY: the checker for the device path. Imported by
 tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg; never by the
 product package.
+
+tracking_evaluate is a numpy restatement (density matrices) of the tracking
+kernel; tests/test_tracking_oracle.py pins it (squared overlaps from scipy
+expm products, a central difference of the left-perturbed propagator, the
+exact derivative for commuting generators, Hermitian-part and linearity
+invariants).
 """
 from __future__ import annotations
 

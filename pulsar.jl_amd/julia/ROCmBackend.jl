@@ -19,6 +19,7 @@ const PR_LINDBLAD = Cint(1)
 const PR_GATE = Cint(0)
 const PR_PAIRS_SQUARE = Cint(1)
 const PR_PAIRS_REAL = Cint(2)
+const PR_TRACKING = Cint(3)
 
 # Memory layout identical to `pr_problem` (include/pulsar_rocm.h).
 struct PRProblem
@@ -39,6 +40,10 @@ struct PRProblem
     n_pairs::Cint
     psi_init::Ptr{Cdouble}
     psi_targ::Ptr{Cdouble}
+    n_track::Cint
+    track_step::Ptr{Cint}
+    track_weight::Ptr{Cdouble}
+    track_target::Ptr{Cdouble}
 end
 
 struct PROptions
@@ -154,7 +159,7 @@ function rocm_problem(H0s, Hc::Vector{<:AbstractMatrix}, n_t::Int, dt::Float64, 
         Ut = ComplexF64.(target.target_unitary)
         push!(roots, Ut)
         prob = PRProblem(d, length(Hc), n_t, dt, C_NULL, PR_CLOSED, PR_GATE, M, _cptr(H0), permember,
-                         _cptr(HcA), _cptr(sc), _cptr(w), _cptr(Ut), 0, C_NULL, C_NULL)
+                         _cptr(HcA), _cptr(sc), _cptr(w), _cptr(Ut), 0, C_NULL, C_NULL, 0, C_NULL, C_NULL, C_NULL)
     else
         ψt = ComplexF64.(target.target_state)
         ψi = (fixed_point || target.initial_state === nothing) ? ψt : ComplexF64.(target.initial_state)
@@ -164,7 +169,7 @@ function rocm_problem(H0s, Hc::Vector{<:AbstractMatrix}, n_t::Int, dt::Float64, 
         push!(roots, ψi, ψt)
         w === nothing && (w = ones(M) ./ M; push!(roots, w))
         prob = PRProblem(d, length(Hc), n_t, dt, C_NULL, PR_CLOSED, PR_PAIRS_SQUARE, M, _cptr(H0), permember,
-                         _cptr(HcA), _cptr(sc), _cptr(w), C_NULL, 1, _cptr(ψi), _cptr(ψt))
+                         _cptr(HcA), _cptr(sc), _cptr(w), C_NULL, 1, _cptr(ψi), _cptr(ψt), 0, C_NULL, C_NULL, C_NULL)
     end
     return prob, roots
 end

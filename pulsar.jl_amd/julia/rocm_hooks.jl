@@ -78,6 +78,39 @@ end
 # dt_slices = ctrl.pulse_dt, objective = :square → PR_PAIRS_SQUARE, :real → PR_PAIRS_REAL;
 # the Lindblad branch passes ctrl._L_drifts / ctrl._L_controls with dynamics = PR_LINDBLAD.
 
+# grape_tracking_kernel(waveform, ctrl)  (src/Application/MR/GRAPETracking.jl:145-157)
+# gains `elseif ctrl.backend == :rocm` → `_rocm_grape_tracking_kernel(waveform, ctrl)`:
+# one PR_TRACKING problem over the members [(H, p) for H in ctrl.drifts for p in
+# ctrl.pwr_levels], initial states ctrl.rho_init and checkpoints ctrl.tracking; the
+# penalties are applied afterwards, as the CPU path does (:260).
+function _rocm_grape_tracking_kernel(waveform::Matrix{Float64}, ctrl)
+    isempty(ctrl.tracking) && error("grape_tracking_kernel: ctrl.tracking is empty. " *
+                                    "Add TrackingPoint entries or call grape_state_kernel instead.")
+    n_ctrl, n_t = size(waveform)
+    d = size(ctrl.drifts[1], 1)
+    members = [(H, p) for H in ctrl.drifts for p in ctrl.pwr_levels]
+    M = length(members)
+    H0 = cat([ComplexF64.(H) for (H, _) in members]...; dims = 3)
+    HcA = cat([ComplexF64.(o) for o in ctrl.operators]...; dims = 3)
+    sc = Matrix{Float64}(undef, n_ctrl, M)                        # C layout [M][n_ctrl]
+    for (m, (_, p)) in enumerate(members)
+        sc[:, m] .= p
+    end
+    ψ0 = reduce(hcat, [ComplexF64.(vec(r)) for r in ctrl.rho_init])   # C layout [n_pairs][d]
+    dts = Vector{Float64}(ctrl.pulse_dt)
+    steps = Cint[tp.step for tp in ctrl.tracking]
+    tw = Float64[tp.weight for tp in ctrl.tracking]
+    tt = cat([ComplexF64.(tp.target_dm) for tp in ctrl.tracking]...; dims = 3)
+    roots = Any[H0, HcA, sc, ψ0, dts, steps, tw, tt]
+    prob = PRProblem(d, n_ctrl, n_t, 0.0, pointer(dts), PR_CLOSED, PR_TRACKING, M, _cptr(H0), 1,
+                     _cptr(HcA), _cptr(sc), C_NULL, C_NULL, length(ctrl.rho_init), _cptr(ψ0), C_NULL,
+                     length(steps), pointer(steps), pointer(tw), _cptr(tt))
+    plan = ROCmPlan([prob], roots)
+    F, G = rocm_eval(plan, [waveform])
+    fidelity = _apply_penalties!(F[1], G[1], waveform, ctrl)
+    return fidelity, G[1]
+end
+
 # grape_optimize(system, target, controls_init; config)  (src/Optimization/GRAPE.jl:469-643)
 # The whole gradient-ascent loop in the library (pr_grape_optimize): controls,
 # gradient and best controls stay in HBM; the step adaptation, update and

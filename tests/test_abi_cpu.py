@@ -125,3 +125,16 @@ def test_oversized_dim_unsupported():
     with pytest.raises(native.RocmError) as e:
         native.Plan([pairs])
     assert e.value.code == -5
+
+
+def test_problem_struct_mirrors_match_the_header():
+    # pr_problem field order in the header, the ctypes mirror and the Julia mirror
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct \{(.*?)\} pr_problem;", text, flags=re.S).group(1)
+    c_fields = re.findall(r"(\w+)\s*;", body)
+    from pulsar_rocm.problem import CProblem
+    py_fields = [f[0] for f in CProblem._fields_]
+    jl = open(os.path.join(ROOT, "pulsar.jl_amd", "julia", "ROCmBackend.jl")).read()
+    jl_body = re.search(r"struct PRProblem\n(.*?)\nend", jl, flags=re.S).group(1)
+    jl_fields = re.findall(r"^\s*(\w+)::", jl_body, flags=re.M)
+    assert c_fields == py_fields == jl_fields, (c_fields, py_fields, jl_fields)
