@@ -15,8 +15,8 @@ from .api import (BatchedEnsemble, ControlSequence, EnsembleObjective, GRAPEConf
 from .verify import (AutoDiffConfig, compute_gradient_autodiff, finite_diff_gradient_ad,  # noqa: F401
                      finite_difference_gradient, grape_gradient_check, select_autodiff_backend,
                      verify_gradient, verify_gradient_autodiff)
-from .mr import (LindbladMRControl, MRControl, TrackingPoint, build_control_liouvillian,  # noqa: F401
-                 build_drift_liouvillian, grape_lindblad_kernel, grape_state_kernel,
-                 grape_tracking_kernel, pure_state_to_vec_rho, vec_rho)
+from .mr import (FidelityForward, LindbladMRControl, MRControl, TrackingPoint,  # noqa: F401
+                 build_control_liouvillian, build_drift_liouvillian, fidelity_forward, grape_lindblad_kernel,
+                 grape_state_kernel, grape_tracking_kernel, pure_state_to_vec_rho, vec_rho)
 from .problem import (PR_CLOSED, PR_FP64, PR_GATE, PR_GRAD_EXACT, PR_GRAD_REFERENCE,  # noqa: F401
                       PR_LINDBLAD, PR_MIXED, PR_PAIRS_REAL, PR_PAIRS_SQUARE, PR_TRACKING, Problem)

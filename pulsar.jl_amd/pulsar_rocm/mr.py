@@ -7,6 +7,7 @@
   build_drift_liouvillian, build_control_liouvillian, pure_state_to_vec_rho
                                           src/Physics/Lindblad.jl:89-197
   TrackingPoint / grape_tracking_kernel   src/Application/MR/GRAPETracking.jl:89-157
+  fidelity_forward                        src/Application/MR/GRAPEState.jl:406-566
 
 Ensemble members are the (drift, power level) pairs in the reference's order
 (`for H in drifts for p in pwr_levels`, GRAPEState.jl:146); every member
@@ -201,3 +202,53 @@ def grape_tracking_kernel(waveform: np.ndarray, ctrl: MRControl):
         return kp(waveform)
     finally:
         kp.close()
+
+
+class FidelityForward:
+    """fidelity_forward (GRAPEState.jl:406-566) on the device with a reusable
+    plan: the ensemble-averaged state-transfer fidelity of one waveform
+    (F only, no gradient walk), or of a population of waveforms in one launch
+    (`batch`), as derivative-free optimizers evaluate them. Members are
+    [(H, p) for H in drifts for p in pwr_levels]; F averages over members and
+    pairs (N_ens = n_drifts n_pwr n_pairs, :512)."""
+
+    def __init__(self, drifts, pwr_levels, operators, rho_init, rho_targ, pulse_dt,
+                 fidelity_type: str = "real", population: int = 1):
+        import copy
+        ctrl = MRControl(drifts=list(drifts), operators=list(operators), rho_init=list(rho_init),
+                         rho_targ=list(rho_targ), pulse_dt=np.asarray(pulse_dt, np.float64),
+                         pwr_levels=[float(p) for p in pwr_levels], fidelity=str(fidelity_type))
+        self.problem = ctrl.problem()
+        self.population = int(population)
+        if self.population < 1:
+            raise ValueError("population must be >= 1")
+        self.plan = native.Plan([copy.copy(self.problem) for _ in range(self.population)])
+
+    def batch(self, waveforms) -> np.ndarray:
+        """F of each of `population` waveforms [n_ctrl x n_t]."""
+        ws = [np.asarray(w, dtype=np.float64) for w in waveforms]
+        if len(ws) != self.population:
+            raise ValueError(f"expected {self.population} waveforms, got {len(ws)}")
+        F, _ = self.plan.eval(ws, grad=False)
+        return np.asarray(F, dtype=np.float64)
+
+    def __call__(self, waveform) -> float:
+        if self.population != 1:
+            raise ValueError("a population plan evaluates with batch()")
+        return float(self.batch([waveform])[0])
+
+    def close(self):
+        self.plan.close()
+
+
+def fidelity_forward(waveform, drifts, pwr_levels, operators, rho_init, rho_targ, pulse_dt,
+                     fidelity_type: str = "real", backend: str = "rocm") -> float:
+    """fidelity_forward (GRAPEState.jl:434-466) with backend :rocm: one
+    F-only device evaluation (for repeated calls keep a FidelityForward)."""
+    if str(backend).lstrip(":") != "rocm":
+        raise ValueError(f"fidelity_forward: backend :{str(backend).lstrip(':')} is not this library's (use :rocm)")
+    ff = FidelityForward(drifts, pwr_levels, operators, rho_init, rho_targ, pulse_dt, fidelity_type)
+    try:
+        return ff(waveform)
+    finally:
+        ff.close()

@@ -135,8 +135,12 @@ class Plan:
         self.problems = list(problems)
         n = len(self.problems)
         arr = (type(to_c(self.problems[0])) * n)()
+        # every struct's arrays stay referenced until the plan has copied them,
+        # also when one Problem object appears several times
+        self._keeps = []
         for i, p in enumerate(self.problems):
             arr[i] = to_c(p)
+            self._keeps.append(p._keep)
         self._structs = arr
         opt = COptions(int(precision), int(grad_mode), int(segment), int(store_props))
         h = ctypes.c_void_p()

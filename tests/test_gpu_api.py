@@ -265,3 +265,37 @@ def test_device_grape_loop_matches_host_loop(case):
     assert dev.n_fidelity_evaluations == host.n_fidelity_evaluations
     assert dev.n_gradient_evaluations == host.n_gradient_evaluations
     assert dev.metadata["step_size_final"] == pytest.approx(host.metadata["step_size_final"], rel=1e-12)
+
+
+@pytest.mark.parametrize("ftype", ["real", "square"])
+def test_fidelity_forward_single_and_population(ftype):
+    # fidelity_forward (GRAPEState.jl:406-566): F only, members drift x pwr,
+    # averaged over members and pairs; a population plan gives the same F per
+    # waveform as single evaluations
+    rng = np.random.default_rng(31)
+    Iz, Ix, Iy = SZ / 2, SX / 2, SY / 2
+    H2 = np.kron(Iz, np.eye(2)) * 2 * np.pi * 40 + np.kron(np.eye(2), Iz) * 2 * np.pi * -25
+    H2 = H2 + 2 * np.pi * 5 * np.kron(Iz, Iz)
+    drifts = [H2, 1.02 * H2]
+    ops = [np.kron(Ix, np.eye(2)) + np.kron(np.eye(2), Ix), np.kron(Iy, np.eye(2)) + np.kron(np.eye(2), Iy)]
+    rho_i = [W.random_state(4, rng) for _ in range(3)]
+    rho_t = [W.random_state(4, rng) for _ in range(3)]
+    dts = rng.uniform(1e-6, 3e-6, 80)
+    pwr = [2 * np.pi * 4e4, 2 * np.pi * 5e4]
+    wfs = [rng.uniform(-1, 1, (2, 80)) for _ in range(5)]
+    ctrl = P.MRControl(drifts=drifts, operators=ops, rho_init=rho_i, rho_targ=rho_t, pulse_dt=dts,
+                       pwr_levels=pwr, fidelity=ftype)
+    F1 = P.fidelity_forward(wfs[0], drifts, pwr, ops, rho_i, rho_t, dts, fidelity_type=ftype)
+    F0, _ = O.evaluate(ctrl.problem(), wfs[0], grad=False)
+    assert abs(F1 - F0) <= 1e-12, (F1, F0)
+    ff = P.FidelityForward(drifts, pwr, ops, rho_i, rho_t, dts, fidelity_type=ftype, population=5)
+    try:
+        Fb = ff.batch(wfs)
+    finally:
+        ff.close()
+    assert Fb[0] == F1
+    for w, f in zip(wfs, Fb):
+        Fo, _ = O.evaluate(ctrl.problem(), w, grad=False)
+        assert abs(f - Fo) <= 1e-12
+    with pytest.raises(ValueError):
+        P.fidelity_forward(wfs[0], drifts, pwr, ops, rho_i, rho_t, dts, backend="cuda")
