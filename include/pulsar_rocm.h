@@ -22,6 +22,7 @@
  *   grape_state_kernel (:cpu/:cuda branches)  src/Application/MR/GRAPEState.jl:73-85, 136-250
  *   grape_lindblad_kernel                     src/Application/MR/GRAPELindblad.jl:83-92, 96-215
  *   grape_tracking_kernel                     src/Application/MR/GRAPETracking.jl:145-263
+ *   compute_grape_gradient(_powder) for MAS   src/Physics/MRPhysics.jl:155-188, 312-336
  *
  * Conventions (identical to Julia's in-memory layout, so a ccall passes
  * arrays without copies):
@@ -63,9 +64,13 @@ extern "C" {
 #define PR_LINDBLAD 1 /* Liouville space, generator L (any complex matrix, dim = N^2) */
 
 /* objectives */
-#define PR_GATE 0         /* F_m = |Tr(U_t^dag U_total)/dim|^2 (Objectives.jl:239-246) */
-#define PR_PAIRS_SQUARE 1 /* F_m = sum_s |<t_s|U_total|i_s>|^2 (Objectives.jl:221) */
-#define PR_PAIRS_REAL 2   /* F_m = sum_s Re <t_s|U_total|i_s>  (Objectives.jl:220) */
+#define PR_GATE 0         /* F_m = |Tr(U_t^dag U_total)/dim|^2 (Objectives.jl:239-246); gradient of
+                             _gate_gradient_into! (Gradients.jl:254-291) with the backward
+                             propagators its caller builds: Q_k^dag = U_n ... U_k, slice k
+                             included (Propagators.jl:375-380) */
+#define PR_PAIRS_SQUARE 1 /* F_m = sum_s |<t_s|U_total|i_s>|^2 (Objectives.jl:221); gradient of the
+                             MR state kernel (GRAPEState.jl:218-229): <lambda_{k+1}|A_j|psi_k> */
+#define PR_PAIRS_REAL 2   /* F_m = sum_s Re <t_s|U_total|i_s>  (Objectives.jl:220), as above */
 #define PR_TRACKING 3     /* F_m = sum_s sum_c w_c Re Tr(rho_c P_{n_c} |i_s><i_s| P_{n_c}^dag),
                              P_n = U_n ... U_1: density-matrix checkpoints of
                              _grape_tracking_cpu_threaded (GRAPETracking.jl:162-263);
@@ -73,6 +78,16 @@ extern "C" {
                              -dt Re(i Tr(Lambda^dag [s_mj A_j, rho_{n+1}])) (:238-248).
                              Closed dynamics, dim <= 64, PR_GRAD_REFERENCE; runs FP64
                              with stored U_k images whatever precision/store_props say */
+#define PR_STATE 5        /* closed dynamics: F_m as PR_PAIRS_SQUARE; gradient of
+                             _state_gradient_into! (Gradients.jl:357-399), lambda_k = Q_k psi_t
+                             with slice k included: 2 Re(conj(z) (-i dt) <lambda_k|A_j|P_k psi_i>)
+                             (compute_grape_gradient with a state target; the caller
+                             normalises the states as Gradients.jl:369-374 does) */
+#define PR_GATE_PROPAGATORS 4 /* F as PR_GATE; gradient of _grape_gradient_from_propagators
+                                 (src/Physics/MRPhysics.jl:312-336, the MAS/DNP gate gradient):
+                                 G_jk = 2 Re(conj(Phi) Tr(Q_k^dag (-i dt A_j) P_k)) / dim^2,
+                                 P_k = U_{k-1}...U_1, Q_k^dag = U_n...U_k (no U_target^dag in
+                                 the trace). dim <= 64, PR_GRAD_REFERENCE */
 
 /* precision of the device path */
 #define PR_FP64 0  /* every product in FP64 (FP64 MFMA) */

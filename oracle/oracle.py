@@ -8,7 +8,10 @@ tracking_evaluate is a numpy restatement (density matrices) of the tracking
 kernel; tests/test_tracking_oracle.py pins it (squared overlaps from scipy
 expm products, a central difference of the left-perturbed propagator, the
 exact derivative for commuting generators, Hermitian-part and linearity
-invariants).
+invariants). mas_powder_gradient is a numpy restatement of the MAS powder
+gradient; tests/test_mas_oracle.py pins it (Wigner d-matrix relations of the
+Fourier components, the magic angle, a central difference of the trace of the
+right-perturbed propagator, the gate oracle with U_t = I).
 """
 from __future__ import annotations
 
@@ -202,6 +205,130 @@ def tracking_evaluate(p: Problem, u: np.ndarray, grad: bool = True):
     return F, (G if grad else None)
 
 
+_SPIN_I = {"1H": 0.5, "2H": 1.0, "13C": 0.5, "14N": 1.0, "15N": 0.5, "17O": 2.5, "19F": 0.5, "23Na": 1.5,
+           "29Si": 0.5, "31P": 0.5, "129Xe": 0.5}
+
+
+def spin_operators(isotopes):
+    """(Ix, Iy, Iz) per spin in the tensor-product space (NMRSpinSystem.jl:65-171):
+    Iz = diag(I, I-1, ..., -I), I+ = sqrt(I(I+1) - m(m+1)) above the diagonal."""
+    Is = [_SPIN_I[i] for i in isotopes]
+    dims = [int(round(2 * I + 1)) for I in Is]
+
+    def single(I):
+        n = int(round(2 * I + 1))
+        ms = [I - r for r in range(n)]
+        Iz = np.diag(ms).astype(complex)
+        Ip = np.zeros((n, n), complex)
+        for r in range(n - 1):
+            m = ms[r + 1]
+            Ip[r, r + 1] = np.sqrt(I * (I + 1) - m * (m + 1))
+        return (Ip + Ip.conj().T) / 2, (Ip - Ip.conj().T) / 2j, Iz
+
+    def embed(k, op):
+        out = np.ones((1, 1), complex)
+        for i, di in enumerate(dims):
+            out = np.kron(out, op if i == k else np.eye(di))
+        return out
+
+    ops = [single(I) for I in Is]
+    return ([embed(k, ops[k][a]) for k in range(len(Is))] for a in range(3))
+
+
+def mas_fourier(isotopes, csa, dipolar, orientation=(0.0, 0.0, 0.0)):
+    """{m: H_m} of the MAS drift for the system rotated by `orientation`
+    (Euler angles added, MASSpinSystem.jl:190-199; components :108-166: CSA
+    tensor i on spin i, dipolar couplings on Iz_i Iz_j, sidebands only when
+    sin(beta) > 1e-12). csa: (delta_iso, delta_aniso, eta, (a, b, g));
+    dipolar: (i, j, b_ij, (a, b, g)) with 1-based spins."""
+    Ix, Iy, Iz = spin_operators(isotopes)
+    D = Iz[0].shape[0]
+    oa, ob, og = orientation
+    Hf = {m: np.zeros((D, D), complex) for m in range(-2, 3)}
+    for i, (d_iso, d_an, _eta, (a, b, g)) in enumerate(csa):
+        if i >= len(Iz):
+            break
+        a, b = a + oa, b + ob
+        T20 = d_an * np.sqrt(2 / 3)
+        Hf[0] += (d_iso + T20 * (3 * np.cos(b) ** 2 - 1) / 2) * Iz[i]
+        if np.sin(b) > 1e-12:
+            c1 = -np.sqrt(3 / 2) * T20 * np.sin(b) * np.cos(b) * np.exp(-1j * a)
+            c2 = np.sqrt(3 / 8) * T20 * np.sin(b) ** 2 * np.exp(-2j * a)
+            Hf[1] += c1 * Iz[i]
+            Hf[-1] += np.conj(c1) * Iz[i]
+            Hf[2] += c2 * Iz[i]
+            Hf[-2] += np.conj(c2) * Iz[i]
+    for (si, sj, bij, (a, b, g)) in dipolar:
+        if not (si <= len(Iz) and sj <= len(Iz)):
+            continue
+        a, b, g = a + oa, b + ob, g + og
+        ZZ = Iz[si - 1] @ Iz[sj - 1]
+        Hf[0] += 2 * (bij * (3 * np.cos(b) ** 2 - 1) / 2) * ZZ
+        if np.sin(b) > 1e-12:
+            c1 = -bij * np.sqrt(6.0) * np.sin(b) * np.cos(b) * np.exp(-1j * (a + g))
+            c2 = bij * np.sqrt(6.0) / 4 * np.sin(b) ** 2 * np.exp(-2j * (a + g))
+            Hf[1] += c1 * ZZ
+            Hf[-1] += np.conj(c1) * ZZ
+            Hf[2] += c2 * ZZ
+            Hf[-2] += np.conj(c2) * ZZ
+    return Hf
+
+
+def mas_propagators(Hf, Hc, omega_r_hz, u, dt, rotor_phase_0=0.0):
+    """U_k = exp(-i dt (sum_m e^{i m w_r t_k} H_m + sum_j u_jk H_j)),
+    t_k = (k - 1/2) dt + phi0 / w_r (MASPropagators.jl:14-35)."""
+    w_r = 2 * np.pi * omega_r_hz
+    u = np.asarray(u, np.float64)
+    D = Hf[0].shape[0]
+    Us = []
+    for k in range(u.shape[1]):
+        t = (k + 0.5) * dt + rotor_phase_0 / w_r
+        H = np.zeros((D, D), complex)
+        for m in range(-2, 3):
+            H = H + np.exp(1j * m * w_r * t) * Hf[m]
+        for j in range(u.shape[0]):
+            H = H + u[j, k] * Hc[j]
+        Us.append(expm_neg_i(H, dt))
+    return Us
+
+
+def propagator_gradient(Us, Hc, U_target, dt):
+    """_grape_gradient_from_propagators, unitary branch (MRPhysics.jl:312-336):
+    P_k = U_{k-1}..U_1, Q_k = (U_n..U_k)^dag (Propagators.jl:277-380),
+    G[j,k] = 2 Re(conj(Phi) Tr(Q_k^dag (-i dt H_j) P_k)) / d^2."""
+    nt, D = len(Us), Us[0].shape[0]
+    P = [np.eye(D, dtype=complex)]
+    for k in range(nt):
+        P.append(Us[k] @ P[k])
+    Q = [None] * (nt + 1)
+    Q[nt] = np.eye(D, dtype=complex)
+    for k in range(nt - 1, -1, -1):
+        Q[k] = Us[k].conj().T @ Q[k + 1]
+    Phi = np.trace(np.asarray(U_target, complex).conj().T @ P[nt]) / D
+    G = np.zeros((len(Hc), nt))
+    for k in range(nt):
+        for j, Hj in enumerate(Hc):
+            inner = np.trace(Q[k].conj().T @ (-1j * dt * Hj) @ P[k])
+            G[j, k] = 2 * np.real(np.conj(Phi) * inner) / D ** 2
+    return G
+
+
+def mas_powder_gradient(isotopes, csa, dipolar, omega_r_hz, u, dt, U_target, orientations,
+                        rotor_phase_0: float = 0.0):
+    """compute_grape_gradient_powder (src/Physics/MRPhysics.jl:170-188):
+    G = sum_o w_o G_o over orientations (a, b, g, w), each G_o from
+    mas_fourier -> mas_propagators -> propagator_gradient with the RF controls
+    [Ix_1, Iy_1, Ix_2, Iy_2, ...] (MASSpinSystem.jl:95-100)."""
+    Ix, Iy, _ = spin_operators(isotopes)
+    Hc = [h for k in range(len(Ix)) for h in (Ix[k], Iy[k])]
+    G = 0.0
+    for (oa, ob, og, ow) in orientations:
+        Hf = mas_fourier(isotopes, csa, dipolar, (oa, ob, og))
+        Us = mas_propagators(Hf, Hc, omega_r_hz, u, dt, rotor_phase_0)
+        G = G + ow * propagator_gradient(Us, Hc, U_target, dt)
+    return G
+
+
 def total_propagator(p: Problem, u: np.ndarray, member: int = 0):
     c = to_c(p)
     ub = controls_cm(u)
@@ -213,4 +340,5 @@ def total_propagator(p: Problem, u: np.ndarray, member: int = 0):
 
 
 __all__ = ["build", "lib", "expm_neg_i", "expm", "expm_frechet", "eigh", "evaluate",
-           "tracking_evaluate", "total_propagator", "PR_GRAD_EXACT", "PR_GRAD_REFERENCE"]
+           "tracking_evaluate", "spin_operators", "mas_fourier", "mas_propagators", "propagator_gradient",
+           "mas_powder_gradient", "total_propagator", "PR_GRAD_EXACT", "PR_GRAD_REFERENCE"]

@@ -553,9 +553,15 @@ static void member_generator(const pr_problem* p, int m, const double* u, int k,
  * backward Q (Propagators.jl:358-382), then _gate_gradient_into!
  * (Gradients.jl:254-291): Phi = Tr(Ut^dag U_total)/d,
  * A_k = (P_k Ut^dag) Q_k^dag, s = sum_{q,p} A_k[p,q] H_j[q,p],
- * G[j,k] = 2 Re(conj(Phi) (-i dt s / d)). F is compute_fidelity's
- * |Tr(Ut^dag U)/d|^2 (Objectives.jl:239-246). Exact mode replaces A_k by the
- * Frechet derivative L(X_k, A_k), the exact adjoint of the first-order term. */
+ * G[j,k] = 2 Re(conj(Phi) (-i dt s / d)). The reference's loop builds
+ * Q[k] = U_k^dag Q[k+1] from propagators[k] (Propagators.jl:375-380), so
+ * Q_k^dag = U_n ... U_k includes slice k itself (its docstring's
+ * U[k+1]^dag ... U[n]^dag does not): here R_k = U_n ... U_{k+1} and
+ * Q_k^dag = R_k U_k. F is compute_fidelity's |Tr(Ut^dag U)/d|^2
+ * (Objectives.jl:239-246). Exact mode uses the Frechet derivative
+ * L(X_k, P_k Ut^dag R_k), the exact adjoint. PR_GATE_PROPAGATORS is
+ * _grape_gradient_from_propagators (MRPhysics.jl:312-336): the trace without
+ * Ut^dag and divided by d once more. */
 static int gate_member(const pr_problem* p, int m, const double* u, int grad_mode, double* Fm,
                        double* Gm, work_t* W) {
   int d = p->dim, nc = p->n_ctrl, nt = p->n_t;
@@ -572,14 +578,23 @@ static int gate_member(const pr_problem* p, int m, const double* u, int grad_mod
   cx Phi = tr / (double)d;
   *Fm = creal(Phi * conj(Phi));
   if (!Gm) return PR_OK;
-  /* backward propagators (0-based slice k): Q[nt-1] = I, Q[k] = U_{k+1}^dag Q[k+1] */
+  /* R_k^dag (0-based slice k): R[nt-1] = I, R[k] = U_{k+1}^dag R[k+1] */
   mat_eye(d, W->Q + (size_t)(nt - 1) * dd);
   for (int k = nt - 2; k >= 0; k--) mm_cn(d, W->Us + (k + 1) * dd, W->Q + (k + 1) * dd, W->Q + k * dd);
+  const int helper = p->objective == PR_GATE_PROPAGATORS;
   for (int k = 0; k < nt; k++) {
     double dtk = slice_dt(p, k);
-    mm_nc(d, W->P + k * dd, Ut, W->T1);      /* tmp = P_k Ut^dag */
-    mm_nc(d, W->T1, W->Q + k * dd, W->T2);   /* A_k = tmp Q_k^dag */
+    if (helper) {
+      mm_nc(d, W->P + k * dd, W->Q + k * dd, W->T2);  /* P_k R_k */
+    } else {
+      mm_nc(d, W->P + k * dd, Ut, W->T1);      /* tmp = P_k Ut^dag */
+      mm_nc(d, W->T1, W->Q + k * dd, W->T2);   /* tmp R_k */
+    }
     cx* A = W->T2;
+    if (grad_mode != PR_GRAD_EXACT) {
+      mm_nn(d, W->T2, W->Us + k * dd, W->T1);  /* A_k = P_k Ut^dag R_k U_k */
+      memcpy(W->T2, W->T1, sizeof(cx) * dd);
+    }
     if (grad_mode == PR_GRAD_EXACT) {
       member_generator(p, m, u, k, W);
       int rc;
@@ -597,6 +612,7 @@ static int gate_member(const pr_problem* p, int m, const double* u, int grad_mod
       for (int q = 0; q < d; q++)
         for (int pp = 0; pp < d; pp++) s += A[pp + (size_t)q * d] * Hj[q + (size_t)pp * d];
       cx inner = -I * dtk * s / (double)d;
+      if (helper) inner /= (double)d;
       Gm[j + (size_t)k * nc] = 2.0 * creal(conj(Phi) * inner);
     }
   }
@@ -617,7 +633,11 @@ static int pairs_member(const pr_problem* p, int m, const double* u, int grad_mo
   int d = p->dim, nc = p->n_ctrl, nt = p->n_t, np = p->n_pairs;
   size_t dd = (size_t)d * d;
   int closed = p->dynamics == PR_CLOSED;
-  int square = p->objective == PR_PAIRS_SQUARE;
+  int square = p->objective == PR_PAIRS_SQUARE || p->objective == PR_STATE;
+  /* PR_STATE: _state_gradient_into! (Gradients.jl:357-399), lambda_k = Q_k psi_t
+   * with Q_k including U_k (Propagators.jl:375-380): <lambda[k]|A|psi[k]>;
+   * the MR kernels pair lambda[k+1] with psi[k] (GRAPEState.jl:218-229) */
+  int after = p->objective == PR_STATE;
   double F = 0;
   if (Gm) memset(Gm, 0, sizeof(double) * nc * nt);
   cx* zs = (cx*)malloc(sizeof(cx) * np);
@@ -638,7 +658,7 @@ static int pairs_member(const pr_problem* p, int m, const double* u, int grad_mo
     if (grad_mode == PR_GRAD_EXACT) continue;
     for (int k = 0; k < nt; k++) {
       double dtk = slice_dt(p, k);
-      const cx* lv = lam + (size_t)(k + 1) * d;
+      const cx* lv = lam + (size_t)(k + 1 - after) * d;
       const cx* pv = psi + (size_t)k * d;
       for (int j = 0; j < nc; j++) {
         mv(d, W->Hs + j * dd, pv, W->tmpv);
@@ -689,6 +709,8 @@ static int pairs_member(const pr_problem* p, int m, const double* u, int grad_mo
   return PR_OK;
 }
 
+static int is_gate(const pr_problem* p) { return p->objective == PR_GATE || p->objective == PR_GATE_PROPAGATORS; }
+
 static int member_eval(const pr_problem* p, int m, const double* u, int grad_mode, int separate_fg,
                        double* Fm, double* Gm, work_t* W) {
   int rc;
@@ -705,17 +727,17 @@ static int member_eval(const pr_problem* p, int m, const double* u, int grad_mod
     rc = member_props(p, m, u, W);
     if (rc) return rc;
     double Fdummy;
-    rc = p->objective == PR_GATE ? gate_member(p, m, u, grad_mode, Fm, NULL, W)
+    rc = is_gate(p) ? gate_member(p, m, u, grad_mode, Fm, NULL, W)
                                  : pairs_member(p, m, u, grad_mode, Fm, NULL, W);
     if (rc || !Gm) return rc;
     rc = member_props(p, m, u, W);
     if (rc) return rc;
-    return p->objective == PR_GATE ? gate_member(p, m, u, grad_mode, &Fdummy, Gm, W)
+    return is_gate(p) ? gate_member(p, m, u, grad_mode, &Fdummy, Gm, W)
                                    : pairs_member(p, m, u, grad_mode, &Fdummy, Gm, W);
   }
   rc = member_props(p, m, u, W);
   if (rc) return rc;
-  return p->objective == PR_GATE ? gate_member(p, m, u, grad_mode, Fm, Gm, W)
+  return is_gate(p) ? gate_member(p, m, u, grad_mode, Fm, Gm, W)
                                  : pairs_member(p, m, u, grad_mode, Fm, Gm, W);
 }
 
@@ -723,9 +745,10 @@ static int check_problem(const pr_problem* p) {
   if (!p || p->dim < 1 || p->n_ctrl < 1 || p->n_t < 1 || p->n_members < 1 || !p->H0 || !p->Hc)
     return PR_ERR_ARG;
   if (p->dynamics != PR_CLOSED && p->dynamics != PR_LINDBLAD) return PR_ERR_ARG;
-  if (p->objective == PR_GATE) {
+  if (is_gate(p)) {
     if (!p->U_target || p->dynamics != PR_CLOSED) return PR_ERR_ARG;
-  } else if (p->objective == PR_PAIRS_SQUARE || p->objective == PR_PAIRS_REAL) {
+  } else if (p->objective == PR_PAIRS_SQUARE || p->objective == PR_PAIRS_REAL ||
+             (p->objective == PR_STATE && p->dynamics == PR_CLOSED)) {
     if (p->n_pairs < 1 || !p->psi_init || !p->psi_targ) return PR_ERR_ARG;
   } else {
     return PR_ERR_ARG;
@@ -784,7 +807,7 @@ int or_eval(const pr_problem* p, const double* controls, int grad_mode, int sepa
     double Fs = 0;
     for (int m = 0; m < M; m++) {
       double w = p->weight ? p->weight[m]
-                           : (p->objective == PR_GATE ? 1.0 / M : 1.0 / ((double)M * p->n_pairs));
+                           : (is_gate(p) ? 1.0 / M : 1.0 / ((double)M * p->n_pairs));
       Fs += w * Fm[m];
       if (G) {
         if (m == 0)

@@ -355,6 +355,8 @@ __global__ __launch_bounds__(1024) void k_gradient_big(const ProbDev* __restrict
     const double2* U = P.Ust + ((size_t)m * P.nt + k) * MB;
     big::mm<B, N, C>(Mk, Cm, U, LP);  // M_k = C_k U_k^dag
     // G_m[j,k] = Re(-i dt s_mj Tr(A_j M)): wavefront w takes controls j = w, w + 16, ...
+    // (M = C_k when the first-order term sits after U_k, P.after)
+    const double2* Mt = P.after ? Cm : Mk;
     for (int j = w; j < nc; j += big::W) {
       double sv = 0;
       const int e1 = gptr(P.tr_ptr)[j + 1];
@@ -362,7 +364,7 @@ __global__ __launch_bounds__(1024) void k_gradient_big(const ProbDev* __restrict
         // LDS-tile index of the element (b, a) of M -> image offset of that element
         const int idx = gptr(P.tr_idx)[e];
         const int blk = idx / 272, rem = idx % 272, row = rem / 17, col = rem % 17;
-        const double2 mv = gload2(Mk, (size_t)blk * 256 + (row >> 2) * 64 + col + 16 * (row & 3));
+        const double2 mv = gload2(Mt, (size_t)blk * 256 + (row >> 2) * 64 + col + 16 * (row & 3));
         const double2 a = gload2(P.tr_v, e);
         sv += a.x * mv.x + a.y * mv.y;  // Im(a * M[b][a]), a stored as (Im, Re)
       }

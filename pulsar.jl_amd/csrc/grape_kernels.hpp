@@ -43,6 +43,10 @@ struct ProbDev {
                               // (uniform dt), so assemble() needs no final scaling
   int recompute;              // 1: no U_k images (Ust is null); the gradient walk
                               // recomputes U_k (closed dynamics, d <= 64, FP64)
+  int after;                  // first-order gradient term of slice k: 1 with A_j after U_k,
+                              // Tr(.. U_k A_j P_k) (compute_grape_gradient, whose
+                              // Q_k^dag = U_n..U_k, Propagators.jl:375-380); 0 in place of
+                              // U_k (the MR kernels, GRAPEState.jl:218-229)
   double cre, cim;            // generator factor c: (0,-1) closed, (1,0) Lindblad
   double dt0;                 // the slice length when prescaled (uniform dt)
   const double2* H0;          // [M or 1][NB][256] tile images of the drift
@@ -781,8 +785,10 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
     v = block_max<B>(v, red, x);
     if (threadIdx.x == 0) P.udef[m] = v;
   }
-  if (P.obj == 0) {
-    // gate: Phi = Tr(U_t^dag U)/d, F = |Phi|^2, w = 2 Phi / d, Omega = conj(w) U_t^dag U
+  if (P.obj == 0 || P.obj == 4) {
+    // gate: Phi = Tr(U_t^dag U)/d, F = |Phi|^2, w = 2 Phi / d, Omega = conj(w) U_t^dag U;
+    // propagator-helper convention (objective 4): Omega = conj(w / d) U, i.e.
+    // G = 2 Re(conj(Phi) Tr(Q_k^dag (-i dt A_j) P_k)) / d^2 with no U_t^dag
     Blk Tg = load_blk(P.Ut, x);
     double2 part = make_double2(0, 0);
 #pragma unroll
@@ -794,6 +800,10 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
     const double phr = tr.x / P.d, phi = tr.y / P.d;
     if (threadIdx.x == 0) P.Fm[m] = phr * phr + phi * phi;
     const double wr = 2.0 * phr / P.d, wi = 2.0 * phi / P.d;
+    if (P.obj == 4) {
+      store_blk(P.Om + (size_t)m * MB, cscale(wr / P.d, -wi / P.d, Ut), x);
+      return;
+    }
     Blk Y = mm<B, C, N>(Tg, Ut, LA, LB, x);
     store_blk(P.Om + (size_t)m * MB, cscale(wr, -wi, Y), x);
     return;
@@ -808,7 +818,7 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
       if (rowof(x.q, r) == x.c) zb[16 * x.I + x.c] = make_double2(Z.re[r], Z.im[r]);
   }
   __syncthreads();
-  const bool square = (P.obj == 1);
+  const bool square = (P.obj == 1 || P.obj == 5);
   if (threadIdx.x == 0) {
     double F = 0;
     for (int s = 0; s < P.npairs; s++) {
@@ -865,12 +875,17 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
 // two wavefronts), group g walks the nonzeros of control j0 + g and reduces
 // them inside its lanes.
 constexpr int floor_pow2(int n) { return n >= 64 ? 64 : n >= 32 ? 32 : n >= 16 ? 16 : n >= 8 ? 8 : n >= 4 ? 4 : n >= 2 ? 2 : 1; }
-template <int B, int V, bool OWN32 = false>
+// TRANS (B == 1): the register block holds M^T.
+template <int B, int V, bool OWN32 = false, bool TRANS = false>
 __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, double2* LA, const Ln& x) {
   constexpr int THREADS = 64 * B * B;
   constexpr int G = floor_pow2(THREADS / V);
   const int nc = P.nc;
-  if constexpr (OWN32) {
+  static_assert(!TRANS || (B == 1 && !OWN32), "transposed contraction is single-block FP64");
+  if constexpr (TRANS) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) LA[lidx(0, x.c, rowof(x.q, r))] = make_double2(Mk.re[r], Mk.im[r]);
+  } else if constexpr (OWN32) {
     // M held in the FP32 own layout (rows 4q + r), same LDS positions
 #pragma unroll
     for (int r = 0; r < 4; r++) LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Mk.re[r], Mk.im[r]);
@@ -936,14 +951,18 @@ __device__ inline void trace_regs_load(const ProbDev& P, int m, TraceRegs<V>& t,
 // FP32 own layout (rows 4q + r) instead of the FP64 one. dtk: the length of
 // slice k, fetched by the caller together with its prefetch of the next slice
 // (a load here would wait for that prefetch too).
-template <int V, bool OWN32>
+template <int V, bool OWN32, bool TRANS = false>
 __device__ inline void contract_regs(const ProbDev& P, int m, int k, double dtk, const Blk& Mk, double2* LA,
                                      const TraceRegs<V>& t, const Ln& x) {
   constexpr int G = 64 / V;
   const int grp = x.lane / G, gl = x.lane % G;
 #pragma unroll
-  for (int r = 0; r < 4; r++)
-    LA[lidx(0, OWN32 ? rowof32(x.q, r) : rowof(x.q, r), x.c)] = make_double2(Mk.re[r], Mk.im[r]);
+  for (int r = 0; r < 4; r++) {
+    if constexpr (TRANS)
+      LA[lidx(0, x.c, rowof(x.q, r))] = make_double2(Mk.re[r], Mk.im[r]);
+    else
+      LA[lidx(0, OWN32 ? rowof32(x.q, r) : rowof(x.q, r), x.c)] = make_double2(Mk.re[r], Mk.im[r]);
+  }
   bsync<1>();
   double sv = 0;
 #pragma unroll
@@ -1061,6 +1080,20 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
     TraceRegs<V> tregs;
     if constexpr (REG) trace_regs_load<V>(P, m, tregs, x);
     double dtk = gptr(P.dts)[it.k0];
+    // first-order term after U_k: slice k contracts C_k itself, which the walk
+    // holds before its products (for B == 1 as G = C^T)
+    const bool aft = !EXACT && P.after;
+    auto contract_c = [&](int kk, double dtkk) {
+      if constexpr (B == 1) {
+        if constexpr (REG)
+          contract_regs<V, false, true>(P, m, kk, dtkk, Cm, LA, tregs, x);
+        else
+          contract<1, V, false, true>(P, m, kk, Cm, LA, x);
+      } else {
+        contract<B, V>(P, m, kk, Cm, LA, x);
+      }
+    };
+    if (aft) contract_c(it.k0, dtk);
     Blk Ut;
     if (RECOMP && P.recompute)
       Ut = recompute_ut<B>(P, m, it.k0, u, LA, LB, red, x);
@@ -1087,6 +1120,8 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
         Cm = mm<B, T, N>(Ut, Mk, LA, LB, x);      // U_k M_k
       if constexpr (EXACT) {
         contract<B, V>(P, m, k, exact_direction<B>(P, m, k, u, H0, Mk, LA, LB, red, x), LA, x);
+      } else if (aft) {
+        if (k + 1 < it.k1) contract_c(k + 1, dtn);  // C_{k+1}
       } else if constexpr (REG && B == 1) {
         contract_regs<V, false>(P, m, k, dtk, Mk, LA, tregs, x);
       } else {
@@ -1170,6 +1205,14 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
   int kb = it.k0;
   unsigned long long bigmask = __ballot(kb + x.lane < it.k1 && gptr(big)[kb + x.lane] != 0);
   double dtk = gptr(P.dts)[it.k0];
+  // first-order term after U_k (P.after): slice k contracts C_k
+  auto contract_c = [&](int kk, double dtkk) {
+    if constexpr (REG)
+      contract_regs<V, true>(P, m, kk, dtkk, Cm, LA, tregs, x);
+    else
+      contract<1, V, true>(P, m, kk, Cm, LA, x);
+  };
+  if (P.after) contract_c(it.k0, dtk);
   Blk32 D;
   if (!(bigmask & 1ull)) D = load_blk32(P.Udm + ((size_t)m * P.nt + it.k0) * 256, x);
   for (int k = it.k0; k < it.k1; k++) {
@@ -1237,10 +1280,13 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
       bsync<1>();
       if (nsmall) D = load_blk32(P.Udm + ((size_t)m * P.nt + k + 1) * 256, x);
     }
-    if constexpr (REG)
+    if (P.after) {
+      if (k + 1 < it.k1) contract_c(k + 1, dtn);
+    } else if constexpr (REG) {
       contract_regs<V, true>(P, m, k, dtk, Mk, LA, tregs, x);
-    else
+    } else {
       contract<1, V, true>(P, m, k, Mk, LA, x);
+    }
     dtk = dtn;
   }
 }

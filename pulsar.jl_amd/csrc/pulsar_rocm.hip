@@ -362,13 +362,15 @@ int validate(const pr_problem& p, int idx) {
     return fail(PR_ERR_UNSUPPORTED, "problem %d: dim %d > 64 is supported for closed-dynamics gate problems only",
                 idx, p.dim);
   if (!p.dt_slices && !(p.dt >= 0.0)) return fail(PR_ERR_ARG, "problem %d: dt must be non-negative", idx);
-  if (p.objective == PR_GATE) {
+  if (p.objective == PR_GATE || p.objective == PR_GATE_PROPAGATORS) {
     if (!p.U_target) return fail(PR_ERR_ARG, "problem %d: gate objective needs U_target", idx);
     if (p.dynamics != PR_CLOSED) return fail(PR_ERR_UNSUPPORTED, "problem %d: gate objective needs closed dynamics", idx);
     double dev = unitary_dev(p.U_target, p.dim);
     if (dev > 1e-8) return fail(PR_ERR_ARG, "problem %d: target is not unitary: ||U^dag U - I||/dim = %g > 1e-8", idx, dev);
-  } else if (p.objective == PR_PAIRS_SQUARE || p.objective == PR_PAIRS_REAL) {
+  } else if (p.objective == PR_PAIRS_SQUARE || p.objective == PR_PAIRS_REAL || p.objective == PR_STATE) {
     if (p.n_pairs < 1 || !p.psi_init || !p.psi_targ) return fail(PR_ERR_ARG, "problem %d: pair objective needs states", idx);
+    if (p.objective == PR_STATE && p.dynamics != PR_CLOSED)
+      return fail(PR_ERR_UNSUPPORTED, "problem %d: the state objective needs closed dynamics", idx);
     if (p.n_pairs > 16 * ((p.dim + 15) / 16))
       return fail(PR_ERR_UNSUPPORTED, "problem %d: n_pairs %d exceeds the padded dimension", idx, p.n_pairs);
   } else if (p.objective == PR_TRACKING) {
@@ -747,10 +749,15 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     const bool track = q.objective == PR_TRACKING;
     const size_t MB = (size_t)NB * 256, dd = (size_t)d * d;
     D.d = d; D.B = B; D.nc = nc; D.nt = nt; D.M = M; D.dyn = q.dynamics; D.obj = q.objective;
-    D.npairs = q.objective == PR_GATE ? 0 : q.n_pairs;
+    const bool gate = q.objective == PR_GATE || q.objective == PR_GATE_PROPAGATORS;
+    D.npairs = gate ? 0 : q.n_pairs;
     D.h0pm = q.H0_per_member ? 1 : 0;
     D.unitary = q.dynamics == PR_CLOSED ? 1 : 0;
     D.grad_exact = pl.opt.grad_mode == PR_GRAD_EXACT;
+    // compute_grape_gradient's first-order term sits after U_k; the MR kernels'
+    // in place of it
+    D.after = q.dynamics == PR_CLOSED &&
+              (q.objective == PR_GATE || q.objective == PR_GATE_PROPAGATORS || q.objective == PR_STATE);
     D.cre = q.dynamics == PR_CLOSED ? 0.0 : 1.0;
     D.cim = q.dynamics == PR_CLOSED ? -1.0 : 0.0;
     // time segmentation: about 8192 wavefronts of work over all problems of
@@ -887,7 +894,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     if (q.ctrl_scale) std::copy(q.ctrl_scale, q.ctrl_scale + (size_t)M * nc, sc.begin());
     if ((rc = pl.upload(&dd_, sc.data(), sc.size()))) return rc;
     D.scale = dd_;
-    std::vector<double> wt(M, q.objective == PR_GATE ? 1.0 / M : 1.0 / ((double)M * q.n_pairs));
+    std::vector<double> wt(M, gate ? 1.0 / M : 1.0 / ((double)M * q.n_pairs));
     if (q.weight) std::copy(q.weight, q.weight + M, wt.begin());
     if ((rc = pl.upload(&dd_, wt.data(), M))) return rc;
     D.weight = dd_;
@@ -895,7 +902,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     if (q.dt_slices) std::copy(q.dt_slices, q.dt_slices + nt, dts.begin());
     if ((rc = pl.upload(&dd_, dts.data(), nt))) return rc;
     D.dts = dd_;
-    if (q.objective == PR_GATE) {
+    if (gate) {
       tile_image(q.U_target, d, B, false, img.data());
       if ((rc = pl.upload(&dp, img.data(), MB))) return rc;
       D.Ut = dp;
@@ -1088,8 +1095,9 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
   for (int p = 0; p < n_probs; p++) {
     if (probs[p].dim > 64 && o.grad_mode == PR_GRAD_EXACT)
       return fail(PR_ERR_UNSUPPORTED, "problem %d: the exact gradient is available for dim <= 64", p);
-    if (probs[p].objective == PR_TRACKING && o.grad_mode == PR_GRAD_EXACT)
-      return fail(PR_ERR_UNSUPPORTED, "problem %d: the tracking objective has the reference gradient only", p);
+    if ((probs[p].objective == PR_TRACKING || probs[p].objective == PR_GATE_PROPAGATORS) &&
+        o.grad_mode == PR_GRAD_EXACT)
+      return fail(PR_ERR_UNSUPPORTED, "problem %d: objective %d has the reference gradient only", p, probs[p].objective);
     int rc = validate(probs[p], p);
     if (rc) return rc;
   }

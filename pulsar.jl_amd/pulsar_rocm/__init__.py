@@ -19,4 +19,4 @@ from .mr import (FidelityForward, LindbladMRControl, MRControl, TrackingPoint,  
                  build_control_liouvillian, build_drift_liouvillian, fidelity_forward, grape_lindblad_kernel,
                  grape_state_kernel, grape_tracking_kernel, pure_state_to_vec_rho, vec_rho)
 from .problem import (PR_CLOSED, PR_FP64, PR_GATE, PR_GRAD_EXACT, PR_GRAD_REFERENCE,  # noqa: F401
-                      PR_LINDBLAD, PR_MIXED, PR_PAIRS_REAL, PR_PAIRS_SQUARE, PR_TRACKING, Problem)
+                      PR_LINDBLAD, PR_MIXED, PR_PAIRS_REAL, PR_PAIRS_SQUARE, PR_STATE, PR_TRACKING, Problem)

@@ -35,7 +35,7 @@ import numpy as np
 
 from . import native
 from .problem import (PR_CLOSED, PR_FP64, PR_GATE, PR_GRAD_EXACT, PR_GRAD_REFERENCE, PR_LINDBLAD,
-                      PR_PAIRS_REAL, PR_PAIRS_SQUARE, Problem)
+                      PR_PAIRS_REAL, PR_PAIRS_SQUARE, PR_STATE, Problem)
 
 # --------------------------------------------------------------------------- types
 
@@ -237,7 +237,7 @@ def _state_problem(system, n_t, dt, psi0, psit, normalize: bool) -> Problem:
         psi0 = psi0 / np.linalg.norm(psi0)
         psit = psit / np.linalg.norm(psit)
     return Problem(dim=system.dim, n_ctrl=system.n_controls, n_t=n_t, H0=system.H_drift,
-                   Hc=np.stack(system.H_controls), dt=dt, dynamics=PR_CLOSED, objective=PR_PAIRS_SQUARE,
+                   Hc=np.stack(system.H_controls), dt=dt, dynamics=PR_CLOSED, objective=PR_STATE,
                    n_members=1, psi_init=psi0[None, :], psi_targ=psit[None, :], weight=np.ones(1))
 
 
@@ -686,7 +686,7 @@ def ensemble_problems(systems, target: QuantumTarget, n_t: int, dt: float, in_op
             psit = target.target_state / np.linalg.norm(target.target_state)
             psi_i = target.target_state if (in_optimizer or target.initial_state is None) else target.initial_state
             psi_i = psi_i / np.linalg.norm(psi_i)
-            probs.append(Problem(objective=PR_PAIRS_SQUARE, psi_init=psi_i[None], psi_targ=psit[None], **common))
+            probs.append(Problem(objective=PR_STATE, psi_init=psi_i[None], psi_targ=psit[None], **common))
     return probs
 
 

@@ -18,7 +18,7 @@ from typing import Optional
 import numpy as np
 
 PR_CLOSED, PR_LINDBLAD = 0, 1
-PR_GATE, PR_PAIRS_SQUARE, PR_PAIRS_REAL, PR_TRACKING = 0, 1, 2, 3
+PR_GATE, PR_PAIRS_SQUARE, PR_PAIRS_REAL, PR_TRACKING, PR_GATE_PROPAGATORS, PR_STATE = 0, 1, 2, 3, 4, 5
 PR_FP64, PR_MIXED = 0, 1
 PR_GRAD_REFERENCE, PR_GRAD_EXACT = 0, 1
 
@@ -93,7 +93,7 @@ class Problem:
             raise ValueError("weight must be [n_members]")
         if self.dt_slices is not None and np.asarray(self.dt_slices).shape != (self.n_t,):
             raise ValueError("dt_slices must be [n_t]")
-        if self.objective == PR_GATE:
+        if self.objective in (PR_GATE, PR_GATE_PROPAGATORS):
             if self.U_target is None or np.asarray(self.U_target).shape != (d, d):
                 raise ValueError("gate objective needs U_target (dim x dim)")
         elif self.objective == PR_TRACKING:

@@ -26,7 +26,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .problem import (PR_CLOSED, PR_GATE, PR_LINDBLAD, PR_PAIRS_REAL, PR_PAIRS_SQUARE,
+from .problem import (PR_CLOSED, PR_GATE, PR_LINDBLAD, PR_PAIRS_REAL, PR_PAIRS_SQUARE, PR_STATE,
                       PR_TRACKING, Problem)
 
 SX = np.array([[0, 1], [1, 0]], dtype=np.complex128)
@@ -71,7 +71,7 @@ def single_qubit_state(seed: int = 42, psi_init=None):
     tgt = np.array([[0, 1]], dtype=np.complex128)
     ini = tgt if psi_init is None else np.asarray(psi_init, np.complex128).reshape(1, 2)
     p = Problem(dim=2, n_ctrl=2, n_t=n_t, H0=H0, Hc=Hc, dt=dt, dynamics=PR_CLOSED,
-                objective=PR_PAIRS_SQUARE, n_members=1, psi_init=ini, psi_targ=tgt,
+                objective=PR_STATE, n_members=1, psi_init=ini, psi_targ=tgt,
                 weight=np.ones(1))
     return p, u
 

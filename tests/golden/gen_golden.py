@@ -36,7 +36,8 @@ def haar(d, rng):
 def gate_reference(H0s, Hc, scales, u, dt, Ut):
     """Ensemble mean of F = |Tr(Ut^dag U)/d|^2 and of the first-order gradient
     G[j,k] = 2 Re(conj(Phi) (-i dt) Tr(P_k Ut^dag R_k H_j)/d), P_k = U_{k-1}..U_1,
-    R_k = U_n..U_{k+1}."""
+    R_k = Q_k^dag = U_n..U_k: the backward propagators of
+    src/Computation/Propagators.jl:375-380 include slice k's own U_k."""
     M = len(H0s)
     nc, nt = u.shape
     d = Ut.shape[0]
@@ -51,7 +52,7 @@ def gate_reference(H0s, Hc, scales, u, dt, Ut):
         G = np.zeros((nc, nt))
         for k in range(nt):
             R = np.eye(d)
-            for kk in range(k + 1, nt):
+            for kk in range(k, nt):
                 R = Us[kk] @ R
             A = P[k] @ Ut.conj().T @ R
             for j in range(nc):

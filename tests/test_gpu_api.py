@@ -259,7 +259,10 @@ def test_device_grape_loop_matches_host_loop(case):
     assert dev.converged == host.converged and dev.termination_reason.split()[0] == host.termination_reason.split()[0]
     assert len(dev.fidelity_history) == len(host.fidelity_history)
     assert np.abs(np.array(dev.fidelity_history) - np.array(host.fidelity_history)).max() <= 1e-13
-    assert np.allclose(dev.gradient_norm_history, host.gradient_norm_history, rtol=1e-12, atol=0)
+    # |G| is summed on the device in one loop and by numpy in the other; near
+    # the optimum it falls to rounding level, so the tolerance has an absolute floor
+    gn = np.asarray(host.gradient_norm_history)
+    assert np.allclose(dev.gradient_norm_history, gn, rtol=1e-12, atol=1e-12 * gn.max())
     assert np.abs(dev.controls - host.controls).max() <= 1e-12 * max(1.0, np.abs(host.controls).max())
     assert abs(dev.fidelity - host.fidelity) <= 1e-13
     assert dev.n_fidelity_evaluations == host.n_fidelity_evaluations

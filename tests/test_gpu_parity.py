@@ -448,3 +448,32 @@ def test_closed_and_lindblad_problems_in_one_single_block_group():
         assert np.abs(G[0] - G0).max() <= TOL_G * np.abs(G0).max()
         assert abs(F[1] - float(FIX["lind_F"])) < 1e-13
         assert np.abs(G[1] - FIX["lind_G"]).max() < 1e-12
+
+
+@pytest.mark.parametrize("d,M,nt,npairs", [(2, 1, 100, 1), (7, 3, 40, 2), (16, 4, 60, 3), (24, 2, 30, 2), (40, 1, 12, 1)])
+def test_state_objective_compute_grape_gradient_convention(d, M, nt, npairs):
+    # PR_STATE: compute_grape_gradient with a state target (Gradients.jl:357-399),
+    # the first-order term after U_k; the same states under the MR convention
+    # (PR_PAIRS_SQUARE) give the same F and a different gradient
+    from pulsar_rocm.problem import PR_STATE
+    rng = np.random.default_rng(d + nt)
+    H0 = np.stack([W.random_hermitian(d, rng, 3.0) for _ in range(M)])
+    Hc = np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(3)])
+    psi_i = np.stack([_random_state(d, rng) for _ in range(npairs)])
+    psi_t = np.stack([_random_state(d, rng) for _ in range(npairs)])
+    u = rng.uniform(-2, 2, (3, nt))
+    p = Problem(dim=d, n_ctrl=3, n_t=nt, H0=H0, Hc=Hc, dt=0.1, objective=PR_STATE, n_members=M,
+                ctrl_scale=rng.uniform(0.9, 1.1, (M, 3)), psi_init=psi_i, psi_targ=psi_t)
+    F, G, _ = check(p, u)
+    q = Problem(**{**p.__dict__, "objective": PR_PAIRS_SQUARE, "_keep": []})
+    Fq, Gq, _ = run(q, u)
+    assert abs(Fq - F) <= 1e-13 and np.abs(Gq - G).max() > 1e-6 * np.abs(G).max()
+
+
+def test_state_objective_mixed_precision():
+    from pulsar_rocm.problem import PR_MIXED, PR_STATE
+    p, u = W.single_qubit_state(psi_init=[1, 0])
+    assert p.objective == PR_STATE
+    F0, G0 = O.evaluate(p, u)
+    F, G, _ = run(p, u, precision=PR_MIXED)
+    assert abs(F - F0) <= 1e-9 and np.abs(G - G0).max() <= 1e-6 * np.abs(G0).max()

@@ -111,8 +111,10 @@ def test_lindblad_fixture():
 
 
 def test_gate_gradient_single_slice_closed_form():
-    # n_t = 1: P_1 = I and R_1 = I, so G_j = 2 Re(conj(Phi) (-i dt) Tr(Ut^dag H_j)/d),
-    # independent of U_1 except through Phi (a P_{k+1} = U_1 P_k mix-up fails here)
+    # n_t = 1: P_1 = I and Q_1 = U_1^dag (Propagators.jl:375-380 builds Q[k] from
+    # propagators[k], so Q_k^dag = U_n ... U_k includes slice k), hence
+    # G_j = 2 Re(conj(Phi) (-i dt) Tr(Ut^dag U_1 H_j)/d). Leaving U_1 out, as the
+    # docstring of compute_backward_propagators reads, fails here.
     rng = np.random.default_rng(5)
     d = 4
     H0 = W.random_hermitian(d, rng, 3.0)
@@ -126,7 +128,7 @@ def test_gate_gradient_single_slice_closed_form():
     Phi = np.trace(Ut.conj().T @ U1) / d
     assert abs(F - abs(Phi) ** 2) < 1e-15
     for j in range(2):
-        g = 2 * np.real(np.conj(Phi) * (-1j * dt) * np.trace(Ut.conj().T @ Hc[j]) / d)
+        g = 2 * np.real(np.conj(Phi) * (-1j * dt) * np.trace(Ut.conj().T @ U1 @ Hc[j]) / d)
         assert abs(G[j, 0] - g) < 1e-15
 
 
@@ -279,3 +281,57 @@ def test_gate_fidelity_bounds_and_phase():
     for uu in (0.0, 0.7, 2.5):
         F, _ = O.evaluate(p, np.full((1, 1), uu), grad=False)
         assert abs(F - ref) < 1e-14 and 0 <= F <= 1
+
+
+def test_gate_gradient_follows_the_backward_propagator_loop():
+    # the reference's loop, restated literally on n_t = 3: Q[n+1] = I,
+    # Q[k] = U_k^dag Q[k+1]; A_k = P_k Ut^dag Q_k^dag; G = 2 Re(conj(Phi) (-i dt) Tr(A_k H_j)/d)
+    rng = np.random.default_rng(8)
+    d, nt, dt = 3, 3, 0.2
+    H0 = W.random_hermitian(d, rng, 2.0)
+    Hc = np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)])
+    Ut = W.haar_unitary(d, rng)
+    u = rng.uniform(-1, 1, (2, nt))
+    p = Problem(dim=d, n_ctrl=2, n_t=nt, H0=H0, Hc=Hc, dt=dt, objective=PR_GATE, U_target=Ut)
+    _, G = O.evaluate(p, u)
+    Us = [O.expm_neg_i(H0 + u[0, k] * Hc[0] + u[1, k] * Hc[1], dt) for k in range(nt)]
+    P = [np.eye(d)]
+    for k in range(nt):
+        P.append(Us[k] @ P[k])
+    Q = [None] * (nt + 1)
+    Q[nt] = np.eye(d)
+    for k in range(nt - 1, -1, -1):
+        Q[k] = Us[k].conj().T @ Q[k + 1]
+    Phi = np.trace(Ut.conj().T @ P[nt]) / d
+    for k in range(nt):
+        A = P[k] @ Ut.conj().T @ Q[k].conj().T
+        for j in range(2):
+            g = 2 * np.real(np.conj(Phi) * (-1j * dt) * np.trace(A @ Hc[j]) / d)
+            assert abs(G[j, k] - g) < 1e-14
+
+
+def test_state_gradient_follows_gradients_jl():
+    # PR_STATE: _state_gradient_into! (Gradients.jl:357-399) with lambda_k = Q_k psi_t,
+    # slice k included; n_t = 2 restated literally
+    from pulsar_rocm.problem import PR_STATE
+    rng = np.random.default_rng(9)
+    d, nt, dt = 4, 2, 0.3
+    H0 = W.random_hermitian(d, rng, 2.0)
+    Hc = np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)])
+    psi_i = W.random_state(d, rng)
+    psi_t = W.random_state(d, rng)
+    u = rng.uniform(-1, 1, (2, nt))
+    p = Problem(dim=d, n_ctrl=2, n_t=nt, H0=H0, Hc=Hc, dt=dt, objective=PR_STATE,
+                psi_init=psi_i[None], psi_targ=psi_t[None])
+    F, G = O.evaluate(p, u)
+    Us = [O.expm_neg_i(H0 + u[0, k] * Hc[0] + u[1, k] * Hc[1], dt) for k in range(nt)]
+    Ut = Us[1] @ Us[0]
+    chi = psi_t.conj() @ Ut @ psi_i
+    assert abs(F - abs(chi) ** 2) < 1e-14
+    Pk = [np.eye(d), Us[0]]
+    Qk = [Us[0].conj().T @ Us[1].conj().T, Us[1].conj().T]
+    for k in range(nt):
+        lam, phi = Qk[k] @ psi_t, Pk[k] @ psi_i
+        for j in range(2):
+            g = 2 * np.real(np.conj(chi) * (-1j * dt) * (lam.conj() @ Hc[j] @ phi))
+            assert abs(G[j, k] - g) < 1e-14
