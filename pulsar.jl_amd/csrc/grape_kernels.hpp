@@ -1093,7 +1093,6 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
         contract<B, V>(P, m, kk, Cm, LA, x);
       }
     };
-    if (aft) contract_c(it.k0, dtk);
     Blk Ut;
     if (RECOMP && P.recompute)
       Ut = recompute_ut<B>(P, m, it.k0, u, LA, LB, red, x);
@@ -1114,6 +1113,8 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
         else
           Un = load_blk(P.Ust + ((size_t)m * P.nt + k + 1) * MB, x);
       }
+      // C_k is contracted while M_k = C_k U_k^dag is on the matrix cores
+      if (aft) contract_c(k, dtk);
       if constexpr (B == 1)
         Cm = mm<1, T, N>(Mk, Ut, LA, LB, x);      // M^T U^T = (U_k M_k)^T
       else
@@ -1121,7 +1122,6 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
       if constexpr (EXACT) {
         contract<B, V>(P, m, k, exact_direction<B>(P, m, k, u, H0, Mk, LA, LB, red, x), LA, x);
       } else if (aft) {
-        if (k + 1 < it.k1) contract_c(k + 1, dtn);  // C_{k+1}
       } else if constexpr (REG && B == 1) {
         contract_regs<V, false>(P, m, k, dtk, Mk, LA, tregs, x);
       } else {
@@ -1281,6 +1281,8 @@ __global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ 
       if (nsmall) D = load_blk32(P.Udm + ((size_t)m * P.nt + k + 1) * 256, x);
     }
     if (P.after) {
+      // C_{k+1} for the next slice, from the FP64 accumulators just formed
+      // (C_{k0} was contracted before the walk)
       if (k + 1 < it.k1) contract_c(k + 1, dtn);
     } else if constexpr (REG) {
       contract_regs<V, true>(P, m, k, dtk, Mk, LA, tregs, x);
