@@ -20,6 +20,8 @@ const PR_GATE = Cint(0)
 const PR_PAIRS_SQUARE = Cint(1)
 const PR_PAIRS_REAL = Cint(2)
 const PR_TRACKING = Cint(3)
+const PR_GATE_PROPAGATORS = Cint(4)
+const PR_STATE = Cint(5)   # compute_grape_gradient with a state target (Gradients.jl:357-399)
 
 # Memory layout identical to `pr_problem` (include/pulsar_rocm.h).
 struct PRProblem
@@ -168,7 +170,7 @@ function rocm_problem(H0s, Hc::Vector{<:AbstractMatrix}, n_t::Int, dt::Float64, 
         end
         push!(roots, ψi, ψt)
         w === nothing && (w = ones(M) ./ M; push!(roots, w))
-        prob = PRProblem(d, length(Hc), n_t, dt, C_NULL, PR_CLOSED, PR_PAIRS_SQUARE, M, _cptr(H0), permember,
+        prob = PRProblem(d, length(Hc), n_t, dt, C_NULL, PR_CLOSED, PR_STATE, M, _cptr(H0), permember,
                          _cptr(HcA), _cptr(sc), _cptr(w), C_NULL, 1, _cptr(ψi), _cptr(ψt), 0, C_NULL, C_NULL, C_NULL)
     end
     return prob, roots
