@@ -17,6 +17,6 @@ from .verify import (AutoDiffConfig, compute_gradient_autodiff, finite_diff_grad
                      verify_gradient, verify_gradient_autodiff)
 from .mr import (FidelityForward, LindbladMRControl, MRControl, TrackingPoint,  # noqa: F401
                  build_control_liouvillian, build_drift_liouvillian, fidelity_forward, grape_lindblad_kernel,
-                 grape_state_kernel, grape_tracking_kernel, pure_state_to_vec_rho, vec_rho)
+                 grape_state_kernel, grape_tracking_kernel, optimcon, pure_state_to_vec_rho, vec_rho)
 from .problem import (PR_CLOSED, PR_FP64, PR_GATE, PR_GRAD_EXACT, PR_GRAD_REFERENCE,  # noqa: F401
                       PR_LINDBLAD, PR_MIXED, PR_PAIRS_REAL, PR_PAIRS_SQUARE, PR_STATE, PR_TRACKING, Problem)

@@ -8,6 +8,7 @@
                                           src/Physics/Lindblad.jl:89-197
   TrackingPoint / grape_tracking_kernel   src/Application/MR/GRAPETracking.jl:89-157
   fidelity_forward                        src/Application/MR/GRAPEState.jl:406-566
+  optimcon (:lbfgs, :grape; tracking)     src/Application/MR/OptControl.jl:310-611, 1040-1236
 
 Ensemble members are the (drift, power level) pairs in the reference's order
 (`for H in drifts for p in pwr_levels`, GRAPEState.jl:146); every member
@@ -94,6 +95,13 @@ class MRControl:
     backend: str = "rocm"
     precision: str = "f64"
     tracking: List[TrackingPoint] = field(default_factory=list)
+    # optimcon settings (OptControl.jl:210-220 defaults)
+    l_bound: float = -1.0
+    u_bound: float = 1.0
+    method: str = "lbfgs"
+    max_iter: int = 500
+    grad_tol: float = 1e-7
+    lbfgs_memory: int = 10
 
     def tracking_problem(self) -> Problem:
         """The tracking objective over the same ensemble: initial states
@@ -139,6 +147,13 @@ class LindbladMRControl:
     pwr_levels: List[float] = field(default_factory=lambda: [1.0])
     fidelity: str = "square"
     backend: str = "rocm"
+    l_bound: float = -1.0
+    u_bound: float = 1.0
+    method: str = "lbfgs"
+    max_iter: int = 500
+    grad_tol: float = 1e-7
+    lbfgs_memory: int = 10
+    tracking: List[TrackingPoint] = field(default_factory=list)  # always empty (Hilbert-space feature)
 
     def problem(self) -> Problem:
         Ld = [build_drift_liouvillian(h, self.jump_ops, self.decay_rates) for h in self.drifts]
@@ -252,3 +267,178 @@ def fidelity_forward(waveform, drifts, pwr_levels, operators, rho_init, rho_targ
         return ff(waveform)
     finally:
         ff.close()
+
+
+# ------------------------------------------------------------------ optimcon
+def _lbfgs_direction(g, s_list, y_list):
+    """Two-loop recursion, ascent direction (OptControl.jl:565-590)."""
+    k = len(s_list)
+    if k == 0:
+        return g.copy()
+    q = g.copy()
+    a = np.zeros(k)
+    rho = [1.0 / np.dot(y_list[i], s_list[i]) for i in range(k)]
+    for i in range(k - 1, -1, -1):
+        a[i] = rho[i] * np.dot(s_list[i], q)
+        q = q - a[i] * y_list[i]
+    gamma = np.dot(s_list[-1], y_list[-1]) / np.dot(y_list[-1], y_list[-1])
+    r = gamma * q
+    for i in range(k):
+        b = rho[i] * np.dot(y_list[i], r)
+        r = r + (a[i] - b) * s_list[i]
+    return r
+
+
+def _line_search(w, d, F0, a0, f_only, lo, hi):
+    """Armijo backtracking (OptControl.jl:594-611): c1 = 1e-4, up to 20 halvings;
+    returns (alpha, w_new, F_new, evaluations counted as the reference counts them)."""
+    a = a0
+    for halving in range(1, 21):
+        w_new = np.clip(w + a * d, lo, hi)
+        F_new = f_only(w_new)
+        if F_new >= F0 - 1e-4 * abs(a):
+            return a, w_new, F_new, halving + 1
+        a *= 0.5
+    w_new = np.clip(w + a * d, lo, hi)
+    F_new = f_only(w_new)
+    return a, (w_new if F_new > F0 else w.copy()), max(F_new, F0), 21
+
+
+def _vec(m):
+    return np.asarray(m, np.float64).reshape(-1, order="F")  # Julia's vec (column-major)
+
+
+def optimcon_lbfgs_loop(fg, f_only, guess, l_bound, u_bound, max_iter, grad_tol, lbfgs_memory, metadata=None):
+    """_optimcon_lbfgs / _optimcon_lbfgs_tracking (OptControl.jl:382-484, 1040-1139)
+    with fg(w) -> (F, G) and f_only(w) -> F."""
+    from .api import OptimizationResult
+    import time
+    t0 = time.perf_counter()
+    w = np.clip(np.array(guess, np.float64), l_bound, u_bound)
+    F_curr, G_curr = fg(w)
+    best_w, best_F = w.copy(), F_curr
+    n_fid = n_grad = 1
+    f_hist, g_hist = [F_curr], [float(np.linalg.norm(G_curr))]
+    s_list, y_list = [], []
+    converged, reason = False, "max iterations reached"
+    for _ in range(max_iter):
+        g_flat = _vec(G_curr)
+        d_flat = _lbfgs_direction(g_flat, s_list, y_list)
+        d_max = np.max(np.abs(d_flat))
+        if d_max < 1e-14:
+            converged, reason = True, "zero gradient direction"
+            break
+        d_flat = d_flat / d_max
+        d_mat = d_flat.reshape(w.shape, order="F")
+        a0 = 0.05 * (u_bound - l_bound)
+        _, w_new, _, n_ls = _line_search(w, d_mat, F_curr, a0, f_only, l_bound, u_bound)
+        n_fid += n_ls
+        F_new2, G_new = fg(w_new)
+        n_fid += 1
+        n_grad += 1
+        s_vec = _vec(w_new - w)
+        y_vec = _vec(G_new) - g_flat
+        if np.dot(y_vec, s_vec) > 1e-14 * np.linalg.norm(s_vec) ** 2:
+            s_list.append(s_vec)
+            y_list.append(y_vec)
+            if len(s_list) > lbfgs_memory:
+                s_list.pop(0)
+                y_list.pop(0)
+        w, G_curr = w_new, G_new
+        F_prev, F_curr = F_curr, F_new2
+        f_hist.append(F_curr)
+        gnorm = float(np.linalg.norm(G_curr))
+        g_hist.append(gnorm)
+        if F_curr > best_F:
+            best_F, best_w = F_curr, w.copy()
+        if max_iter > 0 and F_curr >= 1.0 - 1e-8:
+            converged, reason = True, "near-perfect fidelity"
+            break
+        if gnorm < grad_tol:
+            converged, reason = True, f"gradient norm < {grad_tol}"
+            break
+        if len(f_hist) >= 2 and abs(F_curr - F_prev) < 1e-9:
+            converged, reason = True, "fidelity stalled"
+            break
+    return OptimizationResult(best_w, best_F, f_hist, g_hist, len(f_hist) - 1, converged, reason,
+                              time.perf_counter() - t0, n_fid, n_grad, dict(metadata or {}))
+
+
+def optimcon_grape_loop(fg, guess, l_bound, u_bound, max_iter, grad_tol, metadata=None):
+    """_optimcon_grape / _optimcon_grape_tracking (OptControl.jl:488-561, 1141-1216):
+    steps of 0.05 (u_bound - l_bound) / max|G| along G, clipped."""
+    from .api import OptimizationResult
+    import time
+    t0 = time.perf_counter()
+    w = np.clip(np.array(guess, np.float64), l_bound, u_bound)
+    F_curr, G_curr = fg(w)
+    best_w, best_F = w.copy(), F_curr
+    n_fid = n_grad = 1
+    f_hist, g_hist = [F_curr], [float(np.linalg.norm(G_curr))]
+    converged, reason = False, "max iterations reached"
+    for _ in range(max_iter):
+        g_max = np.max(np.abs(G_curr))
+        if g_max > 1e-14:
+            w = np.clip(w + (0.05 * (u_bound - l_bound) / g_max) * G_curr, l_bound, u_bound)
+        F_curr, G_curr = fg(w)
+        n_fid += 1
+        n_grad += 1
+        f_hist.append(F_curr)
+        gnorm = float(np.linalg.norm(G_curr))
+        g_hist.append(gnorm)
+        if F_curr > best_F:
+            best_F, best_w = F_curr, w.copy()
+        if F_curr >= 1.0 - 1e-8:
+            converged, reason = True, "near-perfect fidelity"
+            break
+        if gnorm < grad_tol:
+            converged, reason = True, f"gradient norm < {grad_tol}"
+            break
+    return OptimizationResult(best_w, best_F, f_hist, g_hist, len(f_hist) - 1, converged, reason,
+                              time.perf_counter() - t0, n_fid, n_grad, dict(metadata or {}))
+
+
+def _validate(ctrl, guess):
+    """_validate_mr_control (OptControl.jl:615-650), the shape checks."""
+    for name in ("drifts", "operators", "rho_init", "pwr_levels", "pulse_dt"):
+        if len(getattr(ctrl, name)) == 0:
+            raise ValueError(f"ctrl.{name} must be non-empty")
+    n_ctrl, n_t = np.asarray(guess).shape
+    if n_ctrl != len(ctrl.operators):
+        raise ValueError(f"guess has {n_ctrl} rows but ctrl.operators has {len(ctrl.operators)} entries")
+    if n_t != len(ctrl.pulse_dt):
+        raise ValueError(f"guess has {n_t} columns but ctrl.pulse_dt has {len(ctrl.pulse_dt)} entries")
+
+
+def optimcon(ctrl, guess):
+    """optimcon(ctrl, guess) for :lbfgs and :grape (OptControl.jl:310-367) with
+    every kernel evaluation on the device through one reusable plan (the
+    line search evaluates F only). Non-empty ctrl.tracking selects the
+    tracking kernel, as there."""
+    _validate(ctrl, guess)
+    method = str(ctrl.method).lstrip(":")
+    if method not in ("lbfgs", "grape"):
+        raise ValueError(f"method :{method} is not part of this library (use :lbfgs or :grape)")
+    tracking = bool(getattr(ctrl, "tracking", []))
+    kp = KernelPlan(ctrl, tracking=tracking)
+    try:
+        def fg(w):
+            F, G = kp.plan.eval([np.asarray(w, np.float64)], grad=True)
+            return float(F[0]), G[0]
+
+        def f_only(w):
+            F, _ = kp.plan.eval([np.asarray(w, np.float64)], grad=False)
+            return float(F[0])
+
+        n_ens = kp.problem.n_members * kp.problem.n_pairs
+        if method == "lbfgs":
+            meta = ({"algorithm": "MR Tracking L-BFGS-GRAPE", "n_checkpoints": len(ctrl.tracking)} if tracking else
+                    {"algorithm": ("Lindblad" if isinstance(ctrl, LindbladMRControl) else "MR") + " L-BFGS-GRAPE",
+                     "n_ensemble": n_ens})
+            meta["lbfgs_memory"] = ctrl.lbfgs_memory
+            return optimcon_lbfgs_loop(fg, f_only, guess, ctrl.l_bound, ctrl.u_bound, ctrl.max_iter,
+                                       ctrl.grad_tol, ctrl.lbfgs_memory, meta)
+        meta = {"algorithm": ("MR Tracking GRAPE" if tracking else "MR GRAPE"), "n_ensemble": n_ens}
+        return optimcon_grape_loop(fg, guess, ctrl.l_bound, ctrl.u_bound, ctrl.max_iter, ctrl.grad_tol, meta)
+    finally:
+        kp.close()

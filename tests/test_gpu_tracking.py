@@ -137,3 +137,42 @@ def test_mr_tracking_kernel_single_spin_oscillation():
 def test_four_qubit_tracking_workload_members_sample():
     p, u = W.shrink(*W.four_qubit_tracking(n_t=200), n_members=3)
     check(p, u)
+
+
+@pytest.mark.parametrize("kind", ["state_lbfgs", "state_grape", "tracking_lbfgs", "lindblad_lbfgs"])
+def test_mr_optimcon_on_device_matches_oracle_driven_loop(kind):
+    # optimcon (OptControl.jl:310-611, 1040-1236): the same loop with device
+    # evaluations and with oracle evaluations gives the same trajectory
+    Ix, Iy, Iz = W.SX / 2, W.SY / 2, W.SZ / 2
+    up, dn = np.array([1, 0], complex), np.array([0, 1], complex)
+    mIy = np.array([1, -1j], complex) / np.sqrt(2)
+    common = dict(drifts=[2 * np.pi * 300 * Iz, -2 * np.pi * 200 * Iz], operators=[Ix, Iy], rho_init=[up],
+                  rho_targ=[mIy], pulse_dt=np.full(40, 5e-6), pwr_levels=[2 * np.pi * 8e3, 2 * np.pi * 1e4],
+                  max_iter=12)
+    if kind == "lindblad_lbfgs":
+        ctrl = mr.LindbladMRControl(**common, jump_ops=[np.array([[0, 1], [0, 0]], complex)], decay_rates=[300.0])
+        p = ctrl.problem()
+        fg = lambda w: O.evaluate(p, w)  # noqa: E731
+        f = lambda w: O.evaluate(p, w, grad=False)[0]  # noqa: E731
+    elif kind == "tracking_lbfgs":
+        cps = [mr.TrackingPoint(10 * k, np.outer(dn, dn) if k % 2 else np.outer(up, up), 0.25) for k in range(1, 5)]
+        ctrl = mr.MRControl(**common, tracking=cps)
+        p = ctrl.tracking_problem()
+        fg = lambda w: O.tracking_evaluate(p, w)  # noqa: E731
+        f = lambda w: O.tracking_evaluate(p, w, grad=False)[0]  # noqa: E731
+    else:
+        ctrl = mr.MRControl(**common, method="grape" if kind == "state_grape" else "lbfgs")
+        p = ctrl.problem()
+        fg = lambda w: O.evaluate(p, w)  # noqa: E731
+        f = lambda w: O.evaluate(p, w, grad=False)[0]  # noqa: E731
+    guess = 0.2 * np.random.default_rng(4).standard_normal((2, 40))
+    dev = mr.optimcon(ctrl, guess)
+    if kind == "state_grape":
+        ref = mr.optimcon_grape_loop(fg, guess, ctrl.l_bound, ctrl.u_bound, ctrl.max_iter, ctrl.grad_tol)
+    else:
+        ref = mr.optimcon_lbfgs_loop(fg, f, guess, ctrl.l_bound, ctrl.u_bound, ctrl.max_iter, ctrl.grad_tol,
+                                     ctrl.lbfgs_memory)
+    assert dev.n_iterations == ref.n_iterations and dev.termination_reason == ref.termination_reason
+    assert dev.n_fidelity_evaluations == ref.n_fidelity_evaluations
+    assert np.abs(np.array(dev.fidelity_history) - np.array(ref.fidelity_history)).max() <= 1e-10
+    assert np.abs(dev.controls - ref.controls).max() <= 1e-7
