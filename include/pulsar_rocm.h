@@ -83,6 +83,10 @@ extern "C" {
                              with slice k included: 2 Re(conj(z) (-i dt) <lambda_k|A_j|P_k psi_i>)
                              (compute_grape_gradient with a state target; the caller
                              normalises the states as Gradients.jl:369-374 does) */
+#define PR_STATE_PROPAGATORS 6 /* closed dynamics, state branch of _grape_gradient_from_propagators
+                                  (MRPhysics.jl:337-351): F_m as PR_PAIRS_SQUARE; gradient
+                                  2 Im(conj(z) <Q_k psi_t| -i dt A_j |P_k psi_i>), states as given.
+                                  PR_GRAD_REFERENCE */
 #define PR_GATE_PROPAGATORS 4 /* F as PR_GATE; gradient of _grape_gradient_from_propagators
                                  (src/Physics/MRPhysics.jl:312-336, the MAS/DNP gate gradient):
                                  G_jk = 2 Re(conj(Phi) Tr(Q_k^dag (-i dt A_j) P_k)) / dim^2,

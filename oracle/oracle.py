@@ -292,10 +292,13 @@ def mas_propagators(Hf, Hc, omega_r_hz, u, dt, rotor_phase_0=0.0):
     return Us
 
 
-def propagator_gradient(Us, Hc, U_target, dt):
-    """_grape_gradient_from_propagators, unitary branch (MRPhysics.jl:312-336):
-    P_k = U_{k-1}..U_1, Q_k = (U_n..U_k)^dag (Propagators.jl:277-380),
-    G[j,k] = 2 Re(conj(Phi) Tr(Q_k^dag (-i dt H_j) P_k)) / d^2."""
+def propagator_gradient(Us, Hc, U_target, dt, psi_init=None, psi_targ=None):
+    """_grape_gradient_from_propagators (MRPhysics.jl:312-351):
+    P_k = U_{k-1}..U_1, Q_k = (U_n..U_k)^dag (Propagators.jl:277-380);
+    unitary branch G[j,k] = 2 Re(conj(Phi) Tr(Q_k^dag (-i dt H_j) P_k)) / d^2;
+    with psi_targ given, the state branch (:337-351):
+    G[j,k] = 2 Im(conj(ov) <Q_k psi_t| -i dt H_j |P_k psi_i>), ov = <psi_t|U psi_i>,
+    psi_i = psi_init or psi_t, states as given."""
     nt, D = len(Us), Us[0].shape[0]
     P = [np.eye(D, dtype=complex)]
     for k in range(nt):
@@ -304,8 +307,17 @@ def propagator_gradient(Us, Hc, U_target, dt):
     Q[nt] = np.eye(D, dtype=complex)
     for k in range(nt - 1, -1, -1):
         Q[k] = Us[k].conj().T @ Q[k + 1]
-    Phi = np.trace(np.asarray(U_target, complex).conj().T @ P[nt]) / D
     G = np.zeros((len(Hc), nt))
+    if psi_targ is not None:
+        pt = np.asarray(psi_targ, complex).reshape(-1)
+        pi = pt if psi_init is None else np.asarray(psi_init, complex).reshape(-1)
+        ov = np.vdot(pt, P[nt] @ pi)
+        for k in range(nt):
+            psi_k, lam_k = P[k] @ pi, Q[k] @ pt
+            for j, Hj in enumerate(Hc):
+                G[j, k] = 2 * np.imag(np.conj(ov) * np.vdot(lam_k, (-1j * dt * Hj) @ psi_k))
+        return G
+    Phi = np.trace(np.asarray(U_target, complex).conj().T @ P[nt]) / D
     for k in range(nt):
         for j, Hj in enumerate(Hc):
             inner = np.trace(Q[k].conj().T @ (-1j * dt * Hj) @ P[k])
@@ -314,7 +326,7 @@ def propagator_gradient(Us, Hc, U_target, dt):
 
 
 def mas_powder_gradient(isotopes, csa, dipolar, omega_r_hz, u, dt, U_target, orientations,
-                        rotor_phase_0: float = 0.0):
+                        rotor_phase_0: float = 0.0, psi_init=None, psi_targ=None):
     """compute_grape_gradient_powder (src/Physics/MRPhysics.jl:170-188):
     G = sum_o w_o G_o over orientations (a, b, g, w), each G_o from
     mas_fourier -> mas_propagators -> propagator_gradient with the RF controls
@@ -325,7 +337,7 @@ def mas_powder_gradient(isotopes, csa, dipolar, omega_r_hz, u, dt, U_target, ori
     for (oa, ob, og, ow) in orientations:
         Hf = mas_fourier(isotopes, csa, dipolar, (oa, ob, og))
         Us = mas_propagators(Hf, Hc, omega_r_hz, u, dt, rotor_phase_0)
-        G = G + ow * propagator_gradient(Us, Hc, U_target, dt)
+        G = G + ow * propagator_gradient(Us, Hc, U_target, dt, psi_init, psi_targ)
     return G
 
 

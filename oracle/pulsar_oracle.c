@@ -633,11 +633,15 @@ static int pairs_member(const pr_problem* p, int m, const double* u, int grad_mo
   int d = p->dim, nc = p->n_ctrl, nt = p->n_t, np = p->n_pairs;
   size_t dd = (size_t)d * d;
   int closed = p->dynamics == PR_CLOSED;
-  int square = p->objective == PR_PAIRS_SQUARE || p->objective == PR_STATE;
+  int square = p->objective == PR_PAIRS_SQUARE || p->objective == PR_STATE || p->objective == PR_STATE_PROPAGATORS;
+  /* PR_STATE_PROPAGATORS: the state branch of _grape_gradient_from_propagators
+   * (MRPhysics.jl:337-351), 2 Im(conj(z) <lambda_k| -i dt A |psi_k>), lambda_k
+   * with slice k included */
+  int helper = p->objective == PR_STATE_PROPAGATORS;
   /* PR_STATE: _state_gradient_into! (Gradients.jl:357-399), lambda_k = Q_k psi_t
    * with Q_k including U_k (Propagators.jl:375-380): <lambda[k]|A|psi[k]>;
    * the MR kernels pair lambda[k+1] with psi[k] (GRAPEState.jl:218-229) */
-  int after = p->objective == PR_STATE;
+  int after = p->objective == PR_STATE || helper;
   double F = 0;
   if (Gm) memset(Gm, 0, sizeof(double) * nc * nt);
   cx* zs = (cx*)malloc(sizeof(cx) * np);
@@ -664,7 +668,8 @@ static int pairs_member(const pr_problem* p, int m, const double* u, int grad_mo
         mv(d, W->Hs + j * dd, pv, W->tmpv);
         cx inner = vdot(d, lv, W->tmpv);
         double g;
-        if (closed) g = square ? 2.0 * dtk * cimag(conj(z) * inner) : dtk * cimag(inner);
+        if (helper) g = 2.0 * cimag(conj(z) * (-I * dtk) * inner);
+        else if (closed) g = square ? 2.0 * dtk * cimag(conj(z) * inner) : dtk * cimag(inner);
         else g = square ? 2.0 * dtk * creal(conj(z) * inner) : dtk * creal(inner);
         Gm[j + (size_t)k * nc] += g;
       }
@@ -748,7 +753,7 @@ static int check_problem(const pr_problem* p) {
   if (is_gate(p)) {
     if (!p->U_target || p->dynamics != PR_CLOSED) return PR_ERR_ARG;
   } else if (p->objective == PR_PAIRS_SQUARE || p->objective == PR_PAIRS_REAL ||
-             (p->objective == PR_STATE && p->dynamics == PR_CLOSED)) {
+             ((p->objective == PR_STATE || p->objective == PR_STATE_PROPAGATORS) && p->dynamics == PR_CLOSED)) {
     if (p->n_pairs < 1 || !p->psi_init || !p->psi_targ) return PR_ERR_ARG;
   } else {
     return PR_ERR_ARG;

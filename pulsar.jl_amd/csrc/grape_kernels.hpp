@@ -818,7 +818,7 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
       if (rowof(x.q, r) == x.c) zb[16 * x.I + x.c] = make_double2(Z.re[r], Z.im[r]);
   }
   __syncthreads();
-  const bool square = (P.obj == 1 || P.obj == 5);
+  const bool square = (P.obj == 1 || P.obj == 5 || P.obj == 6);
   if (threadIdx.x == 0) {
     double F = 0;
     for (int s = 0; s < P.npairs; s++) {
@@ -827,10 +827,13 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
     }
     P.Fm[m] = F;
   }
-  // w_s = 2 z_s (square) or 1 (real); zero beyond the pairs
+  // w_s = 2 z_s (square) or 1 (real); zero beyond the pairs. The propagator
+  // helper's state branch (objective 6) takes Im where the others take Re:
+  // w_s = 2 i z_s
   auto wgt = [&](int s) -> double2 {
     if (s >= P.npairs) return make_double2(0, 0);
     double2 z = zb[s];
+    if (P.obj == 6) return make_double2(-2 * z.y, 2 * z.x);
     return square ? make_double2(2 * z.x, 2 * z.y) : make_double2(1, 0);
   };
   if (P.unitary) {

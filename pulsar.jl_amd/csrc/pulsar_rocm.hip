@@ -367,9 +367,10 @@ int validate(const pr_problem& p, int idx) {
     if (p.dynamics != PR_CLOSED) return fail(PR_ERR_UNSUPPORTED, "problem %d: gate objective needs closed dynamics", idx);
     double dev = unitary_dev(p.U_target, p.dim);
     if (dev > 1e-8) return fail(PR_ERR_ARG, "problem %d: target is not unitary: ||U^dag U - I||/dim = %g > 1e-8", idx, dev);
-  } else if (p.objective == PR_PAIRS_SQUARE || p.objective == PR_PAIRS_REAL || p.objective == PR_STATE) {
+  } else if (p.objective == PR_PAIRS_SQUARE || p.objective == PR_PAIRS_REAL || p.objective == PR_STATE ||
+             p.objective == PR_STATE_PROPAGATORS) {
     if (p.n_pairs < 1 || !p.psi_init || !p.psi_targ) return fail(PR_ERR_ARG, "problem %d: pair objective needs states", idx);
-    if (p.objective == PR_STATE && p.dynamics != PR_CLOSED)
+    if ((p.objective == PR_STATE || p.objective == PR_STATE_PROPAGATORS) && p.dynamics != PR_CLOSED)
       return fail(PR_ERR_UNSUPPORTED, "problem %d: the state objective needs closed dynamics", idx);
     if (p.n_pairs > 16 * ((p.dim + 15) / 16))
       return fail(PR_ERR_UNSUPPORTED, "problem %d: n_pairs %d exceeds the padded dimension", idx, p.n_pairs);
@@ -757,7 +758,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     // compute_grape_gradient's first-order term sits after U_k; the MR kernels'
     // in place of it
     D.after = q.dynamics == PR_CLOSED &&
-              (q.objective == PR_GATE || q.objective == PR_GATE_PROPAGATORS || q.objective == PR_STATE);
+              (q.objective == PR_GATE || q.objective == PR_GATE_PROPAGATORS || q.objective == PR_STATE ||
+               q.objective == PR_STATE_PROPAGATORS);
     D.cre = q.dynamics == PR_CLOSED ? 0.0 : 1.0;
     D.cim = q.dynamics == PR_CLOSED ? -1.0 : 0.0;
     // time segmentation: about 8192 wavefronts of work over all problems of
@@ -1095,7 +1097,8 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
   for (int p = 0; p < n_probs; p++) {
     if (probs[p].dim > 64 && o.grad_mode == PR_GRAD_EXACT)
       return fail(PR_ERR_UNSUPPORTED, "problem %d: the exact gradient is available for dim <= 64", p);
-    if ((probs[p].objective == PR_TRACKING || probs[p].objective == PR_GATE_PROPAGATORS) &&
+    if ((probs[p].objective == PR_TRACKING || probs[p].objective == PR_GATE_PROPAGATORS ||
+         probs[p].objective == PR_STATE_PROPAGATORS) &&
         o.grad_mode == PR_GRAD_EXACT)
       return fail(PR_ERR_UNSUPPORTED, "problem %d: objective %d has the reference gradient only", p, probs[p].objective);
     int rc = validate(probs[p], p);

@@ -6,7 +6,7 @@
   rotate_spin_system, build_mas_hamiltonian
                                           src/Types/MASSpinSystem.jl:16-199
   compute_grape_gradient (MAS), compute_grape_gradient_powder
-                                          src/Physics/MRPhysics.jl:150-188, 312-336
+                                          src/Physics/MRPhysics.jl:150-188, 312-351
 
 The MAS drift is H(t) = sum_{m=-2..2} exp(i m w_r t) H_m, evaluated at the
 slice midpoints t_k = (k - 1/2) dt + phi0 / w_r (MASPropagators.jl:14-35).
@@ -20,9 +20,10 @@ th_k = w_r t_k. On the device this is one plan problem: the members are the
 powder orientations (weights w, drift H_0 of that orientation), the operator
 list is the RF controls followed by four rotor channels per O_b (cos th, sin th,
 cos 2th, sin 2th) whose waveform rows are fixed and whose per-member scales
-are the coefficients above. The objective is PR_GATE_PROPAGATORS, the
-gradient convention of _grape_gradient_from_propagators (MRPhysics.jl:312-336);
-the gradient rows of the rotor channels are discarded.
+are the coefficients above. The objective is PR_GATE_PROPAGATORS (unitary
+targets) or PR_STATE_PROPAGATORS (state targets), the two branches of
+_grape_gradient_from_propagators (MRPhysics.jl:312-351); the gradient rows of
+the rotor channels are discarded.
 """
 from __future__ import annotations
 
@@ -32,7 +33,7 @@ from typing import Dict, List, Sequence, Tuple
 import numpy as np
 
 from . import native
-from .problem import PR_CLOSED, PR_GATE_PROPAGATORS, Problem
+from .problem import PR_CLOSED, PR_GATE_PROPAGATORS, PR_STATE_PROPAGATORS, Problem
 
 # nuclear spin quantum numbers (NMRSpinSystem.jl:48-61)
 SPIN_QUANTUM_NUMBER = {"1H": 0.5, "2H": 1.0, "13C": 0.5, "14N": 1.0, "15N": 0.5, "17O": 2.5, "19F": 0.5,
@@ -196,7 +197,24 @@ def build_mas_hamiltonian(sys: MASSpinSystem, t: float) -> np.ndarray:
     return sum(np.exp(1j * m * sys.omega_r * t) * sys.H_fourier[m] for m in range(-2, 3))
 
 
-def mas_powder_problem(sys: MASSpinSystem, controls, U_target, orientations, rotor_phase_0: float = 0.0):
+def _target_fields(target):
+    """Objective fields of a QuantumTarget (or a bare unitary matrix) for the
+    helper: U_t for a unitary target; psi_t and psi_i = initial_state or
+    psi_t, as given (MRPhysics.jl:338-339), for a state target."""
+    if isinstance(target, np.ndarray):
+        return dict(objective=PR_GATE_PROPAGATORS, U_target=np.asarray(target, np.complex128))
+    kind = getattr(target, "type", "unitary")
+    if kind == "unitary" and getattr(target, "target_unitary", None) is not None:
+        return dict(objective=PR_GATE_PROPAGATORS, U_target=np.asarray(target.target_unitary, np.complex128))
+    if kind == "state" and getattr(target, "target_state", None) is not None:
+        psit = np.asarray(target.target_state, np.complex128).reshape(-1)
+        init = getattr(target, "initial_state", None)
+        psii = psit if init is None else np.asarray(init, np.complex128).reshape(-1)
+        return dict(objective=PR_STATE_PROPAGATORS, psi_init=psii[None, :], psi_targ=psit[None, :])
+    raise ValueError("the MAS gradient needs a unitary or a state target")
+
+
+def mas_powder_problem(sys: MASSpinSystem, controls, target, orientations, rotor_phase_0: float = 0.0):
     """(Problem, extended controls) for the powder average: one member per
     orientation (alpha, beta, gamma, weight); see the module docstring."""
     ori = [tuple(float(v) for v in o) for o in orientations]
@@ -222,22 +240,15 @@ def mas_powder_problem(sys: MASSpinSystem, controls, U_target, orientations, rot
         for t, (_, co) in enumerate(mem.terms):
             scale[i, nc + 4 * t:nc + 4 * t + 4] = [2 * co[1].real, -2 * co[1].imag, 2 * co[2].real, -2 * co[2].imag]
     p = Problem(dim=sys.dim, n_ctrl=len(ops), n_t=nt, H0=np.stack(H0), Hc=np.stack(ops), dt=dt,
-                dynamics=PR_CLOSED, objective=PR_GATE_PROPAGATORS, n_members=len(ori), ctrl_scale=scale,
-                weight=np.array([o[3] for o in ori]), U_target=np.asarray(U_target, np.complex128))
+                dynamics=PR_CLOSED, n_members=len(ori), ctrl_scale=scale,
+                weight=np.array([o[3] for o in ori]), **_target_fields(target))
     return p, np.ascontiguousarray(u_ext)
-
-
-def _unitary_of(target):
-    if getattr(target, "type", "unitary") != "unitary" or getattr(target, "target_unitary", None) is None:
-        raise ValueError("the :rocm MAS gradient covers unitary targets (state targets: "
-                         "MRPhysics.jl:338-351 is not part of this library)")
-    return target.target_unitary
 
 
 def compute_grape_gradient_powder(sys: MASSpinSystem, controls, target, orientations) -> np.ndarray:
     """sum_o w_o G(rotate_spin_system(sys, alpha_o, beta_o, gamma_o)) with the
     propagator-helper gradient (MRPhysics.jl:170-188), [n_controls x n_t]."""
-    p, u = mas_powder_problem(sys, controls, _unitary_of(target), orientations)
+    p, u = mas_powder_problem(sys, controls, target, orientations)
     plan = native.Plan([p])
     try:
         _, G = plan.eval([u])
@@ -249,7 +260,7 @@ def compute_grape_gradient_powder(sys: MASSpinSystem, controls, target, orientat
 def compute_grape_gradient_mas(sys: MASSpinSystem, controls, target, rotor_phase_0: float = 0.0) -> np.ndarray:
     """compute_grape_gradient(sys::MASSpinSystem, ...) (MRPhysics.jl:155-162):
     one orientation, the system as given."""
-    p, u = mas_powder_problem(sys, controls, _unitary_of(target), [(0.0, 0.0, 0.0, 1.0)], rotor_phase_0)
+    p, u = mas_powder_problem(sys, controls, target, [(0.0, 0.0, 0.0, 1.0)], rotor_phase_0)
     plan = native.Plan([p])
     try:
         _, G = plan.eval([u])

@@ -63,3 +63,20 @@ def test_mas_single_orientation_with_rotor_phase():
     Us = O.mas_propagators(Hf, [Ix[0], Iy[0]], 10e3, u, 1e-6, rotor_phase_0=0.9)
     G0 = O.propagator_gradient(Us, [Ix[0], Iy[0]], Ut, 1e-6)
     assert np.abs(G - G0).max() <= 1e-10 * np.abs(G0).max()
+
+
+def test_mas_state_target_matches_oracle():
+    from pulsar_rocm.api import state_target
+    csa = [(TWO_PI * 1e3, TWO_PI * 8e3, 0.0, (0.3, 0.9, 0.1)), (-TWO_PI * 2e3, TWO_PI * 5e3, 0.0, (1.1, 0.4, 0.2))]
+    dip = [(1, 2, TWO_PI * 15e3, (0.2, 1.2, 0.5))]
+    s = MAS.mas_spin_system(MAS.mr_system(["1H", "13C"]), 12.5e3, [MAS.CSATensor(*c) for c in csa],
+                            [MAS.DipolarCoupling(*d) for d in dip])
+    rng = np.random.default_rng(21)
+    u = rng.uniform(-TWO_PI * 3e4, TWO_PI * 3e4, (4, 40))
+    ctrl = ControlSequence.from_controls(u, 2e-6, 80e-6, 40)
+    pt = np.array([0, 1, 1j, 0], complex)  # unnormalised: the helper uses states as given
+    pi = np.array([1, 0, 0, 1], complex)
+    ori = orientations(6, 6)
+    G = MAS.compute_grape_gradient_powder(s, ctrl, state_target(pt, psi_init=pi), ori)
+    G0 = O.mas_powder_gradient(["1H", "13C"], csa, dip, 12.5e3, u, 2e-6, None, ori, psi_init=pi, psi_targ=pt)
+    assert np.abs(G - G0).max() <= 1e-10 * np.abs(G0).max()

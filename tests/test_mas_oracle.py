@@ -155,10 +155,27 @@ def test_rotor_channels_reproduce_the_fourier_drift():
             assert np.abs(H1 - H2).max() <= 1e-12 * np.abs(H1).max()
 
 
-def test_state_targets_are_rejected_on_this_device():
-    u, dt, Ut = _case()
-    s = MAS.mas_spin_system(MAS.mr_system(["1H"]), 12e3, [MAS.CSATensor(*CSA[0])])
-    from pulsar_rocm.api import state_target
-    ctrl = ControlSequence.from_controls(u[:2], dt, u.shape[1] * dt, u.shape[1])
-    with pytest.raises(ValueError, match="unitary targets"):
-        MAS.compute_grape_gradient_powder(s, ctrl, state_target([0, 1]), [(0, 0, 0, 1.0)])
+def test_state_branch_is_the_overlap_derivative_times_minus_i():
+    # state branch: G = 2 Im(conj(ov) <Q_k psi_t| -i dt H |P_k psi_i>); the
+    # bracket is d<psi_t|U|psi_i>/de for U_k -> U_k exp(-i e dt H_j) (central differences)
+    u, dt, _ = _case(seed=4)
+    Ix, Iy, Iz = O.spin_operators(["1H", "13C"])
+    Hc = [Ix[0], Iy[0], Ix[1], Iy[1]]
+    Hf = O.mas_fourier(["1H", "13C"], CSA, DIP, (0.3, 0.8, 0.2))
+    Us = O.mas_propagators(Hf, Hc, 12e3, u, dt)
+    rng = np.random.default_rng(12)
+    pi = rng.standard_normal(4) + 1j * rng.standard_normal(4)  # unnormalised, as given
+    pt = rng.standard_normal(4) + 1j * rng.standard_normal(4)
+    G = O.propagator_gradient(Us, Hc, None, dt, pi, pt)
+
+    def ov(k, j, e):
+        W = np.eye(4, dtype=complex)
+        for n, U in enumerate(Us):
+            W = (U @ sla.expm(-1j * e * dt * Hc[j]) if n == k else U) @ W
+        return np.vdot(pt, W @ pi)
+
+    z = ov(0, 0, 0.0)
+    h = 100.0
+    for k, j in [(0, 0), (4, 1), (11, 3)]:
+        dz = (ov(k, j, h) - ov(k, j, -h)) / (2 * h)
+        assert abs(G[j, k] - 2 * np.imag(np.conj(z) * dz)) <= 1e-6 * np.abs(G).max()
