@@ -706,10 +706,12 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   std::vector<std::vector<pr::WorkItem>> items(13);
   std::vector<std::vector<int2>> members(13);
   std::vector<int> vmax(13, 1);
-  std::vector<long long> group_members(13, 0);
   std::vector<int> tr_nnz_max(13, 0);  // most nonzeros of one control, per group
   std::vector<int> nc_max(13, 0);
-  for (int p = 0; p < n_probs; p++) group_members[group_of(probs[p])] += probs[p].n_members;
+  std::vector<double> group_work(13, 0.0);  // member-slices per group
+  for (int p = 0; p < n_probs; p++) {
+    group_work[group_of(probs[p])] += (double)probs[p].n_members * probs[p].n_t;
+  }
   // the mixed-precision kernels cover single-wavefront (d <= 16) unitary
   // problems under the reference gradient formula; any other problem of the
   // plan, or a d <= 16 group holding a Lindblad problem, runs in FP64
@@ -763,16 +765,16 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     D.cre = q.dynamics == PR_CLOSED ? 0.0 : 1.0;
     D.cim = q.dynamics == PR_CLOSED ? -1.0 : 0.0;
     // time segmentation: about 8192 wavefronts of work over all problems of
-    // the same block count (members of the group share the segments)
+    // the same block count, every work item about the same number of slices
+    // (so problems of different lengths do not leave a tail of long items)
     int L;
     if (pl.opt.segment > 0) {
       L = pl.opt.segment;
     } else {
       // (d > 64: workgroups of 16 wavefronts, about 512 of them)
       long long want = B > 4 ? 512 : std::max(1LL, (long long)(8192 / NB));
-      long long S = (want + group_members[gi] - 1) / group_members[gi];
-      S = std::min<long long>(std::max<long long>(S, 1), nt);
-      L = (int)((nt + S - 1) / S);
+      const double per_item = std::ceil(group_work[gi] / (double)want);
+      L = (int)std::min<double>(std::max<double>(per_item, 1.0), (double)nt);
       L = std::max(L, std::min(nt, 4));
     }
     L = std::min(L, nt);
