@@ -214,6 +214,16 @@ def other_workload(args):
             "ms_per_eval": round(1e3 * elapsed / args.steps, 4), "steps": args.steps,
             "algorithmic_tflops": round(flops * rate / 1e12, 3),
             "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}}
+    if args.workload == "four_qubit_tracking" and "gradient" in kt:
+        # the per-segment walk: U_k^dag [Psi | L] (8 d^2 2P), Psi L^dag (8 d^2 P) and the
+        # traces (8 flops per operator nonzero) per member-slice
+        npairs = p.n_pairs
+        nnz = sum(int(np.count_nonzero(h)) for h in np.asarray(p.Hc))
+        walk_flops = (8 * 16 * 16 * 3 * npairs + 8 * nnz) * p.n_members * p.n_t
+        walk_s = kt["gradient"][0] / args.steps * 1e-3
+        line["roofline"] = {"bound": "mfma", "kernel": "k_track_walk", "achieved": round(walk_flops / walk_s / 1e12, 3),
+                            "peak": 78.6, "unit": "TFLOP/s", "frac": round(walk_flops / walk_s / 78.6e12, 4),
+                            "traffic": None, "algorithmic_flops_per_launch": walk_flops}
     if args.workload == "four_qubit_tracking" and args.cpu_sample > 0:
         # the tracking oracle (density matrices, one thread) on a member sample
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
