@@ -1397,8 +1397,10 @@ __device__ inline Blk outer_pairs(const double2* LZ, int np, const Ln& x) {
 
 // PACK (d <= 16, n_pairs <= 8): states and co-states share one block,
 // Z = [Psi | L], so U_k^dag Z is one product and M = Psi L^dag takes
-// ceil(n_pairs / 4) k-steps instead of a full product.
-template <int B, int V, bool PACK = false>
+// ceil(n_pairs / 4) k-steps instead of a full product; Z and U_k are staged
+// in LDS together (U_k in a third tile after the reduction scratch), and REG
+// keeps the trace operand in registers for the whole walk.
+template <int B, int V, bool PACK = false, bool REG = false>
 __global__ __launch_bounds__(64 * B * B) void k_track_walk(const ProbDev* __restrict__ probs,
                                                            const WorkItem* __restrict__ items) {
   const WorkItem it = items[blockIdx.x];
@@ -1417,27 +1419,37 @@ __global__ __launch_bounds__(64 * B * B) void k_track_walk(const ProbDev* __rest
   if constexpr (PACK) {
     static_assert(B == 1, "the packed walk is single-wavefront");
     const int np = P.npairs;
+    double2* LC = zb;
+    TraceRegs<V> tregs;
+    if constexpr (REG) trace_regs_load<V>(P, m, tregs, x);
     Blk Z = S;
     {
       const Blk Ls = shift_cols(L, np, x);
       Z.re += Ls.re;
       Z.im += Ls.im;
     }
+    double dtn = gptr(P.dts)[it.k1 - 1];
     for (int k = it.k1 - 1; k >= it.k0; k--) {
       const Blk Uk = Vn;
-      if (k > it.k0) Vn = load_blk(Um + (size_t)(k - 1) * MB, x);
+      const double dtk = dtn;
+      if (k > it.k0) {
+        Vn = load_blk(Um + (size_t)(k - 1) * MB, x);
+        dtn = gptr(P.dts)[k - 1];
+      }
       const int c = gptr(P.ck_of)[k];
       if (c >= 0) {
         const Blk Y = shift_cols(mm<1, N, N>(load_blk(P.Ck + (size_t)c * MB, x), Z, LA, LB, x), np, x);
         Z = axpby(1.0, Z, 2.0, Y);  // L += 2 C Psi
       }
       put<1>(LB, Z, x);
+      put<1>(LC, Uk, x);
       bsync<1>();
       const Blk Mk = outer_pairs(LB, np, x);  // Psi L^dag
-      contract<1, V>(P, m, k, axpby(2.0, Mk, 0.0, Mk), LA, x);
-      put<1>(LA, Uk, x);
-      bsync<1>();
-      Z = gemm_prod<1, J, N>(LA, Uk, LB, Z, x);  // U_k^dag [Psi | L]
+      if constexpr (REG)
+        contract_regs<V, false>(P, m, k, dtk, axpby(2.0, Mk, 0.0, Mk), LA, tregs, x);
+      else
+        contract<1, V>(P, m, k, axpby(2.0, Mk, 0.0, Mk), LA, x);
+      Z = gemm_prod<1, J, N>(LC, Uk, LB, Z, x);  // U_k^dag [Psi | L]
       bsync<1>();
     }
     return;
