@@ -477,3 +477,26 @@ def test_state_objective_mixed_precision():
     F0, G0 = O.evaluate(p, u)
     F, G, _ = run(p, u, precision=PR_MIXED)
     assert abs(F - F0) <= 1e-9 and np.abs(G - G0).max() <= 1e-6 * np.abs(G0).max()
+
+
+def test_state_objective_exact_gradient_matches_oracle():
+    from pulsar_rocm.problem import PR_GRAD_EXACT, PR_STATE
+    rng = np.random.default_rng(77)
+    d = 8
+    p = Problem(dim=d, n_ctrl=2, n_t=30, H0=W.random_hermitian(d, rng, 3.0),
+                Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)]), dt=0.1, objective=PR_STATE,
+                psi_init=_random_state(d, rng)[None], psi_targ=_random_state(d, rng)[None])
+    u = rng.uniform(-2, 2, (2, 30))
+    F0, G0 = O.evaluate(p, u, grad_mode=PR_GRAD_EXACT)
+    F, G, _ = run(p, u, grad_mode=PR_GRAD_EXACT)
+    assert abs(F - F0) <= TOL_F and np.abs(G - G0).max() <= TOL_G * np.abs(G0).max()
+
+
+def test_propagator_helper_objective_fp64_mixed_and_recompute():
+    from pulsar_rocm.problem import PR_GATE_PROPAGATORS, PR_MIXED
+    p, u = W.shrink(*W.four_qubit_ensemble(n_t=60, n_det=2, n_amp=2))
+    p.objective = PR_GATE_PROPAGATORS
+    F0, G0 = O.evaluate(p, u)
+    for kw, tol in (({}, TOL_G), ({"store_props": 0}, TOL_G), ({"precision": PR_MIXED}, 1e-6)):
+        F, G, _ = run(p, u, **kw)
+        assert abs(F - F0) <= 1e-9 and np.abs(G - G0).max() <= tol * np.abs(G0).max(), kw
