@@ -239,6 +239,31 @@ def other_workload(args):
     print(json.dumps(line))
 
 
+def mr_workload(args):
+    """MR state transfer of one spin over 64 offsets x 16 RF power levels
+    (1024 members, 500 slices of 2 us): the grape_state_kernel evaluation
+    with the members packed 8 per 16 x 16 block, and unpacked."""
+    from pulsar_rocm import mr
+    ctrl, w = workloads.mr_spin_ensemble()
+    native.init(0)
+    out = {"workload": "mr_spin_ensemble",
+           "description": "1 spin, 64 offsets x 16 power levels (1024 members), 2 controls, 500 slices, 1 pair"}
+    for pack in (True, False):
+        kp = mr.KernelPlan(ctrl, pack=pack)
+        try:
+            for _ in range(args.warmup):
+                kp(w)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                kp(w)
+            dt = (time.perf_counter() - t0) / args.steps
+        finally:
+            kp.close()
+        out["packed" if pack else "unpacked"] = {"evals_per_s": round(1 / dt, 2), "ms_per_eval": round(1e3 * dt, 4)}
+    out["note"] = "host-buffer evaluations (controls up, gradient down each call)"
+    print(json.dumps(out))
+
+
 def timed_form(p, u, prec, args):
     """Warm-up, then K evaluations with the library's per-kernel events and K
     without them (the headline time is the faster loop). Returns the open plan,
@@ -279,7 +304,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--workload", default="four_qubit_ensemble",
                     choices=["four_qubit_ensemble", "three_qubit_lindblad", "uneven_batch", "single_qubit_state",
-                             "spin_chain_128", "four_qubit_tracking"],
+                             "spin_chain_128", "four_qubit_tracking", "mr_spin_ensemble"],
                     help="the headline metric is four_qubit_ensemble; the others print timing lines of their own")
     ap.add_argument("--cpu-sample", type=int, default=32, help="members timed for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
@@ -297,6 +322,9 @@ def main():
         return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.workload == "mr_spin_ensemble":
+        mr_workload(args)
         return
     if args.workload != "four_qubit_ensemble":
         other_workload(args)

@@ -174,18 +174,93 @@ class LindbladMRControl:
                        psi_init=s0, psi_targ=st)
 
 
+def _blockdiag_at(i, f, A):
+    d = A.shape[0]
+    out = np.zeros((f * d, f * d), np.complex128)
+    out[i * d:(i + 1) * d, i * d:(i + 1) * d] = A
+    return out
+
+
+class PackedMembers:
+    """Small-dimension MR ensembles packed into 16 x 16 blocks.
+
+    A closed state-pair problem with d <= 8 spends a whole 16 x 16 block per
+    member. With f the largest divisor of M with f d <= 16 (and f n_pairs <= 16),
+    members m f .. m f + f - 1 become the diagonal
+    blocks of one packed member: block-diagonal drift, and per control j and
+    block i a channel blockdiag(0, .., A_j, .., 0) with the member's scale
+    s_{mf+i, j} and the waveform row u_j; the pairs of every block are pairs
+    of the packed member. Exp, chains and traces of a block-diagonal matrix
+    act block by block, so F and the per-block gradient rows are those of the
+    members (up to rounding: a pack shares one scaling exponent); the gradient
+    of control j is the sum of its f channel rows. (The pairs belong to the
+    problem, not to a member, so every pack is full: f divides M.)"""
+
+    @staticmethod
+    def factor(p: Problem) -> int:
+        top = min(16 // p.dim, 16 // max(p.n_pairs, 1), p.n_members)
+        return max([f for f in range(1, top + 1) if p.n_members % f == 0])
+
+    def __init__(self, p: Problem):
+        d, M, nc, npairs = p.dim, p.n_members, p.n_ctrl, p.n_pairs
+        f = self.factor(p)
+        self.f, self.nc, self.original = f, nc, p
+        Mp = M // f
+        H0 = np.asarray(p.H0, np.complex128)
+        H0s = H0 if H0.ndim == 3 else np.broadcast_to(H0, (M, d, d))
+        sc = np.ones((M, nc)) if p.ctrl_scale is None else np.asarray(p.ctrl_scale, np.float64)
+        H0p = np.zeros((Mp, f * d, f * d), np.complex128)
+        scp = np.zeros((Mp, f * nc))
+        for m in range(M):
+            q, i = divmod(m, f)
+            H0p[q, i * d:(i + 1) * d, i * d:(i + 1) * d] = H0s[m]
+            scp[q, i * nc:(i + 1) * nc] = sc[m]
+        Hc = np.asarray(p.Hc, np.complex128)
+        Hcp = np.stack([_blockdiag_at(i, f, Hc[j]) for i in range(f) for j in range(nc)])
+        pi, pt = np.asarray(p.psi_init, np.complex128), np.asarray(p.psi_targ, np.complex128)
+        psi_i = np.zeros((f * npairs, f * d), np.complex128)
+        psi_t = np.zeros((f * npairs, f * d), np.complex128)
+        for i in range(f):
+            psi_i[i * npairs:(i + 1) * npairs, i * d:(i + 1) * d] = pi
+            psi_t[i * npairs:(i + 1) * npairs, i * d:(i + 1) * d] = pt
+        self.problem = Problem(dim=f * d, n_ctrl=f * nc, n_t=p.n_t, H0=H0p, Hc=Hcp, dt=p.dt,
+                               dt_slices=p.dt_slices, dynamics=p.dynamics, objective=p.objective, n_members=Mp,
+                               ctrl_scale=scp, weight=np.full(Mp, 1.0 / (M * npairs)),
+                               psi_init=psi_i, psi_targ=psi_t)
+
+    @staticmethod
+    def applies(p: Problem) -> bool:
+        return (p.dynamics == PR_CLOSED and p.objective in (PR_PAIRS_SQUARE, PR_PAIRS_REAL) and p.dim <= 8
+                and p.weight is None and p.n_members >= 2 and p.n_pairs <= 8 and PackedMembers.factor(p) >= 2)
+
+    def controls(self, u):
+        return np.tile(np.asarray(u, np.float64), (self.f, 1))
+
+    def gradient(self, G):
+        return np.asarray(G).reshape(self.f, self.nc, -1).sum(axis=0)
+
+
 class KernelPlan:
     """A reusable device plan for repeated kernel calls with one control context
-    (tracking=True: the tracking objective of an MRControl)."""
+    (tracking=True: the tracking objective of an MRControl). Small closed
+    state-pair ensembles run packed (PackedMembers) unless pack=False."""
 
-    def __init__(self, ctrl, grad_mode: int = PR_GRAD_REFERENCE, tracking: bool = False):
+    def __init__(self, ctrl, grad_mode: int = PR_GRAD_REFERENCE, tracking: bool = False, pack: bool = True):
         self.problem = ctrl.tracking_problem() if tracking else ctrl.problem()
-        self.plan = native.Plan([self.problem], grad_mode=grad_mode)
+        self.packed = PackedMembers(self.problem) if pack and PackedMembers.applies(self.problem) else None
+        run = self.packed.problem if self.packed else self.problem
+        self.plan = native.Plan([run], grad_mode=grad_mode)
+
+    def eval(self, waveform: np.ndarray, grad: bool = True):
+        w = np.asarray(waveform, dtype=np.float64)
+        if self.packed:
+            F, G = self.plan.eval([self.packed.controls(w)], grad=grad)
+            return float(F[0]), (self.packed.gradient(G[0]) if grad else None)
+        F, G = self.plan.eval([w], grad=grad)
+        return float(F[0]), (G[0] if grad else None)
 
     def __call__(self, waveform: np.ndarray):
-        w = np.asarray(waveform, dtype=np.float64)
-        F, G = self.plan.eval([w], grad=True)
-        return float(F[0]), G[0]
+        return self.eval(waveform, grad=True)
 
     def close(self):
         self.plan.close()
@@ -237,13 +312,17 @@ class FidelityForward:
         self.population = int(population)
         if self.population < 1:
             raise ValueError("population must be >= 1")
-        self.plan = native.Plan([copy.copy(self.problem) for _ in range(self.population)])
+        self.packed = PackedMembers(self.problem) if PackedMembers.applies(self.problem) else None
+        run = self.packed.problem if self.packed else self.problem
+        self.plan = native.Plan([copy.copy(run) for _ in range(self.population)])
 
     def batch(self, waveforms) -> np.ndarray:
         """F of each of `population` waveforms [n_ctrl x n_t]."""
         ws = [np.asarray(w, dtype=np.float64) for w in waveforms]
         if len(ws) != self.population:
             raise ValueError(f"expected {self.population} waveforms, got {len(ws)}")
+        if self.packed:
+            ws = [self.packed.controls(w) for w in ws]
         F, _ = self.plan.eval(ws, grad=False)
         return np.asarray(F, dtype=np.float64)
 
@@ -423,12 +502,10 @@ def optimcon(ctrl, guess):
     kp = KernelPlan(ctrl, tracking=tracking)
     try:
         def fg(w):
-            F, G = kp.plan.eval([np.asarray(w, np.float64)], grad=True)
-            return float(F[0]), G[0]
+            return kp.eval(w, grad=True)
 
         def f_only(w):
-            F, _ = kp.plan.eval([np.asarray(w, np.float64)], grad=False)
-            return float(F[0])
+            return kp.eval(w, grad=False)[0]
 
         n_ens = kp.problem.n_members * kp.problem.n_pairs
         if method == "lbfgs":

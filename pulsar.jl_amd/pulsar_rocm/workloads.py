@@ -297,3 +297,18 @@ def shrink(p: Problem, u: np.ndarray, n_members: int | None = None, n_t: int | N
             q.track_target = np.asarray(p.track_target)[keep]
     q._keep = []
     return q, np.ascontiguousarray(uu)
+
+
+def mr_spin_ensemble(seed: int = 29, n_off: int = 64, n_pwr: int = 16, n_t: int = 500):
+    """MR state transfer of one spin-1/2, +z -> -y (the OptControl.jl:296-305
+    example): offsets in +-2 kHz, RF power levels 0.8..1.2 x 10 kHz, 2 us
+    slices; returns (MRControl, waveform [2 x n_t])."""
+    from .mr import MRControl
+    Iz, Ix, Iy = SZ / 2, SX / 2, SY / 2
+    up = np.array([1, 0], np.complex128)
+    mIy = np.array([1, -1j], np.complex128) / np.sqrt(2)
+    ctrl = MRControl(drifts=[TWO_PI * o * Iz for o in np.linspace(-2e3, 2e3, n_off)], operators=[Ix, Iy],
+                     rho_init=[up], rho_targ=[mIy], pulse_dt=np.full(n_t, 2e-6),
+                     pwr_levels=list(TWO_PI * 1e4 * np.linspace(0.8, 1.2, n_pwr)))
+    w = 0.05 * np.random.default_rng(seed).standard_normal((2, n_t))
+    return ctrl, w

@@ -176,3 +176,25 @@ def test_mr_optimcon_on_device_matches_oracle_driven_loop(kind):
     assert dev.n_fidelity_evaluations == ref.n_fidelity_evaluations
     assert np.abs(np.array(dev.fidelity_history) - np.array(ref.fidelity_history)).max() <= 1e-10
     assert np.abs(dev.controls - ref.controls).max() <= 1e-7
+
+
+@pytest.mark.parametrize("nd,pw,pairs", [(16, [0.9, 1.0, 1.1, 1.2], 1), (5, [0.9, 1.1], 2)])
+def test_packed_mr_ensemble_on_device(nd, pw, pairs):
+    # d = 2 members packed f per block (PackedMembers) against the unpacked oracle
+    Ix, Iy, Iz = W.SX / 2, W.SY / 2, W.SZ / 2
+    up, dn = np.array([1, 0], complex), np.array([0, 1], complex)
+    mIy = np.array([1, -1j], complex) / np.sqrt(2)
+    ctrl = mr.MRControl(drifts=[2 * np.pi * o * Iz for o in np.linspace(-800, 800, nd)], operators=[Ix, Iy],
+                        rho_init=[up, dn][:pairs], rho_targ=[mIy, up][:pairs], pulse_dt=np.full(60, 4e-6),
+                        pwr_levels=[2 * np.pi * 1e4 * x for x in pw])
+    w = np.random.default_rng(nd).uniform(-1, 1, (2, 60))
+    kp = mr.KernelPlan(ctrl)
+    assert kp.packed is not None and kp.packed.f >= 2
+    try:
+        F, G = kp(w)
+    finally:
+        kp.close()
+    F0, G0 = O.evaluate(ctrl.problem(), w)
+    assert abs(F - F0) <= TOL_F and np.abs(G - G0).max() <= TOL_G * np.abs(G0).max()
+    Fs, Gs = mr.grape_state_kernel(w, ctrl)
+    assert Fs == F and np.array_equal(Gs, G)

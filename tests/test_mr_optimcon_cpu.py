@@ -87,3 +87,23 @@ def test_optimcon_rejects_other_methods_and_bad_shapes():
         mr.optimcon(ctrl, np.zeros((2, 40)))
     with pytest.raises(ValueError, match="rows"):
         mr.optimcon(_mr_ctrl(), np.zeros((3, 40)))
+
+
+@pytest.mark.parametrize("nd,pw,pairs", [(7, [1.0, 1.2], 1), (5, [0.9, 1.0, 1.1], 2), (4, [1.0, 1.1], 2)])
+def test_packed_members_equal_the_ensemble(nd, pw, pairs):
+    # host logic of PackedMembers: block-diagonal packing of f members per
+    # 16 x 16 block gives the ensemble's F and (summed channel rows) G
+    Iz, Ix, Iy = np.diag([0.5, -0.5]).astype(complex), np.array([[0, 0.5], [0.5, 0]], complex), \
+        np.array([[0, -0.5j], [0.5j, 0]], complex)
+    up, dn, mIy = np.array([1, 0], complex), np.array([0, 1], complex), np.array([1, -1j], complex) / np.sqrt(2)
+    ctrl = mr.MRControl(drifts=[2 * np.pi * o * Iz for o in np.linspace(-500, 500, nd)], operators=[Ix, Iy],
+                        rho_init=[up, dn][:pairs], rho_targ=[mIy, up][:pairs], pulse_dt=np.full(30, 5e-6),
+                        pwr_levels=[2 * np.pi * 8e3 * x for x in pw])
+    p = ctrl.problem()
+    u = np.random.default_rng(nd).uniform(-1, 1, (2, 30))
+    assert mr.PackedMembers.applies(p)
+    pk = mr.PackedMembers(p)
+    assert p.n_members % pk.f == 0 and pk.f * p.dim <= 16 and pk.f * p.n_pairs <= 16 and pk.f >= 2
+    F0, G0 = O.evaluate(p, u)
+    F1, G1 = O.evaluate(pk.problem, pk.controls(u))
+    assert abs(F0 - F1) < 1e-14 and np.abs(G0 - pk.gradient(G1)).max() <= 1e-13 * np.abs(G0).max()
