@@ -184,8 +184,8 @@ def _blockdiag_at(i, f, A):
 class PackedMembers:
     """Small-dimension MR ensembles packed into 16 x 16 blocks.
 
-    A closed state-pair problem with d <= 8 spends a whole 16 x 16 block per
-    member. With f the largest divisor of M with f d <= 16 (and f n_pairs <= 16),
+    A state-pair problem with d <= 8 (closed, or Lindblad with a 2-level
+    system, d = 4) spends a whole 16 x 16 block per member. With f the largest divisor of M with f d <= 16 (and f n_pairs <= 16),
     members m f .. m f + f - 1 become the diagonal
     blocks of one packed member: block-diagonal drift, and per control j and
     block i a channel blockdiag(0, .., A_j, .., 0) with the member's scale
@@ -230,7 +230,7 @@ class PackedMembers:
 
     @staticmethod
     def applies(p: Problem) -> bool:
-        return (p.dynamics == PR_CLOSED and p.objective in (PR_PAIRS_SQUARE, PR_PAIRS_REAL) and p.dim <= 8
+        return (p.objective in (PR_PAIRS_SQUARE, PR_PAIRS_REAL) and p.dim <= 8
                 and p.weight is None and p.n_members >= 2 and p.n_pairs <= 8 and PackedMembers.factor(p) >= 2)
 
     def controls(self, u):

@@ -198,3 +198,23 @@ def test_packed_mr_ensemble_on_device(nd, pw, pairs):
     assert abs(F - F0) <= TOL_F and np.abs(G - G0).max() <= TOL_G * np.abs(G0).max()
     Fs, Gs = mr.grape_state_kernel(w, ctrl)
     assert Fs == F and np.array_equal(Gs, G)
+
+
+def test_packed_lindblad_mr_ensemble_on_device():
+    # N = 2 Lindblad members (4 x 4 Liouvillians) packed 4 per block
+    Ix, Iy, Iz = W.SX / 2, W.SY / 2, W.SZ / 2
+    up = np.array([1, 0], complex)
+    mIy = np.array([1, -1j], complex) / np.sqrt(2)
+    ctrl = mr.LindbladMRControl(drifts=[2 * np.pi * o * Iz for o in np.linspace(-300, 300, 6)], operators=[Ix, Iy],
+                                rho_init=[up], rho_targ=[mIy], pulse_dt=np.full(40, 5e-6),
+                                pwr_levels=[2 * np.pi * 8e3, 2 * np.pi * 1e4],
+                                jump_ops=[np.array([[0, 1], [0, 0]], complex)], decay_rates=[300.0])
+    w = np.random.default_rng(3).uniform(-1, 1, (2, 40))
+    kp = mr.KernelPlan(ctrl)
+    assert kp.packed is not None and kp.packed.f == 4
+    try:
+        F, G = kp(w)
+    finally:
+        kp.close()
+    F0, G0 = O.evaluate(ctrl.problem(), w)
+    assert abs(F - F0) <= TOL_F and np.abs(G - G0).max() <= TOL_G * np.abs(G0).max()
