@@ -193,8 +193,9 @@ class PackedMembers:
     of the packed member. Exp, chains and traces of a block-diagonal matrix
     act block by block, so F and the per-block gradient rows are those of the
     members (up to rounding: a pack shares one scaling exponent); the gradient
-    of control j is the sum of its f channel rows. (The pairs belong to the
-    problem, not to a member, so every pack is full: f divides M.)"""
+    of control j is the sum of its f channel rows. Tracking checkpoints
+    become blockdiag(C, .., C). (The pairs belong to the problem, not to a
+    member, so every pack is full: f divides M.)"""
 
     @staticmethod
     def factor(p: Problem) -> int:
@@ -217,20 +218,29 @@ class PackedMembers:
             scp[q, i * nc:(i + 1) * nc] = sc[m]
         Hc = np.asarray(p.Hc, np.complex128)
         Hcp = np.stack([_blockdiag_at(i, f, Hc[j]) for i in range(f) for j in range(nc)])
-        pi, pt = np.asarray(p.psi_init, np.complex128), np.asarray(p.psi_targ, np.complex128)
-        psi_i = np.zeros((f * npairs, f * d), np.complex128)
-        psi_t = np.zeros((f * npairs, f * d), np.complex128)
-        for i in range(f):
-            psi_i[i * npairs:(i + 1) * npairs, i * d:(i + 1) * d] = pi
-            psi_t[i * npairs:(i + 1) * npairs, i * d:(i + 1) * d] = pt
+        def pack_states(v):
+            if v is None:
+                return None
+            v = np.asarray(v, np.complex128)
+            out = np.zeros((f * npairs, f * d), np.complex128)
+            for i in range(f):
+                out[i * npairs:(i + 1) * npairs, i * d:(i + 1) * d] = v
+            return out
+
+        track = {}
+        if p.objective == PR_TRACKING:
+            # every block is compared with the same checkpoint density matrices
+            tt = np.asarray(p.track_target, np.complex128)
+            track = dict(track_step=np.asarray(p.track_step), track_weight=np.asarray(p.track_weight),
+                         track_target=np.stack([np.kron(np.eye(f), t) for t in tt]))
         self.problem = Problem(dim=f * d, n_ctrl=f * nc, n_t=p.n_t, H0=H0p, Hc=Hcp, dt=p.dt,
                                dt_slices=p.dt_slices, dynamics=p.dynamics, objective=p.objective, n_members=Mp,
                                ctrl_scale=scp, weight=np.full(Mp, 1.0 / (M * npairs)),
-                               psi_init=psi_i, psi_targ=psi_t)
+                               psi_init=pack_states(p.psi_init), psi_targ=pack_states(p.psi_targ), **track)
 
     @staticmethod
     def applies(p: Problem) -> bool:
-        return (p.objective in (PR_PAIRS_SQUARE, PR_PAIRS_REAL) and p.dim <= 8
+        return (p.objective in (PR_PAIRS_SQUARE, PR_PAIRS_REAL, PR_TRACKING) and p.dim <= 8
                 and p.weight is None and p.n_members >= 2 and p.n_pairs <= 8 and PackedMembers.factor(p) >= 2)
 
     def controls(self, u):

@@ -218,3 +218,22 @@ def test_packed_lindblad_mr_ensemble_on_device():
         kp.close()
     F0, G0 = O.evaluate(ctrl.problem(), w)
     assert abs(F - F0) <= TOL_F and np.abs(G - G0).max() <= TOL_G * np.abs(G0).max()
+
+
+def test_packed_tracking_mr_ensemble_on_device():
+    # one spin, 12 members packed 6 per block, checkpoints blockdiag(C, .., C)
+    Ix, Iy, Iz = W.SX / 2, W.SY / 2, W.SZ / 2
+    up, dn = np.array([1, 0], complex), np.array([0, 1], complex)
+    cps = [mr.TrackingPoint(10 * k, np.outer(dn, dn) if k % 2 else np.outer(up, up), 0.25) for k in range(1, 5)]
+    ctrl = mr.MRControl(drifts=[2 * np.pi * o * Iz for o in np.linspace(-300, 300, 6)], operators=[Ix, Iy],
+                        rho_init=[up], rho_targ=[dn], pulse_dt=np.full(40, 5e-6),
+                        pwr_levels=[2 * np.pi * 8e3, 2 * np.pi * 1e4], tracking=cps)
+    w = np.random.default_rng(6).uniform(-1, 1, (2, 40))
+    kp = mr.KernelPlan(ctrl, tracking=True)
+    assert kp.packed is not None and kp.packed.f == 6
+    try:
+        F, G = kp(w)
+    finally:
+        kp.close()
+    F0, G0 = O.tracking_evaluate(ctrl.tracking_problem(), w)
+    assert abs(F - F0) <= TOL_F and np.abs(G - G0).max() <= TOL_G * np.abs(G0).max()
