@@ -186,7 +186,7 @@ def other_workload(args):
         desc = "single_qubit_state: d=2, 100 slices, state transfer"
         flops = (72 * 8 + 16 * 2 * 4) * 100
     native.init(0)
-    plan = native.Plan(probs, precision=PR_FP64 if args.precision == "fp64" else PR_MIXED)
+    plan = native.Plan(probs, precision=PR_FP64 if args.precision == "fp64" else PR_MIXED, segment=args.segment)
     plan.upload_controls(us)
     d_u, d_F, d_G = plan.buffers()
     for _ in range(args.warmup):
