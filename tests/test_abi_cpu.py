@@ -138,3 +138,29 @@ def test_problem_struct_mirrors_match_the_header():
     jl_body = re.search(r"struct PRProblem\n(.*?)\nend", jl, flags=re.S).group(1)
     jl_fields = re.findall(r"^\s*(\w+)::", jl_body, flags=re.M)
     assert c_fields == py_fields == jl_fields, (c_fields, py_fields, jl_fields)
+
+
+@pytest.mark.parametrize("case,code", [("state_lindblad", -5), ("helper_exact", -5), ("helper_state_exact", -5),
+                                       ("helper_wide", -5), ("helper_nonunitary", -1)])
+def test_new_objectives_validate_before_device_use(case, code):
+    from pulsar_rocm.problem import PR_GATE_PROPAGATORS, PR_LINDBLAD, PR_STATE, PR_STATE_PROPAGATORS
+    d = 80 if case == "helper_wide" else 4
+    rng = np.random.default_rng(0)
+    H0 = np.diag(rng.standard_normal(d)).astype(complex)
+    Hc = np.stack([np.diag(rng.standard_normal(d)).astype(complex)])
+    kw = {}
+    if case == "state_lindblad":
+        p = Problem(dim=d, n_ctrl=1, n_t=3, H0=H0, Hc=Hc, dt=0.1, dynamics=PR_LINDBLAD, objective=PR_STATE,
+                    psi_init=np.eye(d, dtype=complex)[:1], psi_targ=np.eye(d, dtype=complex)[:1])
+    elif case == "helper_state_exact":
+        p = Problem(dim=d, n_ctrl=1, n_t=3, H0=H0, Hc=Hc, dt=0.1, objective=PR_STATE_PROPAGATORS,
+                    psi_init=np.eye(d, dtype=complex)[:1], psi_targ=np.eye(d, dtype=complex)[:1])
+        kw["grad_mode"] = PR_GRAD_EXACT
+    else:
+        Ut = 1.01 * np.eye(d, dtype=complex) if case == "helper_nonunitary" else np.eye(d, dtype=complex)
+        p = Problem(dim=d, n_ctrl=1, n_t=3, H0=H0, Hc=Hc, dt=0.1, objective=PR_GATE_PROPAGATORS, U_target=Ut)
+        if case == "helper_exact":
+            kw["grad_mode"] = PR_GRAD_EXACT
+    with pytest.raises(native.RocmError) as e:
+        native.Plan([p], **kw)
+    assert e.value.code == code, str(e.value)
