@@ -52,8 +52,10 @@ HBM_PEAK_GBS = 8000.0
 # read for a 256 MiB device-to-device copy (FETCH x 2.0, WRITE x 1.0); the
 # commands and per-kernel figures are in profiles/grape_four_qubit_ensemble_hbm_traffic.md.
 # k_segment writes the U_k^T images (256 x 2000 x 4 KiB = 2.10 GB) and
-# k_gradient reads them back once.
-MEASURED_TRAFFIC_BYTES = {"k_segment": 0.022e9 + 2.131e9, "k_gradient": 2.141e9 + 0.037e9}
+# k_gradient reads them back once. k_track_walk (four_qubit_tracking) reads the
+# images once too (profiles/grape_four_qubit_tracking_hbm_traffic.md).
+MEASURED_TRAFFIC_BYTES = {"k_segment": 0.022e9 + 2.131e9, "k_gradient": 2.141e9 + 0.037e9,
+                          "k_track_walk": 2.166e9 + 0.036e9}
 
 
 def probe_rates():
@@ -215,15 +217,16 @@ def other_workload(args):
             "algorithmic_tflops": round(flops * rate / 1e12, 3),
             "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}}
     if args.workload == "four_qubit_tracking" and "gradient" in kt:
-        # the per-segment walk: U_k^dag [Psi | L] (8 d^2 2P), Psi L^dag (8 d^2 P) and the
-        # traces (8 flops per operator nonzero) per member-slice
-        npairs = p.n_pairs
-        nnz = sum(int(np.count_nonzero(h)) for h in np.asarray(p.Hc))
-        walk_flops = (8 * 16 * 16 * 3 * npairs + 8 * nnz) * p.n_members * p.n_t
+        # the per-segment backward walk reads every U_k image once (16 x 16
+        # complex FP64, 4 KiB) and writes the member's gradient row (n_c
+        # doubles) per member-slice: HBM is its bound (its MFMA work, 8 d^2 2P
+        # + 8 d^2 P + 8 per operator nonzero flops, is 13.1 GFLOP, 0.17 ms at peak)
+        walk_bytes = (16 * 16 * 16 + 8 * p.n_ctrl) * p.n_members * p.n_t
         walk_s = kt["gradient"][0] / args.steps * 1e-3
-        line["roofline"] = {"bound": "mfma", "kernel": "k_track_walk", "achieved": round(walk_flops / walk_s / 1e12, 3),
-                            "peak": 78.6, "unit": "TFLOP/s", "frac": round(walk_flops / walk_s / 78.6e12, 4),
-                            "traffic": None, "algorithmic_flops_per_launch": walk_flops}
+        line["roofline"] = {"bound": "hbm", "kernel": "k_track_walk", "achieved": round(walk_bytes / walk_s / 1e9, 1),
+                            "peak": 8000.0, "unit": "GB/s", "frac": round(walk_bytes / walk_s / 8e12, 4),
+                            "traffic": MEASURED_TRAFFIC_BYTES["k_track_walk"],
+                            "algorithmic_bytes_per_launch": walk_bytes}
     if args.workload == "four_qubit_tracking" and args.cpu_sample > 0:
         # the tracking oracle (density matrices, one thread) on a member sample
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
