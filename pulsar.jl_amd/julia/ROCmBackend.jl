@@ -22,6 +22,7 @@ const PR_PAIRS_REAL = Cint(2)
 const PR_TRACKING = Cint(3)
 const PR_GATE_PROPAGATORS = Cint(4)
 const PR_STATE = Cint(5)   # compute_grape_gradient with a state target (Gradients.jl:357-399)
+const PR_STATE_PROPAGATORS = Cint(6)   # state branch of the MAS/DNP propagator helper
 
 # Memory layout identical to `pr_problem` (include/pulsar_rocm.h).
 struct PRProblem

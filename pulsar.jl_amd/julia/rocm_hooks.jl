@@ -111,6 +111,67 @@ function _rocm_grape_tracking_kernel(waveform::Matrix{Float64}, ctrl)
     return fidelity, G[1]
 end
 
+# compute_grape_gradient(sys::MASSpinSystem, ctrl, target; rotor_phase_0) and
+# compute_grape_gradient_powder(sys, ctrl, target, orientations)
+# (src/Physics/MRPhysics.jl:155-188) gain a `get_device() === :rocm` branch calling
+# these. One problem per orientation: the drift is H_0 of the rotated system and the
+# rotor modulation enters as four channels K_m^c = H_m + H_-m, K_m^s = i (H_m - H_-m)
+# (m = 1, 2) with the fixed waveform rows cos(m w_r t_k), sin(m w_r t_k); the objective
+# is the propagator helper's (:312-351), whose trace has no U_target^dag.
+function _rocm_mas_problem(sys, ctrl, target, rotor_phase_0::Float64)
+    n_c, n_t = size(ctrl.controls)
+    d = sys.dim
+    Hf = sys.H_fourier
+    chans = [ComplexF64.(h) for h in sys.H_controls]
+    for m in 1:2
+        push!(chans, Hf[m] .+ Hf[-m])
+        push!(chans, 1im .* (Hf[m] .- Hf[-m]))
+    end
+    t = ((1:n_t) .- 0.5) .* ctrl.dt .+ rotor_phase_0 / sys.omega_r
+    θ = sys.omega_r .* t
+    u = vcat(Matrix{Float64}(ctrl.controls), permutedims(hcat(cos.(θ), sin.(θ), cos.(2θ), sin.(2θ))))
+    H0 = ComplexF64.(Hf[0])
+    HcA = cat(chans...; dims = 3)
+    roots = Any[H0, HcA, u]
+    if target.type == "unitary"
+        Ut = ComplexF64.(target.target_unitary)
+        push!(roots, Ut)
+        prob = PRProblem(d, length(chans), n_t, ctrl.dt, C_NULL, PR_CLOSED, PR_GATE_PROPAGATORS, 1, _cptr(H0), 0,
+                         _cptr(HcA), C_NULL, C_NULL, _cptr(Ut), 0, C_NULL, C_NULL, 0, C_NULL, C_NULL, C_NULL)
+    else
+        ψt = ComplexF64.(target.target_state)
+        ψi = target.initial_state === nothing ? ψt : ComplexF64.(target.initial_state)   # as given
+        w1 = ones(1)
+        push!(roots, ψi, ψt, w1)
+        prob = PRProblem(d, length(chans), n_t, ctrl.dt, C_NULL, PR_CLOSED, PR_STATE_PROPAGATORS, 1, _cptr(H0), 0,
+                         _cptr(HcA), C_NULL, _cptr(w1), C_NULL, 1, _cptr(ψi), _cptr(ψt), 0, C_NULL, C_NULL, C_NULL)
+    end
+    return prob, roots, u
+end
+
+function _rocm_compute_grape_gradient_powder(sys, ctrl, target, orientations)
+    probs, roots, us = PRProblem[], Any[], Matrix{Float64}[]
+    for (α, β, γ, _) in orientations
+        p, r, u = _rocm_mas_problem(rotate_spin_system(sys, α, β, γ), ctrl, target, 0.0)
+        push!(probs, p); push!(roots, r); push!(us, u)
+    end
+    plan = ROCmPlan(probs, roots)
+    _, Gs = rocm_eval(plan, us)
+    n_c = size(ctrl.controls, 1)
+    G = zeros(Float64, size(ctrl.controls))
+    for (i, ori) in enumerate(orientations)
+        G .+= ori[4] .* Gs[i][1:n_c, :]
+    end
+    return G
+end
+
+function _rocm_compute_grape_gradient_mas(sys, ctrl, target; rotor_phase_0::Float64 = 0.0)
+    p, r, u = _rocm_mas_problem(sys, ctrl, target, rotor_phase_0)
+    plan = ROCmPlan([p], Any[r])
+    _, Gs = rocm_eval(plan, [u])
+    return Gs[1][1:size(ctrl.controls, 1), :]
+end
+
 # grape_optimize(system, target, controls_init; config)  (src/Optimization/GRAPE.jl:469-643)
 # The whole gradient-ascent loop in the library (pr_grape_optimize): controls,
 # gradient and best controls stay in HBM; the step adaptation, update and
