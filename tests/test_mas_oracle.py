@@ -179,3 +179,26 @@ def test_state_branch_is_the_overlap_derivative_times_minus_i():
     for k, j in [(0, 0), (4, 1), (11, 3)]:
         dz = (ov(k, j, h) - ov(k, j, -h)) / (2 * h)
         assert abs(G[j, k] - 2 * np.imag(np.conj(z) * dz)) <= 1e-6 * np.abs(G).max()
+
+
+def test_per_orientation_rotor_channels_reproduce_the_powder_gradient():
+    # the construction of the Julia glue (rocm_hooks.jl): per orientation one
+    # PR_GATE_PROPAGATORS problem with channels H_m + H_-m and i (H_m - H_-m)
+    # driven by cos(m w_r t_k), sin(m w_r t_k); evaluated by the C oracle
+    from pulsar_rocm.problem import PR_GATE_PROPAGATORS
+    u, dt, Ut = _case(seed=6)
+    w_r = TWO_PI * 12e3
+    ori = [(0.1, 0.5, 0.3, 0.4), (1.0, 1.3, 0.2, 0.6)]
+    Ix, Iy, Iz = O.spin_operators(["1H", "13C"])
+    Hc = [Ix[0], Iy[0], Ix[1], Iy[1]]
+    G = 0.0
+    for a, b, g, wt in ori:
+        Hf = O.mas_fourier(["1H", "13C"], CSA, DIP, (a, b, g))
+        chans = Hc + [Hf[1] + Hf[-1], 1j * (Hf[1] - Hf[-1]), Hf[2] + Hf[-2], 1j * (Hf[2] - Hf[-2])]
+        th = w_r * (np.arange(u.shape[1]) + 0.5) * dt
+        ue = np.vstack([u, np.cos(th), np.sin(th), np.cos(2 * th), np.sin(2 * th)])
+        p = Problem(dim=4, n_ctrl=len(chans), n_t=u.shape[1], H0=Hf[0], Hc=np.stack(chans), dt=dt,
+                    objective=PR_GATE_PROPAGATORS, U_target=Ut)
+        G = G + wt * O.evaluate(p, ue)[1][:4]
+    G0 = O.mas_powder_gradient(["1H", "13C"], CSA, DIP, 12e3, u, dt, Ut, ori)
+    assert np.abs(G - G0).max() <= 1e-12 * np.abs(G0).max()
