@@ -80,3 +80,34 @@ def test_mas_state_target_matches_oracle():
     G = MAS.compute_grape_gradient_powder(s, ctrl, state_target(pt, psi_init=pi), ori)
     G0 = O.mas_powder_gradient(["1H", "13C"], csa, dip, 12.5e3, u, 2e-6, None, ori, psi_init=pi, psi_targ=pt)
     assert np.abs(G - G0).max() <= 1e-10 * np.abs(G0).max()
+
+
+def test_per_orientation_problems_in_one_plan():
+    # the Julia glue's layout: one PR_GATE_PROPAGATORS problem per orientation
+    from pulsar_rocm.problem import PR_GATE_PROPAGATORS, Problem
+    from pulsar_rocm.workloads import haar_unitary
+    csa = [(TWO_PI * 1e3, TWO_PI * 8e3, 0.0, (0.3, 0.9, 0.1)), (-TWO_PI * 2e3, TWO_PI * 5e3, 0.0, (1.1, 0.4, 0.2))]
+    dip = [(1, 2, TWO_PI * 15e3, (0.2, 1.2, 0.5))]
+    rng = np.random.default_rng(31)
+    u = rng.uniform(-TWO_PI * 3e4, TWO_PI * 3e4, (4, 30))
+    dt, w_r = 2e-6, TWO_PI * 12.5e3
+    Ut = haar_unitary(4, np.random.default_rng(2))
+    Ix, Iy, Iz = O.spin_operators(["1H", "13C"])
+    Hc = [Ix[0], Iy[0], Ix[1], Iy[1]]
+    ori = orientations(5, 5)
+    probs, us = [], []
+    for a, b, g, _ in ori:
+        Hf = O.mas_fourier(["1H", "13C"], csa, dip, (a, b, g))
+        chans = Hc + [Hf[1] + Hf[-1], 1j * (Hf[1] - Hf[-1]), Hf[2] + Hf[-2], 1j * (Hf[2] - Hf[-2])]
+        th = w_r * (np.arange(30) + 0.5) * dt
+        us.append(np.vstack([u, np.cos(th), np.sin(th), np.cos(2 * th), np.sin(2 * th)]))
+        probs.append(Problem(dim=4, n_ctrl=8, n_t=30, H0=Hf[0], Hc=np.stack(chans), dt=dt,
+                             objective=PR_GATE_PROPAGATORS, U_target=Ut))
+    plan = native.Plan(probs)
+    try:
+        _, Gs = plan.eval(us)
+    finally:
+        plan.close()
+    G = sum(o[3] * g[:4] for o, g in zip(ori, Gs))
+    G0 = O.mas_powder_gradient(["1H", "13C"], csa, dip, 12.5e3, u, dt, Ut, ori)
+    assert np.abs(G - G0).max() <= 1e-10 * np.abs(G0).max()
