@@ -231,7 +231,7 @@ def other_workload(args):
         # the tracking oracle (density matrices, one thread) on a member sample
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        n = min(args.cpu_sample, 2)
+        n = min(args.cpu_sample, 16)  # about 0.65 s of one core per member
         q, uu = workloads.shrink(p, u, n_members=n)
         t0 = time.perf_counter()
         O.tracking_evaluate(q, uu)
@@ -309,7 +309,8 @@ def main():
                     choices=["four_qubit_ensemble", "three_qubit_lindblad", "uneven_batch", "single_qubit_state",
                              "spin_chain_128", "four_qubit_tracking", "mr_spin_ensemble"],
                     help="the headline metric is four_qubit_ensemble; the others print timing lines of their own")
-    ap.add_argument("--cpu-sample", type=int, default=32, help="members timed for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=256,
+                    help="members timed for the CPU baseline (0: skip); the default is the whole ensemble")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--fp64-only", action="store_true",
                     help="skip the mixed-precision form that the FP64 run also times")
