@@ -546,7 +546,8 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
 // at 512 / waves); 0 leaves the register budget to the compiler. The d <= 16
 // FP64 segment walk runs faster at 3 waves (168 VGPRs, no spills) than at 4
 // (128 VGPRs with spills); the mixed walk is equal at 3 and 4 and keeps 4;
-// the gradient walk is not occupancy-bound.
+// the FP64 gradient walk fits 128 VGPRs without spills and gains 0.7% at 4
+// (150 VGPRs, 3 waves, when unconstrained).
 #ifndef PR_SEGMENT_WAVES
 #define PR_SEGMENT_WAVES 3
 #endif
@@ -554,7 +555,7 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
 #define PR_MIXED_SEGMENT_WAVES 4
 #endif
 #ifndef PR_GRADIENT_WAVES
-#define PR_GRADIENT_WAVES 0
+#define PR_GRADIENT_WAVES 4
 #endif
 // CLOSED: every problem of the launch has closed dynamics (anti-Hermitian
 // generators), so the d <= 16 exp takes the register-only form unconditionally.
