@@ -55,7 +55,7 @@ HBM_PEAK_GBS = 8000.0
 # k_gradient reads them back once. k_track_walk (four_qubit_tracking) reads the
 # images once too (profiles/grape_four_qubit_tracking_hbm_traffic.md).
 MEASURED_TRAFFIC_BYTES = {"k_segment": 0.022e9 + 2.131e9, "k_gradient": 2.141e9 + 0.037e9,
-                          "k_track_walk": 2.166e9 + 0.036e9}
+                          "k_track_walk": 2.166e9 + 0.040e9}
 
 
 def probe_rates():

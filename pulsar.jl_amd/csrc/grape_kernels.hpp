@@ -1398,9 +1398,10 @@ __device__ inline Blk outer_pairs(const double2* LZ, int np, const Ln& x) {
 
 // PACK (d <= 16, n_pairs <= 8): states and co-states share one block,
 // Z = [Psi | L], so U_k^dag Z is one product and M = Psi L^dag takes
-// ceil(n_pairs / 4) k-steps instead of a full product; Z and U_k are staged
-// in LDS together (U_k in a third tile after the reduction scratch), and REG
-// keeps the trace operand in registers for the whole walk.
+// ceil(n_pairs / 4) k-steps instead of a full product. For B == 1, U_k is
+// loaded transposed from its U_k^T image, so the U_k^dag products take both
+// operands from registers (in the packed walk only Z goes through LDS, for M);
+// REG keeps the trace operand in registers for the whole walk.
 template <int B, int V, bool PACK = false, bool REG = false>
 __global__ __launch_bounds__(64 * B * B) void k_track_walk(const ProbDev* __restrict__ probs,
                                                            const WorkItem* __restrict__ items) {
@@ -1416,11 +1417,17 @@ __global__ __launch_bounds__(64 * B * B) void k_track_walk(const ProbDev* __rest
   const double2* Um = P.Ust + (size_t)m * P.nt * MB;
   Blk S = load_blk(P.Pend + sb, x);
   Blk L = load_blk(P.Lend + sb, x);
-  Blk Vn = load_blk(Um + (size_t)(it.k1 - 1) * MB, x);
+  // U_k^dag as an A operand: op C on U_k in registers (B == 1), else op J on
+  // the U_k^T block through LDS
+  constexpr int OU = B == 1 ? C : J;
+  Blk Vn;
+  if constexpr (B == 1)
+    Vn = load_blk_t(Um + (size_t)(it.k1 - 1) * MB, x);
+  else
+    Vn = load_blk(Um + (size_t)(it.k1 - 1) * MB, x);
   if constexpr (PACK) {
     static_assert(B == 1, "the packed walk is single-wavefront");
     const int np = P.npairs;
-    double2* LC = zb;
     TraceRegs<V> tregs;
     if constexpr (REG) trace_regs_load<V>(P, m, tregs, x);
     Blk Z = S;
@@ -1431,10 +1438,10 @@ __global__ __launch_bounds__(64 * B * B) void k_track_walk(const ProbDev* __rest
     }
     double dtn = gptr(P.dts)[it.k1 - 1];
     for (int k = it.k1 - 1; k >= it.k0; k--) {
-      const Blk Uk = Vn;
+      const Blk Uk = Vn;  // U_k
       const double dtk = dtn;
       if (k > it.k0) {
-        Vn = load_blk(Um + (size_t)(k - 1) * MB, x);
+        Vn = load_blk_t(Um + (size_t)(k - 1) * MB, x);
         dtn = gptr(P.dts)[k - 1];
       }
       const int c = gptr(P.ck_of)[k];
@@ -1443,21 +1450,25 @@ __global__ __launch_bounds__(64 * B * B) void k_track_walk(const ProbDev* __rest
         Z = axpby(1.0, Z, 2.0, Y);  // L += 2 C Psi
       }
       put<1>(LB, Z, x);
-      put<1>(LC, Uk, x);
       bsync<1>();
       const Blk Mk = outer_pairs(LB, np, x);  // Psi L^dag
       if constexpr (REG)
         contract_regs<V, false>(P, m, k, dtk, axpby(2.0, Mk, 0.0, Mk), LA, tregs, x);
       else
         contract<1, V>(P, m, k, axpby(2.0, Mk, 0.0, Mk), LA, x);
-      Z = gemm_prod<1, J, N>(LC, Uk, LB, Z, x);  // U_k^dag [Psi | L]
+      Z = gemm_prod<1, OU, N>(LA, Uk, LB, Z, x);  // U_k^dag [Psi | L], registers only
       bsync<1>();
     }
     return;
   }
   for (int k = it.k1 - 1; k >= it.k0; k--) {
     const Blk Uk = Vn;
-    if (k > it.k0) Vn = load_blk(Um + (size_t)(k - 1) * MB, x);
+    if (k > it.k0) {
+      if constexpr (B == 1)
+        Vn = load_blk_t(Um + (size_t)(k - 1) * MB, x);
+      else
+        Vn = load_blk(Um + (size_t)(k - 1) * MB, x);
+    }
     const int c = gptr(P.ck_of)[k];
     if (c >= 0) {
       const Blk Y = mm<B, N, N>(load_blk(P.Ck + (size_t)c * MB, x), S, LA, LB, x);
@@ -1465,8 +1476,8 @@ __global__ __launch_bounds__(64 * B * B) void k_track_walk(const ProbDev* __rest
     }
     const Blk Mk = mm<B, N, C>(S, L, LA, LB, x);  // Psi L^dag
     contract<B, V>(P, m, k, axpby(2.0, Mk, 0.0, Mk), LA, x);
-    L = mm<B, J, N>(Uk, L, LA, LB, x);  // U_k^dag L
-    S = mm<B, J, N>(Uk, S, LA, LB, x);  // U_k^dag Psi
+    L = mm<B, OU, N>(Uk, L, LA, LB, x);  // U_k^dag L
+    S = mm<B, OU, N>(Uk, S, LA, LB, x);  // U_k^dag Psi
   }
 }
 

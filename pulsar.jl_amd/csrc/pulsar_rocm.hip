@@ -574,10 +574,10 @@ int launch_track(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t 
   case VV:                                                                                                           \
     if (B == 1 && g.track_packed && g.trace_regs)                                                                    \
       hipLaunchKernelGGL((pr::k_track_walk<1, VV, true, true>), dim3(g.n_items), blk,                               \
-                         pr::Smem<1>::base + pr::Smem<1>::tile, st, pl.d_probs, g.d_items);                          \
+                         pr::Smem<1>::base, st, pl.d_probs, g.d_items);                                               \
     else if (B == 1 && g.track_packed)                                                                               \
       hipLaunchKernelGGL((pr::k_track_walk<1, VV, true>), dim3(g.n_items), blk,                                     \
-                         pr::Smem<1>::base + pr::Smem<1>::tile, st, pl.d_probs, g.d_items);                          \
+                         pr::Smem<1>::base, st, pl.d_probs, g.d_items);                                               \
     else                                                                                                             \
       hipLaunchKernelGGL((pr::k_track_walk<B, VV>), dim3(g.n_items), blk, pr::Smem<B>::base, st, pl.d_probs,         \
                          g.d_items);                                                                                 \

@@ -110,6 +110,22 @@ __device__ inline Blk load_blk(const double2* __restrict__ img, const Ln& x) {
   return b;
 }
 
+// One 16 x 16 image (row-major, element [i][j] at 16 i + j) loaded transposed:
+// lane (c, q) takes [c][q + 4r] into register r, so the block holds the image's
+// transpose (an image of U^T gives U, an A operand of op C then reads U^dag).
+// Each load is sixteen 64-byte runs instead of one 1 KiB run.
+__device__ inline Blk load_blk_t(const double2* __restrict__ img, const Ln& x) {
+  Blk b;
+  const int base = x.c * 16 + x.q;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    double2 v = gload2(img, base + 4 * r);
+    b.re[r] = v.x;
+    b.im[r] = v.y;
+  }
+  return b;
+}
+
 __device__ inline void store_blk(double2* __restrict__ img, const Blk& b, const Ln& x) {
   const size_t base = (size_t)x.w * 256;
 #pragma unroll
