@@ -547,7 +547,9 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
 // FP64 segment walk runs faster at 3 waves (168 VGPRs, no spills) than at 4
 // (128 VGPRs with spills); the mixed walk is equal at 3 and 4 and keeps 4;
 // the FP64 gradient walk fits 128 VGPRs without spills and gains 0.7% at 4
-// (150 VGPRs, 3 waves, when unconstrained).
+// (150 VGPRs, 3 waves, when unconstrained); the mixed gradient walk spills 28
+// bytes at 4 (130 VGPRs unconstrained) and is still 4% faster there. The
+// d <= 16 tracking walks fit 4 waves without spills.
 #ifndef PR_SEGMENT_WAVES
 #define PR_SEGMENT_WAVES 3
 #endif
@@ -556,6 +558,9 @@ __device__ inline Blk exact_direction(const ProbDev& P, int m, int k, const doub
 #endif
 #ifndef PR_GRADIENT_WAVES
 #define PR_GRADIENT_WAVES 4
+#endif
+#ifndef PR_MIXED_GRADIENT_WAVES
+#define PR_MIXED_GRADIENT_WAVES 4
 #endif
 // CLOSED: every problem of the launch has closed dynamics (anti-Hermitian
 // generators), so the d <= 16 exp takes the register-only form unconditionally.
@@ -1165,7 +1170,7 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
 // forms the increments C D and D^dag M on FP32 MFMA and accumulates C and M in
 // FP64 (held in the FP32 own layout); the traces are FP64.
 template <int V, bool REG>
-__global__ __launch_bounds__(64) void k_gradient_mx(const ProbDev* __restrict__ probs,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_GRADIENT_WAVES))) void k_gradient_mx(const ProbDev* __restrict__ probs,
                                                     const WorkItem* __restrict__ items,
                                                     const double* __restrict__ u) {
   const WorkItem it = items[blockIdx.x];
@@ -1403,7 +1408,8 @@ __device__ inline Blk outer_pairs(const double2* LZ, int np, const Ln& x) {
 // operands from registers (in the packed walk only Z goes through LDS, for M);
 // REG keeps the trace operand in registers for the whole walk.
 template <int B, int V, bool PACK = false, bool REG = false>
-__global__ __launch_bounds__(64 * B * B) void k_track_walk(const ProbDev* __restrict__ probs,
+__global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B == 1 ? 4 : 0))) void k_track_walk(
+    const ProbDev* __restrict__ probs,
                                                            const WorkItem* __restrict__ items) {
   const WorkItem it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
