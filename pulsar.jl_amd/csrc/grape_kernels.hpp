@@ -1344,11 +1344,18 @@ __global__ __launch_bounds__(64 * B * B) void k_track_scan(const ProbDev* __rest
   Blk S = load_blk(P.Psi0, x);
   double f = 0;
   for (int s = 0; s < P.nseg; s++) {
-    const Blk Y = mm<B, N, N>(load_blk(P.Zseg + base + s * MB, x), S, LA, LB, x);  // Z_s Psi
+    Blk Y;
+    if constexpr (B == 1)  // Z_s and T_s read transposed: op T gives register A operands
+      Y = mm<1, T, N>(load_blk_t(P.Zseg + base + s * MB, x), S, LA, LB, x);  // Z_s Psi
+    else
+      Y = mm<B, N, N>(load_blk(P.Zseg + base + s * MB, x), S, LA, LB, x);
 #pragma unroll
     for (int r = 0; r < 4; r++) f += S.re[r] * Y.re[r] + S.im[r] * Y.im[r];
     store_blk(P.Zseg + base + s * MB, axpby(2.0, Y, 0.0, Y), x);  // local co-state at the segment start
-    S = mm<B, N, N>(load_blk(P.Tseg + base + s * MB, x), S, LA, LB, x);
+    if constexpr (B == 1)
+      S = mm<1, T, N>(load_blk_t(P.Tseg + base + s * MB, x), S, LA, LB, x);
+    else
+      S = mm<B, N, N>(load_blk(P.Tseg + base + s * MB, x), S, LA, LB, x);
     store_blk(P.Pend + base + s * MB, S, x);
   }
   const double2 fs = block_sum2<B>(make_double2(f, 0.0), red, x);
