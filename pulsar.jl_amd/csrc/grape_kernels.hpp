@@ -1341,30 +1341,44 @@ __global__ __launch_bounds__(64 * B * B) void k_track_scan(const ProbDev* __rest
   smem_parts<B>(LA, LB, red, zb);
   constexpr size_t MB = (size_t)B * B * 256;
   const size_t base = (size_t)m * P.nseg * MB;
+  // Z_s and T_s as A operands: for B == 1 read transposed, so op T takes them
+  // from registers; each step's pair is fetched one segment ahead
+  constexpr int OP = B == 1 ? T : N;
+  auto ld = [&](const double2* img) {
+    if constexpr (B == 1)
+      return load_blk_t(img, x);
+    else
+      return load_blk(img, x);
+  };
   Blk S = load_blk(P.Psi0, x);
   double f = 0;
+  Blk Zn = ld(P.Zseg + base), Tn = ld(P.Tseg + base);
   for (int s = 0; s < P.nseg; s++) {
-    Blk Y;
-    if constexpr (B == 1)  // Z_s and T_s read transposed: op T gives register A operands
-      Y = mm<1, T, N>(load_blk_t(P.Zseg + base + s * MB, x), S, LA, LB, x);  // Z_s Psi
-    else
-      Y = mm<B, N, N>(load_blk(P.Zseg + base + s * MB, x), S, LA, LB, x);
+    const Blk Zc = Zn, Tc = Tn;
+    if (s + 1 < P.nseg) {
+      Zn = ld(P.Zseg + base + (s + 1) * MB);
+      Tn = ld(P.Tseg + base + (s + 1) * MB);
+    }
+    const Blk Y = mm<B, OP, N>(Zc, S, LA, LB, x);  // Z_s Psi
 #pragma unroll
     for (int r = 0; r < 4; r++) f += S.re[r] * Y.re[r] + S.im[r] * Y.im[r];
     store_blk(P.Zseg + base + s * MB, axpby(2.0, Y, 0.0, Y), x);  // local co-state at the segment start
-    if constexpr (B == 1)
-      S = mm<1, T, N>(load_blk_t(P.Tseg + base + s * MB, x), S, LA, LB, x);
-    else
-      S = mm<B, N, N>(load_blk(P.Tseg + base + s * MB, x), S, LA, LB, x);
+    S = mm<B, OP, N>(Tc, S, LA, LB, x);
     store_blk(P.Pend + base + s * MB, S, x);
   }
   const double2 fs = block_sum2<B>(make_double2(f, 0.0), red, x);
   if (threadIdx.x == 0) gptr_w(P.Fm)[m] = fs.x;
   Blk L = zero_blk();
-  for (int s = P.nseg - 1; s >= 0; s--) {
+  const int sl = P.nseg - 1;
+  Blk Tb = load_blk(P.Tseg + base + sl * MB, x), Zb = load_blk(P.Zseg + base + sl * MB, x);
+  for (int s = sl; s >= 0; s--) {
     store_blk(P.Lend + base + s * MB, L, x);
-    const Blk TL = mm<B, C, N>(load_blk(P.Tseg + base + s * MB, x), L, LA, LB, x);  // T_s^dag L
-    const Blk Ll = load_blk(P.Zseg + base + s * MB, x);
+    const Blk Tc = Tb, Ll = Zb;
+    if (s > 0) {
+      Tb = load_blk(P.Tseg + base + (s - 1) * MB, x);
+      Zb = load_blk(P.Zseg + base + (s - 1) * MB, x);
+    }
+    const Blk TL = mm<B, C, N>(Tc, L, LA, LB, x);  // T_s^dag L
     L.re = TL.re + Ll.re;
     L.im = TL.im + Ll.im;
   }
