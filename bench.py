@@ -58,70 +58,38 @@ MEASURED_TRAFFIC_BYTES = {"k_segment": 0.022e9 + 2.131e9, "k_gradient": 2.141e9 
                           "k_track_walk": 2.166e9 + 0.040e9}
 
 
-def probe_rates():
-    """Sustained TFLOP/s of dense FP64 MFMA, FP32 MFMA, FP64 VALU FMA, and FP64
-    MFMA interleaved with an equal issue time of FP64 VALU FMAs. If the two
-    overlapped, the interleaved loop would take as long as either alone; if
-    they serialise, it takes the sum (rate = about half the two rates' mean)."""
-    import ctypes
-
-    lib = ctypes.CDLL(os.path.join(ROOT, "pulsar.jl_amd", "pulsar_rocm", "libpulsar_probe.so"))
-    lib.pr_probe_mfma.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
-    out = {}
-    for name, kind in (("mfma_f64", 0), ("mfma_f32", 1), ("valu_f64_fma", 2), ("mfma_f64_with_valu_f64", 3)):
-        v = ctypes.c_double(0)
-        rc = lib.pr_probe_mfma(kind, ctypes.byref(v))
-        if rc != 0:
-            raise RuntimeError(f"pr_probe_mfma({kind}) failed with {rc}")
-        out[name + "_tflops"] = round(v.value, 2)
-    # time of the interleaved loop relative to MFMA alone and VALU alone
-    flops_mfma = 2.0 * 16 * 16 * 4 * 4
-    flops_valu = 2.0 * 64 * 64
-    t_m = flops_mfma / out["mfma_f64_tflops"]
-    t_v = flops_valu / out["valu_f64_fma_tflops"]
-    t_b = (flops_mfma + flops_valu) / out["mfma_f64_with_valu_f64_tflops"]
-    out["interleaved_time_over_sum_of_times"] = round(t_b / (t_m + t_v), 3)
-    out["interleaved_time_over_max_of_times"] = round(t_b / max(t_m, t_v), 3)
-    return out
+def product_flops(d):
+    """Real flops of one complex d x d product as this library runs it: three
+    real d x d products (Gauss), 6 d^3. No complex product on real matrix cores
+    needs fewer, so a fraction of the MFMA peak computed from this count cannot
+    exceed the fraction of time the matrix cores are busy."""
+    return 6 * d ** 3
 
 
-def measured_mfma_peaks():
-    """Sustained dense MFMA TFLOP/s of this device (libpulsar_probe.so): FP64 and
-    FP32 16x16x4 loops with eight wavefronts per SIMD and no memory traffic."""
-    import ctypes
-
-    path = os.path.join(ROOT, "pulsar.jl_amd", "pulsar_rocm", "libpulsar_probe.so")
-    lib = ctypes.CDLL(path)
-    lib.pr_probe_mfma.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
-    out = {}
-    for name, kind in (("fp64", 0), ("fp32", 1)):
-        v = ctypes.c_double(0)
-        rc = lib.pr_probe_mfma(kind, ctypes.byref(v))
-        if rc != 0:
-            raise RuntimeError(f"pr_probe_mfma({name}) failed with {rc}")
-        out[name] = v.value
-    return out
+def products_per_launch(kernel, counts, units, lindblad=False):
+    """Complex products one launch needs, from the exp counts the library
+    reports (pr_plan_exp_counts: member-slices at degree 6, at degree 9,
+    squarings). k_segment: the exp (Paterson-Stockmeyer, 3 products at degree
+    6, 4 at degree 9, one per squaring) and the chain product U_k T; k_gradient:
+    the walk's M_k = C_k U_k^dag and C_{k+1} = U_k M_k (unitary dynamics), or
+    the forward state, P_k Lambda^dag and U_k^dag Lambda (Lindblad)."""
+    n6, n9, nsq = (int(c) for c in counts)
+    if kernel == "k_segment":
+        return 3 * n6 + 4 * n9 + nsq + units
+    return (3 if lindblad else 2) * units
 
 
-def executed_mfma_flops(kernel, precision, M, nt):
-    """MFMA flops the d=16 kernels issue per launch on this workload (every
-    slice takes the degree-6 polynomial, no squarings): 3 real 16x16x16
-    products per complex product (Gauss), each 4 MFMAs of 2048 flops;
-    k_segment: 3 exp products + 1 chain product, k_gradient: 2 walk products."""
-    per_product = 3 * 4 * 2048
-    products = 4 if kernel == "k_segment" else 2
-    return products * per_product * M * nt
+def vector_flops(d, nc):
+    """VALU flops per member-slice outside the products: assembly of the
+    generator (n_c real-weighted complex axpys, 4 n_c d^2) in k_segment and the
+    n_c control traces (8 n_c d^2) in k_gradient."""
+    return {"k_segment": 4 * nc * d ** 2, "k_gradient": 8 * nc * d ** 2}
 
 
-def flops_model(d, nc):
-    """Algorithmic flops per member-slice: 9 complex d x d products (nominal
-    6 for exp, forward, backward, gradient) + control traces and assembly,
-    split over the two main kernels."""
-    total = 72 * d**3 + 16 * nc * d**2
-    segment = 56 * d**3 + 8 * nc * d**2  # exp (6 products) + chain product + assembly
-    gradient = 16 * d**3 + 8 * nc * d**2  # adjoint product + contraction product + traces
-    assert segment + gradient == total
-    return total, segment, gradient
+def survey_flops(d, nc):
+    """The fixed comparison model of the workload description: 72 d^3 + 16 n_c d^2
+    per member-slice (nine 8 d^3 complex products, six of them for exp)."""
+    return 72 * d ** 3 + 16 * nc * d ** 2
 
 
 def cpu_baseline(p, u, n_sample, threads):
@@ -210,12 +178,20 @@ def other_workload(args):
         plan.eval_device(d_u, d_F, d_G)
     plan.synchronize()
     elapsed = time.perf_counter() - t0
+    # the host-buffer entry point (pr_eval: one packed copy of the controls up,
+    # one of the gradients down, through a pinned staging buffer)
+    plan.eval(us)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.eval(us)
+    host_elapsed = time.perf_counter() - t0
     plan.close()
     rate = args.steps / elapsed
     line = {"workload": args.workload, "description": desc, "evals_per_s": round(rate, 3),
             "ms_per_eval": round(1e3 * elapsed / args.steps, 4), "steps": args.steps,
             "algorithmic_tflops": round(flops * rate / 1e12, 3),
-            "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}}
+            "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+            "host_buffers_ms_per_eval": round(1e3 * host_elapsed / args.steps, 4)}
     if args.workload == "four_qubit_tracking" and "gradient" in kt:
         # the per-segment backward walk reads every U_k image once (16 x 16
         # complex FP64, 4 KiB) and writes the member's gradient row (n_c
@@ -316,14 +292,9 @@ def main():
                     help="skip the mixed-precision form that the FP64 run also times")
     ap.add_argument("--segment", type=int, default=0,
                     help="slices per time segment of the chain scan (0: the library's choice)")
-    ap.add_argument("--probe", action="store_true",
-                    help="only print the device's sustained MFMA and FP64 VALU rates and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
-    if args.probe:
-        print(json.dumps(probe_rates()))
-        return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
@@ -348,18 +319,14 @@ def main():
     evals_s = args.steps / elapsed
     M, nt, d, nc = p.n_members, p.n_t, p.dim, p.n_ctrl
     units = M * nt
-    f_tot, f_seg, f_grad = flops_model(d, nc)
     seg_ms = kt["segment"][0] / max(kt["segment"][1], 1)
     grad_ms = kt["gradient"][0] / max(kt["gradient"][1], 1)
-    dominant, dom_ms, dom_flops = ("k_segment", seg_ms, f_seg) if seg_ms >= grad_ms else ("k_gradient", grad_ms, f_grad)
-    achieved = dom_flops * units / (dom_ms * 1e-3) / 1e12
-    peak = round(mfma_peak(dominant, args.precision), 1)
-    probe = measured_mfma_peaks()
-    if args.precision == "fp64" or dominant != "k_segment":
-        mpeak = probe["fp64"] if args.precision == "fp64" else probe["fp32"]
-    else:
-        mpeak = 4.0 / (3.0 / probe["fp32"] + 1.0 / probe["fp64"])
-
+    dominant, dom_ms = ("k_segment", seg_ms) if seg_ms >= grad_ms else ("k_gradient", grad_ms)
+    counts = plan.exp_counts()[0]  # member-slices at degree 6 and 9, squarings
+    prods = products_per_launch(dominant, counts, units)
+    dom_flops = prods * product_flops(d)
+    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    peak = mfma_peak(dominant, args.precision)
     # accuracy against the FP64 oracle on a member sample (same inputs)
     out = {
         "metric": "GRAPE fidelity+gradient evals/s (4q x 256 ens) at |dF|<=1e-9 vs FP64",
@@ -381,12 +348,13 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                      "traffic": MEASURED_TRAFFIC_BYTES[dominant] if prec == PR_FP64 else None,
                      "kernel_ms": round(dom_ms, 4),
-                     "algorithmic_flops_per_launch": dom_flops * units,
-                     "measured_dense_mfma_peak": round(mpeak, 1),
-                     "frac_of_measured_peak": round(achieved / mpeak, 4),
-                     "executed_mfma_tflops": round(executed_mfma_flops(dominant, args.precision, M, nt)
-                                                   / (dom_ms * 1e-3) / 1e12, 3)},
-        "eval_tflops_algorithmic": round(f_tot * units * evals_s / 1e12, 3),
+                     "flops_per_launch": dom_flops,
+                     "work_model": f"{prods} complex {d}x{d} products per launch ({prods / units:g} per "
+                                   f"member-slice over {units} member-slices) x 6 d^3 flops (three real "
+                                   f"products each; exp counts {[int(c) for c in counts]}); "
+                                   f"plus {vector_flops(d, nc)[dominant]} VALU flops per member-slice outside "
+                                   "the products, not counted against the MFMA peak"},
+        "survey_model_tflops": round(survey_flops(d, nc) * units * evals_s / 1e12, 3),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
     }
     if args.cpu_sample > 0:

@@ -87,6 +87,10 @@ extern "C" {
                                   (MRPhysics.jl:337-351): F_m as PR_PAIRS_SQUARE; gradient
                                   2 Im(conj(z) <Q_k psi_t| -i dt A_j |P_k psi_i>), states as given.
                                   PR_GRAD_REFERENCE */
+#define PR_PAIRS_MODULUS 7 /* closed dynamics: F_m = sum_s |<t_s|U_total|i_s>| (Objectives.jl:222);
+                              gradient of the MR state kernel with the :modulus prefactor
+                              dt Im(inner / |z_s|), 0 when |z_s| < 1e-14 (Objectives.jl:356-361;
+                              accepted by GRAPEState via OptControl.jl:647) */
 #define PR_GATE_PROPAGATORS 4 /* F as PR_GATE; gradient of _grape_gradient_from_propagators
                                  (src/Physics/MRPhysics.jl:312-336, the MAS/DNP gate gradient):
                                  G_jk = 2 Re(conj(Phi) Tr(Q_k^dag (-i dt A_j) P_k)) / dim^2,
@@ -155,7 +159,7 @@ typedef struct {
  * diag[4*p + 0] = max over members of ||U_total^dag U_total - I||_max (closed)
  *                 or 0 (Lindblad: the channel is not unitary)
  * diag[4*p + 1] = largest scaling exponent used by exp over the problem
- * diag[4*p + 2] = Taylor degree used by exp
+ * diag[4*p + 2] = largest Taylor degree used by exp (6 or 9)
  * diag[4*p + 3] = number of time segments of the chain scan */
 #define PR_DIAG_PER_PROBLEM 4
 
@@ -199,6 +203,13 @@ int pr_plan_synchronize(void* plan);
 /* Total number of control values of the plan (sum of n_ctrl*n_t), i.e. the
  * length of d_controls / d_G for pr_eval_device. */
 long long pr_plan_n_params(void* plan);
+
+/* Exp work of the last evaluation (no counterpart in the reference; it
+ * serves work models of the kernels): per problem p, counts[3p] member-slices
+ * whose exp took the degree-6 polynomial, counts[3p + 1] those that took
+ * degree 9, counts[3p + 2] the squarings over all member-slices. Waits for the
+ * plan's stream. Tracking problems report zeros. */
+int pr_plan_exp_counts(void* plan, long long* counts);
 
 /* Per-kernel timing (no counterpart in the reference, which has no device
  * profiling): when enabled, every evaluation records a HIP event pair on its

@@ -634,6 +634,9 @@ static int pairs_member(const pr_problem* p, int m, const double* u, int grad_mo
   size_t dd = (size_t)d * d;
   int closed = p->dynamics == PR_CLOSED;
   int square = p->objective == PR_PAIRS_SQUARE || p->objective == PR_STATE || p->objective == PR_STATE_PROPAGATORS;
+  /* :modulus (closed): F = |z| (Objectives.jl:222), prefactor dt Im(inner / |z|),
+   * 0 for |z| < 1e-14 (Objectives.jl:356-361) */
+  int modulus = p->objective == PR_PAIRS_MODULUS;
   /* PR_STATE_PROPAGATORS: the state branch of _grape_gradient_from_propagators
    * (MRPhysics.jl:337-351), 2 Im(conj(z) <lambda_k| -i dt A |psi_k>), lambda_k
    * with slice k included */
@@ -656,7 +659,7 @@ static int pairs_member(const pr_problem* p, int m, const double* u, int grad_mo
     for (int k = 0; k < nt; k++) mv(d, W->Us + k * dd, psi + (size_t)k * d, psi + (size_t)(k + 1) * d);
     cx z = vdot(d, lam + (size_t)nt * d, psi + (size_t)nt * d);
     zs[s] = z;
-    F += square ? creal(conj(z) * z) : creal(z);
+    F += square ? creal(conj(z) * z) : modulus ? cabs(z) : creal(z);
     if (!Gm) continue;
     for (int k = nt - 1; k >= 0; k--) mv_c(d, W->Us + k * dd, lam + (size_t)(k + 1) * d, lam + (size_t)k * d);
     if (grad_mode == PR_GRAD_EXACT) continue;
@@ -669,6 +672,7 @@ static int pairs_member(const pr_problem* p, int m, const double* u, int grad_mo
         cx inner = vdot(d, lv, W->tmpv);
         double g;
         if (helper) g = 2.0 * cimag(conj(z) * (-I * dtk) * inner);
+        else if (modulus) g = cabs(z) < 1e-14 ? 0.0 : dtk * cimag(inner / cabs(z));
         else if (closed) g = square ? 2.0 * dtk * cimag(conj(z) * inner) : dtk * cimag(inner);
         else g = square ? 2.0 * dtk * creal(conj(z) * inner) : dtk * creal(inner);
         Gm[j + (size_t)k * nc] += g;
@@ -683,6 +687,8 @@ static int pairs_member(const pr_problem* p, int m, const double* u, int grad_mo
       memset(M, 0, sizeof(cx) * dd);
       for (int s = 0; s < np; s++) {
         cx wgt = square ? 2.0 * conj(zs[s]) : 1.0;
+        /* :modulus: the exact derivative of |z| */
+        if (modulus) wgt = cabs(zs[s]) < 1e-14 ? 0.0 : conj(zs[s]) / cabs(zs[s]);
         const cx* pv = W->psi + (size_t)s * d * (nt + 1) + (size_t)k * d;
         const cx* lv = W->lam + (size_t)s * d * (nt + 1) + (size_t)(k + 1) * d;
         for (int b = 0; b < d; b++)
@@ -753,7 +759,8 @@ static int check_problem(const pr_problem* p) {
   if (is_gate(p)) {
     if (!p->U_target || p->dynamics != PR_CLOSED) return PR_ERR_ARG;
   } else if (p->objective == PR_PAIRS_SQUARE || p->objective == PR_PAIRS_REAL ||
-             ((p->objective == PR_STATE || p->objective == PR_STATE_PROPAGATORS) && p->dynamics == PR_CLOSED)) {
+             ((p->objective == PR_STATE || p->objective == PR_STATE_PROPAGATORS || p->objective == PR_PAIRS_MODULUS) &&
+              p->dynamics == PR_CLOSED)) {
     if (p->n_pairs < 1 || !p->psi_init || !p->psi_targ) return PR_ERR_ARG;
   } else {
     return PR_ERR_ARG;

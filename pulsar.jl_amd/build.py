@@ -16,9 +16,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "pulsar_rocm", "libpulsar_rocm.so")
 SOURCES = ["pulsar_rocm.hip"]
-# the benchmark's MFMA throughput probe, a library of its own
-PROBE_OUT = os.path.join(HERE, "pulsar_rocm", "libpulsar_probe.so")
-PROBE_SOURCES = ["mfma_probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
@@ -55,5 +52,3 @@ if __name__ == "__main__":
     out = args[args.index("--out") + 1] if "--out" in args else OUT
     defs = [a[2:] for a in args if a.startswith("-D")]
     print(build(debug="--debug" in args, verbose=True, out=out, defines=defs))
-    if "--out" not in args:
-        print(build(debug="--debug" in args, verbose=True, out=PROBE_OUT, sources=PROBE_SOURCES))

@@ -246,11 +246,13 @@ __global__ __launch_bounds__(1024) void k_segment_big(const ProbDev* __restrict_
   const int m = it.member;
   big::store_eye_blocks(T, K_::NB, B);
   __syncthreads();
-  int smax = 0;
+  int smax = 0, n9 = 0, nsq = 0;
   for (int k = it.k0; k < it.k1; k++) {
     bool deg6;
     const int s = choose_scaling(big::assemble_norm<B>(P, m, k, u, X, colsum, red), deg6);
     smax = max(smax, s);
+    n9 += deg6 ? 0 : 1;
+    nsq += s;
     if (s) big::scale_blocks<B>(X, ldexp(1.0, -s));
     big::mm<B, N, N>(X2, X, X, LP);
     big::mm<B, N, N>(X3, X, X2, LP);
@@ -277,7 +279,7 @@ __global__ __launch_bounds__(1024) void k_segment_big(const ProbDev* __restrict_
     const Ln x = big::blk_info<B>(b);
     store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * MB, load_blk(T, x), x);
   }
-  if (threadIdx.x == 0) P.smax[(size_t)m * P.nseg + it.seg] = smax;
+  store_exp_counts(P, m, it.seg, smax, n9, nsq);
 }
 
 template <int B>

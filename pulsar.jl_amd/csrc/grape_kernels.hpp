@@ -115,7 +115,8 @@ struct ProbDev {
   double* Fm;                 // [M]
   double* Gm;                 // [M][nc*nt]
   double* udef;               // [M] ||U^dag U - I||_max (closed)
-  int* smax;                  // [M*nseg] largest scaling exponent per segment
+  int* smax;                  // [M*nseg][3] per segment: largest scaling exponent, slices
+                              // that took the degree-9 polynomial, squarings (exp work counts)
   // tracking objective: ck_of[k] = index of the checkpoint matrix of slice k
   // or -1; Ck holds the images of C_c = sum w Herm(rho_target) per checkpoint
   // slice. Per member and segment: Zseg the checkpoint operator of the segment
@@ -135,6 +136,16 @@ struct WorkItem {
 // Length of slice k: the uniform dt from the problem record (a scalar load)
 // when the slices are uniform, else the per-slice array.
 __device__ inline double slice_dt(const ProbDev& P, int k) { return P.prescaled ? P.dt0 : P.dts[k]; }
+
+// exp work of one (member, segment) work item, for the diagnostics
+__device__ inline void store_exp_counts(const ProbDev& P, int m, int seg, int smax, int n9, int nsq) {
+  if (threadIdx.x == 0) {
+    int* c = P.smax + 3 * ((size_t)m * P.nseg + seg);
+    c[0] = smax;
+    c[1] = n9;
+    c[2] = nsq;
+  }
+}
 
 // exp truncation: the Taylor remainder of degree m is below 2^-53 relative
 // for ||X||_1 <= theta_m: ||X||^7/7! <= 1.1e-16 for ||X|| <= 0.01777 (m = 6),
@@ -598,7 +609,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
   // chain product U_k T for B == 1; staged through LB for wider blocks)
   Blk Tseg = eye_blk(x);
   Blk Z = zero_blk();
-  int smax = 0;
+  int smax = 0, n9 = 0, nsq = 0;
   // B == 1 with closed dynamics: X^T is assembled from the transposed tables,
   // so exp gives V = U^T in the register layout. The chain product U T takes
   // V as a transposed A operand and the stored image is V itself: no LDS.
@@ -618,6 +629,8 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
       X.im *= f;
     }
     smax = max(smax, s);
+    n9 += deg6 ? 0 : 1;
+    nsq += s;
     if constexpr (TR) {
       const Blk V = expm_ah1(X, s, deg6, LA, LB, x);         // U^T
       const Blk acc = gemm_prod<1, T, N>(LA, V, LB, Tseg, x);  // U T
@@ -663,7 +676,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
   }
   store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * NB * 256, Tseg, x);
   if constexpr (TRK) store_blk(P.Zseg + ((size_t)m * P.nseg + it.seg) * NB * 256, Z, x);
-  if (threadIdx.x == 0) P.smax[(size_t)m * P.nseg + it.seg] = smax;
+  store_exp_counts(P, m, it.seg, smax, n9, nsq);
 }
 
 // ------------------------------------------------------ mixed precision, d <= 16
@@ -753,7 +766,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
     Tseg = acc;
   }
   store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * 256, Tseg, x);
-  if (threadIdx.x == 0) P.smax[(size_t)m * P.nseg + it.seg] = smax;
+  store_exp_counts(P, m, it.seg, smax, 0, 0);
 }
 
 // ---------------------------------------------------------------- k_combine
@@ -825,21 +838,29 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
   }
   __syncthreads();
   const bool square = (P.obj == 1 || P.obj == 5 || P.obj == 6);
+  const bool modulus = P.obj == 7;
   if (threadIdx.x == 0) {
     double F = 0;
     for (int s = 0; s < P.npairs; s++) {
       double2 z = zb[s];
-      F += square ? (z.x * z.x + z.y * z.y) : z.x;
+      F += square ? (z.x * z.x + z.y * z.y) : modulus ? sqrt(z.x * z.x + z.y * z.y) : z.x;
     }
     P.Fm[m] = F;
   }
   // w_s = 2 z_s (square) or 1 (real); zero beyond the pairs. The propagator
   // helper's state branch (objective 6) takes Im where the others take Re:
-  // w_s = 2 i z_s
+  // w_s = 2 i z_s. :modulus (objective 7): the reference's prefactor
+  // dt Im(inner / |z|) is w_s = 1 / |z_s|; the exact gradient of |z_s| is
+  // w_s = z_s / |z_s|; both are zero for |z_s| < 1e-14
   auto wgt = [&](int s) -> double2 {
     if (s >= P.npairs) return make_double2(0, 0);
     double2 z = zb[s];
     if (P.obj == 6) return make_double2(-2 * z.y, 2 * z.x);
+    if (modulus) {
+      const double a = sqrt(z.x * z.x + z.y * z.y);
+      if (a < 1e-14) return make_double2(0, 0);
+      return P.grad_exact ? make_double2(z.x / a, z.y / a) : make_double2(1.0 / a, 0);
+    }
     return square ? make_double2(2 * z.x, 2 * z.y) : make_double2(1, 0);
   };
   if (P.unitary) {

@@ -283,6 +283,7 @@ struct Plan {
   double* d_F = nullptr;
   double* d_G = nullptr;
   double* d_part = nullptr;  // [RED_CHUNKS][n_params] partial member sums
+  double* h_stage = nullptr; // pinned host staging: n_params controls or gradients, then n_probs F
   struct Group {
     int B = 0, V = 0;
     bool lindblad = false;  // some problem of the group has Lindblad dynamics
@@ -331,6 +332,7 @@ struct Plan {
     for (auto e : ev_free) (void)hipEventDestroy(e);
     if (ev_open) (void)hipEventDestroy(ev_open);
     for (void* p : allocs) (void)hipFree(p);
+    if (h_stage) (void)hipHostFree(h_stage);
     if (stream) (void)hipStreamDestroy(stream);
   }
   template <class T>
@@ -368,10 +370,13 @@ int validate(const pr_problem& p, int idx) {
     double dev = unitary_dev(p.U_target, p.dim);
     if (dev > 1e-8) return fail(PR_ERR_ARG, "problem %d: target is not unitary: ||U^dag U - I||/dim = %g > 1e-8", idx, dev);
   } else if (p.objective == PR_PAIRS_SQUARE || p.objective == PR_PAIRS_REAL || p.objective == PR_STATE ||
-             p.objective == PR_STATE_PROPAGATORS) {
+             p.objective == PR_STATE_PROPAGATORS || p.objective == PR_PAIRS_MODULUS) {
     if (p.n_pairs < 1 || !p.psi_init || !p.psi_targ) return fail(PR_ERR_ARG, "problem %d: pair objective needs states", idx);
     if ((p.objective == PR_STATE || p.objective == PR_STATE_PROPAGATORS) && p.dynamics != PR_CLOSED)
       return fail(PR_ERR_UNSUPPORTED, "problem %d: the state objective needs closed dynamics", idx);
+    // lindblad_grad_prefactor has :square and :real only (Lindblad.jl:220-232)
+    if (p.objective == PR_PAIRS_MODULUS && p.dynamics != PR_CLOSED)
+      return fail(PR_ERR_UNSUPPORTED, "problem %d: the :modulus objective needs closed dynamics", idx);
     if (p.n_pairs > 16 * ((p.dim + 15) / 16))
       return fail(PR_ERR_UNSUPPORTED, "problem %d: n_pairs %d exceeds the padded dimension", idx, p.n_pairs);
   } else if (p.objective == PR_TRACKING) {
@@ -969,7 +974,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     if ((rc = pl.alloc(&D.Gm, (size_t)M * nc * nt))) return rc;
     if ((rc = pl.alloc(&D.udef, M))) return rc;
     HIPCHK(hipMemset(D.udef, 0, sizeof(double) * M));
-    if ((rc = pl.alloc(&D.smax, (size_t)M * S))) return rc;
+    if ((rc = pl.alloc(&D.smax, (size_t)3 * M * S))) return rc;
     if (track) {
       if ((rc = pl.alloc(&D.Zseg, (size_t)M * S * MB))) return rc;
       if ((rc = pl.alloc(&D.Pend, (size_t)M * S * MB))) return rc;
@@ -996,6 +1001,13 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   if ((rc = pl.alloc(&pl.d_F, n_probs))) return rc;
   if ((rc = pl.alloc(&pl.d_G, pl.n_params))) return rc;
   if ((rc = pl.alloc(&pl.d_part, (size_t)pr::RED_CHUNKS * pl.n_params))) return rc;
+  {
+    void* h = nullptr;
+    const hipError_t e = hipHostMalloc(&h, sizeof(double) * (pl.n_params + pl.n_probs), hipHostMallocDefault);
+    if (e != hipSuccess) return fail(PR_ERR_NOMEM, "hipHostMalloc(%lld doubles): %s", pl.n_params + pl.n_probs,
+                                     hipGetErrorString(e));
+    pl.h_stage = static_cast<double*>(h);
+  }
   for (int gi = 1; gi <= 12; gi++) {
     if (items[gi].empty()) continue;
     const int B = gi > 8 ? gi - 8 : gi;
@@ -1041,22 +1053,35 @@ int fill_diag(const Plan& pl, double* diag) {
   for (int p = 0; p < pl.n_probs; p++) {
     const pr::ProbDev& D = pl.host_probs[p];
     std::vector<double> ud(D.M);
-    std::vector<int> sm((size_t)D.M * D.nseg);
+    std::vector<int> sm((size_t)3 * D.M * D.nseg);
     HIPCHK(hipMemcpy(ud.data(), D.udef, sizeof(double) * D.M, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(sm.data(), D.smax, sizeof(int) * sm.size(), hipMemcpyDeviceToHost));
     double umax = 0;
     for (double v : ud) umax = std::max(umax, v);
-    int smax = 0;
-    for (int v : sm) smax = std::max(smax, v);
+    int smax = 0, any9 = 0;
+    for (size_t i = 0; i < sm.size(); i += 3) {
+      smax = std::max(smax, sm[i]);
+      any9 |= sm[i + 1] > 0;
+    }
     diag[PR_DIAG_PER_PROBLEM * p + 0] = umax;
     diag[PR_DIAG_PER_PROBLEM * p + 1] = smax;
-    diag[PR_DIAG_PER_PROBLEM * p + 2] = pr::TAYLOR_DEG;
+    diag[PR_DIAG_PER_PROBLEM * p + 2] = any9 ? 9 : 6;
     diag[PR_DIAG_PER_PROBLEM * p + 3] = D.nseg;
   }
   return PR_OK;
 }
 
 }  // namespace
+
+// Every problem's controls packed back to back into the plan's pinned staging
+// buffer, so that one host-to-device copy moves them all.
+int stage_controls(Plan& pl, const double* const* controls, const char* who) {
+  for (int p = 0; p < pl.n_probs; p++) {
+    if (!controls[p]) return fail(PR_ERR_ARG, "%s: controls[%d] is NULL", who, p);
+    memcpy(pl.h_stage + pl.coff[p], controls[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]));
+  }
+  return PR_OK;
+}
 
 extern "C" {
 
@@ -1134,20 +1159,20 @@ int pr_eval(void* plan, const double* const* controls, double* F, double* const*
   if (!plan || !controls || !F) return fail(PR_ERR_ARG, "pr_eval: plan, controls and F are required");
   Plan& pl = *static_cast<Plan*>(plan);
   if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
-  for (int p = 0; p < pl.n_probs; p++) {
-    if (!controls[p]) return fail(PR_ERR_ARG, "pr_eval: controls[%d] is NULL", p);
-    HIPCHK(hipMemcpyAsync(pl.d_controls + pl.coff[p], controls[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]),
-                          hipMemcpyHostToDevice, pl.stream));
-  }
-  int rc = run_graph(pl, pl.d_controls, pl.d_F, G ? pl.d_G : nullptr, pl.stream);
+  int rc = stage_controls(pl, controls, "pr_eval");
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(F, pl.d_F, sizeof(double) * pl.n_probs, hipMemcpyDeviceToHost, pl.stream));
+  HIPCHK(hipMemcpyAsync(pl.d_controls, pl.h_stage, sizeof(double) * pl.n_params, hipMemcpyHostToDevice, pl.stream));
+  rc = run_graph(pl, pl.d_controls, pl.d_F, G ? pl.d_G : nullptr, pl.stream);
+  if (rc) return rc;
+  // F and G come back in one copy each, through the pinned staging buffer
+  HIPCHK(hipMemcpyAsync(pl.h_stage + pl.n_params, pl.d_F, sizeof(double) * pl.n_probs, hipMemcpyDeviceToHost,
+                        pl.stream));
+  if (G) HIPCHK(hipMemcpyAsync(pl.h_stage, pl.d_G, sizeof(double) * pl.n_params, hipMemcpyDeviceToHost, pl.stream));
+  HIPCHK(hipStreamSynchronize(pl.stream));
+  memcpy(F, pl.h_stage + pl.n_params, sizeof(double) * pl.n_probs);
   if (G)
     for (int p = 0; p < pl.n_probs; p++)
-      if (G[p])
-        HIPCHK(hipMemcpyAsync(G[p], pl.d_G + pl.coff[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]),
-                              hipMemcpyDeviceToHost, pl.stream));
-  HIPCHK(hipStreamSynchronize(pl.stream));
+      if (G[p]) memcpy(G[p], pl.h_stage + pl.coff[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]));
   if (diag) return fill_diag(pl, diag);
   return PR_OK;
 }
@@ -1174,11 +1199,9 @@ int pr_plan_upload_controls(void* plan, const double* const* controls) {
   if (!plan || !controls) return fail(PR_ERR_ARG, "pr_plan_upload_controls: NULL argument");
   Plan& pl = *static_cast<Plan*>(plan);
   if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
-  for (int p = 0; p < pl.n_probs; p++) {
-    if (!controls[p]) return fail(PR_ERR_ARG, "pr_plan_upload_controls: controls[%d] is NULL", p);
-    HIPCHK(hipMemcpyAsync(pl.d_controls + pl.coff[p], controls[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]),
-                          hipMemcpyHostToDevice, pl.stream));
-  }
+  int rc = stage_controls(pl, controls, "pr_plan_upload_controls");
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(pl.d_controls, pl.h_stage, sizeof(double) * pl.n_params, hipMemcpyHostToDevice, pl.stream));
   HIPCHK(hipStreamSynchronize(pl.stream));
   return PR_OK;
 }
@@ -1213,6 +1236,27 @@ int pr_plan_kernel_times(void* plan, double* ms, long long* launches) {
     launches[c] = pl.acc_n[c];
     pl.acc_ms[c] = 0;
     pl.acc_n[c] = 0;
+  }
+  return PR_OK;
+}
+
+int pr_plan_exp_counts(void* plan, long long* counts) {
+  if (!plan || !counts) return fail(PR_ERR_ARG, "pr_plan_exp_counts: NULL argument");
+  Plan& pl = *static_cast<Plan*>(plan);
+  if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
+  HIPCHK(hipStreamSynchronize(pl.stream));
+  for (int p = 0; p < pl.n_probs; p++) {
+    const pr::ProbDev& D = pl.host_probs[p];
+    std::vector<int> sm((size_t)3 * D.M * D.nseg);
+    HIPCHK(hipMemcpy(sm.data(), D.smax, sizeof(int) * sm.size(), hipMemcpyDeviceToHost));
+    long long n9 = 0, nsq = 0;
+    for (size_t i = 0; i < sm.size(); i += 3) {
+      n9 += sm[i + 1];
+      nsq += sm[i + 2];
+    }
+    counts[3 * p + 0] = (long long)D.M * D.nt - n9;
+    counts[3 * p + 1] = n9;
+    counts[3 * p + 2] = nsq;
   }
   return PR_OK;
 }

@@ -23,6 +23,11 @@ const PR_TRACKING = Cint(3)
 const PR_GATE_PROPAGATORS = Cint(4)
 const PR_STATE = Cint(5)   # compute_grape_gradient with a state target (Gradients.jl:357-399)
 const PR_STATE_PROPAGATORS = Cint(6)   # state branch of the MAS/DNP propagator helper
+const PR_PAIRS_MODULUS = Cint(7)       # :modulus state pairs (Objectives.jl:222, 356-361)
+const PR_FP64 = Cint(0)
+const PR_MIXED = Cint(1)
+const PR_GRAD_REFERENCE = Cint(0)
+const PR_GRAD_EXACT = Cint(1)
 
 # Memory layout identical to `pr_problem` (include/pulsar_rocm.h).
 struct PRProblem

@@ -5,12 +5,36 @@
 # Each hook keeps the reference signature and falls back to the CPU code on a
 # library error, as CUDABackend.jl:144-147 does.
 
+# Plans are cached per (system, target, shape, objective form): a GRAPE loop that
+# calls compute_fidelity / compute_grape_gradient on the same system and target
+# every iteration validates and uploads the operators once, and at C2 sizes
+# allocates the U_k images once. Keys are content hashes; a hit is confirmed by
+# comparing the stored inputs, so a hash collision cannot return a wrong plan.
+const _ROCM_PLAN_CACHE = Vector{Tuple{UInt, Any, ROCmPlan}}()
+const _ROCM_PLAN_CACHE_MAX = 8
+
+function _rocm_cached_plan(build::Function, key_inputs::Tuple)
+    h = hash(key_inputs)
+    for (kh, kin, plan) in _ROCM_PLAN_CACHE
+        kh == h && isequal(kin, key_inputs) && plan.handle != C_NULL && return plan
+    end
+    plan = build()
+    length(_ROCM_PLAN_CACHE) >= _ROCM_PLAN_CACHE_MAX && popfirst!(_ROCM_PLAN_CACHE)
+    push!(_ROCM_PLAN_CACHE, (h, deepcopy(key_inputs), plan))
+    return plan
+end
+
+_target_key(t::QuantumTarget) = (t.type, t.target_unitary, t.target_state, t.initial_state)
+
 # compute_fidelity(system, controls, target)  (src/Physics/Objectives.jl:632-639)
 function _rocm_compute_fidelity(system::AbstractQuantumSystem, controls::ControlSequence,
                                 target::QuantumTarget)::Float64
-    prob, roots = rocm_problem(system.H_drift, system.H_controls, controls.n_steps, controls.dt, target;
-                               state_normalize = false)
-    plan = ROCmPlan([prob], roots)
+    key = (:fidelity, system.H_drift, system.H_controls, controls.n_steps, controls.dt, _target_key(target))
+    plan = _rocm_cached_plan(key) do
+        prob, roots = rocm_problem(system.H_drift, system.H_controls, controls.n_steps, controls.dt, target;
+                                   state_normalize = false)
+        ROCmPlan([prob], roots)
+    end
     F, _ = rocm_eval(plan, [Matrix(controls.controls)]; grad = false)
     return F[1]
 end
@@ -18,10 +42,57 @@ end
 # compute_grape_gradient(system, controls, target)  (src/Physics/Gradients.jl:80-127)
 function _rocm_compute_grape_gradient(system::AbstractQuantumSystem, controls::ControlSequence,
                                       target::QuantumTarget)::Matrix{Float64}
-    prob, roots = rocm_problem(system.H_drift, system.H_controls, controls.n_steps, controls.dt, target)
-    plan = ROCmPlan([prob], roots)
+    key = (:gradient, system.H_drift, system.H_controls, controls.n_steps, controls.dt, _target_key(target))
+    plan = _rocm_cached_plan(key) do
+        prob, roots = rocm_problem(system.H_drift, system.H_controls, controls.n_steps, controls.dt, target)
+        ROCmPlan([prob], roots)
+    end
     _, G = rocm_eval(plan, [Matrix(controls.controls)])
     return G[1]
+end
+
+# _fidelity_flat(u_flat, system, target, n_controls, n_timesteps, dt)
+# (src/Physics/AutoDiff.jl:254-280) with its exact gradient: the primal and
+# pullback of the ChainRulesCore rrule (rocm_zygote_rrule.jl). The objective is
+# compute_fidelity(U_total, target) (Objectives.jl:609-631): the gate fidelity,
+# or |<psi_t|U psi_i>|^2 with the states as given (psi_i = target_state when no
+# initial state is set), hence PR_GATE or PR_PAIRS_SQUARE with weight 1.
+function _rocm_fidelity_flat_and_gradient(u_flat::AbstractVector, system::AbstractQuantumSystem,
+                                          target::QuantumTarget, n_controls::Int, n_timesteps::Int,
+                                          dt::Float64; grad::Bool = true)
+    key = (:flat, system.H_drift, system.H_controls, n_timesteps, dt, _target_key(target))
+    plan = _rocm_cached_plan(key) do
+        H0 = ComplexF64.(system.H_drift)
+        HcA = cat([ComplexF64.(h) for h in system.H_controls]...; dims = 3)
+        d = size(H0, 1)
+        if target.type == "unitary"
+            Ut = ComplexF64.(target.target_unitary)
+            roots = Any[H0, HcA, Ut]
+            prob = PRProblem(d, n_controls, n_timesteps, dt, C_NULL, PR_CLOSED, PR_GATE, 1, _cptr(H0), 0,
+                             _cptr(HcA), C_NULL, C_NULL, _cptr(Ut), 0, C_NULL, C_NULL, 0, C_NULL, C_NULL, C_NULL)
+        else
+            ψt = ComplexF64.(target.target_state)
+            ψi = target.initial_state === nothing ? ψt : ComplexF64.(target.initial_state)
+            w1 = ones(1)
+            roots = Any[H0, HcA, ψi, ψt, w1]
+            prob = PRProblem(d, n_controls, n_timesteps, dt, C_NULL, PR_CLOSED, PR_PAIRS_SQUARE, 1, _cptr(H0), 0,
+                             _cptr(HcA), C_NULL, _cptr(w1), C_NULL, 1, _cptr(ψi), _cptr(ψt), 0, C_NULL, C_NULL,
+                             C_NULL)
+        end
+        ROCmPlan([prob], roots; grad_mode = PR_GRAD_EXACT)
+    end
+    F, G = rocm_eval(plan, [reshape(real.(Vector(u_flat)), n_controls, n_timesteps)]; grad = grad)
+    return F[1], vec(G[1])
+end
+
+# The device form of _fidelity_flat (same arguments, same value). The Zygote
+# extension differentiates this function instead of _fidelity_flat when
+# get_device() === :rocm; its rrule (rocm_zygote_rrule.jl) takes F and dF/du
+# from one device evaluation.
+function _rocm_fidelity_flat(u_flat::AbstractVector, system::AbstractQuantumSystem, target::QuantumTarget,
+                             n_controls::Int, n_timesteps::Int, dt::Float64)::Float64
+    F, _ = _rocm_fidelity_flat_and_gradient(u_flat, system, target, n_controls, n_timesteps, dt; grad = false)
+    return F
 end
 
 # The reference's entry points gain one branch each, e.g. in Objectives.jl:
@@ -72,11 +143,57 @@ function _rocm_ensemble_objective(systems::Vector{<:AbstractQuantumSystem}, targ
 end
 
 # grape_state_kernel / grape_lindblad_kernel  (src/Application/MR/GRAPEState.jl:73-85,
-# GRAPELindblad.jl:83-92) gain `elseif ctrl.backend == :rocm` branches that build
-# one pr_problem per call: members = [(H, p) for H in ctrl.drifts for p in ctrl.pwr_levels]
-# (H0_per_member = 1, ctrl_scale[m, :] .= p), pairs = ctrl.rho_init / ctrl.rho_targ,
-# dt_slices = ctrl.pulse_dt, objective = :square → PR_PAIRS_SQUARE, :real → PR_PAIRS_REAL;
-# the Lindblad branch passes ctrl._L_drifts / ctrl._L_controls with dynamics = PR_LINDBLAD.
+# GRAPELindblad.jl:83-92) gain `elseif ctrl.backend == :rocm` branches calling these:
+# one problem whose members are [(H, p) for H in ctrl.drifts for p in ctrl.pwr_levels]
+# (per-member drifts, ctrl_scale[m, :] .= p), pairs ctrl.rho_init / ctrl.rho_targ,
+# dt_slices ctrl.pulse_dt; the penalties are applied afterwards as the CPU paths do.
+function _rocm_mr_objective(fidelity::Symbol, lindblad::Bool)
+    fidelity === :square && return PR_PAIRS_SQUARE
+    fidelity === :real && return PR_PAIRS_REAL
+    !lindblad && fidelity === :modulus && return PR_PAIRS_MODULUS
+    lindblad && throw(ArgumentError("lindblad_grad_prefactor: ':$fidelity' not supported. Use :square or :real."))
+    throw(ArgumentError("ctrl.fidelity ':$fidelity' is not GRAPE-compatible. Use one of: real, square, modulus"))
+end
+
+function _rocm_mr_plan(ctrl, drifts, operators, rho_init, rho_targ, lindblad::Bool)
+    key = (:mr, lindblad, drifts, operators, rho_init, rho_targ, ctrl.pwr_levels, ctrl.pulse_dt, ctrl.fidelity)
+    return _rocm_cached_plan(key) do
+        members = [(H, p) for H in drifts for p in ctrl.pwr_levels]
+        M = length(members)
+        n_ctrl = length(operators)
+        d = size(operators[1], 1)
+        H0 = cat([ComplexF64.(H) for (H, _) in members]...; dims = 3)
+        HcA = cat([ComplexF64.(o) for o in operators]...; dims = 3)
+        sc = Matrix{Float64}(undef, n_ctrl, M)                        # C layout [M][n_ctrl]
+        for (m, (_, p)) in enumerate(members)
+            sc[:, m] .= p
+        end
+        ψ0 = reduce(hcat, [ComplexF64.(vec(r)) for r in rho_init])   # C layout [n_pairs][d]
+        ψt = reduce(hcat, [ComplexF64.(vec(r)) for r in rho_targ])
+        dts = Vector{Float64}(ctrl.pulse_dt)
+        roots = Any[H0, HcA, sc, ψ0, ψt, dts]
+        prob = PRProblem(d, n_ctrl, length(dts), 0.0, pointer(dts), lindblad ? PR_LINDBLAD : PR_CLOSED,
+                         _rocm_mr_objective(ctrl.fidelity, lindblad), M, _cptr(H0), 1, _cptr(HcA), _cptr(sc),
+                         C_NULL, C_NULL, length(rho_init), _cptr(ψ0), _cptr(ψt), 0, C_NULL, C_NULL, C_NULL)
+        ROCmPlan([prob], roots)
+    end
+end
+
+function _rocm_grape_state_kernel(waveform::Matrix{Float64}, ctrl)
+    plan = _rocm_mr_plan(ctrl, ctrl.drifts, ctrl.operators, ctrl.rho_init, ctrl.rho_targ, false)
+    F, G = rocm_eval(plan, [waveform])
+    fidelity = _apply_penalties!(F[1], G[1], waveform, ctrl)
+    return fidelity, G[1]
+end
+
+# the Liouvillians and vectorised states the reference precomputes in the
+# LindbladMRControl constructor (OptControl.jl:855-870)
+function _rocm_grape_lindblad_kernel(waveform::Matrix{Float64}, ctrl)
+    plan = _rocm_mr_plan(ctrl, ctrl._L_drifts, ctrl._L_controls, ctrl._sigma_init, ctrl._sigma_targ, true)
+    F, G = rocm_eval(plan, [waveform])
+    fidelity = _apply_penalties!(F[1], G[1], waveform, ctrl)
+    return fidelity, G[1]
+end
 
 # grape_tracking_kernel(waveform, ctrl)  (src/Application/MR/GRAPETracking.jl:145-157)
 # gains `elseif ctrl.backend == :rocm` → `_rocm_grape_tracking_kernel(waveform, ctrl)`:

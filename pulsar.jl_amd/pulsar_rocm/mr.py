@@ -24,7 +24,8 @@ from typing import List, Optional
 import numpy as np
 
 from . import native
-from .problem import PR_CLOSED, PR_GRAD_REFERENCE, PR_LINDBLAD, PR_PAIRS_REAL, PR_PAIRS_SQUARE, PR_TRACKING, Problem
+from .problem import (PR_CLOSED, PR_GRAD_REFERENCE, PR_LINDBLAD, PR_PAIRS_MODULUS, PR_PAIRS_REAL, PR_PAIRS_SQUARE,
+                      PR_TRACKING, Problem)
 
 
 def build_drift_liouvillian(H_drift, jump_ops=(), decay_rates=()):
@@ -61,13 +62,20 @@ def pure_state_to_vec_rho(psi):
     return vec_rho(np.outer(c, c.conj()))
 
 
-def _fid_code(fidelity: str) -> int:
+def _fid_code(fidelity: str, lindblad: bool = False) -> int:
+    """The device objective of an MR fidelity symbol: the GRAPE-compatible
+    types of _validate_ctrl (OptControl.jl:646-650) for closed dynamics, the
+    two of lindblad_grad_prefactor (Lindblad.jl:220-232) for Lindblad."""
     f = fidelity.lstrip(":")
     if f == "square":
         return PR_PAIRS_SQUARE
     if f == "real":
         return PR_PAIRS_REAL
-    raise ValueError(f"fidelity :{f} is not supported on the :rocm device (use :square or :real)")
+    if f == "modulus" and not lindblad:
+        return PR_PAIRS_MODULUS
+    if lindblad:
+        raise ValueError(f"lindblad_grad_prefactor: ':{f}' not supported. Use :square or :real.")
+    raise ValueError(f"ctrl.fidelity ':{f}' is not GRAPE-compatible. Use one of: real, square, modulus")
 
 
 @dataclass
@@ -91,7 +99,7 @@ class MRControl:
     rho_targ: List[np.ndarray]
     pulse_dt: np.ndarray
     pwr_levels: List[float] = field(default_factory=lambda: [1.0])
-    fidelity: str = "square"
+    fidelity: str = "real"  # the MRControl keyword default (OptControl.jl:217)
     backend: str = "rocm"
     precision: str = "f64"
     tracking: List[TrackingPoint] = field(default_factory=list)
@@ -145,7 +153,7 @@ class LindbladMRControl:
     jump_ops: List[np.ndarray] = field(default_factory=list)
     decay_rates: List[float] = field(default_factory=list)
     pwr_levels: List[float] = field(default_factory=lambda: [1.0])
-    fidelity: str = "square"
+    fidelity: str = "square"  # the LindbladMRControl keyword default (OptControl.jl:818)
     backend: str = "rocm"
     l_bound: float = -1.0
     u_bound: float = 1.0
@@ -170,7 +178,7 @@ class LindbladMRControl:
         scale = np.array([[p] * len(Lc) for _ in Ld for p in self.pwr_levels], dtype=np.float64)
         dts = np.asarray(self.pulse_dt, dtype=np.float64)
         return Problem(dim=Lc.shape[1], n_ctrl=len(Lc), n_t=len(dts), H0=L0, Hc=Lc, dt_slices=dts,
-                       dynamics=PR_LINDBLAD, objective=_fid_code(self.fidelity), n_members=M, ctrl_scale=scale,
+                       dynamics=PR_LINDBLAD, objective=_fid_code(self.fidelity, lindblad=True), n_members=M, ctrl_scale=scale,
                        psi_init=s0, psi_targ=st)
 
 

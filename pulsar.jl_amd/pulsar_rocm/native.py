@@ -27,7 +27,7 @@ PR_ERR = {
 }
 
 EXPORTS = ("pr_abi_version", "pr_device_count", "pr_init", "pr_last_error", "pr_plan_create",
-           "pr_eval", "pr_eval_device", "pr_plan_n_params", "pr_plan_set_timing",
+           "pr_eval", "pr_eval_device", "pr_plan_n_params", "pr_plan_exp_counts", "pr_plan_set_timing",
            "pr_plan_kernel_times", "pr_plan_buffers", "pr_plan_upload_controls",
            "pr_plan_synchronize", "pr_plan_destroy", "pr_grape_optimize", "pr_lbfgs_optimize")
 KERNEL_CLASSES = ("segment", "combine", "gradient", "reduce")
@@ -89,6 +89,7 @@ def load(path: str = LIB_PATH):
                                  ctypes.c_void_p]
     L.pr_plan_n_params.argtypes = [ctypes.c_void_p]
     L.pr_plan_n_params.restype = ctypes.c_longlong
+    L.pr_plan_exp_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
     L.pr_plan_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.pr_plan_kernel_times.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(ctypes.c_longlong)]
     L.pr_plan_buffers.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_void_p)] * 3
@@ -200,6 +201,14 @@ class Plan:
 
     def set_timing(self, enable: bool) -> None:
         check(load().pr_plan_set_timing(self._h, 1 if enable else 0))
+
+    def exp_counts(self) -> np.ndarray:
+        """[n_probs, 3]: member-slices whose exp took degree 6, degree 9, and the
+        squarings, of the last evaluation."""
+        n = len(self.problems)
+        out = (ctypes.c_longlong * (3 * n))()
+        check(load().pr_plan_exp_counts(self._h, out))
+        return np.array(out[:], dtype=np.int64).reshape(n, 3)
 
     def kernel_times(self) -> dict:
         """{class: (total_ms, launches)} since the previous call (HIP events)."""

@@ -32,18 +32,6 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
-def test_probe_library_exports_its_header():
-    header = os.path.join(ROOT, "include", "pulsar_probe.h")
-    lib = ctypes.CDLL(os.path.join(os.path.dirname(native.LIB_PATH), "libpulsar_probe.so"))
-    names = declared_functions(header)
-    assert names == ["pr_probe_mfma"]
-    for name in names:
-        assert hasattr(lib, name), name
-    # argument checks run before any device use
-    v = ctypes.c_double(0)
-    assert lib.pr_probe_mfma(7, ctypes.byref(v)) == -1
-
-
 def test_abi_version_and_no_device_here():
     L = native.load()
     assert L.pr_abi_version() == 1
@@ -140,8 +128,18 @@ def test_problem_struct_mirrors_match_the_header():
     assert c_fields == py_fields == jl_fields, (c_fields, py_fields, jl_fields)
 
 
+def test_julia_constants_match_the_header():
+    # every PR_* constant the Julia glue defines has the header's value
+    defs = dict((k, int(v)) for k, v in re.findall(r"#define (PR_[A-Z0-9_]+) \(?(-?\d+)\)?", open(HEADER).read()))
+    jl = open(os.path.join(ROOT, "pulsar.jl_amd", "julia", "ROCmBackend.jl")).read()
+    consts = re.findall(r"^const (PR_[A-Z0-9_]+) = Cint\((-?\d+)\)", jl, flags=re.M)
+    assert len(consts) >= 12
+    for k, v in consts:
+        assert defs[k] == int(v), k
+
+
 @pytest.mark.parametrize("case,code", [("state_lindblad", -5), ("helper_exact", -5), ("helper_state_exact", -5),
-                                       ("helper_wide", -5), ("helper_nonunitary", -1)])
+                                       ("helper_wide", -5), ("helper_nonunitary", -1), ("modulus_lindblad", -5)])
 def test_new_objectives_validate_before_device_use(case, code):
     from pulsar_rocm.problem import PR_GATE_PROPAGATORS, PR_LINDBLAD, PR_STATE, PR_STATE_PROPAGATORS
     d = 80 if case == "helper_wide" else 4
@@ -149,8 +147,10 @@ def test_new_objectives_validate_before_device_use(case, code):
     H0 = np.diag(rng.standard_normal(d)).astype(complex)
     Hc = np.stack([np.diag(rng.standard_normal(d)).astype(complex)])
     kw = {}
-    if case == "state_lindblad":
-        p = Problem(dim=d, n_ctrl=1, n_t=3, H0=H0, Hc=Hc, dt=0.1, dynamics=PR_LINDBLAD, objective=PR_STATE,
+    if case in ("state_lindblad", "modulus_lindblad"):
+        from pulsar_rocm.problem import PR_PAIRS_MODULUS
+        obj = PR_STATE if case == "state_lindblad" else PR_PAIRS_MODULUS
+        p = Problem(dim=d, n_ctrl=1, n_t=3, H0=H0, Hc=Hc, dt=0.1, dynamics=PR_LINDBLAD, objective=obj,
                     psi_init=np.eye(d, dtype=complex)[:1], psi_targ=np.eye(d, dtype=complex)[:1])
     elif case == "helper_state_exact":
         p = Problem(dim=d, n_ctrl=1, n_t=3, H0=H0, Hc=Hc, dt=0.1, objective=PR_STATE_PROPAGATORS,

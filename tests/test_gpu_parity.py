@@ -87,6 +87,36 @@ def test_large_norm_uses_scaling_and_squaring():
     u = rng.uniform(-3, 3, (2, 12))
     F, G, D = check(p, u)
     assert D[1] >= 3  # several squarings were needed
+    assert D[2] == 9  # the largest degree used
+
+
+def test_exp_counts_report_degree_and_squarings():
+    # pr_plan_exp_counts: per problem, member-slices at degree 6, at degree 9, squarings.
+    # The small-norm C2 slices all take degree 6; the large-norm problem takes
+    # degree 9 with ceil(log2(||X||_1 / 0.11)) squarings per slice, which the
+    # 1-norm of the host-built generator bounds from below
+    rng = np.random.default_rng(7)
+    d = 8
+    big = Problem(dim=d, n_ctrl=2, n_t=12, H0=W.random_hermitian(d, rng, 40.0),
+                  Hc=np.stack([W.random_hermitian(d, rng, 10.0) for _ in range(2)]), dt=0.1,
+                  objective=PR_GATE, U_target=W.haar_unitary(d, rng))
+    ub = rng.uniform(-3, 3, (2, 12))
+    small, us = W.four_qubit_ensemble(n_t=40)
+    small, us = W.shrink(small, us, n_members=3)
+    plan = native.Plan([small, big])
+    try:
+        plan.eval([us, ub])
+        c = plan.exp_counts()
+    finally:
+        plan.close()
+    assert list(c[0]) == [3 * 40, 0, 0]
+    assert c[1][0] + c[1][1] == 12 and c[1][1] == 12
+    lower = 0
+    for k in range(12):
+        X = 0.1 * (big.H0 + ub[0, k] * big.Hc[0] + ub[1, k] * big.Hc[1])
+        n1 = np.abs(X).sum(axis=0).max()
+        lower += max(0, int(np.ceil(np.log2(n1 / 0.11))))
+    assert lower <= c[1][2] <= lower + 12
 
 
 def test_gate_fixture_on_device():
@@ -500,3 +530,24 @@ def test_propagator_helper_objective_fp64_mixed_and_recompute():
     for kw, tol in (({}, TOL_G), ({"store_props": 0}, TOL_G), ({"precision": PR_MIXED}, 1e-6)):
         F, G, _ = run(p, u, **kw)
         assert abs(F - F0) <= 1e-9 and np.abs(G - G0).max() <= tol * np.abs(G0).max(), kw
+
+
+@pytest.mark.parametrize("d,grad_mode,precision", [(5, 0, 0), (20, 0, 0), (5, 1, 0), (12, 0, 1), (16, 0, 1)])
+def test_modulus_objective(d, grad_mode, precision):
+    # :modulus (Objectives.jl:222, 356-361): F = sum |z_s|, reference prefactor
+    # dt Im(inner / |z|); exact mode the derivative of |z|
+    from pulsar_rocm.problem import PR_GRAD_EXACT, PR_MIXED, PR_PAIRS_MODULUS
+    rng = np.random.default_rng(40 + d)
+    npairs, nt = 3, 30
+    p = Problem(dim=d, n_ctrl=3, n_t=nt, H0=np.stack([W.random_hermitian(d, rng, 2.0) for _ in range(2)]),
+                Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(3)]), dt_slices=rng.uniform(0.005, 0.02, nt),
+                objective=PR_PAIRS_MODULUS, n_members=2, ctrl_scale=np.array([[1.0, 0.9, 1.1], [1.1, 1.0, 0.95]]),
+                psi_init=np.stack([W.random_state(d, rng) for _ in range(npairs)]),
+                psi_targ=np.stack([W.random_state(d, rng) for _ in range(npairs)]))
+    u = rng.uniform(-1, 1, (3, nt))
+    print(f"case modulus d={d} grad_mode={grad_mode} precision={precision}", flush=True)
+    F0, G0 = O.evaluate(p, u, grad_mode=grad_mode)
+    F, G, _ = run(p, u, grad_mode=grad_mode, precision=precision)
+    tf, tg = (TOL_F, TOL_G) if precision == 0 else (1e-9, 1e-6)
+    assert abs(F - F0) <= tf, (F, F0)
+    assert np.abs(G - G0).max() / np.abs(G0).max() <= tg

@@ -335,3 +335,147 @@ def test_state_gradient_follows_gradients_jl():
         for j in range(2):
             g = 2 * np.real(np.conj(chi) * (-1j * dt) * (lam.conj() @ Hc[j] @ phi))
             assert abs(G[j, k] - g) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# Independent pins of the state, state-pair and Lindblad gradient conventions.
+# Each reference-mode gradient is checked three ways, none of which restates the
+# formula it checks:
+#   * it converges to the exact gradient (itself pinned by central differences
+#     in test_pairs_and_lindblad_exact_vs_fd) at first order in dt: the error
+#     halves when dt halves, so a dropped factor 2, a wrong sign or a missing
+#     term (whose error would not vanish with dt) fails;
+#   * with n_t = 1 the chains are trivial, which fixes where lambda sits:
+#     GRAPEState.jl:218-229 pairs lambda[n+1] = psi_t with psi[n] = psi_0, while
+#     _state_gradient_into! (Gradients.jl:357-399) uses lambda_1 = U_1^dag psi_t;
+#   * the objectives are tied to one another: F_square = Re(z)^2 + Im(z)^2 with
+#     Re(z) and Im(z) two :real objectives, checked on F and its exact gradient.
+
+
+def _pair_problem(obj, dt, nt, seed=21, dynamics=PR_CLOSED, n_pairs=2):
+    rng = np.random.default_rng(seed)
+    if dynamics == PR_CLOSED:
+        d = 3
+        H0 = W.random_hermitian(d, rng, 2.0)
+        Hc = np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)])
+        psi0 = np.stack([W.random_state(d, rng) for _ in range(n_pairs)])
+        psit = np.stack([W.random_state(d, rng) for _ in range(n_pairs)])
+    else:
+        N, d = 2, 4
+        Hs = W.random_hermitian(N, rng, 2.0)
+        Lm = np.array([[0, 1], [0, 0]], complex)
+        H0 = W.drift_liouvillian(Hs, [Lm], [0.4])
+        Hc = np.stack([W.control_liouvillian(W.random_hermitian(N, rng, 1.0)) for _ in range(2)])
+        vs = []
+        for _ in range(2 * n_pairs):
+            v = W.random_state(N, rng)
+            vs.append(np.outer(v, v.conj()).reshape(-1, order="F"))
+        psi0, psit = np.stack(vs[:n_pairs]), np.stack(vs[n_pairs:])
+    u = rng.uniform(-1, 1, (2, nt))
+    p = Problem(dim=d, n_ctrl=2, n_t=nt, H0=H0, Hc=Hc, dt=dt, dynamics=dynamics, objective=obj,
+                psi_init=psi0, psi_targ=psit, ctrl_scale=np.array([[0.8, 1.3]]), weight=np.ones(1))
+    return p, u
+
+
+def _objectives():
+    from pulsar_rocm.problem import PR_STATE
+    return [("state", PR_STATE, PR_CLOSED), ("square", PR_PAIRS_SQUARE, PR_CLOSED),
+            ("real", PR_PAIRS_REAL, PR_CLOSED), ("lindblad_square", PR_PAIRS_SQUARE, PR_LINDBLAD),
+            ("lindblad_real", PR_PAIRS_REAL, PR_LINDBLAD)]
+
+
+@pytest.mark.parametrize("name,obj,dyn", _objectives())
+def test_pair_reference_gradients_are_first_order(name, obj, dyn):
+    # the exact gradients of these objectives match central differences
+    p, u = _pair_problem(obj, 0.2, 3, dynamics=dyn)
+    _, Gx = O.evaluate(p, u, grad_mode=PR_GRAD_EXACT)
+    assert np.abs(Gx - _fd(p, u)).max() / np.abs(Gx).max() < 1e-7
+    errs = []
+    for dt in (2e-3, 1e-3, 5e-4):
+        p, u = _pair_problem(obj, dt, 4, dynamics=dyn)
+        _, Gr = O.evaluate(p, u)
+        _, Gx = O.evaluate(p, u, grad_mode=PR_GRAD_EXACT)
+        errs.append(np.abs(Gr - Gx).max() / np.abs(Gx).max())
+    assert errs[0] < 5e-2, errs
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 1.85 < r1 < 2.15 and 1.85 < r2 < 2.15, (name, errs)
+
+
+@pytest.mark.parametrize("name,obj,dyn", _objectives())
+def test_pair_gradients_single_slice_closed_form(name, obj, dyn):
+    from pulsar_rocm.problem import PR_STATE
+    p, u = _pair_problem(obj, 0.35, 1, dynamics=dyn)
+    F, G = O.evaluate(p, u)
+    s = p.ctrl_scale[0]
+    gen = p.H0 + s[0] * u[0, 0] * p.Hc[0] + s[1] * u[1, 0] * p.Hc[1]
+    U1 = O.expm_neg_i(gen, p.dt) if dyn == PR_CLOSED else O.expm(gen * p.dt)
+    Fref, Gref = 0.0, np.zeros(2)
+    for a, b in zip(p.psi_init, p.psi_targ):
+        z = b.conj() @ U1 @ a
+        sq = obj in (PR_PAIRS_SQUARE, PR_STATE)
+        Fref += abs(z) ** 2 if sq else z.real
+        for j in range(2):
+            if obj == PR_STATE:
+                # lambda_1 = Q_1 psi_t = U_1^dag psi_t, psi_1 = psi_0
+                inner = (U1.conj().T @ b).conj() @ (s[j] * p.Hc[j]) @ a
+                g = 2 * np.real(np.conj(z) * (-1j * p.dt) * inner)
+            else:
+                # lambda[2] = psi_t, psi[1] = psi_0
+                inner = b.conj() @ (s[j] * p.Hc[j]) @ a
+                if dyn == PR_CLOSED:
+                    g = 2 * p.dt * np.imag(np.conj(z) * inner) if sq else p.dt * np.imag(inner)
+                else:
+                    g = 2 * p.dt * np.real(np.conj(z) * inner) if sq else p.dt * np.real(inner)
+            Gref[j] += g
+    assert abs(F - Fref) < 1e-14
+    assert np.abs(G[:, 0] - Gref).max() < 1e-14 * max(1.0, np.abs(Gref).max())
+
+
+def test_square_and_real_objectives_are_tied():
+    # one pair: F_square = |z|^2 and F_real = Re z of the same overlap; the
+    # exact gradients satisfy dF_square = 2 Re(conj z dz) where dF_real = Re dz
+    # and the derivative of Im z is the :real gradient of the pair (psi_0, i psi_t)
+    for dyn in (PR_CLOSED, PR_LINDBLAD):
+        ps, u = _pair_problem(PR_PAIRS_SQUARE, 0.2, 3, dynamics=dyn, n_pairs=1)
+        pr, _ = _pair_problem(PR_PAIRS_REAL, 0.2, 3, dynamics=dyn, n_pairs=1)
+        pi, _ = _pair_problem(PR_PAIRS_REAL, 0.2, 3, dynamics=dyn, n_pairs=1)
+        pi.psi_targ = 1j * pi.psi_targ  # Re <i t|U|i> = Im <t|U|i>
+        Fs, Gs = O.evaluate(ps, u, grad_mode=PR_GRAD_EXACT)
+        Fr, Gr = O.evaluate(pr, u, grad_mode=PR_GRAD_EXACT)
+        Fi, Gi = O.evaluate(pi, u, grad_mode=PR_GRAD_EXACT)
+        assert abs(Fs - (Fr ** 2 + Fi ** 2)) < 1e-14
+        assert np.abs(Gs - 2 * (Fr * Gr + Fi * Gi)).max() < 1e-13 * np.abs(Gs).max()
+
+
+def test_modulus_objective():
+    # :modulus (Objectives.jl:222, 356-361): F = |z|; the reference prefactor
+    # dt Im(inner / |z|) is the :real one divided by |z| (per pair), and the
+    # exact mode is the derivative of |z| (central differences)
+    from pulsar_rocm.problem import PR_PAIRS_MODULUS
+    pm, u = _pair_problem(PR_PAIRS_MODULUS, 0.3, 3, n_pairs=1)
+    ps, _ = _pair_problem(PR_PAIRS_SQUARE, 0.3, 3, n_pairs=1)
+    pr, _ = _pair_problem(PR_PAIRS_REAL, 0.3, 3, n_pairs=1)
+    Fm, Gm = O.evaluate(pm, u)
+    Fs, _ = O.evaluate(ps, u)
+    _, Gr = O.evaluate(pr, u)
+    assert abs(Fm - np.sqrt(Fs)) < 1e-15
+    assert np.abs(Gm * Fm - Gr).max() < 1e-14 * np.abs(Gr).max()
+    _, Gx = O.evaluate(pm, u, grad_mode=PR_GRAD_EXACT)
+    assert np.abs(Gx - _fd(pm, u)).max() / np.abs(Gx).max() < 1e-7
+    # several pairs: the sum of the single-pair objectives
+    p2, u2 = _pair_problem(PR_PAIRS_MODULUS, 0.3, 3, n_pairs=2)
+    F2, G2 = O.evaluate(p2, u2)
+    Fsum, Gsum = 0.0, 0.0
+    for s in range(2):
+        q, _ = _pair_problem(PR_PAIRS_MODULUS, 0.3, 3, n_pairs=2)
+        q.psi_init, q.psi_targ = q.psi_init[s:s + 1], q.psi_targ[s:s + 1]
+        Fq, Gq = O.evaluate(q, u2)
+        Fsum, Gsum = Fsum + Fq, Gsum + Gq
+    assert abs(F2 - Fsum) < 1e-14 and np.abs(G2 - Gsum).max() < 1e-14 * np.abs(Gsum).max()
+    # an overlap below 1e-14 contributes no gradient
+    q, _ = _pair_problem(PR_PAIRS_MODULUS, 0.3, 1, n_pairs=1)
+    q.H0 = np.zeros_like(q.H0)
+    q.psi_init = np.array([[1, 0, 0]], complex)
+    q.psi_targ = np.array([[0, 1, 0]], complex)
+    Fz, Gz = O.evaluate(q, np.zeros((2, 1)))
+    assert Fz == 0.0 and np.all(Gz == 0.0)
