@@ -192,6 +192,22 @@ int pr_eval(void* plan, const double* const* controls, double* F, double* const*
  * that keep controls resident in HBM across optimizer iterations. */
 int pr_eval_device(void* plan, const double* d_controls, double* d_F, double* d_G, void* stream);
 
+/* pr_eval_device with no allocation and no graph capture of its own: the same
+ * kernels and results, with every scratch buffer of the evaluation (segment
+ * products, prefixes, adjoint seeds, U_k images, per-member fidelities and
+ * gradients, reduction partials) in a static array of 64 MiB per GPU instead
+ * of the plan's memory. It has no counterpart in the reference. It is for
+ * callers that cannot allocate device memory while they run: code under HIP
+ * graph capture (the launches go straight onto `stream`, with fill kernels
+ * instead of memsets), tight serving loops, harnesses that forbid allocation.
+ * Limits: a plan whose scratch exceeds 64 MiB (see pr_options.store_props = 0
+ * to drop the images) or that holds problems with dim > 64 is rejected with
+ * PR_ERR_UNSUPPORTED. Every call on a GPU uses that GPU's one array, so calls
+ * that overlap on one GPU (two streams, two threads) race: serialise them. The
+ * array keeps the last call's contents; what must start at zero is cleared on
+ * every call. pr_init must have been called for the device. */
+int pr_eval_device_static(void* plan, const double* d_controls, double* d_F, double* d_G, void* stream);
+
 /* The plan's own device buffers used by pr_eval (controls and gradient:
  * pr_plan_n_params doubles each; F: n_probs doubles). */
 int pr_plan_buffers(void* plan, double** d_controls, double** d_F, double** d_G);

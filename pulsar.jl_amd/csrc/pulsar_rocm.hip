@@ -25,7 +25,62 @@
 
 using cd = std::complex<double>;
 
+// ------------------------------------------------------- static scratch
+// The scratch memory of pr_eval_device_static: one array per GPU, held from
+// the library's load until the process ends. Host code never uses the name
+// `scratch` as an address (it is a host-side stand-in); pr_init looks up the
+// array's device address on each device it initialises and keeps it per device.
+#define PR_STATIC_SCRATCH_BYTES (64ull << 20)
+#define PR_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[PR_STATIC_SCRATCH_BYTES];
+
+namespace pr {
+
+// Byte offsets into the static scratch of one problem's scratch buffers (-1:
+// the plan allocates none of that buffer for the problem).
+struct StaticOff {
+  long long Ust, Udm, big, seg_big, Tseg, Eseg, Om, Bb, Pscr, Fm, Gm, udef, smax, Zseg, Pend, Lend;
+};
+
+// dst[p] = src[p] with every scratch pointer moved into the static scratch at
+// base (the plan's own problem records stay as they are).
+__global__ void k_bind_static(const ProbDev* __restrict__ src, ProbDev* __restrict__ dst,
+                              const StaticOff* __restrict__ off, int n, unsigned char* base) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  ProbDev d = src[p];
+  const StaticOff o = off[p];
+  auto at = [&](long long x) -> void* { return x < 0 ? nullptr : (void*)(base + x); };
+  d.Ust = (double2*)at(o.Ust);
+  d.Udm = (float2*)at(o.Udm);
+  d.big = (unsigned char*)at(o.big);
+  d.seg_big = (int*)at(o.seg_big);
+  d.Tseg = (double2*)at(o.Tseg);
+  d.Eseg = (double2*)at(o.Eseg);
+  d.Om = (double2*)at(o.Om);
+  d.Bb = (double2*)at(o.Bb);
+  d.Pscr = (double2*)at(o.Pscr);
+  d.Fm = (double*)at(o.Fm);
+  d.Gm = (double*)at(o.Gm);
+  d.udef = (double*)at(o.udef);
+  d.smax = (int*)at(o.smax);
+  d.Zseg = (double2*)at(o.Zseg);
+  d.Pend = (double2*)at(o.Pend);
+  d.Lend = (double2*)at(o.Lend);
+  dst[p] = d;
+}
+
+// zero n doubles (a fill kernel: no memset node in a caller's captured graph)
+__global__ void k_zero(double* __restrict__ p, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+}  // namespace pr
+
 namespace {
+
+unsigned char* g_scratch_ptr[PR_MAX_DEVICES] = {nullptr};
 
 thread_local std::string g_err;
 
@@ -284,6 +339,14 @@ struct Plan {
   double* d_G = nullptr;
   double* d_part = nullptr;  // [RED_CHUNKS][n_params] partial member sums
   double* h_stage = nullptr; // pinned host staging: n_params controls or gradients, then n_probs F
+  // pr_eval_device_static: the byte sizes of each problem's scratch buffers,
+  // their offsets in the static scratch (udef arrays first, so one fill zeroes
+  // them), the reduction partials' offset, the total, and the device copies
+  std::vector<pr::StaticOff> static_sz, static_off;
+  long long static_part = 0, static_udef_bytes = 0, static_bytes = 0;
+  bool static_ok = true;  // false when a group keeps per-item scratch outside the records (dim > 64)
+  pr::StaticOff* d_static_off = nullptr;
+  pr::ProbDev* d_probs_static = nullptr;
   struct Group {
     int B = 0, V = 0;
     bool lindblad = false;  // some problem of the group has Lindblad dynamics
@@ -704,9 +767,49 @@ int run_graph(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t
   return PR_OK;
 }
 
+// Offsets of every problem's scratch buffers in the static scratch, 256-byte
+// aligned: all udef arrays first (the one range that must start at zero), then
+// the other buffers problem by problem, then the reduction partials. The device
+// copies of the offsets and of the problem records they produce are part of the
+// plan, so pr_eval_device_static itself allocates nothing.
+int static_layout(Plan& pl) {
+  auto align = [](long long x) { return (x + 255) & ~255LL; };
+  long long at = 0;
+  pl.static_off.assign(pl.n_probs, pr::StaticOff{});
+  for (int p = 0; p < pl.n_probs; p++) {
+    pl.static_off[p].udef = at;
+    at = align(at + pl.static_sz[p].udef);
+  }
+  pl.static_udef_bytes = at;
+  for (int p = 0; p < pl.n_probs; p++) {
+    const pr::StaticOff& z = pl.static_sz[p];
+    pr::StaticOff& o = pl.static_off[p];
+    auto place = [&](long long bytes) -> long long {
+      if (bytes == 0) return -1;
+      const long long r = at;
+      at = align(at + bytes);
+      return r;
+    };
+    o.Ust = place(z.Ust); o.Udm = place(z.Udm); o.big = place(z.big); o.seg_big = place(z.seg_big);
+    o.Tseg = place(z.Tseg); o.Eseg = place(z.Eseg); o.Om = place(z.Om); o.Bb = place(z.Bb);
+    o.Pscr = place(z.Pscr); o.Fm = place(z.Fm); o.Gm = place(z.Gm); o.smax = place(z.smax);
+    o.Zseg = place(z.Zseg); o.Pend = place(z.Pend); o.Lend = place(z.Lend);
+  }
+  pl.static_part = at;
+  at = align(at + (long long)sizeof(double) * pr::RED_CHUNKS * pl.n_params);
+  pl.static_bytes = at;
+  for (const auto& g : pl.groups)
+    if (g.big_scratch) pl.static_ok = false;
+  int rc;
+  if ((rc = pl.upload(&pl.d_static_off, pl.static_off.data(), pl.n_probs))) return rc;
+  if ((rc = pl.alloc(&pl.d_probs_static, pl.n_probs))) return rc;
+  return PR_OK;
+}
+
 int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   pl.n_probs = n_probs;
   pl.host_probs.resize(n_probs);
+  pl.static_sz.resize(n_probs);
   pl.coff.assign(n_probs + 1, 0);
   // groups: index B (1..8) by block count; tracking problems, which run their
   // own walk kernel, in groups 8 + B (B <= 4)
@@ -954,32 +1057,41 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       if ((rc = pl.upload(&dp, cimg.data(), cimg.size()))) return rc;
       D.Ck = dp;
     }
-    // scratch
+    // scratch (the byte size of each buffer is recorded for the static layout)
+    pr::StaticOff& z = pl.static_sz[p];
+    z = pr::StaticOff{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#define PR_SCRATCH(field, count)                                              \
+  do {                                                                        \
+    const size_t cnt_ = (size_t)(count);                                      \
+    if ((rc = pl.alloc(&D.field, cnt_))) return rc;                           \
+    z.field = (long long)(cnt_ * sizeof(*D.field));                           \
+  } while (0)
     D.recompute = recompute && can_recompute(q) ? 1 : 0;
-    if (!D.recompute && (rc = pl.alloc(&D.Ust, (size_t)M * nt * MB))) return rc;
+    if (!D.recompute) PR_SCRATCH(Ust, (size_t)M * nt * MB);
     if (B == 1 && mixed1 && !track) {
       // FP32 images of small-generator slices; the FP64 images above hold the others
-      if ((rc = pl.alloc(&D.Udm, (size_t)M * nt * 256))) return rc;
-      if ((rc = pl.alloc(&D.big, (size_t)M * nt))) return rc;
-      if ((rc = pl.alloc(&D.seg_big, (size_t)M * S))) return rc;
+      PR_SCRATCH(Udm, (size_t)M * nt * 256);
+      PR_SCRATCH(big, (size_t)M * nt);
+      PR_SCRATCH(seg_big, (size_t)M * S);
     }
-    if ((rc = pl.alloc(&D.Tseg, (size_t)M * S * MB))) return rc;
-    if ((rc = pl.alloc(&D.Eseg, (size_t)M * (S + 1) * MB))) return rc;
-    if ((rc = pl.alloc(&D.Om, (size_t)M * MB))) return rc;
+    PR_SCRATCH(Tseg, (size_t)M * S * MB);
+    PR_SCRATCH(Eseg, (size_t)M * (S + 1) * MB);
+    PR_SCRATCH(Om, (size_t)M * MB);
     if (!D.unitary) {
-      if ((rc = pl.alloc(&D.Bb, (size_t)M * S * MB))) return rc;
-      if ((rc = pl.alloc(&D.Pscr, (size_t)M * S * L * MB))) return rc;
+      PR_SCRATCH(Bb, (size_t)M * S * MB);
+      PR_SCRATCH(Pscr, (size_t)M * S * L * MB);
     }
-    if ((rc = pl.alloc(&D.Fm, M))) return rc;
-    if ((rc = pl.alloc(&D.Gm, (size_t)M * nc * nt))) return rc;
-    if ((rc = pl.alloc(&D.udef, M))) return rc;
+    PR_SCRATCH(Fm, M);
+    PR_SCRATCH(Gm, (size_t)M * nc * nt);
+    PR_SCRATCH(udef, M);
     HIPCHK(hipMemset(D.udef, 0, sizeof(double) * M));
-    if ((rc = pl.alloc(&D.smax, (size_t)3 * M * S))) return rc;
+    PR_SCRATCH(smax, (size_t)3 * M * S);
     if (track) {
-      if ((rc = pl.alloc(&D.Zseg, (size_t)M * S * MB))) return rc;
-      if ((rc = pl.alloc(&D.Pend, (size_t)M * S * MB))) return rc;
-      if ((rc = pl.alloc(&D.Lend, (size_t)M * S * MB))) return rc;
+      PR_SCRATCH(Zseg, (size_t)M * S * MB);
+      PR_SCRATCH(Pend, (size_t)M * S * MB);
+      PR_SCRATCH(Lend, (size_t)M * S * MB);
     }
+#undef PR_SCRATCH
     D.coff = pl.coff[p];
     pl.coff[p + 1] = pl.coff[p] + (long long)nc * nt;
     for (int m = 0; m < M; m++) {
@@ -1046,7 +1158,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     }
     if (rc) return rc;
   }
-  return PR_OK;
+  return static_layout(pl);
 }
 
 int fill_diag(const Plan& pl, double* diag) {
@@ -1103,6 +1215,14 @@ int pr_init(int device) {
   if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(PR_ERR_DEVICE, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
   HIPCHK(hipSetDevice(device));
+  if (device < PR_MAX_DEVICES && !g_scratch_ptr[device]) {
+    // the static scratch's address on this device (each GPU holds its own copy)
+    void* ptr = nullptr;
+    const hipError_t es = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (es != hipSuccess || !ptr)
+      return fail(PR_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(es));
+    g_scratch_ptr[device] = static_cast<unsigned char*>(ptr);
+  }
   g_device = device;
   return PR_OK;
 }
@@ -1184,6 +1304,48 @@ int pr_eval_device(void* plan, const double* d_controls, double* d_F, double* d_
   if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : pl.stream;
   return run_graph(pl, d_controls, d_F, d_G, st);
+}
+
+int pr_eval_device_static(void* plan, const double* d_controls, double* d_F, double* d_G, void* stream) {
+  if (fault_is("eval")) return fail(PR_ERR_INJECTED, "failure injected by PULSAR_ROCM_FAULT=eval");
+  if (!plan || !d_controls || !d_F)
+    return fail(PR_ERR_ARG, "pr_eval_device_static: plan, d_controls and d_F are required");
+  Plan& pl = *static_cast<Plan*>(plan);
+  if (g_device < 0 || g_device >= PR_MAX_DEVICES || !g_scratch_ptr[g_device])
+    return fail(PR_ERR_DEVICE, "pr_eval_device_static: pr_init has not set up this device");
+  if (!pl.static_ok)
+    return fail(PR_ERR_UNSUPPORTED, "pr_eval_device_static: plans with dim > 64 problems keep per-item scratch "
+                                    "of their own; use pr_eval_device");
+  if (pl.static_bytes > (long long)PR_STATIC_SCRATCH_BYTES)
+    return fail(PR_ERR_UNSUPPORTED, "pr_eval_device_static: the plan needs %lld bytes of scratch; the static "
+                                    "scratch holds %llu (use store_props = 0, fewer members or pr_eval_device)",
+                pl.static_bytes, (unsigned long long)PR_STATIC_SCRATCH_BYTES);
+  HIPCHK(hipSetDevice(g_device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : pl.stream;
+  unsigned char* base = g_scratch_ptr[g_device];
+  // problem records pointing into the static scratch, and the range that must
+  // start at zero (the unitarity defects)
+  hipLaunchKernelGGL(pr::k_bind_static, dim3((pl.n_probs + 63) / 64), dim3(64), 0, st, pl.d_probs,
+                     pl.d_probs_static, pl.d_static_off, pl.n_probs, base);
+  HIPCHK(hipGetLastError());
+  if (pl.static_udef_bytes > 0) {
+    hipLaunchKernelGGL(pr::k_zero, dim3(64), dim3(256), 0, st, reinterpret_cast<double*>(base),
+                       pl.static_udef_bytes / (long long)sizeof(double));
+    HIPCHK(hipGetLastError());
+  }
+  // the plan's launch sequence on those records, launched directly (no graph
+  // capture of its own, no timing events), so a caller may capture it
+  pr::ProbDev* own_probs = pl.d_probs;
+  double* own_part = pl.d_part;
+  const bool own_timing = pl.timing;
+  pl.d_probs = pl.d_probs_static;
+  pl.d_part = reinterpret_cast<double*>(base + pl.static_part);
+  pl.timing = false;
+  const int rc = run(pl, d_controls, d_F, d_G, st);
+  pl.d_probs = own_probs;
+  pl.d_part = own_part;
+  pl.timing = own_timing;
+  return rc;
 }
 
 int pr_plan_buffers(void* plan, double** d_controls, double** d_F, double** d_G) {

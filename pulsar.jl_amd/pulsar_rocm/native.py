@@ -27,7 +27,7 @@ PR_ERR = {
 }
 
 EXPORTS = ("pr_abi_version", "pr_device_count", "pr_init", "pr_last_error", "pr_plan_create",
-           "pr_eval", "pr_eval_device", "pr_plan_n_params", "pr_plan_exp_counts", "pr_plan_set_timing",
+           "pr_eval", "pr_eval_device", "pr_eval_device_static", "pr_plan_n_params", "pr_plan_exp_counts", "pr_plan_set_timing",
            "pr_plan_kernel_times", "pr_plan_buffers", "pr_plan_upload_controls",
            "pr_plan_synchronize", "pr_plan_destroy", "pr_grape_optimize", "pr_lbfgs_optimize")
 KERNEL_CLASSES = ("segment", "combine", "gradient", "reduce")
@@ -89,6 +89,7 @@ def load(path: str = LIB_PATH):
                                  ctypes.c_void_p]
     L.pr_plan_n_params.argtypes = [ctypes.c_void_p]
     L.pr_plan_n_params.restype = ctypes.c_longlong
+    L.pr_eval_device_static.argtypes = [ctypes.c_void_p] * 5
     L.pr_plan_exp_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
     L.pr_plan_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.pr_plan_kernel_times.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(ctypes.c_longlong)]
@@ -181,6 +182,13 @@ class Plan:
         check(load().pr_eval_device(self._h, ctypes.c_void_p(d_controls), ctypes.c_void_p(d_F),
                                     ctypes.c_void_p(d_G) if d_G else None,
                                     ctypes.c_void_p(stream) if stream else None))
+
+    def eval_device_static(self, d_controls: int, d_F: int, d_G: int | None, stream: int | None = None) -> None:
+        """pr_eval_device_static: the same evaluation with the library's static
+        scratch and no allocation (direct launches)."""
+        check(load().pr_eval_device_static(self._h, ctypes.c_void_p(d_controls), ctypes.c_void_p(d_F),
+                                           ctypes.c_void_p(d_G) if d_G else None,
+                                           ctypes.c_void_p(stream) if stream else None))
 
     def buffers(self):
         """(d_controls, d_F, d_G) device addresses of the plan-owned buffers."""
