@@ -105,6 +105,14 @@ extern "C" {
                       traces and reductions FP64. Time segments with a generator
                       above the degree-6 bound, and other problems, run as PR_FP64. */
 
+#define PR_SPLIT16 2 /* d <= 16 closed dynamics, reference gradient: as PR_MIXED with the
+                        low-precision products on FP16 matrix cores (v_mfma_f32_16x16x32_f16),
+                        each operand split into two FP16 parts (three products,
+                        FP32-class, FP32 accumulation; the square of the exp
+                        polynomial keeps the fourth, lo x lo). Time segments with a
+                        generator whose 1-norm bound exceeds 0.03, and other
+                        problems, run as PR_FP64. */
+
 /* gradient formula */
 #define PR_GRAD_REFERENCE 0 /* first-order GRAPE formula of Gradients.jl:274-289 */
 #define PR_GRAD_EXACT 1     /* exact derivative (Frechet derivative of exp) */
@@ -221,9 +229,10 @@ int pr_plan_synchronize(void* plan);
 long long pr_plan_n_params(void* plan);
 
 /* Exp work of the last evaluation (no counterpart in the reference; it
- * serves work models of the kernels): per problem p, counts[3p] member-slices
- * whose exp took the degree-6 polynomial, counts[3p + 1] those that took
- * degree 9, counts[3p + 2] the squarings over all member-slices. Waits for the
+ * serves work models of the kernels): per problem p, counts[4p] member-slices
+ * whose exp took the degree-6 polynomial, counts[4p + 1] those that took
+ * degree 9, counts[4p + 2] the squarings over all member-slices, counts[4p + 3]
+ * the member-slices a reduced-precision plan redid in FP64. Waits for the
  * plan's stream. Tracking problems report zeros. */
 int pr_plan_exp_counts(void* plan, long long* counts);
 

@@ -279,7 +279,7 @@ __global__ __launch_bounds__(1024) void k_segment_big(const ProbDev* __restrict_
     const Ln x = big::blk_info<B>(b);
     store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * MB, load_blk(T, x), x);
   }
-  store_exp_counts(P, m, it.seg, smax, n9, nsq);
+  store_exp_counts(P, m, it.seg, smax, n9, nsq, 0);
 }
 
 template <int B>

@@ -115,8 +115,9 @@ struct ProbDev {
   double* Fm;                 // [M]
   double* Gm;                 // [M][nc*nt]
   double* udef;               // [M] ||U^dag U - I||_max (closed)
-  int* smax;                  // [M*nseg][3] per segment: largest scaling exponent, slices
-                              // that took the degree-9 polynomial, squarings (exp work counts)
+  int* smax;                  // [M*nseg][4] per segment: largest scaling exponent, slices
+                              // that took the degree-9 polynomial, squarings, slices redone in
+                              // FP64 by a reduced-precision plan (exp work counts)
   // tracking objective: ck_of[k] = index of the checkpoint matrix of slice k
   // or -1; Ck holds the images of C_c = sum w Herm(rho_target) per checkpoint
   // slice. Per member and segment: Zseg the checkpoint operator of the segment
@@ -138,12 +139,13 @@ struct WorkItem {
 __device__ inline double slice_dt(const ProbDev& P, int k) { return P.prescaled ? P.dt0 : P.dts[k]; }
 
 // exp work of one (member, segment) work item, for the diagnostics
-__device__ inline void store_exp_counts(const ProbDev& P, int m, int seg, int smax, int n9, int nsq) {
+__device__ inline void store_exp_counts(const ProbDev& P, int m, int seg, int smax, int n9, int nsq, int nredo) {
   if (threadIdx.x == 0) {
-    int* c = P.smax + 3 * ((size_t)m * P.nseg + seg);
+    int* c = P.smax + 4 * ((size_t)m * P.nseg + seg);
     c[0] = smax;
     c[1] = n9;
     c[2] = nsq;
+    c[3] = nredo;
   }
 }
 
@@ -676,7 +678,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
   }
   store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * NB * 256, Tseg, x);
   if constexpr (TRK) store_blk(P.Zseg + ((size_t)m * P.nseg + it.seg) * NB * 256, Z, x);
-  store_exp_counts(P, m, it.seg, smax, n9, nsq);
+  store_exp_counts(P, m, it.seg, smax, n9, nsq, FIX ? it.k1 - it.k0 : 0);
 }
 
 // ------------------------------------------------------ mixed precision, d <= 16
@@ -766,7 +768,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
     Tseg = acc;
   }
   store_blk(P.Tseg + ((size_t)m * P.nseg + it.seg) * 256, Tseg, x);
-  store_exp_counts(P, m, it.seg, smax, 0, 0);
+  store_exp_counts(P, m, it.seg, smax, 0, 0, 0);
 }
 
 // ---------------------------------------------------------------- k_combine
@@ -1055,13 +1057,18 @@ __device__ inline Blk recompute_ut(const ProbDev& P, int m, int k, const double*
 // ---------------------------------------------------------------- k_gradient
 // RECOMP: problems flagged P.recompute get U_k from recompute_ut instead of
 // their (absent) images.
-template <int B, int V, bool EXACT, bool REG = false, bool RECOMP = false>
+// REDO (reduced-precision plans): only the work items that the segment kernel
+// redid in FP64 (P.seg_big) run; the split walk skips them.
+template <int B, int V, bool EXACT, bool REG = false, bool RECOMP = false, bool REDO = false>
 __global__ __launch_bounds__(64 * B * B)
 __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAVES : 0))) void k_gradient(const ProbDev* __restrict__ probs,
                                                           const WorkItem* __restrict__ items,
                                                           const double* __restrict__ u) {
   const WorkItem it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
+  if constexpr (REDO) {
+    if (!P.seg_big[(size_t)it.member * P.nseg + it.seg]) return;
+  }
   const Ln x = lane_info<B>();
   double2 *LA, *LB, *zb;
   double* red;

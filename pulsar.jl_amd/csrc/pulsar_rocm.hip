@@ -21,6 +21,7 @@
 
 #include "../../include/pulsar_rocm.h"
 #include "grape_kernels.hpp"
+#include "split_kernels.hpp"
 #include "big_kernels.hpp"
 
 using cd = std::complex<double>;
@@ -350,7 +351,10 @@ struct Plan {
   struct Group {
     int B = 0, V = 0;
     bool lindblad = false;  // some problem of the group has Lindblad dynamics
-    bool mixed = false;     // FP32-MFMA exp and gradient walk (d <= 16 only)
+    bool mixed = false;     // low-precision exp and gradient walk (d <= 16 only): FP32 MFMA, or
+    bool split = false;     // FP16x2 split products (PR_SPLIT16)
+    int split_np = 0;       // slot passes of the split segment kernel's assembly held in registers
+                            // (the group's common pass count, 1..3), or 0: tables read per slice
     bool trace_regs = false;  // mixed walk keeps the trace operand in registers
     bool recompute = false;   // some problem of the group recomputes U_k in the gradient walk
     bool track = false;       // tracking-objective problems (k_segment, k_track_scan, k_track_walk)
@@ -543,7 +547,42 @@ int tend(Plan& pl, int cls, hipStream_t st) {
   return PR_OK;
 }
 
+int launch_gradient_split(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t st) {
+  const size_t sm = pr::Smem<1>::gradient_unitary;
+  switch (g.V) {
+#define PR_HCASE(VV)                                                                                    \
+  case VV:                                                                                              \
+    if (g.trace_regs)                                                                                   \
+      hipLaunchKernelGGL((pr::k_gradient_h<VV, true>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u); \
+    else                                                                                                \
+      hipLaunchKernelGGL((pr::k_gradient_h<VV, false>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u); \
+    break;
+    PR_HCASE(2) PR_HCASE(4) PR_HCASE(8) PR_HCASE(16) PR_HCASE(32)
+#undef PR_HCASE
+    default:
+      return fail(PR_ERR_LAUNCH, "internal: bad reduction width %d", g.V);
+  }
+  HIPCHK(hipGetLastError());
+  // the work items redone in FP64 by the segment kernels walk in FP64
+  switch (g.V) {
+#define PR_RCASE(VV)                                                                                    \
+  case VV:                                                                                              \
+    if (g.trace_regs)                                                                                   \
+      hipLaunchKernelGGL((pr::k_gradient<1, VV, false, true, false, true>), dim3(g.n_items), dim3(64), sm, st,   \
+                         pl.d_probs, g.d_items, d_u);                                                   \
+    else                                                                                                \
+      hipLaunchKernelGGL((pr::k_gradient<1, VV, false, false, false, true>), dim3(g.n_items), dim3(64), sm, st,  \
+                         pl.d_probs, g.d_items, d_u);                                                   \
+    break;
+    PR_RCASE(2) PR_RCASE(4) PR_RCASE(8) PR_RCASE(16) PR_RCASE(32)
+#undef PR_RCASE
+  }
+  HIPCHK(hipGetLastError());
+  return PR_OK;
+}
+
 int launch_gradient_mixed(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t st) {
+  if (g.split) return launch_gradient_split(pl, g, d_u, st);
   const size_t sm = pr::Smem<1>::gradient_unitary;
   switch (g.V) {
 #define PR_MCASE(VV)                                                                                    \
@@ -598,7 +637,16 @@ int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
   int rc;
   if ((rc = tbegin(pl, st))) return rc;
   if (B == 1 && g.mixed) {
-    hipLaunchKernelGGL(pr::k_segment_mx, dim3(g.n_items), blk, pr::Smem<1>::segment, st, pl.d_probs, g.d_items, d_u);
+    if (g.split) {
+      const size_t sm = pr::Smem<1>::segment;
+      switch (g.split_np) {
+        case 1: hipLaunchKernelGGL(pr::k_segment_h<1>, dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); break;
+        case 2: hipLaunchKernelGGL(pr::k_segment_h<2>, dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); break;
+        case 3: hipLaunchKernelGGL(pr::k_segment_h<3>, dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); break;
+        default: hipLaunchKernelGGL(pr::k_segment_h<0>, dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u);
+      }
+    } else
+      hipLaunchKernelGGL(pr::k_segment_mx, dim3(g.n_items), blk, pr::Smem<1>::segment, st, pl.d_probs, g.d_items, d_u);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL((pr::k_segment<1, true, true>), dim3(g.n_items), blk, pr::Smem<1>::segment, st, pl.d_probs,
                        g.d_items, d_u);
@@ -826,7 +874,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   // the mixed-precision kernels cover single-wavefront (d <= 16) unitary
   // problems under the reference gradient formula; any other problem of the
   // plan, or a d <= 16 group holding a Lindblad problem, runs in FP64
-  bool mixed1 = pl.opt.precision == PR_MIXED && pl.opt.grad_mode == PR_GRAD_REFERENCE;
+  bool mixed1 = (pl.opt.precision == PR_MIXED || pl.opt.precision == PR_SPLIT16) &&
+                pl.opt.grad_mode == PR_GRAD_REFERENCE;
   for (int p = 0; p < n_probs; p++)
     if (probs[p].dim <= 16 && probs[p].dynamics != PR_CLOSED) mixed1 = false;
   // (tracking problems run FP64 in their own groups, with stored images)
@@ -1085,7 +1134,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     PR_SCRATCH(Gm, (size_t)M * nc * nt);
     PR_SCRATCH(udef, M);
     HIPCHK(hipMemset(D.udef, 0, sizeof(double) * M));
-    PR_SCRATCH(smax, (size_t)3 * M * S);
+    PR_SCRATCH(smax, (size_t)4 * M * S);
     if (track) {
       PR_SCRATCH(Zseg, (size_t)M * S * MB);
       PR_SCRATCH(Pend, (size_t)M * S * MB);
@@ -1135,6 +1184,17 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     for (int p = 0; p < n_probs; p++)
       if (group_of(probs[p]) == gi && probs[p].dynamics == PR_LINDBLAD) g.lindblad = true;
     g.mixed = B == 1 && mixed1 && !g.track;
+    g.split = g.mixed && pl.opt.precision == PR_SPLIT16;
+    if (g.split) {
+      int np = -1;
+      for (int p = 0; p < n_probs; p++) {
+        if (group_of(probs[p]) != gi) continue;
+        const pr::ProbDev& D = pl.host_probs[p];
+        const int n = D.so_idx ? D.so_pass[1] - D.so_pass[0] : 0;
+        np = (np < 0 || np == n) ? n : 0;
+      }
+      g.split_np = (np >= 1 && np <= 3) ? np : 0;
+    }
     for (int p = 0; p < n_probs; p++)
       if (group_of(probs[p]) == gi && pl.host_probs[p].recompute) g.recompute = true;
     // every control fits one lane group of 64 / V lanes with <= TR_REG nonzeros per lane
@@ -1165,13 +1225,13 @@ int fill_diag(const Plan& pl, double* diag) {
   for (int p = 0; p < pl.n_probs; p++) {
     const pr::ProbDev& D = pl.host_probs[p];
     std::vector<double> ud(D.M);
-    std::vector<int> sm((size_t)3 * D.M * D.nseg);
+    std::vector<int> sm((size_t)4 * D.M * D.nseg);
     HIPCHK(hipMemcpy(ud.data(), D.udef, sizeof(double) * D.M, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(sm.data(), D.smax, sizeof(int) * sm.size(), hipMemcpyDeviceToHost));
     double umax = 0;
     for (double v : ud) umax = std::max(umax, v);
     int smax = 0, any9 = 0;
-    for (size_t i = 0; i < sm.size(); i += 3) {
+    for (size_t i = 0; i < sm.size(); i += 4) {
       smax = std::max(smax, sm[i]);
       any9 |= sm[i + 1] > 0;
     }
@@ -1239,7 +1299,8 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
   *plan = nullptr;
   pr_options o{PR_FP64, PR_GRAD_REFERENCE, 0, -1};
   if (opt) o = *opt;
-  if (o.precision != PR_FP64 && o.precision != PR_MIXED) return fail(PR_ERR_ARG, "unknown precision %d", o.precision);
+  if (o.precision != PR_FP64 && o.precision != PR_MIXED && o.precision != PR_SPLIT16)
+    return fail(PR_ERR_ARG, "unknown precision %d", o.precision);
   if (o.grad_mode != PR_GRAD_REFERENCE && o.grad_mode != PR_GRAD_EXACT)
     return fail(PR_ERR_ARG, "unknown grad_mode %d", o.grad_mode);
   if (o.store_props < -1 || o.store_props > 1) return fail(PR_ERR_ARG, "store_props %d is not -1, 0 or 1", o.store_props);
@@ -1409,16 +1470,18 @@ int pr_plan_exp_counts(void* plan, long long* counts) {
   HIPCHK(hipStreamSynchronize(pl.stream));
   for (int p = 0; p < pl.n_probs; p++) {
     const pr::ProbDev& D = pl.host_probs[p];
-    std::vector<int> sm((size_t)3 * D.M * D.nseg);
+    std::vector<int> sm((size_t)4 * D.M * D.nseg);
     HIPCHK(hipMemcpy(sm.data(), D.smax, sizeof(int) * sm.size(), hipMemcpyDeviceToHost));
-    long long n9 = 0, nsq = 0;
-    for (size_t i = 0; i < sm.size(); i += 3) {
+    long long n9 = 0, nsq = 0, nredo = 0;
+    for (size_t i = 0; i < sm.size(); i += 4) {
       n9 += sm[i + 1];
       nsq += sm[i + 2];
+      nredo += sm[i + 3];
     }
-    counts[3 * p + 0] = (long long)D.M * D.nt - n9;
-    counts[3 * p + 1] = n9;
-    counts[3 * p + 2] = nsq;
+    counts[4 * p + 0] = (long long)D.M * D.nt - n9;
+    counts[4 * p + 1] = n9;
+    counts[4 * p + 2] = nsq;
+    counts[4 * p + 3] = nredo;
   }
   return PR_OK;
 }

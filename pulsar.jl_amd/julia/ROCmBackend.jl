@@ -26,6 +26,7 @@ const PR_STATE_PROPAGATORS = Cint(6)   # state branch of the MAS/DNP propagator 
 const PR_PAIRS_MODULUS = Cint(7)       # :modulus state pairs (Objectives.jl:222, 356-361)
 const PR_FP64 = Cint(0)
 const PR_MIXED = Cint(1)
+const PR_SPLIT16 = Cint(2)
 const PR_GRAD_REFERENCE = Cint(0)
 const PR_GRAD_EXACT = Cint(1)
 

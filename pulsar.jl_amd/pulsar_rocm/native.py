@@ -211,12 +211,13 @@ class Plan:
         check(load().pr_plan_set_timing(self._h, 1 if enable else 0))
 
     def exp_counts(self) -> np.ndarray:
-        """[n_probs, 3]: member-slices whose exp took degree 6, degree 9, and the
-        squarings, of the last evaluation."""
+        """[n_probs, 4]: member-slices whose exp took degree 6, degree 9, the
+        squarings, and the member-slices a reduced-precision plan redid in FP64,
+        of the last evaluation."""
         n = len(self.problems)
-        out = (ctypes.c_longlong * (3 * n))()
+        out = (ctypes.c_longlong * (4 * n))()
         check(load().pr_plan_exp_counts(self._h, out))
-        return np.array(out[:], dtype=np.int64).reshape(n, 3)
+        return np.array(out[:], dtype=np.int64).reshape(n, 4)
 
     def kernel_times(self) -> dict:
         """{class: (total_ms, launches)} since the previous call (HIP events)."""

@@ -21,7 +21,7 @@ PR_CLOSED, PR_LINDBLAD = 0, 1
 PR_GATE, PR_PAIRS_SQUARE, PR_PAIRS_REAL, PR_TRACKING, PR_GATE_PROPAGATORS, PR_STATE = 0, 1, 2, 3, 4, 5
 PR_STATE_PROPAGATORS = 6
 PR_PAIRS_MODULUS = 7
-PR_FP64, PR_MIXED = 0, 1
+PR_FP64, PR_MIXED, PR_SPLIT16 = 0, 1, 2
 PR_GRAD_REFERENCE, PR_GRAD_EXACT = 0, 1
 
 

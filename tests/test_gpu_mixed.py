@@ -1,9 +1,11 @@
-"""Mixed-precision device path (PR_MIXED) against the FP64 oracle.
+"""Reduced-precision device paths against the FP64 oracle: PR_MIXED (FP32
+matrix cores) and PR_SPLIT16 (FP16x2 split products on FP16 matrix cores).
 
-Acceptance bar of the mixed form: |dF| <= 1e-9 and max|dG| / max|G| <= 1e-6
-against the FP64 oracle on the same inputs. The mixed kernels cover d <= 16
-closed-system problems under the reference gradient formula; everything else
-in a mixed plan runs the FP64 kernels and keeps the FP64 tolerances.
+Acceptance bar of both forms: |dF| <= 1e-9 and max|dG| / max|G| <= 1e-6
+against the FP64 oracle on the same inputs. The reduced-precision kernels
+cover d <= 16 closed-system problems under the reference gradient formula;
+everything else in such a plan runs the FP64 kernels and keeps the FP64
+tolerances. Every test runs for both forms.
 """
 import numpy as np
 import pytest
@@ -11,12 +13,17 @@ from scipy.linalg import expm
 
 import oracle as O
 from pulsar_rocm import native, workloads as W
-from pulsar_rocm.problem import PR_FP64, PR_GATE, PR_LINDBLAD, PR_MIXED, PR_PAIRS_SQUARE, Problem
+from pulsar_rocm.problem import PR_FP64, PR_GATE, PR_LINDBLAD, PR_MIXED, PR_PAIRS_SQUARE, PR_SPLIT16, Problem
 
 pytestmark = pytest.mark.gpu
 
 TOL_F = 1e-9
 TOL_G = 1e-6
+
+
+@pytest.fixture(params=[PR_MIXED, PR_SPLIT16], ids=["fp32", "split16"])
+def prec(request):
+    return request.param
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -33,18 +40,18 @@ def run(p, u, **kw):
     return F[0], G[0], D[0]
 
 
-def errors(p, u, **kw):
-    print(f"case d={p.dim} M={p.n_members} n_t={p.n_t} obj={p.objective} {kw}", flush=True)
+def errors(p, u, prec, **kw):
+    print(f"case d={p.dim} M={p.n_members} n_t={p.n_t} obj={p.objective} precision={prec} {kw}", flush=True)
     F0, G0 = O.evaluate(p, u)
-    F, G, D = run(p, u, precision=PR_MIXED, **kw)
+    F, G, D = run(p, u, precision=prec, **kw)
     return abs(F - F0), np.abs(G - G0).max() / np.abs(G0).max(), F0, D
 
 
-def test_four_qubit_ensemble_full_length_member_sample():
+def test_four_qubit_ensemble_full_length_member_sample(prec):
     # the benchmark workload (random controls, Haar target): all 2000 slices, 16 members
     p, u = W.four_qubit_ensemble()
     p, u = W.shrink(p, u, n_members=16)
-    dF, dG, _, D = errors(p, u)
+    dF, dG, _, D = errors(p, u, prec)
     print("mixed C2 sample: dF", dF, "relG", dG, flush=True)
     assert dF <= TOL_F and dG <= TOL_G
     # ||U^dag U - I||: the FP32 rounding of R_k = exp(X_k) - I - X_k (~1e-11 per
@@ -53,7 +60,7 @@ def test_four_qubit_ensemble_full_length_member_sample():
     assert D[0] < 1e-8
 
 
-def test_near_optimum_target():
+def test_near_optimum_target(prec):
     # a target 0.15 away (in generator norm) from member 0's U_total: F ~ 0.99
     p, u = W.four_qubit_ensemble()
     p, u = W.shrink(p, u, n_members=4)
@@ -62,14 +69,14 @@ def test_near_optimum_target():
     H = (H + H.conj().T) / 2
     H /= np.linalg.norm(H, 2)
     p.U_target = O.total_propagator(p, u, member=0) @ expm(-1j * 0.15 * H)
-    dF, dG, F0, _ = errors(p, u)
+    dF, dG, F0, _ = errors(p, u, prec)
     print("mixed near optimum: F", F0, "dF", dF, "relG", dG, flush=True)
     assert F0 > 0.5
     assert dF <= TOL_F and dG <= TOL_G
 
 
 @pytest.mark.parametrize("d", [2, 4, 8, 16])
-def test_small_dimensions_and_segments(d):
+def test_small_dimensions_and_segments(d, prec):
     rng = np.random.default_rng(300 + d)
     nq = int(np.log2(d))
     H0 = W.random_hermitian(d, rng, 2 * np.pi * 100)
@@ -79,11 +86,11 @@ def test_small_dimensions_and_segments(d):
     p = Problem(dim=d, n_ctrl=nc, n_t=500, H0=H0, Hc=Hc, dt=1e-3, objective=PR_GATE, n_members=3,
                 ctrl_scale=rng.uniform(0.9, 1.1, (3, nc)), U_target=W.haar_unitary(d, rng))
     for seg in (0, 37):
-        dF, dG, _, _ = errors(p, u, segment=seg)
+        dF, dG, _, _ = errors(p, u, prec, segment=seg)
         assert dF <= TOL_F and dG <= TOL_G, (seg, dF, dG)
 
 
-def test_scaling_and_squaring_in_mixed_mode():
+def test_scaling_and_squaring_in_mixed_mode(prec):
     # generators above the degree-6 bound take degree 9 plus squarings
     rng = np.random.default_rng(9)
     d = 8
@@ -91,18 +98,18 @@ def test_scaling_and_squaring_in_mixed_mode():
                 Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)]), dt=0.1,
                 objective=PR_GATE, U_target=W.haar_unitary(d, rng))
     u = rng.uniform(-1, 1, (2, 40))
-    dF, dG, _, D = errors(p, u)
+    dF, dG, _, D = errors(p, u, prec)
     assert D[1] >= 1
     assert dF <= TOL_F and dG <= TOL_G
 
 
-def test_state_pairs_in_mixed_mode():
+def test_state_pairs_in_mixed_mode(prec):
     p, u = W.single_qubit_state(psi_init=[1, 0])
-    dF, dG, _, _ = errors(p, u)
+    dF, dG, _, _ = errors(p, u, prec)
     assert dF <= TOL_F and dG <= TOL_G
 
 
-def test_mixed_plan_runs_other_problems_in_fp64():
+def test_mixed_plan_runs_other_problems_in_fp64(prec):
     # a d=32 gate problem and a d=4 Lindblad problem in a mixed plan keep FP64 results
     rng = np.random.default_rng(11)
     d = 32
@@ -117,7 +124,7 @@ def test_mixed_plan_runs_other_problems_in_fp64():
                    objective=PR_PAIRS_SQUARE, n_members=2, ctrl_scale=np.array([[1.0, 1.0], [0.95, 0.95]]),
                    psi_init=np.array([[1, 0, 0, 0]], complex), psi_targ=np.array([[0, 0, 0, 1]], complex))
     ul = rng.uniform(-2, 2, (2, 30))
-    plan = native.Plan([big, lind], precision=PR_MIXED)
+    plan = native.Plan([big, lind], precision=prec)
     try:
         F, G = plan.eval([ub, ul])
     finally:
@@ -128,16 +135,16 @@ def test_mixed_plan_runs_other_problems_in_fp64():
         assert np.abs(g - G0).max() <= 1e-10 * np.abs(G0).max()
 
 
-def test_mixed_matches_fp64_device_result_closely():
+def test_mixed_matches_fp64_device_result_closely(prec):
     p, u = W.four_qubit_ensemble(n_t=200)
     p, u = W.shrink(p, u, n_members=8)
     F64, G64, _ = run(p, u, precision=PR_FP64)
-    Fm, Gm, _ = run(p, u, precision=PR_MIXED)
+    Fm, Gm, _ = run(p, u, precision=prec)
     assert abs(Fm - F64) <= 1e-10
     assert np.abs(Gm - G64).max() <= 1e-7 * np.abs(G64).max()
 
 
-def test_some_slices_above_the_bound_redo_their_segments_in_fp64():
+def test_some_slices_above_the_bound_redo_their_segments_in_fp64(prec):
     # small generators except a few spikes: segments holding a spike are redone
     # in FP64, the others stay mixed; both kinds meet the mixed tolerance
     p, u = W.four_qubit_ensemble(n_t=400)
@@ -146,5 +153,5 @@ def test_some_slices_above_the_bound_redo_their_segments_in_fp64():
     u[:, 37] *= 40.0
     u[:, 250] *= 40.0
     for seg in (0, 16):
-        dF, dG, _, _ = errors(p, u, segment=seg)
+        dF, dG, _, _ = errors(p, u, prec, segment=seg)
         assert dF <= TOL_F and dG <= TOL_G, (seg, dF, dG)

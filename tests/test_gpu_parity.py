@@ -109,7 +109,7 @@ def test_exp_counts_report_degree_and_squarings():
         c = plan.exp_counts()
     finally:
         plan.close()
-    assert list(c[0]) == [3 * 40, 0, 0]
+    assert list(c[0]) == [3 * 40, 0, 0, 0]
     assert c[1][0] + c[1][1] == 12 and c[1][1] == 12
     lower = 0
     for k in range(12):
@@ -391,6 +391,7 @@ def test_mixed_plan_with_large_generators_falls_back_per_slice():
                 U_target=W.haar_unitary(d, rng))
     u = rng.uniform(-1, 1, (3, 30))
     check(p, u, precision=PR_MIXED)
+    check(p, u, precision=2)  # PR_SPLIT16
 
 
 def _random_state(d, rng):
@@ -505,8 +506,9 @@ def test_state_objective_mixed_precision():
     p, u = W.single_qubit_state(psi_init=[1, 0])
     assert p.objective == PR_STATE
     F0, G0 = O.evaluate(p, u)
-    F, G, _ = run(p, u, precision=PR_MIXED)
-    assert abs(F - F0) <= 1e-9 and np.abs(G - G0).max() <= 1e-6 * np.abs(G0).max()
+    for prec in (PR_MIXED, 2):  # FP32 and FP16x2 split products
+        F, G, _ = run(p, u, precision=prec)
+        assert abs(F - F0) <= 1e-9 and np.abs(G - G0).max() <= 1e-6 * np.abs(G0).max(), prec
 
 
 def test_state_objective_exact_gradient_matches_oracle():
@@ -527,12 +529,13 @@ def test_propagator_helper_objective_fp64_mixed_and_recompute():
     p, u = W.shrink(*W.four_qubit_ensemble(n_t=60, n_det=2, n_amp=2))
     p.objective = PR_GATE_PROPAGATORS
     F0, G0 = O.evaluate(p, u)
-    for kw, tol in (({}, TOL_G), ({"store_props": 0}, TOL_G), ({"precision": PR_MIXED}, 1e-6)):
+    for kw, tol in (({}, TOL_G), ({"store_props": 0}, TOL_G), ({"precision": PR_MIXED}, 1e-6),
+                    ({"precision": 2}, 1e-6)):
         F, G, _ = run(p, u, **kw)
         assert abs(F - F0) <= 1e-9 and np.abs(G - G0).max() <= tol * np.abs(G0).max(), kw
 
 
-@pytest.mark.parametrize("d,grad_mode,precision", [(5, 0, 0), (20, 0, 0), (5, 1, 0), (12, 0, 1), (16, 0, 1)])
+@pytest.mark.parametrize("d,grad_mode,precision", [(5, 0, 0), (20, 0, 0), (5, 1, 0), (12, 0, 1), (16, 0, 1), (12, 0, 2), (16, 0, 2)])
 def test_modulus_objective(d, grad_mode, precision):
     # :modulus (Objectives.jl:222, 356-361): F = sum |z_s|, reference prefactor
     # dt Im(inner / |z|); exact mode the derivative of |z|
