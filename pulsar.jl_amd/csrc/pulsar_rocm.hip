@@ -353,6 +353,7 @@ struct Plan {
     bool lindblad = false;  // some problem of the group has Lindblad dynamics
     bool mixed = false;     // low-precision exp and gradient walk (d <= 16 only): FP32 MFMA, or
     bool split = false;     // FP16x2 split products (PR_SPLIT16)
+    bool prescaled = false; // every problem of the group has operators prescaled by c dt
     int split_np = 0;       // slot passes of the split segment kernel's assembly held in registers
                             // (the group's common pass count, 1..3), or 0: tables read per slice
     bool trace_regs = false;  // mixed walk keeps the trace operand in registers
@@ -639,12 +640,23 @@ int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
   if (B == 1 && g.mixed) {
     if (g.split) {
       const size_t sm = pr::Smem<1>::segment;
-      switch (g.split_np) {
-        case 1: hipLaunchKernelGGL(pr::k_segment_h<1>, dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); break;
-        case 2: hipLaunchKernelGGL(pr::k_segment_h<2>, dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); break;
-        case 3: hipLaunchKernelGGL(pr::k_segment_h<3>, dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u); break;
-        default: hipLaunchKernelGGL(pr::k_segment_h<0>, dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u);
+#define PR_SEGH(NP, PRE) hipLaunchKernelGGL((pr::k_segment_h<NP, PRE>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u)
+      if (g.prescaled) {
+        switch (g.split_np) {
+          case 1: PR_SEGH(1, true); break;
+          case 2: PR_SEGH(2, true); break;
+          case 3: PR_SEGH(3, true); break;
+          default: PR_SEGH(0, false);
+        }
+      } else {
+        switch (g.split_np) {
+          case 1: PR_SEGH(1, false); break;
+          case 2: PR_SEGH(2, false); break;
+          case 3: PR_SEGH(3, false); break;
+          default: PR_SEGH(0, false);
+        }
       }
+#undef PR_SEGH
     } else
       hipLaunchKernelGGL(pr::k_segment_mx, dim3(g.n_items), blk, pr::Smem<1>::segment, st, pl.d_probs, g.d_items, d_u);
     HIPCHK(hipGetLastError());
@@ -1185,6 +1197,9 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       if (group_of(probs[p]) == gi && probs[p].dynamics == PR_LINDBLAD) g.lindblad = true;
     g.mixed = B == 1 && mixed1 && !g.track;
     g.split = g.mixed && pl.opt.precision == PR_SPLIT16;
+    g.prescaled = true;
+    for (int p = 0; p < n_probs; p++)
+      if (group_of(probs[p]) == gi && !pl.host_probs[p].prescaled) g.prescaled = false;
     if (g.split) {
       int np = -1;
       for (int p = 0; p < n_probs; p++) {

@@ -198,7 +198,7 @@ __device__ inline void split_asm_load(const ProbDev& P, SplitAsm<NP>& a, const L
 // X_k in the own layout, FP64: with NP > 0 from the registers of SplitAsm and
 // the lane weight wl = s_mj u_jk of lane j (fetched by ds_bpermute), else by
 // the general assembly (tables read from memory).
-template <int NP>
+template <int NP, bool PRE>
 __device__ inline Blk split_asm(const ProbDev& P, int m, int k, const double* __restrict__ u, double wl,
                                 const SplitAsm<NP>& a, const Blk& H0, const Ln& x) {
   if constexpr (NP == 0) {
@@ -217,9 +217,12 @@ __device__ inline Blk split_asm(const ProbDev& P, int m, int k, const double* __
         H.im[r] = fma(wj, a.v[i][r].y, H.im[r]);
       }
     }
-    if (P.prescaled) return H;
-    const double dt = slice_dt(P, k);
-    return cscale(P.cre * dt, P.cim * dt, H);
+    if constexpr (PRE) {
+      return H;
+    } else {
+      const double dt = slice_dt(P, k);
+      return cscale(P.cre * dt, P.cim * dt, H);
+    }
   }
 }
 
@@ -253,7 +256,8 @@ constexpr int SPLIT_GROUP = 128;
 // product per slice. The first group of a segment gives T = I + F; a later
 // one (segments above SPLIT_GROUP slices) is folded in with one FP64 product
 // T <- T + F T.
-template <int NP>
+// PRE: every problem of the launch holds operators prescaled by c dt (uniform dt).
+template <int NP, bool PRE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP > 0 ? PR_SPLIT_SEGMENT_WAVES_REG
                                                                            : PR_SPLIT_SEGMENT_WAVES))) void k_segment_h(
     const ProbDev* __restrict__ probs, const WorkItem* __restrict__ items, const double* __restrict__ u) {
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP > 0 ? PR_
       H0.re[r] = v.x;
       H0.im[r] = v.y;
     }
-    const Blk X = split_asm<NP>(P, m, k, u, wl, A, H0, x);
+    const Blk X = split_asm<NP, PRE>(P, m, k, u, wl, A, H0, x);
     // F enters the product E_k F as it was before this slice; X goes into F at once
     Blk32 f32;
     f32.re = __builtin_convertvector(F.re, f4);
@@ -485,19 +489,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SPLIT_GRA
   if (P.after) contract_c(it.k0, dtk);
   for (int k = it.k0; k < it.k1; k++) {
     const double dtn = k + 1 < it.k1 ? slice_dt(P, k + 1) : dtk;
-    // own(C^T) through an FP32 tile, then its split: the A operand of C
-    Blk32 c32;
-    c32.re = __builtin_convertvector(Cm.re, f4);
-    c32.im = __builtin_convertvector(Cm.im, f4);
-#pragma unroll
-    for (int r = 0; r < 4; r++) L32[lidx(0, rowof32(x.q, r), x.c)] = make_float2(c32.re[r], c32.im[r]);
-    bsync<1>();
+    // own(C^T), then its split: the A operand of C. With the first-order term
+    // after U_k the trace of slice k has just left C_k in the FP64 tile (own
+    // positions); otherwise C goes through the tile in FP32.
     Blk32 ct;
+    if (P.after) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const float2 v = L32[lidx(0, x.c, rowof32(x.q, r))];  // C[c][4q + r]
-      ct.re[r] = v.x;
-      ct.im[r] = v.y;
+      for (int r = 0; r < 4; r++) {
+        const double2 v = LA[lidx(0, x.c, rowof32(x.q, r))];  // C[c][4q + r]
+        ct.re[r] = (float)v.x;
+        ct.im[r] = (float)v.y;
+      }
+    } else {
+      Blk32 c32;
+      c32.re = __builtin_convertvector(Cm.re, f4);
+      c32.im = __builtin_convertvector(Cm.im, f4);
+#pragma unroll
+      for (int r = 0; r < 4; r++) L32[lidx(0, rowof32(x.q, r), x.c)] = make_float2(c32.re[r], c32.im[r]);
+      bsync<1>();
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const float2 v = L32[lidx(0, x.c, rowof32(x.q, r))];  // C[c][4q + r]
+        ct.re[r] = v.x;
+        ct.im[r] = v.y;
+      }
     }
     const BOp bD{__builtin_bit_cast(h8, dh), __builtin_bit_cast(h8, dl)};
     const Blk32 p1 = mm16<false>(aop_t(split_scaled(ct, sc)), bD);  // sc 2^10 C D
