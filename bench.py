@@ -39,8 +39,8 @@ FORMS = {"fp64": (PR_FP64, "f64"), "mixed_fp32": (PR_MIXED, "f64+f32"), "split16
 WORK_MODEL = {
     "fp64": "complex products on FP64 matrix cores, 6 d^3 flops each (three real products)",
     "mixed_fp32": "exp and walk products on FP32 matrix cores and the chain product on FP64, 6 d^3 flops each",
-    "split16": "FP16x2 split products on FP16 matrix cores, 8 d^3 flops per part-product (10 per slice in "
-               "k_segment, 6 in k_gradient), the chain product on FP64 (6 d^3)",
+    "split16": "FP16x2 split products on FP16 matrix cores, 8 d^3 flops per part-product (13 per slice in "
+               "k_segment, 6 in k_gradient), no FP64 chain product",
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -48,14 +48,17 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by what the same counters
 # read for a 256 MiB device-to-device copy (FETCH x 2.0, WRITE x 1.0); the
 # commands and per-kernel figures are in profiles/grape_four_qubit_ensemble_hbm_traffic.md.
-# k_segment writes the U_k^T images (256 x 2000 x 4 KiB = 2.10 GB) and
-# k_gradient reads them back once. k_track_walk (four_qubit_tracking) reads the
-# images once too (profiles/grape_four_qubit_tracking_hbm_traffic.md).
+# FP64: k_segment writes the U_k^T images (256 x 2000 x 4 KiB = 2.10 GB) and
+# k_gradient reads them back once; split16: the split images of D_k (2 KiB,
+# 1.05 GB), profiles/grape_four_qubit_ensemble_split16_hbm_traffic.md.
+# k_track_walk (four_qubit_tracking) reads the FP64 images once too
+# (profiles/grape_four_qubit_tracking_hbm_traffic.md).
 MEASURED_TRAFFIC_BYTES = {("fp64", "k_segment"): 0.022e9 + 2.131e9, ("fp64", "k_gradient"): 2.141e9 + 0.037e9,
+                          ("split16", "k_segment"): 0.012e9 + 1.084e9, ("split16", "k_gradient"): 1.092e9 + 0.034e9,
                           "k_track_walk": 2.166e9 + 0.040e9}
 
 
-def mfma_work(kernel, form, counts, units, d):
+def mfma_work(kernel, form, counts, units, d, items):
     """Matrix-core flops one launch of the dominant kernels needs on this
     workload, per core type, from the exp counts the library reports
     (pr_plan_exp_counts: member-slices at degree 6, at degree 9, squarings,
@@ -68,10 +71,11 @@ def mfma_work(kernel, form, counts, units, d):
     mixed_fp32: the degree-6 exp and both walk products on FP32 matrix cores
     (6 d^3 each), the chain product on FP64.
     split16: the exp's products X X (four FP16 part-products), X X^2 and X^3 Q
-    (three each), and the walk's two products (three each), as 16x16x32 FP16
-    matrix products of 8 d^3 flops per part-product (the real 2x2 block form),
-    the chain product on FP64 (6 d^3). Member-slices redone in FP64 count
-    their FP64 exp and walk products."""
+    (three each), the segment product's E_k F (three, every slice but the
+    first of a work item) and the walk's two products (three each), as
+    16x16x32 FP16 matrix products of 8 d^3 flops per part-product (the real 2x2
+    block form); no FP64 chain product (segments of at most 128 slices).
+    Member-slices redone in FP64 count their FP64 exp and walk products."""
     n6, n9, nsq, nredo = (int(c) for c in counts)
     g = 6 * d ** 3
     if form == "fp64":
@@ -85,7 +89,8 @@ def mfma_work(kernel, form, counts, units, d):
             return {"fp32": g * 3 * low, "fp64": g * (redo_exp + units)}
         return {"fp32": g * 2 * low, "fp64": g * 2 * nredo}
     if kernel == "k_segment":
-        return {"fp16": 8 * d ** 3 * (4 + 3 + 3) * low, "fp64": g * (redo_exp + units)}
+        return {"fp16": 8 * d ** 3 * ((4 + 3 + 3) * low + 3 * max(low - items, 0)),
+                "fp64": g * (redo_exp + nredo)}
     return {"fp16": 8 * d ** 3 * 3 * 2 * low, "fp64": g * 2 * nredo}
 
 
@@ -350,11 +355,12 @@ def main():
         prec = FORMS[name][0]
         plan, elapsed, kt = timed_form(p, u, prec, args)
         counts = plan.exp_counts()[0]
+        D0 = plan.eval([u], diag=True)[2][0]  # diag[3]: time segments per member
         plan.close()
         seg_ms = kt["segment"][0] / max(kt["segment"][1], 1)
         grad_ms = kt["gradient"][0] / max(kt["gradient"][1], 1)
         dominant, dom_ms = ("k_segment", seg_ms) if seg_ms >= grad_ms else ("k_gradient", grad_ms)
-        work = mfma_work(dominant, name, counts, units, d)
+        work = mfma_work(dominant, name, counts, units, d, M * int(D0[3]))
         achieved, peak, frac = roofline_of(work, dom_ms)
         f = {"value": round(args.steps / elapsed, 3), "ms_per_step": round(1e3 * elapsed / args.steps, 4),
              "dtype": FORMS[name][1],
