@@ -269,6 +269,35 @@ def mr_workload(args):
     print(json.dumps(out))
 
 
+def accuracy(q, u, Fo, Go, prec, members):
+    """The form's F, G and unitarity defect on the oracle's member sample q
+    against the oracle's Fo, Go."""
+    qplan = native.Plan([q], precision=prec)
+    Fd, Gd, D = qplan.eval([u], diag=True)
+    qplan.close()
+    return {"members": members, "abs_dF": float(abs(Fd[0] - Fo)),
+            "max_rel_dG": float(np.abs(Gd[0] - Go).max() / np.abs(Go).max()),
+            "max_abs_dG": float(np.abs(Gd[0] - Go).max()), "max_abs_G": float(np.abs(Go).max()),
+            "unitarity_defect": float(D[0, 0]), "F": float(Fd[0])}
+
+
+def lbfgs_check(p, u_start, prec):
+    """84 device L-BFGS iterations on p from u_start with the form's F and G;
+    returns the FP64 fidelity of the result."""
+    cfg = native.LbfgsConfig(max_iter=84, convergence_tol=1e-15, step_size=0.05 * np.abs(u_start).max(),
+                             gradient_norm_tol=1e-15, lbfgs_memory=10, amplitude_limit=0.0)
+    plan = native.Plan([p], precision=prec)
+    t0 = time.perf_counter()
+    best, fh, _, res = native.lbfgs_optimize_device(plan, u_start, cfg)
+    el = time.perf_counter() - t0
+    plan.close()
+    ref = native.Plan([p], precision=PR_FP64)
+    F64 = ref.eval([best], grad=False)[0][0]
+    ref.close()
+    return {"iterations": int(res.n_iterations), "seconds": round(el, 3), "F_in_fp64": float(F64),
+            "F_own": float(fh[-1])}
+
+
 def timed_form(p, u, prec, args):
     """Warm-up, then K evaluations with the library's per-kernel events and K
     without them (the headline time is the faster loop). Returns the open plan,
@@ -349,6 +378,21 @@ def main():
     if args.cpu_sample > 0:
         cpu_rate, cpu_s, Fo, Go, q = cpu_baseline(p, u, args.cpu_sample, args.cpu_threads)
         cpu = (cpu_rate, cpu_s, Fo, Go, q)
+    # the GRAPE-optimized point (tests/golden/c2_optimized.npz: the nominal
+    # system's propagator at the reference controls as the target, controls
+    # from the device's ensemble L-BFGS, ensemble F = 0.99967): accuracy near
+    # F = 1 and the time there
+    opt = None
+    if args.cpu_sample > 0:
+        fx = np.load(os.path.join(ROOT, "tests", "golden", "c2_optimized.npz"))
+        po, uo = workloads.shrink(p, u)
+        po.U_target = fx["target"]
+        uo = fx["controls"]
+        qo, _ = workloads.shrink(po, uo, n_members=args.cpu_sample)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        Foo, Goo = oracle.evaluate(qo, uo, n_threads=args.cpu_threads)
+        opt = (po, uo, Foo, Goo, qo)
     forms = {}
     names = [n for n in FORMS if args.forms == "all" or n in args.forms.split(",")]
     for name in names:
@@ -365,6 +409,7 @@ def main():
         f = {"value": round(args.steps / elapsed, 3), "ms_per_step": round(1e3 * elapsed / args.steps, 4),
              "dtype": FORMS[name][1],
              "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+             "segments_per_member": int(D0[3]),
              "exp_counts": {"deg6": int(counts[0]), "deg9": int(counts[1]), "squarings": int(counts[2]),
                             "fp64_redone": int(counts[3])},
              "roofline": {"bound": "mfma", "kernel": dominant, "achieved": round(achieved, 3),
@@ -377,16 +422,28 @@ def main():
                                         f"({vector_flops(d, nc)[dominant]} per member-slice) not counted"}}
         if cpu is not None:
             _, _, Fo, Go, q = cpu
-            qplan = native.Plan([q], precision=prec)
-            Fd, Gd, D = qplan.eval([u], diag=True)
-            qplan.close()
-            f["accuracy"] = {"members": args.cpu_sample, "abs_dF": float(abs(Fd[0] - Fo)),
-                             "max_rel_dG": float(np.abs(Gd[0] - Go).max() / np.abs(Go).max()),
-                             "unitarity_defect": float(D[0, 0]), "F": float(Fd[0])}
+            f["accuracy"] = accuracy(q, u, Fo, Go, prec, args.cpu_sample)
             f["meets_gate"] = f["accuracy"]["abs_dF"] <= 1e-9 and f["accuracy"]["max_rel_dG"] <= 1e-6
+        if opt is not None:
+            po, uo, Foo, Goo, qo = opt
+            plan, el_o, kt_o = timed_form(po, uo, prec, args)
+            plan.close()
+            f["at_optimized_point"] = {"value": round(args.steps / el_o, 3),
+                                       "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt_o.items()},
+                                       "accuracy": accuracy(qo, uo, Foo, Goo, prec, args.cpu_sample)}
+            # at the ensemble optimum the members' gradients cancel in the mean
+            # (max|G| there is ~1e-4 of its value at the benchmark point), so
+            # the gradient's relative error is gated at the benchmark point
+            # and only reported here; F is gated at both
+            a = f["at_optimized_point"]["accuracy"]
+            f["meets_gate"] = f.get("meets_gate", True) and a["abs_dF"] <= 1e-9
+            # what the gradient is for: 84 L-BFGS iterations from the
+            # reference controls with this form's F and G; F of the result in FP64
+            f["at_optimized_point"]["lbfgs_84"] = lbfgs_check(po, u, prec)
         forms[name] = f
-    # the headline: the fastest form within the metric's gate (|dF| <= 1e-9 and
-    # max relative gradient error <= 1e-6 against the oracle); FP64 without an oracle run
+    # the headline: the fastest form within the gate against the oracle: |dF|
+    # <= 1e-9 at the benchmark point and at the optimized point, max relative
+    # gradient error <= 1e-6 at the benchmark point; FP64 without an oracle run
     gated = [n for n in names if forms[n].get("meets_gate", n == "fp64")]
     head = max(gated, key=lambda n: forms[n]["value"]) if gated else names[0]
     h = forms[head]
@@ -409,6 +466,8 @@ def main():
         "roofline": h["roofline"],
         "survey_model_tflops": round(survey_flops(d, nc) * units * h["value"] / 1e12, 3),
         "kernel_ms_per_step": h["kernel_ms_per_step"],
+        "gate": "vs the FP64 oracle: |dF| <= 1e-9 at these controls and at GRAPE-optimized controls "
+                "(tests/golden/c2_optimized.npz, F = 0.99967); max|dG|/max|G| <= 1e-6 at these controls",
     }
     if cpu is not None:
         cpu_rate, cpu_s = cpu[0], cpu[1]

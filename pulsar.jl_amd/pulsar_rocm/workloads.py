@@ -86,14 +86,32 @@ def four_qubit_drift(omega, J) -> np.ndarray:
     return H
 
 
-def four_qubit_ensemble(seed: int = 7, n_t: int = 2000, n_det: int = 16, n_amp: int = 16,
-                        target_seed: int = 2):
-    """d=16 closed-system gate synthesis with a detuning x amplitude ensemble."""
+def four_qubit_parameters(seed: int = 7):
+    """(rng, omega, J, Hc) of the four-qubit system: the qubit frequencies and
+    couplings are the first draws of the seeded stream, which then goes on to
+    draw the controls."""
     rng = np.random.default_rng(seed)
     nq = 4
     omega = rng.uniform(-TWO_PI * 50, TWO_PI * 50, nq)
     J = rng.uniform(TWO_PI * 5, TWO_PI * 20, nq - 1)
     Hc = np.stack([embed(s, nq, q) / 2 for q in range(nq) for s in (SX, SY)])
+    return rng, omega, J, Hc
+
+
+def four_qubit_nominal(seed: int = 7, n_t: int = 2000):
+    """The nominal system of the ensemble (no detuning, amplitude scale 1) as a
+    one-member problem with the ensemble's controls; its target is left unset."""
+    rng, omega, J, Hc = four_qubit_parameters(seed)
+    u = rng.uniform(-TWO_PI * 1000, TWO_PI * 1000, (Hc.shape[0], n_t))
+    p = Problem(dim=16, n_ctrl=Hc.shape[0], n_t=n_t, H0=four_qubit_drift(omega, J), Hc=Hc, dt=1e-6,
+                dynamics=PR_CLOSED, objective=PR_GATE, n_members=1, U_target=np.eye(16, dtype=np.complex128))
+    return p, u
+
+
+def four_qubit_ensemble(seed: int = 7, n_t: int = 2000, n_det: int = 16, n_amp: int = 16,
+                        target_seed: int = 2):
+    """d=16 closed-system gate synthesis with a detuning x amplitude ensemble."""
+    rng, omega, J, Hc = four_qubit_parameters(seed)
     dets = np.linspace(-0.05, 0.05, n_det)
     amps = np.linspace(0.9, 1.1, n_amp)
     H0s, scales = [], []

@@ -479,3 +479,21 @@ def test_modulus_objective():
     q.psi_targ = np.array([[0, 1, 0]], complex)
     Fz, Gz = O.evaluate(q, np.zeros((2, 1)))
     assert Fz == 0.0 and np.all(Gz == 0.0)
+
+
+def test_optimized_control_fixture_is_near_the_optimum():
+    # tests/golden/c2_optimized.npz, the point where the reduced-precision
+    # forms are checked near F = 1: the oracle on a spread member sample at
+    # full length reaches F > 0.999, and the target is the nominal system's
+    # propagator (unit fidelity for the nominal member at the reference controls)
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "c2_optimized.npz"))
+    nom, u_ref = W.four_qubit_nominal()
+    assert np.allclose(O.total_propagator(nom, u_ref), fx["target"], atol=1e-12)
+    p, _ = W.four_qubit_ensemble()
+    p.U_target = fx["target"]
+    pick = np.array([0, 85, 170, 255])
+    p.n_members = len(pick)
+    p.H0 = np.asarray(p.H0)[pick]
+    p.ctrl_scale = np.asarray(p.ctrl_scale)[pick]
+    F, _ = O.evaluate(p, fx["controls"])
+    assert F > 0.999 and F <= 1.0
