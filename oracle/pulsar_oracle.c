@@ -40,8 +40,21 @@ static void mat_eye(int n, cx* A) {
   for (int i = 0; i < n; i++) A[i + i * n] = 1.0;
 }
 
+/* Products of n >= 32 outside a parallel region share their columns among
+ * tl_columns threads: or_eval sets it for the calling thread when a problem
+ * has fewer members than threads (a member evaluated alone, its chains); every
+ * column is computed in the same order either way. */
+static _Thread_local int tl_columns = 1;
+#ifdef _OPENMP
+#define PAR_COLUMNS \
+  _Pragma("omp parallel for schedule(static) num_threads(tl_columns) if (tl_columns > 1 && n >= 32 && !omp_in_parallel())")
+#else
+#define PAR_COLUMNS
+#endif
+
 /* C = A * B (column-major n x n) */
 static void mm_nn(int n, const cx* A, const cx* B, cx* C) {
+  PAR_COLUMNS
   for (int j = 0; j < n; j++) {
     cx* c = C + (size_t)j * n;
     for (int i = 0; i < n; i++) c[i] = 0;
@@ -54,6 +67,7 @@ static void mm_nn(int n, const cx* A, const cx* B, cx* C) {
 }
 /* C = A * B^dag */
 static void mm_nc(int n, const cx* A, const cx* B, cx* C) {
+  PAR_COLUMNS
   for (int j = 0; j < n; j++) {
     cx* c = C + (size_t)j * n;
     for (int i = 0; i < n; i++) c[i] = 0;
@@ -66,6 +80,7 @@ static void mm_nc(int n, const cx* A, const cx* B, cx* C) {
 }
 /* C = A^dag * B */
 static void mm_cn(int n, const cx* A, const cx* B, cx* C) {
+  PAR_COLUMNS
   for (int j = 0; j < n; j++)
     for (int i = 0; i < n; i++) {
       cx s = 0;
@@ -424,6 +439,7 @@ int or_expm_frechet(int n, const double* X, const double* E, double* L) {
 typedef struct {
   cx *H, *Hs, *Us, *P, *Q, *T1, *T2, *T3, *V, *A0, *psi, *lam, *tmpv, *ps0, *pst;
   double* w;
+  int inner; /* threads over the slices of member_props (1: none) */
 } work_t;
 
 static void work_free(work_t* W) {
@@ -448,6 +464,7 @@ static void work_free(work_t* W) {
 static int work_alloc(work_t* W, const pr_problem* p) {
   size_t d = p->dim, dd = d * d, nt = p->n_t, np = p->n_pairs > 0 ? p->n_pairs : 1;
   memset(W, 0, sizeof(*W));
+  W->inner = 1;
   W->H = (cx*)malloc(sizeof(cx) * dd);
   W->Hs = (cx*)malloc(sizeof(cx) * dd * p->n_ctrl);
   W->Us = (cx*)malloc(sizeof(cx) * dd * nt);
@@ -497,30 +514,67 @@ static int member_props(const pr_problem* p, int m, const double* u, work_t* W) 
     double s = cscale(p, m, j);
     for (size_t i = 0; i < dd; i++) W->Hs[j * dd + i] = s * ld(p->Hc, j * dd + i);
   }
-  for (int k = 0; k < p->n_t; k++) {
-    double dtk = slice_dt(p, k);
-    cx* Uk = W->Us + k * dd;
-    if (p->dynamics == PR_CLOSED) {
-      for (size_t i = 0; i < dd; i++) W->H[i] = ld(h0, i);
-      for (int j = 0; j < nc; j++) {
-        double a = u[j + (size_t)k * nc];
-        const cx* hs = W->Hs + j * dd;
-        for (size_t i = 0; i < dd; i++) W->H[i] += a * hs[i];
+  /* The slices are independent: with W->inner > 1 (fewer members than
+   * threads) they are shared among threads, each with scratch of its own
+   * (thread 0 uses W's). */
+  int err = PR_OK;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(W->inner) if (W->inner > 1)
+#endif
+  {
+    cx *H = W->H, *V = W->V, *T1 = W->T1;
+    double* w = W->w;
+    int own = 0;
+#ifdef _OPENMP
+    if (omp_get_thread_num() > 0) {
+      H = (cx*)malloc(sizeof(cx) * dd);
+      V = (cx*)malloc(sizeof(cx) * dd);
+      T1 = (cx*)malloc(sizeof(cx) * dd);
+      w = (double*)malloc(sizeof(double) * d);
+      own = 1;
+    }
+#endif
+    int lerr = (!H || !V || !T1 || !w) ? PR_ERR_NOMEM : PR_OK;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int k = 0; k < p->n_t; k++) {
+      if (lerr) continue;
+      double dtk = slice_dt(p, k);
+      cx* Uk = W->Us + k * dd;
+      for (size_t i = 0; i < dd; i++) H[i] = ld(h0, i);
+      int rc;
+      if (p->dynamics == PR_CLOSED) {
+        for (int j = 0; j < nc; j++) {
+          double a = u[j + (size_t)k * nc];
+          const cx* hs = W->Hs + j * dd;
+          for (size_t i = 0; i < dd; i++) H[i] += a * hs[i];
+        }
+        rc = expm_neg_i_cx(d, H, dtk, Uk, V, T1, w);
+      } else {
+        for (int j = 0; j < nc; j++) {
+          double coef = cscale(p, m, j) * u[j + (size_t)k * nc];
+          for (size_t i = 0; i < dd; i++) H[i] += coef * ld(p->Hc, j * dd + i);
+        }
+        for (size_t i = 0; i < dd; i++) T1[i] = H[i] * dtk;
+        rc = expm_cx(d, T1, Uk);
       }
-      int rc = expm_neg_i_cx(d, W->H, dtk, Uk, W->V, W->T1, W->w);
-      if (rc) return rc;
-    } else {
-      for (size_t i = 0; i < dd; i++) W->H[i] = ld(h0, i);
-      for (int j = 0; j < nc; j++) {
-        double coef = cscale(p, m, j) * u[j + (size_t)k * nc];
-        for (size_t i = 0; i < dd; i++) W->H[i] += coef * ld(p->Hc, j * dd + i);
-      }
-      for (size_t i = 0; i < dd; i++) W->T1[i] = W->H[i] * dtk;
-      int rc = expm_cx(d, W->T1, Uk);
-      if (rc) return rc;
+      if (rc) lerr = rc;
+    }
+    if (lerr) {
+#ifdef _OPENMP
+#pragma omp critical(member_props_err)
+#endif
+      err = lerr;
+    }
+    if (own) {
+      free(H);
+      free(V);
+      free(T1);
+      free(w);
     }
   }
-  return PR_OK;
+  return err;
 }
 
 /* Exact-mode generator of slice k: closed dynamics rebuild H_k into W->H
@@ -788,12 +842,20 @@ int or_eval(const pr_problem* p, const double* controls, int grad_mode, int sepa
   }
   int err = PR_OK;
 #ifdef _OPENMP
+  /* threads over members; with fewer members than threads, the members run
+   * one after another and their slices' propagators in parallel */
   int nth = n_threads > 0 ? n_threads : omp_get_max_threads();
-#pragma omp parallel num_threads(nth)
+  int outer = M < nth ? 1 : nth, inner = M < nth ? nth : 1;
+  const int columns_before = tl_columns;
+  tl_columns = inner;
+#pragma omp parallel num_threads(outer)
+#else
+  int inner = 1;
 #endif
   {
     work_t W;
     int lrc = work_alloc(&W, p);
+    W.inner = inner;
     if (lrc) {
 #ifdef _OPENMP
 #pragma omp atomic write
@@ -815,6 +877,9 @@ int or_eval(const pr_problem* p, const double* controls, int grad_mode, int sepa
     }
     if (!lrc) work_free(&W);
   }
+#ifdef _OPENMP
+  tl_columns = columns_before;
+#endif
   if (err == PR_OK) {
     double Fs = 0;
     for (int m = 0; m < M; m++) {

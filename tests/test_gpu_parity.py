@@ -175,6 +175,42 @@ def test_three_qubit_lindblad_full_length_one_member():
     check(p, u)
 
 
+def test_three_qubit_lindblad_four_members_full_length():
+    # the three-qubit Lindblad ensemble (64 x 64 superoperators) at 4 of its
+    # 64 members and all 1000 slices
+    p, u = W.three_qubit_lindblad(n_members=4)
+    check(p, u)
+
+
+def test_uneven_batch_full_length_subset_matches_oracle():
+    # 32 problems of the benchmarked 1024-problem uneven batch at full length
+    # (the first 6 of d = 2 and 4, the first 5 of d = 8, 16, 32 and 64; 233 to
+    # 1945 slices), reference gradients, all in one plan
+    from concurrent.futures import ThreadPoolExecutor
+    batch = W.uneven_batch()
+    pick = []
+    for d, n in ((2, 6), (4, 6), (8, 5), (16, 5), (32, 5), (64, 5)):
+        pick += [i for i, b in enumerate(batch) if b[0].dim == d][:n]
+    probs = [batch[i][0] for i in pick]
+    us = [batch[i][1] for i in pick]
+    print("uneven subset", [(q.dim, q.n_t) for q in probs], flush=True)
+    plan = native.Plan(probs)
+    try:
+        F, G = plan.eval(us)
+    finally:
+        plan.close()
+    small = [i for i, q in enumerate(probs) if q.dim < 32]
+    with ThreadPoolExecutor(16) as ex:
+        ref = dict(zip(small, ex.map(lambda i: O.evaluate(probs[i], us[i], n_threads=1), small)))
+    for i, q in enumerate(probs):
+        if q.dim >= 32:
+            ref[i] = O.evaluate(q, us[i])  # one member: its slices in parallel
+    for i, q in enumerate(probs):
+        F0, G0 = ref[i]
+        assert abs(F[i] - F0) <= TOL_F, (i, q.dim, q.n_t, F[i], F0)
+        assert np.abs(G[i] - G0).max() / np.abs(G0).max() <= TOL_G, (i, q.dim, q.n_t)
+
+
 def test_uneven_batch_in_one_plan():
     batch = W.uneven_batch(n_problems=24, seed=21, n_min=20, n_max=60)
     probs = [b[0] for b in batch]
@@ -331,6 +367,13 @@ def test_spin_chain_128_small():
     F, G, D = check(p, u)
     assert D[0] < 1e-12
     assert D[1] >= 1  # generators above the degree-9 bound: scaling and squaring
+
+
+def test_spin_chain_128_two_members_500_slices():
+    # the 7-qubit Heisenberg chain (d = 128) at 2 of its 64 members and all 500 slices
+    p, u = W.spin_chain_128(n_members=2)
+    F, G, D = check(p, u)
+    assert D[0] < 1e-12
 
 
 @pytest.mark.parametrize("d,seg", [(65, 0), (80, 3), (100, 0), (128, 5)])
