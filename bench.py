@@ -181,8 +181,8 @@ def other_workload(args):
         desc = "single_qubit_state: d=2, 100 slices, state transfer"
         flops = (72 * 8 + 16 * 2 * 4) * 100
     native.init(0)
-    plan = native.Plan(probs, precision={"fp64": PR_FP64, "mixed": PR_MIXED, "split16": PR_SPLIT16}[args.precision],
-                       segment=args.segment)
+    prec = {"fp64": PR_FP64, "mixed": PR_MIXED, "split16": PR_SPLIT16}[args.precision]
+    plan = native.Plan(probs, precision=prec, segment=args.segment)
     plan.upload_controls(us)
     d_u, d_F, d_G = plan.buffers()
     for _ in range(args.warmup):
@@ -196,6 +196,7 @@ def other_workload(args):
     plan.synchronize()
     kt = plan.kernel_times()
     plan.set_timing(False)
+    counts = plan.exp_counts()
     # ... and the throughput of replayed evaluation graphs
     plan.eval_device(d_u, d_F, d_G)
     plan.synchronize()
@@ -204,8 +205,12 @@ def other_workload(args):
         plan.eval_device(d_u, d_F, d_G)
     plan.synchronize()
     elapsed = time.perf_counter() - t0
-    # the host-buffer entry point (pr_eval: one packed copy of the controls up,
-    # one of the gradients down, through a pinned staging buffer)
+    # the host-buffer entry point pr_eval (one packed copy of the controls up,
+    # one of the gradients down, through a pinned staging buffer), called as
+    # a ccall would, with the per-problem host arrays made once ...
+    abi_ms = c_abi_eval_ms(plan, probs, us, args.steps)
+    # ... and through the Python mirror, which also converts every problem's
+    # controls and gradient between numpy and column-major buffers per call
     plan.eval(us)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -217,7 +222,12 @@ def other_workload(args):
             "ms_per_eval": round(1e3 * elapsed / args.steps, 4), "steps": args.steps,
             "algorithmic_tflops": round(flops * rate / 1e12, 3),
             "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
-            "host_buffers_ms_per_eval": round(1e3 * host_elapsed / args.steps, 4)}
+            "pr_eval_host_buffers_ms_per_eval": round(abi_ms, 4),
+            "python_mirror_eval_ms_per_eval": round(1e3 * host_elapsed / args.steps, 4)}
+    if prec == PR_FP64 and args.workload != "four_qubit_tracking":
+        line["roofline"] = fp64_roofline(probs, counts, kt, args.steps)
+    if args.cpu_sample > 0 and args.workload != "four_qubit_tracking":
+        line["accuracy"] = sample_accuracy(args.workload, probs, us, prec)
     if args.workload == "four_qubit_tracking" and "gradient" in kt:
         # the per-segment backward walk reads every U_k image once (16 x 16
         # complex FP64, 4 KiB) and writes the member's gradient row (n_c
@@ -236,12 +246,99 @@ def other_workload(args):
         n = min(args.cpu_sample, 16)  # about 0.65 s of one core per member
         q, uu = workloads.shrink(p, u, n_members=n)
         t0 = time.perf_counter()
-        O.tracking_evaluate(q, uu)
+        Fo, Go = O.tracking_evaluate(q, uu)
         cpu_s = time.perf_counter() - t0
+        qplan = native.Plan([q], precision=prec)
+        Fd, Gd = qplan.eval([uu])
+        qplan.close()
+        line["accuracy"] = {"sample": f"{n} members, all {p.n_t} slices, against the tracking oracle",
+                            "abs_dF": float(abs(Fd[0] - Fo)),
+                            "max_rel_dG": float(np.abs(Gd[0] - Go).max() / np.abs(Go).max())}
         line["cpu_baseline"] = {"value": round(n / (cpu_s * p.n_members), 6), "unit": "evals/s", "cores": 1,
                                 "kind": "port", "sample": f"{n} of {p.n_members} members, all {p.n_t} slices, "
                                 f"{cpu_s:.1f} s; rate scaled to the full ensemble"}
     print(json.dumps(line))
+
+
+def c_abi_eval_ms(plan, probs, us, steps):
+    """ms per pr_eval call with host buffers, the per-problem column-major
+    control and gradient arrays and their pointer tables made beforehand."""
+    import ctypes
+    from pulsar_rocm.problem import controls_cm
+    L = native.load()
+    n = len(probs)
+    dp = ctypes.POINTER(ctypes.c_double)
+    bufs = [controls_cm(u) for u in us]
+    gbufs = [np.zeros(q.n_ctrl * q.n_t) for q in probs]
+    cptrs = (dp * n)(*[b.ctypes.data_as(dp) for b in bufs])
+    gptrs = (dp * n)(*[g.ctypes.data_as(dp) for g in gbufs])
+    F = np.zeros(n)
+    native.check(L.pr_eval(plan.handle, cptrs, F.ctypes.data_as(dp), gptrs, None))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        native.check(L.pr_eval(plan.handle, cptrs, F.ctypes.data_as(dp), gptrs, None))
+    return 1e3 * (time.perf_counter() - t0) / steps
+
+
+def fp64_roofline(probs, counts, kt, steps):
+    """FP64 matrix-core work of the segment and gradient kernels per
+    evaluation, from the exp counts: each complex product of dimension D (the
+    generator's: d closed, d^2 Lindblad) costs 6 D^3 flops as three real
+    products (Gauss, D <= 64) or 8 D^3 as four (D > 64). k_segment: the exp's
+    products (3 at degree 6, 4 at degree 9, one per squaring) and the chain
+    product per member-slice; k_gradient: 2 products per member-slice for
+    unitary dynamics (M = C U^dag, C' = U M), 3 for Lindblad (forward state,
+    P Lambda^dag, U^dag Lambda). Traces and assembly (VALU) are not counted.
+    The dominant kernel class is the one with more time per evaluation."""
+    from pulsar_rocm.problem import PR_LINDBLAD as LIND
+    seg = grad = 0.0
+    for q, c in zip(probs, counts):
+        D = q.dim
+        g = (6 if D <= 64 else 8) * D ** 3
+        units = q.n_members * q.n_t
+        seg += g * (3 * c[0] + 4 * c[1] + c[2] + units)
+        grad += g * (3 if q.dynamics == LIND else 2) * units
+    seg_ms, grad_ms = kt["segment"][0] / steps, kt["gradient"][0] / steps
+    kern, flops, ms = ("k_segment", seg, seg_ms) if seg_ms >= grad_ms else ("k_gradient", grad, grad_ms)
+    achieved = flops / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "kernel": kern, "achieved": round(achieved, 3), "peak": MFMA_PEAK_TFLOPS["fp64"],
+            "unit": "TFLOP/s", "frac": round(achieved / MFMA_PEAK_TFLOPS["fp64"], 4), "traffic": None,
+            "kernel_ms": round(ms, 4), "flops_per_eval": int(flops),
+            "work_model": "FP64 complex products, 6 D^3 flops (Gauss, D <= 64) or 8 D^3 (D > 64) each; "
+                          "k_segment exp + chain per member-slice, k_gradient 2 (unitary) or 3 (Lindblad); "
+                          "all launches of the kernel class in one evaluation"}
+
+
+def sample_accuracy(workload, probs, us, prec):
+    """|dF|, max relative dG and the unitarity defect of the device on a
+    bounded sample of the workload against the FP64 oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    if workload == "uneven_batch":  # the first 2 problems of each dimension
+        pick = []
+        for d in sorted({q.dim for q in probs}):
+            pick += [i for i, q in enumerate(probs) if q.dim == d][:2]
+        sp, su = [probs[i] for i in pick], [us[i] for i in pick]
+        desc = f"{len(sp)} problems (the first 2 of each dimension), full length"
+    elif workload in ("three_qubit_lindblad", "spin_chain_128"):
+        q, uu = workloads.shrink(probs[0], us[0], n_members=2)
+        sp, su = [q], [uu]
+        desc = f"2 of {probs[0].n_members} members, all {q.n_t} slices"
+    else:
+        sp, su = probs, us
+        desc = "the whole workload"
+    plan = native.Plan(sp, precision=prec)
+    Fd, Gd, D = plan.eval(su, diag=True)
+    plan.close()
+    dF = dG = 0.0
+    for i, (q, uu) in enumerate(zip(sp, su)):
+        Fo, Go = O.evaluate(q, uu)
+        dF = max(dF, abs(Fd[i] - Fo))
+        dG = max(dG, float(np.abs(Gd[i] - Go).max() / np.abs(Go).max()))
+    from pulsar_rocm.problem import PR_CLOSED as CLOSED
+    closed = [i for i, q in enumerate(sp) if q.dynamics == CLOSED]
+    return {"sample": desc, "abs_dF": float(dF), "max_rel_dG": dG,
+            "unitarity_defect": float(np.max(D[closed, 0])) if closed else None}
 
 
 def mr_workload(args):
@@ -319,6 +416,7 @@ def timed_form(p, u, prec, args):
     elapsed = time.perf_counter() - t0
     kt = plan.kernel_times()
     plan.set_timing(False)
+    counts = plan.exp_counts()
 
     # the same timed loop without event records, for the headline number
     plan.synchronize()

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pulsar_rocm.h"
@@ -340,6 +341,11 @@ struct Plan {
   double* d_G = nullptr;
   double* d_part = nullptr;  // [RED_CHUNKS][n_params] partial member sums
   double* h_stage = nullptr; // pinned host staging: n_params controls or gradients, then n_probs F
+  // pr_eval's staging ranges: problems [stage_cut[r], stage_cut[r + 1]) are
+  // packed by thread r and copied by one transfer each, event stage_ev[r]
+  // marking the range's gradient download
+  std::vector<int> stage_cut;
+  std::vector<hipEvent_t> stage_ev;
   // pr_eval_device_static: the byte sizes of each problem's scratch buffers,
   // their offsets in the static scratch (udef arrays first, so one fill zeroes
   // them), the reduction partials' offset, the total, and the device copies
@@ -398,6 +404,7 @@ struct Plan {
       (void)hipEventDestroy(r.b);
     }
     for (auto e : ev_free) (void)hipEventDestroy(e);
+    for (auto e : stage_ev) (void)hipEventDestroy(e);
     if (ev_open) (void)hipEventDestroy(ev_open);
     for (void* p : allocs) (void)hipFree(p);
     if (h_stage) (void)hipHostFree(h_stage);
@@ -1270,6 +1277,93 @@ int stage_controls(Plan& pl, const double* const* controls, const char* who) {
   return PR_OK;
 }
 
+// Staging ranges of pr_eval: one per host thread, of about equal bytes, at
+// least 2 MiB each and at most 8 (a plan of a few MiB is packed by the
+// calling thread alone).
+int stage_ranges(Plan& pl) {
+  if (!pl.stage_cut.empty()) return PR_OK;
+  const size_t min_range = (size_t)1 << 18;  // doubles
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int R = (int)std::max<size_t>(1, std::min<size_t>({8, hw, (size_t)pl.n_params / min_range}));
+  pl.stage_cut.assign(1, 0);
+  for (int r = 1; r < R; r++) {
+    const long long target = (long long)((double)pl.n_params * r / R);
+    int p = pl.stage_cut.back();
+    while (p < pl.n_probs && pl.coff[p] < target) p++;
+    if (p > pl.stage_cut.back() && p < pl.n_probs) pl.stage_cut.push_back(p);
+  }
+  pl.stage_cut.push_back(pl.n_probs);
+  pl.stage_ev.assign(pl.stage_cut.size() - 1, nullptr);
+  for (auto& e : pl.stage_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return PR_OK;
+}
+
+// pr_eval's transfers with host buffers: the ranges' controls are packed in
+// parallel and each range is uploaded as soon as it is packed (the next
+// ranges are still being packed meanwhile); after the evaluation, each
+// range's gradients are downloaded and unpacked by their thread once their
+// own download has finished, while later ranges are still in flight.
+int staged_eval(Plan& pl, const double* const* controls, double* F, double* const* G) {
+  for (int p = 0; p < pl.n_probs; p++)
+    if (!controls[p]) return fail(PR_ERR_ARG, "pr_eval: controls[%d] is NULL", p);
+  int rc = stage_ranges(pl);
+  if (rc) return rc;
+  const int R = (int)pl.stage_ev.size();
+  auto range_of = [&](int r, size_t& off, size_t& len) {
+    off = (size_t)pl.coff[pl.stage_cut[r]];
+    len = (size_t)(pl.coff[pl.stage_cut[r + 1]] - pl.coff[pl.stage_cut[r]]);
+  };
+  auto pack = [&](int r) {
+    for (int p = pl.stage_cut[r]; p < pl.stage_cut[r + 1]; p++)
+      memcpy(pl.h_stage + pl.coff[p], controls[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]));
+  };
+  auto unpack = [&](int r) {
+    for (int p = pl.stage_cut[r]; p < pl.stage_cut[r + 1]; p++)
+      if (G[p]) memcpy(G[p], pl.h_stage + pl.coff[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]));
+  };
+  std::vector<std::thread> th;
+  for (int r = 1; r < R; r++) th.emplace_back([&, r] { pack(r); });
+  pack(0);
+  hipError_t e = hipSuccess;
+  for (int r = 0; r < R; r++) {
+    if (r > 0) th[r - 1].join();
+    size_t off, len;
+    range_of(r, off, len);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(pl.d_controls + off, pl.h_stage + off, sizeof(double) * len, hipMemcpyHostToDevice,
+                         pl.stream);
+  }
+  th.clear();
+  if (e != hipSuccess) return fail(PR_ERR_DEVICE, "pr_eval: hipMemcpyAsync: %s", hipGetErrorString(e));
+  rc = run_graph(pl, pl.d_controls, pl.d_F, G ? pl.d_G : nullptr, pl.stream);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(pl.h_stage + pl.n_params, pl.d_F, sizeof(double) * pl.n_probs, hipMemcpyDeviceToHost,
+                        pl.stream));
+  if (G) {
+    for (int r = 0; r < R; r++) {
+      size_t off, len;
+      range_of(r, off, len);
+      HIPCHK(hipMemcpyAsync(pl.h_stage + off, pl.d_G + off, sizeof(double) * len, hipMemcpyDeviceToHost, pl.stream));
+      HIPCHK(hipEventRecord(pl.stage_ev[r], pl.stream));
+    }
+    std::vector<hipError_t> we(R, hipSuccess);
+    for (int r = 1; r < R; r++)
+      th.emplace_back([&, r] {
+        if (g_device >= 0) (void)hipSetDevice(g_device);
+        we[r] = hipEventSynchronize(pl.stage_ev[r]);
+        if (we[r] == hipSuccess) unpack(r);
+      });
+    we[0] = hipEventSynchronize(pl.stage_ev[0]);
+    if (we[0] == hipSuccess) unpack(0);
+    for (auto& t : th) t.join();
+    for (int r = 0; r < R; r++)
+      if (we[r] != hipSuccess) return fail(PR_ERR_DEVICE, "pr_eval: hipEventSynchronize: %s", hipGetErrorString(we[r]));
+  }
+  HIPCHK(hipStreamSynchronize(pl.stream));
+  memcpy(F, pl.h_stage + pl.n_params, sizeof(double) * pl.n_probs);
+  return PR_OK;
+}
+
 extern "C" {
 
 int pr_abi_version(void) { return PR_ABI_VERSION; }
@@ -1355,20 +1449,8 @@ int pr_eval(void* plan, const double* const* controls, double* F, double* const*
   if (!plan || !controls || !F) return fail(PR_ERR_ARG, "pr_eval: plan, controls and F are required");
   Plan& pl = *static_cast<Plan*>(plan);
   if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
-  int rc = stage_controls(pl, controls, "pr_eval");
+  int rc = staged_eval(pl, controls, F, G);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(pl.d_controls, pl.h_stage, sizeof(double) * pl.n_params, hipMemcpyHostToDevice, pl.stream));
-  rc = run_graph(pl, pl.d_controls, pl.d_F, G ? pl.d_G : nullptr, pl.stream);
-  if (rc) return rc;
-  // F and G come back in one copy each, through the pinned staging buffer
-  HIPCHK(hipMemcpyAsync(pl.h_stage + pl.n_params, pl.d_F, sizeof(double) * pl.n_probs, hipMemcpyDeviceToHost,
-                        pl.stream));
-  if (G) HIPCHK(hipMemcpyAsync(pl.h_stage, pl.d_G, sizeof(double) * pl.n_params, hipMemcpyDeviceToHost, pl.stream));
-  HIPCHK(hipStreamSynchronize(pl.stream));
-  memcpy(F, pl.h_stage + pl.n_params, sizeof(double) * pl.n_probs);
-  if (G)
-    for (int p = 0; p < pl.n_probs; p++)
-      if (G[p]) memcpy(G[p], pl.h_stage + pl.coff[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]));
   if (diag) return fill_diag(pl, diag);
   return PR_OK;
 }

@@ -256,6 +256,38 @@ def test_device_resident_path_matches_host_path():
         plan.close()
 
 
+def test_host_buffer_ranges_match_the_device_resident_evaluation():
+    # 256 problems of the uneven batch: pr_eval packs, uploads, downloads and
+    # unpacks their controls and gradients in several ranges on host threads;
+    # its F and G equal the device-resident evaluation's device buffers
+    import ctypes
+    from pulsar_rocm.problem import controls_cm
+    batch = W.uneven_batch(n_problems=256, seed=8)
+    probs, us = [b[0] for b in batch], [b[1] for b in batch]
+    plan = native.Plan(probs)
+    try:
+        assert plan.n_params > 1 << 19  # more than one range of 2 MiB
+        F, G = plan.eval(us)
+        plan.upload_controls(us)
+        dc, dF, dG = plan.buffers()
+        plan.eval_device(dc, dF, dG)
+        plan.synchronize()
+        hip = ctypes.CDLL("libamdhip64.so")
+        Fd = np.zeros(len(probs))
+        Gd = np.zeros(plan.n_params)
+        assert hip.hipMemcpy(Fd.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(dF), ctypes.c_size_t(Fd.nbytes), 2) == 0
+        assert hip.hipMemcpy(Gd.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(dG), ctypes.c_size_t(Gd.nbytes), 2) == 0
+        assert np.array_equal(F, Fd)
+        off = 0
+        for p, g in zip(probs, G):
+            n = p.n_ctrl * p.n_t
+            assert np.array_equal(controls_cm(g), Gd[off:off + n])
+            off += n
+        assert off == plan.n_params
+    finally:
+        plan.close()
+
+
 def test_graph_replays_match_direct_launches():
     # evaluations replay a captured HIP graph; with per-kernel timing on, the
     # same evaluation launches its kernels directly. Several replays with new
