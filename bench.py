@@ -226,6 +226,8 @@ def other_workload(args):
             "python_mirror_eval_ms_per_eval": round(1e3 * host_elapsed / args.steps, 4)}
     if prec == PR_FP64 and args.workload != "four_qubit_tracking":
         line["roofline"] = fp64_roofline(probs, counts, kt, args.steps)
+    if args.workload == "spin_chain_128":
+        line["recompute_mode"] = recompute_timing(probs, us, args)
     if args.cpu_sample > 0 and args.workload != "four_qubit_tracking":
         line["accuracy"] = sample_accuracy(args.workload, probs, us, prec)
     if args.workload == "four_qubit_tracking" and "gradient" in kt:
@@ -258,6 +260,37 @@ def other_workload(args):
                                 "kind": "port", "sample": f"{n} of {p.n_members} members, all {p.n_t} slices, "
                                 f"{cpu_s:.1f} s; rate scaled to the full ensemble"}
     print(json.dumps(line))
+
+
+def recompute_timing(probs, us, args):
+    """The same evaluation without stored U_k images (store_props = 0): the
+    gradient walk recomputes each U_k from segment checkpoints."""
+    plan = native.Plan(probs, store_props=0, segment=args.segment)
+    try:
+        plan.upload_controls(us)
+        d_u, d_F, d_G = plan.buffers()
+        for _ in range(args.warmup):
+            plan.eval_device(d_u, d_F, d_G)
+        plan.synchronize()
+        plan.set_timing(True)
+        plan.kernel_times()
+        for _ in range(args.steps):
+            plan.eval_device(d_u, d_F, d_G)
+        plan.synchronize()
+        kt = plan.kernel_times()
+        plan.set_timing(False)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.eval_device(d_u, d_F, d_G)
+        plan.synchronize()
+        el = time.perf_counter() - t0
+        D = plan.eval(us, diag=True)[2]
+    finally:
+        plan.close()
+    images = sum(q.n_members * q.n_t * ((q.dim + 15) // 16) ** 2 * 4096 for q in probs)
+    return {"evals_per_s": round(args.steps / el, 3), "ms_per_eval": round(1e3 * el / args.steps, 3),
+            "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+            "segments_per_member": int(D[0, 3]), "image_bytes_saved": int(images)}
 
 
 def c_abi_eval_ms(plan, probs, us, steps):

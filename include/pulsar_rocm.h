@@ -158,9 +158,10 @@ typedef struct {
   int grad_mode;   /* PR_GRAD_REFERENCE or PR_GRAD_EXACT */
   int segment;     /* slices per time segment of the parallel chain scan; 0 = auto */
   int store_props; /* 1: keep U_k in HBM between the passes; 0: recompute U_k in the
-                      gradient walk for closed-dynamics FP64 problems with dim <= 64
-                      (the others keep their images); -1: recompute only when all
-                      images would take more than half of the free device memory */
+                      gradient walk for closed-dynamics FP64 problems (dim > 64: from
+                      segment checkpoints every 32 slices at most; the others keep
+                      their images); -1: recompute only when all images would take
+                      more than half of the free device memory */
 } pr_options;
 
 /* Per-problem diagnostics filled by pr_eval when diag != NULL:

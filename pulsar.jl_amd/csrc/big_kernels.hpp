@@ -223,8 +223,46 @@ __device__ void scale_blocks(double2* X, double f) {
 
 }  // namespace big
 
-// scratch matrices per work item of the big kernels
-constexpr int BIG_SCRATCH = 7;
+// scratch matrices per work item of the big kernels: the segment walk's X,
+// X^2, X^3, R, Q, T, T' and, without stored images, U_k; the gradient walk's
+// C, M, Y and, recomputing U_k, X, X^2, X^3, R, Q and U_k
+constexpr int BIG_SCRATCH = 9;
+
+namespace big {
+
+// U = exp(X_k) for slice k of member m into the image U: the generator and
+// its 1-norm bound, the scaling exponent and degree (choose_scaling), the
+// Taylor polynomial of degree 6 or 9 in Paterson-Stockmeyer form and the
+// squarings. The segment walk and, for plans that keep no U_k images, the
+// gradient walk both call it, so a recomputed U_k is the stored one bit for
+// bit. Returns the scaling exponent; deg6 tells the degree.
+template <int B>
+__device__ int exp_slice(const ProbDev& P, int m, int k, const double* __restrict__ u, double2* X, double2* X2,
+                         double2* X3, double2* R, double2* Q, double2* U, double2* LP, double* colsum,
+                         double* red, bool& deg6) {
+  const double c2 = 1.0 / 2, c3 = 1.0 / 6, c4 = 1.0 / 24, c5 = 1.0 / 120, c6 = 1.0 / 720, c7 = 1.0 / 5040,
+               c8 = 1.0 / 40320, c9 = 1.0 / 362880;
+  const int s = choose_scaling(assemble_norm<B>(P, m, k, u, X, colsum, red), deg6);
+  if (s) scale_blocks<B>(X, ldexp(1.0, -s));
+  mm<B, N, N>(X2, X, X, LP);
+  mm<B, N, N>(X3, X, X2, LP);
+  if (deg6) {
+    combo<B>(R, c3, c4, X, c5, X2, c6, X3);
+  } else {
+    combo<B>(Q, c6, c7, X, c8, X2, c9, X3);
+    mm<B, N, N>(R, X3, Q, LP);
+    combo<B>(R, c3, 1.0, R, c5, X2, c4, X);
+  }
+  mm<B, N, N>(Q, X3, R, LP);
+  combo<B>(U, 1.0, 1.0, Q, c2, X2, 1.0, X);
+  for (int i = 0; i < s; i++) {
+    mm<B, N, N>(Q, U, U, LP);
+    combo<B>(U, 0.0, 1.0, Q, 0.0, Q, 0.0, nullptr);
+  }
+  return s;
+}
+
+}  // namespace big
 
 template <int B>
 __global__ __launch_bounds__(1024) void k_segment_big(const ProbDev* __restrict__ probs,
@@ -240,36 +278,19 @@ __global__ __launch_bounds__(1024) void k_segment_big(const ProbDev* __restrict_
   double* red = colsum + 16 * B * B;
   double2* S = scratch + (size_t)blockIdx.x * BIG_SCRATCH * MB;
   double2 *X = S, *X2 = S + MB, *X3 = S + 2 * MB, *R = S + 3 * MB, *Q = S + 4 * MB, *T = S + 5 * MB,
-          *T2 = S + 6 * MB;
-  const double c2 = 1.0 / 2, c3 = 1.0 / 6, c4 = 1.0 / 24, c5 = 1.0 / 120, c6 = 1.0 / 720, c7 = 1.0 / 5040,
-               c8 = 1.0 / 40320, c9 = 1.0 / 362880;
+          *T2 = S + 6 * MB, *Uw = S + 7 * MB;
   const int m = it.member;
   big::store_eye_blocks(T, K_::NB, B);
   __syncthreads();
   int smax = 0, n9 = 0, nsq = 0;
   for (int k = it.k0; k < it.k1; k++) {
+    // the image of U_k itself, or (no stored images) a scratch matrix
+    double2* U = P.recompute ? Uw : P.Ust + ((size_t)m * P.nt + k) * MB;
     bool deg6;
-    const int s = choose_scaling(big::assemble_norm<B>(P, m, k, u, X, colsum, red), deg6);
+    const int s = big::exp_slice<B>(P, m, k, u, X, X2, X3, R, Q, U, LP, colsum, red, deg6);
     smax = max(smax, s);
     n9 += deg6 ? 0 : 1;
     nsq += s;
-    if (s) big::scale_blocks<B>(X, ldexp(1.0, -s));
-    big::mm<B, N, N>(X2, X, X, LP);
-    big::mm<B, N, N>(X3, X, X2, LP);
-    if (deg6) {
-      big::combo<B>(R, c3, c4, X, c5, X2, c6, X3);
-    } else {
-      big::combo<B>(Q, c6, c7, X, c8, X2, c9, X3);
-      big::mm<B, N, N>(R, X3, Q, LP);
-      big::combo<B>(R, c3, 1.0, R, c5, X2, c4, X);
-    }
-    double2* U = P.Ust + ((size_t)m * P.nt + k) * MB;  // image of U_k itself
-    big::mm<B, N, N>(Q, X3, R, LP);
-    big::combo<B>(U, 1.0, 1.0, Q, c2, X2, 1.0, X);
-    for (int i = 0; i < s; i++) {
-      big::mm<B, N, N>(Q, U, U, LP);
-      big::combo<B>(U, 0.0, 1.0, Q, 0.0, Q, 0.0, nullptr);
-    }
     big::mm<B, N, N>(T2, U, T, LP);
     double2* tmp = T;
     T = T2;
@@ -336,10 +357,12 @@ __global__ __launch_bounds__(1024) void k_combine_big(const ProbDev* __restrict_
   }
 }
 
+// Without stored U_k images (P.recompute) the walk starts from the segment's
+// checkpoint E_s and rebuilds each U_k with exp_slice, as k_segment_big made it.
 template <int B>
 __global__ __launch_bounds__(1024) void k_gradient_big(const ProbDev* __restrict__ probs,
                                                        const WorkItem* __restrict__ items,
-                                                       double2* __restrict__ scratch) {
+                                                       const double* __restrict__ u, double2* __restrict__ scratch) {
   using K_ = big::Cfg<B>;
   constexpr size_t MB = K_::MB;
   const WorkItem it = items[blockIdx.x];
@@ -347,14 +370,22 @@ __global__ __launch_bounds__(1024) void k_gradient_big(const ProbDev* __restrict
   const int m = it.member;
   extern __shared__ double2 smem_raw[];
   double2* LP = smem_raw;
+  double* colsum = reinterpret_cast<double*>(reinterpret_cast<char*>(smem_raw) + K_::panel_bytes);
+  double* red = colsum + 16 * B * B;
   double2* S = scratch + (size_t)blockIdx.x * BIG_SCRATCH * MB;
   double2 *Cm = S, *Mk = S + MB, *Y = S + 2 * MB;
+  double2 *X = S + 3 * MB, *X2 = S + 4 * MB, *X3 = S + 5 * MB, *R = S + 6 * MB, *Q = S + 7 * MB, *Uw = S + 8 * MB;
   const double2* Es = P.Eseg + ((size_t)m * (P.nseg + 1) + it.seg) * MB;
   big::mm<B, N, N>(Y, Es, P.Om + (size_t)m * MB, LP);
   big::mm<B, N, C>(Cm, Y, Es, LP);
   const int nc = P.nc, w = big::wave_id(), lane = threadIdx.x & 63;
   for (int k = it.k0; k < it.k1; k++) {
     const double2* U = P.Ust + ((size_t)m * P.nt + k) * MB;
+    if (P.recompute) {
+      bool deg6;
+      (void)big::exp_slice<B>(P, m, k, u, X, X2, X3, R, Q, Uw, LP, colsum, red, deg6);
+      U = Uw;
+    }
     big::mm<B, N, C>(Mk, Cm, U, LP);  // M_k = C_k U_k^dag
     // G_m[j,k] = Re(-i dt s_mj Tr(A_j M)): wavefront w takes controls j = w, w + 16, ...
     // (M = C_k when the first-order term sits after U_k, P.after)

@@ -42,7 +42,7 @@ struct ProbDev {
   int prescaled;              // 1: drift images and control entries already hold c dt A
                               // (uniform dt), so assemble() needs no final scaling
   int recompute;              // 1: no U_k images (Ust is null); the gradient walk
-                              // recomputes U_k (closed dynamics, d <= 64, FP64)
+                              // recomputes U_k (closed dynamics, FP64)
   int after;                  // first-order gradient term of slice k: 1 with A_j after U_k,
                               // Tr(.. U_k A_j P_k) (compute_grape_gradient, whose
                               // Q_k^dag = U_n..U_k, Propagators.jl:375-380); 0 in place of

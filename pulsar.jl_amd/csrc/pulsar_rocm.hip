@@ -750,7 +750,8 @@ int launch_big(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hip
   if ((rc = tend(pl, 1, st))) return rc;
   if (grad) {
     if ((rc = tbegin(pl, st))) return rc;
-    hipLaunchKernelGGL((pr::k_gradient_big<B>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, g.big_scratch);
+    hipLaunchKernelGGL((pr::k_gradient_big<B>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u,
+                       g.big_scratch);
     HIPCHK(hipGetLastError());
     if ((rc = tend(pl, 2, st))) return rc;
   }
@@ -899,8 +900,9 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     if (probs[p].dim <= 16 && probs[p].dynamics != PR_CLOSED) mixed1 = false;
   // (tracking problems run FP64 in their own groups, with stored images)
   // U_k images between the passes (store_props): closed-dynamics FP64 problems
-  // with d <= 64 can recompute U_k in the gradient walk instead. Auto (-1)
-  // recomputes when all images would take more than half of the free HBM.
+  // can recompute U_k in the gradient walk instead (d > 64: from checkpoints
+  // every 32 slices at most). Auto (-1) recomputes when all images would take
+  // more than half of the free HBM.
   // (a d <= 16 closed problem sharing its group with a Lindblad problem runs
   // the general segment kernel, whose U_k image the recomputation would not
   // reproduce bit for bit: it keeps its images)
@@ -908,7 +910,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   for (int p = 0; p < n_probs; p++)
     if (probs[p].dim <= 16 && probs[p].dynamics != PR_CLOSED) lindblad1 = true;
   auto can_recompute = [&](const pr_problem& q) {
-    return q.dynamics == PR_CLOSED && q.dim <= 64 && q.objective != PR_TRACKING && !(q.dim <= 16 && (mixed1 || lindblad1));
+    return q.dynamics == PR_CLOSED && q.objective != PR_TRACKING && !(q.dim <= 16 && (mixed1 || lindblad1));
   };
   bool recompute = pl.opt.store_props == 0;
   if (pl.opt.store_props < 0) {
@@ -955,6 +957,9 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       const double per_item = std::ceil(group_work[gi] / (double)want);
       L = (int)std::min<double>(std::max<double>(per_item, 1.0), (double)nt);
       L = std::max(L, std::min(nt, 4));
+      // d > 64 without stored images: the segment starts E_s are the
+      // checkpoints the gradient walk recomputes from, every 32 slices at most
+      if (B > 4 && recompute && can_recompute(q)) L = std::min(L, 32);
     }
     L = std::min(L, nt);
     const int S = (nt + L - 1) / L;

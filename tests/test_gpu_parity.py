@@ -402,10 +402,37 @@ def test_spin_chain_128_small():
 
 
 def test_spin_chain_128_two_members_500_slices():
-    # the 7-qubit Heisenberg chain (d = 128) at 2 of its 64 members and all 500 slices
+    # the 7-qubit Heisenberg chain (d = 128) at 2 of its 64 members and all 500
+    # slices, with stored U_k images and without them (store_props = 0: the
+    # gradient walk recomputes U_k from segment checkpoints of <= 32 slices)
     p, u = W.spin_chain_128(n_members=2)
-    F, G, D = check(p, u)
-    assert D[0] < 1e-12
+    print(f"case spin chain d={p.dim} M={p.n_members} n_t={p.n_t}", flush=True)
+    F0, G0 = O.evaluate(p, u)
+    for sp in (1, 0):
+        F, G, D = run(p, u, store_props=sp)
+        print("store_props", sp, "segments", D[3], "dF", abs(F - F0), flush=True)
+        assert abs(F - F0) <= TOL_F
+        assert np.abs(G - G0).max() / np.abs(G0).max() <= TOL_G
+        assert D[0] < 1e-12
+        if sp == 0:
+            assert D[3] >= 16  # checkpoints every 32 slices at most
+
+
+@pytest.mark.parametrize("seg", [32, 7])
+def test_spin_chain_128_recomputed_propagators_equal_stored_images(seg):
+    # same segmentation in both plans: the recomputed U_k are the stored ones
+    # bit for bit, so F and G are identical
+    p, u = W.spin_chain_128(n_members=2, n_t=70)
+    out = []
+    for sp in (1, 0):
+        plan = native.Plan([p], store_props=sp, segment=seg)
+        try:
+            out.append(plan.eval([u]))
+        finally:
+            plan.close()
+    (F1, G1), (F0, G0) = out
+    assert F0[0] == F1[0]
+    assert np.array_equal(G0[0], G1[0])
 
 
 @pytest.mark.parametrize("d,seg", [(65, 0), (80, 3), (100, 0), (128, 5)])
