@@ -319,6 +319,47 @@ typedef struct {
 int pr_lbfgs_optimize(void* plan, const double* controls_init, const pr_lbfgs_config* cfg, double* controls_best,
                       double* fidelity_history, double* gnorm_history, pr_grape_result* result);
 
+/* The MR optimcon loops on the device: replaces _optimcon_lbfgs /
+ * _optimcon_grape and their tracking forms (src/Application/MR/OptControl.jl:
+ * 380-484, 488-561, 1040-1216) for one plan problem built from an MRControl
+ * or LindbladMRControl (state pairs, Lindblad or tracking objective).
+ *   PR_MR_LBFGS: the two-loop recursion of OptControl.jl:565-590 (no
+ *     safeguards), the direction divided by max|d| (stop PR_STOP_ZERO_DIRECTION
+ *     below 1e-14), the Armijo backtracking of :594-611 from
+ *     a0 = 0.05 (u_bound - l_bound) (accepted at F_new >= F - 1e-4 |a|, up to
+ *     20 halvings, then the last trial if it improves F, else no move; F-only
+ *     evaluations, counted as the reference counts them), the pair
+ *     s = w_new - w, y = G_new - G kept when y.s > 1e-14 |s|^2, the oldest
+ *     dropped beyond lbfgs_memory.
+ *   PR_MR_GRAPE: w <- clip(w + 0.05 (u_bound - l_bound) / max|G| G).
+ * Every trial point is clipped to [l_bound, u_bound], the guess too. Stops
+ * after an iteration: PR_STOP_PERFECT (F >= 1 - 1e-8), PR_STOP_GRADIENT
+ * (|G| < grad_tol), and for L-BFGS PR_STOP_FIDELITY (|F_n - F_{n-1}| < 1e-9,
+ * "fidelity stalled"). The waveform, gradient, curvature pairs and best
+ * waveform stay in device memory; per evaluation the host reads F (and |G|,
+ * max|d|, the pair's curvature). tile > 1: the plan holds packed members
+ * (f = tile diagonal blocks per member), its controls are tile copies of the
+ * waveform rows ([tile n_ctrl x n_t]) and the waveform's gradient is the sum
+ * of the tile channel rows. */
+#define PR_MR_LBFGS 0
+#define PR_MR_GRAPE 1
+typedef struct {
+  int method;        /* PR_MR_LBFGS or PR_MR_GRAPE */
+  int max_iter;
+  double grad_tol;
+  int lbfgs_memory;  /* 1..64 */
+  double l_bound;
+  double u_bound;
+  int tile;          /* >= 1 */
+} pr_mr_config;
+
+/* controls_init / controls_best: the waveform, [n_ctrl x n_t] column-major
+ * (n_ctrl = the plan's n_ctrl / tile). fidelity_history and gnorm_history need
+ * max_iter + 1 entries each (entry 0: the guess). result->fidelity is the best
+ * F of the history, result->n_iterations the history's length - 1. */
+int pr_mr_optimcon(void* plan, const double* controls_init, const pr_mr_config* cfg, double* controls_best,
+                   double* fidelity_history, double* gnorm_history, pr_grape_result* result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1643,9 +1643,14 @@ __device__ inline double lbfgs_block_sum(double v, double* ws) {
   return t;
 }
 
+// floor: the lower bound of y.s and y.y in the divisions (1e-14 for the
+// ensemble optimizer; 0 for the MR optimcon loop, whose two-loop recursion
+// divides by them as they are, OptControl.jl:565-590, and keeps only pairs
+// with y.s > 1e-14 |s|^2 > 0)
 __global__ void __launch_bounds__(LBFGS_THREADS)
     k_lbfgs_direction(const double* __restrict__ g, const double* __restrict__ S, const double* __restrict__ Y,
-                      int k, int first, int rows, long long n, double* __restrict__ d, double* __restrict__ out) {
+                      int k, int first, int rows, long long n, double floor, double* __restrict__ d,
+                      double* __restrict__ out) {
   __shared__ double ws[LBFGS_THREADS / 64];
   __shared__ double alpha[64];
   const int t = threadIdx.x;
@@ -1660,7 +1665,7 @@ __global__ void __launch_bounds__(LBFGS_THREADS)
   for (int i = k - 1; i >= 0; i--) {
     const double* s = row(S, i);
     const double* y = row(Y, i);
-    const double rho = 1.0 / fmax(dot(y, s), 1e-14);
+    const double rho = 1.0 / fmax(dot(y, s), floor);
     const double a = rho * dot(s, d);
     if (t == 0) alpha[i] = a;
     for (long long j = t; j < n; j += LBFGS_THREADS) d[j] -= a * y[j];
@@ -1669,14 +1674,14 @@ __global__ void __launch_bounds__(LBFGS_THREADS)
   if (k > 0) {
     const double* s = row(S, k - 1);
     const double* y = row(Y, k - 1);
-    const double gamma = dot(s, y) / fmax(dot(y, y), 1e-14);
+    const double gamma = dot(s, y) / fmax(dot(y, y), floor);
     for (long long j = t; j < n; j += LBFGS_THREADS) d[j] *= gamma;
     __syncthreads();
   }
   for (int i = 0; i < k; i++) {
     const double* s = row(S, i);
     const double* y = row(Y, i);
-    const double rho = 1.0 / fmax(dot(y, s), 1e-14);
+    const double rho = 1.0 / fmax(dot(y, s), floor);
     const double beta = rho * dot(y, d);
     const double c = alpha[i] - beta;
     for (long long j = t; j < n; j += LBFGS_THREADS) d[j] += c * s[j];
@@ -1730,6 +1735,78 @@ __global__ void __launch_bounds__(LBFGS_THREADS)
   if (threadIdx.x == 0) {
     out[0] = ys;
     out[1] = ss;
+  }
+}
+
+// ------------------------------------------------------- MR optimcon loop
+// trial point clip(u + a d, lo, hi), the product and the sum rounded
+// separately (as the host loop's numpy does)
+__global__ void k_mr_trial(const double* __restrict__ u, const double* __restrict__ d, double a, double lo, double hi,
+                           long long n, double* __restrict__ un) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  un[i] = fmin(fmax(__dadd_rn(u[i], __dmul_rn(a, d[i])), lo), hi);
+}
+
+// packed members: the plan's controls are f copies of the waveform rows,
+// out[k f nc + i nc + j] = x[k nc + j]
+__global__ void k_mr_tile(const double* __restrict__ x, long long n, int nc, int f, double* __restrict__ out) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const long long k = e / nc;
+  const int j = (int)(e - k * nc);
+  const double v = x[e];
+  for (int i = 0; i < f; i++) out[k * f * nc + (long long)i * nc + j] = v;
+}
+
+// ... and the waveform's gradient is the sum of the f channel rows, in order
+__global__ void k_mr_untile(const double* __restrict__ Gp, long long n, int nc, int f, double* __restrict__ g) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const long long k = e / nc;
+  const int j = (int)(e - k * nc);
+  double s = Gp[k * f * nc + j];
+  for (int i = 1; i < f; i++) s += Gp[k * f * nc + (long long)i * nc + j];
+  g[e] = s;
+}
+
+// curvature pair from the accepted step: s = un - u, y = gn - g into row
+// `slot`; out[0] = y.s, out[1] = s.s
+__global__ void __launch_bounds__(LBFGS_THREADS)
+    k_mr_pair(const double* __restrict__ u, const double* __restrict__ un, const double* __restrict__ gn,
+              const double* __restrict__ g, long long n, double* __restrict__ s, double* __restrict__ y,
+              double* __restrict__ out) {
+  __shared__ double ws[LBFGS_THREADS / 64];
+  double ys = 0, ss = 0;
+  for (long long i = threadIdx.x; i < n; i += LBFGS_THREADS) {
+    const double sv = un[i] - u[i], yv = gn[i] - g[i];
+    s[i] = sv;
+    y[i] = yv;
+    ys += yv * sv;
+    ss += sv * sv;
+  }
+  ys = lbfgs_block_sum(ys, ws);
+  ss = lbfgs_block_sum(ss, ws);
+  if (threadIdx.x == 0) {
+    out[0] = ys;
+    out[1] = ss;
+  }
+}
+
+// out[0] = max |g_i| (one workgroup)
+__global__ void __launch_bounds__(LBFGS_THREADS) k_absmax(const double* __restrict__ g, long long n,
+                                                         double* __restrict__ out) {
+  __shared__ double ws[LBFGS_THREADS / 64];
+  double m = 0;
+  for (long long i = threadIdx.x; i < n; i += LBFGS_THREADS) m = fmax(m, fabs(g[i]));
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0;
+    for (int w = 0; w < LBFGS_THREADS / 64; w++) r = fmax(r, ws[w]);
+    out[0] = r;
   }
 }
 

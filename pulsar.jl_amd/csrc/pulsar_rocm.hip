@@ -1794,7 +1794,7 @@ int pr_lbfgs_optimize(void* plan, const double* controls_init, const pr_lbfgs_co
     const double gnorm = std::sqrt(g2);
     gnorm_history[it - 1] = gnorm;
     hipLaunchKernelGGL(pr::k_lbfgs_direction, dim3(1), dim3(pr::LBFGS_THREADS), 0, st, d_g, d_S, d_Y, k, first, R,
-                       n, d_d, d_sc);
+                       n, 1e-14, d_d, d_sc);
     HIPCHK(hipGetLastError());
     double dmax = 0;
     if ((rc = read(d_sc, &dmax, 1))) return rc;
@@ -1870,6 +1870,192 @@ int pr_lbfgs_optimize(void* plan, const double* controls_init, const pr_lbfgs_co
   result->n_gradient_evaluations = n_g;
   result->fidelity = best_F;
   result->step_size_final = a_last;
+  return PR_OK;
+}
+
+int pr_mr_optimcon(void* plan, const double* controls_init, const pr_mr_config* cfg, double* controls_best,
+                   double* fidelity_history, double* gnorm_history, pr_grape_result* result) {
+  if (fault_is("eval")) return fail(PR_ERR_INJECTED, "failure injected by PULSAR_ROCM_FAULT=eval");
+  if (!plan || !controls_init || !cfg || !controls_best || !fidelity_history || !gnorm_history || !result)
+    return fail(PR_ERR_ARG, "pr_mr_optimcon: NULL argument");
+  Plan& pl = *static_cast<Plan*>(plan);
+  if (pl.n_probs != 1) return fail(PR_ERR_ARG, "pr_mr_optimcon: the plan must hold one problem");
+  const int f = cfg->tile;
+  if ((cfg->method != PR_MR_LBFGS && cfg->method != PR_MR_GRAPE) || cfg->max_iter < 0 || f < 1 ||
+      cfg->lbfgs_memory < 1 || cfg->lbfgs_memory > 64 || !(cfg->u_bound >= cfg->l_bound))
+    return fail(PR_ERR_ARG, "pr_mr_optimcon: invalid configuration");
+  const pr::ProbDev& D0 = pl.host_probs[0];
+  if (D0.nc % f != 0) return fail(PR_ERR_ARG, "pr_mr_optimcon: n_ctrl %d is not a multiple of tile %d", D0.nc, f);
+  if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
+  const int nc = D0.nc / f;
+  const long long n = (long long)nc * D0.nt;  // waveform entries
+  const int mem = cfg->lbfgs_memory, R = mem + 1;
+  const double lo = cfg->l_bound, hi = cfg->u_bound, a0 = 0.05 * (hi - lo);
+  hipStream_t st = pl.stream;
+  // device vectors: w, w_new, G, G_new, d, best, (S, Y) rows, scalars
+  DevBuf vecs, hist, scal;
+  HIPCHK(hipMalloc(&vecs.p, sizeof(double) * (size_t)n * 6));
+  if (cfg->method == PR_MR_LBFGS) HIPCHK(hipMalloc(&hist.p, sizeof(double) * (size_t)n * 2 * R));
+  HIPCHK(hipMalloc(&scal.p, sizeof(double) * (pr::NORM_BLOCKS + 4)));
+  double* d_w = static_cast<double*>(vecs.p);
+  double* d_wn = d_w + n;
+  double* d_g = d_wn + n;
+  double* d_gn = d_g + n;
+  double* d_d = d_gn + n;
+  double* d_best = d_d + n;
+  double* d_S = static_cast<double*>(hist.p);
+  double* d_Y = d_S ? d_S + (size_t)n * R : nullptr;
+  double* d_part = static_cast<double*>(scal.p);
+  double* d_sc = d_part + pr::NORM_BLOCKS;
+  const dim3 eb((unsigned)((n + 255) / 256)), et(256);
+  int rc;
+  auto read = [&](const double* src, double* dst, int cnt) -> int {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PR_OK;
+  };
+  // F (and G into gout when it is not NULL) at the waveform x
+  auto eval = [&](const double* x, double* gout, double* F) -> int {
+    const double* cx = x;
+    if (f > 1) {
+      hipLaunchKernelGGL(pr::k_mr_tile, eb, et, 0, st, x, n, nc, f, pl.d_controls);
+      HIPCHK(hipGetLastError());
+      cx = pl.d_controls;
+    }
+    int r = run_graph(pl, cx, pl.d_F, gout ? (f > 1 ? pl.d_G : gout) : nullptr, st);
+    if (r) return r;
+    if (gout && f > 1) {
+      hipLaunchKernelGGL(pr::k_mr_untile, eb, et, 0, st, pl.d_G, n, nc, f, gout);
+      HIPCHK(hipGetLastError());
+    }
+    return read(pl.d_F, F, 1);
+  };
+  auto gnorm = [&](const double* g, double* out) -> int {
+    hipLaunchKernelGGL(pr::k_norm2_part, dim3(pr::NORM_BLOCKS), dim3(256), 0, st, g, n, d_part);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(pr::k_norm2_final, dim3(1), dim3(64), 0, st, d_part, d_sc);
+    HIPCHK(hipGetLastError());
+    double g2 = 0;
+    int r = read(d_sc, &g2, 1);
+    *out = std::sqrt(g2);
+    return r;
+  };
+  auto copy = [&](double* dst, const double* src) -> int {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    return PR_OK;
+  };
+  // the guess, clipped
+  HIPCHK(hipMemcpyAsync(d_wn, controls_init, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(pr::k_mr_trial, eb, et, 0, st, d_wn, d_wn, 0.0, lo, hi, n, d_w);
+  HIPCHK(hipGetLastError());
+  std::vector<double> fh, gh;
+  double F = 0, gn = 0, best_F;
+  if ((rc = eval(d_w, d_g, &F)) || (rc = gnorm(d_g, &gn))) return rc;
+  int n_f = 1, n_g = 1, converged = 0, reason = PR_STOP_MAX_ITER, k = 0, first = 0;
+  fh.push_back(F);
+  gh.push_back(gn);
+  best_F = F;
+  if ((rc = copy(d_best, d_w))) return rc;
+  for (int it = 0; it < cfg->max_iter; it++) {
+    double Fn = 0;
+    if (cfg->method == PR_MR_LBFGS) {
+      hipLaunchKernelGGL(pr::k_lbfgs_direction, dim3(1), dim3(pr::LBFGS_THREADS), 0, st, d_g, d_S, d_Y, k, first,
+                         R, n, 0.0, d_d, d_sc);
+      HIPCHK(hipGetLastError());
+      double dmax = 0;
+      if ((rc = read(d_sc, &dmax, 1))) return rc;
+      if (dmax < 1e-14) {
+        converged = 1;
+        reason = PR_STOP_ZERO_DIRECTION;
+        break;
+      }
+      hipLaunchKernelGGL(pr::k_lbfgs_scale, eb, et, 0, st, d_d, d_sc, n);
+      HIPCHK(hipGetLastError());
+      // Armijo backtracking (F only); w_new ends in d_wn
+      double a = a0, Ft = 0;
+      bool ok = false;
+      for (int h = 1; h <= 20; h++) {
+        hipLaunchKernelGGL(pr::k_mr_trial, eb, et, 0, st, d_w, d_d, a, lo, hi, n, d_wn);
+        HIPCHK(hipGetLastError());
+        if ((rc = eval(d_wn, nullptr, &Ft))) return rc;
+        if (Ft >= F - 1e-4 * std::fabs(a)) {
+          ok = true;
+          n_f += h + 1;
+          break;
+        }
+        a *= 0.5;
+      }
+      if (!ok) {
+        hipLaunchKernelGGL(pr::k_mr_trial, eb, et, 0, st, d_w, d_d, a, lo, hi, n, d_wn);
+        HIPCHK(hipGetLastError());
+        if ((rc = eval(d_wn, nullptr, &Ft))) return rc;
+        if (!(Ft > F) && (rc = copy(d_wn, d_w))) return rc;
+        n_f += 21;
+      }
+      if ((rc = eval(d_wn, d_gn, &Fn))) return rc;
+      n_f++;
+      n_g++;
+      const int slot = (first + k) % R;
+      hipLaunchKernelGGL(pr::k_mr_pair, dim3(1), dim3(pr::LBFGS_THREADS), 0, st, d_w, d_wn, d_gn, d_g, n,
+                         d_S + (size_t)slot * n, d_Y + (size_t)slot * n, d_sc + 1);
+      HIPCHK(hipGetLastError());
+      double ysss[2];
+      if ((rc = read(d_sc + 1, ysss, 2))) return rc;
+      if (ysss[0] > 1e-14 * ysss[1]) {
+        if (k < mem)
+          k++;
+        else
+          first = (first + 1) % R;
+      }
+      if ((rc = copy(d_w, d_wn)) || (rc = copy(d_g, d_gn))) return rc;
+    } else {
+      hipLaunchKernelGGL(pr::k_absmax, dim3(1), dim3(pr::LBFGS_THREADS), 0, st, d_g, n, d_sc);
+      HIPCHK(hipGetLastError());
+      double gmax = 0;
+      if ((rc = read(d_sc, &gmax, 1))) return rc;
+      if (gmax > 1e-14) {
+        hipLaunchKernelGGL(pr::k_mr_trial, eb, et, 0, st, d_w, d_g, a0 / gmax, lo, hi, n, d_wn);
+        HIPCHK(hipGetLastError());
+        if ((rc = copy(d_w, d_wn))) return rc;
+      }
+      if ((rc = eval(d_w, d_g, &Fn))) return rc;
+      n_f++;
+      n_g++;
+    }
+    const double F_prev = F;
+    F = Fn;
+    if ((rc = gnorm(d_g, &gn))) return rc;
+    fh.push_back(F);
+    gh.push_back(gn);
+    if (F > best_F) {
+      best_F = F;
+      if ((rc = copy(d_best, d_w))) return rc;
+    }
+    int why = -1;
+    if (F >= 1.0 - 1e-8)
+      why = PR_STOP_PERFECT;
+    else if (gn < cfg->grad_tol)
+      why = PR_STOP_GRADIENT;
+    else if (cfg->method == PR_MR_LBFGS && std::fabs(F - F_prev) < 1e-9)
+      why = PR_STOP_FIDELITY;
+    if (why >= 0) {
+      converged = 1;
+      reason = why;
+      break;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(controls_best, d_best, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::copy(fh.begin(), fh.end(), fidelity_history);
+  std::copy(gh.begin(), gh.end(), gnorm_history);
+  result->n_iterations = (int)fh.size() - 1;
+  result->converged = converged;
+  result->reason = reason;
+  result->n_fidelity_history = (int)fh.size();
+  result->n_fidelity_evaluations = n_f;
+  result->n_gradient_evaluations = n_g;
+  result->fidelity = best_F;
+  result->step_size_final = 0;
   return PR_OK;
 }
 

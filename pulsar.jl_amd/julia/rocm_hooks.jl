@@ -392,3 +392,57 @@ function _rocm_grape_optimize_ensemble(systems::Vector{<:AbstractQuantumSystem},
                                                 "n_systems" => length(systems), "lbfgs_memory" => lbfgs_memory))
 end
 
+
+# optimcon(ctrl::MRControl / LindbladMRControl, guess) (src/Application/MR/OptControl.jl:310-367)
+# gains, before its dispatch on ctrl.method,
+#   ctrl.backend == :rocm && _rocm_optimcon_applies(ctrl) && return _rocm_optimcon(ctrl, guess)
+# The :lbfgs and :grape loops (_optimcon_lbfgs :380-484, _optimcon_grape :488-561) then
+# run in the library (pr_mr_optimcon): waveform, gradient and curvature pairs stay in
+# device memory and the line search evaluates F only. Penalties, non-default
+# parameterizations, tracking and verbose printing keep the Julia loops (over the
+# :rocm kernels above).
+struct PRMrConfig
+    method::Cint
+    max_iter::Cint
+    grad_tol::Cdouble
+    lbfgs_memory::Cint
+    l_bound::Cdouble
+    u_bound::Cdouble
+    tile::Cint
+end
+const PR_MR_LBFGS = Cint(0)
+const PR_MR_GRAPE = Cint(1)
+
+_rocm_optimcon_applies(ctrl) =
+    ctrl.method in (:lbfgs, :grape) && all(==(:none), ctrl.penalties) && !ctrl.verbose &&
+    ctrl.parameterization isa PiecewiseConstant && (!hasproperty(ctrl, :tracking) || isempty(ctrl.tracking))
+
+function _rocm_optimcon(ctrl, guess::Matrix{Float64})::OptimizationResult
+    t0 = time()
+    lindblad = hasproperty(ctrl, :_L_drifts)
+    plan = lindblad ?
+        _rocm_mr_plan(ctrl, ctrl._L_drifts, ctrl._L_controls, ctrl._sigma_init, ctrl._sigma_targ, true) :
+        _rocm_mr_plan(ctrl, ctrl.drifts, ctrl.operators, ctrl.rho_init, ctrl.rho_targ, false)
+    cfg = PRMrConfig(ctrl.method === :lbfgs ? PR_MR_LBFGS : PR_MR_GRAPE, ctrl.max_iter, ctrl.grad_tol,
+                     ctrl.lbfgs_memory, ctrl.l_bound, ctrl.u_bound, 1)
+    best = similar(guess)
+    fh = zeros(ctrl.max_iter + 1)
+    gh = zeros(ctrl.max_iter + 1)
+    res = PRGrapeResult()
+    GC.@preserve guess best fh gh begin
+        _rocm_check(ccall((:pr_mr_optimcon, _ROCM_LIB[]), Cint,
+                          (Ptr{Cvoid}, Ptr{Float64}, Ref{PRMrConfig}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                           Ref{PRGrapeResult}), plan.handle, guess, cfg, best, fh, gh, res),
+                    "pr_mr_optimcon")
+    end
+    n = Int(res.n_fidelity_history)
+    reason = res.reason == 2 ? "gradient norm < $(ctrl.grad_tol)" :
+             ("max iterations reached", "near-perfect fidelity", "", "fidelity stalled",
+              "zero gradient direction")[res.reason + 1]
+    algo = (lindblad ? "Lindblad " : "MR ") * (ctrl.method === :lbfgs ? "L-BFGS-GRAPE" : "GRAPE")
+    meta = Dict{String, Any}("algorithm" => algo, "on_device" => true)
+    ctrl.method === :lbfgs && (meta["lbfgs_memory"] = ctrl.lbfgs_memory)
+    return OptimizationResult(best, res.fidelity, fh[1:n], gh[1:n], Int(res.n_iterations), res.converged != 0,
+                              reason, time() - t0, Int(res.n_fidelity_evaluations),
+                              Int(res.n_gradient_evaluations), meta)
+end

@@ -507,11 +507,18 @@ def _validate(ctrl, guess):
         raise ValueError(f"guess has {n_t} columns but ctrl.pulse_dt has {len(ctrl.pulse_dt)} entries")
 
 
-def optimcon(ctrl, guess):
-    """optimcon(ctrl, guess) for :lbfgs and :grape (OptControl.jl:310-367) with
-    every kernel evaluation on the device through one reusable plan (the
-    line search evaluates F only). Non-empty ctrl.tracking selects the
-    tracking kernel, as there."""
+_STOP_REASON = {0: "max iterations reached", 1: "near-perfect fidelity", 3: "fidelity stalled",
+                4: "zero gradient direction"}
+
+
+def optimcon(ctrl, guess, on_device: bool = True):
+    """optimcon(ctrl, guess) for :lbfgs and :grape (OptControl.jl:310-367).
+    Non-empty ctrl.tracking selects the tracking kernel, as there.
+
+    on_device=True: the whole loop runs in the library (pr_mr_optimcon), with
+    the waveform, gradient and curvature pairs in device memory; the line
+    search evaluates F only. on_device=False: the same loop on the host
+    (optimcon_lbfgs_loop / optimcon_grape_loop) over the same device plan."""
     _validate(ctrl, guess)
     method = str(ctrl.method).lstrip(":")
     if method not in ("lbfgs", "grape"):
@@ -519,13 +526,35 @@ def optimcon(ctrl, guess):
     tracking = bool(getattr(ctrl, "tracking", []))
     kp = KernelPlan(ctrl, tracking=tracking)
     try:
+        n_ens = kp.problem.n_members * kp.problem.n_pairs
+        if on_device:
+            from .api import OptimizationResult
+            import time
+            cfg = native.MrConfig(native.PR_MR_LBFGS if method == "lbfgs" else native.PR_MR_GRAPE,
+                                  int(ctrl.max_iter), float(ctrl.grad_tol), int(ctrl.lbfgs_memory),
+                                  float(ctrl.l_bound), float(ctrl.u_bound), kp.packed.f if kp.packed else 1)
+            t0 = time.perf_counter()
+            best, fh, gh, res = native.mr_optimcon_device(kp.plan, guess, cfg)
+            reason = (f"gradient norm < {ctrl.grad_tol}" if res.reason == 2 else _STOP_REASON[res.reason])
+            algo = ("MR Tracking " if tracking else
+                    ("Lindblad " if isinstance(ctrl, LindbladMRControl) else "MR ")) + \
+                ("L-BFGS-GRAPE" if method == "lbfgs" else "GRAPE")
+            meta = {"algorithm": algo, "on_device": True}
+            if tracking:
+                meta["n_checkpoints"] = len(ctrl.tracking)
+            else:
+                meta["n_ensemble"] = n_ens
+            if method == "lbfgs":
+                meta["lbfgs_memory"] = ctrl.lbfgs_memory
+            return OptimizationResult(best, res.fidelity, fh, gh, res.n_iterations, bool(res.converged), reason,
+                                      time.perf_counter() - t0, res.n_fidelity_evaluations,
+                                      res.n_gradient_evaluations, meta)
         def fg(w):
             return kp.eval(w, grad=True)
 
         def f_only(w):
             return kp.eval(w, grad=False)[0]
 
-        n_ens = kp.problem.n_members * kp.problem.n_pairs
         if method == "lbfgs":
             meta = ({"algorithm": "MR Tracking L-BFGS-GRAPE", "n_checkpoints": len(ctrl.tracking)} if tracking else
                     {"algorithm": ("Lindblad" if isinstance(ctrl, LindbladMRControl) else "MR") + " L-BFGS-GRAPE",

@@ -29,7 +29,7 @@ PR_ERR = {
 EXPORTS = ("pr_abi_version", "pr_device_count", "pr_init", "pr_last_error", "pr_plan_create",
            "pr_eval", "pr_eval_device", "pr_eval_device_static", "pr_plan_n_params", "pr_plan_exp_counts", "pr_plan_set_timing",
            "pr_plan_kernel_times", "pr_plan_buffers", "pr_plan_upload_controls",
-           "pr_plan_synchronize", "pr_plan_destroy", "pr_grape_optimize", "pr_lbfgs_optimize")
+           "pr_plan_synchronize", "pr_plan_destroy", "pr_grape_optimize", "pr_lbfgs_optimize", "pr_mr_optimcon")
 KERNEL_CLASSES = ("segment", "combine", "gradient", "reduce")
 
 
@@ -56,6 +56,15 @@ class LbfgsConfig(ctypes.Structure):
                 ("amplitude_limit", ctypes.c_double)]
 
 
+class MrConfig(ctypes.Structure):
+    """pr_mr_config (the MR optimcon loops' knobs, OptControl.jl:380-561)."""
+
+    _fields_ = [("method", ctypes.c_int), ("max_iter", ctypes.c_int), ("grad_tol", ctypes.c_double),
+                ("lbfgs_memory", ctypes.c_int), ("l_bound", ctypes.c_double), ("u_bound", ctypes.c_double),
+                ("tile", ctypes.c_int)]
+
+
+PR_MR_LBFGS, PR_MR_GRAPE = 0, 1
 PR_STOP = {0: "max_iter", 1: "perfect", 2: "gradient", 3: "fidelity", 4: "zero_direction"}
 PR_STOP_CODES = {v: k for k, v in PR_STOP.items()}
 
@@ -101,6 +110,8 @@ def load(path: str = LIB_PATH):
                                     ctypes.POINTER(GrapeResult)]
     L.pr_lbfgs_optimize.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(LbfgsConfig), dp, dp, dp,
                                     ctypes.POINTER(GrapeResult)]
+    L.pr_mr_optimcon.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(MrConfig), dp, dp, dp,
+                                 ctypes.POINTER(GrapeResult)]
     _lib = L
     return L
 
@@ -269,3 +280,21 @@ def lbfgs_optimize_device(plan: Plan, controls: np.ndarray, cfg: LbfgsConfig):
     check(load().pr_lbfgs_optimize(plan._h, _dptr(u0), ctypes.byref(cfg), _dptr(best), _dptr(fh), _dptr(gh),
                                    ctypes.byref(res)))
     return (grad_from_cm(best, n_c, n_t), fh[:res.n_fidelity_history].tolist(), gh[:res.n_iterations].tolist(), res)
+
+
+def mr_optimcon_device(plan: Plan, guess: np.ndarray, cfg: MrConfig):
+    """pr_mr_optimcon: the MR optimcon loop (:lbfgs or :grape) with the
+    waveform, gradient and curvature pairs resident on the device. guess
+    [n_ctrl, n_t] (cfg.tile copies of its rows make the plan's controls);
+    returns (best waveform, fidelity history, gradient-norm history, GrapeResult)."""
+    u = np.asarray(guess, dtype=np.float64)
+    n_c, n_t = u.shape
+    u0 = controls_cm(u)
+    best = np.zeros_like(u0)
+    fh = np.zeros(cfg.max_iter + 1)
+    gh = np.zeros(cfg.max_iter + 1)
+    res = GrapeResult()
+    check(load().pr_mr_optimcon(plan._h, _dptr(u0), ctypes.byref(cfg), _dptr(best), _dptr(fh), _dptr(gh),
+                                ctypes.byref(res)))
+    n = res.n_fidelity_history
+    return grad_from_cm(best, n_c, n_t), fh[:n].tolist(), gh[:n].tolist(), res

@@ -128,12 +128,35 @@ def test_problem_struct_mirrors_match_the_header():
     assert c_fields == py_fields == jl_fields, (c_fields, py_fields, jl_fields)
 
 
+@pytest.mark.parametrize("c_name,py_name,jl_name,jl_file", [
+    ("pr_options", "COptions", "PROptions", "ROCmBackend.jl"),
+    ("pr_grape_config", "GrapeConfig", "PRGrapeConfig", "rocm_hooks.jl"),
+    ("pr_lbfgs_config", "LbfgsConfig", "PRLbfgsConfig", "rocm_hooks.jl"),
+    ("pr_mr_config", "MrConfig", "PRMrConfig", "rocm_hooks.jl"),
+])
+def test_config_struct_mirrors_match_the_header(c_name, py_name, jl_name, jl_file):
+    # field order of the option and loop-configuration structs in the header,
+    # the ctypes mirrors and the Julia mirrors
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct \{([^{}]*?)\} " + c_name + ";", text, flags=re.S).group(1)
+    c_fields = re.findall(r"(\w+)\s*;", body)
+    import pulsar_rocm.native as nat
+    import pulsar_rocm.problem as prob
+    py = getattr(nat, py_name, None) or getattr(prob, py_name)
+    py_fields = [f[0] for f in py._fields_]
+    jl = open(os.path.join(ROOT, "pulsar.jl_amd", "julia", jl_file)).read()
+    jl_body = re.search(r"struct " + jl_name + r"\n(.*?)\nend", jl, flags=re.S).group(1)
+    jl_fields = re.findall(r"^\s*(\w+)::", jl_body, flags=re.M)
+    assert c_fields == py_fields == jl_fields, (c_fields, py_fields, jl_fields)
+
+
 def test_julia_constants_match_the_header():
     # every PR_* constant the Julia glue defines has the header's value
     defs = dict((k, int(v)) for k, v in re.findall(r"#define (PR_[A-Z0-9_]+) \(?(-?\d+)\)?", open(HEADER).read()))
-    jl = open(os.path.join(ROOT, "pulsar.jl_amd", "julia", "ROCmBackend.jl")).read()
+    jl = "".join(open(os.path.join(ROOT, "pulsar.jl_amd", "julia", f)).read()
+                 for f in ("ROCmBackend.jl", "rocm_hooks.jl"))
     consts = re.findall(r"^const (PR_[A-Z0-9_]+) = Cint\((-?\d+)\)", jl, flags=re.M)
-    assert len(consts) >= 12
+    assert len(consts) >= 14
     for k, v in consts:
         assert defs[k] == int(v), k
 

@@ -166,7 +166,8 @@ def test_mr_optimcon_on_device_matches_oracle_driven_loop(kind):
         fg = lambda w: O.evaluate(p, w)  # noqa: E731
         f = lambda w: O.evaluate(p, w, grad=False)[0]  # noqa: E731
     guess = 0.2 * np.random.default_rng(4).standard_normal((2, 40))
-    dev = mr.optimcon(ctrl, guess)
+    dev = mr.optimcon(ctrl, guess)  # the loop in the library (pr_mr_optimcon)
+    assert dev.metadata["on_device"] is True
     if kind == "state_grape":
         ref = mr.optimcon_grape_loop(fg, guess, ctrl.l_bound, ctrl.u_bound, ctrl.max_iter, ctrl.grad_tol)
     else:
@@ -176,6 +177,56 @@ def test_mr_optimcon_on_device_matches_oracle_driven_loop(kind):
     assert dev.n_fidelity_evaluations == ref.n_fidelity_evaluations
     assert np.abs(np.array(dev.fidelity_history) - np.array(ref.fidelity_history)).max() <= 1e-10
     assert np.abs(dev.controls - ref.controls).max() <= 1e-7
+
+
+@pytest.mark.parametrize("method,pack,bound", [("lbfgs", True, 1.0), ("lbfgs", False, 0.35), ("grape", True, 0.35),
+                                               ("grape", False, 1.0)])
+def test_mr_optimcon_library_loop_matches_host_loop(method, pack, bound):
+    # pr_mr_optimcon against the host loop over the same device plan: packed
+    # (4 members per block, the waveform tiled and the gradient summed on the
+    # device) and unpacked, with the bounds clipping (0.35) or not
+    Ix, Iy, Iz = W.SX / 2, W.SY / 2, W.SZ / 2
+    up = np.array([1, 0], complex)
+    mIy = np.array([1, -1j], complex) / np.sqrt(2)
+    ctrl = mr.MRControl(drifts=[2 * np.pi * 300 * Iz, -2 * np.pi * 200 * Iz], operators=[Ix, Iy], rho_init=[up],
+                        rho_targ=[mIy], pulse_dt=np.full(40, 5e-6), pwr_levels=[2 * np.pi * 8e3, 2 * np.pi * 1e4],
+                        max_iter=25, method=method, l_bound=-bound, u_bound=bound, lbfgs_memory=4)
+    guess = 0.3 * np.random.default_rng(6).standard_normal((2, 40))
+    print(f"case method={method} pack={pack} bound={bound}", flush=True)
+    kp = mr.KernelPlan(ctrl)
+    assert (kp.packed is not None) and kp.packed.f == 4
+    kp.close()
+    if pack:
+        dev = mr.optimcon(ctrl, guess)
+        host = mr.optimcon(ctrl, guess, on_device=False)
+    else:
+        # the unpacked plan: one member per block, tile 1
+        kpu = mr.KernelPlan(ctrl, pack=False)
+        try:
+            cfg = native.MrConfig(native.PR_MR_LBFGS if method == "lbfgs" else native.PR_MR_GRAPE, ctrl.max_iter,
+                                  ctrl.grad_tol, ctrl.lbfgs_memory, ctrl.l_bound, ctrl.u_bound, 1)
+            best, fh, gh, res = native.mr_optimcon_device(kpu.plan, guess, cfg)
+            fg = lambda w: kpu.eval(w)  # noqa: E731
+            f = lambda w: kpu.eval(w, grad=False)[0]  # noqa: E731
+            if method == "lbfgs":
+                host = mr.optimcon_lbfgs_loop(fg, f, guess, ctrl.l_bound, ctrl.u_bound, ctrl.max_iter,
+                                              ctrl.grad_tol, ctrl.lbfgs_memory)
+            else:
+                host = mr.optimcon_grape_loop(fg, guess, ctrl.l_bound, ctrl.u_bound, ctrl.max_iter, ctrl.grad_tol)
+        finally:
+            kpu.close()
+        dev = type(host)(best, res.fidelity, fh, gh, res.n_iterations, bool(res.converged),
+                         host.termination_reason, 0.0, res.n_fidelity_evaluations, res.n_gradient_evaluations, {})
+        assert native.PR_STOP[res.reason] in ("max_iter", "perfect", "gradient", "fidelity", "zero_direction")
+    assert dev.n_iterations == host.n_iterations and dev.termination_reason == host.termination_reason
+    assert dev.n_fidelity_evaluations == host.n_fidelity_evaluations
+    assert dev.n_gradient_evaluations == host.n_gradient_evaluations
+    assert np.abs(np.array(dev.fidelity_history) - np.array(host.fidelity_history)).max() <= 1e-12
+    assert np.allclose(dev.gradient_norm_history, host.gradient_norm_history, rtol=1e-9, atol=0)
+    assert np.abs(dev.controls - host.controls).max() <= 1e-9
+    assert abs(dev.fidelity - host.fidelity) <= 1e-12
+    assert np.all(dev.controls <= bound) and np.all(dev.controls >= -bound)
+    assert dev.fidelity > dev.fidelity_history[0]
 
 
 @pytest.mark.parametrize("nd,pw,pairs", [(16, [0.9, 1.0, 1.1, 1.2], 1), (5, [0.9, 1.1], 2)])
