@@ -446,3 +446,17 @@ function _rocm_optimcon(ctrl, guess::Matrix{Float64})::OptimizationResult
                               reason, time() - t0, Int(res.n_fidelity_evaluations),
                               Int(res.n_gradient_evaluations), meta)
 end
+
+# robust_optimcon_qs (src/Application/QuantumComputing/NoiseModels/QuasiStatic.jl:196-216)
+# builds its quasi-static drift ensemble and calls grape_optimize_ensemble, whose
+# :rocm branch (_rocm_grape_optimize_ensemble above) makes the whole ensemble one
+# plan with the L-BFGS loop in the library: no edit of QuasiStatic.jl is needed.
+#
+# LindbladQCControl (src/Application/QuantumComputing/LindbladQCControl.jl:237-284)
+# reaches grape_lindblad_kernel through _LindbladQCAdapter, whose getproperty answers
+# :backend with :cpu. The one-line edit
+#     name === :backend && return get_device() === :rocm ? :rocm : :cpu
+# sends _qc_kernel to _rocm_grape_lindblad_kernel: the adapter's _L_drifts,
+# _L_controls, _sigma_init, _sigma_targ, pwr_levels = [1.0], pulse_dt and :square
+# fidelity build one PR_LINDBLAD problem (_rocm_mr_plan), and the adapter's own
+# _apply_penalties! subtracts the QC penalties afterwards.

@@ -4,7 +4,7 @@ The compute path is libpulsar_rocm.so (hand-written HIP, C-ABI in
 include/pulsar_rocm.h), loaded through `native`. `api` and `mr` mirror the
 reference's Julia interface for this path.
 """
-from . import native, problem, workloads  # noqa: F401
+from . import native, problem, qc, workloads  # noqa: F401
 from .api import (BatchedEnsemble, ControlSequence, EnsembleObjective, GRAPEConfig,  # noqa: F401
                   OptimizationResult, QuantumSystem, QuantumTarget, adapt_grape_step_size,
                   available_devices, build_ensemble_from_systems, check_grape_convergence,

@@ -302,3 +302,70 @@ def test_fidelity_forward_single_and_population(ftype):
         assert abs(f - Fo) <= 1e-12
     with pytest.raises(ValueError):
         P.fidelity_forward(wfs[0], drifts, pwr, ops, rho_i, rho_t, dts, backend="cuda")
+
+
+# ----------------------------------------------------------------- quantum-computing callers
+
+
+def test_robust_optimcon_qs_is_one_device_ensemble():
+    # robust_optimcon_qs (QuasiStatic.jl:196-216): frequency and amplitude
+    # noise (5 x 3 members) as one plan with the L-BFGS loop in the library;
+    # the same as grape_optimize_ensemble over the systems built by hand and
+    # as the Python loop over the same evaluations
+    from pulsar_rocm import qc
+    Iz, Ix, Iy = 0.5 * SZ, 0.5 * SX, 0.5 * SY
+    srcs = [qc.QuasiStaticNoise("freq", 2e3, n_samples=5), qc.QuasiStaticNoise("amp", 0.05, distribution="uniform",
+                                                                              n_samples=3)]
+    H_fn = lambda p: 2 * np.pi * p[0] * Iz + p[1] * 2 * np.pi * 1e3 * Ix  # noqa: E731
+    tgt = P.unitary_target(np.array([[0, -1j], [-1j, 0]]))
+    sys0 = P.quantum_system(H_fn([0.0, 0.0]), [Ix, Iy])
+    ctrl = P.random_controls(sys0, 300e-6, 30, amplitude=2 * np.pi * 800, rng=np.random.default_rng(21))
+    cfg = P.GRAPEConfig(max_iter=10, step_size=0.05, convergence_tol=1e-12)
+    pwr = [2 * np.pi, 2 * np.pi]
+    res = qc.robust_optimcon_qs(H_fn, [Ix, Iy], srcs, tgt, ctrl, config=cfg, pwr_levels=pwr)
+    assert res.metadata["on_device"] is True and res.metadata["n_systems"] == 15
+    drifts = qc.quasi_static_ensemble(H_fn, srcs)
+    systems = [P.quantum_system(H, [pwr[0] * Ix, pwr[1] * Iy]) for H in drifts]
+    ref = P.grape_optimize_ensemble(systems, tgt, ctrl, config=cfg)
+    host = P.grape_optimize_ensemble(systems, tgt, ctrl, config=cfg, on_device=False)
+    assert res.fidelity_history == ref.fidelity_history and np.array_equal(res.controls, ref.controls)
+    assert np.abs(np.array(res.fidelity_history) - np.array(host.fidelity_history)).max() <= 1e-10
+    assert res.fidelity > res.fidelity_history[0]
+
+
+def test_lindblad_qc_control_kernel_and_grape_loop():
+    # LindbladQCControl (LindbladQCControl.jl:69-406): its kernel is one
+    # PR_LINDBLAD problem on the device, equal to the oracle; the :grape loop
+    # over it equals the loop driven by the oracle
+    from pulsar_rocm import qc
+    sysx = P.quantum_system(2 * np.pi * 0.3 * SZ / 2, [2 * np.pi * SX / 2, 2 * np.pi * SY / 2])
+    tgt = P.state_target(np.array([0, 1], complex))
+    ctrl = P.random_controls(sysx, 2.0, 25, amplitude=0.4, rng=np.random.default_rng(3))
+    sm = np.array([[0, 1], [0, 0]], complex)
+    pen = (lambda w: 1e-3 * float(np.sum(w ** 2)), lambda w: 2e-3 * w)
+    mk = lambda: qc.LindbladQCControl(sysx, tgt, ctrl, jump_ops=[sm], decay_rates=[0.05], method="grape",  # noqa
+                                      max_iter=30, step_size=0.5, penalty_fns=[pen[0]], penalty_grad_fns=[pen[1]])
+    ctx = mk()
+    F, G = qc.qc_kernel(ctrl.controls, ctx)
+    F0, G0 = O.evaluate(ctx.problem(), ctrl.controls)
+    F0 -= pen[0](ctrl.controls)
+    G0 = G0 - pen[1](ctrl.controls)
+    ctx.close()
+    assert abs(F - F0) <= 1e-12 and np.abs(G - G0).max() <= 1e-10 * np.abs(G0).max()
+    res = qc.optimcon(mk())
+    # the same loop with oracle evaluations
+    ref_ctx = mk()
+    w = np.array(ctrl.controls)
+    fh, Fp = [], -np.inf
+    for it in range(1, 31):
+        Fo, Go = O.evaluate(ref_ctx.problem(), w)
+        Fo -= pen[0](w)
+        Go = Go - pen[1](w)
+        fh.append(Fo)
+        if np.linalg.norm(Go) < ref_ctx.grape_config.gradient_norm_tol or (abs(Fo - Fp) < 1e-8 and it > 1):
+            break
+        Fp = Fo
+        w = w + 0.5 * Go
+    assert len(res.fidelity_history) == len(fh)
+    assert np.abs(np.array(res.fidelity_history) - np.array(fh)).max() <= 1e-10
+    assert res.fidelity >= res.fidelity_history[0] and res.metadata["liouville_dim"] == 4
