@@ -40,7 +40,7 @@ WORK_MODEL = {
     "fp64": "complex products on FP64 matrix cores, 6 d^3 flops each (three real products)",
     "mixed_fp32": "exp and walk products on FP32 matrix cores and the chain product on FP64, 6 d^3 flops each",
     "split16": "FP16x2 split products on FP16 matrix cores, 8 d^3 flops per part-product (13 per slice in "
-               "k_segment, 6 in k_gradient), no FP64 chain product",
+               "k_segment, 6 in k_gradient), no FP64 chain product; 2 FP64 products per work item (projection)",
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -74,7 +74,9 @@ def mfma_work(kernel, form, counts, units, d, items):
     (three each), the segment product's E_k F (three, every slice but the
     first of a work item) and the walk's two products (three each), as
     16x16x32 FP16 matrix products of 8 d^3 flops per part-product (the real 2x2
-    block form); no FP64 chain product (segments of at most 128 slices).
+    block form); no FP64 chain product (segments of at most 128 slices), and
+    two FP64 products per work item for the unitary projection of its segment
+    product.
     Member-slices redone in FP64 count their FP64 exp and walk products."""
     n6, n9, nsq, nredo = (int(c) for c in counts)
     g = 6 * d ** 3
@@ -89,8 +91,9 @@ def mfma_work(kernel, form, counts, units, d, items):
             return {"fp32": g * 3 * low, "fp64": g * (redo_exp + units)}
         return {"fp32": g * 2 * low, "fp64": g * 2 * nredo}
     if kernel == "k_segment":
+        # (+ the unitary projection of each work item's segment product: 2 FP64 products)
         return {"fp16": 8 * d ** 3 * ((4 + 3 + 3) * low + 3 * max(low - items, 0)),
-                "fp64": g * (redo_exp + nredo)}
+                "fp64": g * (redo_exp + nredo + 2 * items)}
     return {"fp16": 8 * d ** 3 * 3 * 2 * low, "fp64": g * 2 * nredo}
 
 
