@@ -134,6 +134,38 @@ def cpu_baseline(p, u, n_sample, threads):
     return (n_sample / p.n_members) / dt, dt, F, G, q
 
 
+def cpu_forms(p, u, threads_arg, threads):
+    """The oracle's other timed forms: one fused F and G pass on all threads
+    (the whole ensemble), and the reference-faithful separate passes on one
+    thread (8 members); rates scaled to the full ensemble, with the host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    out = {}
+    t0 = time.perf_counter()
+    oracle.evaluate(p, u, separate_fg=False, n_threads=threads_arg)
+    dt = time.perf_counter() - t0
+    out["fused_fg_all_threads"] = {"value": round(1.0 / dt, 5), "unit": "evals/s", "cores": threads,
+                                   "sample": f"all {p.n_members} members, {dt:.2f} s"}
+    n = 8
+    q, uu = workloads.shrink(p, u, n_members=n)
+    t0 = time.perf_counter()
+    oracle.evaluate(q, uu, separate_fg=True, n_threads=1)
+    dt = time.perf_counter() - t0
+    out["separate_fg_one_thread"] = {"value": round((n / p.n_members) / dt, 6), "unit": "evals/s", "cores": 1,
+                                     "sample": f"{n} of {p.n_members} members, {dt:.2f} s, scaled"}
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    out["host"] = {"cpu_model": model, "nproc": os.cpu_count(), "threads_used": threads}
+    return out
+
+
 def calibrate_copy(mib):
     """A plain device-to-device copy of known size through the HIP runtime, so
     that FETCH_SIZE / WRITE_SIZE of the copy kernel calibrate the counters."""
@@ -231,6 +263,8 @@ def other_workload(args):
         line["roofline"] = fp64_roofline(probs, counts, kt, args.steps)
     if args.workload == "spin_chain_128":
         line["recompute_mode"] = recompute_timing(probs, us, args)
+    if args.workload == "uneven_batch":
+        line["autotuned"] = autotuned_timing(probs, us, args, prec)
     if args.cpu_sample > 0 and args.workload != "four_qubit_tracking":
         line["accuracy"] = sample_accuracy(args.workload, probs, us, prec)
     if args.workload == "four_qubit_tracking" and "gradient" in kt:
@@ -263,6 +297,36 @@ def other_workload(args):
                                 "kind": "port", "sample": f"{n} of {p.n_members} members, all {p.n_t} slices, "
                                 f"{cpu_s:.1f} s; rate scaled to the full ensemble"}
     print(json.dumps(line))
+
+
+def autotuned_timing(probs, us, args, prec):
+    """The same batch with the library's segment-length autotuner
+    (PR_SEGMENT_AUTOTUNE): plan-creation time, evaluations/s and the segment
+    count it chose per dimension."""
+    from pulsar_rocm.problem import PR_SEGMENT_AUTOTUNE
+    t0 = time.perf_counter()
+    plan = native.Plan(probs, precision=prec, segment=PR_SEGMENT_AUTOTUNE)
+    create_s = time.perf_counter() - t0
+    try:
+        plan.upload_controls(us)
+        d_u, d_F, d_G = plan.buffers()
+        for _ in range(args.warmup):
+            plan.eval_device(d_u, d_F, d_G)
+        plan.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.eval_device(d_u, d_F, d_G)
+        plan.synchronize()
+        el = time.perf_counter() - t0
+        D = plan.eval(us, diag=True)[2]
+    finally:
+        plan.close()
+    seglen = {}
+    for q, dd in zip(probs, D):
+        seglen.setdefault(q.dim, set()).add(int(np.ceil(q.n_t / dd[3])))
+    return {"evals_per_s": round(args.steps / el, 3), "ms_per_eval": round(1e3 * el / args.steps, 4),
+            "plan_create_s": round(create_s, 3),
+            "slices_per_segment_by_dim": {int(k): sorted(v)[:3] for k, v in sorted(seglen.items())}}
 
 
 def recompute_timing(probs, us, args):
@@ -610,6 +674,8 @@ def main():
                                "sample": f"{args.cpu_sample} of {M} members, all {nt} slices, FP64 oracle with "
                                          f"separate F and G passes, {threads} OpenMP threads, {cpu_s:.2f} s"}
         out["accuracy"] = h["accuracy"]
+        out["cpu_forms"] = cpu_forms(p, u, args.cpu_threads, threads)
+    out["propagator_steps_per_s"] = round(h["value"] * M * nt, 1)
     if "fp64" in forms:
         for n in names:
             forms[n]["vs_fp64"] = round(forms[n]["value"] / forms["fp64"]["value"], 3)

@@ -153,10 +153,16 @@ typedef struct {
   const double* track_target; /* [n_track][dim*dim] complex */
 } pr_problem;
 
+#define PR_SEGMENT_AUTOTUNE (-1)
+
 typedef struct {
   int precision;   /* PR_FP64 or PR_MIXED */
   int grad_mode;   /* PR_GRAD_REFERENCE or PR_GRAD_EXACT */
-  int segment;     /* slices per time segment of the parallel chain scan; 0 = auto */
+  int segment;     /* slices per time segment of the parallel chain scan; 0 = auto (about
+                      8192 equal work items per dimension group); PR_SEGMENT_AUTOTUNE =
+                      measure 0.5, 1, 2 and 4 times the auto length for every dimension
+                      group at plan creation and keep the fastest (no reference
+                      counterpart: the reference runs slices one after another) */
   int store_props; /* 1: keep U_k in HBM between the passes; 0: recompute U_k in the
                       gradient walk for closed-dynamics FP64 problems (dim > 64: from
                       segment checkpoints every 32 slices at most; the others keep

@@ -341,6 +341,9 @@ struct Plan {
   double* d_G = nullptr;
   double* d_part = nullptr;  // [RED_CHUNKS][n_params] partial member sums
   double* h_stage = nullptr; // pinned host staging: n_params controls or gradients, then n_probs F
+  // segment length of each group relative to the work-per-item rule (the
+  // autotuner's choice; 1 otherwise)
+  std::vector<double> seg_scale = std::vector<double>(13, 1.0);
   // pr_eval's staging ranges: problems [stage_cut[r], stage_cut[r + 1]) are
   // packed by thread r and copied by one transfer each, event stage_ev[r]
   // marking the range's gradient download
@@ -356,6 +359,7 @@ struct Plan {
   pr::ProbDev* d_probs_static = nullptr;
   struct Group {
     int B = 0, V = 0;
+    int gid = 0;            // index of the group (B, or 8 + B for tracking)
     bool lindblad = false;  // some problem of the group has Lindblad dynamics
     bool mixed = false;     // low-precision exp and gradient walk (d <= 16 only): FP32 MFMA, or
     bool split = false;     // FP16x2 split products (PR_SPLIT16)
@@ -758,31 +762,33 @@ int launch_big(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hip
   return PR_OK;
 }
 
+// the launches of one dimension group (segment, combine, gradient walk)
+int launch_one(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hipStream_t st) {
+  if (g.track) {
+    switch (g.B) {
+      case 1: return launch_track<1>(pl, g, d_u, st);
+      case 2: return launch_track<2>(pl, g, d_u, st);
+      case 3: return launch_track<3>(pl, g, d_u, st);
+      case 4: return launch_track<4>(pl, g, d_u, st);
+      default: return fail(PR_ERR_LAUNCH, "internal: block count %d", g.B);
+    }
+  }
+  switch (g.B) {
+    case 1: return launch_group<1>(pl, g, d_u, grad, st);
+    case 2: return launch_group<2>(pl, g, d_u, grad, st);
+    case 3: return launch_group<3>(pl, g, d_u, grad, st);
+    case 4: return launch_group<4>(pl, g, d_u, grad, st);
+    case 5: return launch_big<5>(pl, g, d_u, grad, st);
+    case 6: return launch_big<6>(pl, g, d_u, grad, st);
+    case 7: return launch_big<7>(pl, g, d_u, grad, st);
+    case 8: return launch_big<8>(pl, g, d_u, grad, st);
+    default: return fail(PR_ERR_LAUNCH, "internal: block count %d", g.B);
+  }
+}
+
 int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
   for (const auto& g : pl.groups) {
-    int rc = PR_OK;
-    if (g.track) {
-      switch (g.B) {
-        case 1: rc = launch_track<1>(pl, g, d_u, st); break;
-        case 2: rc = launch_track<2>(pl, g, d_u, st); break;
-        case 3: rc = launch_track<3>(pl, g, d_u, st); break;
-        case 4: rc = launch_track<4>(pl, g, d_u, st); break;
-        default: rc = fail(PR_ERR_LAUNCH, "internal: block count %d", g.B);
-      }
-      if (rc) return rc;
-      continue;
-    }
-    switch (g.B) {
-      case 1: rc = launch_group<1>(pl, g, d_u, d_G != nullptr, st); break;
-      case 2: rc = launch_group<2>(pl, g, d_u, d_G != nullptr, st); break;
-      case 3: rc = launch_group<3>(pl, g, d_u, d_G != nullptr, st); break;
-      case 4: rc = launch_group<4>(pl, g, d_u, d_G != nullptr, st); break;
-      case 5: rc = launch_big<5>(pl, g, d_u, d_G != nullptr, st); break;
-      case 6: rc = launch_big<6>(pl, g, d_u, d_G != nullptr, st); break;
-      case 7: rc = launch_big<7>(pl, g, d_u, d_G != nullptr, st); break;
-      case 8: rc = launch_big<8>(pl, g, d_u, d_G != nullptr, st); break;
-      default: rc = fail(PR_ERR_LAUNCH, "internal: block count %d", g.B);
-    }
+    const int rc = launch_one(pl, g, d_u, d_G != nullptr, st);
     if (rc) return rc;
   }
   int rc;
@@ -955,7 +961,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       // (d > 64: workgroups of 16 wavefronts, about 512 of them)
       long long want = B > 4 ? 512 : std::max(1LL, (long long)(8192 / NB));
       const double per_item = std::ceil(group_work[gi] / (double)want);
-      L = (int)std::min<double>(std::max<double>(per_item, 1.0), (double)nt);
+      L = (int)std::min<double>(std::max<double>(std::round(per_item * pl.seg_scale[gi]), 1.0), (double)nt);
       L = std::max(L, std::min(nt, 4));
       // d > 64 without stored images: the segment starts E_s are the
       // checkpoints the gradient walk recomputes from, every 32 slices at most
@@ -1198,6 +1204,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     const int B = gi > 8 ? gi - 8 : gi;
     Plan::Group g;
     g.B = B;
+    g.gid = gi;
     g.track = gi > 8;
     g.track_packed = gi == 9;
     for (int p = 0; p < n_probs; p++)
@@ -1271,6 +1278,66 @@ int fill_diag(const Plan& pl, double* diag) {
 }
 
 }  // namespace
+
+// The segment-length autotuner (pr_options.segment = PR_SEGMENT_AUTOTUNE): for
+// each candidate multiple of the work-per-item segment length, a trial plan of
+// the same problems is built and its dimension groups are timed one by one
+// (HIP events around each group's launches: 1 warm-up and 3 timed
+// evaluations at zero controls); every group then takes the candidate it ran
+// fastest with. The final plan is built with those lengths; the trial plans
+// are freed before it is.
+int time_groups(Plan& pl, std::vector<double>& ms) {
+  ms.assign(13, 0.0);
+  HIPCHK(hipMemsetAsync(pl.d_controls, 0, sizeof(double) * pl.n_params, pl.stream));
+  const size_t ng = pl.groups.size();
+  std::vector<hipEvent_t> ev(2 * ng, nullptr);
+  hipError_t e = hipSuccess;
+  for (auto& x : ev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
+  int rc = e == hipSuccess ? PR_OK : fail(PR_ERR_DEVICE, "hipEventCreate: %s", hipGetErrorString(e));
+  for (int rep = 0; rep < 4 && rc == PR_OK; rep++) {
+    for (size_t i = 0; i < ng && rc == PR_OK; i++) {
+      if ((e = hipEventRecord(ev[2 * i], pl.stream)) != hipSuccess) break;
+      rc = launch_one(pl, pl.groups[i], pl.d_controls, true, pl.stream);
+      if (rc == PR_OK && (e = hipEventRecord(ev[2 * i + 1], pl.stream)) != hipSuccess) break;
+    }
+    if (rc == PR_OK && e == hipSuccess) e = hipStreamSynchronize(pl.stream);
+    if (rc == PR_OK && e != hipSuccess) rc = fail(PR_ERR_DEVICE, "autotune timing: %s", hipGetErrorString(e));
+    for (size_t i = 0; rep > 0 && i < ng && rc == PR_OK; i++) {
+      float t = 0;
+      if ((e = hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1])) != hipSuccess)
+        rc = fail(PR_ERR_DEVICE, "hipEventElapsedTime: %s", hipGetErrorString(e));
+      ms[pl.groups[i].gid] += t;
+    }
+  }
+  for (auto x : ev)
+    if (x) (void)hipEventDestroy(x);
+  return rc;
+}
+
+int autotune_segments(Plan& pl, const pr_problem* probs, int n_probs) {
+  const double cand[] = {0.5, 1.0, 2.0, 4.0};
+  std::vector<double> best(13, 1e300), scale(13, 1.0), ms;
+  for (double c : cand) {
+    Plan* t = new Plan;
+    t->opt = pl.opt;
+    t->use_graphs = false;
+    t->seg_scale.assign(13, c);
+    hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+    int rc = e == hipSuccess ? PR_OK : fail(PR_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    if (rc == PR_OK) rc = build_plan(*t, probs, n_probs);
+    if (rc == PR_OK) rc = time_groups(*t, ms);
+    delete t;
+    if (rc) return rc;
+    for (int gi = 0; gi < 13; gi++)
+      if (ms[gi] > 0 && ms[gi] < best[gi]) {
+        best[gi] = ms[gi];
+        scale[gi] = c;
+      }
+  }
+  pl.seg_scale = scale;
+  return PR_OK;
+}
 
 // Every problem's controls packed back to back into the plan's pinned staging
 // buffer, so that one host-to-device copy moves them all.
@@ -1418,7 +1485,7 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
   if (o.grad_mode != PR_GRAD_REFERENCE && o.grad_mode != PR_GRAD_EXACT)
     return fail(PR_ERR_ARG, "unknown grad_mode %d", o.grad_mode);
   if (o.store_props < -1 || o.store_props > 1) return fail(PR_ERR_ARG, "store_props %d is not -1, 0 or 1", o.store_props);
-  if (o.segment < 0) return fail(PR_ERR_ARG, "segment must be >= 0");
+  if (o.segment < PR_SEGMENT_AUTOTUNE) return fail(PR_ERR_ARG, "segment must be >= 0, or PR_SEGMENT_AUTOTUNE");
   for (int p = 0; p < n_probs; p++) {
     if (probs[p].dim > 64 && o.grad_mode == PR_GRAD_EXACT)
       return fail(PR_ERR_UNSUPPORTED, "problem %d: the exact gradient is available for dim <= 64", p);
@@ -1439,6 +1506,14 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
   if (e != hipSuccess) {
     delete pl;
     return fail(PR_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  if (o.segment == PR_SEGMENT_AUTOTUNE) {
+    pl->opt.segment = 0;
+    rc = autotune_segments(*pl, probs, n_probs);
+    if (rc) {
+      delete pl;
+      return rc;
+    }
   }
   rc = build_plan(*pl, probs, n_probs);
   if (rc) {

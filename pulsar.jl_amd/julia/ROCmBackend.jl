@@ -29,6 +29,7 @@ const PR_MIXED = Cint(1)
 const PR_SPLIT16 = Cint(2)
 const PR_GRAD_REFERENCE = Cint(0)
 const PR_GRAD_EXACT = Cint(1)
+const PR_SEGMENT_AUTOTUNE = Cint(-1)   # PROptions.segment: the library times candidate segment lengths
 
 # Memory layout identical to `pr_problem` (include/pulsar_rocm.h).
 struct PRProblem

@@ -22,6 +22,7 @@ PR_GATE, PR_PAIRS_SQUARE, PR_PAIRS_REAL, PR_TRACKING, PR_GATE_PROPAGATORS, PR_ST
 PR_STATE_PROPAGATORS = 6
 PR_PAIRS_MODULUS = 7
 PR_FP64, PR_MIXED, PR_SPLIT16 = 0, 1, 2
+PR_SEGMENT_AUTOTUNE = -1  # pr_options.segment: time candidate segment lengths at plan creation
 PR_GRAD_REFERENCE, PR_GRAD_EXACT = 0, 1
 
 

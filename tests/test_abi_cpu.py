@@ -58,6 +58,16 @@ def test_validation_runs_before_device_use():
     assert e.value.code == -1 and "Hermitian" in str(e.value)
 
 
+def test_options_validation_runs_before_device_use():
+    # segment below PR_SEGMENT_AUTOTUNE (-1), an unknown precision or store_props
+    # are PR_ERR_ARG before any device work
+    p, u = workloads.four_qubit_ensemble(n_t=4, n_det=1, n_amp=2)
+    for kw, what in ((dict(segment=-2), "segment"), (dict(precision=7), "precision"), (dict(store_props=3), "store_props")):
+        with pytest.raises(native.RocmError) as e:
+            native.Plan([p], **kw)
+        assert e.value.code == -1 and what in str(e.value)
+
+
 def test_non_unitary_target_rejected():
     p, u = workloads.four_qubit_ensemble(n_t=4, n_det=1, n_amp=2)
     p.U_target = 1.01 * np.asarray(p.U_target)

@@ -288,6 +288,27 @@ def test_host_buffer_ranges_match_the_device_resident_evaluation():
         plan.close()
 
 
+def test_segment_autotuner_plan_matches_oracle():
+    # pr_options.segment = PR_SEGMENT_AUTOTUNE: candidate segment lengths are
+    # timed per dimension group at plan creation; the plan it keeps evaluates
+    # like any other (the oracle's values, deterministic replays)
+    from pulsar_rocm.problem import PR_SEGMENT_AUTOTUNE
+    batch = W.uneven_batch(n_problems=24, seed=31, n_min=40, n_max=160)
+    probs, us = [b[0] for b in batch], [b[1] for b in batch]
+    plan = native.Plan(probs, segment=PR_SEGMENT_AUTOTUNE)
+    try:
+        F, G, D = plan.eval(us, diag=True)
+        F2, G2 = plan.eval(us)
+    finally:
+        plan.close()
+    assert np.array_equal(F, F2) and all(np.array_equal(a, b) for a, b in zip(G, G2))
+    print("autotuned segments per problem", D[:, 3].astype(int).tolist(), flush=True)
+    for i, (p, u) in enumerate(batch):
+        F0, G0 = O.evaluate(p, u)
+        assert abs(F[i] - F0) <= TOL_F, (i, p.dim)
+        assert np.abs(G[i] - G0).max() / np.abs(G0).max() <= TOL_G, (i, p.dim)
+
+
 def test_graph_replays_match_direct_launches():
     # evaluations replay a captured HIP graph; with per-kernel timing on, the
     # same evaluation launches its kernels directly. Several replays with new
