@@ -460,3 +460,93 @@ end
 # _L_controls, _sigma_init, _sigma_targ, pwr_levels = [1.0], pulse_dt and :square
 # fidelity build one PR_LINDBLAD problem (_rocm_mr_plan), and the adapter's own
 # _apply_penalties! subtracts the QC penalties afterwards.
+
+# ensemble_map / ensemble_grad! / ensemble_fobj  (src/Computation/EnsembleMap.jl:365-607)
+# gain a `device === :rocm` branch. Arbitrary closures keep the CPU-thread map (as
+# under :cuda/:metal, :375-415); a member function with a device description is one
+# device evaluation for all members:
+#
+#   elseif device === :rocm
+#       f isa RocmMemberFidelity ? (out .= _rocm_member_values(f.objs, f.θ, indices)) :
+#                                  (@threadsif true for k in 1:n; out[k] = Float64(f(indices[k])); end)
+#
+# and in ensemble_grad!, before the threaded loop:
+#
+#   if device === :rocm && grad_member! isa RocmMemberGradient
+#       ws = w === nothing ? fill(1.0 / n, n) : w ./ sum(w)
+#       g_out .= _rocm_member_grad(grad_member!.objs, θ, indices, ws); return g_out
+#   end
+#
+# Member objectives are the sample closures of build_ensemble_from_systems
+# (SystemBuilder.jl:61-77): F_i = compute_fidelity(systems[i], ctrl(θ), target),
+# g_i = vec(compute_grape_gradient(systems[i], ctrl(θ), target)).
+struct RocmMemberObjectives
+    systems::Vector{AbstractQuantumSystem}
+    target::QuantumTarget
+    n_t::Int
+    dt::Float64
+end
+struct RocmMemberFidelity      # i -> F_i(θ), the closure of the ensemble_map example
+    objs::RocmMemberObjectives
+    θ::Vector{Float64}
+end
+struct RocmMemberF             # (θ, i) -> F_i(θ), ensemble_fobj's f_member
+    objs::RocmMemberObjectives
+end
+struct RocmMemberGradient      # (g, θ, i) -> g_i(θ), grad_member!
+    objs::RocmMemberObjectives
+end
+(f::RocmMemberFidelity)(i) = _rocm_member_values(f.objs, f.θ, [i])[1]
+(f::RocmMemberF)(θ, i) = _rocm_member_values(f.objs, θ, [i])[1]
+(g::RocmMemberGradient)(gv, θ, i) = (gv .= _rocm_member_grad(g.objs, θ, [i], [1.0]); gv)
+
+function _rocm_member_values(o::RocmMemberObjectives, θ::AbstractVector, indices::Vector)
+    n_c = length(o.systems[1].H_controls)
+    key = (:member_F, [(o.systems[i].H_drift, o.systems[i].H_controls) for i in indices], o.n_t, o.dt,
+           _target_key(o.target))
+    plan = _rocm_cached_plan(key) do
+        probs = PRProblem[]; roots = Any[]
+        for i in indices
+            p, r = rocm_problem(o.systems[i].H_drift, o.systems[i].H_controls, o.n_t, o.dt, o.target;
+                                state_normalize = false)
+            push!(probs, p); append!(roots, r)
+        end
+        ROCmPlan(probs, roots)
+    end
+    u = reshape(Vector{Float64}(θ), n_c, o.n_t)
+    F, _ = rocm_eval(plan, fill(u, length(indices)); grad = false)
+    return F
+end
+
+# the weighted mean gradient, reduced on the device: one problem per group of
+# members that share the control operators up to per-channel scales, member
+# weights ws (summing to 1)
+function _rocm_member_grad(o::RocmMemberObjectives, θ::AbstractVector, indices::Vector, ws::Vector{Float64})
+    n_c = length(o.systems[1].H_controls)
+    key = (:member_G, [(o.systems[i].H_drift, o.systems[i].H_controls) for i in indices], ws, o.n_t, o.dt,
+           _target_key(o.target))
+    plan = _rocm_cached_plan(key) do
+        groups = Vector{Tuple{Vector, Vector{Int}, Vector{Vector{Float64}}}}()
+        for (k, i) in enumerate(indices)
+            hc = o.systems[i].H_controls
+            placed = false
+            for (ref, ks, scs) in groups
+                sc = [real(dot(a, b) / dot(a, a)) for (a, b) in zip(ref, hc)]
+                if all(norm(b - c * a) <= 1e-13 * max(norm(b), eps()) for (a, b, c) in zip(ref, hc, sc))
+                    push!(ks, k); push!(scs, sc); placed = true; break
+                end
+            end
+            placed || push!(groups, (hc, [k], [ones(n_c)]))
+        end
+        probs = PRProblem[]; roots = Any[]
+        for (ref, ks, scs) in groups
+            p, r = rocm_problem([o.systems[indices[k]].H_drift for k in ks], ref, o.n_t, o.dt, o.target;
+                                scales = permutedims(hcat(scs...)), weights = ws[ks])
+            push!(probs, p); append!(roots, r)
+        end
+        ROCmPlan(probs, roots)
+    end
+    u = reshape(Vector{Float64}(θ), n_c, o.n_t)
+    _, G = rocm_eval(plan, fill(u, length(plan.n_ctrl)))
+    return vec(sum(G))
+end

@@ -12,6 +12,7 @@ from .api import (BatchedEnsemble, ControlSequence, EnsembleObjective, GRAPEConf
                   evaluate_fidelity, get_device, grape_gradient, grape_optimize, grape_optimize_ensemble,
                   grape_step, quantum_system, random_controls, set_device, state_target, unitary_target,
                   with_device, zero_controls)
+from .ensemble_map import MemberObjectives, ensemble_fobj, ensemble_grad, ensemble_map  # noqa: F401
 from .verify import (AutoDiffConfig, compute_gradient_autodiff, finite_diff_gradient_ad,  # noqa: F401
                      finite_difference_gradient, grape_gradient_check, select_autodiff_backend,
                      verify_gradient, verify_gradient_autodiff)
