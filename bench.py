@@ -261,7 +261,7 @@ def other_workload(args):
             "python_mirror_eval_ms_per_eval": round(1e3 * host_elapsed / args.steps, 4)}
     if prec == PR_FP64 and args.workload != "four_qubit_tracking":
         line["roofline"] = fp64_roofline(probs, counts, kt, args.steps)
-    if args.workload == "spin_chain_128":
+    if args.workload in ("spin_chain_128", "three_qubit_lindblad"):
         line["recompute_mode"] = recompute_timing(probs, us, args)
     if args.workload == "uneven_batch":
         line["autotuned"] = autotuned_timing(probs, us, args, prec)
@@ -331,7 +331,10 @@ def autotuned_timing(probs, us, args, prec):
 
 def recompute_timing(probs, us, args):
     """The same evaluation without stored U_k images (store_props = 0): the
-    gradient walk recomputes each U_k from segment checkpoints."""
+    gradient walk recomputes each U_k from segment checkpoints. Lindblad
+    problems also keep their forward states only for the work items of one
+    launch window (segments of at most 32 slices, 8192 / B^2 work items), in
+    place of every slice's."""
     plan = native.Plan(probs, store_props=0, segment=args.segment)
     try:
         plan.upload_controls(us)
@@ -355,9 +358,22 @@ def recompute_timing(probs, us, args):
     finally:
         plan.close()
     images = sum(q.n_members * q.n_t * ((q.dim + 15) // 16) ** 2 * 4096 for q in probs)
-    return {"evals_per_s": round(args.steps / el, 3), "ms_per_eval": round(1e3 * el / args.steps, 3),
-            "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
-            "segments_per_member": int(D[0, 3]), "image_bytes_saved": int(images)}
+    out = {"evals_per_s": round(args.steps / el, 3), "ms_per_eval": round(1e3 * el / args.steps, 3),
+           "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+           "segments_per_member": int(D[0, 3]), "image_bytes_saved": int(images)}
+    from pulsar_rocm.problem import PR_LINDBLAD
+    if any(q.dynamics == PR_LINDBLAD for q in probs):
+        saved = 0
+        for q, dd in zip(probs, D):
+            if q.dynamics != PR_LINDBLAD:
+                continue
+            nb = ((q.dim + 15) // 16) ** 2
+            S = int(dd[3])
+            L = -(-q.n_t // S)
+            slots = min(q.n_members * S, max(1, 8192 // nb))
+            saved += q.n_members * q.n_t * nb * 4096 - slots * 2 * L * nb * 4096
+        out["forward_state_bytes_saved"] = int(saved)
+    return out
 
 
 def c_abi_eval_ms(plan, probs, us, steps):

@@ -164,9 +164,12 @@ typedef struct {
                       group at plan creation and keep the fastest (no reference
                       counterpart: the reference runs slices one after another) */
   int store_props; /* 1: keep U_k in HBM between the passes; 0: recompute U_k in the
-                      gradient walk for closed-dynamics FP64 problems (dim > 64: from
-                      segment checkpoints every 32 slices at most; the others keep
-                      their images); -1: recompute only when all images would take
+                      gradient walk for FP64 problems (dim > 64 and Lindblad: from
+                      segment checkpoints every 32 slices at most; Lindblad problems
+                      also keep their forward states only for the work items of one
+                      launch window, 8192 / ceil(dim/16)^2 of them; tracking and
+                      reduced-precision problems keep their images); -1: recompute
+                      only when all images (and Lindblad forward states) would take
                       more than half of the free device memory */
 } pr_options;
 

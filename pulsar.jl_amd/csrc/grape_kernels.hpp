@@ -42,7 +42,10 @@ struct ProbDev {
   int prescaled;              // 1: drift images and control entries already hold c dt A
                               // (uniform dt), so assemble() needs no final scaling
   int recompute;              // 1: no U_k images (Ust is null); the gradient walk
-                              // recomputes U_k (closed dynamics, FP64)
+                              // recomputes U_k (FP64; closed or Lindblad dynamics)
+  int chunk;                  // Lindblad: work-item slots of Pscr (the work item m*nseg + s
+                              // uses slot (m*nseg + s) % chunk); chunk = M*nseg when
+                              // images are stored
   int after;                  // first-order gradient term of slice k: 1 with A_j after U_k,
                               // Tr(.. U_k A_j P_k) (compute_grape_gradient, whose
                               // Q_k^dag = U_n..U_k, Propagators.jl:375-380); 0 in place of
@@ -111,7 +114,8 @@ struct ProbDev {
   double2* Eseg;              // [M][nseg+1][NB][256]
   double2* Om;                // [M][NB][256]
   double2* Bb;                // [M][nseg][NB][256] backward boundaries (Lindblad)
-  double2* Pscr;              // [M*nseg][seglen][NB][256] forward states (Lindblad)
+  double2* Pscr;              // Lindblad, per work-item slot: [seglen][NB][256] forward
+                              // states, then (recompute) [seglen][NB][256] recomputed U_k^T
   double* Fm;                 // [M]
   double* Gm;                 // [M][nc*nt]
   double* udef;               // [M] ||U^dag U - I||_max (closed)
@@ -1054,6 +1058,36 @@ __device__ inline Blk recompute_ut(const ProbDev& P, int m, int k, const double*
   return Ut;
 }
 
+// U_k^T of a Lindblad problem, recomputed for its gradient walk: the assembly,
+// scaling and general exp calls of k_segment for non-unitary generators
+// (expm_blk for every block count), so the result is k_segment's image bit
+// for bit; the transpose goes through the LDS tile LA.
+template <int B>
+__device__ inline Blk recompute_ut_lind(const ProbDev& P, int m, int k, const double* __restrict__ u, double2* LA,
+                                        double2* LB, double* red, const Ln& x) {
+  Blk X = assemble<B>(P, m, k, u, load_drift<B>(P, m, x), x);
+  bool deg6;
+  const int s = scaling_of<B>(X, red, x, deg6);
+  {
+    const double f = ldexp(1.0, -s);
+    X.re *= f;
+    X.im *= f;
+  }
+  const Blk U = expm_blk<B>(X, s, deg6, LA, LB, x);
+  bsync<B>();
+  put<B>(LA, U, x);
+  bsync<B>();
+  Blk Ut;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const double2 v = LA[lidx(x.Jb * B + x.I, x.c, rowof(x.q, r))];  // U[16J + c][16I + q + 4r]
+    Ut.re[r] = v.x;
+    Ut.im[r] = v.y;
+  }
+  bsync<B>();
+  return Ut;
+}
+
 // ---------------------------------------------------------------- k_gradient
 // RECOMP: problems flagged P.recompute get U_k from recompute_ut instead of
 // their (absent) images.
@@ -1169,14 +1203,28 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
     }
     return;
   }
-  // Lindblad: forward states of the segment, then Lambda backward
+  // Lindblad: forward states of the segment, then Lambda backward. The work
+  // item's slot of Pscr holds its forward states and, when the images are not
+  // stored, the U_k^T it recomputes on the way forward (the same assembly,
+  // scaling and exp calls as k_segment, so the same bits) for the walk back.
+  // Slots are taken modulo P.chunk: the plan launches at most P.chunk
+  // consecutive work items of a problem at a time.
   Blk P0 = load_blk(P.Psi0, x);
   Blk Pk = mm<B, N, N>(Es, P0, LA, LB, x);
   const size_t wi = (size_t)m * P.nseg + it.seg;
-  double2* scr = P.Pscr + wi * (size_t)P.seglen * MB;
+  const bool rec = RECOMP && P.recompute;
+  const size_t slot = rec ? wi % (size_t)P.chunk : wi;
+  double2* scr = P.Pscr + slot * (size_t)P.seglen * MB * (rec ? 2 : 1);
+  double2* uscr = scr + (size_t)P.seglen * MB;
   for (int k = it.k0; k < it.k1; k++) {
     store_blk(scr + (size_t)(k - it.k0) * MB, Pk, x);
-    Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
+    Blk Ut;
+    if (rec) {
+      Ut = recompute_ut_lind<B>(P, m, k, u, LA, LB, red, x);
+      store_blk(uscr + (size_t)(k - it.k0) * MB, Ut, x);
+    } else {
+      Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
+    }
     Pk = mm<B, T, N>(Ut, Pk, LA, LB, x);
   }
   Blk Lm = load_blk(P.Bb + wi * MB, x);
@@ -1188,7 +1236,7 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
     } else {
       contract<B, V>(P, m, k, Mk, LA, x);
     }
-    Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
+    Blk Ut = rec ? load_blk(uscr + (size_t)(k - it.k0) * MB, x) : load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
     Lm = mm<B, J, N>(Ut, Lm, LA, LB, x);          // U_k^dag Lambda_k  (U^dag = conj(U^T))
   }
 }

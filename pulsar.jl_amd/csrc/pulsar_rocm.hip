@@ -84,6 +84,14 @@ namespace {
 
 unsigned char* g_scratch_ptr[PR_MAX_DEVICES] = {nullptr};
 
+// Lindblad problems without stored images: segments of at most this many
+// slices, and a gradient launch over at most lind_window(B) work items (about
+// 8192 wavefronts), so that the walk's scratch (forward states and recomputed
+// U_k^T, 2 * LIND_RECOMPUTE_SEG slices of NB tiles per work item) stays within
+// 8192 * 64 * 4 KiB = 2 GiB whatever the ensemble and the slice count.
+constexpr int LIND_RECOMPUTE_SEG = 32;
+int lind_window(int B) { return std::max(1, 8192 / (B * B)); }
+
 thread_local std::string g_err;
 
 int fail(int code, const char* fmt, ...) {
@@ -368,6 +376,8 @@ struct Plan {
                             // (the group's common pass count, 1..3), or 0: tables read per slice
     bool trace_regs = false;  // mixed walk keeps the trace operand in registers
     bool recompute = false;   // some problem of the group recomputes U_k in the gradient walk
+    int window = 0;           // > 0: the gradient walk is launched over at most this many
+                              // consecutive work items at a time (Lindblad scratch slots)
     bool track = false;       // tracking-objective problems (k_segment, k_track_scan, k_track_walk)
     bool track_packed = false;  // d <= 16 tracking with n_pairs <= 8: states and co-states in one block
     double2* big_scratch = nullptr;  // d > 64: per work item matrices in HBM
@@ -620,6 +630,31 @@ int launch_gradient(Plan& pl, const Plan::Group& g, const double* d_u, hipStream
   // the recomputing walk runs the segment kernel's exp (two tiles at B == 1)
   if (g.recompute) sm = std::max(sm, pr::Smem<B>::segment);
   const dim3 blk(64 * B * B);
+  if (g.window > 0) {
+    // the Lindblad walk over windows of consecutive work items: a problem's
+    // items in one launch take distinct scratch slots (slot = item % chunk,
+    // chunk >= the window or all of the problem's items)
+    for (int i0 = 0; i0 < g.n_items; i0 += g.window) {
+      const int n = std::min(g.window, g.n_items - i0);
+      switch (g.V) {
+#define PR_WCASE(VV)                                                                                          \
+  case VV:                                                                                                    \
+    if (pl.opt.grad_mode == PR_GRAD_EXACT)                                                                    \
+      hipLaunchKernelGGL((pr::k_gradient<B, VV, true, false, true>), dim3(n), blk, sm, st, pl.d_probs,        \
+                         g.d_items + i0, d_u);                                                                \
+    else                                                                                                      \
+      hipLaunchKernelGGL((pr::k_gradient<B, VV, false, false, true>), dim3(n), blk, sm, st, pl.d_probs,       \
+                         g.d_items + i0, d_u);                                                                \
+    break;
+        PR_WCASE(2) PR_WCASE(4) PR_WCASE(8) PR_WCASE(16) PR_WCASE(32)
+#undef PR_WCASE
+        default:
+          return fail(PR_ERR_LAUNCH, "internal: bad reduction width %d", g.V);
+      }
+      HIPCHK(hipGetLastError());
+    }
+    return PR_OK;
+  }
   switch (g.V) {
 #define PR_GCASE(VV)                                                                                        \
   case VV:                                                                                                  \
@@ -915,16 +950,24 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   bool lindblad1 = false;
   for (int p = 0; p < n_probs; p++)
     if (probs[p].dim <= 16 && probs[p].dynamics != PR_CLOSED) lindblad1 = true;
+  // Lindblad problems (d <= 64) recompute too: their gradient walk keeps the
+  // forward states and recomputed U_k of a bounded number of work items at a
+  // time (segments of at most LIND_RECOMPUTE_SEG slices, at most
+  // lind_window(B) work items per launch).
   auto can_recompute = [&](const pr_problem& q) {
-    return q.dynamics == PR_CLOSED && q.objective != PR_TRACKING && !(q.dim <= 16 && (mixed1 || lindblad1));
+    if (q.objective == PR_TRACKING) return false;
+    if (q.dynamics != PR_CLOSED) return q.dim <= 64;
+    return !(q.dim <= 16 && (mixed1 || lindblad1));
   };
   bool recompute = pl.opt.store_props == 0;
   if (pl.opt.store_props < 0) {
+    // images, and the forward states of Lindblad problems (as large again)
     double bytes = 0;
     for (int p = 0; p < n_probs; p++) {
       if (probs[p].objective == PR_TRACKING) continue;
       const double nb = (double)((probs[p].dim + 15) / 16) * ((probs[p].dim + 15) / 16);
-      bytes += (double)probs[p].n_members * probs[p].n_t * nb * 256 * sizeof(double2);
+      bytes += (probs[p].dynamics == PR_CLOSED ? 1.0 : 2.0) * (double)probs[p].n_members * probs[p].n_t * nb * 256 *
+               sizeof(double2);
     }
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
@@ -964,8 +1007,9 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       L = (int)std::min<double>(std::max<double>(std::round(per_item * pl.seg_scale[gi]), 1.0), (double)nt);
       L = std::max(L, std::min(nt, 4));
       // d > 64 without stored images: the segment starts E_s are the
-      // checkpoints the gradient walk recomputes from, every 32 slices at most
-      if (B > 4 && recompute && can_recompute(q)) L = std::min(L, 32);
+      // checkpoints the gradient walk recomputes from, every 32 slices at most;
+      // Lindblad likewise, which bounds each work item's scratch slot
+      if ((B > 4 || q.dynamics != PR_CLOSED) && recompute && can_recompute(q)) L = std::min(L, LIND_RECOMPUTE_SEG);
     }
     L = std::min(L, nt);
     const int S = (nt + L - 1) / L;
@@ -1158,7 +1202,14 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     PR_SCRATCH(Om, (size_t)M * MB);
     if (!D.unitary) {
       PR_SCRATCH(Bb, (size_t)M * S * MB);
-      PR_SCRATCH(Pscr, (size_t)M * S * L * MB);
+      if (D.recompute) {
+        // forward states and recomputed U_k^T of lind_window(B) work items
+        D.chunk = std::min(M * S, lind_window(B));
+        PR_SCRATCH(Pscr, (size_t)D.chunk * 2 * L * MB);
+      } else {
+        D.chunk = M * S;
+        PR_SCRATCH(Pscr, (size_t)M * S * L * MB);
+      }
     }
     PR_SCRATCH(Fm, M);
     PR_SCRATCH(Gm, (size_t)M * nc * nt);
@@ -1230,7 +1281,10 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       g.split_np = (np >= 1 && np <= 3) ? np : 0;
     }
     for (int p = 0; p < n_probs; p++)
-      if (group_of(probs[p]) == gi && pl.host_probs[p].recompute) g.recompute = true;
+      if (group_of(probs[p]) == gi && pl.host_probs[p].recompute) {
+        g.recompute = true;
+        if (!pl.host_probs[p].unitary) g.window = lind_window(B);
+      }
     // every control fits one lane group of 64 / V lanes with <= TR_REG nonzeros per lane
     g.trace_regs = B == 1 && vmax[gi] >= nc_max[gi] && tr_nnz_max[gi] <= pr::TR_REG * (64 / vmax[gi]);
     if ((rc = pl.upload(&g.d_items, items[gi].data(), items[gi].size()))) return rc;

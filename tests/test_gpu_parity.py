@@ -568,9 +568,9 @@ def test_recomputed_propagators_match_stored(d, objective, grad_mode):
 
 
 def test_recompute_keeps_images_where_it_does_not_apply():
-    # under store_props = 0 a Lindblad problem and the mixed-precision d <= 16
-    # problems keep their images; the closed d = 20 problem beside them
-    # recomputes. Results equal the stored-image plans.
+    # under store_props = 0 the mixed-precision d <= 16 problems keep their
+    # images; the closed d = 20 problem and the Lindblad problem beside them
+    # recompute. Results equal the stored-image plans.
     from pulsar_rocm.problem import PR_MIXED
     lind = Problem(dim=4, n_ctrl=2, n_t=4, H0=FIX["lind_L0"], Hc=FIX["lind_Lc"], dt_slices=FIX["lind_dts"],
                    dynamics=PR_LINDBLAD, objective=PR_PAIRS_SQUARE, n_members=1,
@@ -580,6 +580,60 @@ def test_recompute_keeps_images_where_it_does_not_apply():
     pm, um = _random_gate(8, 2, 25, 2, 93)
     _same(([pg, lind], [ug, FIX["lind_u"]], dict()))
     _same(([pg, pm], [ug, um], dict(precision=PR_MIXED)))
+
+
+def _one_qubit_lindblad(n_members, n_t, seed):
+    """The fixture's one-qubit Liouvillians (4 x 4) over n_t slices of random
+    length, members with their own drift scale and control scale."""
+    rng = np.random.default_rng(seed)
+    L0 = np.stack([FIX["lind_L0"] * rng.uniform(0.9, 1.1) for _ in range(n_members)])
+    p = Problem(dim=4, n_ctrl=2, n_t=n_t, H0=L0, Hc=FIX["lind_Lc"], dt_slices=rng.uniform(1e-3, 3e-3, n_t),
+                dynamics=PR_LINDBLAD, objective=PR_PAIRS_SQUARE, n_members=n_members,
+                ctrl_scale=rng.uniform(0.9, 1.1, (n_members, 2)) * float(FIX["lind_pwr"]),
+                psi_init=FIX["lind_s0"], psi_targ=FIX["lind_st"])
+    return p, rng.uniform(-1, 1, (2, n_t))
+
+
+@pytest.mark.parametrize("case", ["d64_windows", "d4_windows", "d64_exact"])
+def test_lindblad_recompute_matches_stored(case):
+    # store_props = 0 for Lindblad dynamics: no U_k images; the gradient walk
+    # recomputes U_k on its way forward into a scratch slot per work item and
+    # is launched over windows of consecutive work items (512 at d = 64, 8192
+    # at d <= 16), so the slots are reused. With the same segments F and G
+    # equal the stored-image plan's bit for bit.
+    kw = {}
+    if case == "d64_windows":
+        p, u = W.three_qubit_lindblad(n_members=20, n_t=1000)
+        seg = 8  # 20 x 125 = 2500 work items: 5 windows
+    elif case == "d4_windows":
+        p, u = _one_qubit_lindblad(64, 1000, seed=5)
+        seg = 4  # 64 x 250 = 16000 work items: 2 windows
+    else:
+        p, u = W.three_qubit_lindblad(n_members=2, n_t=60)
+        seg, kw = 5, dict(grad_mode=1)
+    print(f"case {case}: d={p.dim} M={p.n_members} n_t={p.n_t} segment={seg} {kw}", flush=True)
+    out = []
+    for sp in (1, 0):
+        plan = native.Plan([p], store_props=sp, segment=seg, **kw)
+        try:
+            out.append(plan.eval([u]))
+        finally:
+            plan.close()
+    (F1, G1), (F0, G0) = out
+    assert F0[0] == F1[0]
+    assert np.array_equal(G0[0], G1[0])
+    if case == "d64_exact":
+        Fo, Go = O.evaluate(p, u, grad_mode=1)
+        assert abs(F0[0] - Fo) <= TOL_F
+        assert np.abs(G0[0] - Go).max() <= TOL_G * np.abs(Go).max()
+
+
+def test_lindblad_recompute_default_segments_match_oracle():
+    # store_props = 0 with the library's own segmentation (at most 32 slices
+    # per segment without images): the three-qubit Lindblad ensemble at 2 of
+    # its members and all 1000 slices against the oracle
+    p, u = W.three_qubit_lindblad(n_members=2)
+    check(p, u, store_props=0)
 
 
 def test_closed_and_lindblad_problems_in_one_single_block_group():
