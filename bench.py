@@ -329,13 +329,13 @@ def autotuned_timing(probs, us, args, prec):
             "slices_per_segment_by_dim": {int(k): sorted(v)[:3] for k, v in sorted(seglen.items())}}
 
 
-def recompute_timing(probs, us, args):
+def recompute_timing(probs, us, args, prec=PR_FP64):
     """The same evaluation without stored U_k images (store_props = 0): the
     gradient walk recomputes each U_k from segment checkpoints. Lindblad
     problems also keep their forward states only for the work items of one
     launch window (segments of at most 32 slices, 8192 / B^2 work items), in
     place of every slice's."""
-    plan = native.Plan(probs, store_props=0, segment=args.segment)
+    plan = native.Plan(probs, store_props=0, segment=args.segment, precision=prec)
     try:
         plan.upload_controls(us)
         d_u, d_F, d_G = plan.buffers()
@@ -357,7 +357,9 @@ def recompute_timing(probs, us, args):
         D = plan.eval(us, diag=True)[2]
     finally:
         plan.close()
-    images = sum(q.n_members * q.n_t * ((q.dim + 15) // 16) ** 2 * 4096 for q in probs)
+    # FP64 images: 4 KiB per 16 x 16 block; split-form images: 2 KiB (d <= 16)
+    per_block = 2048 if prec == PR_SPLIT16 else 4096
+    images = sum(q.n_members * q.n_t * ((q.dim + 15) // 16) ** 2 * per_block for q in probs)
     out = {"evals_per_s": round(args.steps / el, 3), "ms_per_eval": round(1e3 * el / args.steps, 3),
            "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
            "segments_per_member": int(D[0, 3]), "image_bytes_saved": int(images)}
@@ -654,6 +656,9 @@ def main():
             # what the gradient is for: 84 L-BFGS iterations from the
             # reference controls with this form's F and G; F of the result in FP64
             f["at_optimized_point"]["lbfgs_84"] = lbfgs_check(po, u, prec)
+        if name in ("fp64", "split16"):
+            # the same evaluation without stored images (store_props = 0)
+            f["recompute_mode"] = recompute_timing([p], [u], args, prec)
         forms[name] = f
     # the headline: the fastest form within the gate against the oracle: |dF|
     # <= 1e-9 at the benchmark point and at the optimized point, max relative

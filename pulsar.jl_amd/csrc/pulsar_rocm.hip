@@ -570,14 +570,24 @@ int tend(Plan& pl, int cls, hipStream_t st) {
 }
 
 int launch_gradient_split(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t st) {
-  const size_t sm = pr::Smem<1>::gradient_unitary;
+  // the recomputing walk keeps the drift in a second tile
+  const size_t sm = g.recompute ? pr::Smem<1>::segment : pr::Smem<1>::gradient_unitary;
+  const int hnp = g.split_np, hpre = g.prescaled ? 1 : 0;
   switch (g.V) {
 #define PR_HCASE(VV)                                                                                    \
   case VV:                                                                                              \
-    if (g.trace_regs)                                                                                   \
-      hipLaunchKernelGGL((pr::k_gradient_h<VV, true>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u); \
+    if (g.recompute && g.trace_regs)                                                                    \
+      hipLaunchKernelGGL((pr::k_gradient_h<VV, true, true>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, \
+                         g.d_items, d_u, hnp, hpre);                                                   \
+    else if (g.recompute)                                                                               \
+      hipLaunchKernelGGL((pr::k_gradient_h<VV, false, true>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, \
+                         g.d_items, d_u, hnp, hpre);                                                   \
+    else if (g.trace_regs)                                                                              \
+      hipLaunchKernelGGL((pr::k_gradient_h<VV, true>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u, \
+                         hnp, hpre);                                                                   \
     else                                                                                                \
-      hipLaunchKernelGGL((pr::k_gradient_h<VV, false>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u); \
+      hipLaunchKernelGGL((pr::k_gradient_h<VV, false>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u, \
+                         hnp, hpre);                                                                   \
     break;
     PR_HCASE(2) PR_HCASE(4) PR_HCASE(8) PR_HCASE(16) PR_HCASE(32)
 #undef PR_HCASE
@@ -589,7 +599,13 @@ int launch_gradient_split(Plan& pl, const Plan::Group& g, const double* d_u, hip
   switch (g.V) {
 #define PR_RCASE(VV)                                                                                    \
   case VV:                                                                                              \
-    if (g.trace_regs)                                                                                   \
+    if (g.recompute && g.trace_regs)                                                                    \
+      hipLaunchKernelGGL((pr::k_gradient<1, VV, false, true, true, true>), dim3(g.n_items), dim3(64),         \
+                         pr::Smem<1>::segment, st, pl.d_probs, g.d_items, d_u);                         \
+    else if (g.recompute)                                                                               \
+      hipLaunchKernelGGL((pr::k_gradient<1, VV, false, false, true, true>), dim3(g.n_items), dim3(64),        \
+                         pr::Smem<1>::segment, st, pl.d_probs, g.d_items, d_u);                         \
+    else if (g.trace_regs)                                                                              \
       hipLaunchKernelGGL((pr::k_gradient<1, VV, false, true, false, true>), dim3(g.n_items), dim3(64), sm, st,   \
                          pl.d_probs, g.d_items, d_u);                                                   \
     else                                                                                                \
@@ -954,10 +970,12 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   // forward states and recomputed U_k of a bounded number of work items at a
   // time (segments of at most LIND_RECOMPUTE_SEG slices, at most
   // lind_window(B) work items per launch).
+  // The split form (PR_SPLIT16) recomputes its D_k images in the walk as well;
+  // the FP32 form keeps them.
   auto can_recompute = [&](const pr_problem& q) {
     if (q.objective == PR_TRACKING) return false;
     if (q.dynamics != PR_CLOSED) return q.dim <= 64;
-    return !(q.dim <= 16 && (mixed1 || lindblad1));
+    return !(q.dim <= 16 && ((mixed1 && pl.opt.precision != PR_SPLIT16) || lindblad1));
   };
   bool recompute = pl.opt.store_props == 0;
   if (pl.opt.store_props < 0) {
@@ -1192,8 +1210,9 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     D.recompute = recompute && can_recompute(q) ? 1 : 0;
     if (!D.recompute) PR_SCRATCH(Ust, (size_t)M * nt * MB);
     if (B == 1 && mixed1 && !track) {
-      // FP32 images of small-generator slices; the FP64 images above hold the others
-      PR_SCRATCH(Udm, (size_t)M * nt * 256);
+      // FP32 (or split FP16) images of small-generator slices; the FP64 images
+      // above hold the others. Without images the split walk rebuilds them.
+      if (!D.recompute) PR_SCRATCH(Udm, (size_t)M * nt * 256);
       PR_SCRATCH(big, (size_t)M * nt);
       PR_SCRATCH(seg_big, (size_t)M * S);
     }

@@ -220,10 +220,73 @@ __device__ inline Blk split_asm(const ProbDev& P, int m, int k, const double* __
     if constexpr (PRE) {
       return H;
     } else {
+      // (a prescaled problem in a group that is not all prescaled: no factor)
+      if (P.prescaled) return H;
       const double dt = slice_dt(P, k);
       return cscale(P.cre * dt, P.cim * dt, H);
     }
   }
+}
+
+// split_asm with the slot tables read from memory, for any pass count np
+// (the recomputing walk, which cannot hold the tables in registers): the same
+// FMAs in the same order as split_asm<np, PRE>, so the same X.
+__device__ inline Blk split_asm_mem(const ProbDev& P, int m, int k, const double* __restrict__ u, double wl, int np,
+                                    bool pre, const Blk& H0, const Ln& x) {
+  if (np == 0) return assemble<1, true>(P, m, k, u, H0, x);
+  typedef int iv4 __attribute__((ext_vector_type(4)));
+  Blk H = H0;
+  const int lo = __double2loint(wl), hi = __double2hiint(wl);
+  const int p0 = P.so_pass[0];
+  for (int i = 0; i < np; i++) {
+    const iv4 t = ((const PR_GLOBAL iv4*)P.so_idx)[(unsigned)((p0 + i) * 64 + x.lane)];
+    const int ids[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const double2 v = gload2(P.so_v, (unsigned)(((p0 + i) * 4 + r) * 64 + x.lane));
+      const double wj = __hiloint2double(__builtin_amdgcn_ds_bpermute(ids[r], hi),
+                                         __builtin_amdgcn_ds_bpermute(ids[r], lo));
+      H.re[r] = fma(wj, v.x, H.re[r]);
+      H.im[r] = fma(wj, v.y, H.im[r]);
+    }
+  }
+  if (pre || P.prescaled) return H;
+  const double dt = slice_dt(P, k);
+  return cscale(P.cre * dt, P.cim * dt, H);
+}
+
+// coefficients of the split exp polynomial (scaled products, Q4 = 4 Q)
+struct SplitCoef {
+  static constexpr float q1 = 4.0f / 6, qy = 4.0f / 24, qn2 = 4.0f / (120.0f * 512 * 512),
+                         qs3 = 4.0f / (720.0f * 512 * 512 * 512);
+  static constexpr float rn2 = 1.0f / (2.0f * 512 * 512), rp3 = 1.0f / (4.0f * 512 * 512 * 512);
+};
+
+// One slice of the split exp from its FP64 generator X (own layout): Y = X^T =
+// -conj(X) in FP32 and R^T = exp(Y) - I - Y by the degree-6 Paterson-Stockmeyer
+// polynomial on split products,
+//   N2 = -(s Y)^2, S3 = (s Y)^3, Q = I/6 + Y/24 + Y^2/120 + Y^3/720,
+//   R^T = Y^2/2 + Y^3 Q.
+// Returns the split of 2^10 D_k, D_k = U_k^dag - I = conj(Y + R^T) (the stored
+// image), and R^T in r. The segment kernel and the recomputing walk both call
+// it, so they hold the same D_k.
+__device__ inline Sp split_exp_slice(const Blk& X, const f4& diag, Blk32& y, Blk32& r) {
+  y.re = -__builtin_convertvector(X.re, f4);
+  y.im = __builtin_convertvector(X.im, f4);
+  const Sp sY = split_scaled(y, SPLIT_SY);
+  const AOp aY = aop_sym(sY);
+  const Blk32 n2 = mm16<true>(aY, bop(sY));          // -(s Y)^2
+  const Blk32 s3 = mm16<false>(aY, bop(split(n2)));  // -(s Y)(-(s Y)^2) = (s Y)^3
+  Blk32 q;
+  q.re = SplitCoef::qy * y.re - SplitCoef::qn2 * n2.re + SplitCoef::qs3 * s3.re + SplitCoef::q1 * diag;
+  q.im = SplitCoef::qy * y.im - SplitCoef::qn2 * n2.im + SplitCoef::qs3 * s3.im;
+  const Blk32 p3 = mm16<false>(aop_sym(split(s3)), bop(split(q)));  // -(s Y)^3 (4 Q)
+  r.re = -SplitCoef::rn2 * n2.re - SplitCoef::rp3 * p3.re;
+  r.im = -SplitCoef::rn2 * n2.im - SplitCoef::rp3 * p3.im;
+  Blk32 dd;
+  dd.re = y.re + r.re;
+  dd.im = -(y.im + r.im);
+  return split_scaled(dd, SPLIT_SD);
 }
 
 // Slices of a segment whose product is accumulated in the Delta-form (below)
@@ -261,11 +324,6 @@ template <int NP, bool PRE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP > 0 ? PR_SPLIT_SEGMENT_WAVES_REG
                                                                            : PR_SPLIT_SEGMENT_WAVES))) void k_segment_h(
     const ProbDev* __restrict__ probs, const WorkItem* __restrict__ items, const double* __restrict__ u) {
-  const float sy = SPLIT_SY;
-  // coefficients of Q and R^T in the scaled products: Q4 = 4 Q
-  const float q1 = 4.0f / 6, qy = 4.0f / 24, qn2 = 4.0f / (120.0f * 512 * 512),
-              qs3 = 4.0f / (720.0f * 512 * 512 * 512);
-  const float rn2 = 1.0f / (2.0f * 512 * 512), rp3 = 1.0f / (4.0f * 512 * 512 * 512);
   // F's split scale (||F||_2 <= 2, W being unitary) and what undoes it with D's
   const float sf = 8192.f;
   const double finv = 1.0 / (8192.0 * (double)SPLIT_SD);
@@ -334,30 +392,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP > 0 ? PR_
     f32.im = __builtin_convertvector(F.im, f4);
     F.re += X.re;
     F.im += X.im;
-    // Y = X^T = -conj(X) in FP32 and its split
-    Blk32 y;
-    y.re = -__builtin_convertvector(X.re, f4);
-    y.im = __builtin_convertvector(X.im, f4);
-    const Sp sY = split_scaled(y, sy);
-    const AOp aY = aop_sym(sY);
-    const Blk32 n2 = mm16<true>(aY, bop(sY));          // -(s Y)^2
-    const Blk32 s3 = mm16<false>(aY, bop(split(n2)));  // -(s Y)(-(s Y)^2) = (s Y)^3
-    Blk32 q;
-    q.re = qy * y.re - qn2 * n2.re + qs3 * s3.re + q1 * diag;
-    q.im = qy * y.im - qn2 * n2.im + qs3 * s3.im;
-    const Blk32 p3 = mm16<false>(aop_sym(split(s3)), bop(split(q)));  // -(s Y)^3 (4 Q)
-    Blk32 r;  // R^T = Y^2/2 + Y^3 Q
-    r.re = -rn2 * n2.re - rp3 * p3.re;
-    r.im = -rn2 * n2.im - rp3 * p3.im;
+    Blk32 y, r;  // Y = X^T, R^T
+    const Sp sD = split_exp_slice(X, diag, y, r);
     // own(R) = the transpose of own(R^T), through an FP32 tile
 #pragma unroll
     for (int i = 0; i < 4; i++) L32[lidx(0, rowof32(x.q, i), x.c)] = make_float2(r.re[i], r.im[i]);
-    // D = U^dag - I = conj(Y + R^T), stored split; it is also the A operand of E = D^dag
-    Blk32 dd;
-    dd.re = y.re + r.re;
-    dd.im = -(y.im + r.im);
-    const Sp sD = split_scaled(dd, SPLIT_SD);
-    store_split(img + (size_t)k * 512, sD, x);
+    // D = U^dag - I = conj(Y + R^T), stored split (unless the walk recomputes
+    // it); it is also the A operand of E = D^dag
+    if (!P.recompute) store_split(img + (size_t)k * 512, sD, x);
     if (threadIdx.x == 0) P.big[(size_t)m * P.nt + k] = 0;
     bsync<1>();
     Blk32 t;  // R, then R + E F
@@ -445,9 +487,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP > 0 ? PR_
 // start (the walk is unitary, so its entries stay within 16 times that). Work
 // items that the segment kernel redid in FP64 (P.seg_big, every slice of the
 // item) are left to k_gradient<..., REDO>, launched next.
-template <int V, bool REG>
+// RECOMP: problems without stored images (P.recompute) rebuild D_k every slice
+// with the segment kernel's own calls (split_asm_mem with the group's pass
+// count hnp and prescaled flag hpre, then split_exp_slice), so the walk sees
+// the bits the image would have held; the drift sits in the LDS tile LB.
+template <int V, bool REG, bool RECOMP = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SPLIT_GRADIENT_WAVES))) void k_gradient_h(
-    const ProbDev* __restrict__ probs, const WorkItem* __restrict__ items, const double* __restrict__ u) {
+    const ProbDev* __restrict__ probs, const WorkItem* __restrict__ items, const double* __restrict__ u, int hnp,
+    int hpre) {
   const WorkItem it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
   const int m = it.member;
@@ -458,9 +505,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SPLIT_GRA
   smem_parts<1>(LA, LB, red, zb);
   float2* L32 = reinterpret_cast<float2*>(LA);
   const unsigned* img = reinterpret_cast<const unsigned*>(P.Udm) + (size_t)m * P.nt * 512;
-  // the first image is in flight while C_s is formed
-  u4v dh = ((const PR_GLOBAL u4v*)(img + (size_t)it.k0 * 512))[x.lane];
-  u4v dl = ((const PR_GLOBAL u4v*)(img + (size_t)it.k0 * 512))[64 + x.lane];
+  const bool rec = RECOMP && P.recompute;
+  const int nc = P.nc;
+  const double scl = (RECOMP && x.lane < nc) ? gptr(P.scale)[(size_t)m * nc + x.lane] : 0.0;
+  const f4 diag = {rowof32(x.q, 0) == x.c ? 1.f : 0.f, rowof32(x.q, 1) == x.c ? 1.f : 0.f,
+                   rowof32(x.q, 2) == x.c ? 1.f : 0.f, rowof32(x.q, 3) == x.c ? 1.f : 0.f};
+  // D_k's split image, as stored (hi dwords, lo dwords), rebuilt from X_k
+  auto recompute_d = [&](int k, u4v& h, u4v& l) {
+    Blk H0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const double2 v = LB[r * 64 + x.lane];
+      H0.re[r] = v.x;
+      H0.im[r] = v.y;
+    }
+    const double un = x.lane < nc ? gptr(u)[P.coff + (size_t)k * nc + x.lane] : 0.0;
+    const Blk X = split_asm_mem(P, m, k, u, scl * un, hnp, hpre != 0, H0, x);
+    Blk32 y, r;
+    const Sp sD = split_exp_slice(X, diag, y, r);
+    h = u4v{sD.rh.w0, sD.rh.w1, sD.ih.w0, sD.ih.w1};
+    l = u4v{sD.rl.w0, sD.rl.w1, sD.il.w0, sD.il.w1};
+  };
+  u4v dh, dl;
+  if (!rec) {
+    // the first image is in flight while C_s is formed
+    dh = ((const PR_GLOBAL u4v*)(img + (size_t)it.k0 * 512))[x.lane];
+    dl = ((const PR_GLOBAL u4v*)(img + (size_t)it.k0 * 512))[64 + x.lane];
+  }
   Blk Es = load_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + it.seg) * 256, x);
   Blk Om = load_blk(P.Om + (size_t)m * 256, x);
   Blk Y = mm<1, N, N>(Es, Om, LA, LB, x);
@@ -509,9 +580,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SPLIT_GRA
     else
       contract<1, V, true>(P, m, kk, Cm, LA, x);
   };
+  if (rec) {
+    // the member's drift (own layout) into LB, which the walk leaves alone
+    const Blk h = load_blk(P.H0o + (P.h0pm ? (size_t)m * 256 : 0), x);
+#pragma unroll
+    for (int r = 0; r < 4; r++) LB[r * 64 + x.lane] = make_double2(h.re[r], h.im[r]);
+    bsync<1>();
+  }
   if (P.after) contract_c(it.k0, dtk);
   for (int k = it.k0; k < it.k1; k++) {
     const double dtn = k + 1 < it.k1 ? slice_dt(P, k + 1) : dtk;
+    if (rec) recompute_d(k, dh, dl);
     // own(C^T), then its split: the A operand of C. With the first-order term
     // after U_k the trace of slice k has just left C_k in the FP64 tile (own
     // positions); otherwise C goes through the tile in FP32.
@@ -542,7 +621,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SPLIT_GRA
     // A operand of E = U - I = D^dag: own(E^T) = own(conj(D)) = (Re D, -Im D)
     const H4 dih{dh.z, dh.w}, dil{dl.z, dl.w}, drh{dh.x, dh.y}, drl{dl.x, dl.y};
     const AOp aE{bD.h, bD.l, cat(negh(dih), drh), cat(negh(dil), drl)};
-    if (k + 1 < it.k1) {
+    if (!rec && k + 1 < it.k1) {
       dh = ((const PR_GLOBAL u4v*)(img + (size_t)(k + 1) * 512))[x.lane];
       dl = ((const PR_GLOBAL u4v*)(img + (size_t)(k + 1) * 512))[64 + x.lane];
     }

@@ -216,3 +216,88 @@ def test_some_slices_above_the_bound_redo_their_segments_in_fp64(prec):
     for seg in (0, 16):
         dF, dG, _, _ = errors(p, u, prec, segment=seg)
         assert dF <= TOL_F and dG <= TOL_G, (seg, dF, dG)
+
+
+def _split_pair(seg, probs, us):
+    """(F, G, exp counts) of the split-form plan with stored images, then without."""
+    out = []
+    for sp in (1, 0):
+        plan = native.Plan(probs, precision=PR_SPLIT16, store_props=sp, segment=seg)
+        try:
+            F, G = plan.eval(us)
+            out.append((F, G, plan.exp_counts()))
+        finally:
+            plan.close()
+    return out
+
+
+@pytest.mark.parametrize("case", ["ensemble", "spikes", "mixed_tables"])
+def test_split_recompute_matches_stored_images(case):
+    # store_props = 0 in the split form: the walk rebuilds every D_k with the
+    # segment kernel's own assembly and split exp (k_gradient_h<..., RECOMP>),
+    # and items redone in FP64 rebuild U_k in the FP64 walk; F and G equal the
+    # stored-image plan's bit for bit at the same segments
+    if case == "ensemble":
+        p, u = W.four_qubit_ensemble(n_t=300)
+        p, u = W.shrink(p, u, n_members=8)
+        probs, us = [p], [u]
+    elif case == "spikes":  # two slices above the split bound: their items run FP64
+        p, u = W.four_qubit_ensemble(n_t=400)
+        p, u = W.shrink(p, u, n_members=4)
+        u = u.copy()
+        u[:, 37] *= 40.0
+        u[:, 250] *= 40.0
+        probs, us = [p], [u]
+    else:  # problems whose assembly pass counts differ take the general assembly;
+        # one of them has per-slice lengths (not prescaled)
+        rng = np.random.default_rng(17)
+        probs, us = [], []
+        for d, nq, per_slice in ((16, 4, False), (8, 3, True)):
+            Hc = np.stack([2 * np.pi * 50 * W.embed(s, nq, q) / 2 for q in range(nq) for s in (W.SX, W.SY)])
+            nc = Hc.shape[0]
+            # slice lengths that keep every generator below the split bound
+            kw = dict(dt_slices=rng.uniform(0.5e-5, 1.5e-5, 200)) if per_slice else dict(dt=1e-5)
+            probs.append(Problem(dim=d, n_ctrl=nc, n_t=200, H0=W.random_hermitian(d, rng, 2 * np.pi * 100), Hc=Hc,
+                                 objective=PR_GATE, n_members=2, ctrl_scale=rng.uniform(0.9, 1.1, (2, nc)),
+                                 U_target=W.haar_unitary(d, rng), **kw))
+            us.append(rng.uniform(-0.5, 0.5, (nc, 200)))
+    print(f"case {case}", flush=True)
+    (F1, G1, D1), (F0, G0, D0) = _split_pair(23, probs, us)
+    for q, uu, a, b, ga, gb in zip(probs, us, F0, F1, G0, G1):
+        assert a == b
+        assert np.array_equal(ga, gb)
+        Fo, Go = O.evaluate(q, uu)
+        assert abs(a - Fo) <= TOL_F
+        assert np.abs(ga - Go).max() <= TOL_G * np.abs(Go).max()
+    assert np.array_equal(D0, D1)
+    if case == "spikes":
+        assert D0[0][3] > 0  # member-slices redone in FP64
+    else:
+        assert D0[:, 3].sum() == 0
+
+
+def test_split_group_with_prescaled_and_per_slice_problems_matches_oracle():
+    # one group, the same assembly pass count, one problem with a uniform slice
+    # length (operators prescaled by c dt) and one with per-slice lengths: each
+    # takes its own generator scaling
+    rng = np.random.default_rng(23)
+    nq, d = 4, 16
+    Hc = np.stack([2 * np.pi * 50 * W.embed(s, nq, q) / 2 for q in range(nq) for s in (W.SX, W.SY)])
+    nc = Hc.shape[0]
+    probs, us = [], []
+    for per_slice in (False, True):
+        kw = dict(dt_slices=rng.uniform(0.5e-3, 1.5e-3, 150)) if per_slice else dict(dt=1e-3)
+        probs.append(Problem(dim=d, n_ctrl=nc, n_t=150, H0=W.random_hermitian(d, rng, 2 * np.pi * 100), Hc=Hc,
+                             objective=PR_GATE, n_members=3, ctrl_scale=rng.uniform(0.9, 1.1, (3, nc)),
+                             U_target=W.haar_unitary(d, rng), **kw))
+        us.append(rng.uniform(-0.5, 0.5, (nc, 150)))
+    for sp in (1, 0):
+        plan = native.Plan(probs, precision=PR_SPLIT16, store_props=sp)
+        try:
+            F, G = plan.eval(us)
+        finally:
+            plan.close()
+        for q, uu, f, g in zip(probs, us, F, G):
+            Fo, Go = O.evaluate(q, uu)
+            assert abs(f - Fo) <= TOL_F, (sp, f, Fo)
+            assert np.abs(g - Go).max() <= TOL_G * np.abs(Go).max()
