@@ -701,7 +701,7 @@ int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
   if ((rc = tbegin(pl, st))) return rc;
   if (B == 1 && g.mixed) {
     if (g.split) {
-      const size_t sm = pr::Smem<1>::segment;
+      const size_t sm = pr::SPLIT_SEGMENT_SMEM;
 #define PR_SEGH(NP, PRE) hipLaunchKernelGGL((pr::k_segment_h<NP, PRE>), dim3(g.n_items), blk, sm, st, pl.d_probs, g.d_items, d_u)
       if (g.prescaled) {
         switch (g.split_np) {

@@ -292,6 +292,10 @@ __device__ inline Sp split_exp_slice(const Blk& X, const f4& diag, Blk32& y, Blk
 // Slices of a segment whose product is accumulated in the Delta-form (below)
 // before an FP64 product folds it into the running segment product.
 constexpr int SPLIT_GROUP = 128;
+// doubles of controls the split segment kernel stages in LDS at a time
+constexpr int SPLIT_CTRL_CHUNK = 256;
+// its LDS: two tiles and the control chunk
+constexpr size_t SPLIT_SEGMENT_SMEM = Smem<1>::segment + SPLIT_CTRL_CHUNK * sizeof(double);
 
 #ifndef PR_SPLIT_SEGMENT_WAVES_REG
 #define PR_SPLIT_SEGMENT_WAVES_REG 3
@@ -366,7 +370,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP > 0 ? PR_
   SplitAsm<NP> A;
   split_asm_load<NP>(P, A, x);
   const double scl = x.lane < nc ? gptr(P.scale)[(size_t)m * nc + x.lane] : 0.0;
-  double un = x.lane < nc ? gptr(u)[P.coff + (size_t)it.k0 * nc + x.lane] : 0.0;
+  // The item's controls pass through LDS in chunks of ch slices (one global
+  // load and wait per chunk; a per-slice global load would wait on the image
+  // stores issued before it as well, which share its counter).
+  double* LC = reinterpret_cast<double*>(LB + Smem<1>::TILE);
+  const int ch = max(1, SPLIT_CTRL_CHUNK / nc);
+  auto load_chunk = [&](int kc) {
+    const int n = min(ch, it.k1 - kc) * nc;
+    const double* src = u + P.coff + (size_t)kc * nc;
+    for (int e = x.lane; e < n; e += 64) LC[e] = gptr(src)[e];
+    bsync<1>();
+  };
+  load_chunk(it.k0);
+  double un = x.lane < nc ? LC[x.lane] : 0.0;
   unsigned* img = reinterpret_cast<unsigned*>(P.Udm) + (size_t)m * P.nt * 512;
   double2* Tout = P.Tseg + ((size_t)m * P.nseg + it.seg) * 256;
   const f4 diag = {rowof32(x.q, 0) == x.c ? 1.f : 0.f, rowof32(x.q, 1) == x.c ? 1.f : 0.f,
@@ -377,7 +393,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP > 0 ? PR_
   bsync<1>();
   for (int k = it.k0; k < it.k1; k++) {
     const double wl = scl * un;
-    if (k + 1 < it.k1 && x.lane < nc) un = gptr(u)[P.coff + (size_t)(k + 1) * nc + x.lane];
+    if (k + 1 < it.k1) {
+      const int j = (k + 1 - it.k0) % ch;
+      if (j == 0) load_chunk(k + 1);
+      if (x.lane < nc) un = LC[j * nc + x.lane];
+    }
     Blk H0;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -400,7 +420,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP > 0 ? PR_
     // D = U^dag - I = conj(Y + R^T), stored split (unless the walk recomputes
     // it); it is also the A operand of E = D^dag
     if (!P.recompute) store_split(img + (size_t)k * 512, sD, x);
-    if (threadIdx.x == 0) P.big[(size_t)m * P.nt + k] = 0;
     bsync<1>();
     Blk32 t;  // R, then R + E F
 #pragma unroll
@@ -573,7 +592,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SPLIT_GRA
   }
   TraceRegs<V> tregs;
   if constexpr (REG) trace_regs_load<V>(P, m, tregs, x);
-  double dtk = slice_dt(P, it.k0);
+  // slice lengths: with a uniform dt one scalar load before the walk (a
+  // per-slice load would also wait for the image prefetched after it)
+  const bool pre = P.prescaled != 0;
+  const double dt0 = P.dt0;
+  double dtk = pre ? dt0 : gptr(P.dts)[it.k0];
   auto contract_c = [&](int kk, double dtkk) {
     if constexpr (REG)
       contract_regs<V, true>(P, m, kk, dtkk, Cm, LA, tregs, x);
@@ -589,7 +612,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SPLIT_GRA
   }
   if (P.after) contract_c(it.k0, dtk);
   for (int k = it.k0; k < it.k1; k++) {
-    const double dtn = k + 1 < it.k1 ? slice_dt(P, k + 1) : dtk;
+    double dtn = dtk;
+    if (!pre && k + 1 < it.k1) dtn = gptr(P.dts)[k + 1];
     if (rec) recompute_d(k, dh, dl);
     // own(C^T), then its split: the A operand of C. With the first-order term
     // after U_k the trace of slice k has just left C_k in the FP64 tile (own
