@@ -53,45 +53,79 @@ __device__ inline int wave_id() { return threadIdx.x >> 6; }
 
 // Out = op(A) op(B) for OA, OB in {N, T, C}; A, Bm, Out are tile images (Out
 // must not alias A or Bm). LP: LDS panel space (2B tiles).
+//
+// For each inner block K the column panel op(A)(:, K) and the row panel
+// op(B)(K, :) are staged in LDS; every wavefront accumulates its output blocks
+// b = w, w + 16, ... with FP64 MFMA (four real products per k-step). The next
+// K's panels are loaded into registers while the current K's products run
+// (their global latency is hidden behind the matrix cores), then written to
+// LDS after the barrier. The K loop is not unrolled and the function is
+// inlined at every call: a called function spilled its accumulators and
+// saved registers around every product.
 template <int B, int OA, int OB>
-__device__ void mm(double2* __restrict__ Out, const double2* A, const double2* Bm, double2* LP) {
+__device__ __attribute__((always_inline)) inline void mm(double2* __restrict__ Out, const double2* A,
+                                                         const double2* Bm, double2* LP) {
   using K_ = Cfg<B>;
+  constexpr int NE = 2 * B * 256;              // panel elements per K
+  constexpr int PER = (NE + W * 64 - 1) / (W * 64);  // per thread
   const int w = wave_id();
   Blk acc[K_::P];
 #pragma unroll
   for (int p = 0; p < K_::P; p++) acc[p] = zero_blk();
-  for (int K = 0; K < B; K++) {
-    // stage tile i < B: op(A)(i, K) source block; tile B + j: op(B)(K, j)
-    for (int e = threadIdx.x; e < 2 * B * 256; e += W * 64) {
-      const int t = e >> 8, r = (e >> 6) & 3, l = e & 63;
-      int src;
-      const double2* img;
-      if (t < B) {
-        src = (OA == N) ? (t * B + K) : (K * B + t);
-        img = A;
-      } else {
-        const int j = t - B;
-        src = (OB == N) ? (K * B + j) : (j * B + K);
-        img = Bm;
+  double2 pre[PER];
+  // element e of panel K: tile t < B holds op(A)(t, K), tile B + j holds op(B)(K, j)
+  auto fetch = [&](int K) {
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int e = (int)threadIdx.x + i * W * 64;
+      if (NE % (W * 64) == 0 || e < NE) {
+        const int t = e >> 8, r = (e >> 6) & 3, l = e & 63;
+        int src;
+        const double2* img;
+        if (t < B) {
+          src = (OA == N) ? (t * B + K) : (K * B + t);
+          img = A;
+        } else {
+          const int j = t - B;
+          src = (OB == N) ? (K * B + j) : (j * B + K);
+          img = Bm;
+        }
+        pre[i] = gload2(img, (size_t)src * 256 + r * 64 + l);
       }
-      LP[t * 272 + rowof(l >> 4, r) * 17 + (l & 15)] = gload2(img, (size_t)src * 256 + r * 64 + l);
     }
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int e = (int)threadIdx.x + i * W * 64;
+      if (NE % (W * 64) == 0 || e < NE) {
+        const int t = e >> 8, r = (e >> 6) & 3, l = e & 63;
+        LP[t * 272 + rowof(l >> 4, r) * 17 + (l & 15)] = pre[i];
+      }
+    }
+  };
+  fetch(0);
+#pragma unroll 1
+  for (int K = 0; K < B; K++) {
+    stage();
     __syncthreads();
+    if (K + 1 < B) fetch(K + 1);
+    // (k-steps outermost and not unrolled: holding every operand of the
+    // unrolled K step at once spilled the accumulators)
+    const int lane = threadIdx.x & 63, q = lane >> 4, c = lane & 15;
+#pragma unroll 1
+    for (int kb = 0; kb < 4; kb++) {
 #pragma unroll
-    for (int p = 0; p < K_::P; p++) {
-      const int b = w + W * p;
-      if (b < K_::NB) {
-        const int I = b / B, J = b % B;
-        const Ln x = blk_info<B>(b);
-#pragma unroll
-        for (int kb = 0; kb < 4; kb++) {
+      for (int p = 0; p < K_::P; p++) {
+        const int b = w + W * p;
+        if (b < K_::NB) {
+          const int I = b / B, J = b % B;
           // op(A)(I, K)[c][q + 4kb] and op(B)(K, J)[q + 4kb][c] before
           // conjugation: the signs of the imaginary parts go into the MFMA
           // negate modifiers (Re += ar br - ai bi, Im += ar bi + ai br)
-          const double2 a = (OA == N) ? LP[I * 272 + x.c * 17 + rowof(x.q, kb)]
-                                      : LP[I * 272 + rowof(x.q, kb) * 17 + x.c];
-          const double2 bb = (OB == N) ? LP[(B + J) * 272 + rowof(x.q, kb) * 17 + x.c]
-                                       : LP[(B + J) * 272 + x.c * 17 + rowof(x.q, kb)];
+          const double2 a = (OA == N) ? LP[I * 272 + c * 17 + rowof(q, kb)] : LP[I * 272 + rowof(q, kb) * 17 + c];
+          const double2 bb = (OB == N) ? LP[(B + J) * 272 + rowof(q, kb) * 17 + c]
+                                       : LP[(B + J) * 272 + c * 17 + rowof(q, kb)];
           constexpr bool ca = OA == C, cb = OB == C;
           acc[p].re = mfma(a.x, bb.x, acc[p].re);
           acc[p].im = cb ? mfma_nega(a.x, bb.y, acc[p].im) : mfma(a.x, bb.y, acc[p].im);
