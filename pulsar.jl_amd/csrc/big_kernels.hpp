@@ -23,7 +23,12 @@
 namespace pr {
 namespace big {
 
-constexpr int W = 16;  // wavefronts per workgroup
+#ifndef PR_BIG_WAVES
+#define PR_BIG_WAVES 16
+#endif
+constexpr int W = PR_BIG_WAVES;  // wavefronts per workgroup
+// third dword of a raw buffer resource on gfx9-family chips (gfx950)
+constexpr int BUF_DW3 = 0x00020000;
 
 template <int B>
 struct Cfg {
@@ -66,41 +71,45 @@ template <int B, int OA, int OB>
 __device__ __attribute__((always_inline)) inline void mm(double2* __restrict__ Out, const double2* A,
                                                          const double2* Bm, double2* LP) {
   using K_ = Cfg<B>;
-  constexpr int NE = 2 * B * 256;              // panel elements per K
-  constexpr int PER = (NE + W * 64 - 1) / (W * 64);  // per thread
+  // per thread: PH elements of each panel (op(A)(:, K): tiles 0..B-1, op(B)(K, :):
+  // tiles B..2B-1), loaded with the panel's own (wave-uniform) base pointer
+  constexpr int NH = B * 256;                      // elements per panel
+  constexpr int PH = (NH + W * 64 - 1) / (W * 64);
   const int w = wave_id();
   Blk acc[K_::P];
 #pragma unroll
   for (int p = 0; p < K_::P; p++) acc[p] = zero_blk();
-  double2 pre[PER];
-  // element e of panel K: tile t < B holds op(A)(t, K), tile B + j holds op(B)(K, j)
+  double2 pa[PH], pb[PH];
+  // buffer loads: the image base stays in scalar registers and each lane keeps
+  // a 32-bit offset (64-bit per-lane addresses of both panels spilled); the
+  // range check of the resource returns 0 beyond the image
+  const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc((void*)A, 0, (int)(K_::MB * 16), BUF_DW3);
+  const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc((void*)Bm, 0, (int)(K_::MB * 16), BUF_DW3);
   auto fetch = [&](int K) {
+    // element (t, r, l) of panel K: block src = t B + K or K B + t of the image
+    const int ka = ((OA == N) ? K : K * B) * 256 * 16;
+    const int kb = ((OB == N) ? K * B : K) * 256 * 16;
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
+    for (int i = 0; i < PH; i++) {
       const int e = (int)threadIdx.x + i * W * 64;
-      if (NE % (W * 64) == 0 || e < NE) {
-        const int t = e >> 8, r = (e >> 6) & 3, l = e & 63;
-        int src;
-        const double2* img;
-        if (t < B) {
-          src = (OA == N) ? (t * B + K) : (K * B + t);
-          img = A;
-        } else {
-          const int j = t - B;
-          src = (OB == N) ? (K * B + j) : (j * B + K);
-          img = Bm;
-        }
-        pre[i] = gload2(img, (size_t)src * 256 + r * 64 + l);
+      if (NH % (W * 64) == 0 || e < NH) {
+        const int t = e >> 8, rl = e & 255;  // rl = r * 64 + l
+        const int va = (t * ((OA == N) ? B : 1) * 256 + rl) * 16;
+        const int vb = (t * ((OB == N) ? 1 : B) * 256 + rl) * 16;
+        pa[i] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(ra, va, ka, 0));
+        pb[i] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rb, vb, kb, 0));
       }
     }
   };
   auto stage = [&]() {
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
+    for (int i = 0; i < PH; i++) {
       const int e = (int)threadIdx.x + i * W * 64;
-      if (NE % (W * 64) == 0 || e < NE) {
+      if (NH % (W * 64) == 0 || e < NH) {
         const int t = e >> 8, r = (e >> 6) & 3, l = e & 63;
-        LP[t * 272 + rowof(l >> 4, r) * 17 + (l & 15)] = pre[i];
+        const int o = rowof(l >> 4, r) * 17 + (l & 15);
+        LP[t * 272 + o] = pa[i];
+        LP[(B + t) * 272 + o] = pb[i];
       }
     }
   };
@@ -299,7 +308,7 @@ __device__ int exp_slice(const ProbDev& P, int m, int k, const double* __restric
 }  // namespace big
 
 template <int B>
-__global__ __launch_bounds__(1024) void k_segment_big(const ProbDev* __restrict__ probs,
+__global__ __launch_bounds__(64 * big::W) void k_segment_big(const ProbDev* __restrict__ probs,
                                                       const WorkItem* __restrict__ items,
                                                       const double* __restrict__ u, double2* __restrict__ scratch) {
   using K_ = big::Cfg<B>;
@@ -338,7 +347,7 @@ __global__ __launch_bounds__(1024) void k_segment_big(const ProbDev* __restrict_
 }
 
 template <int B>
-__global__ __launch_bounds__(1024) void k_combine_big(const ProbDev* __restrict__ probs,
+__global__ __launch_bounds__(64 * big::W) void k_combine_big(const ProbDev* __restrict__ probs,
                                                       const int2* __restrict__ members,
                                                       double2* __restrict__ scratch) {
   using K_ = big::Cfg<B>;
@@ -394,7 +403,7 @@ __global__ __launch_bounds__(1024) void k_combine_big(const ProbDev* __restrict_
 // Without stored U_k images (P.recompute) the walk starts from the segment's
 // checkpoint E_s and rebuilds each U_k with exp_slice, as k_segment_big made it.
 template <int B>
-__global__ __launch_bounds__(1024) void k_gradient_big(const ProbDev* __restrict__ probs,
+__global__ __launch_bounds__(64 * big::W) void k_gradient_big(const ProbDev* __restrict__ probs,
                                                        const WorkItem* __restrict__ items,
                                                        const double* __restrict__ u, double2* __restrict__ scratch) {
   using K_ = big::Cfg<B>;
