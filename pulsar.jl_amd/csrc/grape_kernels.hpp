@@ -641,7 +641,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
       const Blk V = expm_ah1(X, s, deg6, LA, LB, x);         // U^T
       const Blk acc = gemm_prod<1, T, N>(LA, V, LB, Tseg, x);  // U T
       if (k + 1 < it.k1) Xn = assemble<1, false, true>(P, m, k + 1, u, load_drift_t(P, m, x), x);
-      if (!P.recompute) store_blk(P.Ust + ((size_t)m * P.nt + k) * 256, V, x);
+      if (!P.recompute) store_blk_nt(P.Ust + ((size_t)m * P.nt + k) * 256, V, x);
       if constexpr (FIX) {
         if (threadIdx.x == 0) P.big[(size_t)m * P.nt + k] = 1;
       }
@@ -672,7 +672,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
       Ut.im[r] = v.y;
     }
     if (k + 1 < it.k1) Xn = assemble<B>(P, m, k + 1, u, load_drift<B>(P, m, x), x);
-    if (!P.recompute) store_blk(P.Ust + ((size_t)m * P.nt + k) * NB * 256, Ut, x);
+    if (!P.recompute) store_blk_nt(P.Ust + ((size_t)m * P.nt + k) * NB * 256, Ut, x);
     if constexpr (FIX) {
       if (threadIdx.x == 0) P.big[(size_t)m * P.nt + k] = 1;
     }
@@ -1168,7 +1168,7 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
     if (RECOMP && P.recompute)
       Ut = recompute_ut<B>(P, m, it.k0, u, LA, LB, red, x);
     else
-      Ut = load_blk(P.Ust + ((size_t)m * P.nt + it.k0) * MB, x);
+      Ut = load_blk_nt(P.Ust + ((size_t)m * P.nt + it.k0) * MB, x);
     for (int k = it.k0; k < it.k1; k++) {
       Blk Mk;
       if constexpr (B == 1)
@@ -1182,7 +1182,7 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
         if (RECOMP && P.recompute)
           Un = recompute_ut<B>(P, m, k + 1, u, LA, LB, red, x);
         else
-          Un = load_blk(P.Ust + ((size_t)m * P.nt + k + 1) * MB, x);
+          Un = load_blk_nt(P.Ust + ((size_t)m * P.nt + k + 1) * MB, x);
       }
       // C_k is contracted while M_k = C_k U_k^dag is on the matrix cores
       if (aft) contract_c(k, dtk);
@@ -1223,7 +1223,7 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
       Ut = recompute_ut_lind<B>(P, m, k, u, LA, LB, red, x);
       store_blk(uscr + (size_t)(k - it.k0) * MB, Ut, x);
     } else {
-      Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
+      Ut = load_blk_nt(P.Ust + ((size_t)m * P.nt + k) * MB, x);
     }
     Pk = mm<B, T, N>(Ut, Pk, LA, LB, x);
   }
@@ -1236,7 +1236,7 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
     } else {
       contract<B, V>(P, m, k, Mk, LA, x);
     }
-    Blk Ut = rec ? load_blk(uscr + (size_t)(k - it.k0) * MB, x) : load_blk(P.Ust + ((size_t)m * P.nt + k) * MB, x);
+    Blk Ut = rec ? load_blk(uscr + (size_t)(k - it.k0) * MB, x) : load_blk_nt(P.Ust + ((size_t)m * P.nt + k) * MB, x);
     Lm = mm<B, J, N>(Ut, Lm, LA, LB, x);          // U_k^dag Lambda_k  (U^dag = conj(U^T))
   }
 }
@@ -1331,7 +1331,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_GRA
     } else {
       // FP64 step from the FP64 image: C to the FP64 layout, M = C U^dag,
       // C' = U M, both back to the own layout
-      const Blk Ut = load_blk(P.Ust + ((size_t)m * P.nt + k) * 256, x);
+      const Blk Ut = load_blk_nt(P.Ust + ((size_t)m * P.nt + k) * 256, x);
 #pragma unroll
       for (int r = 0; r < 4; r++) LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Cm.re[r], Cm.im[r]);
       bsync<1>();
@@ -1527,7 +1527,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
   if constexpr (B == 1)
     Vn = load_blk_t(Um + (size_t)(it.k1 - 1) * MB, x);
   else
-    Vn = load_blk(Um + (size_t)(it.k1 - 1) * MB, x);
+    Vn = load_blk_nt(Um + (size_t)(it.k1 - 1) * MB, x);
   if constexpr (PACK) {
     static_assert(B == 1, "the packed walk is single-wavefront");
     const int np = P.npairs;
@@ -1570,7 +1570,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
       if constexpr (B == 1)
         Vn = load_blk_t(Um + (size_t)(k - 1) * MB, x);
       else
-        Vn = load_blk(Um + (size_t)(k - 1) * MB, x);
+        Vn = load_blk_nt(Um + (size_t)(k - 1) * MB, x);
     }
     const int c = gptr(P.ck_of)[k];
     if (c >= 0) {

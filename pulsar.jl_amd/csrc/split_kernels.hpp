@@ -139,11 +139,28 @@ __device__ inline Blk32 mm16(const AOp& a, const BOp& b) {
 // Stored image of D_k = U_k^dag - I: the split of 2^10 D_k, hi dwords then lo
 // dwords of every lane ([2][64 lanes][4 dwords], 2 KiB per member-slice)
 constexpr float SPLIT_SD = 1024.f;
+// the split images pass the caches non-temporally (written once, read once)
+#ifndef PR_NT_IMAGES
+#define PR_NT_IMAGES 2
+#endif
 __device__ inline void store_split(unsigned* __restrict__ img, const Sp& s, const Ln& x) {
   u4v hi = {s.rh.w0, s.rh.w1, s.ih.w0, s.ih.w1};
   u4v lo = {s.rl.w0, s.rl.w1, s.il.w0, s.il.w1};
+#if PR_NT_IMAGES
+  __builtin_nontemporal_store(hi, (PR_GLOBAL u4v*)img + x.lane);
+  __builtin_nontemporal_store(lo, (PR_GLOBAL u4v*)img + 64 + x.lane);
+#else
   ((PR_GLOBAL u4v*)img)[x.lane] = hi;
   ((PR_GLOBAL u4v*)img)[64 + x.lane] = lo;
+#endif
+}
+// one lane's hi (dwords 0..3) or lo (64 + lane) part of a stored split image
+__device__ inline u4v load_split_part(const unsigned* __restrict__ img, int i) {
+#if PR_NT_IMAGES > 1
+  return __builtin_nontemporal_load((const PR_GLOBAL u4v*)img + i);
+#else
+  return ((const PR_GLOBAL u4v*)img)[i];
+#endif
 }
 __device__ inline BOp load_split(const unsigned* __restrict__ img, const Ln& x, Sp& s) {
   const u4v hi = ((const PR_GLOBAL u4v*)img)[x.lane];
@@ -548,8 +565,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SPLIT_GRA
   u4v dh, dl;
   if (!rec) {
     // the first image is in flight while C_s is formed
-    dh = ((const PR_GLOBAL u4v*)(img + (size_t)it.k0 * 512))[x.lane];
-    dl = ((const PR_GLOBAL u4v*)(img + (size_t)it.k0 * 512))[64 + x.lane];
+    dh = load_split_part(img + (size_t)it.k0 * 512, x.lane);
+    dl = load_split_part(img + (size_t)it.k0 * 512, 64 + x.lane);
   }
   Blk Es = load_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + it.seg) * 256, x);
   Blk Om = load_blk(P.Om + (size_t)m * 256, x);
@@ -646,8 +663,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_SPLIT_GRA
     const H4 dih{dh.z, dh.w}, dil{dl.z, dl.w}, drh{dh.x, dh.y}, drl{dl.x, dl.y};
     const AOp aE{bD.h, bD.l, cat(negh(dih), drh), cat(negh(dil), drl)};
     if (!rec && k + 1 < it.k1) {
-      dh = ((const PR_GLOBAL u4v*)(img + (size_t)(k + 1) * 512))[x.lane];
-      dl = ((const PR_GLOBAL u4v*)(img + (size_t)(k + 1) * 512))[64 + x.lane];
+      dh = load_split_part(img + (size_t)(k + 1) * 512, x.lane);
+      dl = load_split_part(img + (size_t)(k + 1) * 512, 64 + x.lane);
     }
     Blk Mk;
     Mk.re = Cm.re + inv * __builtin_convertvector(p1.re, d4);

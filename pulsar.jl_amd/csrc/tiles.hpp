@@ -132,6 +132,32 @@ __device__ inline void store_blk(double2* __restrict__ img, const Blk& b, const 
   for (int r = 0; r < 4; r++) gstore2(img, base + r * 64 + x.lane, b.re[r], b.im[r]);
 }
 
+// The per-slice propagator images (written once by the segment kernel, read
+// once by the gradient walk, gigabytes per evaluation) go through the caches
+// non-temporally, so that they do not evict the checkpoints, controls and
+// gradient rows the kernels reuse.
+__device__ inline Blk load_blk_nt(const double2* __restrict__ img, const Ln& x) {
+  Blk b;
+  const size_t base = (size_t)x.w * 256;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const dv2 v = __builtin_nontemporal_load((const PR_GLOBAL dv2*)img + base + r * 64 + x.lane);
+    b.re[r] = v.x;
+    b.im[r] = v.y;
+  }
+  return b;
+}
+__device__ inline void store_blk_nt(double2* __restrict__ img, const Blk& b, const Ln& x) {
+  const size_t base = (size_t)x.w * 256;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    dv2 v;
+    v.x = b.re[r];
+    v.y = b.im[r];
+    __builtin_nontemporal_store(v, (PR_GLOBAL dv2*)img + base + r * 64 + x.lane);
+  }
+}
+
 // LDS tile: block blk, row, col -> padded index
 __device__ inline int lidx(int blk, int row, int col) { return blk * 272 + row * 17 + col; }
 
@@ -376,11 +402,12 @@ __device__ inline f4 mfma32(float a, float b, f4 c) {
 __device__ constexpr int rowof32(int q, int r) { return 4 * q + r; }
 
 typedef float fv2 __attribute__((ext_vector_type(2)));
+// (FP32 images of D_k: written once, read once, non-temporal like the FP64 ones)
 __device__ inline Blk32 load_blk32(const float2* __restrict__ img, const Ln& x) {
   Blk32 b;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    fv2 v = ((const PR_GLOBAL fv2*)img)[r * 64 + x.lane];
+    const fv2 v = __builtin_nontemporal_load((const PR_GLOBAL fv2*)img + r * 64 + x.lane);
     b.re[r] = v.x;
     b.im[r] = v.y;
   }
@@ -392,7 +419,7 @@ __device__ inline void store_blk32(float2* __restrict__ img, const Blk32& b, con
     fv2 v;
     v.x = b.re[r];
     v.y = b.im[r];
-    ((PR_GLOBAL fv2*)img)[r * 64 + x.lane] = v;
+    __builtin_nontemporal_store(v, (PR_GLOBAL fv2*)img + r * 64 + x.lane);
   }
 }
 
