@@ -1585,8 +1585,10 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
 }
 
 // ---------------------------------------------------------------- reductions
-// G = sum_m w_m G_m in two fixed-order stages: blockIdx.y = member chunk c sums
-// members [c*ch, (c+1)*ch) into part[c][t]; k_reduce_G2 adds the chunks.
+// G = sum_m w_m G_m in two fixed-order stages: blockIdx.y = member chunk c of
+// nch sums members [c*ch, (c+1)*ch) into part[c][t]; k_reduce_G2 adds the
+// chunks. nch = min(RED_CHUNKS, largest ensemble of the plan), fixed per plan;
+// with nch = 1 the first stage writes G itself and the second is not launched.
 constexpr int RED_CHUNKS = 16;
 
 __device__ inline int prob_of(const long long* __restrict__ off, int nprob, long long t) {
@@ -1600,25 +1602,25 @@ __device__ inline int prob_of(const long long* __restrict__ off, int nprob, long
 }
 
 __global__ void k_reduce_G1(const ProbDev* __restrict__ probs, int nprob, const long long* __restrict__ off,
-                            long long total, double* __restrict__ part) {
+                            long long total, int nch, double* __restrict__ part) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const int p = prob_of(off, nprob, t);
   const ProbDev& P = probs[p];
   const long long e = t - off[p];
   const long long ng = (long long)P.nc * P.nt;
-  const int ch = (P.M + RED_CHUNKS - 1) / RED_CHUNKS;
+  const int ch = (P.M + nch - 1) / nch;
   const int m0 = blockIdx.y * ch, m1 = min(P.M, m0 + ch);
   double s = 0;
   for (int m = m0; m < m1; m++) s += P.weight[m] * P.Gm[(size_t)m * ng + e];
   part[(size_t)blockIdx.y * total + t] = s;
 }
 
-__global__ void k_reduce_G2(long long total, const double* __restrict__ part, double* __restrict__ G) {
+__global__ void k_reduce_G2(long long total, int nch, const double* __restrict__ part, double* __restrict__ G) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
   double s = 0;
-  for (int c = 0; c < RED_CHUNKS; c++) s += part[(size_t)c * total + t];
+  for (int c = 0; c < nch; c++) s += part[(size_t)c * total + t];
   G[t] = s;
 }
 

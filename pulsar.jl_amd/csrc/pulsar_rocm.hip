@@ -347,7 +347,8 @@ struct Plan {
   double* d_controls = nullptr;
   double* d_F = nullptr;
   double* d_G = nullptr;
-  double* d_part = nullptr;  // [RED_CHUNKS][n_params] partial member sums
+  double* d_part = nullptr;  // [red_chunks][n_params] partial member sums
+  int red_chunks = 1;        // member chunks of the gradient reduction: min(RED_CHUNKS, largest M)
   double* h_stage = nullptr; // pinned host staging: n_params controls or gradients, then n_probs F
   // segment length of each group relative to the work-per-item rule (the
   // autotuner's choice; 1 otherwise)
@@ -848,12 +849,15 @@ int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
   HIPCHK(hipGetLastError());
   if (d_G) {
     long long total = pl.n_params;
-    hipLaunchKernelGGL(pr::k_reduce_G1, dim3((unsigned)((total + 255) / 256), pr::RED_CHUNKS), dim3(256), 0, st,
-                       pl.d_probs, pl.n_probs, pl.d_off, total, pl.d_part);
+    const int nch = pl.red_chunks;
+    hipLaunchKernelGGL(pr::k_reduce_G1, dim3((unsigned)((total + 255) / 256), nch), dim3(256), 0, st, pl.d_probs,
+                       pl.n_probs, pl.d_off, total, nch, nch == 1 ? d_G : pl.d_part);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(pr::k_reduce_G2, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, pl.d_part,
-                       d_G);
-    HIPCHK(hipGetLastError());
+    if (nch > 1) {
+      hipLaunchKernelGGL(pr::k_reduce_G2, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, nch,
+                         pl.d_part, d_G);
+      HIPCHK(hipGetLastError());
+    }
   }
   if ((rc = tend(pl, 3, st))) return rc;
   return PR_OK;
@@ -921,7 +925,7 @@ int static_layout(Plan& pl) {
     o.Zseg = place(z.Zseg); o.Pend = place(z.Pend); o.Lend = place(z.Lend);
   }
   pl.static_part = at;
-  at = align(at + (long long)sizeof(double) * pr::RED_CHUNKS * pl.n_params);
+  at = align(at + (long long)sizeof(double) * pl.red_chunks * pl.n_params);
   pl.static_bytes = at;
   for (const auto& g : pl.groups)
     if (g.big_scratch) pl.static_ok = false;
@@ -1261,7 +1265,12 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   if ((rc = pl.alloc(&pl.d_controls, pl.n_params))) return rc;
   if ((rc = pl.alloc(&pl.d_F, n_probs))) return rc;
   if ((rc = pl.alloc(&pl.d_G, pl.n_params))) return rc;
-  if ((rc = pl.alloc(&pl.d_part, (size_t)pr::RED_CHUNKS * pl.n_params))) return rc;
+  {
+    int mmax = 1;
+    for (int p = 0; p < n_probs; p++) mmax = std::max(mmax, probs[p].n_members);
+    pl.red_chunks = std::min(pr::RED_CHUNKS, mmax);
+  }
+  if ((rc = pl.alloc(&pl.d_part, (size_t)pl.red_chunks * pl.n_params))) return rc;
   {
     void* h = nullptr;
     const hipError_t e = hipHostMalloc(&h, sizeof(double) * (pl.n_params + pl.n_probs), hipHostMallocDefault);
