@@ -167,9 +167,9 @@ typedef struct {
                       gradient walk for FP64 problems (dim > 64 and Lindblad: from
                       segment checkpoints every 32 slices at most; Lindblad problems
                       also keep their forward states only for the work items of one
-                      launch window, 8192 / ceil(dim/16)^2 of them; PR_SPLIT16
-                      problems rebuild their split images; tracking and PR_MIXED
-                      problems keep their images); -1: recompute
+                      launch window, 8192 / ceil(dim/16)^2 of them; PR_MIXED and
+                      PR_SPLIT16 problems rebuild their reduced-precision images;
+                      tracking problems keep their images); -1: recompute
                       only when all images (and Lindblad forward states) would take
                       more than half of the free device memory */
 } pr_options;

@@ -749,7 +749,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_SEG
     const Blk none = zero_blk();
     const Blk acc = gemm_prod<1, N, N>(LA, none, LB, Tseg, x);  // U_k T (FP64)
     const size_t mk = (size_t)m * P.nt + k;
-    {
+    if (!P.recompute) {  // (without images, Udm is not allocated: the walk rebuilds D)
       Blk32 D;  // D[4q + r][c] = conj(U[c][4q + r]) - delta
 #pragma unroll
       for (int r = 0; r < 4; r++) {
@@ -1241,11 +1241,56 @@ __attribute__((amdgpu_waves_per_eu(B == 1 && !EXACT && !RECOMP ? PR_GRADIENT_WAV
   }
 }
 
+// D_k = U_k^dag - I of slice k in the FP32 own layout, rebuilt for a plan
+// without images with the arithmetic of k_segment_mx (the same assembly,
+// FP32 products, polynomial combination and transpose through the FP64 tile
+// LT), so it equals the image that kernel stores. Only for slices below the
+// degree-6 bound (the others are FP64 slices of redone items).
+__device__ inline Blk32 mixed_d(const ProbDev& P, int m, int k, const double* __restrict__ u, double2* LT,
+                                double* red, const Ln& x) {
+  const float c2 = 1.0f / 2, c3 = 1.0f / 6, c4 = 1.0f / 24, c5 = 1.0f / 120, c6 = 1.0f / 720;
+  const double2* H0o = P.H0o + (P.h0pm ? (size_t)m * 256 : 0);
+  const Blk Xo = assemble<1, true>(P, m, k, u, load_blk(H0o, x), x);
+  bool deg6;
+  (void)scaling_of<1>(Xo, red, x, deg6);
+  Blk32 X32;
+  X32.re = __builtin_convertvector(Xo.re, f4);
+  X32.im = __builtin_convertvector(Xo.im, f4);
+  const Blk32 X2 = prod32_adj(X32, X32);  // -X^2
+  const Blk32 X3 = prod32_adj(X32, X2);   // X^3
+  Blk32 Q;
+  Q.re = c6 * X3.re - c5 * X2.re + c4 * X32.re;
+  Q.im = c6 * X3.im - c5 * X2.im + c4 * X32.im;
+  add_diag32(Q, c3, x);
+  const Blk32 Pq = prod32_adj(X3, Q);  // -X^3 Q
+  Blk Uo;
+  Uo.re = Xo.re - __builtin_convertvector(Pq.re + c2 * X2.re, d4);
+  Uo.im = Xo.im - __builtin_convertvector(Pq.im + c2 * X2.im, d4);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    if (rowof32(x.q, r) == x.c) Uo.re[r] += 1.0;
+    LT[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Uo.re[r], Uo.im[r]);
+  }
+  bsync<1>();
+  Blk32 D;  // D[4q + r][c] = conj(U[c][4q + r]) - delta
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const double2 v = LT[lidx(0, x.c, rowof32(x.q, r))];
+    const double dg = (rowof32(x.q, r) == x.c) ? 1.0 : 0.0;
+    D.re[r] = (float)(v.x - dg);
+    D.im[r] = (float)(-v.y);
+  }
+  bsync<1>();
+  return D;
+}
+
 // k_gradient for d <= 16, unitary dynamics, mixed precision: the walk
 // M_k = C_k U_k^dag = C_k + C_k D_k and C_{k+1} = U_k M_k = M_k + D_k^dag M_k
 // forms the increments C D and D^dag M on FP32 MFMA and accumulates C and M in
 // FP64 (held in the FP32 own layout); the traces are FP64.
-template <int V, bool REG>
+// RECOMP: problems without images (P.recompute) rebuild D_k with mixed_d (in
+// the second LDS tile) and the U_k of FP64 slices with recompute_ut.
+template <int V, bool REG, bool RECOMP = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_GRADIENT_WAVES))) void k_gradient_mx(const ProbDev* __restrict__ probs,
                                                     const WorkItem* __restrict__ items,
                                                     const double* __restrict__ u) {
@@ -1298,10 +1343,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_GRA
       contract<1, V, true>(P, m, kk, Cm, LA, x);
   };
   if (P.after) contract_c(it.k0, dtk);
+  const bool rec = RECOMP && P.recompute;
   Blk32 D;
-  if (!(bigmask & 1ull)) D = load_blk32(P.Udm + ((size_t)m * P.nt + it.k0) * 256, x);
+  if (!rec && !(bigmask & 1ull)) D = load_blk32(P.Udm + ((size_t)m * P.nt + it.k0) * 256, x);
   for (int k = it.k0; k < it.k1; k++) {
     const bool kbig = (bigmask >> (k - kb)) & 1ull;
+    if (rec && !kbig) D = mixed_d(P, m, k, u, LB, red, x);
     if (k + 1 - kb == 64) {
       kb = k + 1;
       bigmask = __ballot(kb + x.lane < it.k1 && gptr(big)[kb + x.lane] != 0);
@@ -1317,7 +1364,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_GRA
       bsync<1>();
       const Blk32 P1 = prod32_lds(L32, D, x);  // C D
       Blk32 Dn = D;
-      if (nsmall) Dn = load_blk32(P.Udm + ((size_t)m * P.nt + k + 1) * 256, x);
+      if (!rec && nsmall) Dn = load_blk32(P.Udm + ((size_t)m * P.nt + k + 1) * 256, x);
       Mk.re = Cm.re + __builtin_convertvector(P1.re, d4);
       Mk.im = Cm.im + __builtin_convertvector(P1.im, d4);
       Blk32 M32;
@@ -1331,7 +1378,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_GRA
     } else {
       // FP64 step from the FP64 image: C to the FP64 layout, M = C U^dag,
       // C' = U M, both back to the own layout
-      const Blk Ut = load_blk_nt(P.Ust + ((size_t)m * P.nt + k) * 256, x);
+      const Blk Ut = rec ? recompute_ut<1>(P, m, k, u, LA, LB, red, x)
+                         : load_blk_nt(P.Ust + ((size_t)m * P.nt + k) * 256, x);
 #pragma unroll
       for (int r = 0; r < 4; r++) LA[lidx(0, rowof32(x.q, r), x.c)] = make_double2(Cm.re[r], Cm.im[r]);
       bsync<1>();
@@ -1363,7 +1411,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PR_MIXED_GRA
         Cm.im[r] = v.y;
       }
       bsync<1>();
-      if (nsmall) D = load_blk32(P.Udm + ((size_t)m * P.nt + k + 1) * 256, x);
+      if (!rec && nsmall) D = load_blk32(P.Udm + ((size_t)m * P.nt + k + 1) * 256, x);
     }
     if (P.after) {
       // C_{k+1} for the next slice, from the FP64 accumulators just formed

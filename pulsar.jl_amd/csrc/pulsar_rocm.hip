@@ -622,11 +622,18 @@ int launch_gradient_split(Plan& pl, const Plan::Group& g, const double* d_u, hip
 
 int launch_gradient_mixed(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t st) {
   if (g.split) return launch_gradient_split(pl, g, d_u, st);
-  const size_t sm = pr::Smem<1>::gradient_unitary;
+  // the recomputing walk rebuilds D_k in a second tile
+  const size_t sm = g.recompute ? pr::Smem<1>::segment : pr::Smem<1>::gradient_unitary;
   switch (g.V) {
 #define PR_MCASE(VV)                                                                                    \
   case VV:                                                                                              \
-    if (g.trace_regs)                                                                                   \
+    if (g.recompute && g.trace_regs)                                                                    \
+      hipLaunchKernelGGL((pr::k_gradient_mx<VV, true, true>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, \
+                         g.d_items, d_u);                                                              \
+    else if (g.recompute)                                                                               \
+      hipLaunchKernelGGL((pr::k_gradient_mx<VV, false, true>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, \
+                         g.d_items, d_u);                                                              \
+    else if (g.trace_regs)                                                                              \
       hipLaunchKernelGGL((pr::k_gradient_mx<VV, true>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u); \
     else                                                                                                \
       hipLaunchKernelGGL((pr::k_gradient_mx<VV, false>), dim3(g.n_items), dim3(64), sm, st, pl.d_probs, g.d_items, d_u); \
@@ -974,12 +981,12 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   // forward states and recomputed U_k of a bounded number of work items at a
   // time (segments of at most LIND_RECOMPUTE_SEG slices, at most
   // lind_window(B) work items per launch).
-  // The split form (PR_SPLIT16) recomputes its D_k images in the walk as well;
-  // the FP32 form keeps them.
+  // The reduced-precision forms (PR_MIXED, PR_SPLIT16) rebuild their D_k
+  // images in the walk as well.
   auto can_recompute = [&](const pr_problem& q) {
     if (q.objective == PR_TRACKING) return false;
     if (q.dynamics != PR_CLOSED) return q.dim <= 64;
-    return !(q.dim <= 16 && ((mixed1 && pl.opt.precision != PR_SPLIT16) || lindblad1));
+    return !(q.dim <= 16 && lindblad1);
   };
   bool recompute = pl.opt.store_props == 0;
   if (pl.opt.store_props < 0) {

@@ -218,11 +218,11 @@ def test_some_slices_above_the_bound_redo_their_segments_in_fp64(prec):
         assert dF <= TOL_F and dG <= TOL_G, (seg, dF, dG)
 
 
-def _split_pair(seg, probs, us):
-    """(F, G, exp counts) of the split-form plan with stored images, then without."""
+def _split_pair(seg, probs, us, precision=PR_SPLIT16):
+    """(F, G, exp counts) of the reduced-precision plan with stored images, then without."""
     out = []
     for sp in (1, 0):
-        plan = native.Plan(probs, precision=PR_SPLIT16, store_props=sp, segment=seg)
+        plan = native.Plan(probs, precision=precision, store_props=sp, segment=seg)
         try:
             F, G = plan.eval(us)
             out.append((F, G, plan.exp_counts()))
@@ -232,11 +232,13 @@ def _split_pair(seg, probs, us):
 
 
 @pytest.mark.parametrize("case", ["ensemble", "spikes", "mixed_tables"])
-def test_split_recompute_matches_stored_images(case):
-    # store_props = 0 in the split form: the walk rebuilds every D_k with the
-    # segment kernel's own assembly and split exp (k_gradient_h<..., RECOMP>),
-    # and items redone in FP64 rebuild U_k in the FP64 walk; F and G equal the
-    # stored-image plan's bit for bit at the same segments
+@pytest.mark.parametrize("form", [PR_SPLIT16, PR_MIXED], ids=["split16", "fp32"])
+def test_reduced_precision_recompute_matches_stored_images(case, form):
+    # store_props = 0 in a reduced-precision form: the walk rebuilds every D_k
+    # with the segment kernel's own arithmetic (split: k_gradient_h<..., RECOMP>;
+    # FP32: mixed_d in k_gradient_mx<..., RECOMP>), and the FP64 slices of
+    # redone items rebuild U_k; F and G equal the stored-image plan's bit for
+    # bit at the same segments
     if case == "ensemble":
         p, u = W.four_qubit_ensemble(n_t=300)
         p, u = W.shrink(p, u, n_members=8)
@@ -261,8 +263,8 @@ def test_split_recompute_matches_stored_images(case):
                                  objective=PR_GATE, n_members=2, ctrl_scale=rng.uniform(0.9, 1.1, (2, nc)),
                                  U_target=W.haar_unitary(d, rng), **kw))
             us.append(rng.uniform(-0.5, 0.5, (nc, 200)))
-    print(f"case {case}", flush=True)
-    (F1, G1, D1), (F0, G0, D0) = _split_pair(23, probs, us)
+    print(f"case {case} form {form}", flush=True)
+    (F1, G1, D1), (F0, G0, D0) = _split_pair(23, probs, us, precision=form)
     for q, uu, a, b, ga, gb in zip(probs, us, F0, F1, G0, G1):
         assert a == b
         assert np.array_equal(ga, gb)
@@ -272,7 +274,7 @@ def test_split_recompute_matches_stored_images(case):
     assert np.array_equal(D0, D1)
     if case == "spikes":
         assert D0[0][3] > 0  # member-slices redone in FP64
-    else:
+    elif form == PR_SPLIT16:  # (the FP32 form's tighter bound redoes some items of mixed_tables)
         assert D0[:, 3].sum() == 0
 
 

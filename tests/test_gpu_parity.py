@@ -567,10 +567,10 @@ def test_recomputed_propagators_match_stored(d, objective, grad_mode):
     assert np.abs(G0 - Go).max() <= TOL_G * np.abs(Go).max()
 
 
-def test_recompute_keeps_images_where_it_does_not_apply():
-    # under store_props = 0 the mixed-precision d <= 16 problems keep their
-    # images; the closed d = 20 problem and the Lindblad problem beside them
-    # recompute. Results equal the stored-image plans.
+def test_recompute_with_other_problems_in_the_plan():
+    # under store_props = 0 the closed d = 20 problem, the Lindblad problem and
+    # the FP32-form d <= 16 problem all recompute; results equal the
+    # stored-image plans
     from pulsar_rocm.problem import PR_MIXED
     lind = Problem(dim=4, n_ctrl=2, n_t=4, H0=FIX["lind_L0"], Hc=FIX["lind_Lc"], dt_slices=FIX["lind_dts"],
                    dynamics=PR_LINDBLAD, objective=PR_PAIRS_SQUARE, n_members=1,
