@@ -318,15 +318,17 @@ def autotuned_timing(probs, us, args, prec):
             plan.eval_device(d_u, d_F, d_G)
         plan.synchronize()
         el = time.perf_counter() - t0
-        D = plan.eval(us, diag=True)[2]
+        S = plan.shapes()
     finally:
         plan.close()
-    seglen = {}
-    for q, dd in zip(probs, D):
-        seglen.setdefault(q.dim, set()).add(int(np.ceil(q.n_t / dd[3])))
+    seglen, regs = {}, {}
+    for q, sh in zip(probs, S):
+        seglen.setdefault(q.dim, set()).add(int(sh[0]))
+        regs.setdefault(q.dim, set()).add(int(sh[2]))
     return {"evals_per_s": round(args.steps / el, 3), "ms_per_eval": round(1e3 * el / args.steps, 4),
             "plan_create_s": round(create_s, 3),
-            "slices_per_segment_by_dim": {int(k): sorted(v)[:3] for k, v in sorted(seglen.items())}}
+            "slices_per_segment_by_dim": {int(k): sorted(v)[:3] for k, v in sorted(seglen.items())},
+            "trace_operand_in_registers_by_dim": {int(k): sorted(v) for k, v in sorted(regs.items())}}
 
 
 def recompute_timing(probs, us, args, prec=PR_FP64):

@@ -160,9 +160,11 @@ typedef struct {
   int grad_mode;   /* PR_GRAD_REFERENCE or PR_GRAD_EXACT */
   int segment;     /* slices per time segment of the parallel chain scan; 0 = auto (about
                       8192 equal work items per dimension group); PR_SEGMENT_AUTOTUNE =
-                      measure 0.5, 1, 2 and 4 times the auto length for every dimension
-                      group at plan creation and keep the fastest (no reference
-                      counterpart: the reference runs slices one after another) */
+                      measure 0.5, 1, 2 and 4 times the auto length, each with the
+                      single-wavefront kernels' trace operand and split assembly tables
+                      in registers and read from memory, for every dimension group at
+                      plan creation and keep the fastest (no reference counterpart: the
+                      reference runs slices one after another) */
   int store_props; /* 1: keep U_k in HBM between the passes; 0: recompute U_k in the
                       gradient walk for FP64 problems (dim > 64 and Lindblad: from
                       segment checkpoints every 32 slices at most; Lindblad problems
@@ -246,6 +248,15 @@ long long pr_plan_n_params(void* plan);
  * the member-slices a reduced-precision plan redid in FP64. Waits for the
  * plan's stream. Tracking problems report zeros. */
 int pr_plan_exp_counts(void* plan, long long* counts);
+
+/* Kernel shapes the plan uses (no counterpart in the reference; it reports
+ * the choices of the segment rule or of PR_SEGMENT_AUTOTUNE): per problem p,
+ * shapes[4p] slices per time segment, shapes[4p + 1] segments per member,
+ * shapes[4p + 2] 1 when the single-wavefront gradient walk holds the trace
+ * operand in registers (0: read from LDS), shapes[4p + 3] the assembly passes
+ * the split segment kernel holds in registers (0: general assembly from
+ * memory, or not a split problem). */
+int pr_plan_shapes(void* plan, int* shapes);
 
 /* Per-kernel timing (no counterpart in the reference, which has no device
  * profiling): when enabled, every evaluation records a HIP event pair on its

@@ -353,6 +353,10 @@ struct Plan {
   // segment length of each group relative to the work-per-item rule (the
   // autotuner's choice; 1 otherwise)
   std::vector<double> seg_scale = std::vector<double>(13, 1.0);
+  // per group (autotuner's choice; 1 otherwise): 1 lets the d <= 16 kernels
+  // hold the trace operand and the split assembly tables in registers, 0
+  // makes them read both from memory (fewer registers, more loads)
+  std::vector<int> tune_regs = std::vector<int>(13, 1);
   // pr_eval's staging ranges: problems [stage_cut[r], stage_cut[r + 1]) are
   // packed by thread r and copied by one transfer each, event stage_ev[r]
   // marking the range's gradient download
@@ -1313,7 +1317,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
         const int n = D.so_idx ? D.so_pass[1] - D.so_pass[0] : 0;
         np = (np < 0 || np == n) ? n : 0;
       }
-      g.split_np = (np >= 1 && np <= 3) ? np : 0;
+      g.split_np = (pl.tune_regs[gi] && np >= 1 && np <= 3) ? np : 0;
     }
     for (int p = 0; p < n_probs; p++)
       if (group_of(probs[p]) == gi && pl.host_probs[p].recompute) {
@@ -1321,7 +1325,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
         if (!pl.host_probs[p].unitary) g.window = lind_window(B);
       }
     // every control fits one lane group of 64 / V lanes with <= TR_REG nonzeros per lane
-    g.trace_regs = B == 1 && vmax[gi] >= nc_max[gi] && tr_nnz_max[gi] <= pr::TR_REG * (64 / vmax[gi]);
+    g.trace_regs = pl.tune_regs[gi] && B == 1 && vmax[gi] >= nc_max[gi] &&
+                   tr_nnz_max[gi] <= pr::TR_REG * (64 / vmax[gi]);
     if ((rc = pl.upload(&g.d_items, items[gi].data(), items[gi].size()))) return rc;
     if ((rc = pl.upload(&g.d_members, members[gi].data(), members[gi].size()))) return rc;
     pl.groups.push_back(g);
@@ -1368,13 +1373,15 @@ int fill_diag(const Plan& pl, double* diag) {
 
 }  // namespace
 
-// The segment-length autotuner (pr_options.segment = PR_SEGMENT_AUTOTUNE): for
-// each candidate multiple of the work-per-item segment length, a trial plan of
-// the same problems is built and its dimension groups are timed one by one
-// (HIP events around each group's launches: 1 warm-up and 3 timed
-// evaluations at zero controls); every group then takes the candidate it ran
-// fastest with. The final plan is built with those lengths; the trial plans
-// are freed before it is.
+// The kernel-shape autotuner (pr_options.segment = PR_SEGMENT_AUTOTUNE): for
+// each candidate (a multiple of the work-per-item segment length, times the
+// operand placement of the single-wavefront kernels: trace operand and split
+// assembly tables in registers or read from memory) a trial plan of the same
+// problems is built and its dimension groups are timed one by one (HIP events
+// around each group's launches: 1 warm-up and 3 timed evaluations at zero
+// controls); every group then takes the candidate it ran fastest with. The
+// final plan is built with those choices; the trial plans are freed before it
+// is.
 int time_groups(Plan& pl, std::vector<double>& ms) {
   ms.assign(13, 0.0);
   HIPCHK(hipMemsetAsync(pl.d_controls, 0, sizeof(double) * pl.n_params, pl.stream));
@@ -1407,24 +1414,34 @@ int time_groups(Plan& pl, std::vector<double>& ms) {
 int autotune_segments(Plan& pl, const pr_problem* probs, int n_probs) {
   const double cand[] = {0.5, 1.0, 2.0, 4.0};
   std::vector<double> best(13, 1e300), scale(13, 1.0), ms;
-  for (double c : cand) {
-    Plan* t = new Plan;
-    t->opt = pl.opt;
-    t->use_graphs = false;
-    t->seg_scale.assign(13, c);
-    hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
-    int rc = e == hipSuccess ? PR_OK : fail(PR_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
-    if (rc == PR_OK) rc = build_plan(*t, probs, n_probs);
-    if (rc == PR_OK) rc = time_groups(*t, ms);
-    delete t;
-    if (rc) return rc;
-    for (int gi = 0; gi < 13; gi++)
-      if (ms[gi] > 0 && ms[gi] < best[gi]) {
-        best[gi] = ms[gi];
-        scale[gi] = c;
-      }
+  std::vector<int> regs(13, 1);
+  // the memory placement is only a different kernel for groups of single-
+  // wavefront problems (block count 1, plain or tracking)
+  bool small = false;
+  for (int p = 0; p < n_probs; p++) small = small || probs[p].dim <= 16;
+  for (int r = 1; r >= (small ? 0 : 1); r--) {
+    for (double c : cand) {
+      Plan* t = new Plan;
+      t->opt = pl.opt;
+      t->use_graphs = false;
+      t->seg_scale.assign(13, c);
+      t->tune_regs.assign(13, r);
+      hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+      int rc = e == hipSuccess ? PR_OK : fail(PR_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+      if (rc == PR_OK) rc = build_plan(*t, probs, n_probs);
+      if (rc == PR_OK) rc = time_groups(*t, ms);
+      delete t;
+      if (rc) return rc;
+      for (int gi = 0; gi < 13; gi++)
+        if (ms[gi] > 0 && ms[gi] < best[gi]) {
+          best[gi] = ms[gi];
+          scale[gi] = c;
+          regs[gi] = r;
+        }
+    }
   }
   pl.seg_scale = scale;
+  pl.tune_regs = regs;
   return PR_OK;
 }
 
@@ -1748,6 +1765,26 @@ int pr_plan_exp_counts(void* plan, long long* counts) {
     counts[4 * p + 1] = n9;
     counts[4 * p + 2] = nsq;
     counts[4 * p + 3] = nredo;
+  }
+  return PR_OK;
+}
+
+int pr_plan_shapes(void* plan, int* shapes) {
+  if (!plan || !shapes) return fail(PR_ERR_ARG, "pr_plan_shapes: NULL argument");
+  const Plan& pl = *static_cast<Plan*>(plan);
+  auto group_of = [](const pr::ProbDev& D) { return (D.obj == PR_TRACKING ? 8 : 0) + D.B; };
+  for (int p = 0; p < pl.n_probs; p++) {
+    const pr::ProbDev& D = pl.host_probs[p];
+    int regs = 0, np = 0;
+    for (const auto& g : pl.groups)
+      if (g.gid == group_of(D)) {
+        regs = g.trace_regs ? 1 : 0;
+        np = g.split ? g.split_np : 0;
+      }
+    shapes[4 * p + 0] = D.seglen;
+    shapes[4 * p + 1] = D.nseg;
+    shapes[4 * p + 2] = regs;
+    shapes[4 * p + 3] = np;
   }
   return PR_OK;
 }

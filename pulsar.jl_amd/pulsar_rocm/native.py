@@ -27,7 +27,7 @@ PR_ERR = {
 }
 
 EXPORTS = ("pr_abi_version", "pr_device_count", "pr_init", "pr_last_error", "pr_plan_create",
-           "pr_eval", "pr_eval_device", "pr_eval_device_static", "pr_plan_n_params", "pr_plan_exp_counts", "pr_plan_set_timing",
+           "pr_eval", "pr_eval_device", "pr_eval_device_static", "pr_plan_n_params", "pr_plan_exp_counts", "pr_plan_shapes", "pr_plan_set_timing",
            "pr_plan_kernel_times", "pr_plan_buffers", "pr_plan_upload_controls",
            "pr_plan_synchronize", "pr_plan_destroy", "pr_grape_optimize", "pr_lbfgs_optimize", "pr_mr_optimcon")
 KERNEL_CLASSES = ("segment", "combine", "gradient", "reduce")
@@ -100,6 +100,7 @@ def load(path: str = LIB_PATH):
     L.pr_plan_n_params.restype = ctypes.c_longlong
     L.pr_eval_device_static.argtypes = [ctypes.c_void_p] * 5
     L.pr_plan_exp_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+    L.pr_plan_shapes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     L.pr_plan_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.pr_plan_kernel_times.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(ctypes.c_longlong)]
     L.pr_plan_buffers.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_void_p)] * 3
@@ -228,6 +229,14 @@ class Plan:
         n = len(self.problems)
         out = (ctypes.c_longlong * (4 * n))()
         check(load().pr_plan_exp_counts(self._h, out))
+        return np.array(out[:], dtype=np.int64).reshape(n, 4)
+
+    def shapes(self) -> np.ndarray:
+        """[n_probs, 4]: slices per segment, segments per member, trace operand in
+        registers (1/0), split assembly passes in registers (pr_plan_shapes)."""
+        n = len(self.problems)
+        out = (ctypes.c_int * (4 * n))()
+        check(load().pr_plan_shapes(self._h, out))
         return np.array(out[:], dtype=np.int64).reshape(n, 4)
 
     def kernel_times(self) -> dict:

@@ -299,10 +299,18 @@ def test_segment_autotuner_plan_matches_oracle():
     try:
         F, G, D = plan.eval(us, diag=True)
         F2, G2 = plan.eval(us)
+        S = plan.shapes()
     finally:
         plan.close()
     assert np.array_equal(F, F2) and all(np.array_equal(a, b) for a, b in zip(G, G2))
     print("autotuned segments per problem", D[:, 3].astype(int).tolist(), flush=True)
+    print("shapes (slices/segment, segments, trace regs, split passes)", S.tolist(), flush=True)
+    # the reported shape is the plan's: segments per member as in the diagnostics,
+    # and segments x slices covering each problem's slices
+    assert np.array_equal(S[:, 1], D[:, 3].astype(int))
+    for (p, _), sh in zip(batch, S):
+        assert sh[0] * (sh[1] - 1) < p.n_t <= sh[0] * sh[1]
+        assert sh[2] in (0, 1) and (p.dim <= 16 or sh[2] == 0) and sh[3] == 0
     for i, (p, u) in enumerate(batch):
         F0, G0 = O.evaluate(p, u)
         assert abs(F[i] - F0) <= TOL_F, (i, p.dim)
