@@ -54,7 +54,7 @@ HBM_PEAK_GBS = 8000.0
 # k_track_walk (four_qubit_tracking) reads the FP64 images once too
 # (profiles/grape_four_qubit_tracking_hbm_traffic.md).
 MEASURED_TRAFFIC_BYTES = {("fp64", "k_segment"): 0.022e9 + 2.131e9, ("fp64", "k_gradient"): 2.141e9 + 0.037e9,
-                          ("split16", "k_segment"): 0.012e9 + 1.084e9, ("split16", "k_gradient"): 1.092e9 + 0.034e9,
+                          ("split16", "k_segment"): 0.010e9 + 1.083e9, ("split16", "k_gradient"): 1.092e9 + 0.033e9,
                           "k_track_walk": 2.166e9 + 0.040e9}
 
 
