@@ -261,7 +261,7 @@ def other_workload(args):
             "python_mirror_eval_ms_per_eval": round(1e3 * host_elapsed / args.steps, 4)}
     if prec == PR_FP64 and args.workload != "four_qubit_tracking":
         line["roofline"] = fp64_roofline(probs, counts, kt, args.steps)
-    if args.workload in ("spin_chain_128", "three_qubit_lindblad"):
+    if args.workload in ("spin_chain_128", "three_qubit_lindblad", "four_qubit_tracking"):
         line["recompute_mode"] = recompute_timing(probs, us, args)
     if args.workload == "uneven_batch":
         line["autotuned"] = autotuned_timing(probs, us, args, prec)

@@ -77,7 +77,7 @@ extern "C" {
                              its gradient is that kernel's co-state formula
                              -dt Re(i Tr(Lambda^dag [s_mj A_j, rho_{n+1}])) (:238-248).
                              Closed dynamics, dim <= 64, PR_GRAD_REFERENCE; runs FP64
-                             with stored U_k images whatever precision/store_props say */
+                             whatever precision says */
 #define PR_STATE 5        /* closed dynamics: F_m as PR_PAIRS_SQUARE; gradient of
                              _state_gradient_into! (Gradients.jl:357-399), lambda_k = Q_k psi_t
                              with slice k included: 2 Re(conj(z) (-i dt) <lambda_k|A_j|P_k psi_i>)
@@ -170,8 +170,8 @@ typedef struct {
                       segment checkpoints every 32 slices at most; Lindblad problems
                       also keep their forward states only for the work items of one
                       launch window, 8192 / ceil(dim/16)^2 of them; PR_MIXED and
-                      PR_SPLIT16 problems rebuild their reduced-precision images;
-                      tracking problems keep their images); -1: recompute
+                      PR_SPLIT16 problems rebuild their reduced-precision images,
+                      tracking problems their U_k); -1: recompute
                       only when all images (and Lindblad forward states) would take
                       more than half of the free device memory */
 } pr_options;

@@ -1552,10 +1552,12 @@ __device__ inline Blk outer_pairs(const double2* LZ, int np, const Ln& x) {
 // loaded transposed from its U_k^T image, so the U_k^dag products take both
 // operands from registers (in the packed walk only Z goes through LDS, for M);
 // REG keeps the trace operand in registers for the whole walk.
-template <int B, int V, bool PACK = false, bool REG = false>
-__global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B == 1 ? 4 : 0))) void k_track_walk(
-    const ProbDev* __restrict__ probs,
-                                                           const WorkItem* __restrict__ items) {
+// RECOMP: problems without images (P.recompute) rebuild U_k^T with
+// recompute_ut, the segment kernel's calls (for B == 1 transposed to U_k
+// through the tile, as the transposed image load gives it).
+template <int B, int V, bool PACK = false, bool REG = false, bool RECOMP = false>
+__global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B == 1 && !RECOMP ? 4 : 0))) void k_track_walk(
+    const ProbDev* __restrict__ probs, const WorkItem* __restrict__ items, const double* __restrict__ u) {
   const WorkItem it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
   const int m = it.member;
@@ -1571,11 +1573,32 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
   // U_k^dag as an A operand: op C on U_k in registers (B == 1), else op J on
   // the U_k^T block through LDS
   constexpr int OU = B == 1 ? C : J;
-  Blk Vn;
-  if constexpr (B == 1)
-    Vn = load_blk_t(Um + (size_t)(it.k1 - 1) * MB, x);
-  else
-    Vn = load_blk_nt(Um + (size_t)(it.k1 - 1) * MB, x);
+  // U_k (B == 1) or its U_k^T block (B > 1), from the image or rebuilt
+  auto fetch_u = [&](int k) -> Blk {
+    if (RECOMP && P.recompute) {
+      const Blk Ut = recompute_ut<B>(P, m, k, u, LA, LB, red, x);
+      if constexpr (B > 1) {
+        return Ut;
+      } else {
+        put<1>(LA, Ut, x);
+        bsync<1>();
+        Blk Uk;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const double2 v = LA[lidx(0, x.c, rowof(x.q, r))];  // U[q + 4r][c] = U^T[c][q + 4r]
+          Uk.re[r] = v.x;
+          Uk.im[r] = v.y;
+        }
+        bsync<1>();
+        return Uk;
+      }
+    }
+    if constexpr (B == 1)
+      return load_blk_t(Um + (size_t)k * MB, x);
+    else
+      return load_blk_nt(Um + (size_t)k * MB, x);
+  };
+  Blk Vn = fetch_u(it.k1 - 1);
   if constexpr (PACK) {
     static_assert(B == 1, "the packed walk is single-wavefront");
     const int np = P.npairs;
@@ -1592,7 +1615,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
       const Blk Uk = Vn;  // U_k
       const double dtk = dtn;
       if (k > it.k0) {
-        Vn = load_blk_t(Um + (size_t)(k - 1) * MB, x);
+        Vn = fetch_u(k - 1);
         dtn = gptr(P.dts)[k - 1];
       }
       const int c = gptr(P.ck_of)[k];
@@ -1614,12 +1637,7 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
   }
   for (int k = it.k1 - 1; k >= it.k0; k--) {
     const Blk Uk = Vn;
-    if (k > it.k0) {
-      if constexpr (B == 1)
-        Vn = load_blk_t(Um + (size_t)(k - 1) * MB, x);
-      else
-        Vn = load_blk_nt(Um + (size_t)(k - 1) * MB, x);
-    }
+    if (k > it.k0) Vn = fetch_u(k - 1);
     const int c = gptr(P.ck_of)[k];
     if (c >= 0) {
       const Blk Y = mm<B, N, N>(load_blk(P.Ck + (size_t)c * MB, x), S, LA, LB, x);

@@ -533,7 +533,12 @@ int set_smem_attr() {
     for (const void* f : {(const void*)pr::k_segment<B, true, false, true>, (const void*)pr::k_track_scan<B>,
                           (const void*)pr::k_track_walk<B, 2>, (const void*)pr::k_track_walk<B, 4>,
                           (const void*)pr::k_track_walk<B, 8>, (const void*)pr::k_track_walk<B, 16>,
-                          (const void*)pr::k_track_walk<B, 32>})
+                          (const void*)pr::k_track_walk<B, 32>,
+                          (const void*)pr::k_track_walk<B, 2, false, false, true>,
+                          (const void*)pr::k_track_walk<B, 4, false, false, true>,
+                          (const void*)pr::k_track_walk<B, 8, false, false, true>,
+                          (const void*)pr::k_track_walk<B, 16, false, false, true>,
+                          (const void*)pr::k_track_walk<B, 32, false, false, true>})
       HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   }
   return PR_OK;
@@ -774,15 +779,24 @@ int launch_track(Plan& pl, const Plan::Group& g, const double* d_u, hipStream_t 
   switch (g.V) {
 #define PR_TCASE(VV)                                                                                                 \
   case VV:                                                                                                           \
-    if (B == 1 && g.track_packed && g.trace_regs)                                                                    \
+    if (B == 1 && g.track_packed && g.trace_regs && g.recompute)                                                     \
+      hipLaunchKernelGGL((pr::k_track_walk<1, VV, true, true, true>), dim3(g.n_items), blk,                         \
+                         pr::Smem<1>::base, st, pl.d_probs, g.d_items, d_u);                                          \
+    else if (B == 1 && g.track_packed && g.recompute)                                                                \
+      hipLaunchKernelGGL((pr::k_track_walk<1, VV, true, false, true>), dim3(g.n_items), blk,                        \
+                         pr::Smem<1>::base, st, pl.d_probs, g.d_items, d_u);                                          \
+    else if (g.recompute)                                                                                            \
+      hipLaunchKernelGGL((pr::k_track_walk<B, VV, false, false, true>), dim3(g.n_items), blk, pr::Smem<B>::base, st, \
+                         pl.d_probs, g.d_items, d_u);                                                                 \
+    else if (B == 1 && g.track_packed && g.trace_regs)                                                               \
       hipLaunchKernelGGL((pr::k_track_walk<1, VV, true, true>), dim3(g.n_items), blk,                               \
-                         pr::Smem<1>::base, st, pl.d_probs, g.d_items);                                               \
+                         pr::Smem<1>::base, st, pl.d_probs, g.d_items, d_u);                                          \
     else if (B == 1 && g.track_packed)                                                                               \
       hipLaunchKernelGGL((pr::k_track_walk<1, VV, true>), dim3(g.n_items), blk,                                     \
-                         pr::Smem<1>::base, st, pl.d_probs, g.d_items);                                               \
+                         pr::Smem<1>::base, st, pl.d_probs, g.d_items, d_u);                                          \
     else                                                                                                             \
       hipLaunchKernelGGL((pr::k_track_walk<B, VV>), dim3(g.n_items), blk, pr::Smem<B>::base, st, pl.d_probs,         \
-                         g.d_items);                                                                                 \
+                         g.d_items, d_u);                                                                            \
     break;
     PR_TCASE(2) PR_TCASE(4) PR_TCASE(8) PR_TCASE(16) PR_TCASE(32)
 #undef PR_TCASE
@@ -988,8 +1002,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   // The reduced-precision forms (PR_MIXED, PR_SPLIT16) rebuild their D_k
   // images in the walk as well.
   auto can_recompute = [&](const pr_problem& q) {
-    if (q.objective == PR_TRACKING) return false;
-    if (q.dynamics != PR_CLOSED) return q.dim <= 64;
+    if (q.dynamics != PR_CLOSED) return q.objective != PR_TRACKING && q.dim <= 64;
+    if (q.objective == PR_TRACKING) return q.dim <= 64;
     return !(q.dim <= 16 && lindblad1);
   };
   bool recompute = pl.opt.store_props == 0;
@@ -997,7 +1011,6 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     // images, and the forward states of Lindblad problems (as large again)
     double bytes = 0;
     for (int p = 0; p < n_probs; p++) {
-      if (probs[p].objective == PR_TRACKING) continue;
       const double nb = (double)((probs[p].dim + 15) / 16) * ((probs[p].dim + 15) / 16);
       bytes += (probs[p].dynamics == PR_CLOSED ? 1.0 : 2.0) * (double)probs[p].n_members * probs[p].n_t * nb * 256 *
                sizeof(double2);

@@ -79,13 +79,36 @@ def test_tracking_every_slice_a_checkpoint():
     check(p, u)
 
 
-def test_tracking_ignores_precision_and_store_props_options():
-    # tracking runs FP64 with stored images whatever the plan options say
+def test_tracking_ignores_precision_and_follows_store_props_options():
+    # tracking runs FP64 whatever the precision option says; store_props = 0
+    # rebuilds U_k in the walk, other segments only reorder the chain sums
     p, u = tracking_case(16, 2, 4, 24, 3, n_ck=4, seed=5)
     F, G = run(p, u)
     for kw in ({"precision": PR_MIXED}, {"store_props": 0}, {"segment": 5}):
         F2, G2 = run(p, u, **kw)
         assert abs(F2 - F) <= 1e-14 and np.abs(G2 - G).max() <= 1e-13 * np.abs(G).max(), kw
+
+
+@pytest.mark.parametrize("d,M,nc,nt,npairs", [(4, 3, 3, 30, 2), (16, 3, 8, 40, 4), (16, 2, 4, 30, 12),
+                                               (24, 2, 4, 20, 5), (64, 1, 2, 8, 3)])
+def test_tracking_recompute_matches_stored_images(d, M, nc, nt, npairs):
+    # store_props = 0: no U_k images; the tracking walk rebuilds each U_k with
+    # the segment kernel's own calls (packed and unpacked single-wavefront
+    # walks, 2 x 2 and 4 x 4 blocks), so F and G equal the stored-image plan's
+    # bit for bit at the same segments, and the oracle's
+    p, u = tracking_case(d, M, nc, nt, npairs, n_ck=5, seed=d + nt + 1)
+    print(f"tracking recompute d={d} M={M} n_t={nt} pairs={npairs}", flush=True)
+    out = []
+    for sp in (1, 0):
+        plan = native.Plan([p], store_props=sp, segment=7)
+        try:
+            out.append(plan.eval([u]))
+        finally:
+            plan.close()
+    (F1, G1), (F0, G0) = out
+    assert F0[0] == F1[0]
+    assert np.array_equal(G0[0], G1[0])
+    check(p, u, store_props=0)
 
 
 def test_tracking_beside_a_gate_problem_in_one_plan():
