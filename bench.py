@@ -259,6 +259,11 @@ def other_workload(args):
             "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
             "pr_eval_host_buffers_ms_per_eval": round(abi_ms, 4),
             "python_mirror_eval_ms_per_eval": round(1e3 * host_elapsed / args.steps, 4)}
+    if args.workload == "uneven_batch":
+        # the same batch without packing small problems (pr_options.pack = 0);
+        # its per-problem exp counts are the work model's (a pack reports its
+        # exps once, for all of its problems)
+        line["unpacked"], counts = unpacked_timing(probs, us, args, prec)
     if prec == PR_FP64 and args.workload != "four_qubit_tracking":
         line["roofline"] = fp64_roofline(probs, counts, kt, args.steps)
     if args.workload in ("spin_chain_128", "three_qubit_lindblad", "four_qubit_tracking"):
@@ -297,6 +302,37 @@ def other_workload(args):
                                 "kind": "port", "sample": f"{n} of {p.n_members} members, all {p.n_t} slices, "
                                 f"{cpu_s:.1f} s; rate scaled to the full ensemble"}
     print(json.dumps(line))
+
+
+def unpacked_timing(probs, us, args, prec):
+    """The batch with every problem its own work (pr_options.pack = 0):
+    evaluations/s, per-kernel times, and the per-problem exp counts."""
+    plan = native.Plan(probs, precision=prec, segment=args.segment, pack=0)
+    try:
+        plan.upload_controls(us)
+        d_u, d_F, d_G = plan.buffers()
+        for _ in range(args.warmup):
+            plan.eval_device(d_u, d_F, d_G)
+        plan.synchronize()
+        plan.set_timing(True)
+        plan.kernel_times()
+        for _ in range(args.steps):
+            plan.eval_device(d_u, d_F, d_G)
+        plan.synchronize()
+        kt = plan.kernel_times()
+        plan.set_timing(False)
+        counts = plan.exp_counts()
+        plan.eval_device(d_u, d_F, d_G)
+        plan.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.eval_device(d_u, d_F, d_G)
+        plan.synchronize()
+        el = time.perf_counter() - t0
+    finally:
+        plan.close()
+    return ({"evals_per_s": round(args.steps / el, 3), "ms_per_eval": round(1e3 * el / args.steps, 4),
+             "kernel_ms_per_eval": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}}, counts)
 
 
 def autotuned_timing(probs, us, args, prec):

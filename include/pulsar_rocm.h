@@ -48,7 +48,7 @@
 extern "C" {
 #endif
 
-#define PR_ABI_VERSION 1
+#define PR_ABI_VERSION 2
 
 /* status codes */
 #define PR_OK 0
@@ -174,9 +174,19 @@ typedef struct {
                       tracking problems their U_k); -1: recompute
                       only when all images (and Lindblad forward states) would take
                       more than half of the free device memory */
+  int pack;        /* 1: closed-dynamics gate problems with one member, dim <= 8, no
+                      per-slice dt and weight 1 are packed, up to 16 rows at a time, as
+                      diagonal blocks of one problem, which one 16 x 16 work item then
+                      evaluates for all of them (problems of a pack share dt; F and G are
+                      each problem's own, equal to an unpacked plan's to rounding; the
+                      diagnostics are the pack's; PR_SPLIT16 plans do not pack). 0: no
+                      packing. No reference counterpart: the reference evaluates
+                      problems one by one. A NULL opt means precision PR_FP64,
+                      grad_mode PR_GRAD_REFERENCE, segment 0, store_props -1, pack 1. */
 } pr_options;
 
-/* Per-problem diagnostics filled by pr_eval when diag != NULL:
+/* Per-problem diagnostics filled by pr_eval when diag != NULL (a packed
+ * problem, pr_options.pack, reports its pack's):
  * diag[4*p + 0] = max over members of ||U_total^dag U_total - I||_max (closed)
  *                 or 0 (Lindblad: the channel is not unitary)
  * diag[4*p + 1] = largest scaling exponent used by exp over the problem
@@ -246,7 +256,9 @@ long long pr_plan_n_params(void* plan);
  * whose exp took the degree-6 polynomial, counts[4p + 1] those that took
  * degree 9, counts[4p + 2] the squarings over all member-slices, counts[4p + 3]
  * the member-slices a reduced-precision plan redid in FP64. Waits for the
- * plan's stream. Tracking problems report zeros. */
+ * plan's stream. Tracking problems report zeros; the problems of a pack
+ * (pr_options.pack) report the pack's counts at its first problem and zeros
+ * at the others. */
 int pr_plan_exp_counts(void* plan, long long* counts);
 
 /* Kernel shapes the plan uses (no counterpart in the reference; it reports
@@ -255,7 +267,7 @@ int pr_plan_exp_counts(void* plan, long long* counts);
  * shapes[4p + 2] 1 when the single-wavefront gradient walk holds the trace
  * operand in registers (0: read from LDS), shapes[4p + 3] the assembly passes
  * the split segment kernel holds in registers (0: general assembly from
- * memory, or not a split problem). */
+ * memory, or not a split problem); a packed problem reports its pack's. */
 int pr_plan_shapes(void* plan, int* shapes);
 
 /* Per-kernel timing (no counterpart in the reference, which has no device

@@ -85,6 +85,13 @@ struct ProbDev {
   const double2* so_v;
   int sl_pass[65];
   int so_pass[2];
+  // packed small problems: the problem is nblk independent gate problems as
+  // diagonal blocks, block b on rows and columns [blk_row[b], blk_row[b + 1]);
+  // the fidelity and the adjoint seed are per block (Fm[b]). nblk = 0 otherwise.
+  int nblk;
+  int blk_row[17];
+  int ncg;                    // channels 0 .. ncg-1 have a gradient (nc, except packed
+                              // problems, whose per-block drift channels come last)
   // d <= 16 closed dynamics: the transposed drift images and operator tables
   // (X^T is assembled directly, so exp gives U^T in the register layout)
   const double2* H0t;
@@ -116,7 +123,7 @@ struct ProbDev {
   double2* Bb;                // [M][nseg][NB][256] backward boundaries (Lindblad)
   double2* Pscr;              // Lindblad, per work-item slot: [seglen][NB][256] forward
                               // states, then (recompute) [seglen][NB][256] recomputed U_k^T
-  double* Fm;                 // [M]
+  double* Fm;                 // [M], or [nblk] for packed problems
   double* Gm;                 // [M][nc*nt]
   double* udef;               // [M] ||U^dag U - I||_max (closed)
   int* smax;                  // [M*nseg][4] per segment: largest scaling exponent, slices
@@ -821,6 +828,34 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
       part.x += Tg.re[r] * Ut.re[r] + Tg.im[r] * Ut.im[r];
       part.y += Tg.re[r] * Ut.im[r] - Tg.im[r] * Ut.re[r];
     }
+    if constexpr (B == 1) {
+      if (P.nblk > 0) {
+        // packed blocks: Tg and U are block diagonal, so the lane's products
+        // (column x.c) belong to the block of that column; per block b,
+        // Phi_b = Tr(U_t,b^dag U_b)/d_b, F_b = |Phi_b|^2 and the rows of
+        // U_t^dag U in the block take conj(w_b), w_b = 2 Phi_b / d_b
+        int bc = -1;
+        for (int b = 0; b < P.nblk; b++)
+          if (x.c >= P.blk_row[b] && x.c < P.blk_row[b + 1]) bc = b;
+        double2 wc = make_double2(0, 0);
+        for (int b = 0; b < P.nblk; b++) {
+          const double2 tb = block_sum2<1>(bc == b ? part : make_double2(0, 0), red, x);
+          const double db = P.blk_row[b + 1] - P.blk_row[b];
+          const double br = tb.x / db, bi = tb.y / db;
+          if (threadIdx.x == 0) P.Fm[b] = br * br + bi * bi;
+          if (bc == b) wc = make_double2(2.0 * br / db, 2.0 * bi / db);
+        }
+        Blk Y = mm<1, C, N>(Tg, Ut, LA, LB, x);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const double a = Y.re[r], b = Y.im[r];
+          Y.re[r] = wc.x * a + wc.y * b;
+          Y.im[r] = wc.x * b - wc.y * a;
+        }
+        store_blk(P.Om + (size_t)m * MB, Y, x);
+        return;
+      }
+    }
     double2 tr = block_sum2<B>(part, red, x);
     const double phr = tr.x / P.d, phi = tr.y / P.d;
     if (threadIdx.x == 0) P.Fm[m] = phr * phr + phi * phi;
@@ -931,10 +966,11 @@ __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, d
   bsync<B>();
   const int t = threadIdx.x;
   const int grp = t / G, gl = t % G;
-  for (int j0 = 0; j0 < nc; j0 += V) {
+  const int ncg = P.ncg;
+  for (int j0 = 0; j0 < ncg; j0 += V) {
     const int j = j0 + grp;
     double sv = 0;
-    if (grp < V && j < nc) {
+    if (grp < V && j < ncg) {
       const int e1 = gptr(P.tr_ptr)[j + 1];
       for (int e = gptr(P.tr_ptr)[j] + gl; e < e1; e += G) {
         const double2 a = gload2(P.tr_v, e);
@@ -943,7 +979,7 @@ __device__ inline void contract(const ProbDev& P, int m, int k, const Blk& Mk, d
       }
     }
     sv = group_sum<G>(sv);
-    if (grp < V && gl == 0 && j < nc) {
+    if (grp < V && gl == 0 && j < ncg) {
       const double f = slice_dt(P, k) * P.scale[(size_t)m * nc + j];
       // Re(c * f * z): closed c = -i -> f Im z ; Lindblad c = 1 -> f Re z
       gptr_w(P.Gm)[(size_t)m * nc * P.nt + (size_t)k * nc + j] = f * sv;
@@ -970,7 +1006,7 @@ __device__ inline void trace_regs_load(const ProbDev& P, int m, TraceRegs<V>& t,
   const int grp = x.lane / G, gl = x.lane % G;
   int e0 = 0, e1 = 0;
   t.sc = 0;
-  if (grp < P.nc) {
+  if (grp < P.ncg) {
     e0 = gptr(P.tr_ptr)[grp];
     e1 = gptr(P.tr_ptr)[grp + 1];
     t.sc = gptr(P.scale)[(size_t)m * P.nc + grp];
@@ -1007,7 +1043,7 @@ __device__ inline void contract_regs(const ProbDev& P, int m, int k, double dtk,
     sv = fma(t.v[i].x, mv.x, fma(t.v[i].y, mv.y, sv));
   }
   sv = group_sum<G>(sv);
-  if (gl == 0 && grp < P.nc) {
+  if (gl == 0 && grp < P.ncg) {
     const double f = dtk * t.sc;
     gptr_w(P.Gm)[(size_t)m * P.nc * P.nt + (size_t)k * P.nc + grp] = f * sv;
   }
@@ -1703,6 +1739,67 @@ __global__ void __launch_bounds__(64) k_reduce_F(const ProbDev* __restrict__ pro
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
   if (lane == 0) F[p] = s;
+}
+
+// ------------------------------------------------------ packed small problems
+// A plan that packs small problems runs its kernels on its own problem list:
+// the caller's controls are gathered into that list's layout first, and the
+// fidelities and gradients are scattered back to the caller's problems after
+// the reductions.
+// dsrc[p]: the caller's problem that device problem p is, or -1 - c0 for a
+// packed problem whose channels are chan[c0 ...]: (caller's problem, control
+// j), j = -1 for a block's drift channel, whose waveform is 1 on the block's
+// own slices. A block's slices beyond its problem's n_t take zero controls and
+// zero drift.
+__global__ void k_pack_controls(const ProbDev* __restrict__ probs, int ndev, const long long* __restrict__ doff,
+                                long long total, const int* __restrict__ dsrc, const int2* __restrict__ chan,
+                                const long long* __restrict__ uoff, const int* __restrict__ unt,
+                                const double* __restrict__ u, double* __restrict__ ud) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int p = prob_of(doff, ndev, t);
+  const long long e = t - doff[p];
+  const int s = dsrc[p];
+  if (s >= 0) {
+    ud[t] = u[uoff[s] + e];
+    return;
+  }
+  const int nc = probs[p].nc;
+  const long long k = e / nc;
+  const int2 ch = chan[-1 - s + (int)(e - k * nc)];
+  double v = 0.0;
+  if (k < unt[ch.x]) {
+    const int ncu = (int)((uoff[ch.x + 1] - uoff[ch.x]) / unt[ch.x]);
+    v = ch.y < 0 ? 1.0 : u[uoff[ch.x] + k * ncu + ch.y];
+  }
+  ud[t] = v;
+}
+
+// umap[p] for the caller's problem p: (device problem, block or -1 when the
+// problem is not packed, first channel of its block, n_ctrl)
+__global__ void k_unpack_G(long long total, int nuser, const long long* __restrict__ uoff,
+                           const int4* __restrict__ umap, const ProbDev* __restrict__ probs,
+                           const long long* __restrict__ doff, const double* __restrict__ Gd,
+                           double* __restrict__ G) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int p = prob_of(uoff, nuser, t);
+  const long long e = t - uoff[p];
+  const int4 mp = umap[p];
+  if (mp.y < 0) {
+    G[t] = Gd[doff[mp.x] + e];
+    return;
+  }
+  const long long k = e / mp.w;
+  G[t] = Gd[doff[mp.x] + k * probs[mp.x].nc + mp.z + (e - k * mp.w)];
+}
+
+__global__ void k_unpack_F(int nuser, const int4* __restrict__ umap, const ProbDev* __restrict__ probs,
+                           const double* __restrict__ Fd, double* __restrict__ F) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nuser) return;
+  const int4 mp = umap[p];
+  F[p] = mp.y < 0 ? Fd[mp.x] : probs[mp.x].Fm[mp.y];
 }
 
 // ---------------------------------------------------------------- GRAPE step

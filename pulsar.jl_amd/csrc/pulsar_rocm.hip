@@ -335,12 +335,29 @@ struct DevBuf {
 };
 
 struct Plan {
+  // the problems the kernels run on: the caller's, or, when small problems are
+  // packed (pr_options.pack), the plan's own list of unpacked and packed ones
   int n_probs = 0;
   pr_options opt{};
   std::vector<pr::ProbDev> host_probs;
   std::vector<int> ng;                // n_ctrl * n_t per problem
   std::vector<long long> coff;        // controls offsets (n_probs + 1)
   long long n_params = 0;
+  // the caller's problems (controls, F and G of pr_eval and pr_eval_device):
+  // the same as above unless packed
+  int n_user = 0;
+  std::vector<long long> ucoff;       // (n_user + 1)
+  long long n_uparams = 0;
+  bool packed = false;
+  std::vector<int4> umap;             // per caller's problem: (problem, block or -1, first channel, n_ctrl)
+  long long* d_uoff = nullptr;
+  int* d_unt = nullptr;               // n_t per caller's problem
+  int* d_dsrc = nullptr;              // per problem: the caller's problem, or -1 - first entry of d_chan
+  int2* d_chan = nullptr;             // per packed channel: (caller's problem, control j, or -1: drift)
+  int4* d_umap = nullptr;
+  double* d_udev = nullptr;           // controls in the plan's own layout (n_params)
+  double* d_Fdev = nullptr;           // (n_probs)
+  double* d_Gdev = nullptr;           // (n_params)
   std::vector<void*> allocs;
   pr::ProbDev* d_probs = nullptr;
   long long* d_off = nullptr;
@@ -367,6 +384,7 @@ struct Plan {
   // them), the reduction partials' offset, the total, and the device copies
   std::vector<pr::StaticOff> static_sz, static_off;
   long long static_part = 0, static_udef_bytes = 0, static_bytes = 0;
+  long long static_udev = 0, static_Fdev = 0, static_Gdev = 0;  // packed plans
   bool static_ok = true;  // false when a group keeps per-item scratch outside the records (dim > 64)
   pr::StaticOff* d_static_off = nullptr;
   pr::ProbDev* d_probs_static = nullptr;
@@ -863,24 +881,54 @@ int launch_one(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hip
   }
 }
 
-int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
-  for (const auto& g : pl.groups) {
-    const int rc = launch_one(pl, g, d_u, d_G != nullptr, st);
-    if (rc) return rc;
-  }
-  int rc;
-  if ((rc = tbegin(pl, st))) return rc;
-  hipLaunchKernelGGL(pr::k_reduce_F, dim3(pl.n_probs), dim3(64), 0, st, pl.d_probs, pl.n_probs, d_F);
+// packed plans: the caller's controls d_u in the plan's own layout (pl.d_udev)
+int pack_controls(Plan& pl, const double* d_u, hipStream_t st) {
+  const long long total = pl.n_params;
+  hipLaunchKernelGGL(pr::k_pack_controls, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, pl.d_probs,
+                     pl.n_probs, pl.d_off, total, pl.d_dsrc, pl.d_chan, pl.d_uoff, pl.d_unt, d_u, pl.d_udev);
   HIPCHK(hipGetLastError());
-  if (d_G) {
+  return PR_OK;
+}
+
+int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
+  int rc;
+  const double* u = d_u;
+  double* F = d_F;
+  double* G = d_G;
+  if (pl.packed) {
+    if ((rc = tbegin(pl, st))) return rc;
+    if ((rc = pack_controls(pl, d_u, st))) return rc;
+    if ((rc = tend(pl, 3, st))) return rc;
+    u = pl.d_udev;
+    F = pl.d_Fdev;
+    G = d_G ? pl.d_Gdev : nullptr;
+  }
+  for (const auto& g : pl.groups) {
+    if ((rc = launch_one(pl, g, u, G != nullptr, st))) return rc;
+  }
+  if ((rc = tbegin(pl, st))) return rc;
+  hipLaunchKernelGGL(pr::k_reduce_F, dim3(pl.n_probs), dim3(64), 0, st, pl.d_probs, pl.n_probs, F);
+  HIPCHK(hipGetLastError());
+  if (G) {
     long long total = pl.n_params;
     const int nch = pl.red_chunks;
     hipLaunchKernelGGL(pr::k_reduce_G1, dim3((unsigned)((total + 255) / 256), nch), dim3(256), 0, st, pl.d_probs,
-                       pl.n_probs, pl.d_off, total, nch, nch == 1 ? d_G : pl.d_part);
+                       pl.n_probs, pl.d_off, total, nch, nch == 1 ? G : pl.d_part);
     HIPCHK(hipGetLastError());
     if (nch > 1) {
       hipLaunchKernelGGL(pr::k_reduce_G2, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, nch,
-                         pl.d_part, d_G);
+                         pl.d_part, G);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  if (pl.packed) {
+    hipLaunchKernelGGL(pr::k_unpack_F, dim3((pl.n_user + 255) / 256), dim3(256), 0, st, pl.n_user, pl.d_umap,
+                       pl.d_probs, F, d_F);
+    HIPCHK(hipGetLastError());
+    if (d_G) {
+      const long long total = pl.n_uparams;
+      hipLaunchKernelGGL(pr::k_unpack_G, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, pl.n_user,
+                         pl.d_uoff, pl.d_umap, pl.d_probs, pl.d_off, G, d_G);
       HIPCHK(hipGetLastError());
     }
   }
@@ -951,6 +999,14 @@ int static_layout(Plan& pl) {
   }
   pl.static_part = at;
   at = align(at + (long long)sizeof(double) * pl.red_chunks * pl.n_params);
+  if (pl.packed) {
+    pl.static_udev = at;
+    at = align(at + (long long)sizeof(double) * pl.n_params);
+    pl.static_Fdev = at;
+    at = align(at + (long long)sizeof(double) * pl.n_probs);
+    pl.static_Gdev = at;
+    at = align(at + (long long)sizeof(double) * pl.n_params);
+  }
   pl.static_bytes = at;
   for (const auto& g : pl.groups)
     if (g.big_scratch) pl.static_ok = false;
@@ -960,7 +1016,163 @@ int static_layout(Plan& pl) {
   return PR_OK;
 }
 
-int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
+// ------------------------------------------------ packing of small problems
+// pr_options.pack: closed-dynamics gate problems with one member, dim <= 8, a
+// uniform slice length and weight 1 become diagonal blocks of one problem of
+// dim <= 16, so that one single-wavefront work item evaluates up to 8 of them
+// (a d = 2 problem alone fills 4 of the 256 elements of its 16 x 16 tile).
+// - Problems of a pack share dt and run the pack's n_t (the longest). Block b's
+//   drift is a control channel of its own whose waveform is 1 on the block's
+//   own slices and 0 after, where its controls are 0 too: the propagators of
+//   those slices are exactly the identity on the block.
+// - Exp, products and traces of a block-diagonal matrix act block by block; the
+//   kernels differ from an unpacked problem only in the shared scaling
+//   exponent and Taylor degree of a slice (the largest of the blocks'), a
+//   rounding-level difference.
+// - F and the adjoint seed are per block (k_combine); control j of block b is
+//   channel ch0_b + j, and its gradient row is the problem's. The drift
+//   channels come after every block's controls and have no gradient (ncg).
+// Packs are filled in order of decreasing n_t, each holding problems whose n_t
+// is within 1/16 of its longest, so little padding is run.
+struct PackBuild {
+  std::vector<pr_problem> dev;               // the plan's problem list
+  std::vector<std::vector<double>> store;    // host arrays of the packed problems
+  std::vector<int> ncg;                      // per problem: channels with a gradient
+  std::vector<std::vector<int>> blk_row;     // per problem: block rows (packed) or empty
+  std::vector<int> dsrc;                     // per problem: caller's problem, or -1 - first channel in chan
+  std::vector<int2> chan;                    // per packed channel: (caller's problem, j or -1)
+  std::vector<int4> umap;                    // per caller's problem: (problem, block or -1, ch0, n_ctrl)
+};
+
+bool packable(const pr_problem& q) {
+  return q.dynamics == PR_CLOSED && q.objective == PR_GATE && q.n_members == 1 && q.dim <= 8 && !q.dt_slices &&
+         (!q.weight || q.weight[0] == 1.0);
+}
+
+// false when nothing is packed (fewer than two packable problems fit together)
+bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
+  std::vector<int> cand;
+  for (int p = 0; p < n_probs; p++)
+    if (packable(probs[p])) cand.push_back(p);
+  if (cand.size() < 2) return false;
+  std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) {
+    if (probs[a].dt != probs[b].dt) return probs[a].dt < probs[b].dt;
+    return probs[a].n_t > probs[b].n_t;
+  });
+  struct Bin {
+    std::vector<int> mem;
+    int rows = 0, chans = 0, nt = 0;
+    double dt = 0;
+  };
+  std::vector<Bin> bins;
+  std::vector<size_t> open;  // indices of bins still taking problems (at most 8)
+  for (int p : cand) {
+    const pr_problem& q = probs[p];
+    int hit = -1;
+    for (size_t i = 0; i < open.size() && hit < 0; i++) {
+      const Bin& b = bins[open[i]];
+      if (b.dt == q.dt && b.rows + q.dim <= 16 && b.chans + q.n_ctrl + 1 <= 64 && b.nt - q.n_t <= b.nt / 16)
+        hit = (int)i;
+    }
+    if (hit < 0) {
+      if (open.size() == 8) open.erase(open.begin());
+      bins.emplace_back();
+      bins.back().nt = q.n_t;
+      bins.back().dt = q.dt;
+      open.push_back(bins.size() - 1);
+      hit = (int)open.size() - 1;
+    }
+    Bin& b = bins[open[hit]];
+    b.mem.push_back(p);
+    b.rows += q.dim;
+    b.chans += q.n_ctrl + 1;
+    if (16 - b.rows < 1) open.erase(open.begin() + hit);
+  }
+  std::vector<int> pack_of(n_probs, -1);
+  int n_packs = 0;
+  for (size_t i = 0; i < bins.size(); i++)
+    if (bins[i].mem.size() > 1) {
+      for (int p : bins[i].mem) pack_of[p] = (int)i;
+      n_packs++;
+    }
+  if (n_packs == 0) return false;
+  pk.umap.assign(n_probs, make_int4(0, -1, 0, 0));
+  // unpacked problems first, in the caller's order, then the packs
+  for (int p = 0; p < n_probs; p++) {
+    if (pack_of[p] >= 0) continue;
+    pk.umap[p] = make_int4((int)pk.dev.size(), -1, 0, probs[p].n_ctrl);
+    pk.dev.push_back(probs[p]);
+    pk.ncg.push_back(probs[p].n_ctrl);
+    pk.blk_row.emplace_back();
+    pk.dsrc.push_back(p);
+  }
+  for (const Bin& bn : bins) {
+    if (bn.mem.size() < 2) continue;
+    const int dp = (int)pk.dev.size();
+    int D = 0, ncg = 0;
+    for (int p : bn.mem) {
+      D += probs[p].dim;
+      ncg += probs[p].n_ctrl;
+    }
+    const int nb = (int)bn.mem.size(), nc = ncg + nb;
+    const size_t DD = (size_t)D * D;
+    std::vector<double> H0(2 * DD, 0.0), Hc(2 * DD * nc, 0.0), Ut(2 * DD, 0.0), sc(nc, 1.0);
+    std::vector<int> rows(1, 0);
+    // element (r, c) of a column-major d x d complex matrix into channel ch at block offset o
+    auto embed = [&](std::vector<double>& dst, size_t base, const double* a, int d, int o) {
+      for (int c = 0; c < d; c++)
+        for (int r = 0; r < d; r++) {
+          const size_t i = (size_t)r + (size_t)c * d, j = base + (size_t)(o + r) + (size_t)(o + c) * D;
+          dst[2 * j] = a[2 * i];
+          dst[2 * j + 1] = a[2 * i + 1];
+        }
+    };
+    int o = 0, ch0 = 0;
+    for (int b = 0; b < nb; b++) {
+      const int p = bn.mem[b];
+      const pr_problem& q = probs[p];
+      const int d = q.dim;
+      const size_t dd = (size_t)d * d;
+      for (int j = 0; j < q.n_ctrl; j++) {
+        embed(Hc, DD * (ch0 + j), q.Hc + 2 * dd * j, d, o);
+        if (q.ctrl_scale) sc[ch0 + j] = q.ctrl_scale[j];
+        pk.chan.push_back(make_int2(p, j));
+      }
+      embed(Hc, DD * (ncg + b), q.H0, d, o);
+      embed(Ut, 0, q.U_target, d, o);
+      pk.umap[p] = make_int4(dp, b, ch0, q.n_ctrl);
+      o += d;
+      ch0 += q.n_ctrl;
+      rows.push_back(o);
+    }
+    for (int b = 0; b < nb; b++) pk.chan.push_back(make_int2(bn.mem[b], -1));
+    pr_problem z = probs[bn.mem[0]];
+    z.dim = D;
+    z.n_ctrl = nc;
+    z.n_t = bn.nt;
+    z.dt = bn.dt;
+    z.dt_slices = nullptr;
+    z.n_members = 1;
+    z.H0_per_member = 0;
+    z.weight = nullptr;
+    pk.store.push_back(std::move(H0));
+    z.H0 = pk.store.back().data();
+    pk.store.push_back(std::move(Hc));
+    z.Hc = pk.store.back().data();
+    pk.store.push_back(std::move(Ut));
+    z.U_target = pk.store.back().data();
+    pk.store.push_back(std::move(sc));
+    z.ctrl_scale = pk.store.back().data();
+    pk.dev.push_back(z);
+    pk.ncg.push_back(ncg);
+    pk.blk_row.push_back(rows);
+    pk.dsrc.push_back(-1 - ((int)pk.chan.size() - nc));
+  }
+  return true;
+}
+
+int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem* uprobs, int n_user,
+               const PackBuild* pk) {
   pl.n_probs = n_probs;
   pl.host_probs.resize(n_probs);
   pl.static_sz.resize(n_probs);
@@ -1028,6 +1240,11 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
     const bool track = q.objective == PR_TRACKING;
     const size_t MB = (size_t)NB * 256, dd = (size_t)d * d;
     D.d = d; D.B = B; D.nc = nc; D.nt = nt; D.M = M; D.dyn = q.dynamics; D.obj = q.objective;
+    D.ncg = pk ? pk->ncg[p] : nc;
+    if (pk && !pk->blk_row[p].empty()) {
+      D.nblk = (int)pk->blk_row[p].size() - 1;
+      std::copy(pk->blk_row[p].begin(), pk->blk_row[p].end(), D.blk_row);
+    }
     const bool gate = q.objective == PR_GATE || q.objective == PR_GATE_PROPAGATORS;
     D.npairs = gate ? 0 : q.n_pairs;
     D.h0pm = q.H0_per_member ? 1 : 0;
@@ -1172,7 +1389,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
       if ((rc = pl.upload(&jp, tr.idx.data(), tr.idx.size()))) return rc;
       if ((rc = pl.upload(&vp, tr.v.data(), tr.v.size()))) return rc;
       D.tr_ptr = ip; D.tr_idx = jp; D.tr_v = vp;
-      for (int j = 0; j < nc; j++) tr_nnz_max[gi] = std::max(tr_nnz_max[gi], tr.ptr[j + 1] - tr.ptr[j]);
+      for (int j = 0; j < D.ncg; j++) tr_nnz_max[gi] = std::max(tr_nnz_max[gi], tr.ptr[j + 1] - tr.ptr[j]);
     }
     std::vector<double> sc((size_t)M * nc, 1.0);
     if (q.ctrl_scale) std::copy(q.ctrl_scale, q.ctrl_scale + (size_t)M * nc, sc.begin());
@@ -1258,8 +1475,10 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
         PR_SCRATCH(Pscr, (size_t)M * S * L * MB);
       }
     }
-    PR_SCRATCH(Fm, M);
+    PR_SCRATCH(Fm, std::max(M, D.nblk));
     PR_SCRATCH(Gm, (size_t)M * nc * nt);
+    // (the drift channels of a packed problem have no gradient: their rows stay 0)
+    if (D.ncg < nc) HIPCHK(hipMemset(D.Gm, 0, sizeof(double) * M * nc * nt));
     PR_SCRATCH(udef, M);
     HIPCHK(hipMemset(D.udef, 0, sizeof(double) * M));
     PR_SCRATCH(smax, (size_t)4 * M * S);
@@ -1279,16 +1498,35 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
         items[gi].push_back(w);
       }
     }
-    vmax[gi] = std::max(vmax[gi], next_pow2_cap32(nc));
-    nc_max[gi] = std::max(nc_max[gi], nc);
+    vmax[gi] = std::max(vmax[gi], next_pow2_cap32(D.ncg));
+    nc_max[gi] = std::max(nc_max[gi], D.ncg);
   }
   pl.n_params = pl.coff[n_probs];
+  // the caller's problems
+  pl.n_user = n_user;
+  pl.ucoff.assign(n_user + 1, 0);
+  for (int p = 0; p < n_user; p++) pl.ucoff[p + 1] = pl.ucoff[p] + (long long)uprobs[p].n_ctrl * uprobs[p].n_t;
+  pl.n_uparams = pl.ucoff[n_user];
+  pl.packed = pk != nullptr;
   int rc;
   if ((rc = pl.upload(&pl.d_probs, pl.host_probs.data(), n_probs))) return rc;
   if ((rc = pl.upload(&pl.d_off, pl.coff.data(), n_probs + 1))) return rc;
-  if ((rc = pl.alloc(&pl.d_controls, pl.n_params))) return rc;
-  if ((rc = pl.alloc(&pl.d_F, n_probs))) return rc;
-  if ((rc = pl.alloc(&pl.d_G, pl.n_params))) return rc;
+  if ((rc = pl.alloc(&pl.d_controls, pl.n_uparams))) return rc;
+  if ((rc = pl.alloc(&pl.d_F, n_user))) return rc;
+  if ((rc = pl.alloc(&pl.d_G, pl.n_uparams))) return rc;
+  if (pk) {
+    std::vector<int> unt(n_user);
+    for (int p = 0; p < n_user; p++) unt[p] = uprobs[p].n_t;
+    pl.umap = pk->umap;
+    if ((rc = pl.upload(&pl.d_uoff, pl.ucoff.data(), n_user + 1))) return rc;
+    if ((rc = pl.upload(&pl.d_unt, unt.data(), n_user))) return rc;
+    if ((rc = pl.upload(&pl.d_dsrc, pk->dsrc.data(), n_probs))) return rc;
+    if ((rc = pl.upload(&pl.d_chan, pk->chan.data(), pk->chan.size()))) return rc;
+    if ((rc = pl.upload(&pl.d_umap, pk->umap.data(), n_user))) return rc;
+    if ((rc = pl.alloc(&pl.d_udev, pl.n_params))) return rc;
+    if ((rc = pl.alloc(&pl.d_Fdev, n_probs))) return rc;
+    if ((rc = pl.alloc(&pl.d_Gdev, pl.n_params))) return rc;
+  }
   {
     int mmax = 1;
     for (int p = 0; p < n_probs; p++) mmax = std::max(mmax, probs[p].n_members);
@@ -1297,8 +1535,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
   if ((rc = pl.alloc(&pl.d_part, (size_t)pl.red_chunks * pl.n_params))) return rc;
   {
     void* h = nullptr;
-    const hipError_t e = hipHostMalloc(&h, sizeof(double) * (pl.n_params + pl.n_probs), hipHostMallocDefault);
-    if (e != hipSuccess) return fail(PR_ERR_NOMEM, "hipHostMalloc(%lld doubles): %s", pl.n_params + pl.n_probs,
+    const hipError_t e = hipHostMalloc(&h, sizeof(double) * (pl.n_uparams + n_user), hipHostMallocDefault);
+    if (e != hipSuccess) return fail(PR_ERR_NOMEM, "hipHostMalloc(%lld doubles): %s", pl.n_uparams + n_user,
                                      hipGetErrorString(e));
     pl.h_stage = static_cast<double*>(h);
   }
@@ -1363,8 +1601,9 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs) {
 }
 
 int fill_diag(const Plan& pl, double* diag) {
-  for (int p = 0; p < pl.n_probs; p++) {
-    const pr::ProbDev& D = pl.host_probs[p];
+  for (int p = 0; p < pl.n_user; p++) {
+    // (a packed problem reports its pack's values)
+    const pr::ProbDev& D = pl.host_probs[pl.packed ? pl.umap[p].x : p];
     std::vector<double> ud(D.M);
     std::vector<int> sm((size_t)4 * D.M * D.nseg);
     HIPCHK(hipMemcpy(ud.data(), D.udef, sizeof(double) * D.M, hipMemcpyDeviceToHost));
@@ -1397,7 +1636,13 @@ int fill_diag(const Plan& pl, double* diag) {
 // is.
 int time_groups(Plan& pl, std::vector<double>& ms) {
   ms.assign(13, 0.0);
-  HIPCHK(hipMemsetAsync(pl.d_controls, 0, sizeof(double) * pl.n_params, pl.stream));
+  HIPCHK(hipMemsetAsync(pl.d_controls, 0, sizeof(double) * pl.n_uparams, pl.stream));
+  const double* u = pl.d_controls;
+  if (pl.packed) {
+    const int rc = pack_controls(pl, pl.d_controls, pl.stream);
+    if (rc) return rc;
+    u = pl.d_udev;
+  }
   const size_t ng = pl.groups.size();
   std::vector<hipEvent_t> ev(2 * ng, nullptr);
   hipError_t e = hipSuccess;
@@ -1407,7 +1652,7 @@ int time_groups(Plan& pl, std::vector<double>& ms) {
   for (int rep = 0; rep < 4 && rc == PR_OK; rep++) {
     for (size_t i = 0; i < ng && rc == PR_OK; i++) {
       if ((e = hipEventRecord(ev[2 * i], pl.stream)) != hipSuccess) break;
-      rc = launch_one(pl, pl.groups[i], pl.d_controls, true, pl.stream);
+      rc = launch_one(pl, pl.groups[i], u, true, pl.stream);
       if (rc == PR_OK && (e = hipEventRecord(ev[2 * i + 1], pl.stream)) != hipSuccess) break;
     }
     if (rc == PR_OK && e == hipSuccess) e = hipStreamSynchronize(pl.stream);
@@ -1424,7 +1669,8 @@ int time_groups(Plan& pl, std::vector<double>& ms) {
   return rc;
 }
 
-int autotune_segments(Plan& pl, const pr_problem* probs, int n_probs) {
+int autotune_segments(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem* uprobs, int n_user,
+                      const PackBuild* pk) {
   const double cand[] = {0.5, 1.0, 2.0, 4.0};
   std::vector<double> best(13, 1e300), scale(13, 1.0), ms;
   std::vector<int> regs(13, 1);
@@ -1441,7 +1687,7 @@ int autotune_segments(Plan& pl, const pr_problem* probs, int n_probs) {
       t->tune_regs.assign(13, r);
       hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
       int rc = e == hipSuccess ? PR_OK : fail(PR_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
-      if (rc == PR_OK) rc = build_plan(*t, probs, n_probs);
+      if (rc == PR_OK) rc = build_plan(*t, probs, n_probs, uprobs, n_user, pk);
       if (rc == PR_OK) rc = time_groups(*t, ms);
       delete t;
       if (rc) return rc;
@@ -1461,9 +1707,9 @@ int autotune_segments(Plan& pl, const pr_problem* probs, int n_probs) {
 // Every problem's controls packed back to back into the plan's pinned staging
 // buffer, so that one host-to-device copy moves them all.
 int stage_controls(Plan& pl, const double* const* controls, const char* who) {
-  for (int p = 0; p < pl.n_probs; p++) {
+  for (int p = 0; p < pl.n_user; p++) {
     if (!controls[p]) return fail(PR_ERR_ARG, "%s: controls[%d] is NULL", who, p);
-    memcpy(pl.h_stage + pl.coff[p], controls[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]));
+    memcpy(pl.h_stage + pl.ucoff[p], controls[p], sizeof(double) * (pl.ucoff[p + 1] - pl.ucoff[p]));
   }
   return PR_OK;
 }
@@ -1475,15 +1721,15 @@ int stage_ranges(Plan& pl) {
   if (!pl.stage_cut.empty()) return PR_OK;
   const size_t min_range = (size_t)1 << 18;  // doubles
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const int R = (int)std::max<size_t>(1, std::min<size_t>({8, hw, (size_t)pl.n_params / min_range}));
+  const int R = (int)std::max<size_t>(1, std::min<size_t>({8, hw, (size_t)pl.n_uparams / min_range}));
   pl.stage_cut.assign(1, 0);
   for (int r = 1; r < R; r++) {
-    const long long target = (long long)((double)pl.n_params * r / R);
+    const long long target = (long long)((double)pl.n_uparams * r / R);
     int p = pl.stage_cut.back();
-    while (p < pl.n_probs && pl.coff[p] < target) p++;
-    if (p > pl.stage_cut.back() && p < pl.n_probs) pl.stage_cut.push_back(p);
+    while (p < pl.n_user && pl.ucoff[p] < target) p++;
+    if (p > pl.stage_cut.back() && p < pl.n_user) pl.stage_cut.push_back(p);
   }
-  pl.stage_cut.push_back(pl.n_probs);
+  pl.stage_cut.push_back(pl.n_user);
   pl.stage_ev.assign(pl.stage_cut.size() - 1, nullptr);
   for (auto& e : pl.stage_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return PR_OK;
@@ -1495,22 +1741,22 @@ int stage_ranges(Plan& pl) {
 // range's gradients are downloaded and unpacked by their thread once their
 // own download has finished, while later ranges are still in flight.
 int staged_eval(Plan& pl, const double* const* controls, double* F, double* const* G) {
-  for (int p = 0; p < pl.n_probs; p++)
+  for (int p = 0; p < pl.n_user; p++)
     if (!controls[p]) return fail(PR_ERR_ARG, "pr_eval: controls[%d] is NULL", p);
   int rc = stage_ranges(pl);
   if (rc) return rc;
   const int R = (int)pl.stage_ev.size();
   auto range_of = [&](int r, size_t& off, size_t& len) {
-    off = (size_t)pl.coff[pl.stage_cut[r]];
-    len = (size_t)(pl.coff[pl.stage_cut[r + 1]] - pl.coff[pl.stage_cut[r]]);
+    off = (size_t)pl.ucoff[pl.stage_cut[r]];
+    len = (size_t)(pl.ucoff[pl.stage_cut[r + 1]] - pl.ucoff[pl.stage_cut[r]]);
   };
   auto pack = [&](int r) {
     for (int p = pl.stage_cut[r]; p < pl.stage_cut[r + 1]; p++)
-      memcpy(pl.h_stage + pl.coff[p], controls[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]));
+      memcpy(pl.h_stage + pl.ucoff[p], controls[p], sizeof(double) * (pl.ucoff[p + 1] - pl.ucoff[p]));
   };
   auto unpack = [&](int r) {
     for (int p = pl.stage_cut[r]; p < pl.stage_cut[r + 1]; p++)
-      if (G[p]) memcpy(G[p], pl.h_stage + pl.coff[p], sizeof(double) * (pl.coff[p + 1] - pl.coff[p]));
+      if (G[p]) memcpy(G[p], pl.h_stage + pl.ucoff[p], sizeof(double) * (pl.ucoff[p + 1] - pl.ucoff[p]));
   };
   std::vector<std::thread> th;
   for (int r = 1; r < R; r++) th.emplace_back([&, r] { pack(r); });
@@ -1528,7 +1774,7 @@ int staged_eval(Plan& pl, const double* const* controls, double* F, double* cons
   if (e != hipSuccess) return fail(PR_ERR_DEVICE, "pr_eval: hipMemcpyAsync: %s", hipGetErrorString(e));
   rc = run_graph(pl, pl.d_controls, pl.d_F, G ? pl.d_G : nullptr, pl.stream);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(pl.h_stage + pl.n_params, pl.d_F, sizeof(double) * pl.n_probs, hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(pl.h_stage + pl.n_uparams, pl.d_F, sizeof(double) * pl.n_user, hipMemcpyDeviceToHost,
                         pl.stream));
   if (G) {
     for (int r = 0; r < R; r++) {
@@ -1551,7 +1797,7 @@ int staged_eval(Plan& pl, const double* const* controls, double* F, double* cons
       if (we[r] != hipSuccess) return fail(PR_ERR_DEVICE, "pr_eval: hipEventSynchronize: %s", hipGetErrorString(we[r]));
   }
   HIPCHK(hipStreamSynchronize(pl.stream));
-  memcpy(F, pl.h_stage + pl.n_params, sizeof(double) * pl.n_probs);
+  memcpy(F, pl.h_stage + pl.n_uparams, sizeof(double) * pl.n_user);
   return PR_OK;
 }
 
@@ -1597,7 +1843,7 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
   if (fault_is("plan")) return fail(PR_ERR_INJECTED, "failure injected by PULSAR_ROCM_FAULT=plan");
   if (!probs || n_probs < 1 || !plan) return fail(PR_ERR_ARG, "pr_plan_create: need problems and a plan pointer");
   *plan = nullptr;
-  pr_options o{PR_FP64, PR_GRAD_REFERENCE, 0, -1};
+  pr_options o{PR_FP64, PR_GRAD_REFERENCE, 0, -1, 1};
   if (opt) o = *opt;
   if (o.precision != PR_FP64 && o.precision != PR_MIXED && o.precision != PR_SPLIT16)
     return fail(PR_ERR_ARG, "unknown precision %d", o.precision);
@@ -1605,6 +1851,7 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
     return fail(PR_ERR_ARG, "unknown grad_mode %d", o.grad_mode);
   if (o.store_props < -1 || o.store_props > 1) return fail(PR_ERR_ARG, "store_props %d is not -1, 0 or 1", o.store_props);
   if (o.segment < PR_SEGMENT_AUTOTUNE) return fail(PR_ERR_ARG, "segment must be >= 0, or PR_SEGMENT_AUTOTUNE");
+  if (o.pack != 0 && o.pack != 1) return fail(PR_ERR_ARG, "pack %d is not 0 or 1", o.pack);
   for (int p = 0; p < n_probs; p++) {
     if (probs[p].dim > 64 && o.grad_mode == PR_GRAD_EXACT)
       return fail(PR_ERR_UNSUPPORTED, "problem %d: the exact gradient is available for dim <= 64", p);
@@ -1626,15 +1873,22 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
     delete pl;
     return fail(PR_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
   }
+  // (PR_SPLIT16 plans do not pack: the split segment kernel bounds a slice's
+  // generator by the sum over its channels, which a pack's blocks would add up)
+  PackBuild pk;
+  const bool packed = o.pack && o.precision != PR_SPLIT16 && build_packs(probs, n_probs, pk);
+  const pr_problem* dprobs = packed ? pk.dev.data() : probs;
+  const int ndev = packed ? (int)pk.dev.size() : n_probs;
+  const PackBuild* pkp = packed ? &pk : nullptr;
   if (o.segment == PR_SEGMENT_AUTOTUNE) {
     pl->opt.segment = 0;
-    rc = autotune_segments(*pl, probs, n_probs);
+    rc = autotune_segments(*pl, dprobs, ndev, probs, n_probs, pkp);
     if (rc) {
       delete pl;
       return rc;
     }
   }
-  rc = build_plan(*pl, probs, n_probs);
+  rc = build_plan(*pl, dprobs, ndev, probs, n_probs, pkp);
   if (rc) {
     delete pl;
     return rc;
@@ -1694,13 +1948,22 @@ int pr_eval_device_static(void* plan, const double* d_controls, double* d_F, dou
   // capture of its own, no timing events), so a caller may capture it
   pr::ProbDev* own_probs = pl.d_probs;
   double* own_part = pl.d_part;
+  double *own_udev = pl.d_udev, *own_Fdev = pl.d_Fdev, *own_Gdev = pl.d_Gdev;
   const bool own_timing = pl.timing;
   pl.d_probs = pl.d_probs_static;
   pl.d_part = reinterpret_cast<double*>(base + pl.static_part);
+  if (pl.packed) {
+    pl.d_udev = reinterpret_cast<double*>(base + pl.static_udev);
+    pl.d_Fdev = reinterpret_cast<double*>(base + pl.static_Fdev);
+    pl.d_Gdev = reinterpret_cast<double*>(base + pl.static_Gdev);
+  }
   pl.timing = false;
   const int rc = run(pl, d_controls, d_F, d_G, st);
   pl.d_probs = own_probs;
   pl.d_part = own_part;
+  pl.d_udev = own_udev;
+  pl.d_Fdev = own_Fdev;
+  pl.d_Gdev = own_Gdev;
   pl.timing = own_timing;
   return rc;
 }
@@ -1720,7 +1983,7 @@ int pr_plan_upload_controls(void* plan, const double* const* controls) {
   if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
   int rc = stage_controls(pl, controls, "pr_plan_upload_controls");
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(pl.d_controls, pl.h_stage, sizeof(double) * pl.n_params, hipMemcpyHostToDevice, pl.stream));
+  HIPCHK(hipMemcpyAsync(pl.d_controls, pl.h_stage, sizeof(double) * pl.n_uparams, hipMemcpyHostToDevice, pl.stream));
   HIPCHK(hipStreamSynchronize(pl.stream));
   return PR_OK;
 }
@@ -1764,8 +2027,11 @@ int pr_plan_exp_counts(void* plan, long long* counts) {
   Plan& pl = *static_cast<Plan*>(plan);
   if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
   HIPCHK(hipStreamSynchronize(pl.stream));
-  for (int p = 0; p < pl.n_probs; p++) {
-    const pr::ProbDev& D = pl.host_probs[p];
+  for (int p = 0; p < pl.n_user; p++) {
+    for (int i = 0; i < 4; i++) counts[4 * p + i] = 0;
+    // a pack's exps are reported by its first problem
+    if (pl.packed && pl.umap[p].y > 0) continue;
+    const pr::ProbDev& D = pl.host_probs[pl.packed ? pl.umap[p].x : p];
     std::vector<int> sm((size_t)4 * D.M * D.nseg);
     HIPCHK(hipMemcpy(sm.data(), D.smax, sizeof(int) * sm.size(), hipMemcpyDeviceToHost));
     long long n9 = 0, nsq = 0, nredo = 0;
@@ -1786,8 +2052,8 @@ int pr_plan_shapes(void* plan, int* shapes) {
   if (!plan || !shapes) return fail(PR_ERR_ARG, "pr_plan_shapes: NULL argument");
   const Plan& pl = *static_cast<Plan*>(plan);
   auto group_of = [](const pr::ProbDev& D) { return (D.obj == PR_TRACKING ? 8 : 0) + D.B; };
-  for (int p = 0; p < pl.n_probs; p++) {
-    const pr::ProbDev& D = pl.host_probs[p];
+  for (int p = 0; p < pl.n_user; p++) {
+    const pr::ProbDev& D = pl.host_probs[pl.packed ? pl.umap[p].x : p];
     int regs = 0, np = 0;
     for (const auto& g : pl.groups)
       if (g.gid == group_of(D)) {
@@ -1804,7 +2070,7 @@ int pr_plan_shapes(void* plan, int* shapes) {
 
 long long pr_plan_n_params(void* plan) {
   if (!plan) return -1;
-  return static_cast<Plan*>(plan)->n_params;
+  return static_cast<Plan*>(plan)->n_uparams;
 }
 
 int pr_grape_optimize(void* fid_plan, void* grad_plan, const double* controls_init, const pr_grape_config* cfg,
@@ -1815,13 +2081,13 @@ int pr_grape_optimize(void* fid_plan, void* grad_plan, const double* controls_in
     return fail(PR_ERR_ARG, "pr_grape_optimize: NULL argument");
   Plan& fp = *static_cast<Plan*>(fid_plan);
   Plan* gp = static_cast<Plan*>(grad_plan);
-  if (fp.n_probs != 1 || (gp && (gp->n_probs != 1 || gp->n_params != fp.n_params)))
+  if (fp.n_user != 1 || (gp && (gp->n_user != 1 || gp->n_uparams != fp.n_uparams)))
     return fail(PR_ERR_ARG, "pr_grape_optimize: plans must hold one problem each, with the same controls");
   if (cfg->max_iter <= 0 || !(cfg->step_size > 0) || !(cfg->min_step_size > 0) ||
       cfg->max_step_size < cfg->step_size || cfg->min_step_size > cfg->step_size)
     return fail(PR_ERR_ARG, "pr_grape_optimize: invalid configuration");
   if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
-  const long long n = fp.n_params;
+  const long long n = fp.n_uparams;
   hipStream_t st = fp.stream;
   DevBuf best, part, nrm;
   HIPCHK(hipMalloc(&best.p, sizeof(double) * n));
@@ -1938,11 +2204,11 @@ int pr_lbfgs_optimize(void* plan, const double* controls_init, const pr_lbfgs_co
   if (!plan || !controls_init || !cfg || !controls_best || !fidelity_history || !gnorm_history || !result)
     return fail(PR_ERR_ARG, "pr_lbfgs_optimize: NULL argument");
   Plan& pl = *static_cast<Plan*>(plan);
-  if (pl.n_probs != 1) return fail(PR_ERR_ARG, "pr_lbfgs_optimize: the plan must hold one problem");
+  if (pl.n_user != 1) return fail(PR_ERR_ARG, "pr_lbfgs_optimize: the plan must hold one problem");
   if (cfg->max_iter <= 0 || !(cfg->step_size > 0) || cfg->lbfgs_memory < 1 || cfg->lbfgs_memory > 64)
     return fail(PR_ERR_ARG, "pr_lbfgs_optimize: invalid configuration");
   if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
-  const long long n = pl.n_params;
+  const long long n = pl.n_uparams;
   const int mem = cfg->lbfgs_memory;
   const bool clip = cfg->amplitude_limit > 0;
   hipStream_t st = pl.stream;
@@ -2093,7 +2359,7 @@ int pr_mr_optimcon(void* plan, const double* controls_init, const pr_mr_config* 
   if (!plan || !controls_init || !cfg || !controls_best || !fidelity_history || !gnorm_history || !result)
     return fail(PR_ERR_ARG, "pr_mr_optimcon: NULL argument");
   Plan& pl = *static_cast<Plan*>(plan);
-  if (pl.n_probs != 1) return fail(PR_ERR_ARG, "pr_mr_optimcon: the plan must hold one problem");
+  if (pl.n_user != 1) return fail(PR_ERR_ARG, "pr_mr_optimcon: the plan must hold one problem");
   const int f = cfg->tile;
   if ((cfg->method != PR_MR_LBFGS && cfg->method != PR_MR_GRAPE) || cfg->max_iter < 0 || f < 1 ||
       cfg->lbfgs_memory < 1 || cfg->lbfgs_memory > 64 || !(cfg->u_bound >= cfg->l_bound))

@@ -61,6 +61,7 @@ struct PROptions
     grad_mode::Cint
     segment::Cint
     store_props::Cint
+    pack::Cint
 end
 
 """
@@ -75,7 +76,7 @@ function rocm_init!(; lib::AbstractString = get(ENV, "PULSAR_ROCM_LIB", "libpuls
     h = Libc.Libdl.dlopen(lib; throw_error = false)
     h === nothing && return false
     _ROCM_LIB[] = lib
-    ccall(Libc.Libdl.dlsym(h, :pr_abi_version), Cint, ()) == 1 || return false
+    ccall(Libc.Libdl.dlsym(h, :pr_abi_version), Cint, ()) == 2 || return false
     ccall(Libc.Libdl.dlsym(h, :pr_device_count), Cint, ()) > 0 || return false
     ccall(Libc.Libdl.dlsym(h, :pr_init), Cint, (Cint,), device) == 0 || return false
     _ROCM_LOADED[] = true
@@ -104,9 +105,10 @@ mutable struct ROCmPlan
     n_ctrl::Vector{Int}
     n_t::Vector{Int}
     function ROCmPlan(probs::Vector{PRProblem}, roots::Vector{Any};
-                      grad_mode::Integer = 0, precision::Integer = 0, segment::Integer = 0)
+                      grad_mode::Integer = 0, precision::Integer = 0, segment::Integer = 0,
+                      pack::Integer = 1)
         h = Ref{Ptr{Cvoid}}(C_NULL)
-        opt = Ref(PROptions(precision, grad_mode, segment, -1))
+        opt = Ref(PROptions(precision, grad_mode, segment, -1, pack))
         GC.@preserve roots probs begin
             _rocm_check(ccall((:pr_plan_create, _ROCM_LIB[]), Cint,
                               (Ptr{PRProblem}, Cint, Ptr{PROptions}, Ptr{Ptr{Cvoid}}),

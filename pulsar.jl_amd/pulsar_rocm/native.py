@@ -144,7 +144,7 @@ class Plan:
     """A device plan over one or more problems (pr_plan_create)."""
 
     def __init__(self, problems: Sequence[Problem], precision: int = 0, grad_mode: int = 0,
-                 segment: int = 0, store_props: int = -1):
+                 segment: int = 0, store_props: int = -1, pack: int = 1):
         L = load()
         self.problems = list(problems)
         n = len(self.problems)
@@ -156,7 +156,7 @@ class Plan:
             arr[i] = to_c(p)
             self._keeps.append(p._keep)
         self._structs = arr
-        opt = COptions(int(precision), int(grad_mode), int(segment), int(store_props))
+        opt = COptions(int(precision), int(grad_mode), int(segment), int(store_props), int(pack))
         h = ctypes.c_void_p()
         check(L.pr_plan_create(ctypes.cast(arr, ctypes.c_void_p), n, ctypes.byref(opt), ctypes.byref(h)))
         self._h = h

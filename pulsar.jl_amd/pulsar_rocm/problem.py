@@ -147,6 +147,7 @@ class COptions(ctypes.Structure):
         ("grad_mode", ctypes.c_int),
         ("segment", ctypes.c_int),
         ("store_props", ctypes.c_int),
+        ("pack", ctypes.c_int),
     ]
 
 

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_no_device_here():
     L = native.load()
-    assert L.pr_abi_version() == 1
+    assert L.pr_abi_version() == 2
     if not os.path.exists("/dev/kfd"):
         assert native.device_count() == 0
 

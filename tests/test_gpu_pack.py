@@ -1,0 +1,190 @@
+"""Packing of small problems (pr_options.pack): closed-dynamics gate problems
+with one member and dim <= 8 run as diagonal blocks of one problem of dim <= 16.
+
+Each problem keeps its own F and G (compute_fidelity / compute_grape_gradient
+per problem, Objectives.jl:609-639, Gradients.jl:80-163): packed plans are
+checked against the FP64 oracle at the parity tolerances, against the same
+plan without packing at rounding level, and their device paths (host buffers,
+device buffers, static scratch, graph replays) against each other bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from pulsar_rocm import native, workloads as W
+from pulsar_rocm.problem import (PR_GATE, PR_GRAD_EXACT, PR_MIXED, PR_PAIRS_SQUARE,
+                                 PR_SEGMENT_AUTOTUNE, Problem, controls_cm)
+
+pytestmark = pytest.mark.gpu
+
+# (imported before the library initialises the device: imported after it,
+# torch found no GPU in the process)
+torch = pytest.importorskip("torch")
+
+TOL_F = 1e-12
+TOL_G = 1e-10
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    native.init(0)
+
+
+def small_gates(n, seed, dims=(1, 2, 3, 4, 5, 6, 7, 8), n_t=(30, 90), dts=(0.02, 0.03)):
+    """Independent gate problems: every dimension up to 8, lengths in n_t, two
+    slice lengths, 1-3 controls, some with control scales or an explicit weight 1."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        d = int(rng.choice(dims))
+        nc = int(rng.integers(1, 4))
+        nt = int(rng.integers(n_t[0], n_t[1] + 1))
+        kw = {}
+        if i % 3 == 1:
+            kw["ctrl_scale"] = rng.uniform(0.8, 1.2, (1, nc))
+        if i % 5 == 2:
+            kw["weight"] = np.ones(1)
+        p = Problem(dim=d, n_ctrl=nc, n_t=nt, dt=float(dts[i % len(dts)]),
+                    H0=W.random_hermitian(d, rng, 3.0) if d > 1 else np.array([[1.3 + 0j]]),
+                    Hc=np.stack([W.random_hermitian(d, rng, 2.0) if d > 1 else np.array([[0.7 + 0j]])
+                                 for _ in range(nc)]),
+                    objective=PR_GATE, U_target=W.haar_unitary(d, rng), **kw)
+        out.append((p, rng.uniform(-3, 3, (nc, nt))))
+    return out
+
+
+def evaluate(probs, us, **kw):
+    plan = native.Plan(probs, **kw)
+    try:
+        F, G, D = plan.eval(us, diag=True)
+        counts = plan.exp_counts()
+    finally:
+        plan.close()
+    return F, G, D, counts
+
+
+def packed_rows(counts):
+    """Problems whose exps were reported by their pack's first problem."""
+    return int(np.sum(np.all(counts == 0, axis=1)))
+
+
+def g_close(g, g0, rel):
+    """max|g - g0| <= rel max|g0|; a d = 1 problem's gradient is zero up to
+    rounding (|F| = 1 whatever the controls), hence the floor of 1e-4."""
+    return np.abs(g - g0).max() <= rel * max(np.abs(g0).max(), 1e-4)
+
+
+def check_oracle(probs, us, F, G, grad_mode=0, tol_f=TOL_F, tol_g=TOL_G):
+    for i, (p, u) in enumerate(zip(probs, us)):
+        F0, G0 = O.evaluate(p, u, grad_mode=grad_mode)
+        assert abs(F[i] - F0) <= tol_f, (i, p.dim, p.n_t, F[i], F0)
+        assert g_close(G[i], G0, tol_g), (i, p.dim, p.n_t)
+
+
+def test_packed_plan_matches_unpacked_plan_and_oracle():
+    batch = small_gates(48, seed=1)
+    probs, us = [b[0] for b in batch], [b[1] for b in batch]
+    print("packing", [(p.dim, p.n_t, p.dt) for p in probs], flush=True)
+    F1, G1, D1, c1 = evaluate(probs, us, pack=1)
+    F0, G0, D0, c0 = evaluate(probs, us, pack=0)
+    assert packed_rows(c0) == 0
+    assert packed_rows(c1) >= 10, c1  # packs were formed
+    # exps actually run: the packs' slices instead of every problem's
+    assert c1[:, :2].sum() < c0[:, :2].sum()
+    assert np.abs(F1 - F0).max() <= 1e-13
+    for g1, g0 in zip(G1, G0):
+        assert g_close(g1, g0, 1e-11)
+    assert D1[:, 0].max() < 1e-12  # unitarity defect of every pack's U_total
+    check_oracle(probs, us, F1, G1)
+
+
+def test_packing_leaves_other_problems_to_their_own_kernels():
+    # packable problems beside an ensemble, a pairs objective, a Lindblad
+    # problem, per-slice dt, a weight other than 1 and a 2 x 2-block problem
+    rng = np.random.default_rng(7)
+    batch = small_gates(20, seed=4, n_t=(40, 44), dts=(0.025,))
+    e, ue = W.four_qubit_ensemble(n_t=40)
+    e, ue = W.shrink(e, ue, n_members=3)
+    d = 5
+    pairs = Problem(dim=d, n_ctrl=2, n_t=30, dt=0.05, H0=W.random_hermitian(d, rng, 2.0),
+                    Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)]),
+                    objective=PR_PAIRS_SQUARE,
+                    psi_init=np.stack([W.random_state(d, rng) for _ in range(2)]),
+                    psi_targ=np.stack([W.random_state(d, rng) for _ in range(2)]))
+    lind, ul = W.three_qubit_lindblad(n_t=8, n_members=1, objective=PR_PAIRS_SQUARE)
+    d = 3
+    base = dict(dim=d, n_ctrl=2, H0=W.random_hermitian(d, rng, 2.0),
+                Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(2)]), objective=PR_GATE,
+                U_target=W.haar_unitary(d, rng))
+    per_slice = Problem(n_t=42, dt=0.025, dt_slices=rng.uniform(0.01, 0.04, 42), **base)
+    weighted = Problem(n_t=42, dt=0.025, weight=np.array([0.5]), **base)
+    d = 24
+    big = Problem(dim=d, n_ctrl=2, n_t=20, dt=0.01, H0=W.random_hermitian(d, rng, 5.0),
+                  Hc=np.stack([W.random_hermitian(d, rng, 2.0) for _ in range(2)]), objective=PR_GATE,
+                  U_target=W.haar_unitary(d, rng))
+    others = [(e, ue), (pairs, rng.uniform(-1, 1, (2, 30))), (lind, ul),
+              (per_slice, rng.uniform(-1, 1, (2, 42))), (weighted, rng.uniform(-1, 1, (2, 42))),
+              (big, rng.uniform(-1, 1, (2, 20)))]
+    # the other problems sit between the packable ones
+    mixed = batch[:7] + others[:3] + batch[7:14] + others[3:] + batch[14:]
+    probs, us = [b[0] for b in mixed], [b[1] for b in mixed]
+    F, G, D, c = evaluate(probs, us)
+    assert packed_rows(c) >= 8
+    check_oracle(probs, us, F, G)
+
+
+def test_packed_exact_gradient_and_recompute():
+    batch = small_gates(24, seed=9, n_t=(50, 53))
+    probs, us = [b[0] for b in batch], [b[1] for b in batch]
+    Fe, Ge, _, c = evaluate(probs, us, grad_mode=PR_GRAD_EXACT)
+    Fu, Gu, _, _ = evaluate(probs, us, grad_mode=PR_GRAD_EXACT, pack=0)
+    assert packed_rows(c) >= 10
+    assert np.abs(Fe - Fu).max() <= 1e-13
+    for a, b in zip(Ge, Gu):
+        assert g_close(a, b, 1e-11)
+    # without stored images the walk rebuilds the pack's propagators bit for bit
+    for mode in (0, PR_GRAD_EXACT):
+        Fs, Gs, _, _ = evaluate(probs, us, grad_mode=mode, store_props=1)
+        Fr, Gr, _, _ = evaluate(probs, us, grad_mode=mode, store_props=0)
+        assert np.array_equal(Fs, Fr)
+        assert all(np.array_equal(a, b) for a, b in zip(Gs, Gr))
+    check_oracle(probs, us, Fe, Ge, grad_mode=PR_GRAD_EXACT)
+
+
+def test_packed_device_paths_agree_bit_for_bit():
+    batch = small_gates(30, seed=12, n_t=(60, 62), dts=(0.02,))
+    probs, us = [b[0] for b in batch], [b[1] for b in batch]
+    plan = native.Plan(probs)
+    try:
+        F, G = plan.eval(us)
+        F2, G2 = plan.eval(us)  # a replay of the captured graph
+        assert np.array_equal(F, F2) and all(np.array_equal(a, b) for a, b in zip(G, G2))
+        assert packed_rows(plan.exp_counts()) >= 15
+        dc = torch.tensor(np.concatenate([controls_cm(u) for u in us]), dtype=torch.float64, device="cuda")
+        gref = np.concatenate([controls_cm(g) for g in G])
+        st = torch.cuda.current_stream().cuda_stream
+        for f in (plan.eval_device, plan.eval_device_static, plan.eval_device):
+            dF = torch.full((len(us),), np.nan, dtype=torch.float64, device="cuda")
+            dG = torch.full((plan.n_params,), np.nan, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            f(dc.data_ptr(), dF.data_ptr(), dG.data_ptr(), st)
+            torch.cuda.synchronize()
+            assert np.array_equal(dF.cpu().numpy(), F), f.__name__
+            assert np.array_equal(dG.cpu().numpy(), gref), f.__name__
+    finally:
+        plan.close()
+
+
+def test_packed_fp32_form_and_autotuned_plan():
+    batch = small_gates(32, seed=21, n_t=(70, 74), dts=(0.02,))
+    probs, us = [b[0] for b in batch], [b[1] for b in batch]
+    Fm, Gm, _, c = evaluate(probs, us, precision=PR_MIXED)
+    assert packed_rows(c) >= 10
+    F0, G0, _, _ = evaluate(probs, us, pack=0)
+    # the FP32 form's gate against FP64 (|dF| <= 1e-9, relative gradient 1e-6)
+    assert np.abs(Fm - F0).max() <= 1e-9
+    for a, b in zip(Gm, G0):
+        assert g_close(a, b, 1e-6)
+    Fa, Ga, _, c = evaluate(probs, us, segment=PR_SEGMENT_AUTOTUNE)
+    assert packed_rows(c) >= 10
+    check_oracle(probs, us, Fa, Ga)

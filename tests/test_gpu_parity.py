@@ -295,7 +295,7 @@ def test_segment_autotuner_plan_matches_oracle():
     from pulsar_rocm.problem import PR_SEGMENT_AUTOTUNE
     batch = W.uneven_batch(n_problems=24, seed=31, n_min=40, n_max=160)
     probs, us = [b[0] for b in batch], [b[1] for b in batch]
-    plan = native.Plan(probs, segment=PR_SEGMENT_AUTOTUNE)
+    plan = native.Plan(probs, segment=PR_SEGMENT_AUTOTUNE, pack=0)  # (packed plans: tests/test_gpu_pack.py)
     try:
         F, G, D = plan.eval(us, diag=True)
         F2, G2 = plan.eval(us)
