@@ -92,6 +92,10 @@ struct ProbDev {
   int blk_row[17];
   int ncg;                    // channels 0 .. ncg-1 have a gradient (nc, except packed
                               // problems, whose per-block drift channels come last)
+  int ck, nck;                // two-level prefix (k_combine_local): ck segments per chunk,
+                              // nck chunks; ck = 0: one chain over the segments
+  double2* Qc;                // [M][nck][NB][256] chunk products, then the prefix at each
+                              // chunk's start
   // d <= 16 closed dynamics: the transposed drift images and operator tables
   // (X^T is assembled directly, so exp gives U^T in the register layout)
   const double2* H0t;
@@ -796,16 +800,31 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
   constexpr int NB = B * B;
   const size_t MB = (size_t)NB * 256;
   Blk E = eye_blk(x);
-  store_blk(P.Eseg + ((size_t)m * (P.nseg + 1)) * MB, E, x);
-  // T_{s+1} is fetched while the product with T_s runs (the prefix is a
-  // sequential chain, so an unhidden load would cost a round trip per step)
-  Blk Ts = load_blk(P.Tseg + ((size_t)m * P.nseg) * MB, x);
-  for (int s = 0; s < P.nseg; s++) {
-    Blk Tn = Ts;
-    if (s + 1 < P.nseg) Tn = load_blk(P.Tseg + ((size_t)m * P.nseg + s + 1) * MB, x);
-    E = mm<B, N, N>(Ts, E, LA, LB, x);
-    store_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + s + 1) * MB, E, x);
-    Ts = Tn;
+  if (P.ck > 0) {
+    // two-level prefix: chain the chunk products Q_c (k_combine_local);
+    // slot c of Qc becomes R_c, the prefix at chunk c's start, and
+    // k_combine_apply forms the E_s inside the chunks
+    Blk Q = load_blk(P.Qc + (size_t)m * P.nck * MB, x);
+    for (int c = 0; c < P.nck; c++) {
+      Blk Qn = Q;
+      if (c + 1 < P.nck) Qn = load_blk(P.Qc + ((size_t)m * P.nck + c + 1) * MB, x);
+      store_blk(P.Qc + ((size_t)m * P.nck + c) * MB, E, x);
+      E = mm<B, N, N>(Q, E, LA, LB, x);
+      Q = Qn;
+    }
+    store_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + P.nseg) * MB, E, x);
+  } else {
+    store_blk(P.Eseg + ((size_t)m * (P.nseg + 1)) * MB, E, x);
+    // T_{s+1} is fetched while the product with T_s runs (the prefix is a
+    // sequential chain, so an unhidden load would cost a round trip per step)
+    Blk Ts = load_blk(P.Tseg + ((size_t)m * P.nseg) * MB, x);
+    for (int s = 0; s < P.nseg; s++) {
+      Blk Tn = Ts;
+      if (s + 1 < P.nseg) Tn = load_blk(P.Tseg + ((size_t)m * P.nseg + s + 1) * MB, x);
+      E = mm<B, N, N>(Ts, E, LA, LB, x);
+      store_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + s + 1) * MB, E, x);
+      Ts = Tn;
+    }
   }
   const Blk& Ut = E;  // U_total
   if (P.unitary) {
@@ -937,6 +956,60 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
     store_blk(P.Bb + ((size_t)m * P.nseg + s) * MB, Lm, x);
   }
   __syncthreads();
+}
+
+// Two-level prefix of the segment products, for groups of few members with
+// many segments, where k_combine's chain of S products on one workgroup per
+// member would leave most CUs idle. k_combine_local runs one workgroup per
+// chunk of ck segments (item: seg = chunk c, segments [k0, k1)): it writes
+// the chunk-local prefixes T_{s-1} ... T_{c ck} into Eseg[s] (the identity at
+// the chunk's first segment) and the chunk product into Qc[c]. k_combine then
+// chains the nck chunk products, and k_combine_apply (one workgroup per segment
+// beyond the first chunk) forms E_s = (T_{s-1} ... T_{c ck}) R_c.
+template <int B>
+__global__ __launch_bounds__(64 * B * B) void k_combine_local(const ProbDev* __restrict__ probs,
+                                                               const WorkItem* __restrict__ items) {
+  const WorkItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const int m = it.member;
+  const Ln x = lane_info<B>();
+  double2 *LA, *LB, *zb;
+  double* red;
+  smem_parts<B>(LA, LB, red, zb);
+  constexpr int NB = B * B;
+  const size_t MB = (size_t)NB * 256;
+  Blk E = eye_blk(x);
+  store_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + it.k0) * MB, E, x);
+  E = load_blk(P.Tseg + ((size_t)m * P.nseg + it.k0) * MB, x);
+  Blk Ts = E;
+  if (it.k0 + 1 < it.k1) Ts = load_blk(P.Tseg + ((size_t)m * P.nseg + it.k0 + 1) * MB, x);
+  for (int s = it.k0 + 1; s < it.k1; s++) {
+    store_blk(P.Eseg + ((size_t)m * (P.nseg + 1) + s) * MB, E, x);
+    Blk Tn = Ts;
+    if (s + 1 < it.k1) Tn = load_blk(P.Tseg + ((size_t)m * P.nseg + s + 1) * MB, x);
+    E = mm<B, N, N>(Ts, E, LA, LB, x);
+    Ts = Tn;
+  }
+  store_blk(P.Qc + ((size_t)m * P.nck + it.seg) * MB, E, x);
+}
+
+template <int B>
+__global__ __launch_bounds__(64 * B * B) void k_combine_apply(const ProbDev* __restrict__ probs,
+                                                               const WorkItem* __restrict__ items) {
+  const WorkItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  if (P.ck == 0 || it.seg < P.ck) return;
+  const int m = it.member;
+  const Ln x = lane_info<B>();
+  double2 *LA, *LB, *zb;
+  double* red;
+  smem_parts<B>(LA, LB, red, zb);
+  constexpr int NB = B * B;
+  const size_t MB = (size_t)NB * 256;
+  double2* Es = P.Eseg + ((size_t)m * (P.nseg + 1) + it.seg) * MB;
+  const Blk L = load_blk(Es, x);
+  const Blk R = load_blk(P.Qc + ((size_t)m * P.nck + it.seg / P.ck) * MB, x);
+  store_blk(Es, mm<B, N, N>(L, R, LA, LB, x), x);
 }
 
 // Gradient contraction of one slice: G_m[j,k] = Re(c dt s_mj Tr(A_j M)) with

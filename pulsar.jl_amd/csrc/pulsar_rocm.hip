@@ -41,7 +41,7 @@ namespace pr {
 // Byte offsets into the static scratch of one problem's scratch buffers (-1:
 // the plan allocates none of that buffer for the problem).
 struct StaticOff {
-  long long Ust, Udm, big, seg_big, Tseg, Eseg, Om, Bb, Pscr, Fm, Gm, udef, smax, Zseg, Pend, Lend;
+  long long Ust, Udm, big, seg_big, Tseg, Eseg, Om, Bb, Pscr, Fm, Gm, udef, smax, Zseg, Pend, Lend, Qc;
 };
 
 // dst[p] = src[p] with every scratch pointer moved into the static scratch at
@@ -69,6 +69,7 @@ __global__ void k_bind_static(const ProbDev* __restrict__ src, ProbDev* __restri
   d.Zseg = (double2*)at(o.Zseg);
   d.Pend = (double2*)at(o.Pend);
   d.Lend = (double2*)at(o.Lend);
+  d.Qc = (double2*)at(o.Qc);
   dst[p] = d;
 }
 
@@ -406,6 +407,8 @@ struct Plan {
     double2* big_scratch = nullptr;  // d > 64: per work item matrices in HBM
     pr::WorkItem* d_items = nullptr;
     int n_items = 0;
+    pr::WorkItem* d_citems = nullptr;  // chunks of the two-level prefix (k_combine_local), if any
+    int n_citems = 0;
     int2* d_members = nullptr;
     int n_members = 0;
   };
@@ -544,6 +547,10 @@ int set_smem_attr() {
     HIPCHK(hipFuncSetAttribute((const void*)pr::k_segment<B, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     HIPCHK(hipFuncSetAttribute((const void*)pr::k_segment<B, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     HIPCHK(hipFuncSetAttribute((const void*)pr::k_combine<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_combine_local<B>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+    HIPCHK(hipFuncSetAttribute((const void*)pr::k_combine_apply<B>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
     int rc;
     if ((rc = set_grad_attr<B, 2>(bytes)) || (rc = set_grad_attr<B, 4>(bytes)) || (rc = set_grad_attr<B, 8>(bytes)) ||
         (rc = set_grad_attr<B, 16>(bytes)) || (rc = set_grad_attr<B, 32>(bytes)))
@@ -766,8 +773,18 @@ int launch_group(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, h
   HIPCHK(hipGetLastError());
   if ((rc = tend(pl, 0, st))) return rc;
   if ((rc = tbegin(pl, st))) return rc;
+  if (g.n_citems > 0) {
+    hipLaunchKernelGGL((pr::k_combine_local<B>), dim3(g.n_citems), blk, pr::Smem<B>::combine, st, pl.d_probs,
+                       g.d_citems);
+    HIPCHK(hipGetLastError());
+  }
   hipLaunchKernelGGL((pr::k_combine<B>), dim3(g.n_members), blk, pr::Smem<B>::combine, st, pl.d_probs, g.d_members);
   HIPCHK(hipGetLastError());
+  if (g.n_citems > 0) {
+    hipLaunchKernelGGL((pr::k_combine_apply<B>), dim3(g.n_items), blk, pr::Smem<B>::combine, st, pl.d_probs,
+                       g.d_items);
+    HIPCHK(hipGetLastError());
+  }
   if ((rc = tend(pl, 1, st))) return rc;
   if (grad) {
     if ((rc = tbegin(pl, st))) return rc;
@@ -995,7 +1012,7 @@ int static_layout(Plan& pl) {
     o.Ust = place(z.Ust); o.Udm = place(z.Udm); o.big = place(z.big); o.seg_big = place(z.seg_big);
     o.Tseg = place(z.Tseg); o.Eseg = place(z.Eseg); o.Om = place(z.Om); o.Bb = place(z.Bb);
     o.Pscr = place(z.Pscr); o.Fm = place(z.Fm); o.Gm = place(z.Gm); o.smax = place(z.smax);
-    o.Zseg = place(z.Zseg); o.Pend = place(z.Pend); o.Lend = place(z.Lend);
+    o.Zseg = place(z.Zseg); o.Pend = place(z.Pend); o.Lend = place(z.Lend); o.Qc = place(z.Qc);
   }
   pl.static_part = at;
   at = align(at + (long long)sizeof(double) * pl.red_chunks * pl.n_params);
@@ -1186,9 +1203,18 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
   std::vector<int> tr_nnz_max(13, 0);  // most nonzeros of one control, per group
   std::vector<int> nc_max(13, 0);
   std::vector<double> group_work(13, 0.0);  // member-slices per group
+  std::vector<int> group_members(13, 0);
+  std::vector<char> group_lind(13, 0);
   for (int p = 0; p < n_probs; p++) {
-    group_work[group_of(probs[p])] += (double)probs[p].n_members * probs[p].n_t;
+    const int gi = group_of(probs[p]);
+    group_work[gi] += (double)probs[p].n_members * probs[p].n_t;
+    group_members[gi] += probs[p].n_members;
+    if (probs[p].dynamics != PR_CLOSED) group_lind[gi] = 1;
   }
+  // two-level prefix (k_combine_local): multi-block groups (2 <= B <= 4) of
+  // closed dynamics whose members cannot fill the CUs (< 128 of them), for
+  // problems of 16 or more segments, in chunks of about sqrt(S) segments
+  std::vector<std::vector<pr::WorkItem>> citems(13);
   // the mixed-precision kernels cover single-wavefront (d <= 16) unitary
   // problems under the reference gradient formula; any other problem of the
   // plan, or a d <= 16 group holding a Lindblad problem, runs in FP64
@@ -1445,7 +1471,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
     }
     // scratch (the byte size of each buffer is recorded for the static layout)
     pr::StaticOff& z = pl.static_sz[p];
-    z = pr::StaticOff{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    z = pr::StaticOff{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #define PR_SCRATCH(field, count)                                              \
   do {                                                                        \
     const size_t cnt_ = (size_t)(count);                                      \
@@ -1463,6 +1489,17 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
     }
     PR_SCRATCH(Tseg, (size_t)M * S * MB);
     PR_SCRATCH(Eseg, (size_t)M * (S + 1) * MB);
+    if (B >= 2 && B <= 4 && !track && !group_lind[gi] && group_members[gi] < 128 && S >= 16) {
+      D.ck = (int)std::ceil(std::sqrt((double)S));
+      D.nck = (S + D.ck - 1) / D.ck;
+      PR_SCRATCH(Qc, (size_t)M * D.nck * MB);
+      for (int m = 0; m < M; m++)
+        for (int c = 0; c < D.nck; c++) {
+          pr::WorkItem w;
+          w.prob = p; w.member = m; w.seg = c; w.k0 = c * D.ck; w.k1 = std::min(S, (c + 1) * D.ck);
+          citems[gi].push_back(w);
+        }
+    }
     PR_SCRATCH(Om, (size_t)M * MB);
     if (!D.unitary) {
       PR_SCRATCH(Bb, (size_t)M * S * MB);
@@ -1579,6 +1616,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
     g.trace_regs = pl.tune_regs[gi] && B == 1 && vmax[gi] >= nc_max[gi] &&
                    tr_nnz_max[gi] <= pr::TR_REG * (64 / vmax[gi]);
     if ((rc = pl.upload(&g.d_items, items[gi].data(), items[gi].size()))) return rc;
+    g.n_citems = (int)citems[gi].size();
+    if (g.n_citems > 0 && (rc = pl.upload(&g.d_citems, citems[gi].data(), citems[gi].size()))) return rc;
     if ((rc = pl.upload(&g.d_members, members[gi].data(), members[gi].size()))) return rc;
     pl.groups.push_back(g);
     if (B > 4) {

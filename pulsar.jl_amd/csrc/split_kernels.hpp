@@ -310,7 +310,10 @@ __device__ inline Sp split_exp_slice(const Blk& X, const f4& diag, Blk32& y, Blk
 // before an FP64 product folds it into the running segment product.
 constexpr int SPLIT_GROUP = 128;
 // doubles of controls the split segment kernel stages in LDS at a time
-constexpr int SPLIT_CTRL_CHUNK = 256;
+#ifndef PR_SPLIT_CTRL_CHUNK
+#define PR_SPLIT_CTRL_CHUNK 256
+#endif
+constexpr int SPLIT_CTRL_CHUNK = PR_SPLIT_CTRL_CHUNK;
 // its LDS: two tiles and the control chunk
 constexpr size_t SPLIT_SEGMENT_SMEM = Smem<1>::segment + SPLIT_CTRL_CHUNK * sizeof(double);
 
