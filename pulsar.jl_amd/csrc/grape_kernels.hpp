@@ -1766,9 +1766,13 @@ __global__ __launch_bounds__(64 * B * B) __attribute__((amdgpu_waves_per_eu(B ==
 // with nch = 1 the first stage writes G itself and the second is not launched.
 constexpr int RED_CHUNKS = 16;
 
-__device__ inline int prob_of(const long long* __restrict__ off, int nprob, long long t) {
-  int lo = 0, hi = nprob - 1;
-  while (lo < hi) {  // last p with off[p] <= t
+// the last p with off[p] <= t, searched between first[b] and first[b + 1]:
+// first[b] is the problem of element 256 b, the first of 256-thread block b
+// (the host's table; n_blocks + 1 entries, the last nprob - 1)
+__device__ inline int prob_of(const long long* __restrict__ off, const int* __restrict__ first, long long t) {
+  const long long b = t >> 8;
+  int lo = first[b], hi = first[b + 1];
+  while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
     if (off[mid] <= t) lo = mid;
     else hi = mid - 1;
@@ -1776,11 +1780,11 @@ __device__ inline int prob_of(const long long* __restrict__ off, int nprob, long
   return lo;
 }
 
-__global__ void k_reduce_G1(const ProbDev* __restrict__ probs, int nprob, const long long* __restrict__ off,
-                            long long total, int nch, double* __restrict__ part) {
+__global__ void k_reduce_G1(const ProbDev* __restrict__ probs, const long long* __restrict__ off,
+                            const int* __restrict__ first, long long total, int nch, double* __restrict__ part) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int p = prob_of(off, nprob, t);
+  const int p = prob_of(off, first, t);
   const ProbDev& P = probs[p];
   const long long e = t - off[p];
   const long long ng = (long long)P.nc * P.nt;
@@ -1824,13 +1828,13 @@ __global__ void __launch_bounds__(64) k_reduce_F(const ProbDev* __restrict__ pro
 // j), j = -1 for a block's drift channel, whose waveform is 1 on the block's
 // own slices. A block's slices beyond its problem's n_t take zero controls and
 // zero drift.
-__global__ void k_pack_controls(const ProbDev* __restrict__ probs, int ndev, const long long* __restrict__ doff,
-                                long long total, const int* __restrict__ dsrc, const int2* __restrict__ chan,
-                                const long long* __restrict__ uoff, const int* __restrict__ unt,
-                                const double* __restrict__ u, double* __restrict__ ud) {
+__global__ void k_pack_controls(const ProbDev* __restrict__ probs, const long long* __restrict__ doff,
+                                const int* __restrict__ dfirst, long long total, const int* __restrict__ dsrc,
+                                const int2* __restrict__ chan, const long long* __restrict__ uoff,
+                                const int* __restrict__ unt, const double* __restrict__ u, double* __restrict__ ud) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int p = prob_of(doff, ndev, t);
+  const int p = prob_of(doff, dfirst, t);
   const long long e = t - doff[p];
   const int s = dsrc[p];
   if (s >= 0) {
@@ -1850,13 +1854,13 @@ __global__ void k_pack_controls(const ProbDev* __restrict__ probs, int ndev, con
 
 // umap[p] for the caller's problem p: (device problem, block or -1 when the
 // problem is not packed, first channel of its block, n_ctrl)
-__global__ void k_unpack_G(long long total, int nuser, const long long* __restrict__ uoff,
+__global__ void k_unpack_G(long long total, const long long* __restrict__ uoff, const int* __restrict__ ufirst,
                            const int4* __restrict__ umap, const ProbDev* __restrict__ probs,
                            const long long* __restrict__ doff, const double* __restrict__ Gd,
                            double* __restrict__ G) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int p = prob_of(uoff, nuser, t);
+  const int p = prob_of(uoff, ufirst, t);
   const long long e = t - uoff[p];
   const int4 mp = umap[p];
   if (mp.y < 0) {

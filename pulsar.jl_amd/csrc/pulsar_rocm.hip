@@ -362,6 +362,8 @@ struct Plan {
   std::vector<void*> allocs;
   pr::ProbDev* d_probs = nullptr;
   long long* d_off = nullptr;
+  int* d_first = nullptr;             // per 256 controls: the problem of the first (pr::prob_of)
+  int* d_ufirst = nullptr;            // the same over the caller's problems (packed plans)
   double* d_controls = nullptr;
   double* d_F = nullptr;
   double* d_G = nullptr;
@@ -902,7 +904,7 @@ int launch_one(Plan& pl, const Plan::Group& g, const double* d_u, bool grad, hip
 int pack_controls(Plan& pl, const double* d_u, hipStream_t st) {
   const long long total = pl.n_params;
   hipLaunchKernelGGL(pr::k_pack_controls, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, pl.d_probs,
-                     pl.n_probs, pl.d_off, total, pl.d_dsrc, pl.d_chan, pl.d_uoff, pl.d_unt, d_u, pl.d_udev);
+                     pl.d_off, pl.d_first, total, pl.d_dsrc, pl.d_chan, pl.d_uoff, pl.d_unt, d_u, pl.d_udev);
   HIPCHK(hipGetLastError());
   return PR_OK;
 }
@@ -930,7 +932,7 @@ int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
     long long total = pl.n_params;
     const int nch = pl.red_chunks;
     hipLaunchKernelGGL(pr::k_reduce_G1, dim3((unsigned)((total + 255) / 256), nch), dim3(256), 0, st, pl.d_probs,
-                       pl.n_probs, pl.d_off, total, nch, nch == 1 ? G : pl.d_part);
+                       pl.d_off, pl.d_first, total, nch, nch == 1 ? G : pl.d_part);
     HIPCHK(hipGetLastError());
     if (nch > 1) {
       hipLaunchKernelGGL(pr::k_reduce_G2, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, nch,
@@ -944,8 +946,8 @@ int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
     HIPCHK(hipGetLastError());
     if (d_G) {
       const long long total = pl.n_uparams;
-      hipLaunchKernelGGL(pr::k_unpack_G, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, pl.n_user,
-                         pl.d_uoff, pl.d_umap, pl.d_probs, pl.d_off, G, d_G);
+      hipLaunchKernelGGL(pr::k_unpack_G, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, pl.d_uoff,
+                         pl.d_ufirst, pl.d_umap, pl.d_probs, pl.d_off, G, d_G);
       HIPCHK(hipGetLastError());
     }
   }
@@ -1548,6 +1550,21 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
   int rc;
   if ((rc = pl.upload(&pl.d_probs, pl.host_probs.data(), n_probs))) return rc;
   if ((rc = pl.upload(&pl.d_off, pl.coff.data(), n_probs + 1))) return rc;
+  // per 256 controls, the problem of the first one (and n - 1 at the end)
+  auto first_table = [](const std::vector<long long>& off, int n) {
+    const long long total = off[n], nb = (total + 255) / 256;
+    std::vector<int> f(nb + 1, n - 1);
+    int p = 0;
+    for (long long b = 0; b < nb; b++) {
+      while (p + 1 < n && off[p + 1] <= 256 * b) p++;
+      f[b] = p;
+    }
+    return f;
+  };
+  {
+    const std::vector<int> f = first_table(pl.coff, n_probs);
+    if ((rc = pl.upload(&pl.d_first, f.data(), f.size()))) return rc;
+  }
   if ((rc = pl.alloc(&pl.d_controls, pl.n_uparams))) return rc;
   if ((rc = pl.alloc(&pl.d_F, n_user))) return rc;
   if ((rc = pl.alloc(&pl.d_G, pl.n_uparams))) return rc;
@@ -1556,6 +1573,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
     for (int p = 0; p < n_user; p++) unt[p] = uprobs[p].n_t;
     pl.umap = pk->umap;
     if ((rc = pl.upload(&pl.d_uoff, pl.ucoff.data(), n_user + 1))) return rc;
+    const std::vector<int> uf = first_table(pl.ucoff, n_user);
+    if ((rc = pl.upload(&pl.d_ufirst, uf.data(), uf.size()))) return rc;
     if ((rc = pl.upload(&pl.d_unt, unt.data(), n_user))) return rc;
     if ((rc = pl.upload(&pl.d_dsrc, pk->dsrc.data(), n_probs))) return rc;
     if ((rc = pl.upload(&pl.d_chan, pk->chan.data(), pk->chan.size()))) return rc;
