@@ -188,3 +188,24 @@ def test_packed_fp32_form_and_autotuned_plan():
     Fa, Ga, _, c = evaluate(probs, us, segment=PR_SEGMENT_AUTOTUNE)
     assert packed_rows(c) >= 10
     check_oracle(probs, us, Fa, Ga)
+
+
+def test_whole_uneven_batch_packed_against_unpacked_and_oracle():
+    # the benchmarked 1024-problem batch (d in {2..64}, 200-2000 slices):
+    # every problem's F and G packed against unpacked, and a sample of the
+    # problems that run behind a longer one of their pack against the oracle
+    batch = W.uneven_batch()
+    probs, us = [b[0] for b in batch], [b[1] for b in batch]
+    F1, G1, _, c1 = evaluate(probs, us)
+    F0, G0, _, _ = evaluate(probs, us, pack=0)
+    behind = np.flatnonzero(np.all(c1 == 0, axis=1))
+    print("packed behind a pack's first problem:", behind.size, flush=True)
+    assert behind.size > 500
+    # full-length chains: the packed exps differ from the unpacked ones in
+    # rounding only (shared scaling choices, block offsets inside the tile)
+    assert np.abs(F1 - F0).max() <= TOL_F
+    for g1, g0 in zip(G1, G0):
+        assert g_close(g1, g0, TOL_G)
+    sample = [int(i) for i in behind[:12]]
+    print("oracle sample", [(probs[i].dim, probs[i].n_t) for i in sample], flush=True)
+    check_oracle([probs[i] for i in sample], [us[i] for i in sample], F1[sample], [G1[i] for i in sample])
