@@ -85,11 +85,14 @@ struct ProbDev {
   const double2* so_v;
   int sl_pass[65];
   int so_pass[2];
-  // packed small problems: the problem is nblk independent gate problems as
-  // diagonal blocks, block b on rows and columns [blk_row[b], blk_row[b + 1]);
-  // the fidelity and the adjoint seed are per block (Fm[b]). nblk = 0 otherwise.
+  // packed small problems: the problem is nblk independent problems as
+  // diagonal blocks, block b on rows and columns [blk_row[b], blk_row[b + 1])
+  // and, for the pair objectives, with pairs [blk_pair[b], blk_pair[b + 1]);
+  // the fidelity is per block (Fm[b]), and so is the adjoint seed of a gate
+  // pack. nblk = 0 otherwise.
   int nblk;
   int blk_row[17];
+  int blk_pair[17];
   int ncg;                    // channels 0 .. ncg-1 have a gradient (nc, except packed
                               // problems, whose per-block drift channels come last)
   int ck, nck;                // two-level prefix (k_combine_local): ck segments per chunk,
@@ -900,12 +903,22 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
   const bool square = (P.obj == 1 || P.obj == 5 || P.obj == 6);
   const bool modulus = P.obj == 7;
   if (threadIdx.x == 0) {
-    double F = 0;
-    for (int s = 0; s < P.npairs; s++) {
-      double2 z = zb[s];
-      F += square ? (z.x * z.x + z.y * z.y) : modulus ? sqrt(z.x * z.x + z.y * z.y) : z.x;
+    auto f = [&](int s) {
+      const double2 z = zb[s];
+      return square ? (z.x * z.x + z.y * z.y) : modulus ? sqrt(z.x * z.x + z.y * z.y) : z.x;
+    };
+    if (P.nblk > 0) {
+      // packed: the pairs of block b give its problem's F
+      for (int b = 0; b < P.nblk; b++) {
+        double F = 0;
+        for (int s = P.blk_pair[b]; s < P.blk_pair[b + 1]; s++) F += f(s);
+        P.Fm[b] = F;
+      }
+    } else {
+      double F = 0;
+      for (int s = 0; s < P.npairs; s++) F += f(s);
+      P.Fm[m] = F;
     }
-    P.Fm[m] = F;
   }
   // w_s = 2 z_s (square) or 1 (real); zero beyond the pairs. The propagator
   // helper's state branch (objective 6) takes Im where the others take Re:
@@ -1853,11 +1866,12 @@ __global__ void k_pack_controls(const ProbDev* __restrict__ probs, const long lo
 }
 
 // umap[p] for the caller's problem p: (device problem, block or -1 when the
-// problem is not packed, first channel of its block, n_ctrl)
+// problem is not packed, first channel of its block, n_ctrl); uw[p] its
+// weight (a packed problem runs at weight 1 and takes its own here)
 __global__ void k_unpack_G(long long total, const long long* __restrict__ uoff, const int* __restrict__ ufirst,
-                           const int4* __restrict__ umap, const ProbDev* __restrict__ probs,
-                           const long long* __restrict__ doff, const double* __restrict__ Gd,
-                           double* __restrict__ G) {
+                           const int4* __restrict__ umap, const double* __restrict__ uw,
+                           const ProbDev* __restrict__ probs, const long long* __restrict__ doff,
+                           const double* __restrict__ Gd, double* __restrict__ G) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const int p = prob_of(uoff, ufirst, t);
@@ -1868,15 +1882,16 @@ __global__ void k_unpack_G(long long total, const long long* __restrict__ uoff, 
     return;
   }
   const long long k = e / mp.w;
-  G[t] = Gd[doff[mp.x] + k * probs[mp.x].nc + mp.z + (e - k * mp.w)];
+  G[t] = uw[p] * Gd[doff[mp.x] + k * probs[mp.x].nc + mp.z + (e - k * mp.w)];
 }
 
-__global__ void k_unpack_F(int nuser, const int4* __restrict__ umap, const ProbDev* __restrict__ probs,
-                           const double* __restrict__ Fd, double* __restrict__ F) {
+__global__ void k_unpack_F(int nuser, const int4* __restrict__ umap, const double* __restrict__ uw,
+                           const ProbDev* __restrict__ probs, const double* __restrict__ Fd,
+                           double* __restrict__ F) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nuser) return;
   const int4 mp = umap[p];
-  F[p] = mp.y < 0 ? Fd[mp.x] : probs[mp.x].Fm[mp.y];
+  F[p] = mp.y < 0 ? Fd[mp.x] : uw[p] * probs[mp.x].Fm[mp.y];
 }
 
 // ---------------------------------------------------------------- GRAPE step

@@ -356,6 +356,7 @@ struct Plan {
   int* d_dsrc = nullptr;              // per problem: the caller's problem, or -1 - first entry of d_chan
   int2* d_chan = nullptr;             // per packed channel: (caller's problem, control j, or -1: drift)
   int4* d_umap = nullptr;
+  double* d_uw = nullptr;             // per caller's problem: its weight (packed problems run at 1)
   double* d_udev = nullptr;           // controls in the plan's own layout (n_params)
   double* d_Fdev = nullptr;           // (n_probs)
   double* d_Gdev = nullptr;           // (n_params)
@@ -942,12 +943,12 @@ int run(Plan& pl, const double* d_u, double* d_F, double* d_G, hipStream_t st) {
   }
   if (pl.packed) {
     hipLaunchKernelGGL(pr::k_unpack_F, dim3((pl.n_user + 255) / 256), dim3(256), 0, st, pl.n_user, pl.d_umap,
-                       pl.d_probs, F, d_F);
+                       pl.d_uw, pl.d_probs, F, d_F);
     HIPCHK(hipGetLastError());
     if (d_G) {
       const long long total = pl.n_uparams;
       hipLaunchKernelGGL(pr::k_unpack_G, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, pl.d_uoff,
-                         pl.d_ufirst, pl.d_umap, pl.d_probs, pl.d_off, G, d_G);
+                         pl.d_ufirst, pl.d_umap, pl.d_uw, pl.d_probs, pl.d_off, G, d_G);
       HIPCHK(hipGetLastError());
     }
   }
@@ -1036,21 +1037,27 @@ int static_layout(Plan& pl) {
 }
 
 // ------------------------------------------------ packing of small problems
-// pr_options.pack: closed-dynamics gate problems with one member, dim <= 8, a
-// uniform slice length and weight 1 become diagonal blocks of one problem of
+// pr_options.pack: closed-dynamics problems with one member, dim <= 8 and a
+// uniform slice length, of the gate objective or a state-pair one (pairs
+// square / real / modulus, state), become diagonal blocks of one problem of
 // dim <= 16, so that one single-wavefront work item evaluates up to 8 of them
 // (a d = 2 problem alone fills 4 of the 256 elements of its 16 x 16 tile).
-// - Problems of a pack share dt and run the pack's n_t (the longest). Block b's
-//   drift is a control channel of its own whose waveform is 1 on the block's
-//   own slices and 0 after, where its controls are 0 too: the propagators of
-//   those slices are exactly the identity on the block.
+// - A pack holds problems of one objective and one dt, at most 16 rows, 16
+//   pairs and 64 channels, and runs the longest n_t. Block b's drift is a
+//   control channel of its own whose waveform is 1 on the block's own slices
+//   and 0 after, where its controls are 0 too: the propagators of those slices
+//   are exactly the identity on the block.
 // - Exp, products and traces of a block-diagonal matrix act block by block; the
 //   kernels differ from an unpacked problem only in the shared scaling
 //   exponent and Taylor degree of a slice (the largest of the blocks'), a
 //   rounding-level difference.
-// - F and the adjoint seed are per block (k_combine); control j of block b is
-//   channel ch0_b + j, and its gradient row is the problem's. The drift
-//   channels come after every block's controls and have no gradient (ncg).
+// - F is per block (k_combine: a gate block's trace, or the sum over its own
+//   pairs), and so is a gate pack's adjoint seed; a pair's seed weight is its
+//   own already. The pack runs at weight 1 and each problem's weight (its
+//   own, or the default 1/n_pairs of the pair objectives) is applied when F
+//   and G are scattered back. Control j of block b is channel ch0_b + j, and
+//   its gradient row is the problem's. The drift channels come after every
+//   block's controls and have no gradient (ncg).
 // Packs are filled in order of decreasing n_t, each holding problems whose n_t
 // is within 1/16 of its longest, so little padding is run.
 struct PackBuild {
@@ -1058,14 +1065,20 @@ struct PackBuild {
   std::vector<std::vector<double>> store;    // host arrays of the packed problems
   std::vector<int> ncg;                      // per problem: channels with a gradient
   std::vector<std::vector<int>> blk_row;     // per problem: block rows (packed) or empty
+  std::vector<std::vector<int>> blk_pair;    // per problem: block pair columns (packed pairs) or empty
   std::vector<int> dsrc;                     // per problem: caller's problem, or -1 - first channel in chan
   std::vector<int2> chan;                    // per packed channel: (caller's problem, j or -1)
   std::vector<int4> umap;                    // per caller's problem: (problem, block or -1, ch0, n_ctrl)
+  std::vector<double> uw;                    // per caller's problem: weight applied when unpacking
 };
 
+bool pair_objective(int obj) {
+  return obj == PR_PAIRS_SQUARE || obj == PR_PAIRS_REAL || obj == PR_PAIRS_MODULUS || obj == PR_STATE;
+}
+
 bool packable(const pr_problem& q) {
-  return q.dynamics == PR_CLOSED && q.objective == PR_GATE && q.n_members == 1 && q.dim <= 8 && !q.dt_slices &&
-         (!q.weight || q.weight[0] == 1.0);
+  return q.dynamics == PR_CLOSED && (q.objective == PR_GATE || pair_objective(q.objective)) && q.n_members == 1 &&
+         q.dim <= 8 && !q.dt_slices;
 }
 
 // false when nothing is packed (fewer than two packable problems fit together)
@@ -1075,22 +1088,25 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
     if (packable(probs[p])) cand.push_back(p);
   if (cand.size() < 2) return false;
   std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) {
+    if (probs[a].objective != probs[b].objective) return probs[a].objective < probs[b].objective;
     if (probs[a].dt != probs[b].dt) return probs[a].dt < probs[b].dt;
     return probs[a].n_t > probs[b].n_t;
   });
   struct Bin {
     std::vector<int> mem;
-    int rows = 0, chans = 0, nt = 0;
+    int rows = 0, pairs = 0, chans = 0, nt = 0, obj = 0;
     double dt = 0;
   };
   std::vector<Bin> bins;
   std::vector<size_t> open;  // indices of bins still taking problems (at most 8)
   for (int p : cand) {
     const pr_problem& q = probs[p];
+    const int np = q.objective == PR_GATE ? 0 : q.n_pairs;
     int hit = -1;
     for (size_t i = 0; i < open.size() && hit < 0; i++) {
       const Bin& b = bins[open[i]];
-      if (b.dt == q.dt && b.rows + q.dim <= 16 && b.chans + q.n_ctrl + 1 <= 64 && b.nt - q.n_t <= b.nt / 16)
+      if (b.obj == q.objective && b.dt == q.dt && b.rows + q.dim <= 16 && b.pairs + np <= 16 &&
+          b.chans + q.n_ctrl + 1 <= 64 && b.nt - q.n_t <= b.nt / 16)
         hit = (int)i;
     }
     if (hit < 0) {
@@ -1098,14 +1114,16 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
       bins.emplace_back();
       bins.back().nt = q.n_t;
       bins.back().dt = q.dt;
+      bins.back().obj = q.objective;
       open.push_back(bins.size() - 1);
       hit = (int)open.size() - 1;
     }
     Bin& b = bins[open[hit]];
     b.mem.push_back(p);
     b.rows += q.dim;
+    b.pairs += np;
     b.chans += q.n_ctrl + 1;
-    if (16 - b.rows < 1) open.erase(open.begin() + hit);
+    if (b.rows == 16 || b.pairs == 16) open.erase(open.begin() + hit);
   }
   std::vector<int> pack_of(n_probs, -1);
   int n_packs = 0;
@@ -1116,6 +1134,7 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
     }
   if (n_packs == 0) return false;
   pk.umap.assign(n_probs, make_int4(0, -1, 0, 0));
+  pk.uw.assign(n_probs, 1.0);
   // unpacked problems first, in the caller's order, then the packs
   for (int p = 0; p < n_probs; p++) {
     if (pack_of[p] >= 0) continue;
@@ -1123,21 +1142,25 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
     pk.dev.push_back(probs[p]);
     pk.ncg.push_back(probs[p].n_ctrl);
     pk.blk_row.emplace_back();
+    pk.blk_pair.emplace_back();
     pk.dsrc.push_back(p);
   }
   for (const Bin& bn : bins) {
     if (bn.mem.size() < 2) continue;
     const int dp = (int)pk.dev.size();
-    int D = 0, ncg = 0;
+    const bool gate = bn.obj == PR_GATE;
+    int D = 0, ncg = 0, npr = 0;
     for (int p : bn.mem) {
       D += probs[p].dim;
       ncg += probs[p].n_ctrl;
+      npr += gate ? 0 : probs[p].n_pairs;
     }
     const int nb = (int)bn.mem.size(), nc = ncg + nb;
     const size_t DD = (size_t)D * D;
-    std::vector<double> H0(2 * DD, 0.0), Hc(2 * DD * nc, 0.0), Ut(2 * DD, 0.0), sc(nc, 1.0);
-    std::vector<int> rows(1, 0);
-    // element (r, c) of a column-major d x d complex matrix into channel ch at block offset o
+    std::vector<double> H0(2 * DD, 0.0), Hc(2 * DD * nc, 0.0), Ut(gate ? 2 * DD : 0, 0.0), sc(nc, 1.0),
+        P0(2 * (size_t)D * npr, 0.0), PT(2 * (size_t)D * npr, 0.0), w1(1, 1.0);
+    std::vector<int> rows(1, 0), pcols(1, 0);
+    // a column-major d x d complex matrix a into dst at element offset base, block offset o
     auto embed = [&](std::vector<double>& dst, size_t base, const double* a, int d, int o) {
       for (int c = 0; c < d; c++)
         for (int r = 0; r < d; r++) {
@@ -1146,7 +1169,7 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
           dst[2 * j + 1] = a[2 * i + 1];
         }
     };
-    int o = 0, ch0 = 0;
+    int o = 0, ch0 = 0, s0 = 0;
     for (int b = 0; b < nb; b++) {
       const int p = bn.mem[b];
       const pr_problem& q = probs[p];
@@ -1158,11 +1181,25 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
         pk.chan.push_back(make_int2(p, j));
       }
       embed(Hc, DD * (ncg + b), q.H0, d, o);
-      embed(Ut, 0, q.U_target, d, o);
+      if (gate) {
+        embed(Ut, 0, q.U_target, d, o);
+        pk.uw[p] = q.weight ? q.weight[0] : 1.0;
+      } else {
+        // pair s of the block: vectors [n_pairs][d] at rows o .. o + d - 1
+        for (int t = 0; t < q.n_pairs; t++)
+          for (int r = 0; r < d; r++)
+            for (int k = 0; k < 2; k++) {
+              P0[2 * ((size_t)(s0 + t) * D + o + r) + k] = q.psi_init[2 * ((size_t)t * d + r) + k];
+              PT[2 * ((size_t)(s0 + t) * D + o + r) + k] = q.psi_targ[2 * ((size_t)t * d + r) + k];
+            }
+        s0 += q.n_pairs;
+        pk.uw[p] = q.weight ? q.weight[0] : 1.0 / q.n_pairs;
+      }
       pk.umap[p] = make_int4(dp, b, ch0, q.n_ctrl);
       o += d;
       ch0 += q.n_ctrl;
       rows.push_back(o);
+      pcols.push_back(s0);
     }
     for (int b = 0; b < nb; b++) pk.chan.push_back(make_int2(bn.mem[b], -1));
     pr_problem z = probs[bn.mem[0]];
@@ -1173,18 +1210,25 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
     z.dt_slices = nullptr;
     z.n_members = 1;
     z.H0_per_member = 0;
-    z.weight = nullptr;
-    pk.store.push_back(std::move(H0));
-    z.H0 = pk.store.back().data();
-    pk.store.push_back(std::move(Hc));
-    z.Hc = pk.store.back().data();
-    pk.store.push_back(std::move(Ut));
-    z.U_target = pk.store.back().data();
-    pk.store.push_back(std::move(sc));
-    z.ctrl_scale = pk.store.back().data();
+    auto keep = [&](std::vector<double>& v) -> const double* {
+      pk.store.push_back(std::move(v));
+      return pk.store.back().data();
+    };
+    z.H0 = keep(H0);
+    z.Hc = keep(Hc);
+    z.ctrl_scale = keep(sc);
+    z.weight = keep(w1);
+    if (gate) {
+      z.U_target = keep(Ut);
+    } else {
+      z.n_pairs = npr;
+      z.psi_init = keep(P0);
+      z.psi_targ = keep(PT);
+    }
     pk.dev.push_back(z);
     pk.ncg.push_back(ncg);
     pk.blk_row.push_back(rows);
+    pk.blk_pair.push_back(gate ? std::vector<int>() : pcols);
     pk.dsrc.push_back(-1 - ((int)pk.chan.size() - nc));
   }
   return true;
@@ -1272,6 +1316,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
     if (pk && !pk->blk_row[p].empty()) {
       D.nblk = (int)pk->blk_row[p].size() - 1;
       std::copy(pk->blk_row[p].begin(), pk->blk_row[p].end(), D.blk_row);
+      std::copy(pk->blk_pair[p].begin(), pk->blk_pair[p].end(), D.blk_pair);
     }
     const bool gate = q.objective == PR_GATE || q.objective == PR_GATE_PROPAGATORS;
     D.npairs = gate ? 0 : q.n_pairs;
@@ -1579,6 +1624,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
     if ((rc = pl.upload(&pl.d_dsrc, pk->dsrc.data(), n_probs))) return rc;
     if ((rc = pl.upload(&pl.d_chan, pk->chan.data(), pk->chan.size()))) return rc;
     if ((rc = pl.upload(&pl.d_umap, pk->umap.data(), n_user))) return rc;
+    if ((rc = pl.upload(&pl.d_uw, pk->uw.data(), n_user))) return rc;
     if ((rc = pl.alloc(&pl.d_udev, pl.n_params))) return rc;
     if ((rc = pl.alloc(&pl.d_Fdev, n_probs))) return rc;
     if ((rc = pl.alloc(&pl.d_Gdev, pl.n_params))) return rc;

@@ -1,8 +1,10 @@
-"""Packing of small problems (pr_options.pack): closed-dynamics gate problems
-with one member and dim <= 8 run as diagonal blocks of one problem of dim <= 16.
+"""Packing of small problems (pr_options.pack): closed-dynamics problems with
+one member and dim <= 8, of the gate or a state-pair objective, run as
+diagonal blocks of one problem of dim <= 16.
 
 Each problem keeps its own F and G (compute_fidelity / compute_grape_gradient
-per problem, Objectives.jl:609-639, Gradients.jl:80-163): packed plans are
+per problem, Objectives.jl:609-639, Gradients.jl:80-163, and the MR state
+kernel's pairs, GRAPEState.jl:218-229): packed plans are
 checked against the FP64 oracle at the parity tolerances, against the same
 plan without packing at rounding level, and their device paths (host buffers,
 device buffers, static scratch, graph replays) against each other bit for bit.
@@ -12,8 +14,8 @@ import pytest
 
 import oracle as O
 from pulsar_rocm import native, workloads as W
-from pulsar_rocm.problem import (PR_GATE, PR_GRAD_EXACT, PR_MIXED, PR_PAIRS_SQUARE,
-                                 PR_SEGMENT_AUTOTUNE, Problem, controls_cm)
+from pulsar_rocm.problem import (PR_GATE, PR_GRAD_EXACT, PR_MIXED, PR_PAIRS_MODULUS, PR_PAIRS_REAL,
+                                 PR_PAIRS_SQUARE, PR_SEGMENT_AUTOTUNE, PR_STATE, Problem, controls_cm)
 
 pytestmark = pytest.mark.gpu
 
@@ -209,3 +211,54 @@ def test_whole_uneven_batch_packed_against_unpacked_and_oracle():
     sample = [int(i) for i in behind[:12]]
     print("oracle sample", [(probs[i].dim, probs[i].n_t) for i in sample], flush=True)
     check_oracle([probs[i] for i in sample], [us[i] for i in sample], F1[sample], [G1[i] for i in sample])
+
+
+def small_states(n, seed, objective, dims=(1, 2, 3, 4, 5, 6, 7, 8), n_t=(40, 43), dt=0.02):
+    """Independent state-pair problems: 1-3 pairs, 1-3 controls, some with
+    control scales, some with an explicit weight."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        d = int(rng.choice(dims))
+        nc = int(rng.integers(1, 4))
+        nt = int(rng.integers(n_t[0], n_t[1] + 1))
+        npairs = int(rng.integers(1, min(d, 3) + 1))
+        kw = {}
+        if i % 3 == 1:
+            kw["ctrl_scale"] = rng.uniform(0.8, 1.2, (1, nc))
+        if i % 4 == 2:
+            kw["weight"] = np.array([0.7])
+        herm = (lambda s: W.random_hermitian(d, rng, s)) if d > 1 else (lambda s: np.array([[0.9 * s + 0j]]))
+        p = Problem(dim=d, n_ctrl=nc, n_t=nt, dt=dt, H0=herm(3.0), Hc=np.stack([herm(2.0) for _ in range(nc)]),
+                    objective=objective, psi_init=np.stack([W.random_state(d, rng) for _ in range(npairs)]),
+                    psi_targ=np.stack([W.random_state(d, rng) for _ in range(npairs)]), **kw)
+        out.append((p, rng.uniform(-3, 3, (nc, nt))))
+    return out
+
+
+@pytest.mark.parametrize("objective,grad_mode", [(PR_PAIRS_SQUARE, 0), (PR_PAIRS_REAL, 0), (PR_PAIRS_MODULUS, 0),
+                                                 (PR_STATE, 0), (PR_PAIRS_SQUARE, PR_GRAD_EXACT)])
+def test_packed_state_pair_objectives(objective, grad_mode):
+    batch = small_states(36, seed=40 + objective, objective=objective)
+    probs, us = [b[0] for b in batch], [b[1] for b in batch]
+    F1, G1, _, c1 = evaluate(probs, us, grad_mode=grad_mode)
+    F0, G0, _, _ = evaluate(probs, us, grad_mode=grad_mode, pack=0)
+    assert packed_rows(c1) >= 12, c1
+    assert np.abs(F1 - F0).max() <= 1e-13
+    for g1, g0 in zip(G1, G0):
+        assert g_close(g1, g0, 1e-11)
+    check_oracle(probs, us, F1, G1, grad_mode=grad_mode)
+
+
+def test_packs_keep_objectives_apart():
+    # gate, pairs and state problems of one plan, interleaved: each objective
+    # packs with its own kind only
+    mixed = []
+    for a, b, c in zip(small_gates(12, seed=51, n_t=(40, 43), dts=(0.02,)),
+                       small_states(12, seed=52, objective=PR_PAIRS_SQUARE),
+                       small_states(12, seed=53, objective=PR_STATE)):
+        mixed += [a, b, c]
+    probs, us = [b[0] for b in mixed], [b[1] for b in mixed]
+    F, G, _, c = evaluate(probs, us)
+    assert packed_rows(c) >= 15
+    check_oracle(probs, us, F, G)
