@@ -508,7 +508,8 @@ def sample_accuracy(workload, probs, us, prec):
 def mr_workload(args):
     """MR state transfer of one spin over 64 offsets x 16 RF power levels
     (1024 members, 500 slices of 2 us): the grape_state_kernel evaluation
-    with the members packed 8 per 16 x 16 block, and unpacked."""
+    with the members packed 8 per 16 x 16 block by the Python mirror
+    (mr.PackedMembers) and by the library (pr_options.pack), and unpacked."""
     from pulsar_rocm import mr
     ctrl, w = workloads.mr_spin_ensemble()
     native.init(0)
@@ -526,6 +527,19 @@ def mr_workload(args):
         finally:
             kp.close()
         out["packed" if pack else "unpacked"] = {"evals_per_s": round(1 / dt, 2), "ms_per_eval": round(1e3 * dt, 4)}
+    # the same ensemble handed to the library as it is, which packs its
+    # members itself (pr_options.pack)
+    plan = native.Plan([ctrl.problem()])
+    try:
+        for _ in range(args.warmup):
+            plan.eval([w])
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.eval([w])
+        dt = (time.perf_counter() - t0) / args.steps
+    finally:
+        plan.close()
+    out["packed_by_the_library"] = {"evals_per_s": round(1 / dt, 2), "ms_per_eval": round(1e3 * dt, 4)}
     out["note"] = "host-buffer evaluations (controls up, gradient down each call)"
     print(json.dumps(out))
 

@@ -174,14 +174,15 @@ typedef struct {
                       tracking problems their U_k); -1: recompute
                       only when all images (and Lindblad forward states) would take
                       more than half of the free device memory */
-  int pack;        /* 1: closed-dynamics problems with one member, dim <= 8 and no
-                      per-slice dt, of objective PR_GATE, PR_PAIRS_SQUARE, PR_PAIRS_REAL,
-                      PR_PAIRS_MODULUS or PR_STATE, are packed, up to 16 rows and 16 pairs
-                      at a time, as diagonal blocks of one problem, which one 16 x 16 work
-                      item then evaluates for all of them (problems of a pack share
-                      objective and dt; F and G are each problem's own, with its weight,
-                      equal to an unpacked plan's to rounding; the diagnostics are the
-                      pack's; PR_SPLIT16 plans do not pack). 0: no packing. No reference counterpart: the reference evaluates
+  int pack;        /* 1: closed-dynamics problems of dim <= 8 and objective PR_GATE,
+                      PR_PAIRS_SQUARE, PR_PAIRS_REAL, PR_PAIRS_MODULUS or PR_STATE are packed
+                      as diagonal blocks of problems of dim <= 16 (up to 16 rows and 16
+                      pairs), so that one 16 x 16 work item evaluates several of them:
+                      one-member problems without per-slice dt, of one objective and dt,
+                      together; the members of an ensemble, several to a wider member.
+                      F and G are each problem's own, with its weights, equal to an
+                      unpacked plan's to rounding; the diagnostics and exp counts are the
+                      packed problem's; PR_SPLIT16 plans do not pack. 0: no packing. No reference counterpart: the reference evaluates
                       problems one by one. A NULL opt means precision PR_FP64,
                       grad_mode PR_GRAD_REFERENCE, segment 0, store_props -1, pack 1. */
 } pr_options;

@@ -93,6 +93,10 @@ struct ProbDev {
   int nblk;
   int blk_row[17];
   int blk_pair[17];
+  int ens;                    // packed members of one ensemble: the blocks of member m are
+                              // members of the caller's problem with weights bw[m * nblk + b],
+                              // which scale their seeds; Fm[m] is their weighted sum
+  const double* bw;           // [M][nblk] (ens)
   int ncg;                    // channels 0 .. ncg-1 have a gradient (nc, except packed
                               // problems, whose per-block drift channels come last)
   int ck, nck;                // two-level prefix (k_combine_local): ck segments per chunk,
@@ -860,13 +864,19 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
         for (int b = 0; b < P.nblk; b++)
           if (x.c >= P.blk_row[b] && x.c < P.blk_row[b + 1]) bc = b;
         double2 wc = make_double2(0, 0);
+        double fsum = 0;
         for (int b = 0; b < P.nblk; b++) {
           const double2 tb = block_sum2<1>(bc == b ? part : make_double2(0, 0), red, x);
           const double db = P.blk_row[b + 1] - P.blk_row[b];
           const double br = tb.x / db, bi = tb.y / db;
-          if (threadIdx.x == 0) P.Fm[b] = br * br + bi * bi;
-          if (bc == b) wc = make_double2(2.0 * br / db, 2.0 * bi / db);
+          const double wb = P.ens ? P.bw[(size_t)m * P.nblk + b] : 1.0;
+          if (P.ens)
+            fsum += wb * (br * br + bi * bi);
+          else if (threadIdx.x == 0)
+            P.Fm[b] = br * br + bi * bi;
+          if (bc == b) wc = make_double2(wb * 2.0 * br / db, wb * 2.0 * bi / db);
         }
+        if (P.ens && threadIdx.x == 0) P.Fm[m] = fsum;
         Blk Y = mm<1, C, N>(Tg, Ut, LA, LB, x);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -908,12 +918,18 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
       return square ? (z.x * z.x + z.y * z.y) : modulus ? sqrt(z.x * z.x + z.y * z.y) : z.x;
     };
     if (P.nblk > 0) {
-      // packed: the pairs of block b give its problem's F
+      // packed: the pairs of block b give its problem's F (packed members:
+      // their weighted sum is the member's)
+      double Fs = 0;
       for (int b = 0; b < P.nblk; b++) {
         double F = 0;
         for (int s = P.blk_pair[b]; s < P.blk_pair[b + 1]; s++) F += f(s);
-        P.Fm[b] = F;
+        if (P.ens)
+          Fs += P.bw[(size_t)m * P.nblk + b] * F;
+        else
+          P.Fm[b] = F;
       }
+      if (P.ens) P.Fm[m] = Fs;
     } else {
       double F = 0;
       for (int s = 0; s < P.npairs; s++) F += f(s);
@@ -925,7 +941,7 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
   // w_s = 2 i z_s. :modulus (objective 7): the reference's prefactor
   // dt Im(inner / |z|) is w_s = 1 / |z_s|; the exact gradient of |z_s| is
   // w_s = z_s / |z_s|; both are zero for |z_s| < 1e-14
-  auto wgt = [&](int s) -> double2 {
+  auto wgt0 = [&](int s) -> double2 {
     if (s >= P.npairs) return make_double2(0, 0);
     double2 z = zb[s];
     if (P.obj == 6) return make_double2(-2 * z.y, 2 * z.x);
@@ -935,6 +951,18 @@ __global__ __launch_bounds__(64 * B * B) void k_combine(const ProbDev* __restric
       return P.grad_exact ? make_double2(z.x / a, z.y / a) : make_double2(1.0 / a, 0);
     }
     return square ? make_double2(2 * z.x, 2 * z.y) : make_double2(1, 0);
+  };
+  // packed members: a pair's weight also carries its member's
+  auto wgt = [&](int s) -> double2 {
+    double2 w = wgt0(s);
+    if (P.ens) {
+      double wb = 0;
+      for (int b = 0; b < P.nblk; b++)
+        if (s >= P.blk_pair[b] && s < P.blk_pair[b + 1]) wb = P.bw[(size_t)m * P.nblk + b];
+      w.x *= wb;
+      w.y *= wb;
+    }
+    return w;
   };
   if (P.unitary) {
     // Omega = Psi0 diag(conj w) PsiT^dag U: scale row s of W = PsiT^dag U by conj(w_s)
@@ -1865,9 +1893,10 @@ __global__ void k_pack_controls(const ProbDev* __restrict__ probs, const long lo
   ud[t] = v;
 }
 
-// umap[p] for the caller's problem p: (device problem, block or -1 when the
-// problem is not packed, first channel of its block, n_ctrl); uw[p] its
-// weight (a packed problem runs at weight 1 and takes its own here)
+// umap[p] for the caller's problem p: (device problem, block, first channel of
+// its block, n_ctrl), the block -1 when the problem is not packed and -f when
+// its members are packed f to a block problem; uw[p] its weight (a packed
+// problem runs at weight 1 and takes its own here)
 __global__ void k_unpack_G(long long total, const long long* __restrict__ uoff, const int* __restrict__ ufirst,
                            const int4* __restrict__ umap, const double* __restrict__ uw,
                            const ProbDev* __restrict__ probs, const long long* __restrict__ doff,
@@ -1877,11 +1906,20 @@ __global__ void k_unpack_G(long long total, const long long* __restrict__ uoff, 
   const int p = prob_of(uoff, ufirst, t);
   const long long e = t - uoff[p];
   const int4 mp = umap[p];
-  if (mp.y < 0) {
+  if (mp.y == -1) {
     G[t] = Gd[doff[mp.x] + e];
     return;
   }
   const long long k = e / mp.w;
+  if (mp.y < -1) {
+    // packed members of one ensemble, -mp.y blocks: control j of block b is
+    // channel b n_ctrl + j, with the member weights already in the seeds
+    const long long base = doff[mp.x] + k * probs[mp.x].nc + (e - k * mp.w);
+    double g = 0;
+    for (int b = 0; b < -mp.y; b++) g += Gd[base + (long long)b * mp.w];
+    G[t] = g;
+    return;
+  }
   G[t] = uw[p] * Gd[doff[mp.x] + k * probs[mp.x].nc + mp.z + (e - k * mp.w)];
 }
 

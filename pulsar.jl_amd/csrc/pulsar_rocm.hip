@@ -1068,8 +1068,9 @@ struct PackBuild {
   std::vector<std::vector<int>> blk_pair;    // per problem: block pair columns (packed pairs) or empty
   std::vector<int> dsrc;                     // per problem: caller's problem, or -1 - first channel in chan
   std::vector<int2> chan;                    // per packed channel: (caller's problem, j or -1)
-  std::vector<int4> umap;                    // per caller's problem: (problem, block or -1, ch0, n_ctrl)
+  std::vector<int4> umap;                    // per caller's problem: (problem, block or -1 or -f, ch0, n_ctrl)
   std::vector<double> uw;                    // per caller's problem: weight applied when unpacking
+  std::vector<std::vector<double>> bw;       // per problem: member weights of packed members, or empty
 };
 
 bool pair_objective(int obj) {
@@ -1081,12 +1082,106 @@ bool packable(const pr_problem& q) {
          q.dim <= 8 && !q.dt_slices;
 }
 
-// false when nothing is packed (fewer than two packable problems fit together)
+// Members of an ensemble (M >= 2, dim <= 8, per-slice dt allowed: the members
+// share the slices) pack the same way, f to a block problem of
+// M' = ceil(M / f) members: block b of member m' is member
+// m' f + b, with its drift, control scales and target (or pairs); the
+// controls are the ensemble's, f times over; the member weights scale the
+// blocks' seeds and F (bw), and the gradient rows of control j in the f blocks
+// are summed when G is scattered back. A last member short of f blocks is
+// filled with copies of member 0 at weight 0. f = the most blocks of d rows
+// (and n_pairs pairs) in 16, at most 64 / n_ctrl and M.
+int ens_blocks(const pr_problem& q) {
+  if (q.dynamics != PR_CLOSED || !(q.objective == PR_GATE || pair_objective(q.objective)) || q.n_members < 2 ||
+      q.dim > 8)
+    return 0;
+  int f = std::min({16 / q.dim, 64 / q.n_ctrl, q.n_members});
+  if (q.objective != PR_GATE) f = std::min(f, 16 / std::max(1, q.n_pairs));
+  return f >= 2 ? f : 0;
+}
+
+void pack_members(const pr_problem& q, int p, int f, PackBuild& pk) {
+  const int d = q.dim, nc = q.n_ctrl, M = q.n_members, Mp = (M + f - 1) / f, D = f * d, nc2 = f * nc;
+  const bool gate = q.objective == PR_GATE;
+  const int np = gate ? 0 : q.n_pairs, np2 = f * np;
+  const size_t dd = (size_t)d * d, DD = (size_t)D * D;
+  const int nh0 = q.H0_per_member ? Mp : 1;
+  std::vector<double> H0(2 * DD * nh0, 0.0), Hc(2 * DD * nc2, 0.0), Ut(gate ? 2 * DD : 0, 0.0),
+      sc((size_t)Mp * nc2, 1.0), wt(Mp, 1.0), P0(2 * (size_t)D * np2, 0.0), PT(2 * (size_t)D * np2, 0.0);
+  std::vector<double> bw((size_t)Mp * f, 0.0);
+  auto embed = [&](std::vector<double>& dst, size_t base, const double* a, int o) {
+    for (int c = 0; c < d; c++)
+      for (int r = 0; r < d; r++) {
+        const size_t i = (size_t)r + (size_t)c * d, j = base + (size_t)(o + r) + (size_t)(o + c) * D;
+        dst[2 * j] = a[2 * i];
+        dst[2 * j + 1] = a[2 * i + 1];
+      }
+  };
+  const double w0 = gate ? 1.0 / M : 1.0 / ((double)M * q.n_pairs);
+  const int c0 = (int)pk.chan.size();
+  for (int b = 0; b < f; b++) {
+    const int o = b * d;
+    for (int j = 0; j < nc; j++) {
+      embed(Hc, DD * (b * nc + j), q.Hc + 2 * dd * j, o);
+      pk.chan.push_back(make_int2(p, j));
+    }
+    if (gate) embed(Ut, 0, q.U_target, o);
+    for (int t = 0; t < np; t++)
+      for (int r = 0; r < d; r++)
+        for (int k = 0; k < 2; k++) {
+          P0[2 * ((size_t)(b * np + t) * D + o + r) + k] = q.psi_init[2 * ((size_t)t * d + r) + k];
+          PT[2 * ((size_t)(b * np + t) * D + o + r) + k] = q.psi_targ[2 * ((size_t)t * d + r) + k];
+        }
+    for (int mp = 0; mp < Mp; mp++) {
+      const int m = mp * f + b, src = m < M ? m : 0;  // (a filling block copies member 0)
+      if (q.H0_per_member || mp == 0) embed(H0, q.H0_per_member ? DD * mp : 0, q.H0 + (q.H0_per_member ? 2 * dd * src : 0), o);
+      for (int j = 0; j < nc; j++)
+        sc[(size_t)mp * nc2 + b * nc + j] = q.ctrl_scale ? q.ctrl_scale[(size_t)src * nc + j] : 1.0;
+      bw[(size_t)mp * f + b] = m < M ? (q.weight ? q.weight[m] : w0) : 0.0;
+    }
+  }
+  pr_problem z = q;
+  z.dim = D;
+  z.n_ctrl = nc2;
+  z.n_members = Mp;
+  auto keep = [&](std::vector<double>& v) -> const double* {
+    pk.store.push_back(std::move(v));
+    return pk.store.back().data();
+  };
+  z.H0 = keep(H0);
+  z.Hc = keep(Hc);
+  z.ctrl_scale = keep(sc);
+  z.weight = keep(wt);
+  if (gate) {
+    z.U_target = keep(Ut);
+  } else {
+    z.n_pairs = np2;
+    z.psi_init = keep(P0);
+    z.psi_targ = keep(PT);
+  }
+  std::vector<int> rows(f + 1), pcols(f + 1);
+  for (int b = 0; b <= f; b++) {
+    rows[b] = b * d;
+    pcols[b] = b * np;
+  }
+  pk.umap[p] = make_int4((int)pk.dev.size(), -f, 0, nc);
+  pk.dev.push_back(z);
+  pk.ncg.push_back(nc2);
+  pk.blk_row.push_back(rows);
+  pk.blk_pair.push_back(gate ? std::vector<int>() : pcols);
+  pk.bw.push_back(bw);
+  pk.dsrc.push_back(-1 - c0);
+}
+
+// false when nothing is packed
 bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
   std::vector<int> cand;
-  for (int p = 0; p < n_probs; p++)
+  bool any_ens = false;
+  for (int p = 0; p < n_probs; p++) {
     if (packable(probs[p])) cand.push_back(p);
-  if (cand.size() < 2) return false;
+    if (ens_blocks(probs[p])) any_ens = true;
+  }
+  if (cand.size() < 2 && !any_ens) return false;
   std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) {
     if (probs[a].objective != probs[b].objective) return probs[a].objective < probs[b].objective;
     if (probs[a].dt != probs[b].dt) return probs[a].dt < probs[b].dt;
@@ -1132,17 +1227,23 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
       for (int p : bins[i].mem) pack_of[p] = (int)i;
       n_packs++;
     }
-  if (n_packs == 0) return false;
+  if (n_packs == 0 && !any_ens) return false;
   pk.umap.assign(n_probs, make_int4(0, -1, 0, 0));
   pk.uw.assign(n_probs, 1.0);
-  // unpacked problems first, in the caller's order, then the packs
+  // unpacked problems and packed ensembles first, in the caller's order, then
+  // the packs of single problems
   for (int p = 0; p < n_probs; p++) {
     if (pack_of[p] >= 0) continue;
+    if (const int f = ens_blocks(probs[p])) {
+      pack_members(probs[p], p, f, pk);
+      continue;
+    }
     pk.umap[p] = make_int4((int)pk.dev.size(), -1, 0, probs[p].n_ctrl);
     pk.dev.push_back(probs[p]);
     pk.ncg.push_back(probs[p].n_ctrl);
     pk.blk_row.emplace_back();
     pk.blk_pair.emplace_back();
+    pk.bw.emplace_back();
     pk.dsrc.push_back(p);
   }
   for (const Bin& bn : bins) {
@@ -1229,6 +1330,7 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
     pk.ncg.push_back(ncg);
     pk.blk_row.push_back(rows);
     pk.blk_pair.push_back(gate ? std::vector<int>() : pcols);
+    pk.bw.emplace_back();
     pk.dsrc.push_back(-1 - ((int)pk.chan.size() - nc));
   }
   return true;
@@ -1317,6 +1419,13 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
       D.nblk = (int)pk->blk_row[p].size() - 1;
       std::copy(pk->blk_row[p].begin(), pk->blk_row[p].end(), D.blk_row);
       std::copy(pk->blk_pair[p].begin(), pk->blk_pair[p].end(), D.blk_pair);
+    }
+    if (pk && !pk->bw[p].empty()) {
+      double* bwp;
+      int rcb = pl.upload(&bwp, pk->bw[p].data(), pk->bw[p].size());
+      if (rcb) return rcb;
+      D.ens = 1;
+      D.bw = bwp;
     }
     const bool gate = q.objective == PR_GATE || q.objective == PR_GATE_PROPAGATORS;
     D.npairs = gate ? 0 : q.n_pairs;
@@ -2468,11 +2577,14 @@ int pr_mr_optimcon(void* plan, const double* controls_init, const pr_mr_config* 
   if ((cfg->method != PR_MR_LBFGS && cfg->method != PR_MR_GRAPE) || cfg->max_iter < 0 || f < 1 ||
       cfg->lbfgs_memory < 1 || cfg->lbfgs_memory > 64 || !(cfg->u_bound >= cfg->l_bound))
     return fail(PR_ERR_ARG, "pr_mr_optimcon: invalid configuration");
-  const pr::ProbDev& D0 = pl.host_probs[0];
-  if (D0.nc % f != 0) return fail(PR_ERR_ARG, "pr_mr_optimcon: n_ctrl %d is not a multiple of tile %d", D0.nc, f);
+  // the caller's problem's n_t and n_ctrl (a plan that packs its members
+  // runs a wider problem of the same n_t)
+  const int nt0 = pl.host_probs[pl.packed ? pl.umap[0].x : 0].nt;
+  const int nc0 = (int)(pl.n_uparams / nt0);
+  if (nc0 % f != 0) return fail(PR_ERR_ARG, "pr_mr_optimcon: n_ctrl %d is not a multiple of tile %d", nc0, f);
   if (g_device >= 0) HIPCHK(hipSetDevice(g_device));
-  const int nc = D0.nc / f;
-  const long long n = (long long)nc * D0.nt;  // waveform entries
+  const int nc = nc0 / f;
+  const long long n = (long long)nc * nt0;  // waveform entries
   const int mem = cfg->lbfgs_memory, R = mem + 1;
   const double lo = cfg->l_bound, hi = cfg->u_bound, a0 = 0.05 * (hi - lo);
   hipStream_t st = pl.stream;

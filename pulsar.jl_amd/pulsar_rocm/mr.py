@@ -267,7 +267,8 @@ class KernelPlan:
         self.problem = ctrl.tracking_problem() if tracking else ctrl.problem()
         self.packed = PackedMembers(self.problem) if pack and PackedMembers.applies(self.problem) else None
         run = self.packed.problem if self.packed else self.problem
-        self.plan = native.Plan([run], grad_mode=grad_mode)
+        # (pack=False: no packing in the library either)
+        self.plan = native.Plan([run], grad_mode=grad_mode, pack=1 if pack else 0)
 
     def eval(self, waveform: np.ndarray, grad: bool = True):
         w = np.asarray(waveform, dtype=np.float64)

@@ -262,3 +262,49 @@ def test_packs_keep_objectives_apart():
     F, G, _, c = evaluate(probs, us)
     assert packed_rows(c) >= 15
     check_oracle(probs, us, F, G)
+
+
+def small_ensemble(M, d, seed, objective=PR_GATE, npairs=1, per_member_drift=True, weights=False, nc=2, n_t=45,
+                   per_slice_dt=False):
+    rng = np.random.default_rng(seed)
+    herm = (lambda s: W.random_hermitian(d, rng, s)) if d > 1 else (lambda s: np.array([[0.9 * s + 0j]]))
+    kw = dict(ctrl_scale=rng.uniform(0.8, 1.2, (M, nc)))
+    if weights:
+        kw["weight"] = rng.uniform(0.1, 1.0, M)
+    if objective == PR_GATE:
+        kw["U_target"] = W.haar_unitary(d, rng)
+    else:
+        kw["psi_init"] = np.stack([W.random_state(d, rng) for _ in range(npairs)])
+        kw["psi_targ"] = np.stack([W.random_state(d, rng) for _ in range(npairs)])
+    if per_slice_dt:
+        kw["dt_slices"] = rng.uniform(0.01, 0.03, n_t)
+    H0 = np.stack([herm(3.0) for _ in range(M)]) if per_member_drift else herm(3.0)
+    p = Problem(dim=d, n_ctrl=nc, n_t=n_t, dt=0.02, H0=H0, Hc=np.stack([herm(2.0) for _ in range(nc)]),
+                objective=objective, n_members=M, **kw)
+    return p, rng.uniform(-3, 3, (nc, n_t))
+
+
+@pytest.mark.parametrize("case", ["gate-d2-M37", "gate-d5-M7-weights", "gate-d1-M20", "square-d3-M10-2pairs",
+                                  "real-d4-M9", "state-d2-M16-shared-drift", "modulus-d2-M11", "exact-gate-d3-M13",
+                                  "exact-square-d2-M9", "real-d2-M12-dtslices"])
+def test_packed_ensemble_members(case):
+    # ensemble members as diagonal blocks of a wider ensemble (the last
+    # packed member filled with weight-0 copies), against the unpacked plan
+    # and the oracle
+    obj = {"gate": PR_GATE, "square": PR_PAIRS_SQUARE, "real": PR_PAIRS_REAL, "state": PR_STATE,
+           "modulus": PR_PAIRS_MODULUS}
+    parts = case.split("-")
+    mode = PR_GRAD_EXACT if parts[0] == "exact" else 0
+    kind = parts[1] if mode else parts[0]
+    d = int([x for x in parts if x.startswith("d")][0][1:])
+    M = int([x for x in parts if x.startswith("M")][0][1:])
+    p, u = small_ensemble(M, d, seed=M * 10 + d, objective=obj[kind], npairs=2 if "2pairs" in parts else 1,
+                          per_member_drift="shared" not in parts, weights="weights" in parts,
+                          per_slice_dt="dtslices" in parts)
+    F1, G1, D1, c1 = evaluate([p], [u], grad_mode=mode)
+    F0, G0, D0, c0 = evaluate([p], [u], grad_mode=mode, pack=0)
+    # exps run: the packed members' slices, not every member's
+    assert c1[0, :2].sum() < c0[0, :2].sum(), (c1, c0)
+    assert abs(F1[0] - F0[0]) <= 1e-13
+    assert g_close(G1[0], G0[0], 1e-11)
+    check_oracle([p], [u], F1, G1, grad_mode=mode)
