@@ -174,12 +174,13 @@ typedef struct {
                       tracking problems their U_k); -1: recompute
                       only when all images (and Lindblad forward states) would take
                       more than half of the free device memory */
-  int pack;        /* 1: closed-dynamics problems of dim <= 8 and objective PR_GATE,
-                      PR_PAIRS_SQUARE, PR_PAIRS_REAL, PR_PAIRS_MODULUS or PR_STATE are packed
+  int pack;        /* 1: problems of dim <= 8 and objective PR_GATE, PR_PAIRS_SQUARE,
+                      PR_PAIRS_REAL, PR_PAIRS_MODULUS or PR_STATE (closed dynamics), or
+                      PR_PAIRS_SQUARE or PR_PAIRS_REAL (Lindblad dynamics), are packed
                       as diagonal blocks of problems of dim <= 16 (up to 16 rows and 16
                       pairs), so that one 16 x 16 work item evaluates several of them:
-                      one-member problems without per-slice dt, of one objective and dt,
-                      together; the members of an ensemble, several to a wider member.
+                      one-member problems without per-slice dt, of one dynamics,
+                      objective and dt, together; the members of an ensemble, several to a wider member.
                       F and G are each problem's own, with its weights, equal to an
                       unpacked plan's to rounding; the diagnostics and exp counts are the
                       packed problem's; PR_SPLIT16 plans do not pack. 0: no packing. No reference counterpart: the reference evaluates

@@ -1037,12 +1037,12 @@ int static_layout(Plan& pl) {
 }
 
 // ------------------------------------------------ packing of small problems
-// pr_options.pack: closed-dynamics problems with one member, dim <= 8 and a
-// uniform slice length, of the gate objective or a state-pair one (pairs
-// square / real / modulus, state), become diagonal blocks of one problem of
-// dim <= 16, so that one single-wavefront work item evaluates up to 8 of them
+// pr_options.pack: problems with one member, dim <= 8 and a uniform slice
+// length, of the gate objective or a state-pair one (pairs square / real /
+// modulus, state; square and real for Lindblad dynamics), become diagonal
+// blocks of one problem of dim <= 16, so that one single-wavefront work item evaluates up to 8 of them
 // (a d = 2 problem alone fills 4 of the 256 elements of its 16 x 16 tile).
-// - A pack holds problems of one objective and one dt, at most 16 rows, 16
+// - A pack holds problems of one dynamics, objective and dt, at most 16 rows, 16
 //   pairs and 64 channels, and runs the longest n_t. Block b's drift is a
 //   control channel of its own whose waveform is 1 on the block's own slices
 //   and 0 after, where its controls are 0 too: the propagators of those slices
@@ -1077,9 +1077,16 @@ bool pair_objective(int obj) {
   return obj == PR_PAIRS_SQUARE || obj == PR_PAIRS_REAL || obj == PR_PAIRS_MODULUS || obj == PR_STATE;
 }
 
+// closed dynamics: the gate and the pair objectives; Lindblad dynamics (whose
+// generators exp and chains treat block by block as well): the pair objectives
+// it allows, square and real
+bool packable_kind(const pr_problem& q) {
+  if (q.dynamics == PR_CLOSED) return q.objective == PR_GATE || pair_objective(q.objective);
+  return q.objective == PR_PAIRS_SQUARE || q.objective == PR_PAIRS_REAL;
+}
+
 bool packable(const pr_problem& q) {
-  return q.dynamics == PR_CLOSED && (q.objective == PR_GATE || pair_objective(q.objective)) && q.n_members == 1 &&
-         q.dim <= 8 && !q.dt_slices;
+  return packable_kind(q) && q.n_members == 1 && q.dim <= 8 && !q.dt_slices;
 }
 
 // Members of an ensemble (M >= 2, dim <= 8, per-slice dt allowed: the members
@@ -1092,9 +1099,7 @@ bool packable(const pr_problem& q) {
 // filled with copies of member 0 at weight 0. f = the most blocks of d rows
 // (and n_pairs pairs) in 16, at most 64 / n_ctrl and M.
 int ens_blocks(const pr_problem& q) {
-  if (q.dynamics != PR_CLOSED || !(q.objective == PR_GATE || pair_objective(q.objective)) || q.n_members < 2 ||
-      q.dim > 8)
-    return 0;
+  if (!packable_kind(q) || q.n_members < 2 || q.dim > 8) return 0;
   int f = std::min({16 / q.dim, 64 / q.n_ctrl, q.n_members});
   if (q.objective != PR_GATE) f = std::min(f, 16 / std::max(1, q.n_pairs));
   return f >= 2 ? f : 0;
@@ -1183,13 +1188,14 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
   }
   if (cand.size() < 2 && !any_ens) return false;
   std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) {
+    if (probs[a].dynamics != probs[b].dynamics) return probs[a].dynamics < probs[b].dynamics;
     if (probs[a].objective != probs[b].objective) return probs[a].objective < probs[b].objective;
     if (probs[a].dt != probs[b].dt) return probs[a].dt < probs[b].dt;
     return probs[a].n_t > probs[b].n_t;
   });
   struct Bin {
     std::vector<int> mem;
-    int rows = 0, pairs = 0, chans = 0, nt = 0, obj = 0;
+    int rows = 0, pairs = 0, chans = 0, nt = 0, obj = 0, dyn = 0;
     double dt = 0;
   };
   std::vector<Bin> bins;
@@ -1200,7 +1206,7 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
     int hit = -1;
     for (size_t i = 0; i < open.size() && hit < 0; i++) {
       const Bin& b = bins[open[i]];
-      if (b.obj == q.objective && b.dt == q.dt && b.rows + q.dim <= 16 && b.pairs + np <= 16 &&
+      if (b.dyn == q.dynamics && b.obj == q.objective && b.dt == q.dt && b.rows + q.dim <= 16 && b.pairs + np <= 16 &&
           b.chans + q.n_ctrl + 1 <= 64 && b.nt - q.n_t <= b.nt / 16)
         hit = (int)i;
     }
@@ -1210,6 +1216,7 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
       bins.back().nt = q.n_t;
       bins.back().dt = q.dt;
       bins.back().obj = q.objective;
+      bins.back().dyn = q.dynamics;
       open.push_back(bins.size() - 1);
       hit = (int)open.size() - 1;
     }

@@ -308,3 +308,42 @@ def test_packed_ensemble_members(case):
     assert abs(F1[0] - F0[0]) <= 1e-13
     assert g_close(G1[0], G0[0], 1e-11)
     check_oracle([p], [u], F1, G1, grad_mode=mode)
+
+
+def lindblad_spin(offsets, pwr, fidelity, n_t=40, rate=3.0):
+    """A relaxing spin 1/2 (4 x 4 Liouvillians, LindbladMRControl) over the
+    given offsets and power levels: one member per (offset, power)."""
+    from pulsar_rocm import mr
+    sx = np.array([[0, 1], [1, 0]], complex) / 2
+    sy = np.array([[0, -1j], [1j, 0]], complex) / 2
+    sz = np.diag([0.5 + 0j, -0.5])
+    return mr.LindbladMRControl(drifts=[2 * np.pi * o * sz for o in offsets], operators=[sx, sy],
+                                rho_init=[np.array([[1, 0], [0, 0]], complex)],
+                                rho_targ=[np.array([[0, 0], [0, 1]], complex)], pulse_dt=np.full(n_t, 0.01),
+                                jump_ops=[np.array([[0, 1], [0, 0]], complex)], decay_rates=[rate],
+                                pwr_levels=list(pwr), fidelity=fidelity).problem()
+
+
+@pytest.mark.parametrize("fidelity", ["square", "real"])
+def test_packed_lindblad_ensemble_and_problems(fidelity):
+    # a 15-member ensemble (5 offsets x 3 power levels, per-slice dt) packs 4
+    # members to a block problem; 10 one-member problems (uniform dt) pack 4
+    # to a problem
+    import dataclasses
+    rng = np.random.default_rng(70)
+    ens = lindblad_spin(np.linspace(-20, 20, 5), 2 * np.pi * np.array([8.0, 10.0, 12.0]), fidelity)
+    singles = []
+    for o in rng.uniform(-30, 30, 10):
+        q = lindblad_spin([o], [2 * np.pi * 10.0], fidelity, n_t=int(rng.integers(40, 43)),
+                          rate=float(rng.uniform(1, 5)))
+        singles.append(dataclasses.replace(q, dt_slices=None, dt=0.01))
+    probs = [ens] + singles
+    us = [rng.uniform(-1, 1, (2, q.n_t)) for q in probs]
+    F1, G1, _, c1 = evaluate(probs, us)
+    F0, G0, _, c0 = evaluate(probs, us, pack=0)
+    assert c1[0, :2].sum() < c0[0, :2].sum()  # the ensemble's members were packed
+    assert packed_rows(c1) >= 4  # and so were the one-member problems
+    assert np.abs(F1 - F0).max() <= 1e-13
+    for g1, g0 in zip(G1, G0):
+        assert g_close(g1, g0, 1e-11)
+    check_oracle(probs, us, F1, G1)
