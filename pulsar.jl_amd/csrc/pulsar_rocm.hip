@@ -1178,13 +1178,90 @@ void pack_members(const pr_problem& q, int p, int f, PackBuild& pk) {
   pk.dsrc.push_back(-1 - c0);
 }
 
+// Tracking ensembles (M >= 2, dim <= 8, one weight for every member) pack f
+// members to a block member, f the largest divisor of M with f d <= 16,
+// f n_pairs <= 16 and f n_ctrl <= 64: the checkpoint targets become
+// blockdiag(rho_c, ..., rho_c), so a packed member's F is the sum of its
+// blocks' (the tracking walk needs nothing per block) and the member weight
+// carries over; the gradient rows of the f blocks are summed when G is
+// scattered back, as for the other packed ensembles.
+int track_blocks(const pr_problem& q) {
+  if (q.objective != PR_TRACKING || q.n_members < 2 || q.dim > 8) return 0;
+  if (q.weight)
+    for (int m = 1; m < q.n_members; m++)
+      if (q.weight[m] != q.weight[0]) return 0;
+  int best = 0;
+  for (int f = 2; f <= q.n_members; f++)
+    if (q.n_members % f == 0 && f * q.dim <= 16 && f * q.n_pairs <= 16 && f * q.n_ctrl <= 64) best = f;
+  return best;
+}
+
+void pack_tracking_members(const pr_problem& q, int p, int f, PackBuild& pk) {
+  const int d = q.dim, nc = q.n_ctrl, M = q.n_members, Mp = M / f, D = f * d, nc2 = f * nc, np = q.n_pairs;
+  const size_t dd = (size_t)d * d, DD = (size_t)D * D;
+  const int nh0 = q.H0_per_member ? Mp : 1;
+  std::vector<double> H0(2 * DD * nh0, 0.0), Hc(2 * DD * nc2, 0.0), sc((size_t)Mp * nc2, 1.0),
+      P0(2 * (size_t)D * f * np, 0.0), TT(2 * DD * q.n_track, 0.0), wt(Mp, q.weight ? q.weight[0] : 0.0);
+  auto embed = [&](std::vector<double>& dst, size_t base, const double* a, int o) {
+    for (int c = 0; c < d; c++)
+      for (int r = 0; r < d; r++) {
+        const size_t i = (size_t)r + (size_t)c * d, j = base + (size_t)(o + r) + (size_t)(o + c) * D;
+        dst[2 * j] = a[2 * i];
+        dst[2 * j + 1] = a[2 * i + 1];
+      }
+  };
+  const int c0 = (int)pk.chan.size();
+  for (int b = 0; b < f; b++) {
+    const int o = b * d;
+    for (int j = 0; j < nc; j++) {
+      embed(Hc, DD * (b * nc + j), q.Hc + 2 * dd * j, o);
+      pk.chan.push_back(make_int2(p, j));
+    }
+    for (int c = 0; c < q.n_track; c++) embed(TT, DD * c, q.track_target + 2 * dd * c, o);
+    for (int t = 0; t < np; t++)
+      for (int r = 0; r < d; r++)
+        for (int k = 0; k < 2; k++)
+          P0[2 * ((size_t)(b * np + t) * D + o + r) + k] = q.psi_init[2 * ((size_t)t * d + r) + k];
+    for (int mp = 0; mp < Mp; mp++) {
+      const int m = mp * f + b;
+      if (q.H0_per_member || mp == 0) embed(H0, q.H0_per_member ? DD * mp : 0, q.H0 + (q.H0_per_member ? 2 * dd * m : 0), o);
+      for (int j = 0; j < nc; j++)
+        sc[(size_t)mp * nc2 + b * nc + j] = q.ctrl_scale ? q.ctrl_scale[(size_t)m * nc + j] : 1.0;
+    }
+  }
+  pr_problem z = q;
+  z.dim = D;
+  z.n_ctrl = nc2;
+  z.n_members = Mp;
+  z.n_pairs = f * np;
+  z.psi_targ = nullptr;
+  auto keep = [&](std::vector<double>& v) -> const double* {
+    pk.store.push_back(std::move(v));
+    return pk.store.back().data();
+  };
+  z.H0 = keep(H0);
+  z.Hc = keep(Hc);
+  z.ctrl_scale = keep(sc);
+  z.psi_init = keep(P0);
+  z.track_target = keep(TT);
+  // (the default weights, 1/(M n_pairs), are the packed ensemble's as well)
+  z.weight = q.weight ? keep(wt) : nullptr;
+  pk.umap[p] = make_int4((int)pk.dev.size(), -f, 0, nc);
+  pk.dev.push_back(z);
+  pk.ncg.push_back(nc2);
+  pk.blk_row.emplace_back();
+  pk.blk_pair.emplace_back();
+  pk.bw.emplace_back();
+  pk.dsrc.push_back(-1 - c0);
+}
+
 // false when nothing is packed
 bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
   std::vector<int> cand;
   bool any_ens = false;
   for (int p = 0; p < n_probs; p++) {
     if (packable(probs[p])) cand.push_back(p);
-    if (ens_blocks(probs[p])) any_ens = true;
+    if (ens_blocks(probs[p]) || track_blocks(probs[p])) any_ens = true;
   }
   if (cand.size() < 2 && !any_ens) return false;
   std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) {
@@ -1243,6 +1320,10 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
     if (pack_of[p] >= 0) continue;
     if (const int f = ens_blocks(probs[p])) {
       pack_members(probs[p], p, f, pk);
+      continue;
+    }
+    if (const int f = track_blocks(probs[p])) {
+      pack_tracking_members(probs[p], p, f, pk);
       continue;
     }
     pk.umap[p] = make_int4((int)pk.dev.size(), -1, 0, probs[p].n_ctrl);

@@ -347,3 +347,32 @@ def test_packed_lindblad_ensemble_and_problems(fidelity):
     for g1, g0 in zip(G1, G0):
         assert g_close(g1, g0, 1e-11)
     check_oracle(probs, us, F1, G1)
+
+
+@pytest.mark.parametrize("d,M,npairs,weights,per_slice_dt", [(2, 12, 1, False, False), (3, 10, 2, True, False),
+                                                            (4, 6, 3, False, True), (8, 4, 1, False, False)])
+def test_packed_tracking_ensembles(d, M, npairs, weights, per_slice_dt):
+    # tracking members packed f to a block member (f the largest divisor of M
+    # that fits) with checkpoints blockdiag(rho_c, ..., rho_c), against the
+    # unpacked plan and the tracking oracle (GRAPETracking.jl:162-263)
+    from pulsar_rocm.problem import PR_TRACKING
+    rng = np.random.default_rng(80 + d)
+    nc, nt = 2, 36
+    steps, w, targets = W.tracking_checkpoints(d, nt, 4, rng)
+    kw = dict(weight=np.full(M, 0.3)) if weights else {}
+    if per_slice_dt:
+        kw["dt_slices"] = rng.uniform(0.05, 0.15, nt)
+    p = Problem(dim=d, n_ctrl=nc, n_t=nt, dt=0.1, objective=PR_TRACKING, n_members=M,
+                H0=np.stack([W.random_hermitian(d, rng, 3.0) for _ in range(M)]),
+                Hc=np.stack([W.random_hermitian(d, rng, 1.0) for _ in range(nc)]),
+                ctrl_scale=rng.uniform(0.8, 1.2, (M, 1)) * np.ones((1, nc)),
+                psi_init=np.stack([W.random_state(d, rng) for _ in range(npairs)]),
+                track_step=np.asarray(steps, np.int32), track_weight=w, track_target=targets, **kw)
+    u = rng.uniform(-2, 2, (nc, nt))
+    F1, G1, _, c1 = evaluate([p], [u])
+    F0, G0, _, _ = evaluate([p], [u], pack=0)
+    assert np.abs(F1 - F0).max() <= 1e-13
+    assert g_close(G1[0], G0[0], 1e-11)
+    Fo, Go = O.tracking_evaluate(p, u)
+    assert abs(F1[0] - Fo) <= TOL_F
+    assert g_close(G1[0], Go, TOL_G)
