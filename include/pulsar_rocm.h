@@ -180,12 +180,15 @@ typedef struct {
                       as diagonal blocks of problems of dim <= 16 (up to 16 rows and 16
                       pairs), so that one 16 x 16 work item evaluates several of them:
                       one-member problems without per-slice dt, of one dynamics,
-                      objective and dt, together; the members of an ensemble, several to a wider member.
+                      objective and dt, together; the members of an ensemble, several to
+                      a wider member (PR_TRACKING ensembles too, when every member has the
+                      same weight: a divisor of the member count to a wider member).
                       F and G are each problem's own, with its weights, equal to an
                       unpacked plan's to rounding; the diagnostics and exp counts are the
-                      packed problem's; PR_SPLIT16 plans do not pack. 0: no packing. No reference counterpart: the reference evaluates
-                      problems one by one. A NULL opt means precision PR_FP64,
-                      grad_mode PR_GRAD_REFERENCE, segment 0, store_props -1, pack 1. */
+                      packed problem's; PR_SPLIT16 plans do not pack. 0: no packing. No
+                      reference counterpart: the reference evaluates problems one by one.
+                      A NULL opt means precision PR_FP64, grad_mode PR_GRAD_REFERENCE,
+                      segment 0, store_props -1, pack 1. */
 } pr_options;
 
 /* Per-problem diagnostics filled by pr_eval when diag != NULL (a packed
