@@ -1073,6 +1073,17 @@ struct PackBuild {
   std::vector<std::vector<double>> bw;       // per problem: member weights of packed members, or empty
 };
 
+// The column-major d x d complex matrix a into the D x D one that starts at
+// element base of dst, on rows and columns o .. o + d - 1.
+void embed_block(std::vector<double>& dst, size_t base, const double* a, int d, int o, int D) {
+  for (int c = 0; c < d; c++)
+    for (int r = 0; r < d; r++) {
+      const size_t i = (size_t)r + (size_t)c * d, j = base + (size_t)(o + r) + (size_t)(o + c) * D;
+      dst[2 * j] = a[2 * i];
+      dst[2 * j + 1] = a[2 * i + 1];
+    }
+}
+
 bool pair_objective(int obj) {
   return obj == PR_PAIRS_SQUARE || obj == PR_PAIRS_REAL || obj == PR_PAIRS_MODULUS || obj == PR_STATE;
 }
@@ -1115,12 +1126,7 @@ void pack_members(const pr_problem& q, int p, int f, PackBuild& pk) {
       sc((size_t)Mp * nc2, 1.0), wt(Mp, 1.0), P0(2 * (size_t)D * np2, 0.0), PT(2 * (size_t)D * np2, 0.0);
   std::vector<double> bw((size_t)Mp * f, 0.0);
   auto embed = [&](std::vector<double>& dst, size_t base, const double* a, int o) {
-    for (int c = 0; c < d; c++)
-      for (int r = 0; r < d; r++) {
-        const size_t i = (size_t)r + (size_t)c * d, j = base + (size_t)(o + r) + (size_t)(o + c) * D;
-        dst[2 * j] = a[2 * i];
-        dst[2 * j + 1] = a[2 * i + 1];
-      }
+    embed_block(dst, base, a, d, o, D);
   };
   const double w0 = gate ? 1.0 / M : 1.0 / ((double)M * q.n_pairs);
   const int c0 = (int)pk.chan.size();
@@ -1139,7 +1145,8 @@ void pack_members(const pr_problem& q, int p, int f, PackBuild& pk) {
         }
     for (int mp = 0; mp < Mp; mp++) {
       const int m = mp * f + b, src = m < M ? m : 0;  // (a filling block copies member 0)
-      if (q.H0_per_member || mp == 0) embed(H0, q.H0_per_member ? DD * mp : 0, q.H0 + (q.H0_per_member ? 2 * dd * src : 0), o);
+      if (q.H0_per_member || mp == 0)
+        embed(H0, q.H0_per_member ? DD * mp : 0, q.H0 + (q.H0_per_member ? 2 * dd * src : 0), o);
       for (int j = 0; j < nc; j++)
         sc[(size_t)mp * nc2 + b * nc + j] = q.ctrl_scale ? q.ctrl_scale[(size_t)src * nc + j] : 1.0;
       bw[(size_t)mp * f + b] = m < M ? (q.weight ? q.weight[m] : w0) : 0.0;
@@ -1203,12 +1210,7 @@ void pack_tracking_members(const pr_problem& q, int p, int f, PackBuild& pk) {
   std::vector<double> H0(2 * DD * nh0, 0.0), Hc(2 * DD * nc2, 0.0), sc((size_t)Mp * nc2, 1.0),
       P0(2 * (size_t)D * f * np, 0.0), TT(2 * DD * q.n_track, 0.0), wt(Mp, q.weight ? q.weight[0] : 0.0);
   auto embed = [&](std::vector<double>& dst, size_t base, const double* a, int o) {
-    for (int c = 0; c < d; c++)
-      for (int r = 0; r < d; r++) {
-        const size_t i = (size_t)r + (size_t)c * d, j = base + (size_t)(o + r) + (size_t)(o + c) * D;
-        dst[2 * j] = a[2 * i];
-        dst[2 * j + 1] = a[2 * i + 1];
-      }
+    embed_block(dst, base, a, d, o, D);
   };
   const int c0 = (int)pk.chan.size();
   for (int b = 0; b < f; b++) {
@@ -1224,7 +1226,8 @@ void pack_tracking_members(const pr_problem& q, int p, int f, PackBuild& pk) {
           P0[2 * ((size_t)(b * np + t) * D + o + r) + k] = q.psi_init[2 * ((size_t)t * d + r) + k];
     for (int mp = 0; mp < Mp; mp++) {
       const int m = mp * f + b;
-      if (q.H0_per_member || mp == 0) embed(H0, q.H0_per_member ? DD * mp : 0, q.H0 + (q.H0_per_member ? 2 * dd * m : 0), o);
+      if (q.H0_per_member || mp == 0)
+        embed(H0, q.H0_per_member ? DD * mp : 0, q.H0 + (q.H0_per_member ? 2 * dd * m : 0), o);
       for (int j = 0; j < nc; j++)
         sc[(size_t)mp * nc2 + b * nc + j] = q.ctrl_scale ? q.ctrl_scale[(size_t)m * nc + j] : 1.0;
     }
@@ -1349,14 +1352,8 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
     std::vector<double> H0(2 * DD, 0.0), Hc(2 * DD * nc, 0.0), Ut(gate ? 2 * DD : 0, 0.0), sc(nc, 1.0),
         P0(2 * (size_t)D * npr, 0.0), PT(2 * (size_t)D * npr, 0.0), w1(1, 1.0);
     std::vector<int> rows(1, 0), pcols(1, 0);
-    // a column-major d x d complex matrix a into dst at element offset base, block offset o
     auto embed = [&](std::vector<double>& dst, size_t base, const double* a, int d, int o) {
-      for (int c = 0; c < d; c++)
-        for (int r = 0; r < d; r++) {
-          const size_t i = (size_t)r + (size_t)c * d, j = base + (size_t)(o + r) + (size_t)(o + c) * D;
-          dst[2 * j] = a[2 * i];
-          dst[2 * j + 1] = a[2 * i + 1];
-        }
+      embed_block(dst, base, a, d, o, D);
     };
     int o = 0, ch0 = 0, s0 = 0;
     for (int b = 0; b < nb; b++) {
