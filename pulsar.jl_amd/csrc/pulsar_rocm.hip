@@ -93,6 +93,11 @@ unsigned char* g_scratch_ptr[PR_MAX_DEVICES] = {nullptr};
 constexpr int LIND_RECOMPUTE_SEG = 32;
 int lind_window(int B) { return std::max(1, 8192 / (B * B)); }
 
+// segments from which a single-wavefront problem takes the two-level prefix
+#ifndef PR_PREFIX_MIN_SEG1
+#define PR_PREFIX_MIN_SEG1 64
+#endif
+
 thread_local std::string g_err;
 
 int fail(int code, const char* fmt, ...) {
@@ -1444,9 +1449,10 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
     group_members[gi] += probs[p].n_members;
     if (probs[p].dynamics != PR_CLOSED) group_lind[gi] = 1;
   }
-  // two-level prefix (k_combine_local): multi-block groups (2 <= B <= 4) of
-  // closed dynamics whose members cannot fill the CUs (< 128 of them), for
-  // problems of 16 or more segments, in chunks of about sqrt(S) segments
+  // two-level prefix (k_combine_local): groups (B <= 4) of closed dynamics
+  // whose members cannot fill the CUs (< 128 of them), for problems of 16 or
+  // more segments (64 for single-wavefront products, which are short against
+  // the two extra launches), in chunks of about sqrt(S) segments
   std::vector<std::vector<pr::WorkItem>> citems(13);
   // the mixed-precision kernels cover single-wavefront (d <= 16) unitary
   // problems under the reference gradient formula; any other problem of the
@@ -1730,7 +1736,8 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
     }
     PR_SCRATCH(Tseg, (size_t)M * S * MB);
     PR_SCRATCH(Eseg, (size_t)M * (S + 1) * MB);
-    if (B >= 2 && B <= 4 && !track && !group_lind[gi] && group_members[gi] < 128 && S >= 16) {
+    if (B <= 4 && !track && !group_lind[gi] && group_members[gi] < 128 &&
+        S >= (B == 1 ? PR_PREFIX_MIN_SEG1 : 16)) {
       D.ck = (int)std::ceil(std::sqrt((double)S));
       D.nck = (S + D.ck - 1) / D.ck;
       PR_SCRATCH(Qc, (size_t)M * D.nck * MB);
