@@ -1542,6 +1542,12 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
       const double per_item = std::ceil(group_work[gi] / (double)want);
       L = (int)std::min<double>(std::max<double>(std::round(per_item * pl.seg_scale[gi]), 1.0), (double)nt);
       L = std::max(L, std::min(nt, 4));
+      // tracking and Lindblad problems chain their segments one after another
+      // in a per-member scan (k_track_scan) or backward pass (k_combine): with
+      // few members the rule's short segments would make that chain the
+      // evaluation's longest part, so their segments are at least sqrt(n_t)
+      // long (a 2000-slice tracking problem of few members: about 1.0 -> 0.33 ms)
+      if (track || q.dynamics != PR_CLOSED) L = std::max(L, (int)std::ceil(std::sqrt((double)nt)));
       // d > 64 without stored images: the segment starts E_s are the
       // checkpoints the gradient walk recomputes from, every 32 slices at most;
       // Lindblad likewise, which bounds each work item's scratch slot
