@@ -182,6 +182,14 @@ def test_three_qubit_lindblad_four_members_full_length():
     check(p, u)
 
 
+def test_three_qubit_lindblad_whole_ensemble():
+    # the benchmarked configuration itself: all 64 members, all 1000 slices
+    # (the oracle takes about a minute on 16 host threads)
+    p, u = W.three_qubit_lindblad()
+    assert p.n_members == 64 and p.n_t == 1000
+    check(p, u)
+
+
 def test_uneven_batch_full_length_subset_matches_oracle():
     # 32 problems of the benchmarked 1024-problem uneven batch at full length
     # (the first 6 of d = 2 and 4, the first 5 of d = 8, 16, 32 and 64; 233 to
