@@ -438,6 +438,16 @@ def test_spin_chain_128_small():
     assert D[1] >= 1  # generators above the degree-9 bound: scaling and squaring
 
 
+def test_spin_chain_128_quarter_of_the_ensemble():
+    # the first 16 of the benchmarked 64 members (control scales 0.95-0.974),
+    # all 500 slices of the d = 128 chain (about 30 s of oracle on 16 host
+    # threads)
+    p, u = W.spin_chain_128()
+    q, uq = W.shrink(p, u, n_members=16)
+    F, G, D = check(q, uq)
+    assert D[0] < 1e-12
+
+
 def test_spin_chain_128_two_members_500_slices():
     # the 7-qubit Heisenberg chain (d = 128) at 2 of its 64 members and all 500
     # slices, with stored U_k images and without them (store_props = 0: the
