@@ -141,7 +141,10 @@ def _dptr(a: np.ndarray):
 
 
 class Plan:
-    """A device plan over one or more problems (pr_plan_create)."""
+    """A device plan over one or more problems (pr_plan_create). The options are
+    pr_options': precision, grad_mode, segment, store_props, and pack (1: small
+    problems and the members of small ensembles run as diagonal blocks of
+    16 x 16 problems; 0: each problem as given)."""
 
     def __init__(self, problems: Sequence[Problem], precision: int = 0, grad_mode: int = 0,
                  segment: int = 0, store_props: int = -1, pack: int = 1):
