@@ -175,8 +175,9 @@ typedef struct {
                       only when all images (and Lindblad forward states) would take
                       more than half of the free device memory */
   int pack;        /* 1: problems of dim <= 8 and objective PR_GATE, PR_PAIRS_SQUARE,
-                      PR_PAIRS_REAL, PR_PAIRS_MODULUS or PR_STATE (closed dynamics), or
-                      PR_PAIRS_SQUARE or PR_PAIRS_REAL (Lindblad dynamics), are packed
+                      PR_PAIRS_REAL, PR_PAIRS_MODULUS, PR_STATE or PR_TRACKING (closed
+                      dynamics), or PR_PAIRS_SQUARE or PR_PAIRS_REAL (Lindblad dynamics),
+                      are packed
                       as diagonal blocks of problems of dim <= 16 (up to 16 rows and 16
                       pairs), so that one 16 x 16 work item evaluates several of them:
                       one-member problems without per-slice dt, of one dynamics,

@@ -1640,8 +1640,25 @@ __global__ __launch_bounds__(64 * B * B) void k_track_scan(const ProbDev* __rest
     S = mm<B, OP, N>(Tc, S, LA, LB, x);
     store_blk(P.Pend + base + s * MB, S, x);
   }
-  const double2 fs = block_sum2<B>(make_double2(f, 0.0), red, x);
-  if (threadIdx.x == 0) gptr_w(P.Fm)[m] = fs.x;
+  bool packed = false;
+  if constexpr (B == 1) {
+    if (P.nblk > 0) {
+      // packed problems: the lane's column is a pair of one block, whose
+      // rows hold that pair's products, so F splits by the column's block
+      packed = true;
+      int bc = -1;
+      for (int b = 0; b < P.nblk; b++)
+        if (x.c >= P.blk_pair[b] && x.c < P.blk_pair[b + 1]) bc = b;
+      for (int b = 0; b < P.nblk; b++) {
+        const double2 fb = block_sum2<1>(make_double2(bc == b ? f : 0.0, 0.0), red, x);
+        if (threadIdx.x == 0) gptr_w(P.Fm)[b] = fb.x;
+      }
+    }
+  }
+  if (!packed) {
+    const double2 fs = block_sum2<B>(make_double2(f, 0.0), red, x);
+    if (threadIdx.x == 0) gptr_w(P.Fm)[m] = fs.x;
+  }
   Blk L = zero_blk();
   const int sl = P.nseg - 1;
   Blk Tb = load_blk(P.Tseg + base + sl * MB, x), Zb = load_blk(P.Zseg + base + sl * MB, x);

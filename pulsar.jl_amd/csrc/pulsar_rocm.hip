@@ -1068,6 +1068,7 @@ int static_layout(Plan& pl) {
 struct PackBuild {
   std::vector<pr_problem> dev;               // the plan's problem list
   std::vector<std::vector<double>> store;    // host arrays of the packed problems
+  std::vector<std::vector<int>> istore;      // (their checkpoint steps)
   std::vector<int> ncg;                      // per problem: channels with a gradient
   std::vector<std::vector<int>> blk_row;     // per problem: block rows (packed) or empty
   std::vector<std::vector<int>> blk_pair;    // per problem: block pair columns (packed pairs) or empty
@@ -1101,8 +1102,11 @@ bool packable_kind(const pr_problem& q) {
   return q.objective == PR_PAIRS_SQUARE || q.objective == PR_PAIRS_REAL;
 }
 
+// one-member problems that pack together: those kinds, and tracking problems
+// (each block keeps its own checkpoints; F splits by the blocks' initial
+// states in k_track_scan)
 bool packable(const pr_problem& q) {
-  return packable_kind(q) && q.n_members == 1 && q.dim <= 8 && !q.dt_slices;
+  return (packable_kind(q) || q.objective == PR_TRACKING) && q.n_members == 1 && q.dim <= 8 && !q.dt_slices;
 }
 
 // Members of an ensemble (M >= 2, dim <= 8, per-slice dt allowed: the members
@@ -1345,17 +1349,21 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
   for (const Bin& bn : bins) {
     if (bn.mem.size() < 2) continue;
     const int dp = (int)pk.dev.size();
-    const bool gate = bn.obj == PR_GATE;
-    int D = 0, ncg = 0, npr = 0;
+    const bool gate = bn.obj == PR_GATE, tracking = bn.obj == PR_TRACKING;
+    int D = 0, ncg = 0, npr = 0, nck = 0;
     for (int p : bn.mem) {
       D += probs[p].dim;
       ncg += probs[p].n_ctrl;
       npr += gate ? 0 : probs[p].n_pairs;
+      nck += tracking ? probs[p].n_track : 0;
     }
     const int nb = (int)bn.mem.size(), nc = ncg + nb;
     const size_t DD = (size_t)D * D;
     std::vector<double> H0(2 * DD, 0.0), Hc(2 * DD * nc, 0.0), Ut(gate ? 2 * DD : 0, 0.0), sc(nc, 1.0),
-        P0(2 * (size_t)D * npr, 0.0), PT(2 * (size_t)D * npr, 0.0), w1(1, 1.0);
+        P0(2 * (size_t)D * npr, 0.0), PT(2 * (size_t)D * npr, 0.0), w1(1, 1.0), TT(2 * DD * nck, 0.0),
+        TW(nck, 0.0);
+    std::vector<int> TS(nck, 1);
+    int ck0 = 0;  // checkpoints placed so far
     std::vector<int> rows(1, 0), pcols(1, 0);
     auto embed = [&](std::vector<double>& dst, size_t base, const double* a, int d, int o) {
       embed_block(dst, base, a, d, o, D);
@@ -1377,14 +1385,21 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
         pk.uw[p] = q.weight ? q.weight[0] : 1.0;
       } else {
         // pair s of the block: vectors [n_pairs][d] at rows o .. o + d - 1
+        // (tracking: the initial states only)
         for (int t = 0; t < q.n_pairs; t++)
           for (int r = 0; r < d; r++)
             for (int k = 0; k < 2; k++) {
               P0[2 * ((size_t)(s0 + t) * D + o + r) + k] = q.psi_init[2 * ((size_t)t * d + r) + k];
-              PT[2 * ((size_t)(s0 + t) * D + o + r) + k] = q.psi_targ[2 * ((size_t)t * d + r) + k];
+              if (!tracking) PT[2 * ((size_t)(s0 + t) * D + o + r) + k] = q.psi_targ[2 * ((size_t)t * d + r) + k];
             }
         s0 += q.n_pairs;
         pk.uw[p] = q.weight ? q.weight[0] : 1.0 / q.n_pairs;
+        // the block's checkpoints, their targets on its rows
+        for (int c = 0; tracking && c < q.n_track; c++, ck0++) {
+          TS[ck0] = q.track_step[c];
+          TW[ck0] = q.track_weight[c];
+          embed(TT, DD * ck0, q.track_target + 2 * dd * c, d, o);
+        }
       }
       pk.umap[p] = make_int4(dp, b, ch0, q.n_ctrl);
       o += d;
@@ -1414,7 +1429,14 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
     } else {
       z.n_pairs = npr;
       z.psi_init = keep(P0);
-      z.psi_targ = keep(PT);
+      z.psi_targ = tracking ? nullptr : keep(PT);
+    }
+    if (tracking) {
+      z.n_track = nck;
+      pk.istore.push_back(std::move(TS));
+      z.track_step = pk.istore.back().data();
+      z.track_weight = keep(TW);
+      z.track_target = keep(TT);
     }
     pk.dev.push_back(z);
     pk.ncg.push_back(ncg);

@@ -376,3 +376,34 @@ def test_packed_tracking_ensembles(d, M, npairs, weights, per_slice_dt):
     Fo, Go = O.tracking_evaluate(p, u)
     assert abs(F1[0] - Fo) <= TOL_F
     assert g_close(G1[0], Go, TOL_G)
+
+
+def test_packed_one_member_tracking_problems():
+    # independent tracking problems (d = 1..5, 1-2 initial states, their own
+    # checkpoints and weights) packed as blocks: each keeps its checkpoints
+    # and its F splits by its initial states; against the unpacked plan and
+    # the tracking oracle
+    from pulsar_rocm.problem import PR_TRACKING
+    rng = np.random.default_rng(91)
+    probs, us = [], []
+    for i in range(14):
+        d = int(rng.integers(1, 6))
+        nc, nt, npairs = int(rng.integers(1, 3)), int(rng.integers(36, 39)), int(rng.integers(1, 3))
+        steps, w, targets = W.tracking_checkpoints(d, nt, int(rng.integers(1, 4)), rng)
+        herm = (lambda s: W.random_hermitian(d, rng, s)) if d > 1 else (lambda s: np.array([[0.9 * s + 0j]]))
+        kw = dict(weight=np.array([0.6])) if i % 4 == 1 else {}
+        probs.append(Problem(dim=d, n_ctrl=nc, n_t=nt, dt=0.1, objective=PR_TRACKING, H0=herm(3.0),
+                             Hc=np.stack([herm(1.0) for _ in range(nc)]),
+                             psi_init=np.stack([W.random_state(d, rng) for _ in range(npairs)]),
+                             track_step=np.asarray(steps, np.int32), track_weight=w, track_target=targets,
+                             **kw))
+        us.append(rng.uniform(-2, 2, (nc, nt)))
+    F1, G1, _, c1 = evaluate(probs, us)
+    F0, G0, _, _ = evaluate(probs, us, pack=0)
+    assert np.abs(F1 - F0).max() <= 1e-13
+    for g1, g0 in zip(G1, G0):
+        assert g_close(g1, g0, 1e-11)
+    for i, (p, u) in enumerate(zip(probs, us)):
+        Fo, Go = O.tracking_evaluate(p, u)
+        assert abs(F1[i] - Fo) <= TOL_F, (i, p.dim)
+        assert g_close(G1[i], Go, TOL_G), (i, p.dim)
