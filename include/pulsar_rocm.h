@@ -186,7 +186,7 @@ typedef struct {
                       same weight: a divisor of the member count to a wider member).
                       F and G are each problem's own, with its weights, equal to an
                       unpacked plan's to rounding; the diagnostics and exp counts are the
-                      packed problem's; PR_SPLIT16 plans do not pack. 0: no packing. No
+                      packed problem's. 0: no packing. No
                       reference counterpart: the reference evaluates problems one by one.
                       A NULL opt means precision PR_FP64, grad_mode PR_GRAD_REFERENCE,
                       segment 0, store_props -1, pack 1. */

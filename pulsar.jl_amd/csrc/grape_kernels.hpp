@@ -93,6 +93,8 @@ struct ProbDev {
   int nblk;
   int blk_row[17];
   int blk_pair[17];
+  int blk_ch[17];             // block b's control channels [blk_ch[b], blk_ch[b + 1]) (and, for
+                              // packed problems with drift channels, channel ncg + b)
   int ens;                    // packed members of one ensemble: the blocks of member m are
                               // members of the caller's problem with weights bw[m * nblk + b],
                               // which scale their seeds; Fm[m] is their weighted sum

@@ -1072,6 +1072,7 @@ struct PackBuild {
   std::vector<int> ncg;                      // per problem: channels with a gradient
   std::vector<std::vector<int>> blk_row;     // per problem: block rows (packed) or empty
   std::vector<std::vector<int>> blk_pair;    // per problem: block pair columns (packed pairs) or empty
+  std::vector<std::vector<int>> blk_ch;      // per problem: block control channels (packed) or empty
   std::vector<int> dsrc;                     // per problem: caller's problem, or -1 - first channel in chan
   std::vector<int2> chan;                    // per packed channel: (caller's problem, j or -1)
   std::vector<int4> umap;                    // per caller's problem: (problem, block or -1 or -f, ch0, n_ctrl)
@@ -1190,6 +1191,9 @@ void pack_members(const pr_problem& q, int p, int f, PackBuild& pk) {
   pk.ncg.push_back(nc2);
   pk.blk_row.push_back(rows);
   pk.blk_pair.push_back(gate ? std::vector<int>() : pcols);
+  std::vector<int> chs(f + 1);
+  for (int b = 0; b <= f; b++) chs[b] = b * nc;
+  pk.blk_ch.push_back(chs);
   pk.bw.push_back(bw);
   pk.dsrc.push_back(-1 - c0);
 }
@@ -1263,6 +1267,7 @@ void pack_tracking_members(const pr_problem& q, int p, int f, PackBuild& pk) {
   pk.ncg.push_back(nc2);
   pk.blk_row.emplace_back();
   pk.blk_pair.emplace_back();
+  pk.blk_ch.emplace_back();
   pk.bw.emplace_back();
   pk.dsrc.push_back(-1 - c0);
 }
@@ -1343,6 +1348,7 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
     pk.ncg.push_back(probs[p].n_ctrl);
     pk.blk_row.emplace_back();
     pk.blk_pair.emplace_back();
+    pk.blk_ch.emplace_back();
     pk.bw.emplace_back();
     pk.dsrc.push_back(p);
   }
@@ -1364,7 +1370,7 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
         TW(nck, 0.0);
     std::vector<int> TS(nck, 1);
     int ck0 = 0;  // checkpoints placed so far
-    std::vector<int> rows(1, 0), pcols(1, 0);
+    std::vector<int> rows(1, 0), pcols(1, 0), chs(1, 0);
     auto embed = [&](std::vector<double>& dst, size_t base, const double* a, int d, int o) {
       embed_block(dst, base, a, d, o, D);
     };
@@ -1406,6 +1412,7 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
       ch0 += q.n_ctrl;
       rows.push_back(o);
       pcols.push_back(s0);
+      chs.push_back(ch0);
     }
     for (int b = 0; b < nb; b++) pk.chan.push_back(make_int2(bn.mem[b], -1));
     pr_problem z = probs[bn.mem[0]];
@@ -1442,6 +1449,7 @@ bool build_packs(const pr_problem* probs, int n_probs, PackBuild& pk) {
     pk.ncg.push_back(ncg);
     pk.blk_row.push_back(rows);
     pk.blk_pair.push_back(gate ? std::vector<int>() : pcols);
+    pk.blk_ch.push_back(chs);
     pk.bw.emplace_back();
     pk.dsrc.push_back(-1 - ((int)pk.chan.size() - nc));
   }
@@ -1532,6 +1540,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
       D.nblk = (int)pk->blk_row[p].size() - 1;
       std::copy(pk->blk_row[p].begin(), pk->blk_row[p].end(), D.blk_row);
       std::copy(pk->blk_pair[p].begin(), pk->blk_pair[p].end(), D.blk_pair);
+      std::copy(pk->blk_ch[p].begin(), pk->blk_ch[p].end(), D.blk_ch);
     }
     if (pk && !pk->bw[p].empty()) {
       double* bwp;
@@ -2206,10 +2215,8 @@ int pr_plan_create(const pr_problem* probs, int n_probs, const pr_options* opt, 
     delete pl;
     return fail(PR_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
   }
-  // (PR_SPLIT16 plans do not pack: the split segment kernel bounds a slice's
-  // generator by the sum over its channels, which a pack's blocks would add up)
   PackBuild pk;
-  const bool packed = o.pack && o.precision != PR_SPLIT16 && build_packs(probs, n_probs, pk);
+  const bool packed = o.pack && build_packs(probs, n_probs, pk);
   const pr_problem* dprobs = packed ? pk.dev.data() : probs;
   const int ndev = packed ? (int)pk.dev.size() : n_probs;
   const PackBuild* pkp = packed ? &pk : nullptr;

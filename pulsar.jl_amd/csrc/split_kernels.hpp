@@ -366,7 +366,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP > 0 ? PR_
       double nb = 0;
       if (k < it.k1) {
         const double* uk = u + P.coff + (size_t)k * nc;
-        for (int j = 0; j < nc; j++) nb = fma(fabs(gptr(uk)[j]), gptr(P.ncn)[j] * gptr(sc)[j], nb);
+        auto term = [&](int j) { return fabs(gptr(uk)[j]) * (gptr(P.ncn)[j] * gptr(sc)[j]); };
+        if (P.nblk > 0) {
+          // packed blocks: the generator is block diagonal, so its norm is
+          // its largest block's (the drift bound n0m covers every block's)
+          for (int b = 0; b < P.nblk; b++) {
+            double nbb = P.ncg < nc ? term(P.ncg + b) : 0.0;
+            for (int j = P.blk_ch[b]; j < P.blk_ch[b + 1]; j++) nbb += term(j);
+            nb = fmax(nb, nbb);
+          }
+        } else {
+          for (int j = 0; j < nc; j++) nb = fma(fabs(gptr(uk)[j]), gptr(P.ncn)[j] * gptr(sc)[j], nb);
+        }
         nb = (n0m + nb) * (P.prescaled ? 1.0 : slice_dt(P, k));
       }
       over = over || __ballot(k < it.k1 && !(nb <= THETA_SPLIT)) != 0;

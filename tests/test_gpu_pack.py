@@ -265,7 +265,7 @@ def test_packs_keep_objectives_apart():
 
 
 def small_ensemble(M, d, seed, objective=PR_GATE, npairs=1, per_member_drift=True, weights=False, nc=2, n_t=45,
-                   per_slice_dt=False):
+                   per_slice_dt=False, dt=0.02):
     rng = np.random.default_rng(seed)
     herm = (lambda s: W.random_hermitian(d, rng, s)) if d > 1 else (lambda s: np.array([[0.9 * s + 0j]]))
     kw = dict(ctrl_scale=rng.uniform(0.8, 1.2, (M, nc)))
@@ -279,7 +279,7 @@ def small_ensemble(M, d, seed, objective=PR_GATE, npairs=1, per_member_drift=Tru
     if per_slice_dt:
         kw["dt_slices"] = rng.uniform(0.01, 0.03, n_t)
     H0 = np.stack([herm(3.0) for _ in range(M)]) if per_member_drift else herm(3.0)
-    p = Problem(dim=d, n_ctrl=nc, n_t=n_t, dt=0.02, H0=H0, Hc=np.stack([herm(2.0) for _ in range(nc)]),
+    p = Problem(dim=d, n_ctrl=nc, n_t=n_t, dt=dt, H0=H0, Hc=np.stack([herm(2.0) for _ in range(nc)]),
                 objective=objective, n_members=M, **kw)
     return p, rng.uniform(-3, 3, (nc, n_t))
 
@@ -407,3 +407,25 @@ def test_packed_one_member_tracking_problems():
         Fo, Go = O.tracking_evaluate(p, u)
         assert abs(F1[i] - Fo) <= TOL_F, (i, p.dim)
         assert g_close(G1[i], Go, TOL_G), (i, p.dim)
+
+
+def test_packed_split16_form():
+    # the FP16x2 split form on packed problems and a packed ensemble: each
+    # pack bounds its generator by its largest block's, so no slice of these
+    # (|X|_1 <= 0.018 per block, 0.03 the split bound) takes the FP64 redo,
+    # which the sum over all blocks' channels would have sent there; against
+    # FP64 unpacked at the split form's gate
+    from pulsar_rocm.problem import PR_SPLIT16
+    batch = small_gates(32, seed=31, n_t=(80, 84), dts=(0.0005,))
+    probs, us = [b[0] for b in batch], [b[1] for b in batch]
+    e, ue = small_ensemble(37, 2, seed=32, dt=0.0005, n_t=80)
+    probs.append(e)
+    us.append(ue)
+    Fs, Gs, _, cs = evaluate(probs, us, precision=PR_SPLIT16)
+    assert packed_rows(cs) >= 10
+    assert cs[:, 3].sum() == 0, cs[:, 3]
+    F0, G0, _, c0 = evaluate(probs, us, pack=0)
+    assert cs[-1, :2].sum() < c0[-1, :2].sum()  # the ensemble's members were packed
+    assert np.abs(Fs - F0).max() <= 1e-9
+    for a, b in zip(Gs, G0):
+        assert g_close(a, b, 1e-6)
