@@ -93,9 +93,13 @@ unsigned char* g_scratch_ptr[PR_MAX_DEVICES] = {nullptr};
 constexpr int LIND_RECOMPUTE_SEG = 32;
 int lind_window(int B) { return std::max(1, 8192 / (B * B)); }
 
-// segments from which a single-wavefront problem takes the two-level prefix
+// segments from which a single-wavefront problem takes the two-level prefix,
+// and the group member count below which groups take it
 #ifndef PR_PREFIX_MIN_SEG1
 #define PR_PREFIX_MIN_SEG1 64
+#endif
+#ifndef PR_PREFIX_MAX_MEMBERS
+#define PR_PREFIX_MAX_MEMBERS 128
 #endif
 
 thread_local std::string g_err;
@@ -1773,7 +1777,7 @@ int build_plan(Plan& pl, const pr_problem* probs, int n_probs, const pr_problem*
     }
     PR_SCRATCH(Tseg, (size_t)M * S * MB);
     PR_SCRATCH(Eseg, (size_t)M * (S + 1) * MB);
-    if (B <= 4 && !track && !group_lind[gi] && group_members[gi] < 128 &&
+    if (B <= 4 && !track && !group_lind[gi] && group_members[gi] < PR_PREFIX_MAX_MEMBERS &&
         S >= (B == 1 ? PR_PREFIX_MIN_SEG1 : 16)) {
       D.ck = (int)std::ceil(std::sqrt((double)S));
       D.nck = (S + D.ck - 1) / D.ck;
