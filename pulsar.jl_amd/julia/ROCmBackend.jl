@@ -95,8 +95,10 @@ _rocm_check(rc::Cint, what::String) =
     rc == 0 || error("libpulsar_rocm $what failed (status $rc): $(_rocm_last_error())")
 
 """
-    ROCmPlan
+    ROCmPlan(probs, roots; grad_mode = 0, precision = 0, segment = 0, pack = 1)
 
+The keywords are pr_options' fields (pack = 1: small problems and the members
+of small ensembles run as diagonal blocks of 16 x 16 problems).
 A device plan for one or more problems; owns device memory and is freed by a
 finalizer (pr_plan_destroy). One Julia task at a time per plan.
 """
